@@ -1,0 +1,966 @@
+/*
+ * oracle.c -- FP64 CPU restatement of PhySim's Newton inner loop (tet FEM).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libpgn and the timed CPU
+ * baseline ("port").  Never linked into, or called by, the product path.
+ *
+ * Element loops use OpenMP like Simulable.cpp's `#pragma omp parallel for`
+ * (Simulable.cpp:1038-1047,1099-1108,1160-1169); assembly and the energy sum
+ * are serial in element order like Simulable.cpp:333-334,396-397,580-581.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ======================================================================= */
+/* Mesh generator                                                          */
+/* ======================================================================= */
+
+int orc_tet_grid_counts(const int dims[3], int* num_nodes, int* num_tets) {
+  *num_nodes = (dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1);
+  *num_tets = 5 * dims[0] * dims[1] * dims[2];
+  return *num_tets;
+}
+
+/* MeshUtils.cpp:927-991: nodes x-fastest on a regular grid, 5 tets per hex
+ * cell with the same split in every cell. */
+int orc_tet_grid(const double scale[3], const int dims[3], double* V, int* T) {
+  const int nx = dims[0] + 1, ny = dims[1] + 1, nz = dims[2] + 1;
+  const double h[3] = {scale[0] / dims[0], scale[1] / dims[1],
+                       scale[2] / dims[2]};
+  int c = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        V[3 * c + 0] = i * h[0];
+        V[3 * c + 1] = j * h[1];
+        V[3 * c + 2] = k * h[2];
+        ++c;
+      }
+  const int line = nx, slab = nx * ny;
+  int e = 0;
+  for (int k = 0; k < dims[2]; ++k)
+    for (int j = 0; j < dims[1]; ++j)
+      for (int i = 0; i < dims[0]; ++i) {
+        const int o = slab * k + line * j + i;
+        /* hex corners: bit0 = +x, bit1 = +y, bit2 = +z */
+        const int h0 = o, h1 = o + 1, h2 = o + line, h3 = o + line + 1;
+        const int h4 = o + slab, h5 = o + slab + 1, h6 = o + slab + line,
+                  h7 = o + slab + line + 1;
+        const int tets[5][4] = {{h0, h1, h2, h4},
+                                {h6, h7, h4, h2},
+                                {h5, h4, h7, h1},
+                                {h3, h2, h1, h7},
+                                {h1, h2, h4, h7}};
+        for (int t = 0; t < 5; ++t) {
+          for (int a = 0; a < 4; ++a) T[4 * e + a] = tets[t][a];
+          ++e;
+        }
+      }
+  return e;
+}
+
+void orc_lame_from_young_poisson(double young, double poisson, double* lame1,
+                                 double* lame2) {
+  const double t0 = 1.0 + poisson, t1 = 1.0 - 2.0 * poisson;
+  *lame1 = (young * poisson) / (t0 * t1);
+  *lame2 = young / (2.0 * t0);
+}
+
+/* ======================================================================= */
+/* Material models.  Column-major vec(F): Fv[i + 3 j] = F(i, j).            */
+/* Closed forms derived independently of the reference's Maple output:     */
+/*   CoNH: Psi = mu/2 (I1 - 3) - mu ln J + lambda/2 ln^2 J                  */
+/*         P   = mu (F - F^-T) + lambda ln J F^-T                           */
+/*   StVK: Psi = lambda/2 tr(E)^2 + mu tr(E^2), E = (F^T F - I)/2           */
+/*         P   = F S, S = 2 mu E + lambda tr(E) I                           */
+/* (MaterialModels.h:166-299 selects between them; lame1 = lambda,          */
+/*  lame2 = mu.)                                                            */
+/* ======================================================================= */
+
+#define FV(i, j) Fv[(i) + 3 * (j)]
+
+static double det3(const double* Fv) {
+  return FV(0, 0) * (FV(1, 1) * FV(2, 2) - FV(1, 2) * FV(2, 1)) -
+         FV(0, 1) * (FV(1, 0) * FV(2, 2) - FV(1, 2) * FV(2, 0)) +
+         FV(0, 2) * (FV(1, 0) * FV(2, 1) - FV(1, 1) * FV(2, 0));
+}
+
+/* inverse (column-major in and out); returns det */
+static double inv3(const double* Fv, double* Iv) {
+  const double d = det3(Fv);
+  const double id = 1.0 / d;
+#define IV(i, j) Iv[(i) + 3 * (j)]
+  IV(0, 0) = (FV(1, 1) * FV(2, 2) - FV(1, 2) * FV(2, 1)) * id;
+  IV(0, 1) = (FV(0, 2) * FV(2, 1) - FV(0, 1) * FV(2, 2)) * id;
+  IV(0, 2) = (FV(0, 1) * FV(1, 2) - FV(0, 2) * FV(1, 1)) * id;
+  IV(1, 0) = (FV(1, 2) * FV(2, 0) - FV(1, 0) * FV(2, 2)) * id;
+  IV(1, 1) = (FV(0, 0) * FV(2, 2) - FV(0, 2) * FV(2, 0)) * id;
+  IV(1, 2) = (FV(0, 2) * FV(1, 0) - FV(0, 0) * FV(1, 2)) * id;
+  IV(2, 0) = (FV(1, 0) * FV(2, 1) - FV(1, 1) * FV(2, 0)) * id;
+  IV(2, 1) = (FV(0, 1) * FV(2, 0) - FV(0, 0) * FV(2, 1)) * id;
+  IV(2, 2) = (FV(0, 0) * FV(1, 1) - FV(0, 1) * FV(1, 0)) * id;
+#undef IV
+  return d;
+}
+
+static void stvk_S(const double* Fv, double lame1, double lame2, double S[9]) {
+  double E[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double c = 0;
+      for (int k = 0; k < 3; ++k) c += FV(k, i) * FV(k, j);
+      E[i + 3 * j] = 0.5 * (c - (i == j ? 1.0 : 0.0));
+    }
+  const double tr = E[0] + E[4] + E[8];
+  for (int q = 0; q < 9; ++q) S[q] = 2.0 * lame2 * E[q];
+  S[0] += lame1 * tr;
+  S[4] += lame1 * tr;
+  S[8] += lame1 * tr;
+}
+
+double orc_psi(int kind, const double* Fv, double lame1, double lame2) {
+  if (kind == ORC_CONH) {
+    double I1 = 0;
+    for (int q = 0; q < 9; ++q) I1 += Fv[q] * Fv[q];
+    const double J = det3(Fv);
+    /* ln(J^2)/2 = ln|J|, as the reference's Maple energy uses log(J^2) */
+    const double lnJ = 0.5 * log(J * J);
+    return 0.5 * lame2 * (I1 - 3.0) - lame2 * lnJ + 0.5 * lame1 * lnJ * lnJ;
+  } else {
+    double E[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double c = 0;
+        for (int k = 0; k < 3; ++k) c += FV(k, i) * FV(k, j);
+        E[i + 3 * j] = 0.5 * (c - (i == j ? 1.0 : 0.0));
+      }
+    const double tr = E[0] + E[4] + E[8];
+    double trE2 = 0;
+    for (int q = 0; q < 9; ++q) trE2 += E[q] * E[q];
+    return 0.5 * lame1 * tr * tr + lame2 * trE2;
+  }
+}
+
+void orc_pk1(int kind, const double* Fv, double lame1, double lame2,
+             double* P) {
+  if (kind == ORC_CONH) {
+    double Fi[9];
+    const double J = inv3(Fv, Fi);
+    const double lnJ = 0.5 * log(J * J);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const double FiT = Fi[j + 3 * i]; /* F^-T (i,j) = F^-1 (j,i) */
+        P[i + 3 * j] = lame2 * (FV(i, j) - FiT) + lame1 * lnJ * FiT;
+      }
+  } else {
+    double S[9];
+    stvk_S(Fv, lame1, lame2, S);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double c = 0;
+        for (int k = 0; k < 3; ++k) c += FV(i, k) * S[k + 3 * j];
+        P[i + 3 * j] = c;
+      }
+  }
+}
+
+/* C[p * 9 + q] = d P_p / d F_q, p = i + 3 j, q = m + 3 n */
+void orc_dpdf(int kind, const double* Fv, double lame1, double lame2,
+              double* C) {
+  if (kind == ORC_CONH) {
+    double Fi[9];
+    const double J = inv3(Fv, Fi);
+    const double lnJ = 0.5 * log(J * J);
+    const double a = lame2 - lame1 * lnJ;
+#define FI(i, j) Fi[(i) + 3 * (j)]
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i)
+        for (int n = 0; n < 3; ++n)
+          for (int m = 0; m < 3; ++m) {
+            double v = a * FI(j, m) * FI(n, i) + lame1 * FI(n, m) * FI(j, i);
+            if (i == m && j == n) v += lame2;
+            C[(i + 3 * j) * 9 + (m + 3 * n)] = v;
+          }
+#undef FI
+  } else {
+    double S[9], FFt[9];
+    stvk_S(Fv, lame1, lame2, S);
+    for (int i = 0; i < 3; ++i)
+      for (int m = 0; m < 3; ++m) {
+        double c = 0;
+        for (int k = 0; k < 3; ++k) c += FV(i, k) * FV(m, k);
+        FFt[i + 3 * m] = c;
+      }
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i)
+        for (int n = 0; n < 3; ++n)
+          for (int m = 0; m < 3; ++m) {
+            double v = lame2 * FV(i, n) * FV(m, j) + lame1 * FV(m, n) * FV(i, j);
+            if (i == m) v += S[n + 3 * j];
+            if (j == n) v += lame2 * FFt[i + 3 * m];
+            C[(i + 3 * j) * 9 + (m + 3 * n)] = v;
+          }
+  }
+}
+#undef FV
+
+/* ======================================================================= */
+/* Tet4 element                                                             */
+/* ======================================================================= */
+
+/* ShapeFunctions.cpp:89-133: J = N0 * B (3x3), BH0 = B * J^-1 (4x3),
+ * DFDx(:, d) = vec_col(E_d * BH0) with E_d the 3x4 unit matrix whose
+ * column-major entry d is 1 (node d/3, coordinate d%3).
+ * EnergyElement_FEM.cpp:90: weight = (1/6) det(J) (Quadratures.h:134). */
+void orc_tet_rest(const double X[12], double DFDx[108], double* V0) {
+  static const double B[4][3] = {
+      {-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double Jm[9]; /* column-major */
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double c = 0;
+      for (int a = 0; a < 4; ++a) c += X[3 * a + i] * B[a][j];
+      Jm[i + 3 * j] = c;
+    }
+  double Ji[9];
+  const double detJ = inv3(Jm, Ji);
+  double BH[4][3];
+  for (int a = 0; a < 4; ++a)
+    for (int j = 0; j < 3; ++j) {
+      double c = 0;
+      for (int k = 0; k < 3; ++k) c += B[a][k] * Ji[k + 3 * j];
+      BH[a][j] = c;
+    }
+  /* (E_d * BH)(i, j) = delta(i, d%3) * BH[d/3][j]; vec index r = i + 3 j */
+  for (int r = 0; r < 9; ++r)
+    for (int d = 0; d < 12; ++d) {
+      const int i = r % 3, j = r / 3;
+      DFDx[r * 12 + d] = (i == d % 3) ? BH[d / 3][j] : 0.0;
+    }
+  *V0 = (1.0 / 6.0) * detJ;
+}
+
+/* EnergyElement_FEM.cpp:125-222 with ShapeFunctions.cpp:135-158 (F). */
+int orc_tet_egh(const double x[12], const double DFDx[108], double V0,
+                int kind, double lame1, double lame2, double* E, double g[12],
+                double H[144]) {
+  double Fv[9];
+  for (int r = 0; r < 9; ++r) {
+    double c = 0;
+    for (int d = 0; d < 12; ++d) c += DFDx[r * 12 + d] * x[d];
+    Fv[r] = c;
+  }
+  /* Cell_Tetra.cpp:27-36 VolumeSpace at the current positions */
+  double Ds[9];
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < 3; ++i) Ds[i + 3 * a] = x[3 * (a + 1) + i] - x[i];
+  const int inverted = (det3(Ds) / 6.0) < 0;
+  if (E) *E = inverted ? HUGE_VAL : V0 * orc_psi(kind, Fv, lame1, lame2);
+  if (g) {
+    double P[9];
+    orc_pk1(kind, Fv, lame1, lame2, P);
+    for (int d = 0; d < 12; ++d) {
+      double c = 0;
+      for (int r = 0; r < 9; ++r) c += DFDx[r * 12 + d] * P[r];
+      g[d] = V0 * c;
+    }
+  }
+  if (H) {
+    double C[81], CD[9 * 12];
+    orc_dpdf(kind, Fv, lame1, lame2, C);
+    for (int p = 0; p < 9; ++p)
+      for (int d = 0; d < 12; ++d) {
+        double c = 0;
+        for (int q = 0; q < 9; ++q) c += C[p * 9 + q] * DFDx[q * 12 + d];
+        CD[p * 12 + d] = c;
+      }
+    for (int d = 0; d < 12; ++d)
+      for (int e = 0; e < 12; ++e) {
+        double c = 0;
+        for (int p = 0; p < 9; ++p) c += DFDx[p * 12 + d] * CD[p * 12 + e];
+        H[d * 12 + e] = V0 * c;
+      }
+  }
+  return inverted;
+}
+
+/* ======================================================================= */
+/* Model                                                                    */
+/* ======================================================================= */
+
+struct orc_model {
+  int n, m;
+  double* X0;
+  int* T;
+  unsigned char* kind;
+  double *lame1, *lame2, *rho;
+  double* DFDx; /* 108 per tet */
+  double* V0;
+  double* elem_mass; /* rho * V0 / 4 per node of the element */
+  unsigned char* fixed; /* 3n */
+  double* x;
+  /* gravity BC */
+  int has_g, g_stages, g_cur;
+  double g_ini[3], g_end[3];
+  /* force BC */
+  int has_f, f_stages, f_cur;
+  double *f_ini, *f_end;
+  /* per-element outputs */
+  double *eE, *eg, *eH;
+  /* lower pattern */
+  long nnz;
+  long* rowptr;
+  int* col;
+  long* slot; /* per element 144 local entries -> CSR slot or -1 */
+};
+
+orc_model* orc_model_create(int n_nodes, const double* X0, int n_tets,
+                            const int* T, const unsigned char* kind,
+                            const double* lame1, const double* lame2,
+                            const double* rho) {
+  orc_model* M = (orc_model*)calloc(1, sizeof(orc_model));
+  M->n = n_nodes;
+  M->m = n_tets;
+  M->X0 = (double*)malloc(sizeof(double) * 3 * n_nodes);
+  memcpy(M->X0, X0, sizeof(double) * 3 * n_nodes);
+  M->x = (double*)malloc(sizeof(double) * 3 * n_nodes);
+  memcpy(M->x, X0, sizeof(double) * 3 * n_nodes);
+  M->T = (int*)malloc(sizeof(int) * 4 * n_tets);
+  memcpy(M->T, T, sizeof(int) * 4 * n_tets);
+  M->kind = (unsigned char*)malloc(n_tets);
+  memcpy(M->kind, kind, n_tets);
+  M->lame1 = (double*)malloc(sizeof(double) * n_tets);
+  M->lame2 = (double*)malloc(sizeof(double) * n_tets);
+  M->rho = (double*)malloc(sizeof(double) * n_tets);
+  memcpy(M->lame1, lame1, sizeof(double) * n_tets);
+  memcpy(M->lame2, lame2, sizeof(double) * n_tets);
+  memcpy(M->rho, rho, sizeof(double) * n_tets);
+  M->DFDx = (double*)malloc(sizeof(double) * 108 * n_tets);
+  M->V0 = (double*)malloc(sizeof(double) * n_tets);
+  M->elem_mass = (double*)malloc(sizeof(double) * n_tets);
+  M->fixed = (unsigned char*)calloc(3 * n_nodes, 1);
+  M->eE = (double*)malloc(sizeof(double) * n_tets);
+  M->eg = (double*)malloc(sizeof(double) * 12 * n_tets);
+  M->eH = (double*)malloc(sizeof(double) * 144 * n_tets);
+  /* EnergyElement_FEM::Init over elements (Simulable.cpp:963 omp loop) */
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < n_tets; ++e) {
+    double Xe[12];
+    for (int a = 0; a < 4; ++a)
+      for (int i = 0; i < 3; ++i) Xe[3 * a + i] = X0[3 * T[4 * e + a] + i];
+    orc_tet_rest(Xe, M->DFDx + 108 * e, M->V0 + e);
+    /* MassElement_Lumped.cpp:47-54: rho * VolumeBasis(Position_0) / nodes */
+    M->elem_mass[e] = rho[e] * M->V0[e] / 4.0;
+  }
+  /* Lower pattern as AMatrixSd stores it: node block (a, b) with a > b is
+   * stored whole below the diagonal; node block (a, a) is stored whole
+   * (BasicTypes.cpp:18-33, DerivativeAssembler.cpp:220). */
+  const int N = 3 * n_nodes;
+  /* node adjacency (lower) */
+  int* deg = (int*)calloc(n_nodes, sizeof(int));
+  for (int e = 0; e < n_tets; ++e)
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        const int na = T[4 * e + a], nb = T[4 * e + b];
+        if (na >= nb) deg[na]++;
+      }
+  long* aptr = (long*)malloc(sizeof(long) * (n_nodes + 1));
+  aptr[0] = 0;
+  for (int i = 0; i < n_nodes; ++i) aptr[i + 1] = aptr[i] + deg[i];
+  int* adj = (int*)malloc(sizeof(int) * aptr[n_nodes]);
+  memset(deg, 0, sizeof(int) * n_nodes);
+  for (int e = 0; e < n_tets; ++e)
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        const int na = T[4 * e + a], nb = T[4 * e + b];
+        if (na >= nb) adj[aptr[na] + deg[na]++] = nb;
+      }
+  /* sort + unique per node */
+  int* ucount = (int*)calloc(n_nodes, sizeof(int));
+  for (int i = 0; i < n_nodes; ++i) {
+    int* L = adj + aptr[i];
+    const int c = deg[i];
+    for (int p = 1; p < c; ++p) { /* insertion sort; lists are short */
+      int v = L[p], q = p - 1;
+      while (q >= 0 && L[q] > v) {
+        L[q + 1] = L[q];
+        --q;
+      }
+      L[q + 1] = v;
+    }
+    int u = 0;
+    for (int p = 0; p < c; ++p)
+      if (u == 0 || L[u - 1] != L[p]) L[u++] = L[p];
+    ucount[i] = u;
+  }
+  M->rowptr = (long*)malloc(sizeof(long) * (N + 1));
+  M->rowptr[0] = 0;
+  for (int i = 0; i < n_nodes; ++i)
+    for (int di = 0; di < 3; ++di)
+      M->rowptr[3 * i + di + 1] = M->rowptr[3 * i + di] + 3L * ucount[i];
+  M->nnz = M->rowptr[N];
+  M->col = (int*)malloc(sizeof(int) * M->nnz);
+  for (int i = 0; i < n_nodes; ++i)
+    for (int di = 0; di < 3; ++di) {
+      long p = M->rowptr[3 * i + di];
+      for (int q = 0; q < ucount[i]; ++q)
+        for (int dj = 0; dj < 3; ++dj) M->col[p++] = 3 * adj[aptr[i] + q] + dj;
+    }
+  /* element slot map: local (d, e) -> CSR index of the stored entry it is
+   * added to, or -1 when the entry is the mirror of a stored one */
+  M->slot = (long*)malloc(sizeof(long) * 144 * (long)n_tets);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < n_tets; ++e) {
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        const int na = M->T[4 * e + a], nb = M->T[4 * e + b];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            const int d = 3 * a + i, f = 3 * b + j;
+            long s = -1;
+            /* The reference walks local blocks (a, b) with b <= a and adds
+             * block (a, b) at (na, nb) when na >= nb, else its transpose at
+             * (nb, na); local blocks with b > a are never read. */
+            if (b <= a) {
+              const int r = (na >= nb) ? 3 * na + i : 3 * nb + j;
+              const int c = (na >= nb) ? 3 * nb + j : 3 * na + i;
+              /* locate (r, c) */
+              long lo = M->rowptr[r], hi = M->rowptr[r + 1] - 1;
+              while (lo <= hi) {
+                long mid = (lo + hi) / 2;
+                if (M->col[mid] == c) {
+                  s = mid;
+                  break;
+                }
+                if (M->col[mid] < c) lo = mid + 1;
+                else hi = mid - 1;
+              }
+            }
+            M->slot[144L * e + d * 12 + f] = s;
+          }
+      }
+  }
+  free(deg);
+  free(aptr);
+  free(adj);
+  free(ucount);
+  return M;
+}
+
+void orc_model_destroy(orc_model* M) {
+  if (!M) return;
+  free(M->X0);
+  free(M->x);
+  free(M->T);
+  free(M->kind);
+  free(M->lame1);
+  free(M->lame2);
+  free(M->rho);
+  free(M->DFDx);
+  free(M->V0);
+  free(M->elem_mass);
+  free(M->fixed);
+  free(M->f_ini);
+  free(M->f_end);
+  free(M->eE);
+  free(M->eg);
+  free(M->eH);
+  free(M->rowptr);
+  free(M->col);
+  free(M->slot);
+  free(M);
+}
+
+int orc_model_num_dofs(const orc_model* M) { return 3 * M->n; }
+
+void orc_model_set_fixed(orc_model* M, const unsigned char* mask) {
+  memcpy(M->fixed, mask, 3 * M->n);
+}
+
+void orc_model_set_gravity(orc_model* M, const double g_ini[3],
+                           const double g_end[3], int stages) {
+  M->has_g = 1;
+  M->g_stages = stages < 1 ? 1 : stages;
+  M->g_cur = 0;
+  memcpy(M->g_ini, g_ini, sizeof(double) * 3);
+  memcpy(M->g_end, g_end, sizeof(double) * 3);
+}
+
+void orc_model_set_forces(orc_model* M, const double* f_ini,
+                          const double* f_end, int stages) {
+  const int N = 3 * M->n;
+  M->has_f = 1;
+  M->f_stages = stages < 1 ? 1 : stages;
+  M->f_cur = 0;
+  free(M->f_ini);
+  free(M->f_end);
+  M->f_ini = (double*)malloc(sizeof(double) * N);
+  M->f_end = (double*)malloc(sizeof(double) * N);
+  memcpy(M->f_ini, f_ini, sizeof(double) * N);
+  memcpy(M->f_end, f_end, sizeof(double) * N);
+}
+
+/* BCondition.cpp:42 StepLoading; BCondition.h:128 IsLoaded */
+int orc_model_step_load(orc_model* M) {
+  int stepped = 0;
+  if (M->has_g && M->g_cur < M->g_stages) {
+    M->g_cur++;
+    stepped = 1;
+  }
+  if (M->has_f && M->f_cur < M->f_stages) {
+    M->f_cur++;
+    stepped = 1;
+  }
+  return stepped;
+}
+
+int orc_model_fully_loaded(const orc_model* M) {
+  if (M->has_g && M->g_cur != M->g_stages) return 0;
+  if (M->has_f && M->f_cur != M->f_stages) return 0;
+  return 1;
+}
+
+void orc_model_set_x(orc_model* M, const double* x) {
+  memcpy(M->x, x, sizeof(double) * 3 * M->n);
+}
+void orc_model_get_x(const orc_model* M, double* x) {
+  memcpy(x, M->x, sizeof(double) * 3 * M->n);
+}
+
+void orc_model_get_mass(const orc_model* M, double* mass3n) {
+  memset(mass3n, 0, sizeof(double) * 3 * M->n);
+  for (int e = 0; e < M->m; ++e)
+    for (int a = 0; a < 4; ++a)
+      for (int i = 0; i < 3; ++i) mass3n[3 * M->T[4 * e + a] + i] += M->elem_mass[e];
+}
+
+/* BCondition.h:140-155 GetCurValues with VectorAnimator_LerpValues */
+static void cur_gravity(const orc_model* M, double g[3]) {
+  if (!M->has_g) {
+    g[0] = g[1] = g[2] = 0;
+    return;
+  }
+  const double a = (double)M->g_cur / (double)M->g_stages;
+  for (int i = 0; i < 3; ++i) g[i] = M->g_ini[i] + (M->g_end[i] - M->g_ini[i]) * a;
+}
+
+static double cur_force(const orc_model* M, int k) {
+  const double a = (double)M->f_cur / (double)M->f_stages;
+  return M->f_ini[k] + (M->f_end[k] - M->f_ini[k]) * a;
+}
+
+static void gather_x(const orc_model* M, int e, double xe[12]) {
+  for (int a = 0; a < 4; ++a)
+    for (int i = 0; i < 3; ++i) xe[3 * a + i] = M->x[3 * M->T[4 * e + a] + i];
+}
+
+double orc_model_energy(orc_model* M) {
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < M->m; ++e) {
+    double xe[12];
+    gather_x(M, e, xe);
+    orc_tet_egh(xe, M->DFDx + 108 * e, M->V0[e], M->kind[e], M->lame1[e],
+                M->lame2[e], M->eE + e, NULL, NULL);
+  }
+  double E = 0;
+  for (int e = 0; e < M->m; ++e) E += M->eE[e];
+  /* MassElement_Lumped.cpp:56-68: E = x . (-m g) per mass element */
+  double g[3];
+  cur_gravity(M, g);
+  for (int e = 0; e < M->m; ++e) {
+    double Ee = 0;
+    for (int a = 0; a < 4; ++a)
+      for (int i = 0; i < 3; ++i)
+        Ee += M->x[3 * M->T[4 * e + a] + i] * (-M->elem_mass[e] * g[i]);
+    E += Ee;
+  }
+  /* EnergyElement_Force.cpp:43-47: E = x . f */
+  if (M->has_f) {
+    double Ef = 0;
+    for (int k = 0; k < 3 * M->n; ++k) Ef += M->x[k] * cur_force(M, k);
+    E += Ef;
+  }
+  return E;
+}
+
+void orc_model_gradient(orc_model* M, double* gout) {
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < M->m; ++e) {
+    double xe[12];
+    gather_x(M, e, xe);
+    orc_tet_egh(xe, M->DFDx + 108 * e, M->V0[e], M->kind[e], M->lame1[e],
+                M->lame2[e], NULL, M->eg + 12 * e, NULL);
+  }
+  const int N = 3 * M->n;
+  memset(gout, 0, sizeof(double) * N);
+  for (int e = 0; e < M->m; ++e)
+    for (int a = 0; a < 4; ++a)
+      for (int i = 0; i < 3; ++i) gout[3 * M->T[4 * e + a] + i] += M->eg[12 * e + 3 * a + i];
+  double g[3];
+  cur_gravity(M, g);
+  for (int e = 0; e < M->m; ++e)
+    for (int a = 0; a < 4; ++a)
+      for (int i = 0; i < 3; ++i) gout[3 * M->T[4 * e + a] + i] += -M->elem_mass[e] * g[i];
+  if (M->has_f)
+    for (int k = 0; k < N; ++k) gout[k] += cur_force(M, k);
+  /* Simulable.cpp:735-744 FixVectorDoFs */
+  for (int k = 0; k < N; ++k)
+    if (M->fixed[k]) gout[k] = 0;
+}
+
+long orc_model_nnz_lower(orc_model* M) { return M->nnz; }
+
+void orc_model_hessian_lower(orc_model* M, long* rowptr, int* col,
+                             double* val) {
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < M->m; ++e) {
+    double xe[12];
+    gather_x(M, e, xe);
+    orc_tet_egh(xe, M->DFDx + 108 * e, M->V0[e], M->kind[e], M->lame1[e],
+                M->lame2[e], NULL, NULL, M->eH + 144 * e);
+  }
+  const int N = 3 * M->n;
+  memcpy(rowptr, M->rowptr, sizeof(long) * (N + 1));
+  memcpy(col, M->col, sizeof(int) * M->nnz);
+  memset(val, 0, sizeof(double) * M->nnz);
+  for (int e = 0; e < M->m; ++e) {
+    const long* s = M->slot + 144L * e;
+    const double* He = M->eH + 144L * e;
+    for (int q = 0; q < 144; ++q)
+      if (s[q] >= 0) val[s[q]] += He[q];
+  }
+  /* Simulable.cpp:746-760 FixMatrixDoFs */
+  for (int r = 0; r < N; ++r)
+    for (long p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+      const int c = col[p];
+      if (M->fixed[r] || M->fixed[c]) val[p] = (r == c) ? 1.0 : 0.0;
+    }
+}
+
+void orc_model_element_egh(orc_model* M, double* E, double* g, double* H) {
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < M->m; ++e) {
+    double xe[12];
+    gather_x(M, e, xe);
+    orc_tet_egh(xe, M->DFDx + 108 * e, M->V0[e], M->kind[e], M->lame1[e],
+                M->lame2[e], E ? E + e : NULL, g ? g + 12L * e : NULL,
+                H ? H + 144L * e : NULL);
+  }
+}
+
+/* ======================================================================= */
+/* Conjugate gradient (Eigen ConjugateGradient, UpLo = Lower, Jacobi        */
+/* preconditioner) -- textbook PCG (Saad, Iterative Methods, Alg. 9.1) with */
+/* Eigen's stopping rule |r|^2 < max(tol^2 |b|^2, DBL_MIN) and warm start.   */
+/* ======================================================================= */
+
+/* y = A x using only stored entries with row >= col (selfadjoint lower) */
+static void symv_lower(int n, const long* rp, const int* ci, const double* v,
+                       const double* x, double* y) {
+  memset(y, 0, sizeof(double) * n);
+  for (int r = 0; r < n; ++r) {
+    double acc = 0;
+    for (long p = rp[r]; p < rp[r + 1]; ++p) {
+      const int c = ci[p];
+      if (c < r) {
+        acc += v[p] * x[c];
+        y[c] += v[p] * x[r];
+      } else if (c == r) {
+        acc += v[p] * x[r];
+      }
+    }
+    y[r] += acc;
+  }
+}
+
+static double dot(int n, const double* a, const double* b) {
+  double s = 0;
+  for (int i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+int orc_cg(int n, const long* rp, const int* ci, const double* v,
+           const double* b, double* x, double tol, int maxit, double* err) {
+  double* r = (double*)malloc(sizeof(double) * n);
+  double* z = (double*)malloc(sizeof(double) * n);
+  double* p = (double*)malloc(sizeof(double) * n);
+  double* q = (double*)malloc(sizeof(double) * n);
+  double* dinv = (double*)malloc(sizeof(double) * n);
+  for (int i = 0; i < n; ++i) dinv[i] = 1.0;
+  for (int r0 = 0; r0 < n; ++r0)
+    for (long k = rp[r0]; k < rp[r0 + 1]; ++k)
+      if (ci[k] == r0 && v[k] != 0.0) dinv[r0] = 1.0 / v[k];
+  symv_lower(n, rp, ci, v, x, q);
+  for (int i = 0; i < n; ++i) r[i] = b[i] - q[i];
+  const double bb = dot(n, b, b);
+  int it = 0;
+  if (bb == 0.0) {
+    for (int i = 0; i < n; ++i) x[i] = 0;
+    *err = 0;
+    goto done;
+  }
+  {
+    double thr = tol * tol * bb;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+    double rr = dot(n, r, r);
+    if (rr < thr) {
+      *err = sqrt(rr / bb);
+      goto done;
+    }
+    for (int i = 0; i < n; ++i) p[i] = dinv[i] * r[i];
+    double rz = dot(n, r, p);
+    while (it < maxit) {
+      symv_lower(n, rp, ci, v, p, q);
+      const double alpha = rz / dot(n, p, q);
+      for (int i = 0; i < n; ++i) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+      }
+      rr = dot(n, r, r);
+      if (rr < thr) break;
+      for (int i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
+      const double rz_old = rz;
+      rz = dot(n, r, z);
+      const double beta = rz / rz_old;
+      for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+      ++it;
+    }
+    *err = sqrt(rr / bb);
+  }
+done:
+  free(r);
+  free(z);
+  free(p);
+  free(q);
+  free(dinv);
+  return it;
+}
+
+int orc_linear_solve(int n, const long* rp, const int* ci, double* v,
+                     const double* b, double* x, double tol, int maxit,
+                     int reg_iters, int* tot_iters, int* reg_used) {
+  /* LinearSolver.cpp:57-66: normA = Frobenius norm of the stored entries */
+  double nA = 0;
+  for (long k = 0; k < rp[n]; ++k) nA += v[k] * v[k];
+  nA = sqrt(nA);
+  const double regT = 1e-6 * nA;
+  double* xprev = (double*)malloc(sizeof(double) * n);
+  memcpy(xprev, x, sizeof(double) * n);
+  int status = 1;
+  *reg_used = 0;
+  for (int i = 0; i < reg_iters; ++i) {
+    if (i != 0) {
+      /* LinearSolver_EigenCG.cpp:73 mA += R * regT * 10^(i-1), cumulative */
+      const double s = regT * pow(10.0, i - 1);
+      for (int r0 = 0; r0 < n; ++r0)
+        for (long k = rp[r0]; k < rp[r0 + 1]; ++k)
+          if (ci[k] == r0) v[k] += s;
+      *reg_used = i;
+    }
+    memcpy(x, xprev, sizeof(double) * n);
+    double err;
+    const int its = orc_cg(n, rp, ci, v, b, x, tol, maxit, &err);
+    *tot_iters += its;
+    if (dot(n, b, x) < 0.0) continue; /* :97-101 indefinite */
+    status = 0;
+    break;
+  }
+  free(xprev);
+  return status;
+}
+
+/* ======================================================================= */
+/* Newton (OptimSolver + OptimSolver_USQP_LS + OptimProblem_BasicStatic)    */
+/* ======================================================================= */
+
+void orc_newton_default_opts(orc_newton_opts* o) {
+  /* OptimSolver.h:47-70 */
+  o->tolMaxError = 1e-6;
+  o->tolMinStep = 1e-5;
+  o->minStepMaxCount = 3;
+  o->tolMinImpr = 1e-9;
+  o->minImprMaxCount = 3;
+  o->numMaxIters = 100;
+  o->trySteepest = 1;
+  o->maxStepSize = 1000;
+  o->lsNumRegTrials = 10;
+  o->lsNumMaxIters = 1000;
+  o->lsTolMaxError = 1e-9;
+  o->lsearch_numTry = 32;
+  o->lSearch_alpha = 0.5;
+}
+
+static double norm2(int n, const double* a) { return sqrt(dot(n, a, a)); }
+
+typedef struct {
+  int N;
+  double *vx, *v0, *vg, *dx, *dxo, *vgNew, *xsave;
+  double obj, opt;
+  int result, minImprCount, minStepCount, steepest;
+  long* rp;
+  int* ci;
+  double* val;
+  int cg_its, bis;
+} nstate;
+
+/* OptimSolver_USQP_LS.cpp:18-68 */
+static void solve_step(orc_model* M, const orc_newton_opts* o, nstate* S,
+                       int steepest) {
+  const int N = S->N;
+  /* OnIterStart: StepBoundaryConditions + GetState */
+  orc_model_step_load(M);
+  memcpy(S->xsave, M->x, sizeof(double) * N);
+  S->obj = orc_model_energy(M);
+  orc_model_gradient(M, S->vg);
+  memcpy(S->vx, M->x, sizeof(double) * N);
+  S->opt = norm2(N, S->vg);
+  if (S->opt < o->tolMaxError) {
+    S->result = ORC_OS_SUCCESS;
+    return;
+  }
+  /* ComputeStep (OptimSolver.cpp:313-352); trial 0 always succeeds */
+  if (!steepest) {
+    orc_model_hessian_lower(M, S->rp, S->ci, S->val);
+    for (int k = 0; k < N; ++k) S->dx[k] = S->vx[k] - S->v0[k]; /* :488 */
+    double* b = S->dxo;
+    for (int k = 0; k < N; ++k) b[k] = -S->vg[k];
+    int reg_used = 0;
+    orc_linear_solve(N, S->rp, S->ci, S->val, b, S->dx, o->lsTolMaxError,
+                     o->lsNumMaxIters, o->lsNumRegTrials, &S->cg_its,
+                     &reg_used);
+  } else {
+    for (int k = 0; k < N; ++k) S->dx[k] = -S->vg[k];
+  }
+  const double len = norm2(N, S->dx);
+  if (len > o->maxStepSize)
+    for (int k = 0; k < N; ++k) S->dx[k] = S->dx[k] / len * o->maxStepSize;
+  /* LineSearch_Simple (OptimSolver.cpp:688-736) */
+  double factor = 1.0, objNew = 0;
+  int improved = 0, numBis = -1;
+  memcpy(M->x, S->xsave, sizeof(double) * N); /* PrePerformStep */
+  do {
+    numBis++;
+    for (int k = 0; k < N; ++k) S->dxo[k] = factor * S->dx[k];
+    for (int k = 0; k < N; ++k) M->x[k] = S->xsave[k] + S->dxo[k];
+    objNew = orc_model_energy(M);
+    orc_model_gradient(M, S->vgNew);
+    if (objNew < S->obj) {
+      improved = 1;
+      break;
+    }
+    factor *= o->lSearch_alpha;
+    memcpy(M->x, S->xsave, sizeof(double) * N);
+  } while (norm2(N, S->dxo) > o->tolMinStep && numBis <= o->lsearch_numTry);
+  (void)improved;
+  S->bis += numBis;
+  /* UpdateStepResult (OptimSolver.cpp:402-466) */
+  const double impr = S->obj - objNew;
+  const double step = norm2(N, S->dxo);
+  if (impr > 0) {
+    const double optNew = norm2(N, S->vgNew);
+    if (impr < o->tolMinImpr) {
+      S->minImprCount++;
+      S->result = ORC_OS_MINIMPR;
+    } else if (step < o->tolMinStep) {
+      S->minStepCount++;
+      S->result = ORC_OS_MINSTEP;
+    } else if (optNew <= o->tolMaxError) {
+      S->result = ORC_OS_SUCCESS;
+      S->minImprCount = 0;
+      S->minStepCount = 0;
+    } else {
+      S->result = ORC_OS_ONGOING;
+      S->minImprCount = 0;
+      S->minStepCount = 0;
+    }
+    memcpy(S->v0, S->vx, sizeof(double) * N);
+    for (int k = 0; k < N; ++k) S->vx[k] += S->dxo[k];
+    memcpy(S->vg, S->vgNew, sizeof(double) * N);
+    S->obj = objNew;
+    S->opt = optNew;
+  } else {
+    S->result = ORC_OS_NONDESC;
+  }
+  /* OnIterFinish: UpdateKinematics (no kinematic state for tets) */
+}
+
+void orc_newton_solve(orc_model* M, const orc_newton_opts* o,
+                      orc_newton_result* R) {
+  nstate S;
+  memset(&S, 0, sizeof(S));
+  const int N = 3 * M->n;
+  S.N = N;
+  S.vx = (double*)malloc(sizeof(double) * N);
+  S.v0 = (double*)malloc(sizeof(double) * N);
+  S.vg = (double*)malloc(sizeof(double) * N);
+  S.dx = (double*)malloc(sizeof(double) * N);
+  S.dxo = (double*)malloc(sizeof(double) * N);
+  S.vgNew = (double*)malloc(sizeof(double) * N);
+  S.xsave = (double*)malloc(sizeof(double) * N);
+  S.rp = (long*)malloc(sizeof(long) * (N + 1));
+  S.ci = (int*)malloc(sizeof(int) * M->nnz);
+  S.val = (double*)malloc(sizeof(double) * M->nnz);
+  /* OptimSolver::Init (:36-55) */
+  memcpy(S.vx, M->x, sizeof(double) * N);
+  memcpy(S.v0, M->x, sizeof(double) * N);
+  S.obj = orc_model_energy(M);
+  orc_model_gradient(M, S.vg);
+  S.opt = norm2(N, S.vg);
+  /* SolveFull (:163-264) */
+  S.result = ORC_OS_SUCCESS;
+  S.steepest = 0;
+  int itCount = 0;
+  while (1) {
+    itCount++;
+    solve_step(M, o, &S, S.steepest);
+    orc_model_gradient(M, S.vg);
+    S.opt = norm2(N, S.vg);
+    if (orc_model_fully_loaded(M)) {
+      if (S.result == ORC_OS_NONDESC) {
+        if (!o->trySteepest) break;
+        if (S.steepest) break;
+        S.steepest = 1;
+      }
+      if (S.result == ORC_OS_MINIMPR) {
+        if (S.minImprCount == o->minImprMaxCount) break;
+        if (o->trySteepest) S.steepest = !S.steepest;
+      }
+      if (S.result == ORC_OS_MINSTEP) {
+        if (S.minStepCount == o->minStepMaxCount) break;
+        if (o->trySteepest) S.steepest = !S.steepest;
+      }
+      if (S.result == ORC_OS_SUCCESS) {
+        if (S.opt <= o->tolMaxError) break;
+        S.result = ORC_OS_ONGOING;
+      }
+      if (S.result == ORC_OS_ONGOING) S.steepest = 0;
+    }
+    if (itCount + 1 > o->numMaxIters) {
+      S.result = ORC_OS_MAXITER;
+      break;
+    }
+  }
+  R->result = S.result;
+  R->iterations = itCount;
+  R->cg_iterations = S.cg_its;
+  R->line_search_bisections = S.bis;
+  R->objective = S.obj;
+  R->optimality = S.opt;
+  free(S.vx);
+  free(S.v0);
+  free(S.vg);
+  free(S.dx);
+  free(S.dxo);
+  free(S.vgNew);
+  free(S.xsave);
+  free(S.rp);
+  free(S.ci);
+  free(S.val);
+}

@@ -1,0 +1,154 @@
+/*
+ * oracle.h -- FP64 CPU restatement of PhySim's quasi-static Newton inner loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product path (libpgn.so) never links it.
+ *
+ * Every function cites the reference file:line it restates.  Parity pins:
+ *   - material Psi / P / dP/dF  : golden vectors from the reference's own Maple
+ *                                 kernels (oracle/_ref) + closed-form invariants
+ *   - Tet4 element E/g/H        : finite differences (reference eps/tol), brute
+ *                                 force identities (sum of g = 0, H*translation=0)
+ *   - mesh generator            : node/tet counts and volumes of the grid beam
+ *   - assembly / fixed DoFs     : dense brute-force sum on tiny meshes
+ *   - PCG                       : scipy/numpy dense solves on small SPD systems
+ *   - Newton driver             : converged equilibrium (unique) and the
+ *                                 Euler-Bernoulli small-load limit.
+ *     Iteration counts / trajectories of the reference's Eigen build are
+ *     "parity unpinned" (Eigen is unpinned and absent).
+ */
+#ifndef PHYSIM_ORACLE_H
+#define PHYSIM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_CONH = 0, ORC_STVK = 1 };
+
+/* ---- geometry ---------------------------------------------------------- */
+/* MeshUtils.cpp:927-991 CreateTetMesh_Grid.  V: 3*numNodes (node-major xyz),
+ * T: 4*numTets.  Returns number of tets. */
+int orc_tet_grid(const double scale[3], const int dims[3], double* V, int* T);
+int orc_tet_grid_counts(const int dims[3], int* num_nodes, int* num_tets);
+
+/* ParameterSet.cpp:54-70 */
+void orc_lame_from_young_poisson(double young, double poisson, double* lame1,
+                                 double* lame2);
+
+/* ---- material models (column-major vec(F), MathUtils.h:97-111) --------- */
+/* MaterialModels.h:166-230 (StVK), :235-299 (CoNH) */
+double orc_psi(int kind, const double* Fv, double lame1, double lame2);
+void orc_pk1(int kind, const double* Fv, double lame1, double lame2,
+             double* P);
+void orc_dpdf(int kind, const double* Fv, double lame1, double lame2,
+              double* C);
+
+/* ---- Tet4 element, 1-point quadrature ---------------------------------- */
+/* ShapeFunctions.cpp:89-133 + EnergyElement_FEM.cpp:59-122.  X: 12 rest
+ * coords (node-major).  DFDx: 9x12 row-major.  *V0 = (1/6) det(J). */
+void orc_tet_rest(const double X[12], double DFDx[108], double* V0);
+/* EnergyElement_FEM.cpp:125-222.  Returns 1 if the element is inverted (then
+ * E = HUGE_VAL and g/H are still evaluated like the reference does). */
+int orc_tet_egh(const double x[12], const double DFDx[108], double V0,
+                int kind, double lame1, double lame2, double* E, double g[12],
+                double H[144]);
+
+/* ---- model (Simulable_FEM + MassElement_Lumped + BCs) ------------------ */
+typedef struct orc_model orc_model;
+
+orc_model* orc_model_create(int n_nodes, const double* X0, int n_tets,
+                            const int* T, const unsigned char* kind,
+                            const double* lame1, const double* lame2,
+                            const double* rho);
+void orc_model_destroy(orc_model* m);
+int orc_model_num_dofs(const orc_model* m);
+void orc_model_set_fixed(orc_model* m, const unsigned char* mask);
+/* BC_Gravity + VectorAnimator_LerpValues; stages >= 1.  The current value is
+ * lerp(g_ini, g_end, cur/stages); cur starts at 0. */
+void orc_model_set_gravity(orc_model* m, const double g_ini[3],
+                           const double g_end[3], int stages);
+/* BC_Force (EnergyElement_Force: E = +x.f, g = +f), lerp over stages. */
+void orc_model_set_forces(orc_model* m, const double* f_ini,
+                          const double* f_end, int stages);
+int orc_model_step_load(orc_model* m);
+int orc_model_fully_loaded(const orc_model* m);
+void orc_model_set_x(orc_model* m, const double* x);
+void orc_model_get_x(const orc_model* m, double* x);
+/* lumped node masses (3n, replicated per component) */
+void orc_model_get_mass(const orc_model* m, double* mass3n);
+
+/* Simulable::AddEnergy (elements in order, then mass, then BC energies) */
+double orc_model_energy(orc_model* m);
+/* Simulable::GetGradient + FixVectorDoFs */
+void orc_model_gradient(orc_model* m, double* g);
+/* lower-triangular CSR (row >= col) pattern size */
+long orc_model_nnz_lower(orc_model* m);
+/* Simulable::GetHessian + FixMatrixDoFs, lower CSR. rowptr: 3n+1. */
+void orc_model_hessian_lower(orc_model* m, long* rowptr, int* col,
+                             double* val);
+/* per-element local outputs (IEnergyElement::GetLocal*) */
+void orc_model_element_egh(orc_model* m, double* E, double* g, double* H);
+
+/* ---- linear solve: Eigen ConjugateGradient<Lower, Diagonal> ------------ */
+/* A given as lower CSR of a symmetric matrix.  Returns iterations, writes the
+ * relative residual to *err. */
+int orc_cg(int n, const long* rowptr, const int* col, const double* val,
+           const double* b, double* x, double tol, int maxit, double* err);
+/* LinearSolver_EigenCG::SolveInternal (+ LinearSolver::GetMatrixData):
+ * b.x < 0 => val += regT*10^(i-1) on the diagonal (cumulative), <= regIters.
+ * Returns 0 on SUCCESS, 1 on FAILURE.  *tot_iters accumulates CG iterations.
+ * val is modified in place like the reference's mA. */
+int orc_linear_solve(int n, const long* rowptr, const int* col, double* val,
+                     const double* b, double* x, double tol, int maxit,
+                     int reg_iters, int* tot_iters, int* reg_used);
+
+/* ---- Newton driver ----------------------------------------------------- */
+/* OptimSolverOptions (OptimSolver.h:23-71) */
+typedef struct {
+  double tolMaxError;
+  double tolMinStep;
+  double tolMinImpr;
+  int minStepMaxCount;
+  int minImprMaxCount;
+  int numMaxIters;
+  double maxStepSize;
+  int trySteepest;
+  int lsearch_numTry;
+  double lSearch_alpha;
+  int lsNumMaxIters;
+  double lsTolMaxError;
+  int lsNumRegTrials;
+} orc_newton_opts;
+
+/* OSResult (BasicTypes.h:69-77) */
+enum {
+  ORC_OS_ONGOING = 0,
+  ORC_OS_SUCCESS,
+  ORC_OS_MINSTEP,
+  ORC_OS_MINIMPR,
+  ORC_OS_FAILURE,
+  ORC_OS_MAXITER,
+  ORC_OS_NONDESC
+};
+
+typedef struct {
+  int result;
+  int iterations;
+  int cg_iterations;
+  int line_search_bisections;
+  double objective;
+  double optimality;
+} orc_newton_result;
+
+void orc_newton_default_opts(orc_newton_opts* o);
+/* OptimSolver::Init + SolveFull (OptimSolver.cpp:36-55,163-264) with
+ * OptimSolver_USQP_LS::SolveStepInternal and OptimProblem_BasicStatic. */
+void orc_newton_solve(orc_model* m, const orc_newton_opts* o,
+                      orc_newton_result* r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

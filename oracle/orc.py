@@ -1,0 +1,282 @@
+"""ctypes wrapper for the FP64 CPU oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, always as the checker or the timed CPU
+baseline -- never as the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+CONH, STVK = 0, 1
+OS_NAMES = ["ONGOING", "SUCCESS", "MINSTEP", "MINIMPR", "FAILURE", "MAXITER",
+            "NONDESC"]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_ucp = ctypes.POINTER(ctypes.c_ubyte)
+
+
+class NewtonOpts(ctypes.Structure):
+    _fields_ = [("tolMaxError", ctypes.c_double),
+                ("tolMinStep", ctypes.c_double),
+                ("tolMinImpr", ctypes.c_double),
+                ("minStepMaxCount", ctypes.c_int),
+                ("minImprMaxCount", ctypes.c_int),
+                ("numMaxIters", ctypes.c_int),
+                ("maxStepSize", ctypes.c_double),
+                ("trySteepest", ctypes.c_int),
+                ("lsearch_numTry", ctypes.c_int),
+                ("lSearch_alpha", ctypes.c_double),
+                ("lsNumMaxIters", ctypes.c_int),
+                ("lsTolMaxError", ctypes.c_double),
+                ("lsNumRegTrials", ctypes.c_int)]
+
+
+class NewtonResult(ctypes.Structure):
+    _fields_ = [("result", ctypes.c_int),
+                ("iterations", ctypes.c_int),
+                ("cg_iterations", ctypes.c_int),
+                ("line_search_bisections", ctypes.c_int),
+                ("objective", ctypes.c_double),
+                ("optimality", ctypes.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        L.orc_psi.restype = ctypes.c_double
+        L.orc_psi.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double]
+        for f in ("orc_pk1", "orc_dpdf"):
+            getattr(L, f).argtypes = [ctypes.c_int, _dp, ctypes.c_double,
+                                      ctypes.c_double, _dp]
+        L.orc_tet_rest.argtypes = [_dp, _dp, _dp]
+        L.orc_tet_egh.restype = ctypes.c_int
+        L.orc_tet_egh.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_tet_grid.argtypes = [_dp, _ip, _dp, _ip]
+        L.orc_model_create.restype = ctypes.c_void_p
+        L.orc_model_create.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _ucp,
+                                       _dp, _dp, _dp]
+        L.orc_model_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_model_set_fixed.argtypes = [ctypes.c_void_p, _ucp]
+        L.orc_model_set_gravity.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.orc_model_set_forces.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.orc_model_step_load.argtypes = [ctypes.c_void_p]
+        L.orc_model_fully_loaded.argtypes = [ctypes.c_void_p]
+        L.orc_model_set_x.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_model_get_x.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_model_get_mass.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_model_energy.restype = ctypes.c_double
+        L.orc_model_energy.argtypes = [ctypes.c_void_p]
+        L.orc_model_gradient.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_model_nnz_lower.restype = ctypes.c_long
+        L.orc_model_nnz_lower.argtypes = [ctypes.c_void_p]
+        L.orc_model_hessian_lower.argtypes = [ctypes.c_void_p, _lp, _ip, _dp]
+        L.orc_model_element_egh.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_cg.restype = ctypes.c_int
+        L.orc_cg.argtypes = [ctypes.c_int, _lp, _ip, _dp, _dp, _dp,
+                             ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_linear_solve.restype = ctypes.c_int
+        L.orc_linear_solve.argtypes = [ctypes.c_int, _lp, _ip, _dp, _dp, _dp,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       _ip, _ip]
+        L.orc_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
+        L.orc_newton_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
+                                       ctypes.POINTER(NewtonResult)]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---- element-level --------------------------------------------------------
+
+def psi(kind, F, lame1, lame2):
+    Fv = f64(np.asarray(F).T.reshape(-1))
+    return lib().orc_psi(kind, _p(Fv), lame1, lame2)
+
+
+def pk1(kind, F, lame1, lame2):
+    Fv = f64(np.asarray(F).T.reshape(-1))
+    out = np.zeros(9)
+    lib().orc_pk1(kind, _p(Fv), lame1, lame2, _p(out))
+    return out
+
+
+def dpdf(kind, F, lame1, lame2):
+    Fv = f64(np.asarray(F).T.reshape(-1))
+    out = np.zeros(81)
+    lib().orc_dpdf(kind, _p(Fv), lame1, lame2, _p(out))
+    return out.reshape(9, 9)
+
+
+def tet_rest(X):
+    X = f64(np.asarray(X).reshape(12))
+    D = np.zeros(108)
+    V0 = ctypes.c_double()
+    lib().orc_tet_rest(_p(X), _p(D), ctypes.byref(V0))
+    return D.reshape(9, 12), V0.value
+
+
+def tet_egh(x, DFDx, V0, kind, lame1, lame2):
+    x = f64(np.asarray(x).reshape(12))
+    D = f64(DFDx)
+    E = ctypes.c_double()
+    g = np.zeros(12)
+    H = np.zeros(144)
+    inv = lib().orc_tet_egh(_p(x), _p(D), V0, kind, lame1, lame2,
+                            ctypes.byref(E), _p(g), _p(H))
+    return E.value, g, H.reshape(12, 12), bool(inv)
+
+
+def tet_grid(scale, dims):
+    dims = np.asarray(dims, dtype=np.int32)
+    n = int(np.prod(dims + 1))
+    m = int(5 * np.prod(dims))
+    V = np.zeros(3 * n)
+    T = np.zeros(4 * m, dtype=np.int32)
+    lib().orc_tet_grid(_p(f64(scale)), _p(dims, _ip), _p(V), _p(T, _ip))
+    return V.reshape(n, 3), T.reshape(m, 4)
+
+
+def lame(young, poisson):
+    l1, l2 = ctypes.c_double(), ctypes.c_double()
+    t0 = 1.0 + poisson
+    t1 = 1.0 - 2.0 * poisson
+    return (young * poisson) / (t0 * t1), young / (2.0 * t0)
+
+
+# ---- model-level ----------------------------------------------------------
+
+class Model:
+    """Simulable_FEM (Tet4 + lumped mass) with hard-fixed DoFs, gravity and
+    nodal-force boundary conditions, restated on the CPU in FP64."""
+
+    def __init__(self, X0, T, kind, lame1, lame2, rho):
+        X0 = f64(X0).reshape(-1)
+        T = np.ascontiguousarray(T, dtype=np.int32).reshape(-1)
+        m = T.size // 4
+        kind = np.ascontiguousarray(np.broadcast_to(kind, (m,)), dtype=np.uint8)
+        l1 = f64(np.broadcast_to(lame1, (m,)))
+        l2 = f64(np.broadcast_to(lame2, (m,)))
+        rh = f64(np.broadcast_to(rho, (m,)))
+        self.n = X0.size // 3
+        self.m = m
+        self.N = 3 * self.n
+        self._h = lib().orc_model_create(self.n, _p(X0), m, _p(T, _ip),
+                                         _p(kind, _ucp), _p(l1), _p(l2), _p(rh))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_model_destroy(h)
+            self._h = None
+
+    def set_fixed(self, mask):
+        mk = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        lib().orc_model_set_fixed(self._h, _p(mk, _ucp))
+
+    def set_gravity(self, g_end, stages, g_ini=(0.0, 0.0, 0.0)):
+        lib().orc_model_set_gravity(self._h, _p(f64(g_ini)), _p(f64(g_end)),
+                                    int(stages))
+
+    def set_forces(self, f_end, stages, f_ini=None):
+        f_end = f64(f_end).reshape(-1)
+        f_ini = np.zeros_like(f_end) if f_ini is None else f64(f_ini).reshape(-1)
+        lib().orc_model_set_forces(self._h, _p(f_ini), _p(f_end), int(stages))
+
+    def step_load(self):
+        return lib().orc_model_step_load(self._h)
+
+    def fully_loaded(self):
+        return bool(lib().orc_model_fully_loaded(self._h))
+
+    def set_x(self, x):
+        x = f64(x).reshape(-1)
+        lib().orc_model_set_x(self._h, _p(x))
+
+    def get_x(self):
+        x = np.zeros(self.N)
+        lib().orc_model_get_x(self._h, _p(x))
+        return x
+
+    def mass(self):
+        m = np.zeros(self.N)
+        lib().orc_model_get_mass(self._h, _p(m))
+        return m
+
+    def energy(self):
+        return lib().orc_model_energy(self._h)
+
+    def gradient(self):
+        g = np.zeros(self.N)
+        lib().orc_model_gradient(self._h, _p(g))
+        return g
+
+    def hessian_lower(self):
+        nnz = lib().orc_model_nnz_lower(self._h)
+        rp = np.zeros(self.N + 1, dtype=np.int64)
+        ci = np.zeros(nnz, dtype=np.int32)
+        v = np.zeros(nnz)
+        lib().orc_model_hessian_lower(self._h, _p(rp, _lp), _p(ci, _ip), _p(v))
+        return rp, ci, v
+
+    def element_egh(self):
+        E = np.zeros(self.m)
+        g = np.zeros((self.m, 12))
+        H = np.zeros((self.m, 12, 12))
+        lib().orc_model_element_egh(self._h, _p(E), _p(g), _p(H))
+        return E, g, H
+
+    def newton(self, **kw):
+        o = NewtonOpts()
+        lib().orc_newton_default_opts(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        r = NewtonResult()
+        lib().orc_newton_solve(self._h, ctypes.byref(o), ctypes.byref(r))
+        return {"result": OS_NAMES[r.result], "iterations": r.iterations,
+                "cg_iterations": r.cg_iterations,
+                "bisections": r.line_search_bisections,
+                "objective": r.objective, "optimality": r.optimality}
+
+
+def cg(rowptr, col, val, b, x0, tol, maxit):
+    n = rowptr.size - 1
+    x = f64(x0).copy()
+    err = ctypes.c_double()
+    it = lib().orc_cg(n, _p(np.ascontiguousarray(rowptr, dtype=np.int64), _lp),
+                      _p(np.ascontiguousarray(col, dtype=np.int32), _ip),
+                      _p(f64(val)), _p(f64(b)), _p(x), tol, maxit, ctypes.byref(err))
+    return x, it, err.value
+
+
+def lower_to_dense(rowptr, col, val):
+    """Symmetric dense matrix from the reference-layout lower CSR (entries with
+    row < col, i.e. the upper part of stored diagonal node blocks, are
+    ignored like Eigen's selfadjointView<Lower>)."""
+    n = rowptr.size - 1
+    A = np.zeros((n, n))
+    for r in range(n):
+        for p in range(rowptr[r], rowptr[r + 1]):
+            c = col[p]
+            if c <= r:
+                A[r, c] = val[p]
+                A[c, r] = val[p]
+    return A
