@@ -1,0 +1,228 @@
+/*
+ * pgn.h -- C ABI of the MI355X-native PhySim Newton inner loop (libpgn.so).
+ *
+ * The library replaces, for the quasi-static FEM path, the C++ virtual
+ * interfaces PhySim's optimizer calls:
+ *   ISimulable       include/PhySim/Physics/Simulables/Simulable.h:29-102
+ *   IEnergyElement   include/PhySim/Physics/Elements/IEnergyElement.h:20-67
+ *   IOptimProblem    include/PhySim/Solvers/OptimProblem.h:61-97
+ *   ILinearSolver    include/PhySim/Solvers/LinearSolver.h:38-52
+ *   OptimSolver      include/PhySim/Solvers/OptimSolver.h:23-71 (options),
+ *                    source/PhySim/Solvers/OptimSolver.cpp:163 (SolveFull)
+ * Each entry point below names the reference method it stands in for.
+ *
+ * Conventions (kept from the reference):
+ *   - DoF vectors are FP64, node-major xyz (KinematicsTree.cpp:33-40);
+ *   - gradients have fixed entries zeroed (Simulable.cpp:735-744);
+ *   - Hessians have fixed rows/cols replaced by identity (Simulable.cpp:746-760);
+ *   - an inverted element has energy +HUGE_VAL (EnergyElement_FEM.cpp:126-131).
+ *
+ * Errors: every int-returning call returns PGN_OK (0) or a negative code; the
+ * message of the last failure on the calling thread is pgn_last_error().
+ * Ownership: the library owns all device memory; host buffers passed in are
+ * copied before the call returns.  Threading: one context = one HIP stream;
+ * calls on one context (and its models) must be externally serialized.
+ */
+#ifndef PGN_H
+#define PGN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGN_OK 0
+#define PGN_ERR_INVALID (-1)  /* bad argument / size / state */
+#define PGN_ERR_HIP (-2)      /* HIP runtime failure (message has details) */
+#define PGN_ERR_NOMEM (-3)    /* device or host allocation failed */
+#define PGN_ERR_NODEVICE (-4) /* no HIP device / extension unusable */
+
+/* element material (MaterialModels.h:166 StVK, :235 CoNH) */
+enum pgn_material { PGN_CONH = 0, PGN_STVK = 1 };
+
+/* arithmetic of a model's Newton path */
+enum pgn_precision {
+  PGN_FP64 = 0,  /* plain path: every kernel and the PCG in FP64 */
+  PGN_MIXED = 1  /* FP32 element Hessians + FP32 PCG; FP64 energy, gradient,
+                    residual norm and iterative-refinement residual */
+};
+
+/* Hessian export layouts */
+enum pgn_layout {
+  PGN_LAYOUT_LOWER = 0, /* the entry set AMatrixSd stores (BasicTypes.cpp:18-33):
+                           node blocks below the diagonal plus whole 3x3
+                           diagonal node blocks; row-major CSR, cols sorted */
+  PGN_LAYOUT_FULL = 1,  /* full symmetric scalar CSR */
+  PGN_LAYOUT_BSR3 = 2   /* full symmetric 3x3-block CSR, blocks row-major */
+};
+
+typedef struct pgn_ctx pgn_ctx;
+typedef struct pgn_model pgn_model;
+
+/* ---- context ------------------------------------------------------------ */
+const char* pgn_last_error(void);
+int pgn_ctx_create(int device, pgn_ctx** out);
+void pgn_ctx_destroy(pgn_ctx* ctx);
+/* the hipStream_t every call of this context is issued on */
+void* pgn_ctx_stream(pgn_ctx* ctx);
+int pgn_ctx_synchronize(pgn_ctx* ctx);
+
+/* ---- model setup -------------------------------------------------------- */
+/* Simulable_Mesh::InitInternal (Simulable_Mesh.cpp:68) +
+ * Simulable_FEM::CreateEnergyElements (Simulable_FEM.cpp:152) +
+ * EnergyElement_FEM::Init (EnergyElement_FEM.cpp:59) +
+ * MassElement_Lumped::ComputeAndStore_Mass (MassElement_Lumped.cpp:47).
+ * X0: 3*n_nodes rest positions; T: 4*n_tets node ids; material/lame1/lame2/
+ * density: per tet (ParameterSet::Param_Lame1/Lame2/Density). */
+int pgn_model_create_tet(pgn_ctx* ctx, int n_nodes, const double* X0,
+                         int n_tets, const int* T,
+                         const unsigned char* material, const double* lame1,
+                         const double* lame2, const double* density,
+                         int precision, pgn_model** out);
+void pgn_model_destroy(pgn_model* model);
+int pgn_model_num_dofs(const pgn_model* model);
+int pgn_model_num_elements(const pgn_model* model);
+/* bytes of device memory the model holds */
+size_t pgn_model_device_bytes(const pgn_model* model);
+
+/* BC_FixDoF hard fixation (BC_FixDoF.cpp:51-77): mask has 3*n_nodes bytes */
+int pgn_model_set_fixed(pgn_model* model, const unsigned char* mask);
+/* BC_Gravity (BC_Gravity.cpp:34) with VectorAnimator_LerpValues
+ * (VectorAnimators.h:36-69): value at stage k is lerp(g_ini, g_end, k/stages) */
+int pgn_model_set_gravity(pgn_model* model, const double g_ini[3],
+                          const double g_end[3], int num_stages);
+/* BC_Force (BC_Force.cpp) / EnergyElement_Force (E = +x.f, g = +f), lerped */
+int pgn_model_set_forces(pgn_model* model, const double* f_ini,
+                         const double* f_end, int num_stages);
+/* Simulable::StepBoundaryConditions (Simulable.cpp:1496) */
+int pgn_model_step_load(pgn_model* model, int* stepped);
+/* Simulable::BoundaryConditionsLoaded (Simulable.cpp:1476) */
+int pgn_model_is_loaded(const pgn_model* model);
+/* x := X0 and every load stage := 0 */
+int pgn_model_reset(pgn_model* model);
+
+/* ISimulable::SetDOFVector / GetDOFVector (Position_X) */
+int pgn_model_set_x(pgn_model* model, const double* x);
+int pgn_model_get_x(pgn_model* model, double* x);
+
+/* ---- evaluation (IOptimProblem / ISimulable) ---------------------------- */
+/* OptimProblem_BasicStatic::GetEnergy -> Simulable::AddEnergy (:321) */
+int pgn_model_get_energy(pgn_model* model, double* energy);
+/* OptimProblem_BasicStatic::GetGradient -> GetGradient + FixVectorDoFs */
+int pgn_model_get_gradient(pgn_model* model, double* gradient);
+/* Hessian pattern of a layout: nnz (scalar entries; blocks for BSR3) */
+int pgn_model_hessian_nnz(pgn_model* model, int layout, long* nnz);
+/* rowptr: 3n+1 (n+1 for BSR3) entries; col: nnz entries */
+int pgn_model_hessian_pattern(pgn_model* model, int layout, long* rowptr,
+                              int* col);
+/* OptimProblem_BasicStatic::GetHessian -> GetHessian + FixMatrixDoFs.
+ * Values in the model's Hessian precision, widened to FP64 (nnz*9 for BSR3) */
+int pgn_model_get_hessian(pgn_model* model, int layout, double* values);
+/* IEnergyElement::GetLocalEnergy/Gradient/Hessian for every element at the
+ * current x, from the fused E+g+H element kernel of the given precision
+ * (PGN_FP64 or PGN_MIXED = all-FP32 kernel).  Any pointer may be NULL.
+ * E: m values; g: 12 per element; H: full 12x12 row-major per element. */
+int pgn_model_element_egh(pgn_model* model, int precision, double* E,
+                          double* g, double* H);
+
+/* ---- linear solve (ILinearSolver / LinearSolver_EigenCG) ---------------- */
+typedef struct {
+  double tol;      /* relative residual |r| <= tol |b| (lsTolMaxError)   */
+  int max_iters;   /* lsNumMaxIters                                       */
+  int reg_iters;   /* LinearSolverOptions::regIters                       */
+  int precond;     /* 0 = Jacobi (reference), 1 = 3x3 block Jacobi         */
+  double inner_tol;/* MIXED only: FP32 inner relative tolerance per IR sweep */
+} pgn_linear_opts;
+
+typedef struct {
+  int status;      /* 0 = SUCCESS, 1 = FAILURE (LSResult)                  */
+  int iterations;  /* PCG iterations over all regularization tries         */
+  int reg_used;    /* regularization try that succeeded                   */
+  int refinements; /* MIXED: outer FP64 refinement sweeps                  */
+  double rel_residual;
+} pgn_linear_result;
+
+void pgn_linear_default_opts(pgn_linear_opts* o);
+/* LinearSolver_EigenCG::SolveInternal (LinearSolver_EigenCG.cpp:51-125) on the
+ * Hessian assembled at the current x: solves H x = b, x is the warm start. */
+int pgn_linear_solve(pgn_model* model, const double* b, double* x,
+                     const pgn_linear_opts* opts, pgn_linear_result* res);
+
+/* ---- Newton solve (OptimSolver::SolveFull) ------------------------------ */
+typedef struct {
+  /* OptimSolverOptions (OptimSolver.h:23-71) */
+  double tolMaxError;
+  double tolMinStep;
+  double tolMinImpr;
+  int minStepMaxCount;
+  int minImprMaxCount;
+  int numMaxIters;
+  double maxStepSize;
+  int trySteepest;
+  int lsearch_numTry;
+  double lSearch_alpha;
+  int lsNumMaxIters;
+  double lsTolMaxError;
+  int lsNumRegTrials;
+  /* MI355X path controls (no reference counterpart) */
+  int precond;        /* 0 = Jacobi, 1 = 3x3 block Jacobi                  */
+  double inner_tol;   /* MIXED: FP32 inner tolerance per refinement sweep */
+  int adaptive_tol;   /* reserved, must be 0                               */
+  double ls_noise_guard; /* 0 = reference line search.  > 0: also accept a
+                         step whose energy rises by at most ls_noise_guard *
+                         (|E_elastic| + |E_load|), i.e. within FP64 rounding
+                         of the total, when |g| decreases                   */
+} pgn_newton_opts;
+
+/* OSResult (BasicTypes.h:69-77) */
+enum pgn_os_result {
+  PGN_OS_ONGOING = 0,
+  PGN_OS_SUCCESS,
+  PGN_OS_MINSTEP,
+  PGN_OS_MINIMPR,
+  PGN_OS_FAILURE,
+  PGN_OS_MAXITER,
+  PGN_OS_NONDESC
+};
+
+typedef struct {
+  int result;         /* pgn_os_result */
+  int iterations;     /* Newton iterations (SolveFull itCount)          */
+  int pcg_iterations; /* total PCG iterations                           */
+  int line_search_bisections;
+  int energy_evals;   /* E+g evaluations                                 */
+  int hessian_evals;
+  double objective;
+  double optimality;  /* |g_free| */
+  double seconds;     /* wall time of the solve                          */
+} pgn_newton_result;
+
+void pgn_newton_default_opts(pgn_newton_opts* o);
+/* OptimSolver::Init + SolveFull (OptimSolver.cpp:36-55,163-264) with
+ * OptimSolver_USQP_LS::SolveStepInternal (OptimSolver_USQP_LS.cpp:18-68),
+ * Newton steps (ComputeStep_Newton :468) and LineSearch_Simple (:688). */
+int pgn_newton_solve(pgn_model* model, const pgn_newton_opts* opts,
+                     pgn_newton_result* res);
+
+/* ---- measurement hooks -------------------------------------------------- */
+/* Runs `reps` launches of one kernel on the model's current state, on the
+ * context stream, and returns the mean launch time (HIP events) in ms.
+ * which: 0 = element E+g+H (model precision), 1 = element E+g (FP64),
+ *        2 = BSR Hessian gather+fix, 3 = BSR SpMV (one PCG matrix product),
+ *        4 = element E+g+H FP64 (plain path kernel)                        */
+int pgn_time_kernel(pgn_model* model, int which, int reps, double* ms_mean);
+/* algorithmic bytes one launch of `which` moves (for roofline accounting) */
+int pgn_kernel_bytes(pgn_model* model, int which, double* bytes);
+
+/* ---- host-only helpers (no GPU needed; used by CPU tests) --------------- */
+/* Builds the BSR3 pattern and the element->block slot map of a tet mesh the
+ * way pgn_model_create_tet does.  nb_out: number of 3x3 blocks. */
+int pgn_host_bsr_pattern(int n_nodes, int n_tets, const int* T, long* nb_out,
+                         long* rowptr /* n+1 or NULL */,
+                         int* col /* nb or NULL */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

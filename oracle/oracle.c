@@ -314,6 +314,7 @@ struct orc_model {
   double *f_ini, *f_end;
   /* per-element outputs */
   double *eE, *eg, *eH;
+  double escale; /* |elastic| + |load| energy of the last evaluation */
   /* lower pattern */
   long nnz;
   long* rowptr;
@@ -571,22 +572,27 @@ double orc_model_energy(orc_model* M) {
   }
   double E = 0;
   for (int e = 0; e < M->m; ++e) E += M->eE[e];
+  const double Eel = E;
   /* MassElement_Lumped.cpp:56-68: E = x . (-m g) per mass element */
   double g[3];
   cur_gravity(M, g);
+  double Eload = 0;
   for (int e = 0; e < M->m; ++e) {
     double Ee = 0;
     for (int a = 0; a < 4; ++a)
       for (int i = 0; i < 3; ++i)
         Ee += M->x[3 * M->T[4 * e + a] + i] * (-M->elem_mass[e] * g[i]);
     E += Ee;
+    Eload += Ee;
   }
   /* EnergyElement_Force.cpp:43-47: E = x . f */
   if (M->has_f) {
     double Ef = 0;
     for (int k = 0; k < 3 * M->n; ++k) Ef += M->x[k] * cur_force(M, k);
     E += Ef;
+    Eload += Ef;
   }
+  M->escale = fabs(Eel) + fabs(Eload);
   return E;
 }
 
@@ -616,6 +622,8 @@ void orc_model_gradient(orc_model* M, double* gout) {
 }
 
 long orc_model_nnz_lower(orc_model* M) { return M->nnz; }
+
+double orc_model_energy_scale(const orc_model* M) { return M->escale; }
 
 void orc_model_hessian_lower(orc_model* M, long* rowptr, int* col,
                              double* val) {
@@ -795,6 +803,7 @@ void orc_newton_default_opts(orc_newton_opts* o) {
   o->lsTolMaxError = 1e-9;
   o->lsearch_numTry = 32;
   o->lSearch_alpha = 0.5;
+  o->ls_noise_guard = 0.0;
 }
 
 static double norm2(int n, const double* a) { return sqrt(dot(n, a, a)); }
@@ -852,6 +861,15 @@ static void solve_step(orc_model* M, const orc_newton_opts* o, nstate* S,
     objNew = orc_model_energy(M);
     orc_model_gradient(M, S->vgNew);
     if (objNew < S->obj) {
+      improved = 1;
+      break;
+    }
+    if (o->ls_noise_guard > 0 &&
+        objNew <= S->obj + o->ls_noise_guard * orc_model_energy_scale(M) &&
+        norm2(N, S->vgNew) < S->opt) {
+      /* energy change below rounding: accept on the gradient norm and book
+       * it as the smallest positive improvement */
+      objNew = S->obj - 1e-300;
       improved = 1;
       break;
     }

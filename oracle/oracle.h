@@ -81,6 +81,8 @@ void orc_model_get_mass(const orc_model* m, double* mass3n);
 
 /* Simulable::AddEnergy (elements in order, then mass, then BC energies) */
 double orc_model_energy(orc_model* m);
+/* |elastic energy| + |load energy| of the last orc_model_energy call */
+double orc_model_energy_scale(const orc_model* m);
 /* Simulable::GetGradient + FixVectorDoFs */
 void orc_model_gradient(orc_model* m, double* g);
 /* lower-triangular CSR (row >= col) pattern size */
@@ -120,6 +122,11 @@ typedef struct {
   int lsNumMaxIters;
   double lsTolMaxError;
   int lsNumRegTrials;
+  /* Not in the reference (0 = reference semantics): also accept a line-search
+   * step whose energy rises by at most ls_noise_guard * Ebar (Ebar = |elastic
+   * energy| + |load energy|, the scale of the FP64 rounding of the total) when
+   * the gradient norm decreases. */
+  double ls_noise_guard;
 } orc_newton_opts;
 
 /* OSResult (BasicTypes.h:69-77) */

@@ -35,7 +35,8 @@ class NewtonOpts(ctypes.Structure):
                 ("lSearch_alpha", ctypes.c_double),
                 ("lsNumMaxIters", ctypes.c_int),
                 ("lsTolMaxError", ctypes.c_double),
-                ("lsNumRegTrials", ctypes.c_int)]
+                ("lsNumRegTrials", ctypes.c_int),
+                ("ls_noise_guard", ctypes.c_double)]
 
 
 class NewtonResult(ctypes.Structure):

@@ -1,0 +1,324 @@
+"""physim_amd -- MI355X-native PhySim Newton inner loop.
+
+The product is libpgn.so (HIP kernels for gfx950 + C++ host driver behind the
+C ABI of include/pgn.h).  This module is a thin ctypes binding of that ABI for
+tests and benchmarks; it has no compute of its own and raises if the library
+is missing or no gfx950 device is present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpgn.so")
+
+CONH, STVK = 0, 1
+FP64, MIXED = 0, 1
+LAYOUT_LOWER, LAYOUT_FULL, LAYOUT_BSR3 = 0, 1, 2
+# Newton options of a solve "to residual 1e-8": the reference's MINIMPR /
+# MINSTEP exits and its energy-only line search would stop short of it
+CONVERGE_OPTS = dict(tolMaxError=1e-8, tolMinImpr=0.0, tolMinStep=0.0,
+                     ls_noise_guard=1e-12)
+OS_NAMES = ["ONGOING", "SUCCESS", "MINSTEP", "MINIMPR", "FAILURE", "MAXITER",
+            "NONDESC"]
+
+# every symbol include/pgn.h declares
+EXPORTS = [
+    "pgn_last_error", "pgn_ctx_create", "pgn_ctx_destroy", "pgn_ctx_stream",
+    "pgn_ctx_synchronize", "pgn_model_create_tet", "pgn_model_destroy",
+    "pgn_model_num_dofs", "pgn_model_num_elements", "pgn_model_device_bytes",
+    "pgn_model_set_fixed", "pgn_model_set_gravity", "pgn_model_set_forces",
+    "pgn_model_step_load", "pgn_model_is_loaded", "pgn_model_reset",
+    "pgn_model_set_x", "pgn_model_get_x", "pgn_model_get_energy",
+    "pgn_model_get_gradient", "pgn_model_hessian_nnz",
+    "pgn_model_hessian_pattern", "pgn_model_get_hessian",
+    "pgn_model_element_egh", "pgn_linear_default_opts", "pgn_linear_solve",
+    "pgn_newton_default_opts", "pgn_newton_solve", "pgn_time_kernel",
+    "pgn_kernel_bytes", "pgn_host_bsr_pattern",
+]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_ucp = ctypes.POINTER(ctypes.c_ubyte)
+_vp = ctypes.c_void_p
+
+
+class LinearOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iters", ctypes.c_int),
+                ("reg_iters", ctypes.c_int), ("precond", ctypes.c_int),
+                ("inner_tol", ctypes.c_double)]
+
+
+class LinearResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("reg_used", ctypes.c_int), ("refinements", ctypes.c_int),
+                ("rel_residual", ctypes.c_double)]
+
+
+class NewtonOpts(ctypes.Structure):
+    _fields_ = [("tolMaxError", ctypes.c_double), ("tolMinStep", ctypes.c_double),
+                ("tolMinImpr", ctypes.c_double), ("minStepMaxCount", ctypes.c_int),
+                ("minImprMaxCount", ctypes.c_int), ("numMaxIters", ctypes.c_int),
+                ("maxStepSize", ctypes.c_double), ("trySteepest", ctypes.c_int),
+                ("lsearch_numTry", ctypes.c_int), ("lSearch_alpha", ctypes.c_double),
+                ("lsNumMaxIters", ctypes.c_int), ("lsTolMaxError", ctypes.c_double),
+                ("lsNumRegTrials", ctypes.c_int), ("precond", ctypes.c_int),
+                ("inner_tol", ctypes.c_double), ("adaptive_tol", ctypes.c_int),
+                ("ls_noise_guard", ctypes.c_double)]
+
+
+class NewtonResult(ctypes.Structure):
+    _fields_ = [("result", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("pcg_iterations", ctypes.c_int),
+                ("line_search_bisections", ctypes.c_int),
+                ("energy_evals", ctypes.c_int), ("hessian_evals", ctypes.c_int),
+                ("objective", ctypes.c_double), ("optimality", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    """Loads libpgn.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with "
+                               "`python -m physim_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.pgn_last_error.restype = ctypes.c_char_p
+        L.pgn_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+        L.pgn_ctx_destroy.argtypes = [_vp]
+        L.pgn_ctx_destroy.restype = None
+        L.pgn_ctx_stream.argtypes = [_vp]
+        L.pgn_ctx_stream.restype = _vp
+        L.pgn_ctx_synchronize.argtypes = [_vp]
+        L.pgn_model_create_tet.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, _ip,
+                                           _ucp, _dp, _dp, _dp, ctypes.c_int,
+                                           ctypes.POINTER(_vp)]
+        L.pgn_model_destroy.argtypes = [_vp]
+        L.pgn_model_destroy.restype = None
+        L.pgn_model_num_dofs.argtypes = [_vp]
+        L.pgn_model_num_elements.argtypes = [_vp]
+        L.pgn_model_device_bytes.argtypes = [_vp]
+        L.pgn_model_device_bytes.restype = ctypes.c_size_t
+        L.pgn_model_set_fixed.argtypes = [_vp, _ucp]
+        L.pgn_model_set_gravity.argtypes = [_vp, _dp, _dp, ctypes.c_int]
+        L.pgn_model_set_forces.argtypes = [_vp, _dp, _dp, ctypes.c_int]
+        L.pgn_model_step_load.argtypes = [_vp, _ip]
+        L.pgn_model_is_loaded.argtypes = [_vp]
+        L.pgn_model_reset.argtypes = [_vp]
+        L.pgn_model_set_x.argtypes = [_vp, _dp]
+        L.pgn_model_get_x.argtypes = [_vp, _dp]
+        L.pgn_model_get_energy.argtypes = [_vp, _dp]
+        L.pgn_model_get_gradient.argtypes = [_vp, _dp]
+        L.pgn_model_hessian_nnz.argtypes = [_vp, ctypes.c_int, _lp]
+        L.pgn_model_hessian_pattern.argtypes = [_vp, ctypes.c_int, _lp, _ip]
+        L.pgn_model_get_hessian.argtypes = [_vp, ctypes.c_int, _dp]
+        L.pgn_model_element_egh.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp]
+        L.pgn_linear_default_opts.argtypes = [ctypes.POINTER(LinearOpts)]
+        L.pgn_linear_default_opts.restype = None
+        L.pgn_linear_solve.argtypes = [_vp, _dp, _dp, ctypes.POINTER(LinearOpts),
+                                       ctypes.POINTER(LinearResult)]
+        L.pgn_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
+        L.pgn_newton_default_opts.restype = None
+        L.pgn_newton_solve.argtypes = [_vp, ctypes.POINTER(NewtonOpts),
+                                       ctypes.POINTER(NewtonResult)]
+        L.pgn_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
+        L.pgn_kernel_bytes.argtypes = [_vp, ctypes.c_int, _dp]
+        L.pgn_host_bsr_pattern.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _lp, _lp, _ip]
+        _LIB = L
+    return _LIB
+
+
+class PgnError(RuntimeError):
+    pass
+
+
+def _ok(rc):
+    if rc != 0:
+        raise PgnError(f"libpgn error {rc}: {lib().pgn_last_error().decode()}")
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    def __init__(self, device=0):
+        h = _vp()
+        _ok(lib().pgn_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stream(self):
+        return lib().pgn_ctx_stream(self._h)
+
+    def synchronize(self):
+        _ok(lib().pgn_ctx_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pgn_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class TetModel:
+    """A device-resident Tet4 FEM model (ISimulable + IOptimProblem)."""
+
+    def __init__(self, ctx, X0, T, material, lame1, lame2, density, precision=FP64):
+        X0 = _f64(X0).reshape(-1)
+        T = np.ascontiguousarray(T, dtype=np.int32).reshape(-1)
+        m = T.size // 4
+        mat = np.ascontiguousarray(np.broadcast_to(material, (m,)), dtype=np.uint8)
+        l1 = _f64(np.broadcast_to(lame1, (m,)))
+        l2 = _f64(np.broadcast_to(lame2, (m,)))
+        rho = _f64(np.broadcast_to(density, (m,)))
+        self.ctx = ctx
+        self.n = X0.size // 3
+        self.m = m
+        self.N = 3 * self.n
+        self.precision = precision
+        h = _vp()
+        _ok(lib().pgn_model_create_tet(ctx.handle, self.n, _p(X0), m, _p(T, _ip),
+                                       _p(mat, _ucp), _p(l1), _p(l2), _p(rho),
+                                       precision, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pgn_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def device_bytes(self):
+        return lib().pgn_model_device_bytes(self._h)
+
+    def set_fixed(self, mask):
+        mk = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        assert mk.size == self.N
+        _ok(lib().pgn_model_set_fixed(self._h, _p(mk, _ucp)))
+
+    def set_gravity(self, g_end, stages, g_ini=(0.0, 0.0, 0.0)):
+        _ok(lib().pgn_model_set_gravity(self._h, _p(_f64(g_ini)), _p(_f64(g_end)),
+                                        int(stages)))
+
+    def set_forces(self, f_end, stages, f_ini=None):
+        f_end = _f64(f_end).reshape(-1)
+        f_ini = np.zeros_like(f_end) if f_ini is None else _f64(f_ini).reshape(-1)
+        _ok(lib().pgn_model_set_forces(self._h, _p(f_ini), _p(f_end), int(stages)))
+
+    def step_load(self):
+        s = ctypes.c_int()
+        _ok(lib().pgn_model_step_load(self._h, ctypes.byref(s)))
+        return bool(s.value)
+
+    def is_loaded(self):
+        return bool(lib().pgn_model_is_loaded(self._h))
+
+    def reset(self):
+        _ok(lib().pgn_model_reset(self._h))
+
+    def set_x(self, x):
+        x = _f64(x).reshape(-1)
+        _ok(lib().pgn_model_set_x(self._h, _p(x)))
+
+    def get_x(self):
+        x = np.zeros(self.N)
+        _ok(lib().pgn_model_get_x(self._h, _p(x)))
+        return x
+
+    def energy(self):
+        e = ctypes.c_double()
+        _ok(lib().pgn_model_get_energy(self._h, ctypes.byref(e)))
+        return e.value
+
+    def gradient(self):
+        g = np.zeros(self.N)
+        _ok(lib().pgn_model_get_gradient(self._h, _p(g)))
+        return g
+
+    def hessian(self, layout=LAYOUT_LOWER):
+        nnz = ctypes.c_long()
+        _ok(lib().pgn_model_hessian_nnz(self._h, layout, ctypes.byref(nnz)))
+        rows = self.n + 1 if layout == LAYOUT_BSR3 else self.N + 1
+        rp = np.zeros(rows, dtype=np.int64)
+        ci = np.zeros(nnz.value, dtype=np.int32)
+        _ok(lib().pgn_model_hessian_pattern(self._h, layout, _p(rp, _lp), _p(ci, _ip)))
+        nv = nnz.value * (9 if layout == LAYOUT_BSR3 else 1)
+        v = np.zeros(nv)
+        _ok(lib().pgn_model_get_hessian(self._h, layout, _p(v)))
+        return rp, ci, v
+
+    def element_egh(self, precision=None):
+        prec = self.precision if precision is None else precision
+        E = np.zeros(self.m)
+        g = np.zeros((self.m, 12))
+        H = np.zeros((self.m, 12, 12))
+        _ok(lib().pgn_model_element_egh(self._h, prec, _p(E), _p(g), _p(H)))
+        return E, g, H
+
+    def linear_solve(self, b, x0=None, **kw):
+        o = LinearOpts()
+        lib().pgn_linear_default_opts(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        b = _f64(b).reshape(-1)
+        x = np.zeros(self.N) if x0 is None else _f64(x0).reshape(-1).copy()
+        r = LinearResult()
+        _ok(lib().pgn_linear_solve(self._h, _p(b), _p(x), ctypes.byref(o), ctypes.byref(r)))
+        return x, {"status": r.status, "iterations": r.iterations,
+                   "reg_used": r.reg_used, "refinements": r.refinements,
+                   "rel_residual": r.rel_residual}
+
+    def newton(self, **kw):
+        o = NewtonOpts()
+        lib().pgn_newton_default_opts(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        r = NewtonResult()
+        _ok(lib().pgn_newton_solve(self._h, ctypes.byref(o), ctypes.byref(r)))
+        return {"result": OS_NAMES[r.result], "iterations": r.iterations,
+                "pcg_iterations": r.pcg_iterations,
+                "bisections": r.line_search_bisections,
+                "energy_evals": r.energy_evals, "hessian_evals": r.hessian_evals,
+                "objective": r.objective, "optimality": r.optimality,
+                "seconds": r.seconds}
+
+    def time_kernel(self, which, reps=20):
+        ms = ctypes.c_double()
+        _ok(lib().pgn_time_kernel(self._h, which, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def kernel_bytes(self, which):
+        b = ctypes.c_double()
+        _ok(lib().pgn_kernel_bytes(self._h, which, ctypes.byref(b)))
+        return b.value
+
+
+def host_bsr_pattern(n_nodes, T):
+    """BSR3 pattern the library builds for a tet mesh (host logic only)."""
+    T = np.ascontiguousarray(T, dtype=np.int32).reshape(-1)
+    m = T.size // 4
+    nb = ctypes.c_long()
+    _ok(lib().pgn_host_bsr_pattern(n_nodes, m, _p(T, _ip), ctypes.byref(nb), None, None))
+    rp = np.zeros(n_nodes + 1, dtype=np.int64)
+    col = np.zeros(nb.value, dtype=np.int32)
+    _ok(lib().pgn_host_bsr_pattern(n_nodes, m, _p(T, _ip), ctypes.byref(nb),
+                                   _p(rp, _lp), _p(col, _ip)))
+    return rp, col

@@ -1,0 +1,579 @@
+// api.cpp -- the extern "C" boundary of libpgn (declared in include/pgn.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <new>
+
+#include "solver.hpp"
+
+using namespace pgn;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return PGN_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return PGN_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return PGN_ERR_INVALID;
+  }
+}
+
+void use_device(const pgn_ctx* c) { PGN_HIP(hipSetDevice(c->device)); }
+
+// unpack the 78-value element Hessian into a full 12x12 row-major matrix
+template <class T>
+void unpack_tet_h(const T* h, double* H) {
+  for (int a = 1; a < 4; ++a)
+    for (int b = 0; b < a; ++b) {
+      const T* o = h + 9 * (a * (a - 1) / 2 + b);
+      for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+          H[(3 * a + i) * 12 + 3 * b + k] = o[3 * i + k];
+          H[(3 * b + k) * 12 + 3 * a + i] = o[3 * i + k];
+        }
+    }
+  for (int a = 0; a < 4; ++a) {
+    const T* d = h + 54 + 6 * a;
+    const int idx[3][3] = {{0, 1, 3}, {1, 2, 4}, {3, 4, 5}};
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) H[(3 * a + i) * 12 + 3 * a + k] = d[idx[i][k]];
+  }
+}
+
+template <class T>
+void bsr_to_layout(const pgn_model* M, const T* bv, int layout, double* out) {
+  const BsrPattern& P = M->pat;
+  if (layout == PGN_LAYOUT_BSR3) {
+    for (long q = 0; q < 9 * P.nb; ++q) out[q] = (double)bv[q];
+    return;
+  }
+  long w = 0;
+  for (int i = 0; i < P.n; ++i)
+    for (int p = 0; p < 3; ++p)
+      for (long s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
+        if (layout == PGN_LAYOUT_LOWER && P.bcol[s] > i) continue;
+        for (int q = 0; q < 3; ++q) out[w++] = (double)bv[9 * s + 3 * p + q];
+      }
+}
+
+long layout_nnz(const pgn_model* M, int layout) {
+  const BsrPattern& P = M->pat;
+  if (layout == PGN_LAYOUT_BSR3) return P.nb;
+  if (layout == PGN_LAYOUT_FULL) return 9 * P.nb;
+  long c = 0;
+  for (int i = 0; i < P.n; ++i)
+    for (long s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s)
+      if (P.bcol[s] <= i) c += 9;
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pgn_last_error(void) { return g_err.c_str(); }
+
+int pgn_ctx_create(int device, pgn_ctx** out) {
+  return guard([&] {
+    PGN_CHECK(out, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+      throw Error(PGN_ERR_NODEVICE, "no HIP device available: libpgn needs an MI355X (gfx950)");
+    PGN_CHECK(device >= 0 && device < count, "device index out of range");
+    PGN_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    PGN_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      throw Error(PGN_ERR_NODEVICE, std::string("libpgn is built for gfx950, device is ") +
+                                        prop.gcnArchName);
+    auto* c = new pgn_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      throw Error(PGN_ERR_HIP, "hipStreamCreate failed");
+    }
+    *out = c;
+  });
+}
+
+void pgn_ctx_destroy(pgn_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+}
+
+void* pgn_ctx_stream(pgn_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int pgn_ctx_synchronize(pgn_ctx* ctx) {
+  return guard([&] {
+    PGN_CHECK(ctx, "null context");
+    PGN_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int pgn_model_create_tet(pgn_ctx* ctx, int n_nodes, const double* X0, int n_tets,
+                         const int* T, const unsigned char* material,
+                         const double* lame1, const double* lame2,
+                         const double* density, int precision, pgn_model** out) {
+  return guard([&] {
+    PGN_CHECK(ctx && out, "null context or output pointer");
+    *out = nullptr;
+    use_device(ctx);
+    auto* M = new pgn_model();
+    M->ctx = ctx;
+    try {
+      M->setup(n_nodes, X0, n_tets, T, material, lame1, lame2, density, precision);
+    } catch (...) {
+      delete M;
+      throw;
+    }
+    *out = M;
+  });
+}
+
+void pgn_model_destroy(pgn_model* model) {
+  if (!model) return;
+  (void)hipSetDevice(model->ctx->device);
+  (void)hipStreamSynchronize(model->ctx->stream);
+  delete model;
+}
+
+int pgn_model_num_dofs(const pgn_model* model) { return model ? 3 * model->n : 0; }
+int pgn_model_num_elements(const pgn_model* model) { return model ? model->m : 0; }
+
+size_t pgn_model_device_bytes(const pgn_model* M) {
+  if (!M) return 0;
+  return M->X0.bytes() + M->x.bytes() + M->tets.bytes() + M->kind.bytes() +
+         M->prm64.bytes() + M->prm32.bytes() + M->mass.bytes() + M->fixmask.bytes() +
+         M->inc_ptr.bytes() + M->inc.bytes() + M->rowptr.bytes() + M->cptr.bytes() +
+         M->diag.bytes() + M->bcol.bytes() + M->brow.bytes() + M->code.bytes() +
+         M->bval.bytes() + M->dinv.bytes() + M->binv.bytes() + M->eg64.bytes() +
+         M->eg32.bytes() + M->eH.bytes() + M->eH_alt.bytes() + M->blockE.bytes() +
+         M->nodeP.bytes() + M->scal.bytes() + M->g.bytes() + M->f_ini.bytes() +
+         M->f_end.bytes() + M->pv.bytes() + M->pb.bytes() + M->part1.bytes() +
+         M->part2.bytes() + M->red64.bytes() + M->sc.bytes() + M->r64.bytes() +
+         M->tmp64a.bytes() + M->tmp64b.bytes() + M->nw_xsave.bytes() +
+         M->nw_v0.bytes() + M->nw_dx.bytes() + M->nw_b.bytes();
+}
+
+int pgn_model_set_fixed(pgn_model* M, const unsigned char* mask) {
+  return guard([&] {
+    PGN_CHECK(M && mask, "null argument");
+    use_device(M->ctx);
+    M->hfix.assign(mask, mask + 3L * M->n);
+    std::vector<uint8_t> fm(M->n);
+    for (int i = 0; i < M->n; ++i)
+      fm[i] = (mask[3L * i] ? 1 : 0) | (mask[3L * i + 1] ? 2 : 0) | (mask[3L * i + 2] ? 4 : 0);
+    M->fixmask.upload(fm, M->stream());
+    M->sync();
+    M->hess_valid = false;
+  });
+}
+
+int pgn_model_set_gravity(pgn_model* M, const double g_ini[3], const double g_end[3],
+                          int num_stages) {
+  return guard([&] {
+    PGN_CHECK(M && g_ini && g_end, "null argument");
+    PGN_CHECK(num_stages >= 1, "num_stages must be >= 1");
+    M->has_g = true;
+    std::copy(g_ini, g_ini + 3, M->g_ini);
+    std::copy(g_end, g_end + 3, M->g_end);
+    M->g_stages = num_stages;
+    M->g_cur = 0;
+  });
+}
+
+int pgn_model_set_forces(pgn_model* M, const double* f_ini, const double* f_end,
+                         int num_stages) {
+  return guard([&] {
+    PGN_CHECK(M && f_ini && f_end, "null argument");
+    PGN_CHECK(num_stages >= 1, "num_stages must be >= 1");
+    use_device(M->ctx);
+    M->f_ini.upload(f_ini, 3L * M->n, M->stream());
+    M->f_end.upload(f_end, 3L * M->n, M->stream());
+    M->sync();
+    M->has_f = true;
+    M->f_stages = num_stages;
+    M->f_cur = 0;
+  });
+}
+
+int pgn_model_step_load(pgn_model* M, int* stepped) {
+  return guard([&] {
+    PGN_CHECK(M, "null model");
+    const bool s = M->step_load();
+    if (stepped) *stepped = s ? 1 : 0;
+  });
+}
+
+int pgn_model_is_loaded(const pgn_model* M) { return M && M->is_loaded() ? 1 : 0; }
+
+int pgn_model_reset(pgn_model* M) {
+  return guard([&] {
+    PGN_CHECK(M, "null model");
+    use_device(M->ctx);
+    PGN_HIP(hipMemcpyAsync(M->x.p, M->X0.p, M->X0.bytes(), hipMemcpyDeviceToDevice,
+                           M->stream()));
+    M->sync();
+    M->g_cur = 0;
+    M->f_cur = 0;
+    M->hess_valid = false;
+  });
+}
+
+int pgn_model_set_x(pgn_model* M, const double* x) {
+  return guard([&] {
+    PGN_CHECK(M && x, "null argument");
+    use_device(M->ctx);
+    M->x.upload(x, 3L * M->n, M->stream());
+    M->sync();
+    M->hess_valid = false;
+  });
+}
+
+int pgn_model_get_x(pgn_model* M, double* x) {
+  return guard([&] {
+    PGN_CHECK(M && x, "null argument");
+    use_device(M->ctx);
+    M->x.download(x, 3L * M->n, M->stream());
+    M->sync();
+  });
+}
+
+int pgn_model_get_energy(pgn_model* M, double* energy) {
+  return guard([&] {
+    PGN_CHECK(M && energy, "null argument");
+    use_device(M->ctx);
+    double gg;
+    M->eval_eg(*energy, gg);
+  });
+}
+
+int pgn_model_get_gradient(pgn_model* M, double* gradient) {
+  return guard([&] {
+    PGN_CHECK(M && gradient, "null argument");
+    use_device(M->ctx);
+    double E, gg;
+    M->eval_eg(E, gg);
+    M->g.download(gradient, 3L * M->n, M->stream());
+    M->sync();
+  });
+}
+
+int pgn_model_hessian_nnz(pgn_model* M, int layout, long* nnz) {
+  return guard([&] {
+    PGN_CHECK(M && nnz, "null argument");
+    PGN_CHECK(layout >= 0 && layout <= 2, "unknown layout");
+    *nnz = layout_nnz(M, layout);
+  });
+}
+
+int pgn_model_hessian_pattern(pgn_model* M, int layout, long* rowptr, int* col) {
+  return guard([&] {
+    PGN_CHECK(M && rowptr && col, "null argument");
+    PGN_CHECK(layout >= 0 && layout <= 2, "unknown layout");
+    const BsrPattern& P = M->pat;
+    if (layout == PGN_LAYOUT_BSR3) {
+      std::copy(P.rowptr.begin(), P.rowptr.end(), rowptr);
+      std::copy(P.bcol.begin(), P.bcol.end(), col);
+      return;
+    }
+    long w = 0;
+    rowptr[0] = 0;
+    for (int i = 0; i < P.n; ++i)
+      for (int p = 0; p < 3; ++p) {
+        for (long s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
+          if (layout == PGN_LAYOUT_LOWER && P.bcol[s] > i) continue;
+          for (int q = 0; q < 3; ++q) col[w++] = 3 * P.bcol[s] + q;
+        }
+        rowptr[3 * i + p + 1] = w;
+      }
+  });
+}
+
+int pgn_model_get_hessian(pgn_model* M, int layout, double* values) {
+  return guard([&] {
+    PGN_CHECK(M && values, "null argument");
+    PGN_CHECK(layout >= 0 && layout <= 2, "unknown layout");
+    use_device(M->ctx);
+    M->eval_hessian();
+    const long nv = 9 * M->pat.nb;
+    if (M->hfp32) {
+      std::vector<float> h(nv);
+      PGN_HIP(hipMemcpyAsync(h.data(), M->bval.p, nv * sizeof(float), hipMemcpyDeviceToHost,
+                             M->stream()));
+      M->sync();
+      bsr_to_layout(M, h.data(), layout, values);
+    } else {
+      std::vector<double> h(nv);
+      PGN_HIP(hipMemcpyAsync(h.data(), M->bval.p, nv * sizeof(double), hipMemcpyDeviceToHost,
+                             M->stream()));
+      M->sync();
+      bsr_to_layout(M, h.data(), layout, values);
+    }
+  });
+}
+
+int pgn_model_element_egh(pgn_model* M, int precision, double* E, double* g, double* H) {
+  return guard([&] {
+    PGN_CHECK(M, "null model");
+    PGN_CHECK(precision == PGN_FP64 || precision == PGN_MIXED, "unknown precision");
+    use_device(M->ctx);
+    const bool f32 = precision == PGN_MIXED;
+    const long m = M->m;
+    if (m == 0) return;
+    hipStream_t s = M->stream();
+    DBuf<double> dE;
+    dE.alloc(m);
+    void* eg;
+    void* eH;
+    if (f32) {
+      if (M->eg32.n < (size_t)(12 * m)) M->eg32.alloc(12 * m);
+      eg = M->eg32.p;
+    } else {
+      eg = M->eg64.p;
+    }
+    const size_t ts = f32 ? sizeof(float) : sizeof(double);
+    if (f32 == M->hfp32) {
+      eH = M->eH.p;
+    } else {
+      if (M->eH_alt.n < (size_t)TET_H_PACK * m * ts) M->eH_alt.alloc((size_t)TET_H_PACK * m * ts);
+      eH = M->eH_alt.p;
+    }
+    launch_tet_eval(s, f32, EVAL_E | EVAL_G | EVAL_H, M->tet_args(f32), nullptr, dE.p, eg, eH);
+    PGN_HIP(hipGetLastError());
+    if (E) dE.download(E, m, s);
+    std::vector<char> hg(12 * m * ts), hh((size_t)TET_H_PACK * m * ts);
+    if (g) PGN_HIP(hipMemcpyAsync(hg.data(), eg, hg.size(), hipMemcpyDeviceToHost, s));
+    if (H) PGN_HIP(hipMemcpyAsync(hh.data(), eH, hh.size(), hipMemcpyDeviceToHost, s));
+    M->sync();
+    for (long e = 0; e < m; ++e) {
+      if (g)
+        for (int q = 0; q < 12; ++q)
+          g[12 * e + q] = f32 ? (double)reinterpret_cast<float*>(hg.data())[12 * e + q]
+                              : reinterpret_cast<double*>(hg.data())[12 * e + q];
+      if (H) {
+        if (f32)
+          unpack_tet_h(reinterpret_cast<float*>(hh.data()) + TET_H_PACK * e, H + 144 * e);
+        else
+          unpack_tet_h(reinterpret_cast<double*>(hh.data()) + TET_H_PACK * e, H + 144 * e);
+      }
+    }
+    M->hess_valid = M->hess_valid && (f32 != M->hfp32);
+  });
+}
+
+void pgn_linear_default_opts(pgn_linear_opts* o) {
+  if (!o) return;
+  o->tol = 1e-9;        // OptimSolverOptions::lsTolMaxError
+  o->max_iters = 1000;  // lsNumMaxIters
+  o->reg_iters = 10;    // LinearSolverOptions::regIters
+  o->precond = 0;
+  o->inner_tol = 1e-5;
+}
+
+int pgn_linear_solve(pgn_model* M, const double* b, double* x, const pgn_linear_opts* opts,
+                     pgn_linear_result* res) {
+  return guard([&] {
+    PGN_CHECK(M && b && x && res, "null argument");
+    use_device(M->ctx);
+    pgn_linear_opts o;
+    pgn_linear_default_opts(&o);
+    if (opts) o = *opts;
+    PGN_CHECK(o.max_iters >= 0 && o.tol >= 0.0, "bad linear options");
+    M->eval_hessian();
+    const long N = 3L * M->n;
+    M->nw_b.alloc(N);
+    M->nw_dx.alloc(N);
+    M->nw_b.upload(b, N, M->stream());
+    M->nw_dx.upload(x, N, M->stream());
+    linear_solve(*M, M->nw_b.p, M->nw_dx.p, o, *res);
+    M->nw_dx.download(x, N, M->stream());
+    M->sync();
+    M->hess_valid = false;  // regularization may have modified it
+  });
+}
+
+void pgn_newton_default_opts(pgn_newton_opts* o) {
+  if (!o) return;
+  // OptimSolverOptions() (OptimSolver.h:47-70)
+  o->tolMaxError = 1e-6;
+  o->tolMinStep = 1e-5;
+  o->tolMinImpr = 1e-9;
+  o->minStepMaxCount = 3;
+  o->minImprMaxCount = 3;
+  o->numMaxIters = 100;
+  o->maxStepSize = 1000;
+  o->trySteepest = 1;
+  o->lsearch_numTry = 32;
+  o->lSearch_alpha = 0.5;
+  o->lsNumMaxIters = 1000;
+  o->lsTolMaxError = 1e-9;
+  o->lsNumRegTrials = 10;
+  o->precond = 0;
+  o->inner_tol = 1e-5;
+  o->adaptive_tol = 0;
+  o->ls_noise_guard = 0.0;
+}
+
+int pgn_newton_solve(pgn_model* M, const pgn_newton_opts* opts, pgn_newton_result* res) {
+  return guard([&] {
+    PGN_CHECK(M && res, "null argument");
+    use_device(M->ctx);
+    pgn_newton_opts o;
+    pgn_newton_default_opts(&o);
+    if (opts) o = *opts;
+    newton_solve(*M, o, *res);
+  });
+}
+
+int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
+  return guard([&] {
+    PGN_CHECK(M && bytes, "null argument");
+    const double m = M->m, n = M->n, nb = (double)M->pat.nb;
+    const double nc = (double)M->pat.code.size();
+    const double hs = M->hfp32 ? 4.0 : 8.0;
+    // compulsory traffic: every input read once, every output written once
+    switch (which) {
+      case 0:  // E+g+H, model precision: tets 16 + kind 1 + prm 12*hs, g 12*hs, H 78*hs
+        *bytes = m * (17 + 12 * hs + 12 * hs + 78 * hs) + n * 48;
+        break;
+      case 1:  // E+g FP64
+        *bytes = m * (17 + 96 + 96) + n * 48;
+        break;
+      case 2:  // BSR gather: codes, ranges, rows/cols, element H, blocks out
+        *bytes = nc * 4 + nb * (8 + 8 + 9 * hs) + m * 78 * hs + n;
+        break;
+      case 3:  // SpMV: values, block cols, row pointers, x, y
+        *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 2 * 3 * n * hs;
+        break;
+      case 4:  // E+g+H FP64
+        *bytes = m * (17 + 96 + 96 + 624) + n * 48;
+        break;
+      case 5:  // E+g+H FP32
+        *bytes = m * (17 + 48 + 48 + 312) + n * 48;
+        break;
+      default:
+        throw Error(PGN_ERR_INVALID, "unknown kernel id");
+    }
+  });
+}
+
+int pgn_time_kernel(pgn_model* M, int which, int reps, double* ms_mean) {
+  return guard([&] {
+    PGN_CHECK(M && ms_mean && reps > 0, "bad argument");
+    use_device(M->ctx);
+    hipStream_t s = M->stream();
+    const long m = M->m;
+    const size_t h64 = (size_t)TET_H_PACK * std::max<long>(m, 1) * sizeof(double);
+    const size_t h32 = (size_t)TET_H_PACK * std::max<long>(m, 1) * sizeof(float);
+    auto ensure_eH = [&](bool f32) -> void* {
+      if (f32 == M->hfp32) return M->eH.p;
+      const size_t need = f32 ? h32 : h64;
+      if (M->eH_alt.n < need) M->eH_alt.alloc(need);
+      return M->eH_alt.p;
+    };
+    auto ensure_eg32 = [&]() -> void* {
+      if (M->eg32.n < (size_t)(12 * std::max<long>(m, 1))) M->eg32.alloc(12 * std::max<long>(m, 1));
+      return M->eg32.p;
+    };
+    std::function<void()> launch;
+    switch (which) {
+      case 0: {
+        const bool f = M->hfp32;
+        void* eg = f ? ensure_eg32() : (void*)M->eg64.p;
+        void* eH = M->eH.p;
+        launch = [=] {
+          launch_tet_eval(s, f, EVAL_E | EVAL_G | EVAL_H, M->tet_args(f), M->blockE.p, nullptr,
+                          eg, eH);
+        };
+        break;
+      }
+      case 1:
+        launch = [=] {
+          launch_tet_eval(s, false, EVAL_E | EVAL_G, M->tet_args(false), M->blockE.p, nullptr,
+                          M->eg64.p, nullptr);
+        };
+        break;
+      case 2:
+        M->eval_hessian();
+        launch = [=] { launch_bsr_gather(s, M->hfp32, M->bsr_args(), M->eH.p, M->bval.p); };
+        break;
+      case 3: {
+        M->eval_hessian();
+        const size_t v = 3L * M->n * M->tsize();
+        char* base = M->pv.p;
+        launch = [=] {
+          launch_spmv(s, M->hfp32, M->n, M->rowptr.p, M->bcol.p, M->bval.p, base + 4 * v,
+                      base + 5 * v);
+        };
+        break;
+      }
+      case 4: {
+        void* eH = ensure_eH(false);
+        launch = [=] {
+          launch_tet_eval(s, false, EVAL_E | EVAL_G | EVAL_H, M->tet_args(false), M->blockE.p,
+                          nullptr, M->eg64.p, eH);
+        };
+        break;
+      }
+      case 5: {
+        void* eH = ensure_eH(true);
+        void* eg = ensure_eg32();
+        launch = [=] {
+          launch_tet_eval(s, true, EVAL_E | EVAL_G | EVAL_H, M->tet_args(true), M->blockE.p,
+                          nullptr, eg, eH);
+        };
+        break;
+      }
+      default:
+        throw Error(PGN_ERR_INVALID, "unknown kernel id");
+    }
+    launch();  // warm-up
+    PGN_HIP(hipGetLastError());
+    hipEvent_t e0, e1;
+    PGN_HIP(hipEventCreate(&e0));
+    PGN_HIP(hipEventCreate(&e1));
+    PGN_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) launch();
+    PGN_HIP(hipEventRecord(e1, s));
+    PGN_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    PGN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_mean = (double)ms / reps;
+    if (which == 2 || which == 3) M->hess_valid = false;
+  });
+}
+
+int pgn_host_bsr_pattern(int n_nodes, int n_tets, const int* T, long* nb_out, long* rowptr,
+                         int* col) {
+  return guard([&] {
+    PGN_CHECK(T && nb_out, "null argument");
+    BsrPattern P;
+    build_bsr_pattern(n_nodes, n_tets, T, P);
+    *nb_out = P.nb;
+    if (rowptr) std::copy(P.rowptr.begin(), P.rowptr.end(), rowptr);
+    if (col) std::copy(P.bcol.begin(), P.bcol.end(), col);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+// kernels.hpp -- host-callable launchers of libpgn's gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pgn {
+
+enum { EVAL_E = 1, EVAL_G = 2, EVAL_H = 4 };
+
+// Number of unique values of one packed tet Hessian: 6 off-diagonal 3x3
+// node blocks (a > b, row-major) followed by 4 diagonal blocks stored as
+// their lower triangles (6 values each).
+constexpr int TET_H_PACK = 78;
+constexpr int TET_BLOCK = 256;  // threads per element-kernel workgroup
+
+// Element inputs of a tet mesh.  prm holds 12 SoA arrays of m values:
+// rows b1, b2, b3 of Dm^-1 (9), V0, lame1, lame2.
+struct TetArgs {
+  int m;
+  const int4* tets;
+  const uint8_t* kind;
+  const void* prm;       // float* or double* (matches the kernel precision)
+  const double* x;       // 3n current positions
+  const double* X0;      // 3n rest positions
+};
+
+// Element evaluation.  blockE: one FP64 partial energy per workgroup
+// (ceil(m / TET_BLOCK) entries) or null; elemE: per-element FP64 energy or
+// null; eg: 12 values per element (precision T); eH: TET_H_PACK per element.
+void launch_tet_eval(hipStream_t s, bool fp32, int flags, const TetArgs& a,
+                     double* blockE, double* elemE, void* eg, void* eH);
+
+// Node gather of element gradients + lumped gravity + nodal forces, fixed
+// DoFs zeroed; also node energy terms and |g|^2, reduced with the element
+// energy partials into out2 = {E, |g|^2}.
+struct NodeArgs {
+  int n;
+  const int* inc_ptr;    // n+1
+  const int* inc;        // element*4 + local node, element order per node
+  const double* mass;    // n lumped masses
+  const uint8_t* fixmask;// n: bit k set => DoF 3i+k fixed
+  const double* x;       // 3n
+  double grav[3];
+  const double* f_ini;   // 3n or null
+  const double* f_end;   // 3n or null
+  double f_alpha;
+};
+void launch_node_gather(hipStream_t s, bool fp32_eg, const NodeArgs& a,
+                        const void* eg, double* g, double* node_partials);
+// Reduces n_elem_partials element partials and n_node_partials node partials
+// (pairs {E, gg}) into out[0] = E, out[1] = |g|^2, out[2] = |E_elastic| +
+// |E_load|.
+void launch_energy_final(hipStream_t s, const double* elem_partials,
+                         int n_elem_partials, const double* node_partials,
+                         int n_node_partials, double* out2);
+int node_gather_blocks(int n);
+
+// BSR3 Hessian gather from packed element Hessians, with the fixed-DoF
+// projection and the Jacobi / block-Jacobi preconditioner of the diagonal.
+struct BsrArgs {
+  long nb;
+  const int* brow;       // nb: block row (node) of each block
+  const int* bcol;       // nb
+  const long* cptr;      // nb+1 contribution ranges
+  const uint32_t* code;  // element*16 + 4*a + b
+  const uint8_t* fixmask;
+};
+void launch_bsr_gather(hipStream_t s, bool fp32, const BsrArgs& a,
+                       const void* eH, void* bval);
+// diag: n block indexes of the diagonal blocks.  Writes dinv (3n, Jacobi) and
+// binv (9n, block-Jacobi inverse) in precision T.
+void launch_precond(hipStream_t s, bool fp32, int n, const long* diag,
+                    const void* bval, void* dinv, void* binv);
+// bval[diag blocks] += shift on the scalar diagonal
+void launch_add_diag(hipStream_t s, bool fp32, int n, const long* diag,
+                     void* bval, double shift);
+// Frobenius norm^2 of the reference's stored lower entry set, reduced into
+// out[0] (partials: pcg_blocks() + 4 doubles, shared with residual / dot).
+int frob_blocks();
+void launch_frob_lower(hipStream_t s, bool fp32, long nb, const int* brow,
+                       const int* bcol, const void* bval, double* partials,
+                       double* out);
+
+// ---- PCG ------------------------------------------------------------------
+struct PcgScalars {
+  double rz[2];
+  double pq;
+  double rr;
+  double bb;
+  double thr;
+  double rz0;
+  int done;
+  int iters;
+  int it;     // iteration index, advanced by the last workgroup of K2
+  int maxit;  // iteration limit of the current solve
+  unsigned ticket1;
+  unsigned ticket2;
+};
+
+struct PcgArgs {
+  int n;                 // block rows (nodes)
+  const long* rowptr;    // n+1
+  const int* bcol;
+  const void* bval;
+  const void* dinv;      // Jacobi (3n)
+  const void* binv;      // block Jacobi (9n)
+  int precond;           // 0 Jacobi, 1 block Jacobi
+  void* x;
+  void* r;
+  void* z;
+  void* p0;
+  void* p1;
+  void* q;
+  double* part1;         // pcg_blocks() doubles
+  double* part2;         // 3 * pcg_blocks() doubles
+  PcgScalars* sc;
+};
+int pcg_blocks();
+// r = b - A x (b in T); z = M^-1 r; p = 0; scalars: bb, thr(tol), rz0, rr,
+// it = 0, maxit, done.  x_zero: start from x = 0 (x is zeroed, no product).
+void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b,
+                     double tol, int maxit, int x_zero);
+// one PCG iteration (two kernels; the index is read from sc->it; no-ops once
+// converged, so a chunk of these can be captured and replayed)
+void launch_pcg_iter(hipStream_t s, bool fp32, const PcgArgs& a);
+
+// r64 = b64 - A x64 with A in precision T and FP64 accumulation; writes
+// |r|^2 into out[0].  partials: pcg_blocks() doubles + one unsigned ticket.
+void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
+                       const int* bcol, const void* bval, const double* x,
+                       const double* b, double* r, double* partials,
+                       double* out);
+// y = A x in precision T (for timing and tests)
+void launch_spmv(hipStream_t s, bool fp32, int n, const long* rowptr,
+                 const int* bcol, const void* bval, const void* x, void* y);
+
+// small vector helpers
+void launch_cast_f64_to_f32(hipStream_t s, long n, const double* a, float* b);
+void launch_axpy_f32_to_f64(hipStream_t s, long n, const float* d, double* x);
+// out = a*x + b*y in FP64 (out may alias x or y)
+void launch_lincomb64(hipStream_t s, long n, double a, const double* x, double b,
+                      const double* y, double* out);
+// out[0] = dot(a, b) (FP64); partials: pcg_blocks() doubles + unsigned ticket
+void launch_dot64(hipStream_t s, long n, const double* a, const double* b,
+                  double* partials, double* out);
+// cast a T vector to FP64
+void launch_widen(hipStream_t s, bool fp32, long n, const void* a, double* b);
+void launch_narrow_or_copy(hipStream_t s, bool fp32, long n, const double* a,
+                           void* b);
+
+}  // namespace pgn

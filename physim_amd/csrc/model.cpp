@@ -1,0 +1,347 @@
+// model.cpp -- setup and evaluation of a device-resident tet FEM model.
+//
+// Setup restates Simulable_Mesh::InitInternal (Simulable_Mesh.cpp:68),
+// EnergyElement_FEM::Init (EnergyElement_FEM.cpp:59-122: Dm^-1 from the rest
+// shape, weight (1/6) det Dm) and MassElement_Lumped::ComputeAndStore_Mass
+// (MassElement_Lumped.cpp:47-54).  The BSR3 pattern and the element->block
+// slot map replace AMatrixSd's triplet assembly + pointer map
+// (BasicTypes.cpp:18-164, DerivativeAssembler.cpp:205-301).
+#include "model.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace pgn {
+
+void build_bsr_pattern(int n, int m, const int* T, BsrPattern& P) {
+  PGN_CHECK(n > 0, "model needs at least one node");
+  PGN_CHECK(m >= 0, "negative element count");
+  PGN_CHECK((long)m * 16 < (1L << 32), "too many elements for 32-bit slot codes");
+  for (long q = 0; q < 4L * m; ++q)
+    PGN_CHECK(T[q] >= 0 && T[q] < n,
+              "element " + std::to_string(q / 4) + " has node id out of range");
+  P.n = n;
+  // node -> (element, local) incidence, element order
+  P.inc_ptr.assign(n + 1, 0);
+  for (long q = 0; q < 4L * m; ++q) P.inc_ptr[T[q] + 1]++;
+  for (int i = 0; i < n; ++i) P.inc_ptr[i + 1] += P.inc_ptr[i];
+  P.inc.resize(4L * m);
+  {
+    std::vector<int> cur(P.inc_ptr.begin(), P.inc_ptr.end() - 1);
+    for (int e = 0; e < m; ++e)
+      for (int a = 0; a < 4; ++a) P.inc[cur[T[4L * e + a]]++] = 4 * e + a;
+  }
+  // rows: sorted unique neighbour nodes (including the node itself)
+  P.rowptr.assign(n + 1, 0);
+  std::vector<int> mark(n, -1), nb;
+  std::vector<int> cols;
+  cols.reserve(27L * n);
+  for (int i = 0; i < n; ++i) {
+    nb.clear();
+    mark[i] = i;
+    nb.push_back(i);
+    for (int p = P.inc_ptr[i]; p < P.inc_ptr[i + 1]; ++p) {
+      const int e = P.inc[p] >> 2;
+      for (int b = 0; b < 4; ++b) {
+        const int j = T[4L * e + b];
+        if (mark[j] != i) {
+          mark[j] = i;
+          nb.push_back(j);
+        }
+      }
+    }
+    std::sort(nb.begin(), nb.end());
+    cols.insert(cols.end(), nb.begin(), nb.end());
+    P.rowptr[i + 1] = (long)cols.size();
+  }
+  P.nb = (long)cols.size();
+  P.bcol.swap(cols);
+  P.brow.resize(P.nb);
+  P.diag.resize(n);
+  for (int i = 0; i < n; ++i)
+    for (long s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
+      P.brow[s] = i;
+      if (P.bcol[s] == i) P.diag[i] = s;
+    }
+  auto find = [&](int i, int j) -> long {
+    const int* b = P.bcol.data() + P.rowptr[i];
+    const int* e = P.bcol.data() + P.rowptr[i + 1];
+    const int* it = std::lower_bound(b, e, j);
+    return P.rowptr[i] + (it - b);
+  };
+  // contributions per block, in element order
+  P.cptr.assign(P.nb + 1, 0);
+  for (int i = 0; i < n; ++i)
+    for (int p = P.inc_ptr[i]; p < P.inc_ptr[i + 1]; ++p) {
+      const int e = P.inc[p] >> 2;
+      for (int b = 0; b < 4; ++b) P.cptr[find(i, T[4L * e + b]) + 1]++;
+    }
+  for (long s = 0; s < P.nb; ++s) P.cptr[s + 1] += P.cptr[s];
+  P.code.resize(P.cptr[P.nb]);
+  std::vector<long> cur(P.cptr.begin(), P.cptr.end() - 1);
+  for (int i = 0; i < n; ++i)
+    for (int p = P.inc_ptr[i]; p < P.inc_ptr[i + 1]; ++p) {
+      const int e = P.inc[p] >> 2, a = P.inc[p] & 3;
+      for (int b = 0; b < 4; ++b) {
+        const long s = find(i, T[4L * e + b]);
+        P.code[cur[s]++] = (uint32_t)e * 16u + (uint32_t)(4 * a + b);
+      }
+    }
+}
+
+}  // namespace pgn
+
+using namespace pgn;
+
+pgn_model::~pgn_model() {
+  if (pcg_graph) (void)hipGraphExecDestroy(pcg_graph);
+  if (hpin) (void)hipHostFree(hpin);
+  if (hsc) (void)hipHostFree(hsc);
+}
+
+void pgn_model::setup(int n_nodes, const double* X0h, int n_tets, const int* Th,
+                      const uint8_t* kindh, const double* l1, const double* l2,
+                      const double* rho, int precision) {
+  PGN_CHECK(precision == PGN_FP64 || precision == PGN_MIXED, "unknown precision");
+  PGN_CHECK(X0h && (n_tets == 0 || (Th && kindh && l1 && l2 && rho)),
+            "null input array");
+  n = n_nodes;
+  m = n_tets;
+  prec = precision;
+  hfp32 = (precision == PGN_MIXED);
+  hipStream_t s = stream();
+  build_bsr_pattern(n, m, Th, pat);
+  hX0.assign(X0h, X0h + 3L * n);
+  hfix.assign(3L * n, 0);
+
+  // element parameters: rows of Dm^-1, V0 = det(Dm)/6, lame1, lame2
+  std::vector<double> p64(12L * m);
+  std::vector<float> p32(12L * m);
+  std::vector<double> hmass(n, 0.0);
+  std::vector<int4> ht(m);
+  for (int e = 0; e < m; ++e) {
+    PGN_CHECK(kindh[e] <= 1, "unknown material kind at element " + std::to_string(e));
+    const int* t = Th + 4L * e;
+    ht[e] = make_int4(t[0], t[1], t[2], t[3]);
+    double Dm[3][3];  // Dm(i, a) = X_{a+1}(i) - X_0(i)
+    for (int a = 0; a < 3; ++a)
+      for (int i = 0; i < 3; ++i) Dm[i][a] = X0h[3L * t[a + 1] + i] - X0h[3L * t[0] + i];
+    const double c00 = Dm[1][1] * Dm[2][2] - Dm[1][2] * Dm[2][1];
+    const double c01 = Dm[1][2] * Dm[2][0] - Dm[1][0] * Dm[2][2];
+    const double c02 = Dm[1][0] * Dm[2][1] - Dm[1][1] * Dm[2][0];
+    const double det = Dm[0][0] * c00 + Dm[0][1] * c01 + Dm[0][2] * c02;
+    PGN_CHECK(det != 0.0 && std::isfinite(det),
+              "degenerate rest element " + std::to_string(e));
+    const double id = 1.0 / det;
+    double Bi[3][3];
+    Bi[0][0] = c00 * id;
+    Bi[0][1] = (Dm[0][2] * Dm[2][1] - Dm[0][1] * Dm[2][2]) * id;
+    Bi[0][2] = (Dm[0][1] * Dm[1][2] - Dm[0][2] * Dm[1][1]) * id;
+    Bi[1][0] = c01 * id;
+    Bi[1][1] = (Dm[0][0] * Dm[2][2] - Dm[0][2] * Dm[2][0]) * id;
+    Bi[1][2] = (Dm[0][2] * Dm[1][0] - Dm[0][0] * Dm[1][2]) * id;
+    Bi[2][0] = c02 * id;
+    Bi[2][1] = (Dm[0][1] * Dm[2][0] - Dm[0][0] * Dm[2][1]) * id;
+    Bi[2][2] = (Dm[0][0] * Dm[1][1] - Dm[0][1] * Dm[1][0]) * id;
+    const double V0 = det / 6.0;
+    for (int c = 0; c < 9; ++c) p64[(long)c * m + e] = Bi[c / 3][c % 3];
+    p64[9L * m + e] = V0;
+    p64[10L * m + e] = l1[e];
+    p64[11L * m + e] = l2[e];
+    for (int c = 0; c < 12; ++c) p32[(long)c * m + e] = (float)p64[(long)c * m + e];
+  }
+  // lumped masses, summed per node in element order
+  for (int i = 0; i < n; ++i)
+    for (int p = pat.inc_ptr[i]; p < pat.inc_ptr[i + 1]; ++p) {
+      const int e = pat.inc[p] >> 2;
+      hmass[i] += rho[e] * p64[9L * m + e] / 4.0;
+    }
+
+  X0.upload(X0h, 3L * n, s);
+  x.upload(X0h, 3L * n, s);
+  tets.upload(ht, s);
+  kind.upload(kindh, m, s);
+  prm64.upload(p64, s);
+  prm32.upload(p32, s);
+  mass.upload(hmass, s);
+  std::vector<uint8_t> fm(n, 0);
+  fixmask.upload(fm, s);
+  inc_ptr.upload(pat.inc_ptr, s);
+  inc.upload(pat.inc, s);
+  rowptr.upload(pat.rowptr, s);
+  cptr.upload(pat.cptr, s);
+  diag.upload(pat.diag, s);
+  bcol.upload(pat.bcol, s);
+  brow.upload(pat.brow, s);
+  code.upload(pat.code, s);
+  const size_t ts = tsize();
+  bval.alloc(9 * pat.nb * ts);
+  dinv.alloc(3L * n * ts);
+  binv.alloc(9L * n * ts);
+  eg64.alloc(12L * std::max(m, 1));
+  eH.alloc((size_t)TET_H_PACK * std::max(m, 1) * ts);
+  blockE.alloc((m + TET_BLOCK - 1) / TET_BLOCK + 1);
+  nodeP.alloc(2L * node_gather_blocks(n) + 2);
+  scal.alloc(16);
+  g.alloc(3L * n);
+  pv.alloc(6 * 3L * n * ts);
+  pb.alloc(3L * n * ts);
+  const int PB = pcg_blocks();
+  part1.alloc(PB + 8);
+  part2.alloc(3L * PB + 8);
+  red64.alloc(PB + 8);
+  sc.alloc(1);
+  r64.alloc(3L * n);
+  tmp64a.alloc(3L * n);
+  tmp64b.alloc(3L * n);
+  PGN_HIP(hipMemsetAsync(part1.p, 0, part1.bytes(), s));
+  PGN_HIP(hipMemsetAsync(part2.p, 0, part2.bytes(), s));
+  PGN_HIP(hipMemsetAsync(red64.p, 0, red64.bytes(), s));
+  PGN_HIP(hipMemsetAsync(sc.p, 0, sc.bytes(), s));
+  PGN_HIP(hipMemsetAsync(pv.p, 0, pv.bytes(), s));
+  PGN_HIP(hipHostMalloc((void**)&hpin, 16 * sizeof(double), hipHostMallocDefault));
+  PGN_HIP(hipHostMalloc((void**)&hsc, sizeof(PcgScalars), hipHostMallocDefault));
+  sync();
+}
+
+void pgn_model::sync() { PGN_HIP(hipStreamSynchronize(stream())); }
+
+// BCondition.h:140-155 GetCurValues with VectorAnimator_LerpValues
+void pgn_model::cur_gravity(double gv[3]) const {
+  if (!has_g) {
+    gv[0] = gv[1] = gv[2] = 0.0;
+    return;
+  }
+  const double a = (double)g_cur / (double)g_stages;
+  for (int i = 0; i < 3; ++i) gv[i] = g_ini[i] + (g_end[i] - g_ini[i]) * a;
+}
+
+double pgn_model::cur_force_alpha() const {
+  return has_f ? (double)f_cur / (double)f_stages : 0.0;
+}
+
+// BCondition.cpp:42 StepLoading for every BC
+bool pgn_model::step_load() {
+  bool stepped = false;
+  if (has_g && g_cur < g_stages) {
+    ++g_cur;
+    stepped = true;
+  }
+  if (has_f && f_cur < f_stages) {
+    ++f_cur;
+    stepped = true;
+  }
+  return stepped;
+}
+
+bool pgn_model::is_loaded() const {
+  if (has_g && g_cur != g_stages) return false;
+  if (has_f && f_cur != f_stages) return false;
+  return true;
+}
+
+TetArgs pgn_model::tet_args(bool fp32) const {
+  TetArgs a;
+  a.m = m;
+  a.tets = tets.p;
+  a.kind = kind.p;
+  a.prm = fp32 ? (const void*)prm32.p : (const void*)prm64.p;
+  a.x = x.p;
+  a.X0 = X0.p;
+  return a;
+}
+
+NodeArgs pgn_model::node_args() const {
+  NodeArgs a;
+  a.n = n;
+  a.inc_ptr = inc_ptr.p;
+  a.inc = inc.p;
+  a.mass = mass.p;
+  a.fixmask = fixmask.p;
+  a.x = x.p;
+  cur_gravity(a.grav);
+  a.f_ini = has_f ? f_ini.p : nullptr;
+  a.f_end = has_f ? f_end.p : nullptr;
+  a.f_alpha = cur_force_alpha();
+  return a;
+}
+
+BsrArgs pgn_model::bsr_args() const {
+  BsrArgs a;
+  a.nb = pat.nb;
+  a.brow = brow.p;
+  a.bcol = bcol.p;
+  a.cptr = cptr.p;
+  a.code = code.p;
+  a.fixmask = fixmask.p;
+  return a;
+}
+
+PcgArgs pgn_model::pcg_args(int /*maxit*/, int precond) const {
+  PcgArgs a;
+  const size_t v = 3L * n * tsize();
+  char* base = pv.p;
+  a.n = n;
+  a.rowptr = rowptr.p;
+  a.bcol = bcol.p;
+  a.bval = bval.p;
+  a.dinv = dinv.p;
+  a.binv = binv.p;
+  a.precond = precond;
+  a.x = base;
+  a.r = base + v;
+  a.z = base + 2 * v;
+  a.p0 = base + 3 * v;
+  a.p1 = base + 4 * v;
+  a.q = base + 5 * v;
+  a.part1 = part1.p;
+  a.part2 = part2.p;
+  a.sc = sc.p;
+  return a;
+}
+
+void pgn_model::launch_eg() {
+  hipStream_t s = stream();
+  launch_tet_eval(s, false, EVAL_E | EVAL_G, tet_args(false), blockE.p, nullptr,
+                  eg64.p, nullptr);
+  const NodeArgs na = node_args();
+  launch_node_gather(s, false, na, eg64.p, g.p, nodeP.p);
+  launch_energy_final(s, blockE.p, (m + TET_BLOCK - 1) / TET_BLOCK, nodeP.p,
+                      node_gather_blocks(n), scal.p);
+}
+
+void pgn_model::eval_eg(double& E, double& gg, double* escale) {
+  launch_eg();
+  PGN_HIP(hipMemcpyAsync(hpin, scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  sync();
+  E = hpin[0];
+  gg = hpin[1];
+  if (escale) *escale = hpin[2];
+}
+
+void pgn_model::update_precond() {
+  launch_precond(stream(), hfp32, n, diag.p, bval.p, dinv.p, binv.p);
+}
+
+void pgn_model::eval_hessian() {
+  hipStream_t s = stream();
+  launch_tet_eval(s, hfp32, EVAL_H, tet_args(hfp32), nullptr, nullptr, nullptr, eH.p);
+  launch_bsr_gather(s, hfp32, bsr_args(), eH.p, bval.p);
+  update_precond();
+  hess_valid = true;
+}
+
+double pgn_model::frob_lower() {
+  launch_frob_lower(stream(), hfp32, pat.nb, brow.p, bcol.p, bval.p, red64.p, scal.p + 5);
+  PGN_HIP(hipMemcpyAsync(hpin + 5, scal.p + 5, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  sync();
+  return std::sqrt(hpin[5]);
+}
+
+double pgn_model::dot(const double* a, const double* b, long len) {
+  launch_dot64(stream(), len, a, b, red64.p, scal.p + 3);
+  PGN_HIP(hipMemcpyAsync(hpin + 3, scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  sync();
+  return hpin[3];
+}

@@ -1,0 +1,112 @@
+// model.hpp -- device-resident tet FEM model (the ISimulable of libpgn).
+#pragma once
+
+#include <memory>
+
+#include "kernels.hpp"
+#include "pgn_internal.hpp"
+
+namespace pgn {
+
+// Host-side pattern of the BSR3 Hessian and the element->block slot map.
+struct BsrPattern {
+  int n = 0;
+  long nb = 0;
+  std::vector<long> rowptr;   // n+1
+  std::vector<int> bcol;      // nb
+  std::vector<int> brow;      // nb
+  std::vector<long> diag;     // n: index of block (i, i)
+  std::vector<long> cptr;     // nb+1
+  std::vector<uint32_t> code; // element*16 + 4a + b, element order per block
+  std::vector<int> inc_ptr;   // n+1: node -> incident (element, local) list
+  std::vector<int> inc;       // element*4 + a
+};
+// Builds the pattern of a tet mesh (throws on bad node ids).
+void build_bsr_pattern(int n, int m, const int* T, BsrPattern& P);
+
+}  // namespace pgn
+
+struct pgn_model {
+  pgn_ctx* ctx = nullptr;
+  int n = 0, m = 0;
+  int prec = PGN_FP64;
+  bool hfp32 = false;  // Hessian / PCG precision is FP32
+
+  pgn::BsrPattern pat;
+  std::vector<double> hX0;
+  std::vector<uint8_t> hfix;  // 3n
+
+  // geometry and element parameters
+  pgn::DBuf<double> X0, x;
+  pgn::DBuf<int4> tets;
+  pgn::DBuf<uint8_t> kind;
+  pgn::DBuf<double> prm64;  // 12 x m SoA
+  pgn::DBuf<float> prm32;   // 12 x m SoA
+  pgn::DBuf<double> mass;   // n
+  pgn::DBuf<uint8_t> fixmask;  // n
+  pgn::DBuf<int> inc_ptr, inc;
+  // BSR Hessian
+  pgn::DBuf<long> rowptr, cptr, diag;
+  pgn::DBuf<int> bcol, brow;
+  pgn::DBuf<uint32_t> code;
+  pgn::DBuf<char> bval, dinv, binv;  // precision of the Hessian
+  // element outputs
+  pgn::DBuf<double> eg64;
+  pgn::DBuf<float> eg32;
+  pgn::DBuf<char> eH;     // Hessian precision
+  pgn::DBuf<char> eH_alt; // the other precision (timing / parity only)
+  pgn::DBuf<double> blockE, nodeP, scal;  // scal: {E, gg, frob, dot, rr, ...}
+  pgn::DBuf<double> g;    // 3n gradient at x
+  // boundary conditions
+  bool has_g = false;
+  double g_ini[3] = {0, 0, 0}, g_end[3] = {0, 0, 0};
+  int g_stages = 1, g_cur = 0;
+  bool has_f = false;
+  pgn::DBuf<double> f_ini, f_end;
+  int f_stages = 1, f_cur = 0;
+  // PCG work space (Hessian precision) + FP64 refinement vectors
+  pgn::DBuf<char> pv;     // 6 vectors of 3n: x r z p0 p1 q
+  pgn::DBuf<char> pb;     // rhs in Hessian precision
+  pgn::DBuf<double> part1, part2, red64;
+  pgn::DBuf<pgn::PcgScalars> sc;
+  pgn::DBuf<double> r64, tmp64a, tmp64b;
+  pgn::DBuf<double> nw_xsave, nw_v0, nw_dx, nw_b;  // Newton work vectors
+  double* hpin = nullptr;  // pinned host scalars
+  pgn::PcgScalars* hsc = nullptr;
+  bool hess_valid = false;
+  int pcg_chunk_max = 64;
+  hipGraphExec_t pcg_graph = nullptr;
+  int pcg_graph_len = 0;
+  bool use_graphs = true;
+
+  ~pgn_model();
+  void setup(int n_nodes, const double* X0h, int n_tets, const int* Th,
+             const uint8_t* kindh, const double* l1, const double* l2,
+             const double* rho, int precision);
+  hipStream_t stream() const { return ctx->stream; }
+  size_t tsize() const { return hfp32 ? sizeof(float) : sizeof(double); }
+
+  // BCs
+  void cur_gravity(double gv[3]) const;
+  double cur_force_alpha() const;
+  bool step_load();
+  bool is_loaded() const;
+
+  pgn::TetArgs tet_args(bool fp32) const;
+  pgn::NodeArgs node_args() const;
+  pgn::BsrArgs bsr_args() const;
+  pgn::PcgArgs pcg_args(int maxit, int precond) const;
+
+  // E+g (FP64) at the current x; returns E, |g|^2 and the energy scale
+  // |E_elastic| + |E_load| (synchronizes)
+  void eval_eg(double& E, double& gg, double* escale = nullptr);
+  // launches E+g (FP64) without reading back
+  void launch_eg();
+  // element Hessians (model precision) + BSR gather + preconditioners
+  void eval_hessian();
+  void update_precond();
+  // Frobenius norm of the stored lower entry set (synchronizes)
+  double frob_lower();
+  double dot(const double* a, const double* b, long n);  // synchronizes
+  void sync();
+};

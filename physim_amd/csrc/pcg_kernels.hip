@@ -1,0 +1,463 @@
+// pcg_kernels.hip -- preconditioned CG on the BSR3 Hessian for gfx950.
+//
+// The algorithm is the textbook Jacobi-PCG that Eigen's ConjugateGradient
+// implements (LinearSolver_EigenCG.cpp:89 solveWithGuess; Saad, Iterative
+// Methods, Alg. 9.1) with Eigen's stopping rule |r|^2 < max(tol^2 |b|^2,
+// DBL_MIN).  One iteration is two launches:
+//   K1: beta = rz_k / rz_{k-1};  p_k = z_k + beta p_{k-1} (recomputed per
+//       column on the fly, written for the own rows into the other p buffer);
+//       q = A p_k;  partial p.q
+//   K2: alpha = rz_k / p.q;  x += alpha p;  r -= alpha q;  z = M^-1 r;
+//       partial r.r, r.z; convergence test
+// Reductions are per-workgroup partials summed in a fixed order by the last
+// workgroup to arrive (ticket counter), so every run gives identical bits.
+// The iteration index and the done flag live in device memory: every launch
+// of a chunk has the same arguments, so a chunk can be captured in a hipGraph
+// and replayed; once converged all further launches return immediately.
+#include "kernels.hpp"
+#include "reduce.hpp"
+
+namespace pgn {
+
+constexpr int PCG_BS = 256;
+constexpr int LPR = 8;                    // lanes per block row in SpMV
+constexpr int ROWS_PER_BLOCK = PCG_BS / LPR;
+constexpr int PCG_MAXGRID = 1024;
+
+int pcg_blocks() { return PCG_MAXGRID; }
+
+static inline int grid_rows(int n) {
+  int g = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
+  return g < 1 ? 1 : (g > PCG_MAXGRID ? PCG_MAXGRID : g);
+}
+static inline int grid_nodes(int n) {
+  int g = (n + PCG_BS - 1) / PCG_BS;
+  return g < 1 ? 1 : (g > PCG_MAXGRID ? PCG_MAXGRID : g);
+}
+
+template <class T>
+__device__ __forceinline__ T group_sum8(T v) {
+  v += __shfl_xor(v, 4, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 1, 64);
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ void apply_precond(int precond, const T* __restrict__ dinv,
+                                              const T* __restrict__ binv, long i,
+                                              const T r[3], T z[3]) {
+  if (precond == 1) {
+    const T* M = binv + 9 * i;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) z[p] = M[3 * p] * r[0] + M[3 * p + 1] * r[1] + M[3 * p + 2] * r[2];
+  } else {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) z[p] = dinv[3 * i + p] * r[p];
+  }
+}
+
+// ---- init: r = b - A x, z = M^-1 r, p0 = p1 = 0 --------------------------------
+template <class T>
+__global__ __launch_bounds__(PCG_BS) void pcg_init_kernel(PcgArgs A, const T* __restrict__ b,
+                                                          double tol, int maxit, int x_zero) {
+  __shared__ double red[3 * PCG_BS / 64];
+  __shared__ bool last;
+  const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
+  T* __restrict__ x = reinterpret_cast<T*>(A.x);
+  T* __restrict__ r = reinterpret_cast<T*>(A.r);
+  T* __restrict__ z = reinterpret_cast<T*>(A.z);
+  T* __restrict__ p0 = reinterpret_cast<T*>(A.p0);
+  T* __restrict__ p1 = reinterpret_cast<T*>(A.p1);
+  const int lane = threadIdx.x & (LPR - 1);
+  double bb = 0.0, rr = 0.0, rz = 0.0;
+  for (int row = (blockIdx.x * PCG_BS + threadIdx.x) / LPR; row < A.n;
+       row += gridDim.x * ROWS_PER_BLOCK) {
+    T y0 = 0, y1 = 0, y2 = 0;
+    if (!x_zero) {
+      const long e1 = A.rowptr[row + 1];
+      for (long s = A.rowptr[row] + lane; s < e1; s += LPR) {
+        const long j = A.bcol[s];
+        const T* B = bval + s * 9;
+        const T xj0 = x[3 * j], xj1 = x[3 * j + 1], xj2 = x[3 * j + 2];
+        y0 += B[0] * xj0 + B[1] * xj1 + B[2] * xj2;
+        y1 += B[3] * xj0 + B[4] * xj1 + B[5] * xj2;
+        y2 += B[6] * xj0 + B[7] * xj1 + B[8] * xj2;
+      }
+      y0 = group_sum8(y0);
+      y1 = group_sum8(y1);
+      y2 = group_sum8(y2);
+    }
+    if (lane == 0) {
+      const long i = row;
+      T ri[3] = {b[3 * i] - y0, b[3 * i + 1] - y1, b[3 * i + 2] - y2};
+      T zi[3];
+      apply_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
+                       reinterpret_cast<const T*>(A.binv), i, ri, zi);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        r[3 * i + p] = ri[p];
+        z[3 * i + p] = zi[p];
+        p0[3 * i + p] = T(0);
+        p1[3 * i + p] = T(0);
+        if (x_zero) x[3 * i + p] = T(0);
+        bb += (double)b[3 * i + p] * (double)b[3 * i + p];
+        rr += (double)ri[p] * (double)ri[p];
+        rz += (double)ri[p] * (double)zi[p];
+      }
+    }
+  }
+  const double sbb = block_sum<PCG_BS>(bb, red);
+  double srr = rr, srz = rz;
+  block_sum2<PCG_BS>(srr, srz, red);
+  if (threadIdx.x == 0) {
+    A.part2[3 * blockIdx.x] = sbb;
+    A.part2[3 * blockIdx.x + 1] = srr;
+    A.part2[3 * blockIdx.x + 2] = srz;
+  }
+  if (last_block(&A.sc->ticket2, &last)) {
+    const double tbb = block_sum_array<PCG_BS>(A.part2, gridDim.x, 3, red);
+    const double trr = block_sum_array<PCG_BS>(A.part2 + 1, gridDim.x, 3, red);
+    const double trz = block_sum_array<PCG_BS>(A.part2 + 2, gridDim.x, 3, red);
+    if (threadIdx.x == 0) {
+      PcgScalars* sc = A.sc;
+      double thr = tol * tol * tbb;
+      if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+      sc->bb = tbb;
+      sc->thr = thr;
+      sc->rr = trr;
+      sc->rz[0] = trz;
+      sc->rz[1] = trz;
+      sc->rz0 = trz;
+      sc->pq = 0.0;
+      sc->iters = 0;
+      sc->it = 0;
+      sc->maxit = maxit;
+      sc->done = (tbb == 0.0 || trr < thr || maxit <= 0) ? 1 : 0;
+    }
+  }
+}
+
+// ---- K1: p_k = z + beta p_{k-1}; q = A p_k; p.q ---------------------------------
+template <class T>
+__global__ __launch_bounds__(PCG_BS) void pcg_k1_kernel(PcgArgs A) {
+  __shared__ double red[PCG_BS / 64];
+  __shared__ bool last;
+  PcgScalars* sc = A.sc;
+  if (sc->done) return;
+  const int k = sc->it;
+  const double beta = (k > 0) ? sc->rz[k & 1] / sc->rz[(k - 1) & 1] : 0.0;
+  // rz[(k+1)&1] is written by the last workgroup of K2 of this iteration,
+  // after every workgroup of this launch has finished.
+  const T tb = (T)beta;
+  const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
+  const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
+  T* __restrict__ pnew = reinterpret_cast<T*>((k & 1) ? A.p1 : A.p0);
+  const T* __restrict__ pold = reinterpret_cast<const T*>((k & 1) ? A.p0 : A.p1);
+  T* __restrict__ q = reinterpret_cast<T*>(A.q);
+  const int lane = threadIdx.x & (LPR - 1);
+  double pq = 0.0;
+  for (int row = (blockIdx.x * PCG_BS + threadIdx.x) / LPR; row < A.n;
+       row += gridDim.x * ROWS_PER_BLOCK) {
+    T y0 = 0, y1 = 0, y2 = 0;
+    const long e1 = A.rowptr[row + 1];
+    for (long s = A.rowptr[row] + lane; s < e1; s += LPR) {
+      const long j = A.bcol[s];
+      const T* B = bval + s * 9;
+      const T pj0 = z[3 * j] + tb * pold[3 * j];
+      const T pj1 = z[3 * j + 1] + tb * pold[3 * j + 1];
+      const T pj2 = z[3 * j + 2] + tb * pold[3 * j + 2];
+      y0 += B[0] * pj0 + B[1] * pj1 + B[2] * pj2;
+      y1 += B[3] * pj0 + B[4] * pj1 + B[5] * pj2;
+      y2 += B[6] * pj0 + B[7] * pj1 + B[8] * pj2;
+    }
+    y0 = group_sum8(y0);
+    y1 = group_sum8(y1);
+    y2 = group_sum8(y2);
+    if (lane == 0) {
+      const long i = row;
+      const T pi0 = z[3 * i] + tb * pold[3 * i];
+      const T pi1 = z[3 * i + 1] + tb * pold[3 * i + 1];
+      const T pi2 = z[3 * i + 2] + tb * pold[3 * i + 2];
+      pnew[3 * i] = pi0;
+      pnew[3 * i + 1] = pi1;
+      pnew[3 * i + 2] = pi2;
+      q[3 * i] = y0;
+      q[3 * i + 1] = y1;
+      q[3 * i + 2] = y2;
+      pq += (double)pi0 * (double)y0 + (double)pi1 * (double)y1 + (double)pi2 * (double)y2;
+    }
+  }
+  const double s = block_sum<PCG_BS>(pq, red);
+  if (threadIdx.x == 0) A.part1[blockIdx.x] = s;
+  if (last_block(&sc->ticket1, &last)) {
+    const double t = block_sum_array<PCG_BS>(A.part1, gridDim.x, 1, red);
+    if (threadIdx.x == 0) sc->pq = t;
+  }
+}
+
+// ---- K2: x += alpha p; r -= alpha q; z = M^-1 r; r.r, r.z; convergence -------
+template <class T>
+__global__ __launch_bounds__(PCG_BS) void pcg_k2_kernel(PcgArgs A) {
+  __shared__ double red[2 * PCG_BS / 64];
+  __shared__ bool last;
+  PcgScalars* sc = A.sc;
+  if (sc->done) return;
+  const int k = sc->it;
+  const double alpha = sc->rz[k & 1] / sc->pq;
+  const T ta = (T)alpha;
+  const T* __restrict__ pk = reinterpret_cast<const T*>((k & 1) ? A.p1 : A.p0);
+  const T* __restrict__ q = reinterpret_cast<const T*>(A.q);
+  T* __restrict__ x = reinterpret_cast<T*>(A.x);
+  T* __restrict__ r = reinterpret_cast<T*>(A.r);
+  T* __restrict__ z = reinterpret_cast<T*>(A.z);
+  double rr = 0.0, rz = 0.0;
+  for (long i = blockIdx.x * PCG_BS + threadIdx.x; i < A.n; i += (long)gridDim.x * PCG_BS) {
+    T ri[3], zi[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      x[3 * i + p] += ta * pk[3 * i + p];
+      ri[p] = r[3 * i + p] - ta * q[3 * i + p];
+      r[3 * i + p] = ri[p];
+    }
+    apply_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
+                     reinterpret_cast<const T*>(A.binv), i, ri, zi);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      z[3 * i + p] = zi[p];
+      rr += (double)ri[p] * (double)ri[p];
+      rz += (double)ri[p] * (double)zi[p];
+    }
+  }
+  block_sum2<PCG_BS>(rr, rz, red);
+  if (threadIdx.x == 0) {
+    A.part2[2 * blockIdx.x] = rr;
+    A.part2[2 * blockIdx.x + 1] = rz;
+  }
+  if (last_block(&sc->ticket2, &last)) {
+    const double trr = block_sum_array<PCG_BS>(A.part2, gridDim.x, 2, red);
+    const double trz = block_sum_array<PCG_BS>(A.part2 + 1, gridDim.x, 2, red);
+    if (threadIdx.x == 0) {
+      sc->rr = trr;
+      sc->rz[(k + 1) & 1] = trz;
+      if (trr < sc->thr) {
+        sc->done = 1;
+        sc->iters = k;
+      } else if (k + 1 >= sc->maxit) {
+        sc->done = 1;
+        sc->iters = k + 1;
+      }
+      sc->it = k + 1;
+    }
+  }
+}
+
+void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b,
+                     double tol, int maxit, int x_zero) {
+  const int g = grid_rows(a.n);
+  if (fp32)
+    hipLaunchKernelGGL(pcg_init_kernel<float>, dim3(g), dim3(PCG_BS), 0, s, a,
+                       reinterpret_cast<const float*>(b), tol, maxit, x_zero);
+  else
+    hipLaunchKernelGGL(pcg_init_kernel<double>, dim3(g), dim3(PCG_BS), 0, s, a,
+                       reinterpret_cast<const double*>(b), tol, maxit, x_zero);
+}
+
+void launch_pcg_iter(hipStream_t s, bool fp32, const PcgArgs& a) {
+  const int g1 = grid_rows(a.n), g2 = grid_nodes(a.n);
+  if (fp32) {
+    hipLaunchKernelGGL(pcg_k1_kernel<float>, dim3(g1), dim3(PCG_BS), 0, s, a);
+    hipLaunchKernelGGL(pcg_k2_kernel<float>, dim3(g2), dim3(PCG_BS), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(pcg_k1_kernel<double>, dim3(g1), dim3(PCG_BS), 0, s, a);
+    hipLaunchKernelGGL(pcg_k2_kernel<double>, dim3(g2), dim3(PCG_BS), 0, s, a);
+  }
+}
+
+// ---- FP64 residual with a precision-T matrix --------------------------------
+template <class TM>
+__global__ __launch_bounds__(PCG_BS) void residual64_kernel(
+    int n, const long* __restrict__ rowptr, const int* __restrict__ bcol,
+    const TM* __restrict__ bval, const double* __restrict__ x,
+    const double* __restrict__ b, double* __restrict__ r, double* part,
+    double* out, unsigned* ticket) {
+  __shared__ double red[PCG_BS / 64];
+  __shared__ bool last;
+  const int lane = threadIdx.x & (LPR - 1);
+  double rr = 0.0;
+  for (int row = (blockIdx.x * PCG_BS + threadIdx.x) / LPR; row < n;
+       row += gridDim.x * ROWS_PER_BLOCK) {
+    double y0 = 0, y1 = 0, y2 = 0;
+    const long e1 = rowptr[row + 1];
+    for (long s = rowptr[row] + lane; s < e1; s += LPR) {
+      const long j = bcol[s];
+      const TM* B = bval + s * 9;
+      const double xj0 = x[3 * j], xj1 = x[3 * j + 1], xj2 = x[3 * j + 2];
+      y0 += (double)B[0] * xj0 + (double)B[1] * xj1 + (double)B[2] * xj2;
+      y1 += (double)B[3] * xj0 + (double)B[4] * xj1 + (double)B[5] * xj2;
+      y2 += (double)B[6] * xj0 + (double)B[7] * xj1 + (double)B[8] * xj2;
+    }
+    y0 = group_sum8(y0);
+    y1 = group_sum8(y1);
+    y2 = group_sum8(y2);
+    if (lane == 0) {
+      const long i = row;
+      const double r0 = b[3 * i] - y0, r1 = b[3 * i + 1] - y1, r2 = b[3 * i + 2] - y2;
+      r[3 * i] = r0;
+      r[3 * i + 1] = r1;
+      r[3 * i + 2] = r2;
+      rr += r0 * r0 + r1 * r1 + r2 * r2;
+    }
+  }
+  const double s = block_sum<PCG_BS>(rr, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (last_block(ticket, &last)) {
+    const double t = block_sum_array<PCG_BS>(part, gridDim.x, 1, red);
+    if (threadIdx.x == 0) out[0] = t;
+  }
+}
+
+// partials: PCG_MAXGRID doubles followed by an unsigned ticket
+void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
+                       const int* bcol, const void* bval, const double* x,
+                       const double* b, double* r, double* partials,
+                       double* out) {
+  const int g = grid_rows(n);
+  unsigned* ticket = reinterpret_cast<unsigned*>(partials + PCG_MAXGRID);
+  if (fp32_mat)
+    hipLaunchKernelGGL(residual64_kernel<float>, dim3(g), dim3(PCG_BS), 0, s, n, rowptr,
+                       bcol, reinterpret_cast<const float*>(bval), x, b, r, partials,
+                       out, ticket);
+  else
+    hipLaunchKernelGGL(residual64_kernel<double>, dim3(g), dim3(PCG_BS), 0, s, n, rowptr,
+                       bcol, reinterpret_cast<const double*>(bval), x, b, r, partials,
+                       out, ticket);
+}
+
+// ---- plain SpMV (timing, tests) ----------------------------------------------
+template <class T>
+__global__ __launch_bounds__(PCG_BS) void spmv_kernel(int n, const long* __restrict__ rowptr,
+                                                     const int* __restrict__ bcol,
+                                                     const T* __restrict__ bval,
+                                                     const T* __restrict__ x,
+                                                     T* __restrict__ y) {
+  const int lane = threadIdx.x & (LPR - 1);
+  for (int row = (blockIdx.x * PCG_BS + threadIdx.x) / LPR; row < n;
+       row += gridDim.x * ROWS_PER_BLOCK) {
+    T y0 = 0, y1 = 0, y2 = 0;
+    const long e1 = rowptr[row + 1];
+    for (long s = rowptr[row] + lane; s < e1; s += LPR) {
+      const long j = bcol[s];
+      const T* B = bval + s * 9;
+      const T xj0 = x[3 * j], xj1 = x[3 * j + 1], xj2 = x[3 * j + 2];
+      y0 += B[0] * xj0 + B[1] * xj1 + B[2] * xj2;
+      y1 += B[3] * xj0 + B[4] * xj1 + B[5] * xj2;
+      y2 += B[6] * xj0 + B[7] * xj1 + B[8] * xj2;
+    }
+    y0 = group_sum8(y0);
+    y1 = group_sum8(y1);
+    y2 = group_sum8(y2);
+    if (lane == 0) {
+      y[3L * row] = y0;
+      y[3L * row + 1] = y1;
+      y[3L * row + 2] = y2;
+    }
+  }
+}
+
+void launch_spmv(hipStream_t s, bool fp32, int n, const long* rowptr,
+                 const int* bcol, const void* bval, const void* x, void* y) {
+  const int g = grid_rows(n);
+  if (fp32)
+    hipLaunchKernelGGL(spmv_kernel<float>, dim3(g), dim3(PCG_BS), 0, s, n, rowptr, bcol,
+                       reinterpret_cast<const float*>(bval),
+                       reinterpret_cast<const float*>(x), reinterpret_cast<float*>(y));
+  else
+    hipLaunchKernelGGL(spmv_kernel<double>, dim3(g), dim3(PCG_BS), 0, s, n, rowptr, bcol,
+                       reinterpret_cast<const double*>(bval),
+                       reinterpret_cast<const double*>(x), reinterpret_cast<double*>(y));
+}
+
+// ---- vector helpers -------------------------------------------------------------
+__global__ void cast_kernel(long n, const double* __restrict__ a, float* __restrict__ b) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    b[i] = (float)a[i];
+}
+__global__ void copy64_kernel(long n, const double* __restrict__ a, double* __restrict__ b) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    b[i] = a[i];
+}
+__global__ void axpy_f32_kernel(long n, const float* __restrict__ d, double* __restrict__ x) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    x[i] += (double)d[i];
+}
+__global__ void lincomb64_kernel(long n, double a, const double* x, double b,
+                                 const double* y, double* out) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    out[i] = a * x[i] + b * y[i];
+}
+template <class T>
+__global__ void widen_kernel(long n, const T* __restrict__ a, double* __restrict__ b) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    b[i] = (double)a[i];
+}
+
+static inline int vgrid(long n) {
+  long g = (n + 255) / 256;
+  return (int)(g < 1 ? 1 : (g > 2048 ? 2048 : g));
+}
+
+void launch_cast_f64_to_f32(hipStream_t s, long n, const double* a, float* b) {
+  hipLaunchKernelGGL(cast_kernel, dim3(vgrid(n)), dim3(256), 0, s, n, a, b);
+}
+void launch_axpy_f32_to_f64(hipStream_t s, long n, const float* d, double* x) {
+  hipLaunchKernelGGL(axpy_f32_kernel, dim3(vgrid(n)), dim3(256), 0, s, n, d, x);
+}
+void launch_lincomb64(hipStream_t s, long n, double a, const double* x, double b,
+                      const double* y, double* out) {
+  hipLaunchKernelGGL(lincomb64_kernel, dim3(vgrid(n)), dim3(256), 0, s, n, a, x, b, y, out);
+}
+void launch_widen(hipStream_t s, bool fp32, long n, const void* a, double* b) {
+  if (fp32)
+    hipLaunchKernelGGL(widen_kernel<float>, dim3(vgrid(n)), dim3(256), 0, s, n,
+                       reinterpret_cast<const float*>(a), b);
+  else
+    hipLaunchKernelGGL(copy64_kernel, dim3(vgrid(n)), dim3(256), 0, s, n,
+                       reinterpret_cast<const double*>(a), b);
+}
+void launch_narrow_or_copy(hipStream_t s, bool fp32, long n, const double* a, void* b) {
+  if (fp32)
+    hipLaunchKernelGGL(cast_kernel, dim3(vgrid(n)), dim3(256), 0, s, n, a,
+                       reinterpret_cast<float*>(b));
+  else
+    hipLaunchKernelGGL(copy64_kernel, dim3(vgrid(n)), dim3(256), 0, s, n, a,
+                       reinterpret_cast<double*>(b));
+}
+
+constexpr int DOT_GRID = 512;
+__global__ __launch_bounds__(256) void dot64_kernel(long n, const double* __restrict__ a,
+                                                   const double* __restrict__ b,
+                                                   double* part, double* out,
+                                                   unsigned* ticket) {
+  __shared__ double red[4];
+  __shared__ bool last;
+  double acc = 0.0;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    acc += a[i] * b[i];
+  const double s = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (last_block(ticket, &last)) {
+    const double t = block_sum_array<256>(part, gridDim.x, 1, red);
+    if (threadIdx.x == 0) out[0] = t;
+  }
+}
+
+// partials: PCG_MAXGRID doubles followed by an unsigned ticket
+void launch_dot64(hipStream_t s, long n, const double* a, const double* b,
+                  double* partials, double* out) {
+  unsigned* ticket = reinterpret_cast<unsigned*>(partials + PCG_MAXGRID);
+  hipLaunchKernelGGL(dot64_kernel, dim3(DOT_GRID), dim3(256), 0, s, n, a, b, partials,
+                     out, ticket);
+}
+
+}  // namespace pgn

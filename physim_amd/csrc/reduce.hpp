@@ -1,0 +1,85 @@
+// reduce.hpp -- deterministic workgroup reductions for wave64 (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pgn {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;  // valid in lane 0
+}
+
+// Sum over the workgroup in a fixed order; the result is valid in thread 0.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) r += lds[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Two sums at once (fixed order), valid in thread 0.
+template <int BS>
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* lds) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[w] = a;
+    lds[BS / 64 + w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ra = 0.0, rb = 0.0;
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) {
+      ra += lds[i];
+      rb += lds[BS / 64 + i];
+    }
+    a = ra;
+    b = rb;
+  }
+  __syncthreads();
+}
+
+// Thread 0 of every workgroup calls this after writing its partial result.
+// Returns true in the workgroup that arrives last (for all its threads); that
+// workgroup then sees every other workgroup's partials.  The ticket is reset
+// for the next launch.
+__device__ __forceinline__ bool last_block(unsigned* ticket, bool* flag_lds) {
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    *flag_lds = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  const bool last = *flag_lds;
+  if (last) {
+    __threadfence();
+    if (threadIdx.x == 0) *ticket = 0u;
+  }
+  return last;
+}
+
+// Sum `count` doubles (stride `stride`) in a fixed order by one workgroup;
+// valid in thread 0.
+template <int BS>
+__device__ __forceinline__ double block_sum_array(const double* a, int count,
+                                                  int stride, double* lds) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < count; i += BS)
+    v += __hip_atomic_load(const_cast<double*>(a) + (long)i * stride, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  return block_sum<BS>(v, lds);
+}
+
+}  // namespace pgn

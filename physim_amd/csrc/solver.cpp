@@ -1,0 +1,291 @@
+// solver.cpp -- linear solve (ILinearSolver) and Newton driver (OptimSolver)
+// over a device-resident model.  Host C++ orchestrates; every vector stays in
+// HBM and only scalars (energy, norms, PCG status) come back to the host.
+#include "solver.hpp"
+
+#include <chrono>
+#include <cmath>
+
+namespace pgn {
+
+// Runs PCG in the model's Hessian precision on rhs `pb` with solution /
+// warm start in the PCG x vector.  Returns iterations (Eigen's count).
+static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero,
+                   double& rr, double& bb) {
+  hipStream_t s = M.stream();
+  const PcgArgs a = M.pcg_args(maxit, precond);
+  launch_pcg_init(s, M.hfp32, a, M.pb.p, tol, maxit, x_zero ? 1 : 0);
+  int k = 0, chunk = 8;
+  bool have = false;
+  while (k < maxit) {
+    const int c = std::min(chunk, maxit - k);
+    for (int j = 0; j < c; ++j) launch_pcg_iter(s, M.hfp32, a);
+    k += c;
+    PGN_HIP(hipMemcpyAsync(M.hsc, M.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    M.sync();
+    have = true;
+    if (M.hsc->done) break;
+    chunk = std::min(2 * chunk, M.pcg_chunk_max);
+  }
+  if (!have) {
+    PGN_HIP(hipMemcpyAsync(M.hsc, M.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    M.sync();
+  }
+  rr = M.hsc->rr;
+  bb = M.hsc->bb;
+  return M.hsc->iters;
+}
+
+// One unregularized solve of A x = b (x: FP64 warm start, in/out).
+static void solve_once(pgn_model& M, const double* b, double* x, const pgn_linear_opts& o,
+                       int& iters, int& refinements, double& relres) {
+  hipStream_t s = M.stream();
+  const long N = 3L * M.n;
+  const size_t ts = M.tsize();
+  char* px = M.pv.p;  // PCG solution vector
+  if (!M.hfp32) {
+    // plain FP64: Eigen ConjugateGradient semantics on (b, x) directly
+    PGN_HIP(hipMemcpyAsync(M.pb.p, b, N * ts, hipMemcpyDeviceToDevice, s));
+    PGN_HIP(hipMemcpyAsync(px, x, N * ts, hipMemcpyDeviceToDevice, s));
+    double rr = 0, bb = 0;
+    iters = pcg_run(M, o.tol, o.max_iters, o.precond, false, rr, bb);
+    if (bb == 0.0) {
+      PGN_HIP(hipMemsetAsync(x, 0, N * sizeof(double), s));
+      relres = 0.0;
+    } else {
+      PGN_HIP(hipMemcpyAsync(x, px, N * ts, hipMemcpyDeviceToDevice, s));
+      relres = std::sqrt(rr / bb);
+    }
+    refinements = 0;
+    return;
+  }
+  // mixed: FP32 PCG corrections inside an FP64 iterative refinement
+  const double bb = M.dot(b, b, N);
+  iters = 0;
+  refinements = 0;
+  if (bb == 0.0) {
+    PGN_HIP(hipMemsetAsync(x, 0, N * sizeof(double), s));
+    relres = 0.0;
+    return;
+  }
+  const double thr = o.tol * o.tol * bb;
+  double rr = 0.0;
+  for (;;) {
+    launch_residual64(s, true, M.n, M.rowptr.p, M.bcol.p, M.bval.p, x, b, M.r64.p,
+                      M.red64.p, M.scal.p + 4);
+    PGN_HIP(hipMemcpyAsync(M.hpin + 4, M.scal.p + 4, sizeof(double), hipMemcpyDeviceToHost, s));
+    M.sync();
+    rr = M.hpin[4];
+    if (!(rr >= thr) || iters >= o.max_iters) break;  // also stops on NaN
+    const double inner = std::max(o.inner_tol, o.tol * std::sqrt(bb / rr));
+    launch_cast_f64_to_f32(s, N, M.r64.p, reinterpret_cast<float*>(M.pb.p));
+    double irr = 0, ibb = 0;
+    const int it = pcg_run(M, inner, o.max_iters - iters, o.precond, true, irr, ibb);
+    launch_axpy_f32_to_f64(s, N, reinterpret_cast<const float*>(px), x);
+    iters += it;
+    ++refinements;
+    if (it == 0) break;  // no progress possible in FP32
+  }
+  relres = std::sqrt(rr / bb);
+}
+
+// LinearSolver_EigenCG::SolveInternal (LinearSolver_EigenCG.cpp:51-125):
+// regularize A += regT 10^(i-1) I (cumulative) while b.x < 0.
+void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opts& o,
+                  pgn_linear_result& res) {
+  PGN_CHECK(M.hess_valid, "Hessian not assembled");
+  hipStream_t s = M.stream();
+  const long N = 3L * M.n;
+  double* xprev = M.tmp64b.p;
+  PGN_HIP(hipMemcpyAsync(xprev, x, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  res.status = 1;
+  res.iterations = 0;
+  res.reg_used = 0;
+  res.refinements = 0;
+  res.rel_residual = 0.0;
+  double regT = -1.0;
+  const int regs = std::max(o.reg_iters, 1);
+  for (int i = 0; i < regs; ++i) {
+    if (i != 0) {
+      if (regT < 0.0) regT = 1e-6 * M.frob_lower();  // LinearSolver.cpp:57-66
+      launch_add_diag(s, M.hfp32, M.n, M.diag.p, M.bval.p, regT * std::pow(10.0, i - 1));
+      M.update_precond();
+      res.reg_used = i;
+    }
+    PGN_HIP(hipMemcpyAsync(x, xprev, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    int it = 0, refs = 0;
+    double rel = 0.0;
+    solve_once(M, b, x, o, it, refs, rel);
+    res.iterations += it;
+    res.refinements += refs;
+    res.rel_residual = rel;
+    if (M.dot(b, x, N) < 0.0) continue;  // indefinite (regSign > 0)
+    res.status = 0;
+    break;
+  }
+}
+
+// ---- Newton -----------------------------------------------------------------
+
+void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) {
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = M.stream();
+  const long N = 3L * M.n;
+  M.nw_xsave.alloc(N);
+  M.nw_v0.alloc(N);
+  M.nw_dx.alloc(N);
+  M.nw_b.alloc(N);
+  double* x = M.x.p;
+  double* xsave = M.nw_xsave.p;
+  double* v0 = M.nw_v0.p;
+  double* dx = M.nw_dx.p;
+  double* bvec = M.nw_b.p;
+  pgn_linear_opts lo;
+  pgn_linear_default_opts(&lo);
+  lo.tol = o.lsTolMaxError;
+  lo.max_iters = o.lsNumMaxIters;
+  lo.precond = o.precond;
+  lo.inner_tol = o.inner_tol;
+  R = pgn_newton_result{};
+
+  auto d2d = [&](double* dst, const double* src) {
+    PGN_HIP(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  };
+
+  // OptimSolver::Init (OptimSolver.cpp:36-55)
+  d2d(v0, x);
+  double obj = 0.0, gg = 0.0;
+  M.eval_eg(obj, gg);
+  R.energy_evals++;
+  double opt = std::sqrt(gg);
+  bool g_valid = true;  // M.g holds the gradient at the current x and loads
+  int result = PGN_OS_SUCCESS;
+  int minImpr = 0, minStep = 0;
+  bool steepest = false;
+  int itCount = 0;
+
+  // OptimSolver_USQP_LS::SolveStepInternal (OptimSolver_USQP_LS.cpp:18-68)
+  auto solve_step = [&](bool use_steepest) {
+    M.step_load();                 // OnIterStart: StepBoundaryConditions
+    d2d(xsave, x);                 //              GetState
+    M.eval_eg(obj, gg);            // GetEnergy + GetGradient
+    R.energy_evals++;
+    g_valid = true;
+    opt = std::sqrt(gg);
+    if (opt < o.tolMaxError) {
+      result = PGN_OS_SUCCESS;
+      return;
+    }
+    // ComputeStep (OptimSolver.cpp:313-352); trial 0 always succeeds
+    if (!use_steepest) {
+      M.eval_hessian();
+      R.hessian_evals++;
+      launch_lincomb64(s, N, 1.0, xsave, -1.0, v0, dx);  // dx0 = x - x_prev (:488)
+      launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, bvec);
+      pgn_linear_result lr;
+      linear_solve(M, bvec, dx, lo, lr);
+      R.pcg_iterations += lr.iterations;
+    } else {
+      launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, dx);
+    }
+    double len = std::sqrt(M.dot(dx, dx, N));
+    if (len > o.maxStepSize) {
+      launch_lincomb64(s, N, o.maxStepSize / len, dx, 0.0, dx, dx);
+      len = o.maxStepSize;
+    }
+    // LineSearch_Simple (OptimSolver.cpp:688-736)
+    double factor = 1.0, objNew = 0.0, ggNew = 0.0, stepLen = 0.0, escale = 0.0;
+    int numBis = -1;
+    do {
+      ++numBis;
+      stepLen = factor * len;
+      launch_lincomb64(s, N, 1.0, xsave, factor, dx, x);  // restore + IncVariables
+      M.eval_eg(objNew, ggNew, &escale);
+      R.energy_evals++;
+      g_valid = true;
+      if (objNew < obj) break;
+      if (o.ls_noise_guard > 0.0 && objNew <= obj + o.ls_noise_guard * escale &&
+          ggNew < gg) {
+        // energy change below FP64 rounding: accept on the gradient norm and
+        // book it as the smallest positive improvement
+        objNew = obj - 1e-300;
+        break;
+      }
+      factor *= o.lSearch_alpha;
+      d2d(x, xsave);  // PrePerformStep
+      g_valid = false;
+    } while (stepLen > o.tolMinStep && numBis <= o.lsearch_numTry);
+    R.line_search_bisections += numBis;
+    // UpdateStepResult (OptimSolver.cpp:402-466)
+    const double impr = obj - objNew;
+    if (impr > 0) {
+      const double optNew = std::sqrt(ggNew);
+      if (impr < o.tolMinImpr) {
+        minImpr++;
+        result = PGN_OS_MINIMPR;
+      } else if (stepLen < o.tolMinStep) {
+        minStep++;
+        result = PGN_OS_MINSTEP;
+      } else if (optNew <= o.tolMaxError) {
+        result = PGN_OS_SUCCESS;
+        minImpr = 0;
+        minStep = 0;
+      } else {
+        result = PGN_OS_ONGOING;
+        minImpr = 0;
+        minStep = 0;
+      }
+      d2d(v0, xsave);
+      obj = objNew;
+      opt = optNew;
+    } else {
+      result = PGN_OS_NONDESC;
+    }
+    // OnIterFinish: UpdateKinematics (no kinematic state for tets)
+  };
+
+  // OptimSolver::SolveFull (OptimSolver.cpp:163-264)
+  for (;;) {
+    ++itCount;
+    solve_step(steepest);
+    if (!g_valid) {
+      double e2 = 0.0;
+      M.eval_eg(e2, gg);
+      R.energy_evals++;
+      g_valid = true;
+    }
+    opt = std::sqrt(gg);
+    if (M.is_loaded()) {
+      if (result == PGN_OS_NONDESC) {
+        if (!o.trySteepest) break;
+        if (steepest) break;
+        steepest = true;
+      }
+      if (result == PGN_OS_MINIMPR) {
+        if (minImpr == o.minImprMaxCount) break;
+        if (o.trySteepest) steepest = !steepest;
+      }
+      if (result == PGN_OS_MINSTEP) {
+        if (minStep == o.minStepMaxCount) break;
+        if (o.trySteepest) steepest = !steepest;
+      }
+      if (result == PGN_OS_SUCCESS) {
+        if (opt <= o.tolMaxError) break;
+        result = PGN_OS_ONGOING;
+      }
+      if (result == PGN_OS_ONGOING) steepest = false;
+    }
+    if (itCount + 1 > o.numMaxIters) {
+      result = PGN_OS_MAXITER;
+      break;
+    }
+  }
+  M.sync();
+  R.result = result;
+  R.iterations = itCount;
+  R.objective = obj;
+  R.optimality = opt;
+  R.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace pgn
