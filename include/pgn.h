@@ -210,10 +210,21 @@ int pgn_newton_solve(pgn_model* model, const pgn_newton_opts* opts,
  * context stream, and returns the mean launch time (HIP events) in ms.
  * which: 0 = element E+g+H (model precision), 1 = element E+g (FP64),
  *        2 = BSR Hessian gather+fix, 3 = BSR SpMV (one PCG matrix product),
- *        4 = element E+g+H FP64 (plain path kernel)                        */
+ *        4 = element E+g+H FP64 (plain path kernel), 5 = element E+g+H FP32,
+ *        7 = calibration copy of 256 MiB (FP32)                            */
 int pgn_time_kernel(pgn_model* model, int which, int reps, double* ms_mean);
-/* algorithmic bytes one launch of `which` moves (for roofline accounting) */
+/* algorithmic bytes one launch of `which` moves (for roofline accounting);
+ * also 7 = the calibration copy of pgn_time_kernel, 8 = PCG SpMV launch (K1),
+ * 9 = PCG vector-update launch (K2) */
 int pgn_kernel_bytes(pgn_model* model, int which, double* bytes);
+/* Live timing of the PCG launches inside linear / Newton solves: when on,
+ * every SpMV launch (K1) and vector-update launch (K2) is bracketed by HIP
+ * events on the context stream.  Switching on resets the counters;
+ * switching off keeps them for pgn_model_get_profile. */
+int pgn_model_set_profiling(pgn_model* model, int on);
+/* which: 0 = K1, 1 = K2; launches that did work and their summed ms */
+int pgn_model_get_profile(pgn_model* model, int which, long* count,
+                          double* total_ms);
 
 /* ---- host-only helpers (no GPU needed; used by CPU tests) --------------- */
 /* Builds the BSR3 pattern and the element->block slot map of a tet mesh the

@@ -35,7 +35,8 @@ EXPORTS = [
     "pgn_model_hessian_pattern", "pgn_model_get_hessian",
     "pgn_model_element_egh", "pgn_linear_default_opts", "pgn_linear_solve",
     "pgn_newton_default_opts", "pgn_newton_solve", "pgn_time_kernel",
-    "pgn_kernel_bytes", "pgn_host_bsr_pattern",
+    "pgn_kernel_bytes", "pgn_host_bsr_pattern", "pgn_model_set_profiling",
+    "pgn_model_get_profile",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -130,6 +131,8 @@ def lib():
         L.pgn_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
         L.pgn_kernel_bytes.argtypes = [_vp, ctypes.c_int, _dp]
         L.pgn_host_bsr_pattern.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _lp, _lp, _ip]
+        L.pgn_model_set_profiling.argtypes = [_vp, ctypes.c_int]
+        L.pgn_model_get_profile.argtypes = [_vp, ctypes.c_int, _lp, _dp]
         _LIB = L
     return _LIB
 
@@ -304,6 +307,16 @@ class TetModel:
         ms = ctypes.c_double()
         _ok(lib().pgn_time_kernel(self._h, which, reps, ctypes.byref(ms)))
         return ms.value
+
+    def set_profiling(self, on=True):
+        _ok(lib().pgn_model_set_profiling(self._h, 1 if on else 0))
+
+    def profile(self, which):
+        """(launches, total ms) of PCG kernel `which` (0 = SpMV, 1 = update)."""
+        c = ctypes.c_long()
+        ms = ctypes.c_double()
+        _ok(lib().pgn_model_get_profile(self._h, which, ctypes.byref(c), ctypes.byref(ms)))
+        return c.value, ms.value
 
     def kernel_bytes(self, which):
         b = ctypes.c_double()

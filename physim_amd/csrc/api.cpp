@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <new>
 
 #include "solver.hpp"
@@ -444,6 +445,26 @@ int pgn_newton_solve(pgn_model* M, const pgn_newton_opts* opts, pgn_newton_resul
   });
 }
 
+int pgn_model_set_profiling(pgn_model* M, int on) {
+  return guard([&] {
+    PGN_CHECK(M, "null model");
+    M->profiling = on != 0;
+    if (on) {
+      M->prof_n[0] = M->prof_n[1] = 0;
+      M->prof_ms[0] = M->prof_ms[1] = 0.0;
+    }
+  });
+}
+
+int pgn_model_get_profile(pgn_model* M, int which, long* count, double* total_ms) {
+  return guard([&] {
+    PGN_CHECK(M && count && total_ms, "null argument");
+    PGN_CHECK(which == 0 || which == 1, "unknown profile slot");
+    *count = M->prof_n[which];
+    *total_ms = M->prof_ms[which];
+  });
+}
+
 int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
   return guard([&] {
     PGN_CHECK(M && bytes, "null argument");
@@ -469,6 +490,16 @@ int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
         break;
       case 5:  // E+g+H FP32
         *bytes = m * (17 + 48 + 48 + 312) + n * 48;
+        break;
+      case 7:  // calibration copy: 256 MiB read + 256 MiB written
+        *bytes = 2.0 * 256.0 * 1024 * 1024;
+        break;
+      case 8:  // PCG K1: SpMV values + block cols + row pointers, z and
+               // p_{k-1} read, p_k and q written
+        *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 4 * 3 * n * hs;
+        break;
+      case 9:  // PCG K2: p, q, x, r, dinv read; x, r, z written
+        *bytes = 3 * n * hs * 8;
         break;
       default:
         throw Error(PGN_ERR_INVALID, "unknown kernel id");
@@ -541,6 +572,13 @@ int pgn_time_kernel(pgn_model* M, int which, int reps, double* ms_mean) {
           launch_tet_eval(s, true, EVAL_E | EVAL_G | EVAL_H, M->tet_args(true), M->blockE.p,
                           nullptr, eg, eH);
         };
+        break;
+      }
+      case 7: {
+        const long nf = 64L * 1024 * 1024;  // 256 MiB of floats
+        auto buf = std::make_shared<DBuf<float>>();
+        buf->alloc(2 * nf);
+        launch = [=] { launch_copy_f32(s, nf, buf->p, buf->p + nf); };
         break;
       }
       default:

@@ -125,6 +125,11 @@ void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b,
 // one PCG iteration (two kernels; the index is read from sc->it; no-ops once
 // converged, so a chunk of these can be captured and replayed)
 void launch_pcg_iter(hipStream_t s, bool fp32, const PcgArgs& a);
+// the two launches of one iteration, separately (for per-kernel timing)
+void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a);
+void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a);
+// plain device copy of n floats (bandwidth calibration)
+void launch_copy_f32(hipStream_t s, long n, const float* a, float* b);
 
 // r64 = b64 - A x64 with A in precision T and FP64 accumulation; writes
 // |r|^2 into out[0].  partials: pcg_blocks() doubles + one unsigned ticket.

@@ -95,6 +95,7 @@ void build_bsr_pattern(int n, int m, const int* T, BsrPattern& P) {
 using namespace pgn;
 
 pgn_model::~pgn_model() {
+  for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
   if (pcg_graph) (void)hipGraphExecDestroy(pcg_graph);
   if (hpin) (void)hipHostFree(hpin);
   if (hsc) (void)hipHostFree(hsc);

@@ -78,6 +78,11 @@ struct pgn_model {
   hipGraphExec_t pcg_graph = nullptr;
   int pcg_graph_len = 0;
   bool use_graphs = true;
+  // live per-kernel timing of the PCG launches (HIP events on the stream)
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_ev;
+  long prof_n[2] = {0, 0};     // K1 (SpMV), K2 (update + precond)
+  double prof_ms[2] = {0, 0};
 
   ~pgn_model();
   void setup(int n_nodes, const double* X0h, int n_tets, const int* Th,

@@ -263,15 +263,25 @@ void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b,
                        reinterpret_cast<const double*>(b), tol, maxit, x_zero);
 }
 
-void launch_pcg_iter(hipStream_t s, bool fp32, const PcgArgs& a) {
-  const int g1 = grid_rows(a.n), g2 = grid_nodes(a.n);
-  if (fp32) {
+void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a) {
+  const int g1 = grid_rows(a.n);
+  if (fp32)
     hipLaunchKernelGGL(pcg_k1_kernel<float>, dim3(g1), dim3(PCG_BS), 0, s, a);
-    hipLaunchKernelGGL(pcg_k2_kernel<float>, dim3(g2), dim3(PCG_BS), 0, s, a);
-  } else {
+  else
     hipLaunchKernelGGL(pcg_k1_kernel<double>, dim3(g1), dim3(PCG_BS), 0, s, a);
+}
+
+void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a) {
+  const int g2 = grid_nodes(a.n);
+  if (fp32)
+    hipLaunchKernelGGL(pcg_k2_kernel<float>, dim3(g2), dim3(PCG_BS), 0, s, a);
+  else
     hipLaunchKernelGGL(pcg_k2_kernel<double>, dim3(g2), dim3(PCG_BS), 0, s, a);
-  }
+}
+
+void launch_pcg_iter(hipStream_t s, bool fp32, const PcgArgs& a) {
+  launch_pcg_k1(s, fp32, a);
+  launch_pcg_k2(s, fp32, a);
 }
 
 // ---- FP64 residual with a precision-T matrix --------------------------------
@@ -402,9 +412,19 @@ __global__ void widen_kernel(long n, const T* __restrict__ a, double* __restrict
     b[i] = (double)a[i];
 }
 
+__global__ void copy_f32_kernel(long n4, const float4* __restrict__ a, float4* __restrict__ b) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n4; i += (long)gridDim.x * 256)
+    b[i] = a[i];
+}
+
 static inline int vgrid(long n) {
   long g = (n + 255) / 256;
   return (int)(g < 1 ? 1 : (g > 2048 ? 2048 : g));
+}
+
+void launch_copy_f32(hipStream_t s, long n, const float* a, float* b) {
+  hipLaunchKernelGGL(copy_f32_kernel, dim3(8192), dim3(256), 0, s, n / 4,
+                     reinterpret_cast<const float4*>(a), reinterpret_cast<float4*>(b));
 }
 
 void launch_cast_f64_to_f32(hipStream_t s, long n, const double* a, float* b) {

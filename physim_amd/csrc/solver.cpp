@@ -15,15 +15,46 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
   hipStream_t s = M.stream();
   const PcgArgs a = M.pcg_args(maxit, precond);
   launch_pcg_init(s, M.hfp32, a, M.pb.p, tol, maxit, x_zero ? 1 : 0);
-  int k = 0, chunk = 8;
+  int k = 0, chunk = 8, done_its = 0;
   bool have = false;
+  if (M.profiling && (int)M.prof_ev.size() < 3 * M.pcg_chunk_max) {
+    while ((int)M.prof_ev.size() < 3 * M.pcg_chunk_max) {
+      hipEvent_t e;
+      PGN_HIP(hipEventCreate(&e));
+      M.prof_ev.push_back(e);
+    }
+  }
   while (k < maxit) {
     const int c = std::min(chunk, maxit - k);
-    for (int j = 0; j < c; ++j) launch_pcg_iter(s, M.hfp32, a);
+    if (M.profiling) {
+      for (int j = 0; j < c; ++j) {
+        PGN_HIP(hipEventRecord(M.prof_ev[3 * j], s));
+        launch_pcg_k1(s, M.hfp32, a);
+        PGN_HIP(hipEventRecord(M.prof_ev[3 * j + 1], s));
+        launch_pcg_k2(s, M.hfp32, a);
+        PGN_HIP(hipEventRecord(M.prof_ev[3 * j + 2], s));
+      }
+    } else {
+      for (int j = 0; j < c; ++j) launch_pcg_iter(s, M.hfp32, a);
+    }
     k += c;
     PGN_HIP(hipMemcpyAsync(M.hsc, M.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     M.sync();
     have = true;
+    if (M.profiling) {
+      // launches after convergence return at once: time only the live ones
+      const int live = std::min(c, M.hsc->it - done_its);
+      for (int j = 0; j < live; ++j) {
+        float t1 = 0.f, t2 = 0.f;
+        PGN_HIP(hipEventElapsedTime(&t1, M.prof_ev[3 * j], M.prof_ev[3 * j + 1]));
+        PGN_HIP(hipEventElapsedTime(&t2, M.prof_ev[3 * j + 1], M.prof_ev[3 * j + 2]));
+        M.prof_ms[0] += t1;
+        M.prof_ms[1] += t2;
+      }
+      M.prof_n[0] += live;
+      M.prof_n[1] += live;
+      done_its = M.hsc->it;
+    }
     if (M.hsc->done) break;
     chunk = std::min(2 * chunk, M.pcg_chunk_max);
   }
