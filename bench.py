@@ -78,7 +78,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--precond", type=int, default=0)
+    ap.add_argument("--precond", type=int, default=1,
+                    help="0 Jacobi, 1 3x3 block Jacobi (default)")
+    ap.add_argument("--adaptive", type=int, default=1,
+                    help="1 (default): Eisenstat-Walker linear tolerance")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also time one solve per (precond, adaptive) variant")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and "
@@ -92,7 +97,7 @@ def main():
     t0 = time.perf_counter()
     scene = scenes.config2()
     ctx = pg.Context(0)
-    opts = dict(pg.CONVERGE_OPTS, precond=args.precond)
+    opts = dict(pg.CONVERGE_OPTS, precond=args.precond, adaptive_tol=args.adaptive)
     M = pg.TetModel(ctx, scene.X0, scene.T, scene.material, scene.lame1, scene.lame2,
                     scene.density, pg.MIXED)
     scene.apply(M)
@@ -103,6 +108,16 @@ def main():
         M.reset()
         r = M.newton(**opts)
         log(f"warmup {w}: {r}")
+
+    sweep = []
+    if args.sweep:
+        for pc in (0, 1):
+            for ad in (0, 1):
+                M.reset()
+                r = M.newton(**dict(pg.CONVERGE_OPTS, precond=pc, adaptive_tol=ad))
+                r.update(precond=pc, adaptive=ad, err_vs_timed_variant=None)
+                sweep.append((r, M.get_x()))
+                log(f"sweep precond={pc} adaptive={ad}: {r}")
 
     M.set_profiling(True)
     results = []
@@ -144,6 +159,12 @@ def main():
         tf = time.perf_counter() - ts
         err = float(np.abs(F.get_x() - x_mixed).max())
         egh64_ms = F.time_kernel(4, 10)
+        # the same FP64 path with the mixed path's solver options, to separate
+        # the effect of precision from that of the solver choices
+        F.reset()
+        ts = time.perf_counter()
+        rs = F.newton(**opts)
+        ts = time.perf_counter() - ts
         fp64 = {"fp64_solve_s": tf, "fp64_result": rf["result"],
                 "fp64_newton_iterations": rf["iterations"],
                 "fp64_pcg_iterations": rf["pcg_iterations"],
@@ -152,7 +173,9 @@ def main():
                 "matches_fp64": bool(err <= 1e-6 * scene.bbox_diag()),
                 "speedup_vs_fp64_solve": tf / (elapsed / args.steps),
                 "egh_fp64_evals_per_s": scene.m / (egh64_ms * 1e-3),
-                "egh_speedup_vs_fp64": egh64_ms / egh_ms}
+                "egh_speedup_vs_fp64": egh64_ms / egh_ms,
+                "fp64_same_opts_solve_s": ts,
+                "speedup_vs_fp64_same_opts": ts / (elapsed / args.steps)}
         ok = ok and fp64["matches_fp64"]
         log(f"fp64 plain path: {rf} in {tf:.2f}s, |dx| {err:.2e}")
         F.close()
@@ -181,7 +204,8 @@ def main():
         "config": {"workload": "beam_40x40x160_conh_lognormal: 1,280,000 tets, "
                                "811,923 DoF, 10 load stages, Newton to |g|<=1e-8",
                    "precision": "mixed", "precond": "jacobi" if args.precond == 0
-                   else "block-jacobi"},
+                   else "block-jacobi",
+                   "linear_tol": "eisenstat-walker" if args.adaptive else "1e-9"},
         "converged": ok,
         "newton_iterations": int(np.mean([r["iterations"] for r in results])),
         "pcg_iterations": int(np.mean([r["pcg_iterations"] for r in results])),
@@ -191,6 +215,7 @@ def main():
         "egh_gbs": egh_bytes / (egh_ms * 1e-3) / 1e9,
         "copy_gbs": copy_gbs,
         **fp64,
+        "pcg_k2_avg_us": k2_ms / max(k2_n, 1) * 1e3,
         "roofline": {"bound": "hbm", "kernel": "pcg_k1_kernel<float> (BSR3 SpMV)",
                      "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -198,7 +223,15 @@ def main():
                      "traffic": None, "launches": k1_n, "avg_us": spmv_ms * 1e3,
                      "bytes_per_launch": spmv_bytes},
         "cpu_baseline": cpu,
+        "spmv_autotune_us": {str(l): M.profile(2 + c)[1] for c, l in enumerate((4, 8, 16))},
+        "spmv_lanes_per_row": M.profile(2)[0],
     }
+    if sweep:
+        line["sweep"] = [dict(precond=r["precond"], adaptive=r["adaptive"],
+                              result=r["result"], seconds=r["seconds"],
+                              newton=r["iterations"], pcg=r["pcg_iterations"],
+                              max_dx_vs_timed=float(np.abs(xs - x_mixed).max()))
+                         for r, xs in sweep]
     print(json.dumps(line), flush=True)
     M.close()
     ctx.close()

@@ -168,7 +168,9 @@ typedef struct {
   /* MI355X path controls (no reference counterpart) */
   int precond;        /* 0 = Jacobi, 1 = 3x3 block Jacobi                  */
   double inner_tol;   /* MIXED: FP32 inner tolerance per refinement sweep */
-  int adaptive_tol;   /* reserved, must be 0                               */
+  int adaptive_tol;   /* 0: every linear solve to lsTolMaxError (reference).
+                         1: Eisenstat-Walker forcing (choice 2, gamma 0.9,
+                         alpha 2, capped at 0.1, floored at lsTolMaxError)  */
   double ls_noise_guard; /* 0 = reference line search.  > 0: also accept a
                          step whose energy rises by at most ls_noise_guard *
                          (|E_elastic| + |E_load|), i.e. within FP64 rounding
@@ -222,7 +224,9 @@ int pgn_kernel_bytes(pgn_model* model, int which, double* bytes);
  * events on the context stream.  Switching on resets the counters;
  * switching off keeps them for pgn_model_get_profile. */
 int pgn_model_set_profiling(pgn_model* model, int on);
-/* which: 0 = K1, 1 = K2; launches that did work and their summed ms */
+/* which: 0 = K1, 1 = K2: launches that did work and their summed ms;
+ * 2, 3, 4: the autotuner's SpMV time (us, in total_ms) with 4, 8, 16 lanes per
+ * block row, and in count the lanes per row it chose */
 int pgn_model_get_profile(pgn_model* model, int which, long* count,
                           double* total_ms);
 

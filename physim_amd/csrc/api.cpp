@@ -459,9 +459,14 @@ int pgn_model_set_profiling(pgn_model* M, int on) {
 int pgn_model_get_profile(pgn_model* M, int which, long* count, double* total_ms) {
   return guard([&] {
     PGN_CHECK(M && count && total_ms, "null argument");
-    PGN_CHECK(which == 0 || which == 1, "unknown profile slot");
-    *count = M->prof_n[which];
-    *total_ms = M->prof_ms[which];
+    PGN_CHECK(which >= 0 && which <= 4, "unknown profile slot");
+    if (which <= 1) {
+      *count = M->prof_n[which];
+      *total_ms = M->prof_ms[which];
+    } else {  // autotuner: SpMV us with 4 / 8 / 16 lanes per row; count = choice
+      *count = M->pcg_lpr;
+      *total_ms = M->tune_us[which - 2];
+    }
   });
 }
 
@@ -553,7 +558,7 @@ int pgn_time_kernel(pgn_model* M, int which, int reps, double* ms_mean) {
         char* base = M->pv.p;
         launch = [=] {
           launch_spmv(s, M->hfp32, M->n, M->rowptr.p, M->bcol.p, M->bval.p, base + 4 * v,
-                      base + 5 * v);
+                      base + 5 * v, M->pcg_lpr);
         };
         break;
       }

@@ -282,10 +282,10 @@ __global__ __launch_bounds__(256) void frob_kernel(long nb, const int* __restric
 void launch_frob_lower(hipStream_t s, bool fp32, long nb, const int* brow,
                        const int* bcol, const void* bval, double* partials,
                        double* out) {
-  // partials holds pcg_blocks() + 4 doubles; this kernel's ticket sits past
+  // partials holds PCG_MAXP + 4 doubles; this kernel's ticket sits past
   // the ticket of the residual / dot kernels (zeroed at allocation and reset
   // by the last workgroup).
-  unsigned* ticket = reinterpret_cast<unsigned*>(partials + pcg_blocks() + 2);
+  unsigned* ticket = reinterpret_cast<unsigned*>(partials + PCG_MAXP + 2);
   if (fp32)
     hipLaunchKernelGGL(frob_kernel<float>, dim3(FROB_GRID), dim3(256), 0, s, nb, brow,
                        bcol, reinterpret_cast<const float*>(bval), partials, out, ticket);

@@ -77,26 +77,25 @@ void launch_precond(hipStream_t s, bool fp32, int n, const long* diag,
 void launch_add_diag(hipStream_t s, bool fp32, int n, const long* diag,
                      void* bval, double shift);
 // Frobenius norm^2 of the reference's stored lower entry set, reduced into
-// out[0] (partials: pcg_blocks() + 4 doubles, shared with residual / dot).
+// out[0] (partials: PCG_MAXP + 4 doubles, shared with residual / dot).
 int frob_blocks();
 void launch_frob_lower(hipStream_t s, bool fp32, long nb, const int* brow,
                        const int* bcol, const void* bval, double* partials,
                        double* out);
 
 // ---- PCG ------------------------------------------------------------------
+// Reductions are fence-free: each producer workgroup writes a partial sum,
+// and every consumer workgroup of the next launch re-sums the partials in a
+// fixed order (identical bits everywhere; kernel boundaries give coherence).
+constexpr int PCG_MAXP = 1024;  // max producer workgroups of one partial set
+
 struct PcgScalars {
-  double rz[2];
-  double pq;
-  double rr;
-  double bb;
-  double thr;
-  double rz0;
+  double thr;   // max(tol^2 |b|^2, DBL_MIN)
+  double bb;    // |b|^2
+  double rr;    // |r|^2 at the last check
   int done;
-  int iters;
-  int it;     // iteration index, advanced by the last workgroup of K2
-  int maxit;  // iteration limit of the current solve
-  unsigned ticket1;
-  unsigned ticket2;
+  int iters;    // Eigen's iteration count once done
+  int pad[2];
 };
 
 struct PcgArgs {
@@ -107,39 +106,41 @@ struct PcgArgs {
   const void* dinv;      // Jacobi (3n)
   const void* binv;      // block Jacobi (9n)
   int precond;           // 0 Jacobi, 1 block Jacobi
+  int lpr;               // lanes per block row in the SpMV (4, 8 or 16)
+  int g1, g2;            // workgroups of K1/init and of K2
+  double tol;
   void* x;
   void* r;
   void* z;
   void* p0;
   void* p1;
   void* q;
-  double* part1;         // pcg_blocks() doubles
-  double* part2;         // 3 * pcg_blocks() doubles
+  double* part1;         // PCG_MAXP: p.q of K1
+  double* part2;         // 2 parities x PCG_MAXP x {r.r, r.z}
+  double* partb;         // PCG_MAXP: |b|^2 of init
   PcgScalars* sc;
 };
-int pcg_blocks();
-// r = b - A x (b in T); z = M^-1 r; p = 0; scalars: bb, thr(tol), rz0, rr,
-// it = 0, maxit, done.  x_zero: start from x = 0 (x is zeroed, no product).
-void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b,
-                     double tol, int maxit, int x_zero);
-// one PCG iteration (two kernels; the index is read from sc->it; no-ops once
-// converged, so a chunk of these can be captured and replayed)
-void launch_pcg_iter(hipStream_t s, bool fp32, const PcgArgs& a);
-// the two launches of one iteration, separately (for per-kernel timing)
-void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a);
-void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a);
+int pcg_grid1(int n, int lpr);
+int pcg_grid2(int n);
+// r = b - A x (b in T); z = M^-1 r; p = 0.  x_zero: start from x = 0.
+void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, int x_zero);
+// iteration k: K1 (beta, p_k, q = A p_k, p.q) and K2 (alpha, x, r, z, r.r, r.z)
+void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a, int k);
+void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k);
+// after iterations [0, k_next): convergence test, sets done / iters / rr
+void launch_pcg_status(hipStream_t s, const PcgArgs& a, int k_next, int maxit);
 // plain device copy of n floats (bandwidth calibration)
 void launch_copy_f32(hipStream_t s, long n, const float* a, float* b);
 
 // r64 = b64 - A x64 with A in precision T and FP64 accumulation; writes
-// |r|^2 into out[0].  partials: pcg_blocks() doubles + one unsigned ticket.
+// |r|^2 into out[0].  partials: PCG_MAXP doubles + one unsigned ticket.
 void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
                        const int* bcol, const void* bval, const double* x,
                        const double* b, double* r, double* partials,
                        double* out);
-// y = A x in precision T (for timing and tests)
+// y = A x in precision T with `lpr` lanes per block row (tuning, timing)
 void launch_spmv(hipStream_t s, bool fp32, int n, const long* rowptr,
-                 const int* bcol, const void* bval, const void* x, void* y);
+                 const int* bcol, const void* bval, const void* x, void* y, int lpr);
 
 // small vector helpers
 void launch_cast_f64_to_f32(hipStream_t s, long n, const double* a, float* b);
@@ -147,7 +148,7 @@ void launch_axpy_f32_to_f64(hipStream_t s, long n, const float* d, double* x);
 // out = a*x + b*y in FP64 (out may alias x or y)
 void launch_lincomb64(hipStream_t s, long n, double a, const double* x, double b,
                       const double* y, double* out);
-// out[0] = dot(a, b) (FP64); partials: pcg_blocks() doubles + unsigned ticket
+// out[0] = dot(a, b) (FP64); partials: PCG_MAXP doubles + unsigned ticket
 void launch_dot64(hipStream_t s, long n, const double* a, const double* b,
                   double* partials, double* out);
 // cast a T vector to FP64

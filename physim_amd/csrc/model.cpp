@@ -188,10 +188,10 @@ void pgn_model::setup(int n_nodes, const double* X0h, int n_tets, const int* Th,
   g.alloc(3L * n);
   pv.alloc(6 * 3L * n * ts);
   pb.alloc(3L * n * ts);
-  const int PB = pcg_blocks();
-  part1.alloc(PB + 8);
-  part2.alloc(3L * PB + 8);
-  red64.alloc(PB + 8);
+  part1.alloc(PCG_MAXP);
+  part2.alloc(4L * PCG_MAXP);
+  partb.alloc(PCG_MAXP);
+  red64.alloc(PCG_MAXP + 8);
   sc.alloc(1);
   r64.alloc(3L * n);
   tmp64a.alloc(3L * n);
@@ -279,8 +279,13 @@ BsrArgs pgn_model::bsr_args() const {
   return a;
 }
 
-PcgArgs pgn_model::pcg_args(int /*maxit*/, int precond) const {
+PcgArgs pgn_model::pcg_args(double tol, int precond) const {
   PcgArgs a;
+  a.lpr = pcg_lpr;
+  a.g1 = pcg_grid1(n, pcg_lpr);
+  a.g2 = pcg_grid2(n);
+  a.tol = tol;
+  a.partb = partb.p;
   const size_t v = 3L * n * tsize();
   char* base = pv.p;
   a.n = n;
@@ -345,4 +350,37 @@ double pgn_model::dot(const double* a, const double* b, long len) {
   PGN_HIP(hipMemcpyAsync(hpin + 3, scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, stream()));
   sync();
   return hpin[3];
+}
+
+// Autotuner: time the BSR SpMV with 4, 8 and 16 lanes per block row on this
+// model's own matrix (assembled or not, the access pattern is the same) and
+// keep the fastest for the PCG.
+void pgn_model::tune() {
+  if (tuned) return;
+  hipStream_t s = stream();
+  const size_t v = 3L * n * tsize();
+  char* base = pv.p;
+  const int lprs[3] = {4, 8, 16};
+  hipEvent_t e0, e1;
+  PGN_HIP(hipEventCreate(&e0));
+  PGN_HIP(hipEventCreate(&e1));
+  double best = 1e30;
+  for (int c = 0; c < 3; ++c) {
+    launch_spmv(s, hfp32, n, rowptr.p, bcol.p, bval.p, base + 4 * v, base + 5 * v, lprs[c]);
+    PGN_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < 5; ++r)
+      launch_spmv(s, hfp32, n, rowptr.p, bcol.p, bval.p, base + 4 * v, base + 5 * v, lprs[c]);
+    PGN_HIP(hipEventRecord(e1, s));
+    PGN_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PGN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    tune_us[c] = 1e3 * ms / 5;
+    if (tune_us[c] < best) {
+      best = tune_us[c];
+      pcg_lpr = lprs[c];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  tuned = true;
 }

@@ -67,7 +67,10 @@ struct pgn_model {
   // PCG work space (Hessian precision) + FP64 refinement vectors
   pgn::DBuf<char> pv;     // 6 vectors of 3n: x r z p0 p1 q
   pgn::DBuf<char> pb;     // rhs in Hessian precision
-  pgn::DBuf<double> part1, part2, red64;
+  pgn::DBuf<double> part1, part2, partb, red64;
+  int pcg_lpr = 8;  // lanes per block row of the PCG SpMV (tuned per model)
+  bool tuned = false;
+  double tune_us[3] = {0, 0, 0};  // SpMV time for 4, 8, 16 lanes per row
   pgn::DBuf<pgn::PcgScalars> sc;
   pgn::DBuf<double> r64, tmp64a, tmp64b;
   pgn::DBuf<double> nw_xsave, nw_v0, nw_dx, nw_b;  // Newton work vectors
@@ -100,7 +103,7 @@ struct pgn_model {
   pgn::TetArgs tet_args(bool fp32) const;
   pgn::NodeArgs node_args() const;
   pgn::BsrArgs bsr_args() const;
-  pgn::PcgArgs pcg_args(int maxit, int precond) const;
+  pgn::PcgArgs pcg_args(double tol, int precond) const;
 
   // E+g (FP64) at the current x; returns E, |g|^2 and the energy scale
   // |E_elastic| + |E_load| (synchronizes)
@@ -113,5 +116,7 @@ struct pgn_model {
   // Frobenius norm of the stored lower entry set (synchronizes)
   double frob_lower();
   double dot(const double* a, const double* b, long n);  // synchronizes
+  // picks the SpMV lanes-per-row of this matrix by timing (once per model)
+  void tune();
   void sync();
 };

@@ -3,44 +3,53 @@
 // The algorithm is the textbook Jacobi-PCG that Eigen's ConjugateGradient
 // implements (LinearSolver_EigenCG.cpp:89 solveWithGuess; Saad, Iterative
 // Methods, Alg. 9.1) with Eigen's stopping rule |r|^2 < max(tol^2 |b|^2,
-// DBL_MIN).  One iteration is two launches:
+// DBL_MIN) and its iteration count.  One iteration k is two launches:
 //   K1: beta = rz_k / rz_{k-1};  p_k = z_k + beta p_{k-1} (recomputed per
 //       column on the fly, written for the own rows into the other p buffer);
-//       q = A p_k;  partial p.q
+//       q = A p_k;  partial p.q.  Also the convergence test of iteration k-1.
 //   K2: alpha = rz_k / p.q;  x += alpha p;  r -= alpha q;  z = M^-1 r;
-//       partial r.r, r.z; convergence test
-// Reductions are per-workgroup partials summed in a fixed order by the last
-// workgroup to arrive (ticket counter), so every run gives identical bits.
-// The iteration index and the done flag live in device memory: every launch
-// of a chunk has the same arguments, so a chunk can be captured in a hipGraph
-// and replayed; once converged all further launches return immediately.
+//       partial r.r, r.z
+// No device-scope fences or atomics: every producer workgroup writes one
+// partial, and each consumer workgroup of the next launch re-sums the
+// partials in a fixed order (the same bits in every workgroup; the kernel
+// boundary makes them visible).  r.r / r.z partials are double-buffered by the
+// parity of the iteration that produced them; init counts as iteration -1.
 #include "kernels.hpp"
 #include "reduce.hpp"
 
 namespace pgn {
 
+constexpr int K1_BS = 512;   // SpMV workgroup
+constexpr int K2_BS = 256;   // vector-update workgroup
+constexpr int K2_MAXGRID = 512;
+constexpr int LPR = 8;       // lanes per row of the plain SpMV / residual kernels
 constexpr int PCG_BS = 256;
-constexpr int LPR = 8;                    // lanes per block row in SpMV
 constexpr int ROWS_PER_BLOCK = PCG_BS / LPR;
-constexpr int PCG_MAXGRID = 1024;
 
-int pcg_blocks() { return PCG_MAXGRID; }
-
+int pcg_grid1(int n, int lpr) {
+  const int rows = K1_BS / lpr;
+  int g = (n + rows - 1) / rows;
+  return g < 1 ? 1 : (g > PCG_MAXP ? PCG_MAXP : g);
+}
+int pcg_grid2(int n) {
+  int g = (n + K2_BS - 1) / K2_BS;
+  return g < 1 ? 1 : (g > K2_MAXGRID ? K2_MAXGRID : g);
+}
 static inline int grid_rows(int n) {
   int g = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
-  return g < 1 ? 1 : (g > PCG_MAXGRID ? PCG_MAXGRID : g);
+  return g < 1 ? 1 : (g > PCG_MAXP ? PCG_MAXP : g);
 }
-static inline int grid_nodes(int n) {
-  int g = (n + PCG_BS - 1) / PCG_BS;
-  return g < 1 ? 1 : (g > PCG_MAXGRID ? PCG_MAXGRID : g);
+
+template <int L, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
 template <class T>
 __device__ __forceinline__ T group_sum8(T v) {
-  v += __shfl_xor(v, 4, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 1, 64);
-  return v;
+  return group_sum<8>(v);
 }
 
 template <class T>
@@ -57,26 +66,47 @@ __device__ __forceinline__ void apply_precond(int precond, const T* __restrict__
   }
 }
 
-// ---- init: r = b - A x, z = M^-1 r, p0 = p1 = 0 --------------------------------
-template <class T>
-__global__ __launch_bounds__(PCG_BS) void pcg_init_kernel(PcgArgs A, const T* __restrict__ b,
-                                                          double tol, int maxit, int x_zero) {
-  __shared__ double red[3 * PCG_BS / 64];
-  __shared__ bool last;
+// Fixed-order sum of `cnt` partials (stride `stride`) by a whole workgroup;
+// the result is returned to every thread.
+template <int BS>
+__device__ __forceinline__ double all_sum(const double* __restrict__ a, int cnt, int stride,
+                                          double* lds) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < cnt; i += BS) v += a[(long)i * stride];
+  v = block_sum<BS>(v, lds);
+  if (threadIdx.x == 0) lds[BS / 64] = v;
+  __syncthreads();
+  const double r = lds[BS / 64];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ const double* rr_slot(const PcgArgs& A, int k) {
+  return A.part2 + (long)(k & 1) * 2 * PCG_MAXP;  // iteration k's {r.r, r.z}
+}
+__device__ __forceinline__ int rr_count(const PcgArgs& A, int k) {
+  return k < 0 ? A.g1 : A.g2;  // init (k = -1) ran on the K1 grid
+}
+
+// ---- init: r = b - A x, z = M^-1 r, p0 = p1 = 0 ------------------------------
+template <class T, int L>
+__global__ __launch_bounds__(K1_BS) void pcg_init_kernel(PcgArgs A, const T* __restrict__ b,
+                                                         int x_zero) {
+  __shared__ double red[3 * K1_BS / 64];
   const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
   T* __restrict__ x = reinterpret_cast<T*>(A.x);
   T* __restrict__ r = reinterpret_cast<T*>(A.r);
   T* __restrict__ z = reinterpret_cast<T*>(A.z);
   T* __restrict__ p0 = reinterpret_cast<T*>(A.p0);
   T* __restrict__ p1 = reinterpret_cast<T*>(A.p1);
-  const int lane = threadIdx.x & (LPR - 1);
+  const int lane = threadIdx.x & (L - 1);
   double bb = 0.0, rr = 0.0, rz = 0.0;
-  for (int row = (blockIdx.x * PCG_BS + threadIdx.x) / LPR; row < A.n;
-       row += gridDim.x * ROWS_PER_BLOCK) {
+  for (int row = (blockIdx.x * K1_BS + threadIdx.x) / L; row < A.n;
+       row += gridDim.x * (K1_BS / L)) {
     T y0 = 0, y1 = 0, y2 = 0;
     if (!x_zero) {
       const long e1 = A.rowptr[row + 1];
-      for (long s = A.rowptr[row] + lane; s < e1; s += LPR) {
+      for (long s = A.rowptr[row] + lane; s < e1; s += L) {
         const long j = A.bcol[s];
         const T* B = bval + s * 9;
         const T xj0 = x[3 * j], xj1 = x[3 * j + 1], xj2 = x[3 * j + 2];
@@ -84,9 +114,9 @@ __global__ __launch_bounds__(PCG_BS) void pcg_init_kernel(PcgArgs A, const T* __
         y1 += B[3] * xj0 + B[4] * xj1 + B[5] * xj2;
         y2 += B[6] * xj0 + B[7] * xj1 + B[8] * xj2;
       }
-      y0 = group_sum8(y0);
-      y1 = group_sum8(y1);
-      y2 = group_sum8(y2);
+      y0 = group_sum<L>(y0);
+      y1 = group_sum<L>(y1);
+      y2 = group_sum<L>(y2);
     }
     if (lane == 0) {
       const long i = row;
@@ -107,61 +137,74 @@ __global__ __launch_bounds__(PCG_BS) void pcg_init_kernel(PcgArgs A, const T* __
       }
     }
   }
-  const double sbb = block_sum<PCG_BS>(bb, red);
-  double srr = rr, srz = rz;
-  block_sum2<PCG_BS>(srr, srz, red);
+  const double sbb = block_sum<K1_BS>(bb, red);
+  block_sum2<K1_BS>(rr, rz, red);
   if (threadIdx.x == 0) {
-    A.part2[3 * blockIdx.x] = sbb;
-    A.part2[3 * blockIdx.x + 1] = srr;
-    A.part2[3 * blockIdx.x + 2] = srz;
-  }
-  if (last_block(&A.sc->ticket2, &last)) {
-    const double tbb = block_sum_array<PCG_BS>(A.part2, gridDim.x, 3, red);
-    const double trr = block_sum_array<PCG_BS>(A.part2 + 1, gridDim.x, 3, red);
-    const double trz = block_sum_array<PCG_BS>(A.part2 + 2, gridDim.x, 3, red);
-    if (threadIdx.x == 0) {
-      PcgScalars* sc = A.sc;
-      double thr = tol * tol * tbb;
-      if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
-      sc->bb = tbb;
-      sc->thr = thr;
-      sc->rr = trr;
-      sc->rz[0] = trz;
-      sc->rz[1] = trz;
-      sc->rz0 = trz;
-      sc->pq = 0.0;
-      sc->iters = 0;
-      sc->it = 0;
-      sc->maxit = maxit;
-      sc->done = (tbb == 0.0 || trr < thr || maxit <= 0) ? 1 : 0;
+    A.partb[blockIdx.x] = sbb;
+    double* o = A.part2 + 2 * PCG_MAXP;  // parity slot of iteration -1
+    o[2 * blockIdx.x] = rr;
+    o[2 * blockIdx.x + 1] = rz;
+    if (blockIdx.x == 0) {
+      A.sc->done = 0;
+      A.sc->iters = 0;
     }
   }
 }
 
-// ---- K1: p_k = z + beta p_{k-1}; q = A p_k; p.q ---------------------------------
-template <class T>
-__global__ __launch_bounds__(PCG_BS) void pcg_k1_kernel(PcgArgs A) {
-  __shared__ double red[PCG_BS / 64];
-  __shared__ bool last;
+// ---- K1: convergence test of k-1; p_k = z + beta p_{k-1}; q = A p_k; p.q ----
+template <class T, int L>
+__global__ __launch_bounds__(K1_BS) void pcg_k1_kernel(PcgArgs A, int k) {
+  __shared__ double red[2 * K1_BS / 64 + 2];
   PcgScalars* sc = A.sc;
   if (sc->done) return;
-  const int k = sc->it;
-  const double beta = (k > 0) ? sc->rz[k & 1] / sc->rz[(k - 1) & 1] : 0.0;
-  // rz[(k+1)&1] is written by the last workgroup of K2 of this iteration,
-  // after every workgroup of this launch has finished.
+  const double* prev = rr_slot(A, k - 1);
+  const int cprev = rr_count(A, k - 1);
+  const double rr = all_sum<K1_BS>(prev, cprev, 2, red);
+  const double rz = all_sum<K1_BS>(prev + 1, cprev, 2, red);
+  double thr, beta;
+  if (k == 0) {
+    const double bb = all_sum<K1_BS>(A.partb, A.g1, 1, red);
+    thr = A.tol * A.tol * bb;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->thr = thr;
+      sc->bb = bb;
+      sc->rr = rr;
+    }
+    if (bb == 0.0 || rr < thr) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->done = 1;
+        sc->iters = 0;
+      }
+      return;
+    }
+    beta = 0.0;
+  } else {
+    thr = sc->thr;
+    if (rr < thr) {  // iteration k-1 converged (Eigen breaks with i = k-1)
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->done = 1;
+        sc->iters = k - 1;
+        sc->rr = rr;
+      }
+      return;
+    }
+    const double rzm = all_sum<K1_BS>(rr_slot(A, k - 2) + 1, rr_count(A, k - 2), 2, red);
+    beta = rz / rzm;
+  }
   const T tb = (T)beta;
   const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
   const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
   T* __restrict__ pnew = reinterpret_cast<T*>((k & 1) ? A.p1 : A.p0);
   const T* __restrict__ pold = reinterpret_cast<const T*>((k & 1) ? A.p0 : A.p1);
   T* __restrict__ q = reinterpret_cast<T*>(A.q);
-  const int lane = threadIdx.x & (LPR - 1);
+  const int lane = threadIdx.x & (L - 1);
   double pq = 0.0;
-  for (int row = (blockIdx.x * PCG_BS + threadIdx.x) / LPR; row < A.n;
-       row += gridDim.x * ROWS_PER_BLOCK) {
+  for (int row = (blockIdx.x * K1_BS + threadIdx.x) / L; row < A.n;
+       row += gridDim.x * (K1_BS / L)) {
     T y0 = 0, y1 = 0, y2 = 0;
     const long e1 = A.rowptr[row + 1];
-    for (long s = A.rowptr[row] + lane; s < e1; s += LPR) {
+    for (long s = A.rowptr[row] + lane; s < e1; s += L) {
       const long j = A.bcol[s];
       const T* B = bval + s * 9;
       const T pj0 = z[3 * j] + tb * pold[3 * j];
@@ -171,9 +214,9 @@ __global__ __launch_bounds__(PCG_BS) void pcg_k1_kernel(PcgArgs A) {
       y1 += B[3] * pj0 + B[4] * pj1 + B[5] * pj2;
       y2 += B[6] * pj0 + B[7] * pj1 + B[8] * pj2;
     }
-    y0 = group_sum8(y0);
-    y1 = group_sum8(y1);
-    y2 = group_sum8(y2);
+    y0 = group_sum<L>(y0);
+    y1 = group_sum<L>(y1);
+    y2 = group_sum<L>(y2);
     if (lane == 0) {
       const long i = row;
       const T pi0 = z[3 * i] + tb * pold[3 * i];
@@ -188,31 +231,26 @@ __global__ __launch_bounds__(PCG_BS) void pcg_k1_kernel(PcgArgs A) {
       pq += (double)pi0 * (double)y0 + (double)pi1 * (double)y1 + (double)pi2 * (double)y2;
     }
   }
-  const double s = block_sum<PCG_BS>(pq, red);
+  const double s = block_sum<K1_BS>(pq, red);
   if (threadIdx.x == 0) A.part1[blockIdx.x] = s;
-  if (last_block(&sc->ticket1, &last)) {
-    const double t = block_sum_array<PCG_BS>(A.part1, gridDim.x, 1, red);
-    if (threadIdx.x == 0) sc->pq = t;
-  }
 }
 
-// ---- K2: x += alpha p; r -= alpha q; z = M^-1 r; r.r, r.z; convergence -------
+// ---- K2: x += alpha p; r -= alpha q; z = M^-1 r; r.r, r.z ---------------------
 template <class T>
-__global__ __launch_bounds__(PCG_BS) void pcg_k2_kernel(PcgArgs A) {
-  __shared__ double red[2 * PCG_BS / 64];
-  __shared__ bool last;
+__global__ __launch_bounds__(K2_BS) void pcg_k2_kernel(PcgArgs A, int k) {
+  __shared__ double red[2 * K2_BS / 64 + 2];
   PcgScalars* sc = A.sc;
   if (sc->done) return;
-  const int k = sc->it;
-  const double alpha = sc->rz[k & 1] / sc->pq;
-  const T ta = (T)alpha;
+  const double pq = all_sum<K2_BS>(A.part1, A.g1, 1, red);
+  const double rz = all_sum<K2_BS>(rr_slot(A, k - 1) + 1, rr_count(A, k - 1), 2, red);
+  const T ta = (T)(rz / pq);
   const T* __restrict__ pk = reinterpret_cast<const T*>((k & 1) ? A.p1 : A.p0);
   const T* __restrict__ q = reinterpret_cast<const T*>(A.q);
   T* __restrict__ x = reinterpret_cast<T*>(A.x);
   T* __restrict__ r = reinterpret_cast<T*>(A.r);
   T* __restrict__ z = reinterpret_cast<T*>(A.z);
-  double rr = 0.0, rz = 0.0;
-  for (long i = blockIdx.x * PCG_BS + threadIdx.x; i < A.n; i += (long)gridDim.x * PCG_BS) {
+  double rr = 0.0, rzn = 0.0;
+  for (long i = blockIdx.x * K2_BS + threadIdx.x; i < A.n; i += (long)gridDim.x * K2_BS) {
     T ri[3], zi[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -226,62 +264,91 @@ __global__ __launch_bounds__(PCG_BS) void pcg_k2_kernel(PcgArgs A) {
     for (int p = 0; p < 3; ++p) {
       z[3 * i + p] = zi[p];
       rr += (double)ri[p] * (double)ri[p];
-      rz += (double)ri[p] * (double)zi[p];
+      rzn += (double)ri[p] * (double)zi[p];
     }
   }
-  block_sum2<PCG_BS>(rr, rz, red);
+  block_sum2<K2_BS>(rr, rzn, red);
   if (threadIdx.x == 0) {
-    A.part2[2 * blockIdx.x] = rr;
-    A.part2[2 * blockIdx.x + 1] = rz;
+    double* o = A.part2 + (long)(k & 1) * 2 * PCG_MAXP;
+    o[2 * blockIdx.x] = rr;
+    o[2 * blockIdx.x + 1] = rzn;
   }
-  if (last_block(&sc->ticket2, &last)) {
-    const double trr = block_sum_array<PCG_BS>(A.part2, gridDim.x, 2, red);
-    const double trz = block_sum_array<PCG_BS>(A.part2 + 1, gridDim.x, 2, red);
+}
+
+// ---- status after iterations [0, k_next) ---------------------------------------
+__global__ __launch_bounds__(256) void pcg_status_kernel(PcgArgs A, int k_next, int maxit) {
+  __shared__ double red[2 * 256 / 64 + 2];
+  PcgScalars* sc = A.sc;
+  if (sc->done) return;
+  const int k = k_next;
+  const double rr = all_sum<256>(rr_slot(A, k - 1), rr_count(A, k - 1), 2, red);
+  double thr;
+  if (k == 0) {
+    const double bb = all_sum<256>(A.partb, A.g1, 1, red);
+    thr = A.tol * A.tol * bb;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
     if (threadIdx.x == 0) {
-      sc->rr = trr;
-      sc->rz[(k + 1) & 1] = trz;
-      if (trr < sc->thr) {
-        sc->done = 1;
-        sc->iters = k;
-      } else if (k + 1 >= sc->maxit) {
-        sc->done = 1;
-        sc->iters = k + 1;
-      }
-      sc->it = k + 1;
+      sc->bb = bb;
+      sc->thr = thr;
+    }
+  } else {
+    thr = sc->thr;
+  }
+  if (threadIdx.x == 0) {
+    sc->rr = rr;
+    if (rr < thr) {
+      sc->done = 1;
+      sc->iters = k > 0 ? k - 1 : 0;
+    } else if (k >= maxit) {
+      sc->done = 1;
+      sc->iters = maxit;
     }
   }
 }
 
-void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b,
-                     double tol, int maxit, int x_zero) {
-  const int g = grid_rows(a.n);
+template <int L>
+static void launch_init_l(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, int xz) {
   if (fp32)
-    hipLaunchKernelGGL(pcg_init_kernel<float>, dim3(g), dim3(PCG_BS), 0, s, a,
-                       reinterpret_cast<const float*>(b), tol, maxit, x_zero);
+    hipLaunchKernelGGL((pcg_init_kernel<float, L>), dim3(a.g1), dim3(K1_BS), 0, s, a,
+                       reinterpret_cast<const float*>(b), xz);
   else
-    hipLaunchKernelGGL(pcg_init_kernel<double>, dim3(g), dim3(PCG_BS), 0, s, a,
-                       reinterpret_cast<const double*>(b), tol, maxit, x_zero);
+    hipLaunchKernelGGL((pcg_init_kernel<double, L>), dim3(a.g1), dim3(K1_BS), 0, s, a,
+                       reinterpret_cast<const double*>(b), xz);
 }
 
-void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a) {
-  const int g1 = grid_rows(a.n);
-  if (fp32)
-    hipLaunchKernelGGL(pcg_k1_kernel<float>, dim3(g1), dim3(PCG_BS), 0, s, a);
-  else
-    hipLaunchKernelGGL(pcg_k1_kernel<double>, dim3(g1), dim3(PCG_BS), 0, s, a);
+void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, int x_zero) {
+  switch (a.lpr) {
+    case 4: launch_init_l<4>(s, fp32, a, b, x_zero); break;
+    case 16: launch_init_l<16>(s, fp32, a, b, x_zero); break;
+    default: launch_init_l<8>(s, fp32, a, b, x_zero); break;
+  }
 }
 
-void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a) {
-  const int g2 = grid_nodes(a.n);
+template <int L>
+static void launch_k1_l(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
   if (fp32)
-    hipLaunchKernelGGL(pcg_k2_kernel<float>, dim3(g2), dim3(PCG_BS), 0, s, a);
+    hipLaunchKernelGGL((pcg_k1_kernel<float, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
   else
-    hipLaunchKernelGGL(pcg_k2_kernel<double>, dim3(g2), dim3(PCG_BS), 0, s, a);
+    hipLaunchKernelGGL((pcg_k1_kernel<double, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
 }
 
-void launch_pcg_iter(hipStream_t s, bool fp32, const PcgArgs& a) {
-  launch_pcg_k1(s, fp32, a);
-  launch_pcg_k2(s, fp32, a);
+void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
+  switch (a.lpr) {
+    case 4: launch_k1_l<4>(s, fp32, a, k); break;
+    case 16: launch_k1_l<16>(s, fp32, a, k); break;
+    default: launch_k1_l<8>(s, fp32, a, k); break;
+  }
+}
+
+void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
+  if (fp32)
+    hipLaunchKernelGGL(pcg_k2_kernel<float>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+  else
+    hipLaunchKernelGGL(pcg_k2_kernel<double>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+}
+
+void launch_pcg_status(hipStream_t s, const PcgArgs& a, int k_next, int maxit) {
+  hipLaunchKernelGGL(pcg_status_kernel, dim3(1), dim3(256), 0, s, a, k_next, maxit);
 }
 
 // ---- FP64 residual with a precision-T matrix --------------------------------
@@ -327,13 +394,13 @@ __global__ __launch_bounds__(PCG_BS) void residual64_kernel(
   }
 }
 
-// partials: PCG_MAXGRID doubles followed by an unsigned ticket
+// partials: PCG_MAXP doubles followed by an unsigned ticket
 void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
                        const int* bcol, const void* bval, const double* x,
                        const double* b, double* r, double* partials,
                        double* out) {
   const int g = grid_rows(n);
-  unsigned* ticket = reinterpret_cast<unsigned*>(partials + PCG_MAXGRID);
+  unsigned* ticket = reinterpret_cast<unsigned*>(partials + PCG_MAXP);
   if (fp32_mat)
     hipLaunchKernelGGL(residual64_kernel<float>, dim3(g), dim3(PCG_BS), 0, s, n, rowptr,
                        bcol, reinterpret_cast<const float*>(bval), x, b, r, partials,
@@ -344,19 +411,19 @@ void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
                        out, ticket);
 }
 
-// ---- plain SpMV (timing, tests) ----------------------------------------------
-template <class T>
-__global__ __launch_bounds__(PCG_BS) void spmv_kernel(int n, const long* __restrict__ rowptr,
-                                                     const int* __restrict__ bcol,
-                                                     const T* __restrict__ bval,
-                                                     const T* __restrict__ x,
-                                                     T* __restrict__ y) {
-  const int lane = threadIdx.x & (LPR - 1);
-  for (int row = (blockIdx.x * PCG_BS + threadIdx.x) / LPR; row < n;
-       row += gridDim.x * ROWS_PER_BLOCK) {
+// ---- plain SpMV (tuning, timing) --------------------------------------------
+template <class T, int L>
+__global__ __launch_bounds__(K1_BS) void spmv_kernel(int n, const long* __restrict__ rowptr,
+                                                    const int* __restrict__ bcol,
+                                                    const T* __restrict__ bval,
+                                                    const T* __restrict__ x,
+                                                    T* __restrict__ y) {
+  const int lane = threadIdx.x & (L - 1);
+  for (int row = (blockIdx.x * K1_BS + threadIdx.x) / L; row < n;
+       row += gridDim.x * (K1_BS / L)) {
     T y0 = 0, y1 = 0, y2 = 0;
     const long e1 = rowptr[row + 1];
-    for (long s = rowptr[row] + lane; s < e1; s += LPR) {
+    for (long s = rowptr[row] + lane; s < e1; s += L) {
       const long j = bcol[s];
       const T* B = bval + s * 9;
       const T xj0 = x[3 * j], xj1 = x[3 * j + 1], xj2 = x[3 * j + 2];
@@ -364,9 +431,9 @@ __global__ __launch_bounds__(PCG_BS) void spmv_kernel(int n, const long* __restr
       y1 += B[3] * xj0 + B[4] * xj1 + B[5] * xj2;
       y2 += B[6] * xj0 + B[7] * xj1 + B[8] * xj2;
     }
-    y0 = group_sum8(y0);
-    y1 = group_sum8(y1);
-    y2 = group_sum8(y2);
+    y0 = group_sum<L>(y0);
+    y1 = group_sum<L>(y1);
+    y2 = group_sum<L>(y2);
     if (lane == 0) {
       y[3L * row] = y0;
       y[3L * row + 1] = y1;
@@ -375,17 +442,27 @@ __global__ __launch_bounds__(PCG_BS) void spmv_kernel(int n, const long* __restr
   }
 }
 
-void launch_spmv(hipStream_t s, bool fp32, int n, const long* rowptr,
-                 const int* bcol, const void* bval, const void* x, void* y) {
-  const int g = grid_rows(n);
+template <int L>
+static void launch_spmv_l(hipStream_t s, bool fp32, int n, const long* rowptr, const int* bcol,
+                          const void* bval, const void* x, void* y) {
+  const int g = pcg_grid1(n, L);
   if (fp32)
-    hipLaunchKernelGGL(spmv_kernel<float>, dim3(g), dim3(PCG_BS), 0, s, n, rowptr, bcol,
+    hipLaunchKernelGGL((spmv_kernel<float, L>), dim3(g), dim3(K1_BS), 0, s, n, rowptr, bcol,
                        reinterpret_cast<const float*>(bval),
                        reinterpret_cast<const float*>(x), reinterpret_cast<float*>(y));
   else
-    hipLaunchKernelGGL(spmv_kernel<double>, dim3(g), dim3(PCG_BS), 0, s, n, rowptr, bcol,
+    hipLaunchKernelGGL((spmv_kernel<double, L>), dim3(g), dim3(K1_BS), 0, s, n, rowptr, bcol,
                        reinterpret_cast<const double*>(bval),
                        reinterpret_cast<const double*>(x), reinterpret_cast<double*>(y));
+}
+
+void launch_spmv(hipStream_t s, bool fp32, int n, const long* rowptr, const int* bcol,
+                 const void* bval, const void* x, void* y, int lpr) {
+  switch (lpr) {
+    case 4: launch_spmv_l<4>(s, fp32, n, rowptr, bcol, bval, x, y); break;
+    case 16: launch_spmv_l<16>(s, fp32, n, rowptr, bcol, bval, x, y); break;
+    default: launch_spmv_l<8>(s, fp32, n, rowptr, bcol, bval, x, y); break;
+  }
 }
 
 // ---- vector helpers -------------------------------------------------------------
@@ -472,10 +549,10 @@ __global__ __launch_bounds__(256) void dot64_kernel(long n, const double* __rest
   }
 }
 
-// partials: PCG_MAXGRID doubles followed by an unsigned ticket
+// partials: PCG_MAXP doubles followed by an unsigned ticket
 void launch_dot64(hipStream_t s, long n, const double* a, const double* b,
                   double* partials, double* out) {
-  unsigned* ticket = reinterpret_cast<unsigned*>(partials + PCG_MAXGRID);
+  unsigned* ticket = reinterpret_cast<unsigned*>(partials + PCG_MAXP);
   hipLaunchKernelGGL(dot64_kernel, dim3(DOT_GRID), dim3(256), 0, s, n, a, b, partials,
                      out, ticket);
 }

@@ -13,10 +13,8 @@ namespace pgn {
 static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero,
                    double& rr, double& bb) {
   hipStream_t s = M.stream();
-  const PcgArgs a = M.pcg_args(maxit, precond);
-  launch_pcg_init(s, M.hfp32, a, M.pb.p, tol, maxit, x_zero ? 1 : 0);
-  int k = 0, chunk = 8, done_its = 0;
-  bool have = false;
+  const PcgArgs a = M.pcg_args(tol, precond);
+  launch_pcg_init(s, M.hfp32, a, M.pb.p, x_zero ? 1 : 0);
   if (M.profiling && (int)M.prof_ev.size() < 3 * M.pcg_chunk_max) {
     while ((int)M.prof_ev.size() < 3 * M.pcg_chunk_max) {
       hipEvent_t e;
@@ -24,26 +22,23 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
       M.prof_ev.push_back(e);
     }
   }
-  while (k < maxit) {
+  int k = 0, chunk = 8;
+  for (;;) {
     const int c = std::min(chunk, maxit - k);
-    if (M.profiling) {
-      for (int j = 0; j < c; ++j) {
-        PGN_HIP(hipEventRecord(M.prof_ev[3 * j], s));
-        launch_pcg_k1(s, M.hfp32, a);
-        PGN_HIP(hipEventRecord(M.prof_ev[3 * j + 1], s));
-        launch_pcg_k2(s, M.hfp32, a);
-        PGN_HIP(hipEventRecord(M.prof_ev[3 * j + 2], s));
-      }
-    } else {
-      for (int j = 0; j < c; ++j) launch_pcg_iter(s, M.hfp32, a);
+    for (int j = 0; j < c; ++j) {
+      if (M.profiling) PGN_HIP(hipEventRecord(M.prof_ev[3 * j], s));
+      launch_pcg_k1(s, M.hfp32, a, k + j);
+      if (M.profiling) PGN_HIP(hipEventRecord(M.prof_ev[3 * j + 1], s));
+      launch_pcg_k2(s, M.hfp32, a, k + j);
+      if (M.profiling) PGN_HIP(hipEventRecord(M.prof_ev[3 * j + 2], s));
     }
     k += c;
+    launch_pcg_status(s, a, k, maxit);
     PGN_HIP(hipMemcpyAsync(M.hsc, M.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     M.sync();
-    have = true;
     if (M.profiling) {
       // launches after convergence return at once: time only the live ones
-      const int live = std::min(c, M.hsc->it - done_its);
+      const int live = M.hsc->done ? std::max(0, std::min(c, M.hsc->iters + 1 - (k - c))) : c;
       for (int j = 0; j < live; ++j) {
         float t1 = 0.f, t2 = 0.f;
         PGN_HIP(hipEventElapsedTime(&t1, M.prof_ev[3 * j], M.prof_ev[3 * j + 1]));
@@ -53,14 +48,9 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
       }
       M.prof_n[0] += live;
       M.prof_n[1] += live;
-      done_its = M.hsc->it;
     }
-    if (M.hsc->done) break;
+    if (M.hsc->done || k >= maxit) break;
     chunk = std::min(2 * chunk, M.pcg_chunk_max);
-  }
-  if (!have) {
-    PGN_HIP(hipMemcpyAsync(M.hsc, M.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-    M.sync();
   }
   rr = M.hsc->rr;
   bb = M.hsc->bb;
@@ -125,6 +115,7 @@ static void solve_once(pgn_model& M, const double* b, double* x, const pgn_linea
 void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opts& o,
                   pgn_linear_result& res) {
   PGN_CHECK(M.hess_valid, "Hessian not assembled");
+  M.tune();
   hipStream_t s = M.stream();
   const long N = 3L * M.n;
   double* xprev = M.tmp64b.p;
@@ -194,6 +185,9 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
   int minImpr = 0, minStep = 0;
   bool steepest = false;
   int itCount = 0;
+  // Eisenstat-Walker forcing (choice 2, gamma 0.9, alpha 2) when adaptive_tol:
+  // the linear tolerance follows the Newton residual's own convergence rate.
+  double eta = 0.1, g_prev = -1.0;
 
   // OptimSolver_USQP_LS::SolveStepInternal (OptimSolver_USQP_LS.cpp:18-68)
   auto solve_step = [&](bool use_steepest) {
@@ -213,6 +207,17 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
       R.hessian_evals++;
       launch_lincomb64(s, N, 1.0, xsave, -1.0, v0, dx);  // dx0 = x - x_prev (:488)
       launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, bvec);
+      if (o.adaptive_tol) {
+        if (g_prev > 0.0) {
+          const double r = opt / g_prev;
+          double e = 0.9 * r * r;
+          const double guard = 0.9 * eta * eta;
+          if (guard > 0.1) e = std::max(e, guard);
+          eta = std::min(0.1, e);
+        }
+        lo.tol = std::max(o.lsTolMaxError, eta);
+        g_prev = opt;
+      }
       pgn_linear_result lr;
       linear_solve(M, bvec, dx, lo, lr);
       R.pcg_iterations += lr.iterations;
