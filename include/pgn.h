@@ -230,6 +230,53 @@ int pgn_model_set_profiling(pgn_model* model, int on);
 int pgn_model_get_profile(pgn_model* model, int which, long* count,
                           double* total_ms);
 
+/* ---- discrete elastic rods (Simulable_DER) ------------------------------ */
+/* A batch of open rods solved together (every rod keeps its own Newton
+ * state).  Reference DoF layout per rod: 3n positions then n-1 edge angles
+ * (KinematicsTree.cpp:33-40); batch vectors concatenate the rods.
+ * Setup: Simulable_DER::CreateKinematicsEle / CreateEnergyElements
+ * (Simulable_DER.cpp:66-150), Mesh_DER::Init (Mesh_DER.cpp:97-164).
+ * X0: 3 per vertex, all rods; frame0_normal: 3 per rod, the normal of the
+ * first edge's reference frame (projected to be orthogonal to the edge);
+ * per rod: Young's modulus (Param_Young), shear modulus (Param_Lame2),
+ * cross-section radii (Size_0 = h, Size_1 = w) and density (kg per metre of
+ * rod: node masses are density * L / 2 per edge). */
+typedef struct pgn_rods pgn_rods;
+int pgn_rods_create(pgn_ctx* ctx, int n_rods, const int* n_verts,
+                    const double* X0, const double* frame0_normal,
+                    const double* young, const double* shear,
+                    const double* radius_h, const double* radius_w,
+                    const double* density, int precision, pgn_rods** out);
+void pgn_rods_destroy(pgn_rods* rods);
+long pgn_rods_num_dofs(const pgn_rods* rods);
+int pgn_rods_set_fixed(pgn_rods* rods, const unsigned char* mask);
+int pgn_rods_set_gravity(pgn_rods* rods, const double g_ini[3],
+                         const double g_end[3], int num_stages);
+int pgn_rods_reset(pgn_rods* rods);
+int pgn_rods_set_x(pgn_rods* rods, const double* x);
+int pgn_rods_get_x(pgn_rods* rods, double* x);
+/* per-rod energies (n_rods) / batch gradient, fixed entries zeroed */
+int pgn_rods_get_energy(pgn_rods* rods, double* energy);
+int pgn_rods_get_gradient(pgn_rods* rods, double* gradient);
+/* dense Hessian of one rod in its reference layout ((4n-1)^2 values, fixed
+ * rows / columns identity) */
+int pgn_rods_get_hessian_dense(pgn_rods* rods, int rod, double* H);
+/* reference frames (tangent, normal: 3 per edge) and reference twist (per
+ * vertex) of every rod, edges / vertices concatenated in rod order */
+int pgn_rods_get_frames(pgn_rods* rods, double* tan, double* nor, double* rt);
+int pgn_rods_set_frames(pgn_rods* rods, const double* tan, const double* nor,
+                        const double* rt);
+/* Simulable::UpdateKinematics: KEleDERMatTwist / KEleDERRefTwist
+ * (KEleDER.cpp:94-98,216-220) */
+int pgn_rods_update_kinematics(pgn_rods* rods);
+/* bend-twist element outputs at every vertex (zero where no element): E,
+ * g over (e1, e2, theta1, theta2) (8), H (64, row-major) */
+int pgn_rods_element_bt(pgn_rods* rods, double* E, double* g8, double* H64);
+/* OptimSolver::SolveFull for every rod (LinearSolver_EigenLDLT semantics on
+ * each rod's banded Hessian); res: n_rods results */
+int pgn_rods_newton(pgn_rods* rods, const pgn_newton_opts* opts,
+                    pgn_newton_result* res);
+
 /* ---- host-only helpers (no GPU needed; used by CPU tests) --------------- */
 /* Builds the BSR3 pattern and the element->block slot map of a tet mesh the
  * way pgn_model_create_tet does.  nb_out: number of 3x3 blocks. */

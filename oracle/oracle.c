@@ -808,77 +808,62 @@ void orc_newton_default_opts(orc_newton_opts* o) {
 
 static double norm2(int n, const double* a) { return sqrt(dot(n, a, a)); }
 
+/* OptimSolver_USQP_LS::SolveStepInternal (OptimSolver_USQP_LS.cpp:18-68),
+ * ComputeStep (OptimSolver.cpp:313-352), LineSearch_Simple (:688-736) and
+ * UpdateStepResult (:402-466). */
 typedef struct {
   int N;
-  double *vx, *v0, *vg, *dx, *dxo, *vgNew, *xsave;
+  double *vx, *v0, *vg, *dx, *dxo, *vgNew, *xsave, *b, *xt;
   double obj, opt;
-  int result, minImprCount, minStepCount, steepest;
-  long* rp;
-  int* ci;
-  double* val;
-  int cg_its, bis;
+  int result, minImprCount, minStepCount;
+  int lin_its, bis;
 } nstate;
 
-/* OptimSolver_USQP_LS.cpp:18-68 */
-static void solve_step(orc_model* M, const orc_newton_opts* o, nstate* S,
+static void solve_step(orc_problem* P, const orc_newton_opts* o, nstate* S,
                        int steepest) {
   const int N = S->N;
-  /* OnIterStart: StepBoundaryConditions + GetState */
-  orc_model_step_load(M);
-  memcpy(S->xsave, M->x, sizeof(double) * N);
-  S->obj = orc_model_energy(M);
-  orc_model_gradient(M, S->vg);
-  memcpy(S->vx, M->x, sizeof(double) * N);
+  P->step_load(P->p);                 /* OnIterStart: StepBoundaryConditions */
+  P->get_x(P->p, S->xsave);           /*              GetState */
+  S->obj = P->energy(P->p);
+  P->gradient(P->p, S->vg);
+  memcpy(S->vx, S->xsave, sizeof(double) * N);
   S->opt = norm2(N, S->vg);
   if (S->opt < o->tolMaxError) {
     S->result = ORC_OS_SUCCESS;
     return;
   }
-  /* ComputeStep (OptimSolver.cpp:313-352); trial 0 always succeeds */
   if (!steepest) {
-    orc_model_hessian_lower(M, S->rp, S->ci, S->val);
     for (int k = 0; k < N; ++k) S->dx[k] = S->vx[k] - S->v0[k]; /* :488 */
-    double* b = S->dxo;
-    for (int k = 0; k < N; ++k) b[k] = -S->vg[k];
-    int reg_used = 0;
-    orc_linear_solve(N, S->rp, S->ci, S->val, b, S->dx, o->lsTolMaxError,
-                     o->lsNumMaxIters, o->lsNumRegTrials, &S->cg_its,
-                     &reg_used);
+    for (int k = 0; k < N; ++k) S->b[k] = -S->vg[k];
+    S->lin_its += P->solve(P->p, S->b, S->dx, o);
   } else {
     for (int k = 0; k < N; ++k) S->dx[k] = -S->vg[k];
   }
   const double len = norm2(N, S->dx);
   if (len > o->maxStepSize)
     for (int k = 0; k < N; ++k) S->dx[k] = S->dx[k] / len * o->maxStepSize;
-  /* LineSearch_Simple (OptimSolver.cpp:688-736) */
   double factor = 1.0, objNew = 0;
-  int improved = 0, numBis = -1;
-  memcpy(M->x, S->xsave, sizeof(double) * N); /* PrePerformStep */
+  int numBis = -1;
   do {
     numBis++;
     for (int k = 0; k < N; ++k) S->dxo[k] = factor * S->dx[k];
-    for (int k = 0; k < N; ++k) M->x[k] = S->xsave[k] + S->dxo[k];
-    objNew = orc_model_energy(M);
-    orc_model_gradient(M, S->vgNew);
-    if (objNew < S->obj) {
-      improved = 1;
-      break;
-    }
+    for (int k = 0; k < N; ++k) S->xt[k] = S->xsave[k] + S->dxo[k];
+    P->set_x(P->p, S->xt);
+    objNew = P->energy(P->p);
+    P->gradient(P->p, S->vgNew);
+    if (objNew < S->obj) break;
     if (o->ls_noise_guard > 0 &&
-        objNew <= S->obj + o->ls_noise_guard * orc_model_energy_scale(M) &&
+        objNew <= S->obj + o->ls_noise_guard * P->energy_scale(P->p) &&
         norm2(N, S->vgNew) < S->opt) {
       /* energy change below rounding: accept on the gradient norm and book
        * it as the smallest positive improvement */
       objNew = S->obj - 1e-300;
-      improved = 1;
       break;
     }
     factor *= o->lSearch_alpha;
-    memcpy(M->x, S->xsave, sizeof(double) * N);
+    P->set_x(P->p, S->xsave); /* PrePerformStep */
   } while (norm2(N, S->dxo) > o->tolMinStep && numBis <= o->lsearch_numTry);
-  (void)improved;
   S->bis += numBis;
-  /* UpdateStepResult (OptimSolver.cpp:402-466) */
   const double impr = S->obj - objNew;
   const double step = norm2(N, S->dxo);
   if (impr > 0) {
@@ -906,59 +891,49 @@ static void solve_step(orc_model* M, const orc_newton_opts* o, nstate* S,
   } else {
     S->result = ORC_OS_NONDESC;
   }
-  /* OnIterFinish: UpdateKinematics (no kinematic state for tets) */
+  P->iter_finish(P->p); /* OnIterFinish: UpdateKinematics */
 }
 
-void orc_newton_solve(orc_model* M, const orc_newton_opts* o,
-                      orc_newton_result* R) {
+/* OptimSolver::Init + SolveFull (OptimSolver.cpp:36-55,163-264) */
+void orc_newton_generic(orc_problem* P, const orc_newton_opts* o,
+                        orc_newton_result* R) {
   nstate S;
   memset(&S, 0, sizeof(S));
-  const int N = 3 * M->n;
+  const int N = P->N;
   S.N = N;
-  S.vx = (double*)malloc(sizeof(double) * N);
-  S.v0 = (double*)malloc(sizeof(double) * N);
-  S.vg = (double*)malloc(sizeof(double) * N);
-  S.dx = (double*)malloc(sizeof(double) * N);
-  S.dxo = (double*)malloc(sizeof(double) * N);
-  S.vgNew = (double*)malloc(sizeof(double) * N);
-  S.xsave = (double*)malloc(sizeof(double) * N);
-  S.rp = (long*)malloc(sizeof(long) * (N + 1));
-  S.ci = (int*)malloc(sizeof(int) * M->nnz);
-  S.val = (double*)malloc(sizeof(double) * M->nnz);
-  /* OptimSolver::Init (:36-55) */
-  memcpy(S.vx, M->x, sizeof(double) * N);
-  memcpy(S.v0, M->x, sizeof(double) * N);
-  S.obj = orc_model_energy(M);
-  orc_model_gradient(M, S.vg);
+  double** bufs[] = {&S.vx, &S.v0, &S.vg, &S.dx, &S.dxo, &S.vgNew, &S.xsave, &S.b, &S.xt};
+  for (int i = 0; i < 9; ++i) *bufs[i] = (double*)calloc(N, sizeof(double));
+  P->get_x(P->p, S.vx);
+  memcpy(S.v0, S.vx, sizeof(double) * N);
+  S.obj = P->energy(P->p);
+  P->gradient(P->p, S.vg);
   S.opt = norm2(N, S.vg);
-  /* SolveFull (:163-264) */
   S.result = ORC_OS_SUCCESS;
-  S.steepest = 0;
-  int itCount = 0;
+  int steepest = 0, itCount = 0;
   while (1) {
     itCount++;
-    solve_step(M, o, &S, S.steepest);
-    orc_model_gradient(M, S.vg);
+    solve_step(P, o, &S, steepest);
+    P->gradient(P->p, S.vg);
     S.opt = norm2(N, S.vg);
-    if (orc_model_fully_loaded(M)) {
+    if (P->fully_loaded(P->p)) {
       if (S.result == ORC_OS_NONDESC) {
         if (!o->trySteepest) break;
-        if (S.steepest) break;
-        S.steepest = 1;
+        if (steepest) break;
+        steepest = 1;
       }
       if (S.result == ORC_OS_MINIMPR) {
         if (S.minImprCount == o->minImprMaxCount) break;
-        if (o->trySteepest) S.steepest = !S.steepest;
+        if (o->trySteepest) steepest = !steepest;
       }
       if (S.result == ORC_OS_MINSTEP) {
         if (S.minStepCount == o->minStepMaxCount) break;
-        if (o->trySteepest) S.steepest = !S.steepest;
+        if (o->trySteepest) steepest = !steepest;
       }
       if (S.result == ORC_OS_SUCCESS) {
         if (S.opt <= o->tolMaxError) break;
         S.result = ORC_OS_ONGOING;
       }
-      if (S.result == ORC_OS_ONGOING) S.steepest = 0;
+      if (S.result == ORC_OS_ONGOING) steepest = 0;
     }
     if (itCount + 1 > o->numMaxIters) {
       S.result = ORC_OS_MAXITER;
@@ -967,18 +942,48 @@ void orc_newton_solve(orc_model* M, const orc_newton_opts* o,
   }
   R->result = S.result;
   R->iterations = itCount;
-  R->cg_iterations = S.cg_its;
+  R->cg_iterations = S.lin_its;
   R->line_search_bisections = S.bis;
   R->objective = S.obj;
   R->optimality = S.opt;
-  free(S.vx);
-  free(S.v0);
-  free(S.vg);
-  free(S.dx);
-  free(S.dxo);
-  free(S.vgNew);
-  free(S.xsave);
-  free(S.rp);
-  free(S.ci);
-  free(S.val);
+  for (int i = 0; i < 9; ++i) free(*bufs[i]);
+}
+
+/* tet model as a problem: Hessian in lower CSR + Jacobi CG with the
+ * reference's regularization loop */
+typedef struct {
+  orc_model* M;
+  long* rp;
+  int* ci;
+  double* val;
+} tet_prob;
+static double tp_energy(void* p) { return orc_model_energy(((tet_prob*)p)->M); }
+static double tp_escale(void* p) { return orc_model_energy_scale(((tet_prob*)p)->M); }
+static void tp_gradient(void* p, double* g) { orc_model_gradient(((tet_prob*)p)->M, g); }
+static int tp_solve(void* p, const double* b, double* dx, const orc_newton_opts* o) {
+  tet_prob* T = (tet_prob*)p;
+  orc_model_hessian_lower(T->M, T->rp, T->ci, T->val);
+  int its = 0, reg = 0;
+  orc_linear_solve(3 * T->M->n, T->rp, T->ci, T->val, b, dx, o->lsTolMaxError,
+                   o->lsNumMaxIters, o->lsNumRegTrials, &its, &reg);
+  return its;
+}
+static int tp_step(void* p) { return orc_model_step_load(((tet_prob*)p)->M); }
+static int tp_loaded(void* p) { return orc_model_fully_loaded(((tet_prob*)p)->M); }
+static void tp_get(void* p, double* x) { orc_model_get_x(((tet_prob*)p)->M, x); }
+static void tp_set(void* p, const double* x) { orc_model_set_x(((tet_prob*)p)->M, x); }
+static void tp_finish(void* p) { (void)p; }
+
+void orc_newton_solve(orc_model* M, const orc_newton_opts* o, orc_newton_result* R) {
+  tet_prob T;
+  T.M = M;
+  T.rp = (long*)malloc(sizeof(long) * (3 * M->n + 1));
+  T.ci = (int*)malloc(sizeof(int) * M->nnz);
+  T.val = (double*)malloc(sizeof(double) * M->nnz);
+  orc_problem P = {&T,       3 * M->n,  tp_energy, tp_escale, tp_gradient, tp_solve,
+                   tp_step,  tp_loaded, tp_get,    tp_set,    tp_finish};
+  orc_newton_generic(&P, o, R);
+  free(T.rp);
+  free(T.ci);
+  free(T.val);
 }

@@ -150,10 +150,68 @@ typedef struct {
 } orc_newton_result;
 
 void orc_newton_default_opts(orc_newton_opts* o);
+
+/* A problem the generic Newton driver runs on (IOptimProblem +
+ * ISimulable::UpdateKinematics + the linear solver). */
+typedef struct {
+  void* p;
+  int N;
+  double (*energy)(void*);
+  double (*energy_scale)(void*);
+  void (*gradient)(void*, double*);
+  /* Hessian at the current state; solve H dx = b with warm start dx;
+   * returns linear iterations */
+  int (*solve)(void*, const double* b, double* dx, const orc_newton_opts*);
+  int (*step_load)(void*);
+  int (*fully_loaded)(void*);
+  void (*get_x)(void*, double*);
+  void (*set_x)(void*, const double*);
+  void (*iter_finish)(void*);
+} orc_problem;
+void orc_newton_generic(orc_problem* P, const orc_newton_opts* o,
+                        orc_newton_result* r);
 /* OptimSolver::Init + SolveFull (OptimSolver.cpp:36-55,163-264) with
  * OptimSolver_USQP_LS::SolveStepInternal and OptimProblem_BasicStatic. */
 void orc_newton_solve(orc_model* m, const orc_newton_opts* o,
                       orc_newton_result* r);
+
+/* ---- discrete elastic rods (rods.c) ------------------------------------ */
+/* bend-twist element: prm = shear[2], young[2], radw[2], radh[2], L0, L1,
+ * K0[2], twist0, nRef1[3], tRef1[3], nRef2[3], tRef2[3], rt (27 values, the
+ * getBT*OPT argument order); g8 / H64 over (e1, e2, theta1, theta2) */
+double orc_rod_bt(const double* prm, const double* e1, const double* e2,
+                  double th1, double th2, double* g8, double* H64);
+double orc_rod_stretch(const double* x12, double L0, double k, double* g6,
+                       double* H36);
+double orc_reference_twist_change(const double* tan1, const double* nor1,
+                                  const double* tan2, const double* nor2,
+                                  double rt);
+typedef struct orc_rod orc_rod;
+/* one open rod of n vertices (edges i = (i, i+1)); frame0_nor: normal of
+ * the first edge's reference frame (projected onto the plane normal to it);
+ * uniform material and cross-section; lumped node mass line_density*L/2.
+ * DoFs: 3n positions, then n-1 edge angles. */
+orc_rod* orc_rod_create(int n, const double* X0, const double* frame0_nor,
+                        double young, double shear, double rh, double rw,
+                        double line_density);
+void orc_rod_destroy(orc_rod* R);
+int orc_rod_num_dofs(const orc_rod* R);
+void orc_rod_set_fixed(orc_rod* R, const unsigned char* mask);
+void orc_rod_set_gravity(orc_rod* R, const double* g_ini, const double* g_end,
+                         int stages);
+int orc_rod_step_load(orc_rod* R);
+int orc_rod_fully_loaded(const orc_rod* R);
+void orc_rod_get_x(const orc_rod* R, double* x);
+void orc_rod_set_x(orc_rod* R, const double* x);
+void orc_rod_get_frames(const orc_rod* R, double* tan, double* nor, double* rt);
+double orc_rod_energy(orc_rod* R);
+double orc_rod_energy_scale(const orc_rod* R);
+void orc_rod_gradient(orc_rod* R, double* g);
+void orc_rod_hessian(orc_rod* R, double* H); /* dense N x N */
+void orc_rod_update_kinematics(orc_rod* R);
+int orc_dense_ldlt_solve(int N, double* A, const double* b, double* x,
+                         double maxError, int regIters, int* reg_used);
+void orc_rod_newton(orc_rod* R, const orc_newton_opts* o, orc_newton_result* r);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,36 @@ def lib():
         L.orc_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
         L.orc_newton_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                        ctypes.POINTER(NewtonResult)]
+        L.orc_model_energy_scale.restype = ctypes.c_double
+        L.orc_model_energy_scale.argtypes = [ctypes.c_void_p]
+        L.orc_rod_bt.restype = ctypes.c_double
+        L.orc_rod_bt.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.orc_rod_stretch.restype = ctypes.c_double
+        L.orc_rod_stretch.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.orc_reference_twist_change.restype = ctypes.c_double
+        L.orc_reference_twist_change.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double]
+        L.orc_rod_create.restype = ctypes.c_void_p
+        L.orc_rod_create.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_rod_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_rod_num_dofs.argtypes = [ctypes.c_void_p]
+        L.orc_rod_set_fixed.argtypes = [ctypes.c_void_p, _ucp]
+        L.orc_rod_set_gravity.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.orc_rod_step_load.argtypes = [ctypes.c_void_p]
+        L.orc_rod_fully_loaded.argtypes = [ctypes.c_void_p]
+        L.orc_rod_get_x.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_rod_set_x.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_rod_get_frames.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_rod_energy.restype = ctypes.c_double
+        L.orc_rod_energy.argtypes = [ctypes.c_void_p]
+        L.orc_rod_gradient.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_rod_hessian.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_rod_update_kinematics.argtypes = [ctypes.c_void_p]
+        L.orc_rod_newton.argtypes = [ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
+                                     ctypes.POINTER(NewtonResult)]
+        L.orc_dense_ldlt_solve.restype = ctypes.c_int
+        L.orc_dense_ldlt_solve.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
+                                           ctypes.c_int, _ip]
         _LIB = L
     return _LIB
 
@@ -281,3 +311,105 @@ def lower_to_dense(rowptr, col, val):
                 A[r, c] = val[p]
                 A[c, r] = val[p]
     return A
+
+
+# ---- discrete elastic rods ---------------------------------------------------
+
+def rod_bt(prm, e1, e2, th1, th2):
+    """Bend-twist element: (E, g8, H 8x8); prm in the getBT*OPT argument order."""
+    g = np.zeros(8)
+    H = np.zeros(64)
+    E = lib().orc_rod_bt(_p(f64(prm)), _p(f64(e1)), _p(f64(e2)), th1, th2, _p(g), _p(H))
+    return E, g, H.reshape(8, 8)
+
+
+def rod_stretch(x12, L0, k):
+    g = np.zeros(6)
+    H = np.zeros(36)
+    E = lib().orc_rod_stretch(_p(f64(x12)), L0, k, _p(g), _p(H))
+    return E, g, H.reshape(6, 6)
+
+
+def reference_twist_change(t1, n1, t2, n2, rt):
+    return lib().orc_reference_twist_change(_p(f64(t1)), _p(f64(n1)), _p(f64(t2)),
+                                            _p(f64(n2)), rt)
+
+
+class Rod:
+    """One open rod (Simulable_DER restated on the CPU)."""
+
+    def __init__(self, X0, frame0_normal, young, shear, radius_h, radius_w, density):
+        X0 = f64(X0).reshape(-1)
+        self.n = X0.size // 3
+        self._h = lib().orc_rod_create(self.n, _p(X0), _p(f64(frame0_normal)), young, shear,
+                                       radius_h, radius_w, density)
+        self.N = lib().orc_rod_num_dofs(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_rod_destroy(h)
+            self._h = None
+
+    def set_fixed(self, mask):
+        mk = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        lib().orc_rod_set_fixed(self._h, _p(mk, _ucp))
+
+    def set_gravity(self, g_end, stages, g_ini=(0.0, 0.0, 0.0)):
+        lib().orc_rod_set_gravity(self._h, _p(f64(g_ini)), _p(f64(g_end)), int(stages))
+
+    def step_load(self):
+        return lib().orc_rod_step_load(self._h)
+
+    def get_x(self):
+        x = np.zeros(self.N)
+        lib().orc_rod_get_x(self._h, _p(x))
+        return x
+
+    def set_x(self, x):
+        lib().orc_rod_set_x(self._h, _p(f64(x)))
+
+    def frames(self):
+        t = np.zeros((self.n - 1, 3))
+        nn = np.zeros((self.n - 1, 3))
+        rt = np.zeros(self.n)
+        lib().orc_rod_get_frames(self._h, _p(t), _p(nn), _p(rt))
+        return t, nn, rt
+
+    def energy(self):
+        return lib().orc_rod_energy(self._h)
+
+    def gradient(self):
+        g = np.zeros(self.N)
+        lib().orc_rod_gradient(self._h, _p(g))
+        return g
+
+    def hessian(self):
+        H = np.zeros(self.N * self.N)
+        lib().orc_rod_hessian(self._h, _p(H))
+        return H.reshape(self.N, self.N)
+
+    def update_kinematics(self):
+        lib().orc_rod_update_kinematics(self._h)
+
+    def newton(self, **kw):
+        o = NewtonOpts()
+        lib().orc_newton_default_opts(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        r = NewtonResult()
+        lib().orc_rod_newton(self._h, ctypes.byref(o), ctypes.byref(r))
+        return {"result": OS_NAMES[r.result], "iterations": r.iterations,
+                "bisections": r.line_search_bisections, "objective": r.objective,
+                "optimality": r.optimality}
+
+
+def dense_ldlt_solve(A, b, x, max_error, reg_iters):
+    """LinearSolver_EigenLDLT::SolveInternal restated on a dense matrix (lower
+    part used, regularization applied in place to a copy).  Returns
+    (status 0 = SUCCESS, regularization try used); x is written."""
+    A = np.array(A, dtype=np.float64, order="C")
+    reg = ctypes.c_int(0)
+    st = lib().orc_dense_ldlt_solve(A.shape[0], _p(A), _p(f64(b)), _p(x), max_error,
+                                    int(reg_iters), ctypes.byref(reg))
+    return st, reg.value

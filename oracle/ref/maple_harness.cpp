@@ -49,3 +49,69 @@ void ref_stvk_dpdf(const double* Fv, double lame1, double lame2, double* out) {
 }
 
 }  // extern "C"
+
+// ---- discrete elastic rods ------------------------------------------------
+// RodEdge3D*OPT.mcg: EnergyElement_DER_Stretch.cpp:53-120 (reads x12 = edge
+// vector, L0, rh, rw, E; writes t15 / vfx[3] / mJx[3][3]).
+// getBT*OPT.h: EnergyElement_DER_BendTwist.cpp:67-249.
+#include <vector>
+#include "getBTEnergyOPT.h"
+#include "getBTGradientOPT.h"
+#include "getBTHessianOPT.h"
+#include "computeReferenceTwistChange.h"
+
+extern "C" {
+
+double ref_rod_stretch_energy(const double* x12, double L0, double rh, double rw,
+                              double E) {
+#include "RodEdge3DEnergyOPT.mcg"
+  return t15;
+}
+
+void ref_rod_stretch_force(const double* x12, double L0, double rh, double rw,
+                           double E, double* out) {
+  std::vector<double> vfx(3);
+#include "RodEdge3DForceOPT.mcg"
+  for (int i = 0; i < 3; ++i) out[i] = vfx[i];
+}
+
+void ref_rod_stretch_jacobian(const double* x12, double L0, double rh, double rw,
+                              double E, double* out) {
+  double mJx[3][3];
+#include "RodEdge3DJacobianOPT.mcg"
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = mJx[i][j];
+}
+
+// prm: shear[2], young[2], radw[2], radh[2], L0, L1, K0[2], twist0,
+//      nRef1[3], tRef1[3], nRef2[3], tRef2[3], rt   (27 values)
+double ref_bt_energy(const double* prm, const double* e1, const double* e2,
+                     double th1, double th2) {
+  double s[2] = {prm[0], prm[1]}, y[2] = {prm[2], prm[3]};
+  double w[2] = {prm[4], prm[5]}, h[2] = {prm[6], prm[7]};
+  return getBTEnergyOPT(s, y, w, h, prm[8], prm[9], prm + 10, prm[12], prm + 13,
+                        prm + 16, prm + 19, prm + 22, prm[25], e1, e2, th1, th2);
+}
+
+void ref_bt_gradient(const double* prm, const double* e1, const double* e2,
+                     double th1, double th2, double* g8) {
+  double s[2] = {prm[0], prm[1]}, y[2] = {prm[2], prm[3]};
+  double w[2] = {prm[4], prm[5]}, h[2] = {prm[6], prm[7]};
+  getBTGradientOPT(s, y, w, h, prm[8], prm[9], prm + 10, prm[12], prm + 13,
+                   prm + 16, prm + 19, prm + 22, prm[25], e1, e2, th1, th2, g8);
+}
+
+void ref_bt_hessian(const double* prm, const double* e1, const double* e2,
+                    double th1, double th2, double* H64) {
+  double s[2] = {prm[0], prm[1]}, y[2] = {prm[2], prm[3]};
+  double w[2] = {prm[4], prm[5]}, h[2] = {prm[6], prm[7]};
+  getBTHessianOPT(s, y, w, h, prm[8], prm[9], prm + 10, prm[12], prm + 13,
+                  prm + 16, prm + 19, prm + 22, prm[25], e1, e2, th1, th2, H64);
+}
+
+double ref_reference_twist_change(double* t1, double* n1, double* t2, double* n2,
+                                  double rt) {
+  return computeReferenceTwistChange(t1, n1, t2, n2, rt);
+}
+
+}  // extern "C"

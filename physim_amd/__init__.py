@@ -36,7 +36,12 @@ EXPORTS = [
     "pgn_model_element_egh", "pgn_linear_default_opts", "pgn_linear_solve",
     "pgn_newton_default_opts", "pgn_newton_solve", "pgn_time_kernel",
     "pgn_kernel_bytes", "pgn_host_bsr_pattern", "pgn_model_set_profiling",
-    "pgn_model_get_profile",
+    "pgn_model_get_profile", "pgn_rods_create", "pgn_rods_destroy",
+    "pgn_rods_num_dofs", "pgn_rods_set_fixed", "pgn_rods_set_gravity",
+    "pgn_rods_reset", "pgn_rods_set_x", "pgn_rods_get_x", "pgn_rods_get_energy",
+    "pgn_rods_get_gradient", "pgn_rods_get_hessian_dense", "pgn_rods_get_frames",
+    "pgn_rods_set_frames", "pgn_rods_update_kinematics", "pgn_rods_element_bt",
+    "pgn_rods_newton",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -132,6 +137,26 @@ def lib():
         L.pgn_kernel_bytes.argtypes = [_vp, ctypes.c_int, _dp]
         L.pgn_host_bsr_pattern.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _lp, _lp, _ip]
         L.pgn_model_set_profiling.argtypes = [_vp, ctypes.c_int]
+        L.pgn_rods_create.argtypes = [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
+                                      _dp, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.pgn_rods_destroy.argtypes = [_vp]
+        L.pgn_rods_destroy.restype = None
+        L.pgn_rods_num_dofs.argtypes = [_vp]
+        L.pgn_rods_num_dofs.restype = ctypes.c_long
+        L.pgn_rods_set_fixed.argtypes = [_vp, _ucp]
+        L.pgn_rods_set_gravity.argtypes = [_vp, _dp, _dp, ctypes.c_int]
+        L.pgn_rods_reset.argtypes = [_vp]
+        L.pgn_rods_set_x.argtypes = [_vp, _dp]
+        L.pgn_rods_get_x.argtypes = [_vp, _dp]
+        L.pgn_rods_get_energy.argtypes = [_vp, _dp]
+        L.pgn_rods_get_gradient.argtypes = [_vp, _dp]
+        L.pgn_rods_get_hessian_dense.argtypes = [_vp, ctypes.c_int, _dp]
+        L.pgn_rods_get_frames.argtypes = [_vp, _dp, _dp, _dp]
+        L.pgn_rods_set_frames.argtypes = [_vp, _dp, _dp, _dp]
+        L.pgn_rods_update_kinematics.argtypes = [_vp]
+        L.pgn_rods_element_bt.argtypes = [_vp, _dp, _dp, _dp]
+        L.pgn_rods_newton.argtypes = [_vp, ctypes.POINTER(NewtonOpts),
+                                      ctypes.POINTER(NewtonResult)]
         L.pgn_model_get_profile.argtypes = [_vp, ctypes.c_int, _lp, _dp]
         _LIB = L
     return _LIB
@@ -335,3 +360,104 @@ def host_bsr_pattern(n_nodes, T):
     _ok(lib().pgn_host_bsr_pattern(n_nodes, m, _p(T, _ip), ctypes.byref(nb),
                                    _p(rp, _lp), _p(col, _ip)))
     return rp, col
+
+
+class Rods:
+    """A batch of discrete elastic rods on the device (Simulable_DER)."""
+
+    def __init__(self, ctx, rods, frame0_normals, young, shear, radius_h, radius_w,
+                 density, precision=FP64):
+        nverts = np.array([len(r) for r in rods], dtype=np.int32)
+        X0 = _f64(np.concatenate([np.asarray(r).reshape(-1, 3) for r in rods]))
+        R = len(rods)
+        b = lambda a: _f64(np.broadcast_to(a, (R,)))
+        self.nverts = nverts
+        self.R = R
+        self.ctx = ctx
+        h = _vp()
+        _ok(lib().pgn_rods_create(ctx.handle, R, _p(nverts, _ip), _p(X0),
+                                  _p(_f64(np.broadcast_to(frame0_normals, (R, 3)))),
+                                  _p(b(young)), _p(b(shear)), _p(b(radius_h)),
+                                  _p(b(radius_w)), _p(b(density)), precision,
+                                  ctypes.byref(h)))
+        self._h = h
+        self.N = lib().pgn_rods_num_dofs(self._h)
+        self.offsets = np.concatenate([[0], np.cumsum(4 * nverts - 1)])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pgn_rods_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_fixed(self, mask):
+        mk = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        assert mk.size == self.N
+        _ok(lib().pgn_rods_set_fixed(self._h, _p(mk, _ucp)))
+
+    def set_gravity(self, g_end, stages, g_ini=(0.0, 0.0, 0.0)):
+        _ok(lib().pgn_rods_set_gravity(self._h, _p(_f64(g_ini)), _p(_f64(g_end)), int(stages)))
+
+    def reset(self):
+        _ok(lib().pgn_rods_reset(self._h))
+
+    def set_x(self, x):
+        _ok(lib().pgn_rods_set_x(self._h, _p(_f64(x).reshape(-1))))
+
+    def get_x(self):
+        x = np.zeros(self.N)
+        _ok(lib().pgn_rods_get_x(self._h, _p(x)))
+        return x
+
+    def energy(self):
+        e = np.zeros(self.R)
+        _ok(lib().pgn_rods_get_energy(self._h, _p(e)))
+        return e
+
+    def gradient(self):
+        g = np.zeros(self.N)
+        _ok(lib().pgn_rods_get_gradient(self._h, _p(g)))
+        return g
+
+    def hessian_dense(self, rod):
+        n = 4 * int(self.nverts[rod]) - 1
+        H = np.zeros((n, n))
+        _ok(lib().pgn_rods_get_hessian_dense(self._h, rod, _p(H)))
+        return H
+
+    def frames(self):
+        ne = int((self.nverts - 1).sum())
+        t = np.zeros((ne, 3))
+        n = np.zeros((ne, 3))
+        rt = np.zeros(int(self.nverts.sum()))
+        _ok(lib().pgn_rods_get_frames(self._h, _p(t), _p(n), _p(rt)))
+        return t, n, rt
+
+    def set_frames(self, tan, nor, rt):
+        _ok(lib().pgn_rods_set_frames(self._h, _p(_f64(tan).reshape(-1)),
+                                      _p(_f64(nor).reshape(-1)), _p(_f64(rt).reshape(-1))))
+
+    def update_kinematics(self):
+        _ok(lib().pgn_rods_update_kinematics(self._h))
+
+    def element_bt(self):
+        nv = int(self.nverts.sum())
+        E = np.zeros(nv)
+        g = np.zeros((nv, 8))
+        H = np.zeros((nv, 8, 8))
+        _ok(lib().pgn_rods_element_bt(self._h, _p(E), _p(g), _p(H)))
+        return E, g, H
+
+    def newton(self, **kw):
+        o = NewtonOpts()
+        lib().pgn_newton_default_opts(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        res = (NewtonResult * self.R)()
+        _ok(lib().pgn_rods_newton(self._h, ctypes.byref(o), res))
+        return [{"result": OS_NAMES[r.result], "iterations": r.iterations,
+                 "bisections": r.line_search_bisections, "energy_evals": r.energy_evals,
+                 "hessian_evals": r.hessian_evals, "objective": r.objective,
+                 "optimality": r.optimality, "seconds": r.seconds} for r in res]

@@ -14,8 +14,9 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpgn.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["tet_kernels.hip", "assembly_kernels.hip", "pcg_kernels.hip",
-           "model.cpp", "solver.cpp", "api.cpp"]
+SOURCES = ["rod_kernels.hip", "tet_kernels.hip", "assembly_kernels.hip",
+           "pcg_kernels.hip", "model.cpp", "solver.cpp", "api.cpp", "rod_model.cpp",
+           "rod_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -34,7 +35,7 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         results = list(ex.map(_compile, SOURCES))
     failed = False
     for src, _, rc, msgs in results:

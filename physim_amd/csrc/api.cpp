@@ -33,6 +33,18 @@ int guard(F&& f) {
 
 void use_device(const pgn_ctx* c) { PGN_HIP(hipSetDevice(c->device)); }
 
+}  // namespace
+
+namespace pgn {
+// error reporting shared by every extern "C" translation unit
+int api_guard_set(const char* msg, int code) {
+  g_err = msg;
+  return code;
+}
+}  // namespace pgn
+
+namespace {
+
 // unpack the 78-value element Hessian into a full 12x12 row-major matrix
 template <class T>
 void unpack_tet_h(const T* h, double* H) {

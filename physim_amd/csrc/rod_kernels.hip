@@ -1,0 +1,600 @@
+// rod_kernels.hip -- discrete elastic rods (DER) on gfx950.
+//
+// Layout: every rod is a chain of vertices; all per-element data is indexed
+// by vertex v.  Edge slot v is the edge (v, v+1) (valid unless v is the last
+// vertex of its rod), bend-twist slot v is the element centred at v (valid
+// for interior vertices).  DoFs are vertex-major [x, y, z, theta_v] where
+// theta_v is the material angle of edge v (the last vertex of a rod carries a
+// dummy angle that is held fixed), so a rod's Hessian is a symmetric band of
+// half-width 10.
+//
+// Elements (reference: EnergyElement_DER_Stretch.cpp:53-120,
+// EnergyElement_DER_BendTwist.cpp:67-249): the stretch energy is closed form;
+// the bend-twist energy is evaluated with forward-mode AD numbers (value,
+// gradient and Hessian over e1, e2, theta1, theta2) -- see hyperdual.hpp and
+// the model statement in rod_bt_energy below.
+#include "hyperdual.hpp"
+#include "reduce.hpp"
+#include "rods.hpp"
+
+namespace pgn {
+
+constexpr int ROD_BS = 128;
+
+// ---- bend-twist energy -------------------------------------------------------
+// t_i = e_i/|e_i|; d_i = PT(u_i -> t_i) n_i (frames of the iteration start);
+// a_i = Rot(t_i, theta_i) d_i; kb = t1 x t2 / (1 + t1.t2);
+// kappa = (kb.(t1 x a1 + t2 x a2), -kb.(a1 + a2));
+// phi = angle about t2 from d2 to Rot(t2, rt) PT(t1 -> t2) d1;
+// tau = theta2 - theta1 + rt - phi;
+// E = kb0 (kappa0 - K0)^2 + kb1 (kappa1 - K1)^2 + kt (tau - tau0)^2.
+template <class S>
+__device__ __forceinline__ V3<S> pt3(const V3<S>& a, const V3<S>& b, const V3<S>& v) {
+  const V3<S> c = cross3(a, b);
+  const S d = dot3(a, b);
+  using T = decltype(d.v);
+  const S k = dot3(c, v) * hd_recip(d + T(1));
+  return add3(add3(scale3(v, d), cross3(c, v)), scale3(c, k));
+}
+
+template <class S>
+__device__ __forceinline__ V3<S> rot3(const V3<S>& t, const S& c, const S& s, const V3<S>& v) {
+  const S tv = dot3(t, v);
+  using T = decltype(c.v);
+  const S omc = c * T(-1) + T(1);
+  return add3(add3(scale3(v, c), scale3(cross3(t, v), s)), scale3(t, tv * omc));
+}
+
+template <class T, bool WH>
+__device__ T rod_bt_energy(const T* xv, const RodBtConst& c, T* g8, T* h36) {
+  using S = HD<T, 8, WH>;
+  V3<S> e1, e2, n1, u1, n2, u2;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    e1.c[i] = hd_var<T, 8, WH>(xv[i], i);
+    e2.c[i] = hd_var<T, 8, WH>(xv[3 + i], 3 + i);
+    n1.c[i] = hd_const<T, 8, WH>((T)c.n1[i]);
+    u1.c[i] = hd_const<T, 8, WH>((T)c.u1[i]);
+    n2.c[i] = hd_const<T, 8, WH>((T)c.n2[i]);
+    u2.c[i] = hd_const<T, 8, WH>((T)c.u2[i]);
+  }
+  const S th1 = hd_var<T, 8, WH>(xv[6], 6), th2 = hd_var<T, 8, WH>(xv[7], 7);
+  const V3<S> t1 = scale3(e1, hd_recip(hd_sqrt(dot3(e1, e1))));
+  const V3<S> t2 = scale3(e2, hd_recip(hd_sqrt(dot3(e2, e2))));
+  const V3<S> d1 = pt3(u1, t1, n1), d2 = pt3(u2, t2, n2);
+  const V3<S> a1 = rot3(t1, hd_cos(th1), hd_sin(th1), d1);
+  const V3<S> a2 = rot3(t2, hd_cos(th2), hd_sin(th2), d2);
+  const V3<S> kb = scale3(cross3(t1, t2), hd_recip(dot3(t1, t2) + T(1)));
+  const S k0 = dot3(kb, add3(cross3(t1, a1), cross3(t2, a2)));
+  const S k1 = dot3(kb, add3(a1, a2)) * T(-1);
+  const V3<S> Q = rot3(t2, hd_const<T, 8, WH>((T)c.crt), hd_const<T, 8, WH>((T)c.srt),
+                       pt3(t1, t2, d1));
+  const S phi = hd_atan2(dot3(cross3(t2, d2), Q), dot3(d2, Q));
+  const S dk0 = k0 + (T)(-c.K0), dk1 = k1 + (T)(-c.K1);
+  const S dt = ((th2 - th1) - phi) + (T)(c.rt - c.tau0);
+  const S E = dk0 * dk0 * (T)c.kb0 + dk1 * dk1 * (T)c.kb1 + dt * dt * (T)c.kt;
+  if (g8)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) g8[i] = E.g[i];
+  if (WH && h36)
+#pragma unroll
+    for (int i = 0; i < 36; ++i) h36[i] = E.h[i];
+  return E.v;
+}
+
+__device__ __forceinline__ RodBtConst load_bt_const(const RodArgs& A, long v) {
+  RodBtConst c;
+  const double* fr = A.frame;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    c.u1[i] = fr[6 * (v - 1) + i];
+    c.n1[i] = fr[6 * (v - 1) + 3 + i];
+    c.u2[i] = fr[6 * v + i];
+    c.n2[i] = fr[6 * v + 3 + i];
+  }
+  const double* p = A.btc + 6 * v;  // kb0 kb1 kt K0 K1 tau0
+  c.kb0 = p[0];
+  c.kb1 = p[1];
+  c.kt = p[2];
+  c.K0 = p[3];
+  c.K1 = p[4];
+  c.tau0 = p[5];
+  c.rt = A.rt[v];
+  c.crt = cos(c.rt);
+  c.srt = sin(c.rt);
+  return c;
+}
+
+// ---- element kernels ---------------------------------------------------------
+// stretch: E = 1/2 k L0 (|e|/L0 - 1)^2 (k = E pi rh rw); gradient of the edge
+// vector (3, head = +, tail = -); Hessian block J (6 values, lower triangle).
+template <bool WH>
+__global__ __launch_bounds__(ROD_BS) void rod_stretch_kernel(RodArgs A, double* __restrict__ es,
+                                                              double* __restrict__ gs,
+                                                              double* __restrict__ Js) {
+  const long v = blockIdx.x * (long)ROD_BS + threadIdx.x;
+  if (v >= A.nv) return;
+  if (!(A.flags[v] & ROD_HAS_EDGE)) {
+    es[v] = 0.0;
+    return;
+  }
+  double e[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) e[i] = A.x[4 * (v + 1) + i] - A.x[4 * v + i];
+  const double l = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  const double L0 = A.L0[v], k = A.kst[v];
+  const double s = l / L0 - 1.0;
+  es[v] = 0.5 * k * L0 * s * s;
+  const double t[3] = {e[0] / l, e[1] / l, e[2] / l};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) gs[3 * v + i] = k * s * t[i];
+  if (WH) {
+    const double a = k / L0, b = k * s / l;
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j, ++q)
+        Js[6 * v + q] = a * t[i] * t[j] + b * ((i == j ? 1.0 : 0.0) - t[i] * t[j]);
+  }
+}
+
+template <class T, bool WH>
+__global__ __launch_bounds__(ROD_BS) void rod_bt_kernel(RodArgs A, double* __restrict__ eb,
+                                                         double* __restrict__ gb,
+                                                         T* __restrict__ Hb) {
+  const long v = blockIdx.x * (long)ROD_BS + threadIdx.x;
+  if (v >= A.nv) return;
+  if (!(A.flags[v] & ROD_HAS_BT)) {
+    eb[v] = 0.0;
+    return;
+  }
+  const RodBtConst c = load_bt_const(A, v);
+  T xv[8];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    xv[i] = (T)(A.x[4 * v + i] - A.x[4 * (v - 1) + i]);
+    xv[3 + i] = (T)(A.x[4 * (v + 1) + i] - A.x[4 * v + i]);
+  }
+  xv[6] = (T)A.x[4 * (v - 1) + 3];
+  xv[7] = (T)A.x[4 * v + 3];
+  T g8[8], h36[WH ? 36 : 1];
+  const T E = rod_bt_energy<T, WH>(xv, c, g8, WH ? h36 : nullptr);
+  eb[v] = (double)E;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) gb[8 * v + i] = (double)g8[i];
+  if (WH)
+#pragma unroll
+    for (int i = 0; i < 36; ++i) Hb[36 * v + i] = h36[i];
+}
+
+// ---- gradient gather + per-vertex energies --------------------------------------
+// DoF 4v+c (c < 3) gathers: stretch v-1 (head, +), stretch v (tail, -), and
+// the bend-twist elements centred at v+1 (as x_{c-1}: -de1), v (x_c:
+// +de1 - de2), v-1 (x_{c+1}: +de2); DoF 4v+3 (theta of edge v) gathers
+// bend-twist v (as theta2) and v+1 (as theta1).
+__global__ __launch_bounds__(ROD_BS) void rod_gather_kernel(RodArgs A, const double* __restrict__ es,
+                                                             const double* __restrict__ gs,
+                                                             const double* __restrict__ eb,
+                                                             const double* __restrict__ gb,
+                                                             double* __restrict__ g,
+                                                             double* __restrict__ vE) {
+  const long v = blockIdx.x * (long)ROD_BS + threadIdx.x;
+  if (v >= A.nv) return;
+  const unsigned f = A.flags[v];
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  if (f & ROD_HAS_PREV_EDGE)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] += gs[3 * (v - 1) + i];
+  if (f & ROD_HAS_EDGE)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] -= gs[3 * v + i];
+  if ((f & ROD_HAS_EDGE) && (A.flags[v + 1] & ROD_HAS_BT)) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] -= gb[8 * (v + 1) + i];
+    r[3] += gb[8 * (v + 1) + 6];
+  }
+  if (f & ROD_HAS_BT) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] += gb[8 * v + i] - gb[8 * v + 3 + i];
+    r[3] += gb[8 * v + 7];
+  }
+  if ((f & ROD_HAS_PREV_EDGE) && (A.flags[v - 1] & ROD_HAS_BT))
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] += gb[8 * (v - 1) + 3 + i];
+  const double m = A.mass[v];
+  double eload = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    r[i] += -m * A.grav[i];
+    eload += A.x[4 * v + i] * (-m * A.grav[i]);
+  }
+  const unsigned fx = A.fixmask[v];
+  double gg = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (fx & (1u << i)) r[i] = 0.0;
+    g[4 * v + i] = r[i];
+    gg += r[i] * r[i];
+  }
+  vE[3 * v] = es[v] + eb[v];
+  vE[3 * v + 1] = eload;
+  vE[3 * v + 2] = gg;
+}
+
+// one workgroup per rod: fixed-order sums of the per-vertex terms
+__global__ __launch_bounds__(256) void rod_reduce_kernel(RodArgs A, const double* __restrict__ vE,
+                                                          double* __restrict__ out) {
+  __shared__ double red[3 * 256 / 64 + 4];
+  const int r = blockIdx.x;
+  const long v0 = A.voff[r], v1 = A.voff[r + 1];
+  double el = 0.0, ld = 0.0, gg = 0.0;
+  for (long v = v0 + threadIdx.x; v < v1; v += 256) {
+    el += vE[3 * v];
+    ld += vE[3 * v + 1];
+    gg += vE[3 * v + 2];
+  }
+  block_sum2<256>(el, ld, red);
+  const double sg = block_sum<256>(gg, red);
+  if (threadIdx.x == 0) {
+    out[4 * r] = el + ld;
+    out[4 * r + 1] = sg;
+    out[4 * r + 2] = fabs(el) + fabs(ld);
+    out[4 * r + 3] = 0.0;
+  }
+}
+
+// ---- band Hessian assembly ---------------------------------------------------------
+// Entry (d, d-k), k = 0..ROD_BW, of row d.  Element contributions are summed
+// in a fixed order: stretch v-1, stretch v, bend-twist v-1, v, v+1.
+struct BtMap {  // one local DoF of a bend-twist element as +-variables
+  int var0, var1;
+  double s0, s1;
+};
+
+__device__ __forceinline__ bool bt_local(long d, long c, BtMap& m) {
+  // local DoFs of the element centred at vertex c, by global DoF
+  const long vd = d >> 2;
+  const int comp = (int)(d & 3);
+  m.var1 = -1;
+  m.s1 = 0.0;
+  if (comp < 3) {
+    if (vd == c - 1) {
+      m.var0 = comp;
+      m.s0 = -1.0;
+      return true;
+    }
+    if (vd == c) {
+      m.var0 = comp;
+      m.s0 = 1.0;
+      m.var1 = 3 + comp;
+      m.s1 = -1.0;
+      return true;
+    }
+    if (vd == c + 1) {
+      m.var0 = 3 + comp;
+      m.s0 = 1.0;
+      return true;
+    }
+    return false;
+  }
+  if (vd == c - 1) {
+    m.var0 = 6;
+    m.s0 = 1.0;
+    return true;
+  }
+  if (vd == c) {
+    m.var0 = 7;
+    m.s0 = 1.0;
+    return true;
+  }
+  return false;
+}
+
+template <class T>
+__device__ __forceinline__ double h36_at(const T* h, int i, int j) {
+  return (i >= j) ? (double)h[i * (i + 1) / 2 + j] : (double)h[j * (j + 1) / 2 + i];
+}
+
+template <class T>
+__global__ __launch_bounds__(ROD_BS) void rod_band_kernel(RodArgs A, const double* __restrict__ Js,
+                                                          const T* __restrict__ Hb,
+                                                          double* __restrict__ band) {
+  const long d = blockIdx.x * (long)ROD_BS + threadIdx.x;
+  if (d >= 4 * A.nv) return;
+  const long v = d >> 2;
+  const int comp = (int)(d & 3);
+  const long dof0 = 4L * A.voff[A.rod_of[v]];  // first DoF of this rod
+  double row[ROD_BW + 1];
+#pragma unroll
+  for (int k = 0; k <= ROD_BW; ++k) row[k] = 0.0;
+  const unsigned f = A.flags[v];
+  // stretch blocks: H6 = [[J, -J], [-J, J]] over (x_tail, x_head)
+  if (comp < 3) {
+    if (f & ROD_HAS_PREV_EDGE) {  // edge v-1: this vertex is the head
+      const double* J = Js + 6 * (v - 1);
+#pragma unroll
+      for (int c2 = 0; c2 < 3; ++c2) {
+        const int a = comp > c2 ? comp : c2, b = comp > c2 ? c2 : comp;
+        const double j = J[a * (a + 1) / 2 + b];
+        row[4 + comp - c2] -= j;                      // (x_v, x_{v-1})
+        if (c2 <= comp) row[comp - c2] += j;          // (x_v, x_v)
+      }
+    }
+    if (f & ROD_HAS_EDGE) {  // edge v: this vertex is the tail
+      const double* J = Js + 6 * v;
+#pragma unroll
+      for (int c2 = 0; c2 <= 2; ++c2)
+        if (c2 <= comp) row[comp - c2] += J[comp * (comp + 1) / 2 + c2];
+    }
+  }
+  // bend-twist elements centred at v-1, v, v+1
+#pragma unroll
+  for (int dc = -1; dc <= 1; ++dc) {
+    const long c = v + dc;
+    if (c < 0 || c >= A.nv) continue;
+    if (!(A.flags[c] & ROD_HAS_BT)) continue;
+    if (A.rod_of[c] != A.rod_of[v]) continue;
+    BtMap mr;
+    if (!bt_local(d, c, mr)) continue;
+    const T* h = Hb + 36 * c;
+#pragma unroll
+    for (int k = 0; k <= ROD_BW; ++k) {
+      const long dc2 = d - k;
+      if (dc2 < dof0) break;
+      BtMap mc;
+      if (!bt_local(dc2, c, mc)) continue;
+      double s = mr.s0 * mc.s0 * h36_at(h, mr.var0, mc.var0);
+      if (mc.var1 >= 0) s += mr.s0 * mc.s1 * h36_at(h, mr.var0, mc.var1);
+      if (mr.var1 >= 0) {
+        s += mr.s1 * mc.s0 * h36_at(h, mr.var1, mc.var0);
+        if (mc.var1 >= 0) s += mr.s1 * mc.s1 * h36_at(h, mr.var1, mc.var1);
+      }
+      row[k] += s;
+    }
+  }
+  // FixMatrixDoFs: fixed rows / columns become identity
+  const bool fr = (A.fixmask[v] >> comp) & 1u;
+#pragma unroll
+  for (int k = 0; k <= ROD_BW; ++k) {
+    const long dc2 = d - k;
+    double val = row[k];
+    if (dc2 < dof0) {
+      val = 0.0;
+    } else {
+      const bool fc = (A.fixmask[dc2 >> 2] >> (dc2 & 3)) & 1u;
+      if (fr || fc) val = (k == 0) ? 1.0 : 0.0;
+    }
+    band[(ROD_BW + 1) * d + k] = val;
+  }
+}
+
+// ---- banded LDL^T per rod (LinearSolver_EigenLDLT::SolveInternal) -------------------
+// One lane per rod.  Factorization without pivoting of band + shift, singular
+// test |D_jj| < sinT, indefiniteness test b^.x^ < -regT, accuracy test, and
+// the cumulative regularization loop.  work: (ROD_BW+1) values per DoF.
+__global__ __launch_bounds__(64) void rod_ldlt_kernel(RodArgs A, const double* __restrict__ band,
+                                                       double* __restrict__ work,
+                                                       const double* __restrict__ b,
+                                                       double* __restrict__ x, int reg_iters,
+                                                       double max_error, int* __restrict__ status,
+                                                       const unsigned char* __restrict__ active) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r >= A.nrods) return;
+  if (active && !active[r]) return;
+  const long d0 = 4L * A.voff[r], d1 = 4L * A.voff[r + 1];
+  const long N = d1 - d0;
+  constexpr int W = ROD_BW + 1;
+  // |A| over the entry set AMatrixSd stores: lower triangle + the upper
+  // entries of every vertex's 3x3 position block
+  double nA = 0.0;
+  for (long d = d0; d < d1; ++d) {
+    for (int k = 0; k < W; ++k) {
+      const double a = band[W * d + k];
+      nA += a * a;
+      const int comp = (int)(d & 3);
+      if (comp < 3 && k > 0 && k <= comp) nA += a * a;
+    }
+  }
+  nA = sqrt(nA);
+  double sinT = fmin(1e-6 * nA, 1e-6);
+  sinT = fmax(sinT, 1e-9);
+  const double regT = 1e-6 * nA;
+  int st = 1;
+  double nb = 0.0;
+  for (long d = d0; d < d1; ++d) nb += b[d] * b[d];
+  nb = sqrt(nb);
+  for (int it = 0; it < reg_iters && st != 0; ++it) {
+    double cum = 0.0;  // LinearSolver_EigenLDLT.cpp:68: A += regT 10^(i-1) I, cumulative
+    for (int q = 1; q <= it; ++q) cum += regT * pow(10.0, q - 1);
+    // factor: work holds L (unit lower, strictly-lower band) and D on k = 0
+    bool bad = false;
+    for (long j = 0; j < N && !bad; ++j) {
+      const long gj = d0 + j;
+      double dj = band[W * gj] + cum;
+      for (int k = 1; k < W && k <= j; ++k) {
+        const double l = work[W * gj + k];
+        dj -= l * l * work[W * (gj - k)];
+      }
+      if (!(fabs(dj) >= sinT)) {
+        bad = true;
+        break;
+      }
+      work[W * gj] = dj;
+      for (int i = 1; i < W && j + i < N; ++i) {
+        const long gi = gj + i;
+        // L(gi, gj) = (A(gi, gj) - sum_m L(gi, m) L(gj, m) D(m)) / dj
+        double s = band[W * gi + i];
+        for (int k = i + 1; k < W && (long)k <= j + i; ++k) {
+          // m = gi - k, needs gj - m = k - i < W
+          const long m = gi - k;
+          s -= work[W * gi + k] * work[W * gj + (k - i)] * work[W * m];
+        }
+        work[W * gi + i] = s / dj;
+      }
+    }
+    if (bad) continue;
+    // solve L y = b, D z = y, L^T x = z
+    for (long j = 0; j < N; ++j) {
+      const long gj = d0 + j;
+      double s = b[gj];
+      for (int k = 1; k < W && k <= j; ++k) s -= work[W * gj + k] * x[gj - k];
+      x[gj] = s;
+    }
+    for (long j = 0; j < N; ++j) x[d0 + j] /= work[W * (d0 + j)];
+    for (long j = N - 1; j >= 0; --j) {
+      const long gj = d0 + j;
+      double s = x[gj];
+      for (int k = 1; k < W && j + k < N; ++k) s -= work[W * (gj + k) + k] * x[gj + k];
+      x[gj] = s;
+    }
+    // indefiniteness: normalized(b).normalized(x) < -regT
+    double nx = 0.0, bx = 0.0;
+    for (long d = d0; d < d1; ++d) {
+      nx += x[d] * x[d];
+      bx += b[d] * x[d];
+    }
+    nx = sqrt(nx);
+    if (nb > 0.0 && nx > 0.0 && bx / (nb * nx) < -regT) continue;
+    // accuracy: |b - (A + shift) x| against maxError (abs and relative)
+    double ae = 0.0;
+    for (long d = d0; d < d1; ++d) {
+      double s = (band[W * d] + cum) * x[d];
+      for (int k = 1; k < W && d - k >= d0; ++k) s += band[W * d + k] * x[d - k];
+      for (int k = 1; k < W && d + k < d1; ++k) s += band[W * (d + k) + k] * x[d + k];
+      ae += (b[d] - s) * (b[d] - s);
+    }
+    ae = sqrt(ae);
+    if (ae > max_error && (nb > 0.0 ? ae / nb : ae) > max_error) continue;
+    st = 0;
+  }
+  status[r] = st;
+}
+
+// ---- reference frames and twist (UpdateKinematics) -----------------------------------
+// Mesh_DER::AdaptReferenceFrame: frame of edge v parallel-transported from its
+// old tangent to the current one (GeometryUtils.cpp:348-367).
+__device__ __forceinline__ void pt_d(const double* u, const double* t1, const double* t2,
+                                     double* r) {
+  const double d = t1[0] * t2[0] + t1[1] * t2[1] + t1[2] * t2[2];
+  const double b[3] = {t1[1] * t2[2] - t1[2] * t2[1], t1[2] * t2[0] - t1[0] * t2[2],
+                       t1[0] * t2[1] - t1[1] * t2[0]};
+  const double bu[3] = {b[1] * u[2] - b[2] * u[1], b[2] * u[0] - b[0] * u[2],
+                        b[0] * u[1] - b[1] * u[0]};
+  const double k = (b[0] * u[0] + b[1] * u[1] + b[2] * u[2]) / (1.0 + d);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) r[i] = u[i] * d + b[i] * k + bu[i];
+}
+
+__global__ __launch_bounds__(ROD_BS) void rod_frames_kernel(RodArgs A, double* __restrict__ frame) {
+  const long v = blockIdx.x * (long)ROD_BS + threadIdx.x;
+  if (v >= A.nv || !(A.flags[v] & ROD_HAS_EDGE)) return;
+  double t[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t[i] = A.x[4 * (v + 1) + i] - A.x[4 * v + i];
+  const double l = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t[i] /= l;
+  double told[3], nold[3], nn[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    told[i] = frame[6 * v + i];
+    nold[i] = frame[6 * v + 3 + i];
+  }
+  pt_d(nold, told, t, nn);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    frame[6 * v + i] = t[i];
+    frame[6 * v + 3 + i] = nn[i];
+  }
+}
+
+// Mesh_DER::ComputeReferenceTwist: rt += -(signed angle about t2 from n2 to
+// Rot(t2, rt) PT(t1 -> t2) n1) with the updated frames of edges v-1 and v.
+__global__ __launch_bounds__(ROD_BS) void rod_twist_kernel(RodArgs A, const double* __restrict__ frame,
+                                                           double* __restrict__ rt) {
+  const long v = blockIdx.x * (long)ROD_BS + threadIdx.x;
+  if (v >= A.nv || !(A.flags[v] & ROD_HAS_BT)) return;
+  const double* t1 = frame + 6 * (v - 1);
+  const double* n1 = t1 + 3;
+  const double* t2 = frame + 6 * v;
+  const double* n2 = t2 + 3;
+  double P[3];
+  pt_d(n1, t1, t2, P);
+  const double r0 = rt[v];
+  const double c = cos(r0), s = sin(r0);
+  const double tp = t2[0] * P[0] + t2[1] * P[1] + t2[2] * P[2];
+  const double txp[3] = {t2[1] * P[2] - t2[2] * P[1], t2[2] * P[0] - t2[0] * P[2],
+                         t2[0] * P[1] - t2[1] * P[0]};
+  double R[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) R[i] = c * P[i] + s * txp[i] + tp * (1.0 - c) * t2[i];
+  const double bn[3] = {t2[1] * n2[2] - t2[2] * n2[1], t2[2] * n2[0] - t2[0] * n2[2],
+                        t2[0] * n2[1] - t2[1] * n2[0]};
+  const double y = bn[0] * R[0] + bn[1] * R[1] + bn[2] * R[2];
+  const double xx = n2[0] * R[0] + n2[1] * R[1] + n2[2] * R[2];
+  rt[v] = r0 - atan2(y, xx);
+}
+
+// x = xsave + f[rod] * dx for rods in the line search; others untouched
+__global__ __launch_bounds__(256) void rod_step_kernel(long nd, const int* __restrict__ rod_of,
+                                                       const double* __restrict__ fac,
+                                                       const double* __restrict__ xsave,
+                                                       const double* __restrict__ dx,
+                                                       double* __restrict__ x) {
+  const long d = blockIdx.x * 256L + threadIdx.x;
+  if (d >= nd) return;
+  const double f = fac[rod_of[d >> 2]];
+  if (f >= 0.0) x[d] = xsave[d] + f * dx[d];
+}
+
+// ---- launchers ------------------------------------------------------------------------
+static inline unsigned blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void rod_launch_eval(hipStream_t s, const RodArgs& a, bool hess, bool fp32_hess, RodBufs& b) {
+  if (a.nv == 0) return;
+  const unsigned g = blocks(a.nv, ROD_BS);
+  (void)fp32_hess;  // rods evaluate their Hessians in FP64
+  if (hess) {
+    hipLaunchKernelGGL(rod_stretch_kernel<true>, dim3(g), dim3(ROD_BS), 0, s, a, b.es, b.gs, b.Js);
+    hipLaunchKernelGGL((rod_bt_kernel<double, true>), dim3(g), dim3(ROD_BS), 0, s, a, b.eb, b.gb,
+                       reinterpret_cast<double*>(b.Hb));
+  } else {
+    hipLaunchKernelGGL(rod_stretch_kernel<false>, dim3(g), dim3(ROD_BS), 0, s, a, b.es, b.gs, b.Js);
+    hipLaunchKernelGGL((rod_bt_kernel<double, false>), dim3(g), dim3(ROD_BS), 0, s, a, b.eb, b.gb,
+                       reinterpret_cast<double*>(b.Hb));
+  }
+  hipLaunchKernelGGL(rod_gather_kernel, dim3(g), dim3(ROD_BS), 0, s, a, b.es, b.gs, b.eb, b.gb,
+                     b.g, b.vE);
+  hipLaunchKernelGGL(rod_reduce_kernel, dim3(a.nrods), dim3(256), 0, s, a, b.vE, b.rodE);
+}
+
+void rod_launch_band(hipStream_t s, const RodArgs& a, bool fp32_hess, RodBufs& b) {
+  const unsigned g = blocks(4 * a.nv, ROD_BS);
+  (void)fp32_hess;
+  hipLaunchKernelGGL(rod_band_kernel<double>, dim3(g), dim3(ROD_BS), 0, s, a, b.Js,
+                     reinterpret_cast<const double*>(b.Hb), b.band);
+}
+
+void rod_launch_ldlt(hipStream_t s, const RodArgs& a, RodBufs& b, const double* rhs, double* x,
+                     int reg_iters, double max_error, const unsigned char* active) {
+  hipLaunchKernelGGL(rod_ldlt_kernel, dim3(blocks(a.nrods, 64)), dim3(64), 0, s, a, b.band, b.work,
+                     rhs, x, reg_iters, max_error, b.status, active);
+}
+
+void rod_launch_kinematics(hipStream_t s, const RodArgs& a, double* frame, double* rt) {
+  const unsigned g = blocks(a.nv, ROD_BS);
+  hipLaunchKernelGGL(rod_frames_kernel, dim3(g), dim3(ROD_BS), 0, s, a, frame);
+  RodArgs a2 = a;
+  a2.frame = frame;
+  hipLaunchKernelGGL(rod_twist_kernel, dim3(g), dim3(ROD_BS), 0, s, a2, frame, rt);
+}
+
+void rod_launch_step(hipStream_t s, const RodArgs& a, const double* fac, const double* xsave,
+                     const double* dx, double* x) {
+  const long nd = 4 * a.nv;
+  hipLaunchKernelGGL(rod_step_kernel, dim3(blocks(nd, 256)), dim3(256), 0, s, nd, a.rod_of,
+                     fac, xsave, dx, x);
+}
+
+}  // namespace pgn

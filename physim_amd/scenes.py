@@ -109,3 +109,68 @@ def config2(seed=2):
     node jitter."""
     return cantilever(dims=(40, 40, 160), young_sigma=0.3, jitter=0.05, seed=seed,
                       name="beam_40x40x160_conh_lognormal")
+
+
+# ---- discrete elastic rods -----------------------------------------------------
+
+def helix(n, radius, pitch, seg_angle=2 * np.pi / 24):
+    """n vertices on a helix about z; consecutive vertices seg_angle apart."""
+    t = np.arange(n) * seg_angle
+    return np.stack([radius * np.cos(t), radius * np.sin(t), pitch * t / (2 * np.pi)], axis=1)
+
+
+class RodScene:
+    """Inputs of a batch of clamped rods under gravity."""
+
+    def __init__(self, rods, young, shear, radius, density, stages, name):
+        self.rods = rods
+        self.young = young
+        self.shear = shear
+        self.radius = radius
+        self.density = density
+        self.stages = stages
+        self.name = name
+
+    @property
+    def nverts(self):
+        return np.array([len(r) for r in self.rods])
+
+    def frame0_normals(self):
+        out = []
+        for r in self.rods:
+            t = r[1] - r[0]
+            a = np.array([0.0, 0.0, 1.0]) if abs(t[2]) < 0.9 * np.linalg.norm(t) else \
+                np.array([1.0, 0.0, 0.0])
+            out.append(a)
+        return np.array(out)
+
+    def fixed_mask(self):
+        """First two vertices and the first edge angle of every rod."""
+        parts = []
+        for r in self.rods:
+            n = len(r)
+            m = np.zeros(4 * n - 1, dtype=np.uint8)
+            m[:6] = 1
+            m[3 * n] = 1
+            parts.append(m)
+        return np.concatenate(parts)
+
+    def bbox_diags(self):
+        return np.array([np.linalg.norm(r.max(0) - r.min(0)) for r in self.rods])
+
+
+def rod_batch(n_rods, n_min=50, n_max=500, seed=4, stages=10, young=1e7,
+              radius=1e-3, density=None, name=None):
+    """Helices with vertex counts U{n_min..n_max}, radius U(0.01, 0.05) m,
+    pitch U(0.005, 0.05) m, E = 1e7 Pa, G = E/3, circular section r = 1e-3 m;
+    density is kg per metre of rod (default: 1000 kg/m^3 x pi r^2)."""
+    rng = np.random.default_rng(seed)
+    rods = []
+    for _ in range(n_rods):
+        n = int(rng.integers(n_min, n_max + 1))
+        R = rng.uniform(0.01, 0.05)
+        p = rng.uniform(0.005, 0.05)
+        rods.append(helix(n, R, p))
+    dens = 1000.0 * np.pi * radius ** 2 if density is None else density
+    return RodScene(rods, young, young / 3.0, radius, dens, stages,
+                    name or f"helices_{n_rods}")
