@@ -71,6 +71,39 @@ def cpu_baseline(scene, counts):
                        f"{t_solve:.1f}s per solve")}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel`, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration
+    recorded there), or None when no summary holds it."""
+    path = os.path.join(ROOT, "profiles", "pcg_k1_beam40x40x160_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def kernels_only(M, scene, opts):
+    """A short, fixed sequence of the hot kernels at the workload's real
+    shape, for hardware-counter (rocprofv3 --pmc) runs: the PCG SpMV
+    (pcg_k1_kernel) launches of one capped solve, the fused element kernel,
+    and the 256 MiB calibration copy whose counters scale the others."""
+    M.reset()
+    while M.step_load():
+        pass
+    g = M.gradient()
+    _, r = M.linear_solve(-g, None, tol=1e-30, max_iters=50, reg_iters=1,
+                          precond=opts["precond"])
+    log(f"kernels-only: PCG {r}")
+    for _ in range(5):
+        M.time_kernel(0, 1)
+        M.time_kernel(7, 1)
+    log("kernels-only: done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +117,10 @@ def main():
                     help="1 (default): Eisenstat-Walker linear tolerance")
     ap.add_argument("--sweep", action="store_true",
                     help="also time one solve per (precond, adaptive) variant")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="for hardware-counter runs: at full load, one Hessian, one "
+                         "PCG solve capped at 50 iterations, 5 E+g+H launches and 5 "
+                         "calibration copies; no solve timing, no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and "
@@ -103,6 +140,12 @@ def main():
     scene.apply(M)
     log(f"setup {time.perf_counter() - t0:.1f}s: {scene.m} tets, {3 * scene.n} DoF, "
         f"{M.device_bytes() / 2**20:.0f} MiB on device")
+
+    if args.kernels_only:
+        kernels_only(M, scene, opts)
+        M.close()
+        ctx.close()
+        return
 
     for w in range(args.warmup):
         M.reset()
@@ -220,7 +263,7 @@ def main():
                      "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "launches": k1_n, "avg_us": spmv_ms * 1e3,
+                     "traffic": measured_traffic("pcg_k1_kernel"), "launches": k1_n, "avg_us": spmv_ms * 1e3,
                      "bytes_per_launch": spmv_bytes},
         "cpu_baseline": cpu,
         "spmv_autotune_us": {str(l): M.profile(2 + c)[1] for c, l in enumerate((4, 8, 16))},
