@@ -80,6 +80,23 @@ int pgn_model_create_tet(pgn_ctx* ctx, int n_nodes, const double* X0,
                          const unsigned char* material, const double* lame1,
                          const double* lame2, const double* density,
                          int precision, pgn_model** out);
+/* Thin shell (Simulable_ThinShell, Simulable_ThinShell.cpp:53-120) on one
+ * node set: StVK membrane triangles (EnergyElement_FEM on Tri3 with
+ * MaterialModel_2Din3D_StVK, MaterialModels.h:376-440; energy = rest area x
+ * thickness x Psi) and discrete-shell hinges (EnergyElement_DiscreteShells,
+ * EnergyElement_DiscreteShells.cpp:22-198; E = kB shape0 (theta - theta0)^2,
+ * SPD Hessian).  F: 3 node ids per triangle; lame1/lame2/thickness/density per
+ * triangle (node masses density * area / 3, MassElement_Lumped with the
+ * triangle's area as volume basis); hinges: 4 node ids each (the hinge edge
+ * v0 v1, then the vertex opposite it in the face where the edge runs v1 -> v0,
+ * then the vertex opposite it in the other face: the half-edge order of
+ * EnergyElement_DiscreteShells.cpp:30-34); bending_k: kB per hinge
+ * (Mat_BendingK).  Rest dihedral and shape factor come from X0. */
+int pgn_model_create_shell(pgn_ctx* ctx, int n_nodes, const double* X0, int n_tris,
+                           const int* F, const double* lame1, const double* lame2,
+                           const double* thickness, const double* density,
+                           int n_hinges, const int* hinges, const double* bending_k,
+                           int precision, pgn_model** out);
 void pgn_model_destroy(pgn_model* model);
 int pgn_model_num_dofs(const pgn_model* model);
 int pgn_model_num_elements(const pgn_model* model);
@@ -120,9 +137,11 @@ int pgn_model_hessian_pattern(pgn_model* model, int layout, long* rowptr,
  * Values in the model's Hessian precision, widened to FP64 (nnz*9 for BSR3) */
 int pgn_model_get_hessian(pgn_model* model, int layout, double* values);
 /* IEnergyElement::GetLocalEnergy/Gradient/Hessian for every element at the
- * current x, from the fused E+g+H element kernel of the given precision
- * (PGN_FP64 or PGN_MIXED = all-FP32 kernel).  Any pointer may be NULL.
- * E: m values; g: 12 per element; H: full 12x12 row-major per element. */
+ * current x, from the fused E+g+H element kernels of the given precision
+ * (PGN_FP64 or PGN_MIXED = all-FP32 kernels).  Any pointer may be NULL.
+ * E: m values; g: 12 per element; H: full 12x12 row-major per element, over
+ * the element's four connectivity slots (shell models: triangles first, each
+ * with zeros in its fourth slot, then hinges). */
 int pgn_model_element_egh(pgn_model* model, int precision, double* E,
                           double* g, double* H);
 

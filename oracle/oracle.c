@@ -303,7 +303,8 @@ struct orc_model {
   double *lame1, *lame2, *rho;
   double* DFDx; /* 108 per tet */
   double* V0;
-  double* elem_mass; /* rho * V0 / 4 per node of the element */
+  double* elem_mass; /* lumped mass per node of the element */
+  double* sp;        /* 8 per element: membrane Dm^-1 (4), A0 t; hinge kB, theta0, shape0 */
   unsigned char* fixed; /* 3n */
   double* x;
   /* gravity BC */
@@ -322,10 +323,21 @@ struct orc_model {
   long* slot; /* per element 144 local entries -> CSR slot or -1 */
 };
 
+/* nodes of an element that carry mass and load: tets 4, membranes 3 (their
+ * fourth connectivity slot repeats node 2), hinges none */
+static int mass_nodes(int kind) { return kind <= 1 ? 4 : (kind == 2 ? 3 : 0); }
+
 orc_model* orc_model_create(int n_nodes, const double* X0, int n_tets,
                             const int* T, const unsigned char* kind,
                             const double* lame1, const double* lame2,
                             const double* rho) {
+  return orc_model_create_mixed(n_nodes, X0, n_tets, T, kind, lame1, lame2, rho, NULL);
+}
+
+orc_model* orc_model_create_mixed(int n_nodes, const double* X0, int n_tets,
+                                  const int* T, const unsigned char* kind,
+                                  const double* lame1, const double* lame2,
+                                  const double* rho, const double* extra) {
   orc_model* M = (orc_model*)calloc(1, sizeof(orc_model));
   M->n = n_nodes;
   M->m = n_tets;
@@ -346,6 +358,7 @@ orc_model* orc_model_create(int n_nodes, const double* X0, int n_tets,
   M->DFDx = (double*)malloc(sizeof(double) * 108 * n_tets);
   M->V0 = (double*)malloc(sizeof(double) * n_tets);
   M->elem_mass = (double*)malloc(sizeof(double) * n_tets);
+  M->sp = (double*)calloc(8 * (size_t)(n_tets > 0 ? n_tets : 1), sizeof(double));
   M->fixed = (unsigned char*)calloc(3 * n_nodes, 1);
   M->eE = (double*)malloc(sizeof(double) * n_tets);
   M->eg = (double*)malloc(sizeof(double) * 12 * n_tets);
@@ -356,9 +369,26 @@ orc_model* orc_model_create(int n_nodes, const double* X0, int n_tets,
     double Xe[12];
     for (int a = 0; a < 4; ++a)
       for (int i = 0; i < 3; ++i) Xe[3 * a + i] = X0[3 * T[4 * e + a] + i];
-    orc_tet_rest(Xe, M->DFDx + 108 * e, M->V0 + e);
-    /* MassElement_Lumped.cpp:47-54: rho * VolumeBasis(Position_0) / nodes */
-    M->elem_mass[e] = rho[e] * M->V0[e] / 4.0;
+    if (kind[e] <= 1) {
+      orc_tet_rest(Xe, M->DFDx + 108 * e, M->V0 + e);
+      /* MassElement_Lumped.cpp:47-54: rho * VolumeBasis(Position_0) / nodes */
+      M->elem_mass[e] = rho[e] * M->V0[e] / 4.0;
+    } else if (kind[e] == 2) {
+      /* membrane: extra[e] = thickness; VolumeBasis of a triangle is its
+       * area (Face_Tri.cpp:33-39), with no thickness factor */
+      double A;
+      orc_membrane_rest(Xe, M->sp + 8 * e, &A);
+      M->sp[8 * e + 4] = A * extra[e];
+      M->V0[e] = A;
+      M->elem_mass[e] = rho[e] * A / 3.0;
+    } else {
+      /* hinge: EnergyElement_DiscreteShells::Init (:50-54); extra[e] = kB */
+      M->sp[8 * e] = extra[e];
+      M->sp[8 * e + 1] = orc_hinge_theta(Xe);
+      M->sp[8 * e + 2] = orc_hinge_shape(Xe);
+      M->V0[e] = 0.0;
+      M->elem_mass[e] = 0.0;
+    }
   }
   /* Lower pattern as AMatrixSd stores it: node block (a, b) with a > b is
    * stored whole below the diagonal; node block (a, a) is stored whole
@@ -467,6 +497,7 @@ void orc_model_destroy(orc_model* M) {
   free(M->DFDx);
   free(M->V0);
   free(M->elem_mass);
+  free(M->sp);
   free(M->fixed);
   free(M->f_ini);
   free(M->f_end);
@@ -538,7 +569,7 @@ void orc_model_get_x(const orc_model* M, double* x) {
 void orc_model_get_mass(const orc_model* M, double* mass3n) {
   memset(mass3n, 0, sizeof(double) * 3 * M->n);
   for (int e = 0; e < M->m; ++e)
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < mass_nodes(M->kind[e]); ++a)
       for (int i = 0; i < 3; ++i) mass3n[3 * M->T[4 * e + a] + i] += M->elem_mass[e];
 }
 
@@ -562,13 +593,43 @@ static void gather_x(const orc_model* M, int e, double xe[12]) {
     for (int i = 0; i < 3; ++i) xe[3 * a + i] = M->x[3 * M->T[4 * e + a] + i];
 }
 
+/* IEnergyElement::ComputeAndStore_* of element e: 12-vector / 12x12 over its
+ * four connectivity slots (a membrane's fourth slot gets zeros) */
+static void elem_egh(const orc_model* M, int e, const double xe[12], double* E,
+                     double* g, double* H) {
+  const int k = M->kind[e];
+  if (k <= 1) {
+    orc_tet_egh(xe, M->DFDx + 108 * e, M->V0[e], k, M->lame1[e], M->lame2[e], E, g, H);
+    return;
+  }
+  if (k == 3) {
+    const double* p = M->sp + 8 * e;
+    const double en = orc_hinge_egh(xe, p[0], p[1], p[2], g, H);
+    if (E) *E = en;
+    return;
+  }
+  const double* p = M->sp + 8 * e;
+  double g9[9], H81[81];
+  const double en = orc_membrane_egh(xe, p, p[4], M->lame1[e], M->lame2[e], g ? g9 : NULL,
+                                     H ? H81 : NULL);
+  if (E) *E = en;
+  if (g) {
+    memset(g, 0, 12 * sizeof(double));
+    memcpy(g, g9, 9 * sizeof(double));
+  }
+  if (H) {
+    memset(H, 0, 144 * sizeof(double));
+    for (int i = 0; i < 9; ++i)
+      for (int j = 0; j < 9; ++j) H[12 * i + j] = H81[9 * i + j];
+  }
+}
+
 double orc_model_energy(orc_model* M) {
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < M->m; ++e) {
     double xe[12];
     gather_x(M, e, xe);
-    orc_tet_egh(xe, M->DFDx + 108 * e, M->V0[e], M->kind[e], M->lame1[e],
-                M->lame2[e], M->eE + e, NULL, NULL);
+    elem_egh(M, e, xe, M->eE + e, NULL, NULL);
   }
   double E = 0;
   for (int e = 0; e < M->m; ++e) E += M->eE[e];
@@ -579,7 +640,7 @@ double orc_model_energy(orc_model* M) {
   double Eload = 0;
   for (int e = 0; e < M->m; ++e) {
     double Ee = 0;
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < mass_nodes(M->kind[e]); ++a)
       for (int i = 0; i < 3; ++i)
         Ee += M->x[3 * M->T[4 * e + a] + i] * (-M->elem_mass[e] * g[i]);
     E += Ee;
@@ -601,8 +662,7 @@ void orc_model_gradient(orc_model* M, double* gout) {
   for (int e = 0; e < M->m; ++e) {
     double xe[12];
     gather_x(M, e, xe);
-    orc_tet_egh(xe, M->DFDx + 108 * e, M->V0[e], M->kind[e], M->lame1[e],
-                M->lame2[e], NULL, M->eg + 12 * e, NULL);
+    elem_egh(M, e, xe, NULL, M->eg + 12 * e, NULL);
   }
   const int N = 3 * M->n;
   memset(gout, 0, sizeof(double) * N);
@@ -612,7 +672,7 @@ void orc_model_gradient(orc_model* M, double* gout) {
   double g[3];
   cur_gravity(M, g);
   for (int e = 0; e < M->m; ++e)
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < mass_nodes(M->kind[e]); ++a)
       for (int i = 0; i < 3; ++i) gout[3 * M->T[4 * e + a] + i] += -M->elem_mass[e] * g[i];
   if (M->has_f)
     for (int k = 0; k < N; ++k) gout[k] += cur_force(M, k);
@@ -631,8 +691,7 @@ void orc_model_hessian_lower(orc_model* M, long* rowptr, int* col,
   for (int e = 0; e < M->m; ++e) {
     double xe[12];
     gather_x(M, e, xe);
-    orc_tet_egh(xe, M->DFDx + 108 * e, M->V0[e], M->kind[e], M->lame1[e],
-                M->lame2[e], NULL, NULL, M->eH + 144 * e);
+    elem_egh(M, e, xe, NULL, NULL, M->eH + 144 * e);
   }
   const int N = 3 * M->n;
   memcpy(rowptr, M->rowptr, sizeof(long) * (N + 1));
@@ -657,9 +716,7 @@ void orc_model_element_egh(orc_model* M, double* E, double* g, double* H) {
   for (int e = 0; e < M->m; ++e) {
     double xe[12];
     gather_x(M, e, xe);
-    orc_tet_egh(xe, M->DFDx + 108 * e, M->V0[e], M->kind[e], M->lame1[e],
-                M->lame2[e], E ? E + e : NULL, g ? g + 12L * e : NULL,
-                H ? H + 144L * e : NULL);
+    elem_egh(M, e, xe, E ? E + e : NULL, g ? g + 12L * e : NULL, H ? H + 144L * e : NULL);
   }
 }
 

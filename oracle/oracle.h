@@ -62,6 +62,15 @@ orc_model* orc_model_create(int n_nodes, const double* X0, int n_tets,
                             const int* T, const unsigned char* kind,
                             const double* lame1, const double* lame2,
                             const double* rho);
+/* Mixed element kinds over one node set: kind 0 CoNH tet, 1 StVK tet,
+ * 2 StVK membrane triangle (T row = v0 v1 v2 v2, extra = thickness, rho per
+ * unit area), 3 discrete-shell hinge (T row = edge v0 v1, opposite v2 v3,
+ * extra = bending stiffness kB; lame / rho unused).  extra may be NULL when
+ * every element is a tet. */
+orc_model* orc_model_create_mixed(int n_nodes, const double* X0, int n_elems,
+                                  const int* T, const unsigned char* kind,
+                                  const double* lame1, const double* lame2,
+                                  const double* rho, const double* extra);
 void orc_model_destroy(orc_model* m);
 int orc_model_num_dofs(const orc_model* m);
 void orc_model_set_fixed(orc_model* m, const unsigned char* mask);
@@ -174,6 +183,22 @@ void orc_newton_generic(orc_problem* P, const orc_newton_opts* o,
  * OptimSolver_USQP_LS::SolveStepInternal and OptimProblem_BasicStatic. */
 void orc_newton_solve(orc_model* m, const orc_newton_opts* o,
                       orc_newton_result* r);
+
+/* ---- thin shells (shells.c) -------------------------------------------- */
+/* hinge over x = [x0 x1 x2 x3] (edge x0-x1, opposite x2 and x3) */
+double orc_hinge_theta(const double x[12]);
+double orc_hinge_shape(const double x[12]);
+void orc_hinge_dtheta(const double x[12], double d[12]);
+double orc_hinge_egh(const double x[12], double kB, double theta0, double shape0,
+                     double* g12, double* H144);
+/* membrane triangle: rest Dm^-1 (2x2 row-major) and area; E/g/H over 9 DoFs */
+void orc_membrane_rest(const double X[9], double Dminv[4], double* area);
+double orc_membrane_egh(const double x[9], const double Dminv[4], double A0t,
+                        double lame1, double lame2, double* g9, double* H81);
+/* 2D StVK over a row-major 3x2 F (FEMMem_StVK_*.mcg layout) */
+double orc_mem_psi(const double* Fv, double lame1, double lame2);
+void orc_mem_pk1(const double* Fv, double lame1, double lame2, double* P);
+void orc_mem_dpdf(const double* Fv, double lame1, double lame2, double* C);
 
 /* ---- discrete elastic rods (rods.c) ------------------------------------ */
 /* bend-twist element: prm = shear[2], young[2], radw[2], radh[2], L0, L1,

@@ -94,6 +94,24 @@ def lib():
         L.orc_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
         L.orc_newton_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                        ctypes.POINTER(NewtonResult)]
+        L.orc_model_create_mixed.restype = ctypes.c_void_p
+        L.orc_model_create_mixed.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _ucp,
+                                             _dp, _dp, _dp, _dp]
+        for f in ("orc_hinge_theta", "orc_hinge_shape"):
+            getattr(L, f).restype = ctypes.c_double
+            getattr(L, f).argtypes = [_dp]
+        L.orc_hinge_dtheta.argtypes = [_dp, _dp]
+        L.orc_hinge_egh.restype = ctypes.c_double
+        L.orc_hinge_egh.argtypes = [_dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    _dp, _dp]
+        L.orc_membrane_rest.argtypes = [_dp, _dp, _dp]
+        L.orc_membrane_egh.restype = ctypes.c_double
+        L.orc_membrane_egh.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, _dp, _dp]
+        L.orc_mem_psi.restype = ctypes.c_double
+        L.orc_mem_psi.argtypes = [_dp, ctypes.c_double, ctypes.c_double]
+        L.orc_mem_pk1.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_mem_dpdf.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_model_energy_scale.restype = ctypes.c_double
         L.orc_model_energy_scale.argtypes = [ctypes.c_void_p]
         L.orc_rod_bt.restype = ctypes.c_double
@@ -199,7 +217,9 @@ class Model:
     """Simulable_FEM (Tet4 + lumped mass) with hard-fixed DoFs, gravity and
     nodal-force boundary conditions, restated on the CPU in FP64."""
 
-    def __init__(self, X0, T, kind, lame1, lame2, rho):
+    def __init__(self, X0, T, kind, lame1, lame2, rho, extra=None):
+        """extra: per-element thickness (membranes, kind 2) or bending
+        stiffness (hinges, kind 3); None for tet-only models."""
         X0 = f64(X0).reshape(-1)
         T = np.ascontiguousarray(T, dtype=np.int32).reshape(-1)
         m = T.size // 4
@@ -210,8 +230,14 @@ class Model:
         self.n = X0.size // 3
         self.m = m
         self.N = 3 * self.n
-        self._h = lib().orc_model_create(self.n, _p(X0), m, _p(T, _ip),
-                                         _p(kind, _ucp), _p(l1), _p(l2), _p(rh))
+        if extra is None:
+            self._h = lib().orc_model_create(self.n, _p(X0), m, _p(T, _ip),
+                                             _p(kind, _ucp), _p(l1), _p(l2), _p(rh))
+        else:
+            ex = f64(np.broadcast_to(extra, (m,)))
+            self._h = lib().orc_model_create_mixed(self.n, _p(X0), m, _p(T, _ip),
+                                                   _p(kind, _ucp), _p(l1), _p(l2), _p(rh),
+                                                   _p(ex))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -413,3 +439,57 @@ def dense_ldlt_solve(A, b, x, max_error, reg_iters):
     st = lib().orc_dense_ldlt_solve(A.shape[0], _p(A), _p(f64(b)), _p(x), max_error,
                                     int(reg_iters), ctypes.byref(reg))
     return st, reg.value
+
+
+# ---- thin shells ---------------------------------------------------------------
+
+def hinge_theta(x):
+    return lib().orc_hinge_theta(_p(f64(x).reshape(-1)))
+
+
+def hinge_shape(x):
+    return lib().orc_hinge_shape(_p(f64(x).reshape(-1)))
+
+
+def hinge_dtheta(x):
+    d = np.zeros(12)
+    lib().orc_hinge_dtheta(_p(f64(x).reshape(-1)), _p(d))
+    return d
+
+
+def hinge_egh(x, kB, theta0, shape0):
+    g = np.zeros(12)
+    H = np.zeros(144)
+    E = lib().orc_hinge_egh(_p(f64(x).reshape(-1)), kB, theta0, shape0, _p(g), _p(H))
+    return E, g, H.reshape(12, 12)
+
+
+def membrane_rest(X):
+    D = np.zeros(4)
+    A = np.zeros(1)
+    lib().orc_membrane_rest(_p(f64(X).reshape(-1)), _p(D), _p(A))
+    return D, A[0]
+
+
+def membrane_egh(x, Dminv, A0t, lame1, lame2):
+    g = np.zeros(9)
+    H = np.zeros(81)
+    E = lib().orc_membrane_egh(_p(f64(x).reshape(-1)), _p(f64(Dminv)), A0t, lame1, lame2,
+                               _p(g), _p(H))
+    return E, g, H.reshape(9, 9)
+
+
+def mem_psi(Fv, l1, l2):
+    return lib().orc_mem_psi(_p(f64(Fv)), l1, l2)
+
+
+def mem_pk1(Fv, l1, l2):
+    P = np.zeros(6)
+    lib().orc_mem_pk1(_p(f64(Fv)), l1, l2, _p(P))
+    return P
+
+
+def mem_dpdf(Fv, l1, l2):
+    C = np.zeros(36)
+    lib().orc_mem_dpdf(_p(f64(Fv)), l1, l2, _p(C))
+    return C.reshape(6, 6)

@@ -115,3 +115,66 @@ double ref_reference_twist_change(double* t1, double* n1, double* t2, double* n2
 }
 
 }  // extern "C"
+
+// ---- discrete shells --------------------------------------------------------
+// DiscreteShellsBending_*.mcg as called by EnergyElement_DiscreteShells.cpp
+// (:56-198): nodes x0, x1 (the hinge edge), x2, x3 (the opposite vertices),
+// kB, theta0, shape0.
+extern "C" {
+
+double ref_ds_energy(const double* x0, const double* x1, const double* x2,
+                     const double* x3, double kB, double theta0, double shape0) {
+#include "DiscreteShellsBending_Energy.mcg"
+  return t105;
+}
+
+void ref_ds_gradient(const double* x0, const double* x1, const double* x2,
+                     const double* x3, double kB, double theta0, double shape0,
+                     double* out) {
+  double vg[12];
+#include "DiscreteShellsBending_Gradient.mcg"
+  for (int i = 0; i < 12; ++i) out[i] = vg[i];
+}
+
+void ref_ds_hessian_spd(const double* x0, const double* x1, const double* x2,
+                        const double* x3, double kB, double theta0, double shape0,
+                        double* out) {
+  (void)theta0;
+  double mHspd[12][12];
+#include "DiscreteShellsBending_HessianSPD.mcg"
+  for (int i = 0; i < 12; ++i)
+    for (int j = 0; j < 12; ++j) out[12 * i + j] = mHspd[i][j];
+}
+
+double ref_ds_shape(const double* x0, const double* x1, const double* x2,
+                    const double* x3) {
+#include "DiscreteShellsBending_Shape.mcg"
+  return t46;
+}
+
+double ref_ds_dihedral(const double* x0, const double* x1, const double* x2,
+                       const double* x3) {
+#include "DiscreteShellsBending_Dihedral.mcg"
+  return t101;
+}
+
+// FEMMem_StVK_*.mcg (MaterialModels.h:376-440): Fv = row-major 3x2 F.
+double ref_mem_stvk_energy(const double* Fv, double lame1, double lame2) {
+#include "FEMMem_StVK_Energy.mcg"
+  return t32;
+}
+
+void ref_mem_stvk_pk1(const double* Fv, double lame1, double lame2, double* out) {
+  double vgx[6];
+#include "FEMMem_StVK_Gradient.mcg"
+  for (int i = 0; i < 6; ++i) out[i] = vgx[i];
+}
+
+void ref_mem_stvk_dpdf(const double* Fv, double lame1, double lame2, double* out) {
+  double mHx[6][6];
+#include "FEMMem_StVK_Hessian.mcg"
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) out[6 * i + j] = mHx[i][j];
+}
+
+}  // extern "C"

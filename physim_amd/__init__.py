@@ -6,6 +6,7 @@ tests and benchmarks; it has no compute of its own and raises if the library
 is missing or no gfx950 device is present.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -36,7 +37,7 @@ EXPORTS = [
     "pgn_model_element_egh", "pgn_linear_default_opts", "pgn_linear_solve",
     "pgn_newton_default_opts", "pgn_newton_solve", "pgn_time_kernel",
     "pgn_kernel_bytes", "pgn_host_bsr_pattern", "pgn_model_set_profiling",
-    "pgn_model_get_profile", "pgn_rods_create", "pgn_rods_destroy",
+    "pgn_model_get_profile", "pgn_model_create_shell", "pgn_rods_create", "pgn_rods_destroy",
     "pgn_rods_num_dofs", "pgn_rods_set_fixed", "pgn_rods_set_gravity",
     "pgn_rods_reset", "pgn_rods_set_x", "pgn_rods_get_x", "pgn_rods_get_energy",
     "pgn_rods_get_gradient", "pgn_rods_get_hessian_dense", "pgn_rods_get_frames",
@@ -137,6 +138,9 @@ def lib():
         L.pgn_kernel_bytes.argtypes = [_vp, ctypes.c_int, _dp]
         L.pgn_host_bsr_pattern.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _lp, _lp, _ip]
         L.pgn_model_set_profiling.argtypes = [_vp, ctypes.c_int]
+        L.pgn_model_create_shell.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp,
+                                             _dp, _dp, _dp, ctypes.c_int, _ip, _dp,
+                                             ctypes.c_int, ctypes.POINTER(_vp)]
         L.pgn_rods_create.argtypes = [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
                                       _dp, ctypes.c_int, ctypes.POINTER(_vp)]
         L.pgn_rods_destroy.argtypes = [_vp]
@@ -180,14 +184,22 @@ def _f64(a):
 
 
 class Context:
+    """A libpgn context (one device, one HIP stream).  Models and rod batches
+    created on it register here, and close() destroys them before the
+    context: the library requires every model to go before its context."""
+
     def __init__(self, device=0):
         h = _vp()
         _ok(lib().pgn_ctx_create(device, ctypes.byref(h)))
         self._h = h
+        self._children = weakref.WeakSet()
 
     @property
     def handle(self):
         return self._h
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     def stream(self):
         return lib().pgn_ctx_stream(self._h)
@@ -197,6 +209,8 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None):
+            for c in list(getattr(self, "_children", ())):
+                c.close()
             lib().pgn_ctx_destroy(self._h)
             self._h = None
 
@@ -225,6 +239,7 @@ class TetModel:
                                        _p(mat, _ucp), _p(l1), _p(l2), _p(rho),
                                        precision, ctypes.byref(h)))
         self._h = h
+        ctx._adopt(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -349,6 +364,34 @@ class TetModel:
         return b.value
 
 
+class ShellModel(TetModel):
+    """A device-resident thin shell (Simulable_ThinShell): StVK membrane
+    triangles + discrete-shell hinges.  Every TetModel method applies; the
+    element outputs list triangles first, then hinges."""
+
+    def __init__(self, ctx, X0, F, lame1, lame2, thickness, density, hinges, bending_k,
+                 precision=FP64):
+        X0 = _f64(X0).reshape(-1)
+        F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1)
+        Hn = np.ascontiguousarray(hinges, dtype=np.int32).reshape(-1)
+        nt, nh = F.size // 3, Hn.size // 4
+        b = lambda a, k: _f64(np.broadcast_to(a, (k,)))
+        self.ctx = ctx
+        self.n = X0.size // 3
+        self.m = nt + nh
+        self.n_tris, self.n_hinges = nt, nh
+        self.N = 3 * self.n
+        self.precision = precision
+        h = _vp()
+        _ok(lib().pgn_model_create_shell(ctx.handle, self.n, _p(X0), nt, _p(F, _ip),
+                                         _p(b(lame1, nt)), _p(b(lame2, nt)),
+                                         _p(b(thickness, nt)), _p(b(density, nt)), nh,
+                                         _p(Hn, _ip), _p(b(bending_k, nh)), precision,
+                                         ctypes.byref(h)))
+        self._h = h
+        ctx._adopt(self)
+
+
 def host_bsr_pattern(n_nodes, T):
     """BSR3 pattern the library builds for a tet mesh (host logic only)."""
     T = np.ascontiguousarray(T, dtype=np.int32).reshape(-1)
@@ -381,6 +424,7 @@ class Rods:
                                   _p(b(radius_w)), _p(b(density)), precision,
                                   ctypes.byref(h)))
         self._h = h
+        ctx._adopt(self)
         self.N = lib().pgn_rods_num_dofs(self._h)
         self.offsets = np.concatenate([[0], np.cumsum(4 * nverts - 1)])
 

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpgn.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["rod_kernels.hip", "tet_kernels.hip", "assembly_kernels.hip",
+SOURCES = ["rod_kernels.hip", "tet_kernels.hip", "shell_kernels.hip", "assembly_kernels.hip",
            "pcg_kernels.hip", "model.cpp", "solver.cpp", "api.cpp", "rod_model.cpp",
            "rod_api.cpp"]
 

@@ -152,7 +152,54 @@ int pgn_model_create_tet(pgn_ctx* ctx, int n_nodes, const double* X0, int n_tets
     auto* M = new pgn_model();
     M->ctx = ctx;
     try {
-      M->setup(n_nodes, X0, n_tets, T, material, lame1, lame2, density, precision);
+      for (int e = 0; e < n_tets && material; ++e)
+        PGN_CHECK(material[e] <= 1, "unknown material kind at element " + std::to_string(e));
+      M->setup(n_nodes, X0, n_tets, T, material, lame1, lame2, density, nullptr, precision);
+    } catch (...) {
+      delete M;
+      throw;
+    }
+    *out = M;
+  });
+}
+
+int pgn_model_create_shell(pgn_ctx* ctx, int n_nodes, const double* X0, int n_tris,
+                           const int* F, const double* lame1, const double* lame2,
+                           const double* thickness, const double* density, int n_hinges,
+                           const int* hinges, const double* bending_k, int precision,
+                           pgn_model** out) {
+  return guard([&] {
+    PGN_CHECK(ctx && out, "null context or output pointer");
+    PGN_CHECK(n_tris >= 0 && n_hinges >= 0, "negative element count");
+    PGN_CHECK((n_tris == 0 || (F && lame1 && lame2 && thickness && density)) &&
+                  (n_hinges == 0 || (hinges && bending_k)),
+              "null input array");
+    *out = nullptr;
+    use_device(ctx);
+    const long m = (long)n_tris + n_hinges;
+    std::vector<int> T(4 * m);
+    std::vector<unsigned char> kind(m);
+    std::vector<double> l1(m, 0.0), l2(m, 0.0), rho(m, 0.0), extra(m, 0.0);
+    for (long t = 0; t < n_tris; ++t) {
+      for (int a = 0; a < 3; ++a) T[4 * t + a] = F[3 * t + a];
+      T[4 * t + 3] = F[3 * t + 2];
+      kind[t] = 2;
+      l1[t] = lame1[t];
+      l2[t] = lame2[t];
+      rho[t] = density[t];
+      extra[t] = thickness[t];
+    }
+    for (long h = 0; h < n_hinges; ++h) {
+      const long e = n_tris + h;
+      for (int a = 0; a < 4; ++a) T[4 * e + a] = hinges[4 * h + a];
+      kind[e] = 3;
+      extra[e] = bending_k[h];
+    }
+    auto* M = new pgn_model();
+    M->ctx = ctx;
+    try {
+      M->setup(n_nodes, X0, (int)m, T.data(), kind.data(), l1.data(), l2.data(), rho.data(),
+               extra.data(), precision);
     } catch (...) {
       delete M;
       throw;
@@ -370,7 +417,7 @@ int pgn_model_element_egh(pgn_model* M, int precision, double* E, double* g, dou
       if (M->eH_alt.n < (size_t)TET_H_PACK * m * ts) M->eH_alt.alloc((size_t)TET_H_PACK * m * ts);
       eH = M->eH_alt.p;
     }
-    launch_tet_eval(s, f32, EVAL_E | EVAL_G | EVAL_H, M->tet_args(f32), nullptr, dE.p, eg, eH);
+    M->launch_elements(f32, EVAL_E | EVAL_G | EVAL_H, nullptr, dE.p, eg, eH);
     PGN_HIP(hipGetLastError());
     if (E) dE.download(E, m, s);
     std::vector<char> hg(12 * m * ts), hh((size_t)TET_H_PACK * m * ts);

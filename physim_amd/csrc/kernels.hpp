@@ -18,7 +18,8 @@ constexpr int TET_BLOCK = 256;  // threads per element-kernel workgroup
 // Element inputs of a tet mesh.  prm holds 12 SoA arrays of m values:
 // rows b1, b2, b3 of Dm^-1 (9), V0, lame1, lame2.
 struct TetArgs {
-  int m;
+  int m;                 // tets (elements [0, m) of the model)
+  long stride;           // SoA stride of prm: all elements of the model
   const int4* tets;
   const uint8_t* kind;
   const void* prm;       // float* or double* (matches the kernel precision)
@@ -31,6 +32,25 @@ struct TetArgs {
 // null; eg: 12 values per element (precision T); eH: TET_H_PACK per element.
 void launch_tet_eval(hipStream_t s, bool fp32, int flags, const TetArgs& a,
                      double* blockE, double* elemE, void* eg, void* eH);
+
+// Thin-shell elements (membrane triangles, hinges) evaluated into the same
+// per-element outputs as tets.  Elements [first, first + count) of the model;
+// prm: 12 SoA arrays of `stride` values.  Membrane: prm rows 0-3 = Dm^-1
+// (2x2 row-major), 9 = rest area x thickness, 10 = lambda, 11 = mu.  Hinge:
+// 0 = kB, 1 = theta0, 2 = shape0.  blockE receives ceil(count / TET_BLOCK)
+// partials.
+enum { SHELL_MEMBRANE = 2, SHELL_HINGE = 3 };
+struct ShellArgs {
+  int first, count;
+  long stride;
+  const int4* elems;
+  const void* prm;
+  const double* prm64;   // the FP64 table (the rest dihedral is read from it)
+  const double* x;
+  const double* X0;
+};
+void launch_shell_eval(hipStream_t s, bool fp32, int kind, int flags, const ShellArgs& a,
+                       double* blockE, double* elemE, void* eg, void* eH);
 
 // Node gather of element gradients + lumped gravity + nodal forces, fixed
 // DoFs zeroed; also node energy terms and |g|^2, reduced with the element

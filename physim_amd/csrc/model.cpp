@@ -101,30 +101,135 @@ pgn_model::~pgn_model() {
   if (hsc) (void)hipHostFree(hsc);
 }
 
-void pgn_model::setup(int n_nodes, const double* X0h, int n_tets, const int* Th,
+namespace {
+
+double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+void cross3(const double* a, const double* b, double* r) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// EnergyElement_DiscreteShells::Init (EnergyElement_DiscreteShells.cpp:50-54):
+// rest dihedral (DiscreteShellsBending_Dihedral) and shape factor
+// |e|^2 / (4 (|e x (x2-x0)| + |(x3-x0) x e|)) (DiscreteShellsBending_Shape)
+void hinge_rest(const double* p0, const double* p1, const double* p2, const double* p3,
+                double& theta, double& shape) {
+  double e[3], a2[3], a3[3], N0[3], N1[3];
+  for (int i = 0; i < 3; ++i) {
+    e[i] = p1[i] - p0[i];
+    a2[i] = p2[i] - p0[i];
+    a3[i] = p3[i] - p0[i];
+  }
+  cross3(e, a2, N0);
+  cross3(a3, e, N1);
+  const double l0 = std::sqrt(dot3(N0, N0)), l1 = std::sqrt(dot3(N1, N1));
+  const double le = std::sqrt(dot3(e, e));
+  PGN_CHECK(l0 > 0.0 && l1 > 0.0 && le > 0.0, "degenerate rest hinge");
+  double n0[3], mv[3], mn[3];
+  for (int i = 0; i < 3; ++i) {
+    n0[i] = N0[i] / l0;
+    mv[i] = n0[i] + N1[i] / l1;
+  }
+  const double lm = std::sqrt(dot3(mv, mv));
+  PGN_CHECK(lm > 0.0, "rest hinge folded flat onto itself");
+  for (int i = 0; i < 3; ++i) mv[i] /= lm;
+  cross3(mv, n0, mn);
+  theta = 2.0 * std::atan(dot3(e, mn) / le / dot3(mv, n0));
+  shape = dot3(e, e) / (4.0 * l0 + 4.0 * l1);
+}
+
+// rest triangle in its own orthonormal 2D frame (t1 along edge 01, t2 =
+// n x t1): Dm^-1 (row-major 2x2) and area
+void membrane_rest(const double* p0, const double* p1, const double* p2, double* Dinv,
+                   double& area) {
+  double e1[3], e2[3], nv[3], t1[3], t2[3];
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p1[i] - p0[i];
+    e2[i] = p2[i] - p0[i];
+  }
+  cross3(e1, e2, nv);
+  const double ln = std::sqrt(dot3(nv, nv)), l1 = std::sqrt(dot3(e1, e1));
+  PGN_CHECK(ln > 0.0 && l1 > 0.0, "degenerate rest triangle");
+  for (int i = 0; i < 3; ++i) {
+    nv[i] /= ln;
+    t1[i] = e1[i] / l1;
+  }
+  cross3(nv, t1, t2);
+  const double a = dot3(e1, t1), b = dot3(e2, t1), c = dot3(e1, t2), d = dot3(e2, t2);
+  const double det = a * d - b * c;
+  Dinv[0] = d / det;
+  Dinv[1] = -b / det;
+  Dinv[2] = -c / det;
+  Dinv[3] = a / det;
+  area = 0.5 * std::fabs(det);
+}
+
+// nodes of an element that carry lumped mass (a membrane's fourth slot
+// repeats its third node; hinges carry none)
+int mass_nodes(int kind) { return kind <= 1 ? 4 : (kind == 2 ? 3 : 0); }
+
+}  // namespace
+
+void pgn_model::setup(int n_nodes, const double* X0h, int n_elems, const int* Th,
                       const uint8_t* kindh, const double* l1, const double* l2,
-                      const double* rho, int precision) {
+                      const double* rho, const double* extra, int precision) {
   PGN_CHECK(precision == PGN_FP64 || precision == PGN_MIXED, "unknown precision");
-  PGN_CHECK(X0h && (n_tets == 0 || (Th && kindh && l1 && l2 && rho)),
+  PGN_CHECK(X0h && (n_elems == 0 || (Th && kindh && l1 && l2 && rho)),
             "null input array");
   n = n_nodes;
-  m = n_tets;
+  m = n_elems;
   prec = precision;
   hfp32 = (precision == PGN_MIXED);
   hipStream_t s = stream();
+  // element order: tets, membranes, hinges
+  m_tet = m_mem = m_hinge = 0;
+  for (int e = 0; e < m; ++e) {
+    const int k = kindh[e];
+    PGN_CHECK(k <= 3, "unknown element kind at element " + std::to_string(e));
+    const int cls = k <= 1 ? 0 : k - 1;
+    const int prev = (m_hinge > 0) ? 2 : (m_mem > 0 ? 1 : 0);
+    PGN_CHECK(cls >= prev, "elements must be ordered tets, membranes, hinges");
+    if (cls == 0) ++m_tet;
+    else if (cls == 1) ++m_mem;
+    else ++m_hinge;
+  }
+  PGN_CHECK(!(m_mem + m_hinge) || extra, "shell elements need thickness / kB values");
   build_bsr_pattern(n, m, Th, pat);
   hX0.assign(X0h, X0h + 3L * n);
   hfix.assign(3L * n, 0);
 
-  // element parameters: rows of Dm^-1, V0 = det(Dm)/6, lame1, lame2
-  std::vector<double> p64(12L * m);
+  // element parameters (12 SoA rows).  Tets: rows of Dm^-1, V0 = det(Dm)/6,
+  // lame1, lame2.  Membranes / hinges: see ShellArgs.
+  std::vector<double> p64(12L * m, 0.0);
   std::vector<float> p32(12L * m);
-  std::vector<double> hmass(n, 0.0);
+  std::vector<double> emass(m, 0.0), hmass(n, 0.0);
   std::vector<int4> ht(m);
   for (int e = 0; e < m; ++e) {
-    PGN_CHECK(kindh[e] <= 1, "unknown material kind at element " + std::to_string(e));
     const int* t = Th + 4L * e;
     ht[e] = make_int4(t[0], t[1], t[2], t[3]);
+    if (kindh[e] == 2) {
+      PGN_CHECK(t[3] == t[2], "membrane element " + std::to_string(e) +
+                                  " must repeat its third node in the fourth slot");
+      double Dinv[4], area;
+      membrane_rest(X0h + 3L * t[0], X0h + 3L * t[1], X0h + 3L * t[2], Dinv, area);
+      for (int c = 0; c < 4; ++c) p64[(long)c * m + e] = Dinv[c];
+      p64[9L * m + e] = area * extra[e];
+      p64[10L * m + e] = l1[e];
+      p64[11L * m + e] = l2[e];
+      // MassElement_Lumped: rho * VolumeBasis / nodes, VolumeBasis of a
+      // triangle = its area (Face_Tri.cpp:33-39)
+      emass[e] = rho[e] * area / 3.0;
+      continue;
+    }
+    if (kindh[e] == 3) {
+      double th, sh;
+      hinge_rest(X0h + 3L * t[0], X0h + 3L * t[1], X0h + 3L * t[2], X0h + 3L * t[3], th, sh);
+      p64[0L * m + e] = extra[e];
+      p64[1L * m + e] = th;
+      p64[2L * m + e] = sh;
+      continue;
+    }
     double Dm[3][3];  // Dm(i, a) = X_{a+1}(i) - X_0(i)
     for (int a = 0; a < 3; ++a)
       for (int i = 0; i < 3; ++i) Dm[i][a] = X0h[3L * t[a + 1] + i] - X0h[3L * t[0] + i];
@@ -150,13 +255,14 @@ void pgn_model::setup(int n_nodes, const double* X0h, int n_tets, const int* Th,
     p64[9L * m + e] = V0;
     p64[10L * m + e] = l1[e];
     p64[11L * m + e] = l2[e];
-    for (int c = 0; c < 12; ++c) p32[(long)c * m + e] = (float)p64[(long)c * m + e];
+    emass[e] = rho[e] * V0 / 4.0;
   }
+  for (long q = 0; q < 12L * m; ++q) p32[q] = (float)p64[q];
   // lumped masses, summed per node in element order
   for (int i = 0; i < n; ++i)
     for (int p = pat.inc_ptr[i]; p < pat.inc_ptr[i + 1]; ++p) {
-      const int e = pat.inc[p] >> 2;
-      hmass[i] += rho[e] * p64[9L * m + e] / 4.0;
+      const int e = pat.inc[p] >> 2, a = pat.inc[p] & 3;
+      if (a < mass_nodes(kindh[e])) hmass[i] += emass[e];
     }
 
   X0.upload(X0h, 3L * n, s);
@@ -182,7 +288,7 @@ void pgn_model::setup(int n_nodes, const double* X0h, int n_tets, const int* Th,
   binv.alloc(9L * n * ts);
   eg64.alloc(12L * std::max(m, 1));
   eH.alloc((size_t)TET_H_PACK * std::max(m, 1) * ts);
-  blockE.alloc((m + TET_BLOCK - 1) / TET_BLOCK + 1);
+  blockE.alloc(elem_partials() + 1);
   nodeP.alloc(2L * node_gather_blocks(n) + 2);
   scal.alloc(16);
   g.alloc(3L * n);
@@ -244,13 +350,43 @@ bool pgn_model::is_loaded() const {
 
 TetArgs pgn_model::tet_args(bool fp32) const {
   TetArgs a;
-  a.m = m;
+  a.m = m_tet;
+  a.stride = m;
   a.tets = tets.p;
   a.kind = kind.p;
   a.prm = fp32 ? (const void*)prm32.p : (const void*)prm64.p;
   a.x = x.p;
   a.X0 = X0.p;
   return a;
+}
+
+ShellArgs pgn_model::shell_args(bool fp32, int k) const {
+  ShellArgs a;
+  a.first = (k == SHELL_MEMBRANE) ? m_tet : m_tet + m_mem;
+  a.count = (k == SHELL_MEMBRANE) ? m_mem : m_hinge;
+  a.stride = m;
+  a.elems = tets.p;
+  a.prm = fp32 ? (const void*)prm32.p : (const void*)prm64.p;
+  a.prm64 = prm64.p;
+  a.x = x.p;
+  a.X0 = X0.p;
+  return a;
+}
+
+static int parts(int count) { return (count + TET_BLOCK - 1) / TET_BLOCK; }
+
+int pgn_model::elem_partials() const { return parts(m_tet) + parts(m_mem) + parts(m_hinge); }
+
+void pgn_model::launch_elements(bool fp32, int flags, double* bE, double* elemE, void* eg,
+                                void* eHp) {
+  hipStream_t s = stream();
+  launch_tet_eval(s, fp32, flags, tet_args(fp32), bE, elemE, eg, eHp);
+  double* b2 = bE ? bE + parts(m_tet) : nullptr;
+  launch_shell_eval(s, fp32, SHELL_MEMBRANE, flags, shell_args(fp32, SHELL_MEMBRANE), b2, elemE,
+                    eg, eHp);
+  double* b3 = bE ? b2 + parts(m_mem) : nullptr;
+  launch_shell_eval(s, fp32, SHELL_HINGE, flags, shell_args(fp32, SHELL_HINGE), b3, elemE, eg,
+                    eHp);
 }
 
 NodeArgs pgn_model::node_args() const {
@@ -309,12 +445,10 @@ PcgArgs pgn_model::pcg_args(double tol, int precond) const {
 
 void pgn_model::launch_eg() {
   hipStream_t s = stream();
-  launch_tet_eval(s, false, EVAL_E | EVAL_G, tet_args(false), blockE.p, nullptr,
-                  eg64.p, nullptr);
+  launch_elements(false, EVAL_E | EVAL_G, blockE.p, nullptr, eg64.p, nullptr);
   const NodeArgs na = node_args();
   launch_node_gather(s, false, na, eg64.p, g.p, nodeP.p);
-  launch_energy_final(s, blockE.p, (m + TET_BLOCK - 1) / TET_BLOCK, nodeP.p,
-                      node_gather_blocks(n), scal.p);
+  launch_energy_final(s, blockE.p, elem_partials(), nodeP.p, node_gather_blocks(n), scal.p);
 }
 
 void pgn_model::eval_eg(double& E, double& gg, double* escale) {
@@ -332,7 +466,7 @@ void pgn_model::update_precond() {
 
 void pgn_model::eval_hessian() {
   hipStream_t s = stream();
-  launch_tet_eval(s, hfp32, EVAL_H, tet_args(hfp32), nullptr, nullptr, nullptr, eH.p);
+  launch_elements(hfp32, EVAL_H, nullptr, nullptr, nullptr, eH.p);
   launch_bsr_gather(s, hfp32, bsr_args(), eH.p, bval.p);
   update_precond();
   hess_valid = true;

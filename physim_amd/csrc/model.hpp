@@ -29,6 +29,7 @@ void build_bsr_pattern(int n, int m, const int* T, BsrPattern& P);
 struct pgn_model {
   pgn_ctx* ctx = nullptr;
   int n = 0, m = 0;
+  int m_tet = 0, m_mem = 0, m_hinge = 0;  // elements [tets | membranes | hinges]
   int prec = PGN_FP64;
   bool hfp32 = false;  // Hessian / PCG precision is FP32
 
@@ -88,9 +89,13 @@ struct pgn_model {
   double prof_ms[2] = {0, 0};
 
   ~pgn_model();
-  void setup(int n_nodes, const double* X0h, int n_tets, const int* Th,
+  // kindh: 0 CoNH tet, 1 StVK tet, 2 StVK membrane triangle (row v0 v1 v2 v2),
+  // 3 discrete-shell hinge (edge v0 v1, opposite v2 v3), ordered tets,
+  // membranes, hinges.  extra: thickness (membranes) / kB (hinges), may be
+  // null for tet-only models.
+  void setup(int n_nodes, const double* X0h, int n_elems, const int* Th,
              const uint8_t* kindh, const double* l1, const double* l2,
-             const double* rho, int precision);
+             const double* rho, const double* extra, int precision);
   hipStream_t stream() const { return ctx->stream; }
   size_t tsize() const { return hfp32 ? sizeof(float) : sizeof(double); }
 
@@ -101,6 +106,12 @@ struct pgn_model {
   bool is_loaded() const;
 
   pgn::TetArgs tet_args(bool fp32) const;
+  pgn::ShellArgs shell_args(bool fp32, int kind) const;
+  // every element kernel of the model (tets, membranes, hinges) with the
+  // given flags; blockE receives elem_partials() FP64 energy partials
+  void launch_elements(bool fp32, int flags, double* blockE, double* elemE, void* eg,
+                       void* eH);
+  int elem_partials() const;
   pgn::NodeArgs node_args() const;
   pgn::BsrArgs bsr_args() const;
   pgn::PcgArgs pcg_args(double tol, int precond) const;

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
   const int e = blockIdx.x * TET_BLOCK + threadIdx.x;
   double Ee = 0.0;
   if (e < A.m) {
-    const long m = A.m;
+    const long m = A.stride;
     const T* __restrict__ prm = reinterpret_cast<const T*>(A.prm);
     const int4 t = A.tets[e];
     const int nd[4] = {t.x, t.y, t.z, t.w};

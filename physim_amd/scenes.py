@@ -174,3 +174,115 @@ def rod_batch(n_rods, n_min=50, n_max=500, seed=4, stages=10, young=1e7,
     dens = 1000.0 * np.pi * radius ** 2 if density is None else density
     return RodScene(rods, young, young / 3.0, radius, dens, stages,
                     name or f"helices_{n_rods}")
+
+
+# ---- thin shells ---------------------------------------------------------------
+
+def grid_tri3(N, M):
+    """create2DGridIndex_Tri3 (MeshUtils.cpp:61-71): node (i, j) = i*M + j,
+    two triangles per cell."""
+    F = []
+    for i in range(N - 1):
+        for j in range(M - 1):
+            F.append((i * M + j, i * M + j + 1, (i + 1) * M + j))
+            F.append(((i + 1) * M + j, i * M + j + 1, (i + 1) * M + j + 1))
+    return np.array(F, dtype=np.int32)
+
+
+def shell_hinges(F):
+    """Interior edges of a triangle mesh as hinges, in the order and node
+    order Simulable_ThinShell / EnergyElement_DiscreteShells use: edges in
+    creation order of Mesh_Face's half-edge build (faces in order, edges
+    (k, k+1) of each face), each hinge [target of half0, target of half1,
+    vertex opposite in half0's face, vertex opposite in half1's face], where
+    half0 is the edge's first half-edge (Mesh_Face.cpp:560-636,
+    EnergyElement_DiscreteShells.cpp:30-34)."""
+    first = {}    # undirected edge -> (a, b, opposite) of its first half-edge
+    order = []
+    second = {}
+    for f in F:
+        for k in range(3):
+            a, b, c = int(f[k]), int(f[(k + 1) % 3]), int(f[(k + 2) % 3])
+            key = (min(a, b), max(a, b))
+            if key not in first:
+                first[key] = (a, b, c)
+                order.append(key)
+            else:
+                second[key] = (a, b, c)
+    H = []
+    for key in order:
+        if key not in second:
+            continue  # boundary edge: no bending element
+        a0, b0, c0 = first[key]
+        a1, b1, c1 = second[key]
+        H.append((b0, b1, c0, c1))
+    return np.array(H, dtype=np.int32).reshape(-1, 4)
+
+
+class ShellScene:
+    """Inputs of one quasi-static thin-shell solve."""
+
+    def __init__(self, name, X0, F, hinges, lame1, lame2, thickness, density, bending_k,
+                 fixed, gravity, stages):
+        self.name = name
+        self.X0 = X0
+        self.F = F
+        self.hinges = hinges
+        self.lame1 = lame1
+        self.lame2 = lame2
+        self.thickness = thickness
+        self.density = density
+        self.bending_k = bending_k
+        self.fixed = fixed
+        self.gravity = gravity
+        self.stages = stages
+
+    @property
+    def n(self):
+        return self.X0.shape[0]
+
+    def bbox_diag(self):
+        return float(np.linalg.norm(self.X0.max(0) - self.X0.min(0)))
+
+    def apply(self, model):
+        model.set_fixed(self.fixed)
+        model.set_gravity(self.gravity, self.stages)
+
+    def oracle_arrays(self):
+        """Element arrays of the oracle's mixed model: triangles (kind 2, row
+        v0 v1 v2 v2) then hinges (kind 3)."""
+        nt, nh = len(self.F), len(self.hinges)
+        T = np.concatenate([np.concatenate([self.F, self.F[:, 2:3]], axis=1), self.hinges])
+        kind = np.concatenate([np.full(nt, 2, np.uint8), np.full(nh, 3, np.uint8)])
+        l1 = np.concatenate([np.broadcast_to(self.lame1, (nt,)), np.zeros(nh)])
+        l2 = np.concatenate([np.broadcast_to(self.lame2, (nt,)), np.zeros(nh)])
+        rho = np.concatenate([np.broadcast_to(self.density, (nt,)), np.zeros(nh)])
+        extra = np.concatenate([np.broadcast_to(self.thickness, (nt,)),
+                                np.broadcast_to(self.bending_k, (nh,))])
+        return T.astype(np.int32), kind, l1, l2, rho, extra
+
+
+def cloth(n, size=1.0, thickness=1e-3, young=1e6, poisson=0.3, density=0.2, stages=10,
+          jitter=0.0, seed=0, name=None):
+    """n x n node sheet of side `size` in the horizontal xz-plane, pinned at
+    the two corners of its x = 0 edge, under gravity -9.81 y.  Membrane:
+    2D StVK with the Lame parameters of (young, poisson); bending stiffness
+    kB = young t^3 / (12 (1 - poisson^2)) per hinge (passed explicitly, as
+    Param_BendingK); density in kg/m^2 (the reference lumps density x area,
+    with no thickness factor).  jitter > 0 moves nodes by U(-j, j) cells in
+    y (a non-flat rest shape)."""
+    h = size / (n - 1)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    X0 = np.stack([i.ravel() * h, np.zeros(n * n), j.ravel() * h], axis=1)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        X0[:, 1] += rng.uniform(-jitter, jitter, n * n) * h
+    F = grid_tri3(n, n)
+    Hn = shell_hinges(F)
+    l1, l2 = lame(young, poisson)
+    kB = young * thickness ** 3 / (12.0 * (1.0 - poisson ** 2))
+    fixed = np.zeros((n * n, 3), dtype=np.uint8)
+    fixed[0] = 1          # node (0, 0)
+    fixed[n - 1] = 1      # node (0, n-1)
+    return ShellScene(name or f"cloth_{n}x{n}", X0, F, Hn, float(l1), float(l2), thickness,
+                      density, kB, fixed, G_EARTH, stages)
