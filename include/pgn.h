@@ -226,6 +226,45 @@ void pgn_newton_default_opts(pgn_newton_opts* o);
 int pgn_newton_solve(pgn_model* model, const pgn_newton_opts* opts,
                      pgn_newton_result* res);
 
+/* ---- fused batches of independent solves -------------------------------- */
+/* Many independent quasi-static problems (tet meshes and thin shells) solved
+ * together: pgn_batch_build uploads them as ONE device model over the
+ * disjoint union of their nodes and elements, so every element, gather,
+ * assembly and PCG launch covers the whole batch.  Each instance keeps the
+ * reference's own Newton state (load stages, LineSearch_Simple, the
+ * SolveFull status machine: OptimSolver.cpp:163-264, 688-736); the linear
+ * solve is one PCG over the block-diagonal batch Hessian.  Instance DoF
+ * vectors use the single-model layout (3 per node, node-major). */
+typedef struct pgn_batch pgn_batch;
+int pgn_batch_create(pgn_ctx* ctx, int precision, pgn_batch** out);
+void pgn_batch_destroy(pgn_batch* batch);
+/* stage one instance (arguments as pgn_model_create_tet / _shell) */
+int pgn_batch_add_tet(pgn_batch* batch, int n_nodes, const double* X0, int n_tets,
+                      const int* T, const unsigned char* material, const double* lame1,
+                      const double* lame2, const double* density, int* instance);
+int pgn_batch_add_shell(pgn_batch* batch, int n_nodes, const double* X0, int n_tris,
+                        const int* F, const double* lame1, const double* lame2,
+                        const double* thickness, const double* density, int n_hinges,
+                        const int* hinges, const double* bending_k, int* instance);
+/* boundary conditions of one staged instance (as pgn_model_set_*) */
+int pgn_batch_set_fixed(pgn_batch* batch, int instance, const unsigned char* mask);
+int pgn_batch_set_gravity(pgn_batch* batch, int instance, const double g_ini[3],
+                          const double g_end[3], int num_stages);
+int pgn_batch_set_forces(pgn_batch* batch, int instance, const double* f_ini,
+                         const double* f_end, int num_stages);
+/* builds the fused device model; no instance can be added afterwards */
+int pgn_batch_build(pgn_batch* batch);
+int pgn_batch_num_instances(const pgn_batch* batch);
+size_t pgn_batch_device_bytes(const pgn_batch* batch);
+/* x := X0 and load stages := 0 for every instance */
+int pgn_batch_reset(pgn_batch* batch);
+/* OptimSolver::SolveFull per instance; res: one result per instance */
+int pgn_batch_newton(pgn_batch* batch, const pgn_newton_opts* opts,
+                     pgn_newton_result* res);
+int pgn_batch_get_x(pgn_batch* batch, int instance, double* x);
+/* per-instance total energy at the current x and load stages */
+int pgn_batch_get_energy(pgn_batch* batch, double* energy);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Runs `reps` launches of one kernel on the model's current state, on the
  * context stream, and returns the mean launch time (HIP events) in ms.

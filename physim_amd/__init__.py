@@ -37,7 +37,11 @@ EXPORTS = [
     "pgn_model_element_egh", "pgn_linear_default_opts", "pgn_linear_solve",
     "pgn_newton_default_opts", "pgn_newton_solve", "pgn_time_kernel",
     "pgn_kernel_bytes", "pgn_host_bsr_pattern", "pgn_model_set_profiling",
-    "pgn_model_get_profile", "pgn_model_create_shell", "pgn_rods_create", "pgn_rods_destroy",
+    "pgn_model_get_profile", "pgn_model_create_shell", "pgn_batch_create",
+    "pgn_batch_destroy", "pgn_batch_add_tet", "pgn_batch_add_shell", "pgn_batch_set_fixed",
+    "pgn_batch_set_gravity", "pgn_batch_set_forces", "pgn_batch_build",
+    "pgn_batch_num_instances", "pgn_batch_device_bytes", "pgn_batch_reset",
+    "pgn_batch_newton", "pgn_batch_get_x", "pgn_batch_get_energy", "pgn_rods_create", "pgn_rods_destroy",
     "pgn_rods_num_dofs", "pgn_rods_set_fixed", "pgn_rods_set_gravity",
     "pgn_rods_reset", "pgn_rods_set_x", "pgn_rods_get_x", "pgn_rods_get_energy",
     "pgn_rods_get_gradient", "pgn_rods_get_hessian_dense", "pgn_rods_get_frames",
@@ -141,6 +145,25 @@ def lib():
         L.pgn_model_create_shell.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp,
                                              _dp, _dp, _dp, ctypes.c_int, _ip, _dp,
                                              ctypes.c_int, ctypes.POINTER(_vp)]
+        L.pgn_batch_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.pgn_batch_destroy.argtypes = [_vp]
+        L.pgn_batch_destroy.restype = None
+        L.pgn_batch_add_tet.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, _ip, _ucp, _dp,
+                                        _dp, _dp, _ip]
+        L.pgn_batch_add_shell.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp,
+                                          _dp, _dp, ctypes.c_int, _ip, _dp, _ip]
+        L.pgn_batch_set_fixed.argtypes = [_vp, ctypes.c_int, _ucp]
+        L.pgn_batch_set_gravity.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.pgn_batch_set_forces.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.pgn_batch_build.argtypes = [_vp]
+        L.pgn_batch_num_instances.argtypes = [_vp]
+        L.pgn_batch_device_bytes.argtypes = [_vp]
+        L.pgn_batch_device_bytes.restype = ctypes.c_size_t
+        L.pgn_batch_reset.argtypes = [_vp]
+        L.pgn_batch_newton.argtypes = [_vp, ctypes.POINTER(NewtonOpts),
+                                       ctypes.POINTER(NewtonResult)]
+        L.pgn_batch_get_x.argtypes = [_vp, ctypes.c_int, _dp]
+        L.pgn_batch_get_energy.argtypes = [_vp, _dp]
         L.pgn_rods_create.argtypes = [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
                                       _dp, ctypes.c_int, ctypes.POINTER(_vp)]
         L.pgn_rods_destroy.argtypes = [_vp]
@@ -390,6 +413,113 @@ class ShellModel(TetModel):
                                          ctypes.byref(h)))
         self._h = h
         ctx._adopt(self)
+
+
+def _result_dict(r):
+    return {"result": OS_NAMES[r.result], "iterations": r.iterations,
+            "pcg_iterations": r.pcg_iterations, "bisections": r.line_search_bisections,
+            "energy_evals": r.energy_evals, "hessian_evals": r.hessian_evals,
+            "objective": r.objective, "optimality": r.optimality, "seconds": r.seconds}
+
+
+class Batch:
+    """A fused batch of independent tet / thin-shell solves (pgn_batch):
+    stage instances with add_tet / add_shell (+ boundary conditions), then
+    build() uploads them as one device model."""
+
+    def __init__(self, ctx, precision=FP64):
+        h = _vp()
+        _ok(lib().pgn_batch_create(ctx.handle, precision, ctypes.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        self.n_nodes = []
+        ctx._adopt(self)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pgn_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_tet(self, X0, T, material, lame1, lame2, density):
+        X0 = _f64(X0).reshape(-1)
+        T = np.ascontiguousarray(T, dtype=np.int32).reshape(-1)
+        m = T.size // 4
+        b = lambda a: _f64(np.broadcast_to(a, (m,)))
+        mat = np.ascontiguousarray(np.broadcast_to(material, (m,)), dtype=np.uint8)
+        k = ctypes.c_int()
+        _ok(lib().pgn_batch_add_tet(self._h, X0.size // 3, _p(X0), m, _p(T, _ip),
+                                    _p(mat, _ucp), _p(b(lame1)), _p(b(lame2)), _p(b(density)),
+                                    ctypes.byref(k)))
+        self.n_nodes.append(X0.size // 3)
+        return k.value
+
+    def add_shell(self, X0, F, lame1, lame2, thickness, density, hinges, bending_k):
+        X0 = _f64(X0).reshape(-1)
+        F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1)
+        Hn = np.ascontiguousarray(hinges, dtype=np.int32).reshape(-1)
+        nt, nh = F.size // 3, Hn.size // 4
+        b = lambda a, c: _f64(np.broadcast_to(a, (c,)))
+        k = ctypes.c_int()
+        _ok(lib().pgn_batch_add_shell(self._h, X0.size // 3, _p(X0), nt, _p(F, _ip),
+                                      _p(b(lame1, nt)), _p(b(lame2, nt)), _p(b(thickness, nt)),
+                                      _p(b(density, nt)), nh, _p(Hn, _ip), _p(b(bending_k, nh)),
+                                      ctypes.byref(k)))
+        self.n_nodes.append(X0.size // 3)
+        return k.value
+
+    def add_scene(self, scene):
+        """Stage a scenes.Scene (tets) or scenes.ShellScene with its fixation
+        and loads; returns the instance index."""
+        if hasattr(scene, "hinges"):
+            k = self.add_shell(scene.X0, scene.F, scene.lame1, scene.lame2, scene.thickness,
+                               scene.density, scene.hinges, scene.bending_k)
+        else:
+            k = self.add_tet(scene.X0, scene.T, scene.material, scene.lame1, scene.lame2,
+                             scene.density)
+        mk = np.ascontiguousarray(scene.fixed, dtype=np.uint8).reshape(-1)
+        _ok(lib().pgn_batch_set_fixed(self._h, k, _p(mk, _ucp)))
+        _ok(lib().pgn_batch_set_gravity(self._h, k, _p(_f64((0.0, 0.0, 0.0))),
+                                        _p(_f64(scene.gravity)), int(scene.stages)))
+        if getattr(scene, "forces", None) is not None:
+            fe = _f64(scene.forces).reshape(-1)
+            _ok(lib().pgn_batch_set_forces(self._h, k, _p(np.zeros_like(fe)), _p(fe),
+                                           int(scene.stages)))
+        return k
+
+    def build(self):
+        _ok(lib().pgn_batch_build(self._h))
+
+    @property
+    def K(self):
+        return lib().pgn_batch_num_instances(self._h)
+
+    def device_bytes(self):
+        return lib().pgn_batch_device_bytes(self._h)
+
+    def reset(self):
+        _ok(lib().pgn_batch_reset(self._h))
+
+    def newton(self, **kw):
+        o = NewtonOpts()
+        lib().pgn_newton_default_opts(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        res = (NewtonResult * self.K)()
+        _ok(lib().pgn_batch_newton(self._h, ctypes.byref(o), res))
+        return [_result_dict(r) for r in res]
+
+    def get_x(self, k):
+        x = np.zeros(3 * self.n_nodes[k])
+        _ok(lib().pgn_batch_get_x(self._h, k, _p(x)))
+        return x.reshape(-1, 3)
+
+    def energy(self):
+        e = np.zeros(self.K)
+        _ok(lib().pgn_batch_get_energy(self._h, _p(e)))
+        return e
 
 
 def host_bsr_pattern(n_nodes, T):

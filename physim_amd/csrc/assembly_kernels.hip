@@ -37,15 +37,24 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_gather_kernel(
       s2 += (double)ge[2];
     }
     const double M = A.mass[i];
-    double gr[3] = {s0 - M * A.grav[0], s1 - M * A.grav[1], s2 - M * A.grav[2]};
+    double gv[3] = {A.grav[0], A.grav[1], A.grav[2]};
+    double fa = A.f_alpha;
+    if (A.node_inst) {
+      const int k = A.node_inst[i];
+      gv[0] = A.grav_inst[3 * k];
+      gv[1] = A.grav_inst[3 * k + 1];
+      gv[2] = A.grav_inst[3 * k + 2];
+      fa = A.falpha_inst[k];
+    }
+    double gr[3] = {s0 - M * gv[0], s1 - M * gv[1], s2 - M * gv[2]};
     const double* xi = A.x + 3L * i;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) eN += xi[k] * (-M * A.grav[k]);
+    for (int k = 0; k < 3; ++k) eN += xi[k] * (-M * gv[k]);
     if (A.f_end) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const double f0 = A.f_ini[3L * i + k];
-        const double f = f0 + (A.f_end[3L * i + k] - f0) * A.f_alpha;
+        const double f = f0 + (A.f_end[3L * i + k] - f0) * fa;
         gr[k] += f;
         eN += xi[k] * f;
       }
@@ -56,6 +65,10 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_gather_kernel(
       if (fm & (1u << k)) gr[k] = 0.0;
       g[3L * i + k] = gr[k];
       gg += gr[k] * gr[k];
+    }
+    if (A.nodeE) {
+      A.nodeE[i] = eN;
+      A.nodeG2[i] = gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2];
     }
   }
   block_sum2<NODE_BLOCK>(eN, gg, red);

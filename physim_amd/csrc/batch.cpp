@@ -1,0 +1,627 @@
+// batch.cpp -- fused batch of independent quasi-static solves (pgn_batch_*).
+//
+// Build: the instances' nodes are concatenated in instance order; elements
+// are concatenated per class (all tets, then all membranes, then all
+// hinges, each class in instance order) so that one launch per element
+// class covers the batch, and the BSR pattern of the union is the union of
+// the instances' patterns.  Newton: OptimSolver::SolveFull
+// (OptimSolver.cpp:163-264) with OptimSolver_USQP_LS::SolveStepInternal
+// (OptimSolver_USQP_LS.cpp:18-68) per instance, all active instances
+// advancing one iteration per batch iteration; per-instance energies and
+// gradient norms come from segmented reductions.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <new>
+
+#include "solver.hpp"
+
+using namespace pgn;
+
+namespace pgn {
+int api_guard_set(const char* msg, int code);
+}
+
+size_t pgn_batch_inst::footprint() const {
+  // per node: x, X0, g, PCG and Newton vectors (~30 x 8 B); per element:
+  // connectivity + parameters + outputs (~16 + 96 + 96 + 78 x 8 B); per
+  // block (~15 per node): values + indices (9 x 8 + 24 B)
+  long m = 0;
+  for (int c = 0; c < 3; ++c) m += (long)T[c].size() / 4;
+  const double ts = 8.0;
+  return (size_t)(n * (30.0 * ts + 15.0 * (9 * ts + 24)) + m * (16 + 24 * ts + 78 * ts + 40));
+}
+
+void pgn_batch::build() {
+  PGN_CHECK(!inst.empty(), "empty batch");
+  const int Kn = K();
+  hnoff.assign(Kn + 1, 0);
+  for (int k = 0; k < Kn; ++k) hnoff[k + 1] = hnoff[k] + inst[k].n;
+  const long n = hnoff[Kn];
+  PGN_CHECK(n < (1L << 31), "batch too large for 32-bit node ids");
+  heoff.assign(3 * (Kn + 1), 0);
+  for (int c = 0; c < 3; ++c) {
+    long base = (c == 0) ? 0 : heoff[c * (Kn + 1) - 1];
+    heoff[c * (Kn + 1)] = base;
+    for (int k = 0; k < Kn; ++k)
+      heoff[c * (Kn + 1) + k + 1] = heoff[c * (Kn + 1) + k] + (long)inst[k].T[c].size() / 4;
+  }
+  const long m = heoff[3 * (Kn + 1) - 1];
+  PGN_CHECK(m < (1L << 28), "batch too large for 32-bit slot codes");
+  std::vector<double> X0(3 * n), l1(m), l2(m), rho(m), extra(m);
+  std::vector<int> T(4 * m), ninst(n);
+  std::vector<uint8_t> kind(m), fixed(3 * n);
+  bool any_f = false;
+  for (int k = 0; k < Kn; ++k) {
+    const pgn_batch_inst& I = inst[k];
+    std::copy(I.X0.begin(), I.X0.end(), X0.begin() + 3 * hnoff[k]);
+    std::copy(I.fixed.begin(), I.fixed.end(), fixed.begin() + 3 * hnoff[k]);
+    for (long i = 0; i < I.n; ++i) ninst[hnoff[k] + i] = k;
+    any_f |= I.has_f;
+    for (int c = 0; c < 3; ++c) {
+      const long e0 = heoff[c * (Kn + 1) + k];
+      const long mc = (long)I.T[c].size() / 4;
+      for (long e = 0; e < mc; ++e) {
+        for (int a = 0; a < 4; ++a) T[4 * (e0 + e) + a] = I.T[c][4 * e + a] + (int)hnoff[k];
+        kind[e0 + e] = I.kind[c][e];
+        l1[e0 + e] = I.l1[c][e];
+        l2[e0 + e] = I.l2[c][e];
+        rho[e0 + e] = I.rho[c][e];
+        extra[e0 + e] = I.extra[c][e];
+      }
+    }
+  }
+  hipStream_t s = ctx->stream;
+  auto model = std::make_unique<pgn_model>();
+  model->ctx = ctx;
+  model->setup((int)n, X0.data(), (int)m, T.data(), kind.data(), l1.data(), l2.data(),
+               rho.data(), extra.data(), prec);
+  model->hfix = fixed;
+  std::vector<uint8_t> fm(n);
+  for (long i = 0; i < n; ++i)
+    fm[i] = (fixed[3 * i] ? 1 : 0) | (fixed[3 * i + 1] ? 2 : 0) | (fixed[3 * i + 2] ? 4 : 0);
+  model->fixmask.upload(fm, s);
+  if (any_f) {
+    std::vector<double> fi(3 * n, 0.0), fe(3 * n, 0.0);
+    for (int k = 0; k < Kn; ++k)
+      if (inst[k].has_f) {
+        std::copy(inst[k].f_ini.begin(), inst[k].f_ini.end(), fi.begin() + 3 * hnoff[k]);
+        std::copy(inst[k].f_end.begin(), inst[k].f_end.end(), fe.begin() + 3 * hnoff[k]);
+      }
+    model->f_ini.upload(fi, s);
+    model->f_end.upload(fe, s);
+    model->has_f = true;
+  }
+  M = std::move(model);
+  noff.upload(hnoff, s);
+  eoff.upload(heoff, s);
+  node_inst.upload(ninst, s);
+  grav.alloc(3 * Kn);
+  falpha.alloc(Kn);
+  fac.alloc(Kn);
+  out.alloc(4 * Kn);
+  n2.alloc(Kn);
+  mode.alloc(Kn);
+  mask.alloc(Kn);
+  elemE.alloc(std::max<long>(m, 1));
+  nodeE.alloc(n);
+  nodeG2.alloc(n);
+  xsave.alloc(3 * n);
+  v0.alloc(3 * n);
+  dx.alloc(3 * n);
+  bvec.alloc(3 * n);
+  g_cur.assign(Kn, 0);
+  f_cur.assign(Kn, 0);
+  PGN_HIP(hipStreamSynchronize(s));
+}
+
+BatchArgs pgn_batch::args() const {
+  BatchArgs a;
+  a.K = K();
+  a.noff = noff.p;
+  a.eoff = eoff.p;
+  a.node_inst = node_inst.p;
+  return a;
+}
+
+bool pgn_batch::step_load(int k) {
+  bool st = false;
+  const pgn_batch_inst& I = inst[k];
+  if (I.has_g && g_cur[k] < I.g_stages) {
+    ++g_cur[k];
+    st = true;
+  }
+  if (I.has_f && f_cur[k] < I.f_stages) {
+    ++f_cur[k];
+    st = true;
+  }
+  return st;
+}
+
+bool pgn_batch::is_loaded(int k) const {
+  const pgn_batch_inst& I = inst[k];
+  if (I.has_g && g_cur[k] != I.g_stages) return false;
+  if (I.has_f && f_cur[k] != I.f_stages) return false;
+  return true;
+}
+
+// BCondition::GetCurValues with VectorAnimator_LerpValues per instance
+void pgn_batch::upload_loads() {
+  const int Kn = K();
+  std::vector<double> g(3 * Kn, 0.0), fa(Kn, 0.0);
+  for (int k = 0; k < Kn; ++k) {
+    const pgn_batch_inst& I = inst[k];
+    if (I.has_g) {
+      const double a = (double)g_cur[k] / I.g_stages;
+      for (int i = 0; i < 3; ++i) g[3 * k + i] = I.g_ini[i] + (I.g_end[i] - I.g_ini[i]) * a;
+    }
+    if (I.has_f) fa[k] = (double)f_cur[k] / I.f_stages;
+  }
+  grav.upload(g, ctx->stream);
+  falpha.upload(fa, ctx->stream);
+}
+
+void pgn_batch::eval_eg(std::vector<double>& o4) {
+  hipStream_t s = ctx->stream;
+  M->launch_elements(false, EVAL_E | EVAL_G, nullptr, elemE.p, M->eg64.p, nullptr);
+  NodeArgs na = M->node_args();
+  na.node_inst = node_inst.p;
+  na.grav_inst = grav.p;
+  na.falpha_inst = falpha.p;
+  na.nodeE = nodeE.p;
+  na.nodeG2 = nodeG2.p;
+  launch_node_gather(s, false, na, M->eg64.p, M->g.p, M->nodeP.p);
+  launch_batch_reduce(s, args(), elemE.p, nodeE.p, nodeG2.p, out.p);
+  o4.resize(4 * K());
+  out.download(o4.data(), o4.size(), s);
+  PGN_HIP(hipStreamSynchronize(s));
+}
+
+void pgn_batch::reset() {
+  PGN_HIP(hipMemcpyAsync(M->x.p, M->X0.p, M->X0.bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+  PGN_HIP(hipStreamSynchronize(ctx->stream));
+  std::fill(g_cur.begin(), g_cur.end(), 0);
+  std::fill(f_cur.begin(), f_cur.end(), 0);
+  M->hess_valid = false;
+}
+
+void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = ctx->stream;
+  const int Kn = K();
+  const long n = M->n, N = 3 * n;
+  const BatchArgs ba = args();
+  pgn_linear_opts lo;
+  pgn_linear_default_opts(&lo);
+  lo.tol = o.lsTolMaxError;
+  lo.max_iters = o.lsNumMaxIters;
+  lo.precond = o.precond;
+  lo.inner_tol = o.inner_tol;
+  enum { ACTIVE = 0, DONE = 1 };
+  std::vector<int> st(Kn, ACTIVE), result(Kn, PGN_OS_SUCCESS), minImpr(Kn, 0), minStep(Kn, 0),
+      steep(Kn, 0), its(Kn, 0), bis(Kn, 0), evals(Kn, 0), hevals(Kn, 0), pcg(Kn, 0),
+      nbis(Kn, 0), searching(Kn, 0);
+  std::vector<uint8_t> hmode(Kn, 0), hmask(Kn, 0);
+  std::vector<double> obj(Kn), opt(Kn), optStart(Kn), objNew(Kn), optNew(Kn), f(Kn), len(Kn),
+      clamp(Kn), stepLen(Kn), hfac(Kn), eta(Kn, 0.1), gprev(Kn, -1.0), hn2(Kn);
+  std::vector<double> E;
+  auto d2d = [&](double* dst, const double* src) {
+    PGN_HIP(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  };
+  // OptimSolver::Init (OptimSolver.cpp:36-55)
+  PGN_HIP(hipMemcpyAsync(v0.p, M->x.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  upload_loads();
+  eval_eg(E);
+  for (int k = 0; k < Kn; ++k) {
+    obj[k] = E[4 * k];
+    opt[k] = std::sqrt(E[4 * k + 1]);
+    evals[k]++;
+  }
+  for (;;) {
+    bool any = false;
+    for (int k = 0; k < Kn; ++k) any |= (st[k] == ACTIVE);
+    if (!any) break;
+    // OnIterStart: StepBoundaryConditions of every active instance
+    for (int k = 0; k < Kn; ++k)
+      if (st[k] == ACTIVE) step_load(k);
+    upload_loads();
+    d2d(xsave.p, M->x.p);
+    eval_eg(E);
+    bool need_h = false;
+    double tol_lin = 1.0;
+    for (int k = 0; k < Kn; ++k) {
+      hmode[k] = 0;
+      if (st[k] != ACTIVE) continue;
+      its[k]++;
+      evals[k]++;
+      obj[k] = E[4 * k];
+      opt[k] = optStart[k] = std::sqrt(E[4 * k + 1]);
+      if (opt[k] < o.tolMaxError) {
+        result[k] = PGN_OS_SUCCESS;
+        continue;
+      }
+      hmode[k] = steep[k] ? 2 : 1;
+      if (hmode[k] == 1) {
+        need_h = true;
+        hevals[k]++;
+        double lt = o.lsTolMaxError;
+        if (o.adaptive_tol) {  // Eisenstat-Walker choice 2 per instance
+          if (gprev[k] > 0.0) {
+            const double r = opt[k] / gprev[k];
+            double e = 0.9 * r * r;
+            const double guard = 0.9 * eta[k] * eta[k];
+            if (guard > 0.1) e = std::max(e, guard);
+            eta[k] = std::min(0.1, e);
+          }
+          gprev[k] = opt[k];
+          lt = std::max(o.lsTolMaxError, eta[k]);
+        }
+        tol_lin = std::min(tol_lin, lt);
+      }
+    }
+    mode.upload(hmode, s);
+    if (need_h) M->eval_hessian();
+    launch_batch_rhs(s, ba, n, mode.p, M->g.p, xsave.p, v0.p, bvec.p, dx.p);
+    if (need_h) {
+      lo.tol = tol_lin;
+      pgn_linear_result lr;
+      linear_solve(*M, bvec.p, dx.p, lo, lr);
+      for (int k = 0; k < Kn; ++k)
+        if (hmode[k] == 1) pcg[k] += lr.iterations;
+    }
+    launch_batch_norm2(s, ba, dx.p, n2.p);
+    n2.download(hn2.data(), Kn, s);
+    PGN_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < Kn; ++k) {
+      searching[k] = 0;
+      if (!hmode[k]) continue;
+      len[k] = std::sqrt(hn2[k]);
+      clamp[k] = 1.0;
+      if (len[k] > o.maxStepSize) {  // OptimSolver.cpp:346
+        clamp[k] = o.maxStepSize / len[k];
+        len[k] = o.maxStepSize;
+      }
+      f[k] = 1.0;
+      nbis[k] = -1;
+      searching[k] = 1;
+    }
+    // LineSearch_Simple (OptimSolver.cpp:688-736), instances in lockstep
+    for (;;) {
+      bool anys = false;
+      for (int k = 0; k < Kn; ++k) {
+        hfac[k] = -1.0;
+        if (searching[k]) {
+          nbis[k]++;
+          stepLen[k] = f[k] * len[k];
+          hfac[k] = f[k] * clamp[k];
+          anys = true;
+        }
+      }
+      if (!anys) break;
+      fac.upload(hfac, s);
+      launch_batch_step(s, ba, n, fac.p, xsave.p, dx.p, M->x.p);
+      eval_eg(E);
+      bool restore = false;
+      for (int k = 0; k < Kn; ++k) {
+        hfac[k] = -1.0;
+        if (!searching[k]) continue;
+        evals[k]++;
+        const double en = E[4 * k], gn = std::sqrt(E[4 * k + 1]);
+        objNew[k] = en;
+        optNew[k] = gn;
+        if (en < obj[k]) {
+          searching[k] = 0;
+          continue;
+        }
+        if (o.ls_noise_guard > 0.0 && en <= obj[k] + o.ls_noise_guard * E[4 * k + 2] &&
+            gn < opt[k]) {
+          objNew[k] = obj[k] - 1e-300;
+          searching[k] = 0;
+          continue;
+        }
+        f[k] *= o.lSearch_alpha;
+        if (!(stepLen[k] > o.tolMinStep && nbis[k] <= o.lsearch_numTry)) {
+          searching[k] = 0;  // exhausted: back to the step start
+          hfac[k] = 0.0;
+          optNew[k] = optStart[k];
+          restore = true;
+        }
+      }
+      if (restore) {
+        fac.upload(hfac, s);
+        launch_batch_step(s, ba, n, fac.p, xsave.p, dx.p, M->x.p);
+      }
+    }
+    // UpdateStepResult (OptimSolver.cpp:402-466)
+    bool any_impr = false;
+    for (int k = 0; k < Kn; ++k) {
+      hmask[k] = 0;
+      if (!hmode[k]) continue;
+      bis[k] += nbis[k];
+      const double impr = obj[k] - objNew[k];
+      if (impr > 0) {
+        if (impr < o.tolMinImpr) {
+          minImpr[k]++;
+          result[k] = PGN_OS_MINIMPR;
+        } else if (stepLen[k] < o.tolMinStep) {
+          minStep[k]++;
+          result[k] = PGN_OS_MINSTEP;
+        } else if (optNew[k] <= o.tolMaxError) {
+          result[k] = PGN_OS_SUCCESS;
+          minImpr[k] = minStep[k] = 0;
+        } else {
+          result[k] = PGN_OS_ONGOING;
+          minImpr[k] = minStep[k] = 0;
+        }
+        obj[k] = objNew[k];
+        opt[k] = optNew[k];
+        hmask[k] = 1;  // v0 := the step start (OptimSolver.cpp:488 warm start)
+        any_impr = true;
+      } else {
+        result[k] = PGN_OS_NONDESC;
+        opt[k] = optStart[k];
+      }
+    }
+    if (any_impr) {
+      mask.upload(hmask, s);
+      launch_batch_copy(s, ba, n, mask.p, xsave.p, v0.p);
+    }
+    // SolveFull status machine per instance
+    for (int k = 0; k < Kn; ++k) {
+      if (st[k] != ACTIVE) continue;
+      bool stop = false;
+      if (is_loaded(k)) {
+        int& res = result[k];
+        if (res == PGN_OS_NONDESC) {
+          if (!o.trySteepest || steep[k]) stop = true;
+          else steep[k] = 1;
+        }
+        if (!stop && res == PGN_OS_MINIMPR) {
+          if (minImpr[k] == o.minImprMaxCount) stop = true;
+          else if (o.trySteepest) steep[k] = !steep[k];
+        }
+        if (!stop && res == PGN_OS_MINSTEP) {
+          if (minStep[k] == o.minStepMaxCount) stop = true;
+          else if (o.trySteepest) steep[k] = !steep[k];
+        }
+        if (!stop && res == PGN_OS_SUCCESS) {
+          if (opt[k] <= o.tolMaxError) stop = true;
+          else res = PGN_OS_ONGOING;
+        }
+        if (!stop && res == PGN_OS_ONGOING) steep[k] = 0;
+      }
+      if (!stop && its[k] + 1 > o.numMaxIters) {
+        result[k] = PGN_OS_MAXITER;
+        stop = true;
+      }
+      if (stop) st[k] = DONE;
+    }
+  }
+  PGN_HIP(hipStreamSynchronize(s));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int k = 0; k < Kn; ++k) {
+    R[k] = pgn_newton_result{};
+    R[k].result = result[k];
+    R[k].iterations = its[k];
+    R[k].pcg_iterations = pcg[k];
+    R[k].line_search_bisections = bis[k];
+    R[k].energy_evals = evals[k];
+    R[k].hessian_evals = hevals[k];
+    R[k].objective = obj[k];
+    R[k].optimality = opt[k];
+    R[k].seconds = secs;
+  }
+}
+
+// ---- C ABI ---------------------------------------------------------------------
+
+namespace {
+template <class F>
+int bguard(F&& f) {
+  try {
+    f();
+    return PGN_OK;
+  } catch (const Error& e) {
+    return api_guard_set(e.what(), e.code);
+  } catch (const std::bad_alloc&) {
+    return api_guard_set("host allocation failed", PGN_ERR_NOMEM);
+  } catch (const std::exception& e) {
+    return api_guard_set(e.what(), PGN_ERR_INVALID);
+  }
+}
+
+pgn_batch_inst& staged(pgn_batch* B, int k) {
+  PGN_CHECK(B, "null batch");
+  PGN_CHECK(!B->built(), "batch already built");
+  PGN_CHECK(k >= 0 && k < B->K(), "instance index out of range");
+  return B->inst[k];
+}
+
+void check_ids(const int* ids, long count, int n) {
+  for (long q = 0; q < count; ++q) PGN_CHECK(ids[q] >= 0 && ids[q] < n, "node id out of range");
+}
+}  // namespace
+
+extern "C" {
+
+int pgn_batch_create(pgn_ctx* ctx, int precision, pgn_batch** out) {
+  return bguard([&] {
+    PGN_CHECK(ctx && out, "null argument");
+    PGN_CHECK(precision == PGN_FP64 || precision == PGN_MIXED, "unknown precision");
+    auto* B = new pgn_batch();
+    B->ctx = ctx;
+    B->prec = precision;
+    *out = B;
+  });
+}
+
+void pgn_batch_destroy(pgn_batch* B) {
+  if (!B) return;
+  (void)hipSetDevice(B->ctx->device);
+  (void)hipStreamSynchronize(B->ctx->stream);
+  delete B;
+}
+
+int pgn_batch_add_tet(pgn_batch* B, int n_nodes, const double* X0, int n_tets, const int* T,
+                      const unsigned char* material, const double* lame1, const double* lame2,
+                      const double* density, int* instance) {
+  return bguard([&] {
+    PGN_CHECK(B && X0 && T && material && lame1 && lame2 && density, "null argument");
+    PGN_CHECK(!B->built(), "batch already built");
+    PGN_CHECK(n_nodes > 0 && n_tets > 0, "empty instance");
+    check_ids(T, 4L * n_tets, n_nodes);
+    pgn_batch_inst I;
+    I.n = n_nodes;
+    I.X0.assign(X0, X0 + 3L * n_nodes);
+    I.T[0].assign(T, T + 4L * n_tets);
+    for (int e = 0; e < n_tets; ++e)
+      PGN_CHECK(material[e] <= 1, "unknown material kind at element " + std::to_string(e));
+    I.kind[0].assign(material, material + n_tets);
+    I.l1[0].assign(lame1, lame1 + n_tets);
+    I.l2[0].assign(lame2, lame2 + n_tets);
+    I.rho[0].assign(density, density + n_tets);
+    I.extra[0].assign(n_tets, 0.0);
+    I.fixed.assign(3L * n_nodes, 0);
+    B->inst.push_back(std::move(I));
+    if (instance) *instance = B->K() - 1;
+  });
+}
+
+int pgn_batch_add_shell(pgn_batch* B, int n_nodes, const double* X0, int n_tris, const int* F,
+                        const double* lame1, const double* lame2, const double* thickness,
+                        const double* density, int n_hinges, const int* hinges,
+                        const double* bending_k, int* instance) {
+  return bguard([&] {
+    PGN_CHECK(B && X0, "null argument");
+    PGN_CHECK(!B->built(), "batch already built");
+    PGN_CHECK(n_nodes > 0 && n_tris >= 0 && n_hinges >= 0 && n_tris + n_hinges > 0,
+              "empty instance");
+    PGN_CHECK((n_tris == 0 || (F && lame1 && lame2 && thickness && density)) &&
+                  (n_hinges == 0 || (hinges && bending_k)),
+              "null input array");
+    check_ids(F, 3L * n_tris, n_nodes);
+    check_ids(hinges, 4L * n_hinges, n_nodes);
+    pgn_batch_inst I;
+    I.n = n_nodes;
+    I.X0.assign(X0, X0 + 3L * n_nodes);
+    for (long t = 0; t < n_tris; ++t) {
+      for (int a = 0; a < 3; ++a) I.T[1].push_back(F[3 * t + a]);
+      I.T[1].push_back(F[3 * t + 2]);
+      I.kind[1].push_back(2);
+      I.l1[1].push_back(lame1[t]);
+      I.l2[1].push_back(lame2[t]);
+      I.rho[1].push_back(density[t]);
+      I.extra[1].push_back(thickness[t]);
+    }
+    for (long h = 0; h < n_hinges; ++h) {
+      for (int a = 0; a < 4; ++a) I.T[2].push_back(hinges[4 * h + a]);
+      I.kind[2].push_back(3);
+      I.l1[2].push_back(0.0);
+      I.l2[2].push_back(0.0);
+      I.rho[2].push_back(0.0);
+      I.extra[2].push_back(bending_k[h]);
+    }
+    I.fixed.assign(3L * n_nodes, 0);
+    B->inst.push_back(std::move(I));
+    if (instance) *instance = B->K() - 1;
+  });
+}
+
+int pgn_batch_set_fixed(pgn_batch* B, int k, const unsigned char* mask) {
+  return bguard([&] {
+    pgn_batch_inst& I = staged(B, k);
+    PGN_CHECK(mask, "null argument");
+    I.fixed.assign(mask, mask + 3L * I.n);
+  });
+}
+
+int pgn_batch_set_gravity(pgn_batch* B, int k, const double g_ini[3], const double g_end[3],
+                          int num_stages) {
+  return bguard([&] {
+    pgn_batch_inst& I = staged(B, k);
+    PGN_CHECK(g_ini && g_end && num_stages >= 1, "bad argument");
+    I.has_g = true;
+    std::copy(g_ini, g_ini + 3, I.g_ini);
+    std::copy(g_end, g_end + 3, I.g_end);
+    I.g_stages = num_stages;
+  });
+}
+
+int pgn_batch_set_forces(pgn_batch* B, int k, const double* f_ini, const double* f_end,
+                         int num_stages) {
+  return bguard([&] {
+    pgn_batch_inst& I = staged(B, k);
+    PGN_CHECK(f_ini && f_end && num_stages >= 1, "bad argument");
+    I.has_f = true;
+    I.f_ini.assign(f_ini, f_ini + 3L * I.n);
+    I.f_end.assign(f_end, f_end + 3L * I.n);
+    I.f_stages = num_stages;
+  });
+}
+
+int pgn_batch_build(pgn_batch* B) {
+  return bguard([&] {
+    PGN_CHECK(B, "null batch");
+    PGN_CHECK(!B->built(), "batch already built");
+    PGN_HIP(hipSetDevice(B->ctx->device));
+    B->build();
+  });
+}
+
+int pgn_batch_num_instances(const pgn_batch* B) { return B ? B->K() : 0; }
+
+size_t pgn_batch_device_bytes(const pgn_batch* B) {
+  if (!B || !B->built()) return 0;
+  return pgn_model_device_bytes(B->M.get()) + B->noff.bytes() + B->eoff.bytes() +
+         B->node_inst.bytes() + B->elemE.bytes() + B->nodeE.bytes() + B->nodeG2.bytes() +
+         B->xsave.bytes() + B->v0.bytes() + B->dx.bytes() + B->bvec.bytes();
+}
+
+int pgn_batch_reset(pgn_batch* B) {
+  return bguard([&] {
+    PGN_CHECK(B && B->built(), "batch not built");
+    PGN_HIP(hipSetDevice(B->ctx->device));
+    B->reset();
+  });
+}
+
+int pgn_batch_newton(pgn_batch* B, const pgn_newton_opts* opts, pgn_newton_result* res) {
+  return bguard([&] {
+    PGN_CHECK(B && res, "null argument");
+    PGN_CHECK(B->built(), "batch not built");
+    PGN_HIP(hipSetDevice(B->ctx->device));
+    pgn_newton_opts o;
+    pgn_newton_default_opts(&o);
+    if (opts) o = *opts;
+    B->newton(o, res);
+  });
+}
+
+int pgn_batch_get_x(pgn_batch* B, int k, double* x) {
+  return bguard([&] {
+    PGN_CHECK(B && x, "null argument");
+    PGN_CHECK(B->built(), "batch not built");
+    PGN_CHECK(k >= 0 && k < B->K(), "instance index out of range");
+    PGN_HIP(hipSetDevice(B->ctx->device));
+    const long n0 = B->hnoff[k], nk = B->hnoff[k + 1] - n0;
+    PGN_HIP(hipMemcpyAsync(x, B->M->x.p + 3 * n0, 3 * nk * sizeof(double),
+                           hipMemcpyDeviceToHost, B->ctx->stream));
+    PGN_HIP(hipStreamSynchronize(B->ctx->stream));
+  });
+}
+
+int pgn_batch_get_energy(pgn_batch* B, double* energy) {
+  return bguard([&] {
+    PGN_CHECK(B && energy, "null argument");
+    PGN_CHECK(B->built(), "batch not built");
+    PGN_HIP(hipSetDevice(B->ctx->device));
+    B->upload_loads();
+    std::vector<double> E;
+    B->eval_eg(E);
+    for (int k = 0; k < B->K(); ++k) energy[k] = E[4 * k];
+  });
+}
+
+}  // extern "C"

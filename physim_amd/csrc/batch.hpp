@@ -1,0 +1,55 @@
+// batch.hpp -- a fused batch of independent quasi-static solves.
+//
+// Instances (tet meshes and thin shells) are staged on the host, then built
+// into ONE device model over the disjoint union of their nodes and elements:
+// every element kernel, gather, assembly and PCG launch covers the whole
+// batch.  Each instance keeps its own Newton state (load stage, line-search
+// factor, status machine of OptimSolver::SolveFull); the linear solve is one
+// PCG over the block-diagonal batch Hessian, whose blocks do not couple.
+#pragma once
+
+#include <memory>
+
+#include "model.hpp"
+
+struct pgn_batch_inst {
+  int n = 0;
+  std::vector<double> X0;            // 3n
+  std::vector<int> T[3];             // per class (tets, membranes, hinges), 4 per element
+  std::vector<uint8_t> kind[3];
+  std::vector<double> l1[3], l2[3], rho[3], extra[3];
+  std::vector<uint8_t> fixed;        // 3n
+  bool has_g = false;
+  double g_ini[3] = {0, 0, 0}, g_end[3] = {0, 0, 0};
+  int g_stages = 1;
+  bool has_f = false;
+  std::vector<double> f_ini, f_end;  // 3n
+  int f_stages = 1;
+  size_t footprint() const;          // device bytes once built (estimate)
+};
+
+struct pgn_batch {
+  pgn_ctx* ctx = nullptr;
+  int prec = PGN_FP64;
+  std::vector<pgn_batch_inst> inst;
+  std::unique_ptr<pgn_model> M;
+  std::vector<long> hnoff, heoff;
+  std::vector<int> g_cur, f_cur;
+  pgn::DBuf<long> noff, eoff;
+  pgn::DBuf<int> node_inst;
+  pgn::DBuf<double> grav, falpha, fac, elemE, nodeE, nodeG2, out, n2;
+  pgn::DBuf<uint8_t> mode, mask;
+  pgn::DBuf<double> xsave, v0, dx, bvec;
+
+  int K() const { return (int)inst.size(); }
+  bool built() const { return (bool)M; }
+  void build();
+  pgn::BatchArgs args() const;
+  void upload_loads();
+  bool step_load(int k);
+  bool is_loaded(int k) const;
+  // per-instance {E, |g|^2, energy scale, elastic energy} at the current x
+  void eval_eg(std::vector<double>& out4);
+  void reset();
+  void newton(const pgn_newton_opts& o, pgn_newton_result* R);
+};

@@ -1,0 +1,137 @@
+// batch_kernels.hip -- per-instance reductions and updates for a fused batch
+// of independent solves.
+//
+// The batch is one device model whose node and element sets are the
+// disjoint union of its instances'; element, gather, assembly and PCG
+// kernels run once over the whole batch.  These kernels give each instance
+// its own Newton state on top: segmented (one workgroup per instance,
+// fixed-order) energy / gradient-norm reductions and masked per-instance
+// vector updates.  Instances own contiguous node ranges and, per element
+// class, contiguous element ranges.
+#include "kernels.hpp"
+#include "reduce.hpp"
+
+namespace pgn {
+
+constexpr int BR_BLOCK = 256;
+
+__global__ __launch_bounds__(BR_BLOCK) void batch_reduce_kernel(
+    BatchArgs A, const double* __restrict__ elemE, const double* __restrict__ nodeE,
+    const double* __restrict__ nodeG2, double* __restrict__ out) {
+  __shared__ double red[2 * BR_BLOCK / 64];
+  const int k = blockIdx.x;
+  double el = 0.0, ld = 0.0, gg = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const long e0 = A.eoff[c * (A.K + 1) + k], e1 = A.eoff[c * (A.K + 1) + k + 1];
+    for (long e = e0 + threadIdx.x; e < e1; e += BR_BLOCK) el += elemE[e];
+  }
+  const long n0 = A.noff[k], n1 = A.noff[k + 1];
+  for (long i = n0 + threadIdx.x; i < n1; i += BR_BLOCK) {
+    ld += nodeE[i];
+    gg += nodeG2[i];
+  }
+  block_sum2<BR_BLOCK>(el, ld, red);
+  const double sg = block_sum<BR_BLOCK>(gg, red);
+  if (threadIdx.x == 0) {
+    out[4 * k] = el + ld;
+    out[4 * k + 1] = sg;
+    out[4 * k + 2] = fabs(el) + fabs(ld);
+    out[4 * k + 3] = el;
+  }
+}
+
+__global__ __launch_bounds__(BR_BLOCK) void batch_norm2_kernel(BatchArgs A,
+                                                               const double* __restrict__ v,
+                                                               double* __restrict__ out) {
+  __shared__ double red[BR_BLOCK / 64];
+  const int k = blockIdx.x;
+  const long d0 = 3 * A.noff[k], d1 = 3 * A.noff[k + 1];
+  double s = 0.0;
+  for (long d = d0 + threadIdx.x; d < d1; d += BR_BLOCK) s += v[d] * v[d];
+  s = block_sum<BR_BLOCK>(s, red);
+  if (threadIdx.x == 0) out[k] = s;
+}
+
+__global__ __launch_bounds__(256) void batch_step_kernel(long nd, const int* __restrict__ inst,
+                                                         const double* __restrict__ fac,
+                                                         const double* __restrict__ xsave,
+                                                         const double* __restrict__ dx,
+                                                         double* __restrict__ x) {
+  const long d = blockIdx.x * 256L + threadIdx.x;
+  if (d >= nd) return;
+  const double f = fac[inst[d / 3]];
+  if (f >= 0.0) x[d] = xsave[d] + f * dx[d];
+}
+
+__global__ __launch_bounds__(256) void batch_rhs_kernel(long nd, const int* __restrict__ inst,
+                                                        const uint8_t* __restrict__ mode,
+                                                        const double* __restrict__ g,
+                                                        const double* __restrict__ xsave,
+                                                        const double* __restrict__ v0,
+                                                        double* __restrict__ b,
+                                                        double* __restrict__ dx) {
+  const long d = blockIdx.x * 256L + threadIdx.x;
+  if (d >= nd) return;
+  const int md = mode[inst[d / 3]];
+  if (md == 1) {
+    b[d] = -g[d];
+    dx[d] = xsave[d] - v0[d];
+  } else if (md == 2) {
+    b[d] = 0.0;
+    dx[d] = -g[d];
+  } else {
+    b[d] = 0.0;
+    dx[d] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_copy_kernel(long nd, const int* __restrict__ inst,
+                                                         const uint8_t* __restrict__ mask,
+                                                         const double* __restrict__ src,
+                                                         double* __restrict__ dst) {
+  const long d = blockIdx.x * 256L + threadIdx.x;
+  if (d >= nd) return;
+  if (mask[inst[d / 3]]) dst[d] = src[d];
+}
+
+static inline unsigned grid_of(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_batch_reduce(hipStream_t s, const BatchArgs& a, const double* elemE,
+                         const double* nodeE, const double* nodeG2, double* out) {
+  if (a.K == 0) return;
+  hipLaunchKernelGGL(batch_reduce_kernel, dim3(a.K), dim3(BR_BLOCK), 0, s, a, elemE, nodeE,
+                     nodeG2, out);
+}
+
+void launch_batch_norm2(hipStream_t s, const BatchArgs& a, const double* v, double* out) {
+  if (a.K == 0) return;
+  hipLaunchKernelGGL(batch_norm2_kernel, dim3(a.K), dim3(BR_BLOCK), 0, s, a, v, out);
+}
+
+void launch_batch_step(hipStream_t s, const BatchArgs& a, long n_nodes, const double* fac,
+                       const double* xsave, const double* dx, double* x) {
+  const long nd = 3 * n_nodes;
+  if (nd == 0) return;
+  hipLaunchKernelGGL(batch_step_kernel, dim3(grid_of(nd, 256)), dim3(256), 0, s, nd,
+                     a.node_inst, fac, xsave, dx, x);
+}
+
+void launch_batch_rhs(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mode,
+                      const double* g, const double* xsave, const double* v0, double* b,
+                      double* dx) {
+  const long nd = 3 * n_nodes;
+  if (nd == 0) return;
+  hipLaunchKernelGGL(batch_rhs_kernel, dim3(grid_of(nd, 256)), dim3(256), 0, s, nd,
+                     a.node_inst, mode, g, xsave, v0, b, dx);
+}
+
+void launch_batch_copy(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mask,
+                       const double* src, double* dst) {
+  const long nd = 3 * n_nodes;
+  if (nd == 0) return;
+  hipLaunchKernelGGL(batch_copy_kernel, dim3(grid_of(nd, 256)), dim3(256), 0, s, nd,
+                     a.node_inst, mask, src, dst);
+}
+
+}  // namespace pgn

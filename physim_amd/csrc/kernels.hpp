@@ -66,6 +66,14 @@ struct NodeArgs {
   const double* f_ini;   // 3n or null
   const double* f_end;   // 3n or null
   double f_alpha;
+  // batches of independent instances: when node_inst is set, node i takes
+  // gravity grav_inst[3 inst] and force ramp falpha_inst[inst] of its
+  // instance, and its load energy and |g_i|^2 go to nodeE / nodeG2
+  const int* node_inst = nullptr;
+  const double* grav_inst = nullptr;
+  const double* falpha_inst = nullptr;
+  double* nodeE = nullptr;
+  double* nodeG2 = nullptr;
 };
 void launch_node_gather(hipStream_t s, bool fp32_eg, const NodeArgs& a,
                         const void* eg, double* g, double* node_partials);
@@ -161,6 +169,34 @@ void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
 // y = A x in precision T with `lpr` lanes per block row (tuning, timing)
 void launch_spmv(hipStream_t s, bool fp32, int n, const long* rowptr,
                  const int* bcol, const void* bval, const void* x, void* y, int lpr);
+
+// ---- batches of independent instances ---------------------------------------
+// Instance k owns nodes [noff[k], noff[k+1]) and, per element class c
+// (0 tets, 1 membranes, 2 hinges), elements [eoff[c*(K+1)+k], eoff[c*(K+1)+k+1]).
+struct BatchArgs {
+  int K;
+  const long* noff;
+  const long* eoff;
+  const int* node_inst;
+};
+// out[4k..4k+2] = {E_k, |g_k|^2, |E_el| + |E_load|} from per-element energies
+// and per-node load energies / squared gradients (one workgroup per
+// instance, fixed order)
+void launch_batch_reduce(hipStream_t s, const BatchArgs& a, const double* elemE,
+                         const double* nodeE, const double* nodeG2, double* out);
+// out[k] = |v_k|^2 over the instance's 3 DoFs per node
+void launch_batch_norm2(hipStream_t s, const BatchArgs& a, const double* v, double* out);
+// x = xsave + fac[k] dx on instances with fac[k] >= 0 (others untouched)
+void launch_batch_step(hipStream_t s, const BatchArgs& a, long n_nodes, const double* fac,
+                       const double* xsave, const double* dx, double* x);
+// per instance mode[k]: 0 -> b = 0, dx = 0; 1 (Newton) -> b = -g,
+// dx = xsave - v0 (warm start); 2 (steepest) -> b = 0, dx = -g
+void launch_batch_rhs(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mode,
+                      const double* g, const double* xsave, const double* v0, double* b,
+                      double* dx);
+// dst = src on instances with mask[k] != 0
+void launch_batch_copy(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mask,
+                       const double* src, double* dst);
 
 // small vector helpers
 void launch_cast_f64_to_f32(hipStream_t s, long n, const double* a, float* b);

@@ -99,6 +99,51 @@ def cantilever(dims=(10, 10, 40), scale=(0.1, 0.1, 0.4), young=1e5, poisson=0.3,
                  np.full(m, float(density)), fixed, G_EARTH, stages, tol=tol)
 
 
+def batch_beam(rng, cell=0.0025, name=None, dmin=(4, 4, 16), dmax=(24, 24, 96)):
+    """One beam of the 256-instance batch: hex-grid dims U{4..24} x U{4..24} x
+    U{16..96} of `cell`-sized cells, CoNH or StVK (50/50), Young's modulus
+    lognormal(ln 1e5, 0.5), Poisson U(0.25, 0.45), density 1000 kg/m^3; one of
+    the 6 faces (uniform) clamped; gravity -9.81 y plus a traction of
+    U(0, 50) Pa in a random direction on the opposite face, lumped to its
+    nodes (each face cell gives a quarter of its force to each corner); both
+    ramped over 10 load stages."""
+    dims = tuple(int(rng.integers(lo, hi + 1)) for lo, hi in zip(dmin, dmax))
+    scale = tuple(d * cell for d in dims)
+    V, T = tet_grid(scale, dims)
+    m = T.shape[0]
+    E = np.exp(np.log(1e5) + 0.5 * rng.standard_normal(m))
+    nu = rng.uniform(0.25, 0.45)
+    l1, l2 = lame(E, nu)
+    mat = np.full(m, int(rng.integers(0, 2)), dtype=np.uint8)
+    face = int(rng.integers(0, 6))
+    ax, hi = face // 2, face % 2
+    lo_v, hi_v = V[:, ax].min(), V[:, ax].max()
+    on_clamp = np.abs(V[:, ax] - (hi_v if hi else lo_v)) < 1e-12
+    on_load = np.abs(V[:, ax] - (lo_v if hi else hi_v)) < 1e-12
+    fixed = np.zeros((V.shape[0], 3), dtype=np.uint8)
+    fixed[on_clamp] = 1
+    # lumped traction: tributary area of each node on the loaded face
+    others = [a for a in range(3) if a != ax]
+    area = np.ones(V.shape[0])
+    for a in others:
+        c = V[:, a]
+        edge = (np.abs(c - c.min()) < 1e-12) | (np.abs(c - c.max()) < 1e-12)
+        area *= np.where(edge, 0.5 * cell, cell)
+    t = rng.uniform(0.0, 50.0)
+    d = rng.standard_normal(3)
+    d /= np.linalg.norm(d)
+    forces = np.zeros_like(V)
+    forces[on_load] = t * area[on_load, None] * d[None, :]
+    return Scene(name or f"beam{dims}_{'stvk' if mat[0] else 'conh'}", V, T, mat, l1, l2,
+                 np.full(m, 1000.0), fixed, G_EARTH, 10, forces=forces)
+
+
+def batch_beams(count=256, seed=3, cell=0.0025):
+    """The 256-instance fused-batch workload (seeded)."""
+    rng = np.random.default_rng(seed)
+    return [batch_beam(rng, cell) for _ in range(count)]
+
+
 def config1():
     """10x10x40 CoNH cantilever, 20,000 tets, FP64 reference case."""
     return cantilever(name="beam_10x10x40_conh")

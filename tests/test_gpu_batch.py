@@ -1,0 +1,109 @@
+"""Fused batches (pgn_batch) on the GPU: every instance keeps its own Newton
+state, so its result matches the FP64 oracle's solo solve and the same
+problem solved alone, within 1e-6 x its bounding-box diagonal; per-instance
+energies match single models to 1e-12 relative."""
+import sys
+
+import numpy as np
+import pytest
+
+import physim_amd as pg
+from oracle import orc
+from physim_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+OPTS = dict(pg.CONVERGE_OPTS, numMaxIters=200)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _small_mix(seed):
+    rng = np.random.default_rng(seed)
+    tets = [scenes.batch_beam(rng, cell=0.01, dmin=(2, 2, 4), dmax=(4, 4, 10)) for _ in range(5)]
+    shells = [scenes.cloth(int(n), size=0.2, stages=10) for n in (5, 7)]
+    return tets + shells
+
+
+def _oracle(scene):
+    if hasattr(scene, "hinges"):
+        T, kind, l1, l2, rho, extra = scene.oracle_arrays()
+        O = orc.Model(scene.X0, T, kind, l1, l2, rho, extra)
+    else:
+        O = orc.Model(scene.X0, scene.T, scene.material, scene.lame1, scene.lame2,
+                      scene.density)
+    scene.apply(O)
+    return O
+
+
+def _solo(ctx, scene, precision):
+    if hasattr(scene, "hinges"):
+        G = pg.ShellModel(ctx, scene.X0, scene.F, scene.lame1, scene.lame2, scene.thickness,
+                          scene.density, scene.hinges, scene.bending_k, precision)
+    else:
+        G = pg.TetModel(ctx, scene.X0, scene.T, scene.material, scene.lame1, scene.lame2,
+                        scene.density, precision)
+    scene.apply(G)
+    return G
+
+
+def test_batch_energy_matches_single_models(ctx):
+    _say("batch energies vs single models")
+    sc = _small_mix(1)
+    B = pg.Batch(ctx, pg.FP64)
+    for s in sc:
+        B.add_scene(s)
+    B.build()
+    E = B.energy()
+    for k, s in enumerate(sc):
+        G = _solo(ctx, s, pg.FP64)
+        e = G.energy()
+        assert abs(E[k] - e) <= 1e-12 * (abs(e) + 1e-12), k
+        G.close()
+
+
+@pytest.mark.parametrize("precision", [pg.FP64, pg.MIXED])
+def test_batch_newton_matches_oracle(ctx, precision):
+    _say(f"batch newton precision={precision}")
+    sc = _small_mix(2)
+    B = pg.Batch(ctx, precision)
+    for s in sc:
+        B.add_scene(s)
+    B.build()
+    res = B.newton(**OPTS)
+    for k, s in enumerate(sc):
+        O = _oracle(s)
+        ro = O.newton(**OPTS)
+        _say(f"  {s.name}: gpu {res[k]['result']} {res[k]['iterations']} its, oracle "
+             f"{ro['result']} {ro['iterations']} its")
+        assert ro["result"] == "SUCCESS", (k, ro)
+        assert res[k]["result"] == "SUCCESS" and res[k]["optimality"] <= 1e-8, (k, res[k])
+        err = np.abs(B.get_x(k) - O.get_x().reshape(-1, 3)).max()
+        assert err <= 1e-6 * s.bbox_diag(), (k, err)
+
+
+def test_batch_result_same_alone_and_in_mix(ctx):
+    _say("batch alone vs mix")
+    sc = _small_mix(3)
+    B = pg.Batch(ctx, pg.MIXED)
+    for s in sc:
+        B.add_scene(s)
+    B.build()
+    B.newton(**OPTS)
+    for k in (1, 5):
+        A = pg.Batch(ctx, pg.MIXED)
+        A.add_scene(sc[k])
+        A.build()
+        r = A.newton(**OPTS)
+        assert r[0]["result"] == "SUCCESS"
+        err = np.abs(A.get_x(0) - B.get_x(k)).max()
+        assert err <= 1e-6 * sc[k].bbox_diag(), (k, err)
+        A.close()
+    # the batch can be solved again from rest with the same result
+    x1 = B.get_x(2).copy()
+    B.reset()
+    B.newton(**OPTS)
+    assert np.abs(B.get_x(2) - x1).max() <= 1e-6 * sc[2].bbox_diag()

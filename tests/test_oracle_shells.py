@@ -190,3 +190,23 @@ def test_cloth_newton_converges():
     assert r["result"] == "SUCCESS", r
     x = M.get_x().reshape(-1, 3)
     assert x[:, 1].min() < -1e-3
+
+
+def test_batch_beam_traction_lumping_closed_form():
+    """The lumped traction of a batch beam sums to traction x face area."""
+    rng = np.random.default_rng(9)
+    for _ in range(5):
+        s = scenes.batch_beam(rng, cell=0.01, dmin=(2, 2, 4), dmax=(6, 6, 12))
+        f = s.forces.sum(0)
+        loaded = np.abs(s.forces).sum(1) > 0
+        if not loaded.any():
+            continue
+        X = s.X0[loaded]
+        ext = X.max(0) - X.min(0)
+        area = np.prod(ext[ext > 1e-12])
+        t = np.linalg.norm(f) / area
+        assert 0.0 <= t <= 50.0
+        # every loaded node carries the same direction
+        d = s.forces[loaded] / np.linalg.norm(s.forces[loaded], axis=1)[:, None]
+        assert np.abs(d - d[0]).max() < 1e-12
+        assert s.fixed.any()
