@@ -265,6 +265,51 @@ int pgn_batch_get_x(pgn_batch* batch, int instance, double* x);
 /* per-instance total energy at the current x and load stages */
 int pgn_batch_get_energy(pgn_batch* batch, double* energy);
 
+/* ---- request loop (serving) ---------------------------------------------- */
+/* A persistent loop that solves independent requests on one GPU.  Requests
+ * are copied at submit and queued; each lane (a worker thread with its own
+ * HIP stream) admits queued requests in arrival order while their estimated
+ * device footprint fits hbm_budget_bytes, solves them as one fused batch
+ * (pgn_batch semantics: every request keeps its own Newton state) and frees
+ * their memory.  Requests arriving while batches run are taken by the next
+ * free lane, so solves overlap on the device.  No reference counterpart:
+ * PhySim solves one Simulable per OptimSolver call (Simulator.cpp:38-72). */
+typedef struct {
+  size_t hbm_budget_bytes; /* device memory the resident requests may hold */
+  int lanes;               /* concurrent batches (streams), 1..16          */
+  int max_batch;           /* requests per batch                           */
+  int precision;           /* PGN_FP64 or PGN_MIXED                        */
+  pgn_newton_opts newton;  /* solver options of every request              */
+} pgn_server_opts;
+typedef struct pgn_server pgn_server;
+void pgn_server_default_opts(pgn_server_opts* o);
+int pgn_server_create(int device, const pgn_server_opts* opts, pgn_server** out);
+/* stops the lanes after their current batch; queued requests are dropped */
+void pgn_server_destroy(pgn_server* server);
+/* submit a tet or thin-shell request (mesh arguments as pgn_model_create_*);
+ * fixed: 3 per node or NULL; gravity: 3 or NULL; forces: 3 per node or NULL;
+ * both loads ramped from zero over `stages` load stages */
+int pgn_server_submit_tet(pgn_server* server, int n_nodes, const double* X0, int n_tets,
+                          const int* T, const unsigned char* material,
+                          const double* lame1, const double* lame2,
+                          const double* density, const unsigned char* fixed,
+                          const double* gravity, const double* forces, int stages,
+                          long* ticket);
+int pgn_server_submit_shell(pgn_server* server, int n_nodes, const double* X0,
+                            int n_tris, const int* F, const double* lame1,
+                            const double* lame2, const double* thickness,
+                            const double* density, int n_hinges, const int* hinges,
+                            const double* bending_k, const unsigned char* fixed,
+                            const double* gravity, const double* forces, int stages,
+                            long* ticket);
+int pgn_server_poll(pgn_server* server, long ticket, int* done);
+/* blocks until the request is solved; x: 3 per node (or NULL); latency_s:
+ * submit -> result on host; queue_s: submit -> its batch started */
+int pgn_server_wait(pgn_server* server, long ticket, double* x, pgn_newton_result* res,
+                    double* latency_s, double* queue_s);
+int pgn_server_stats(pgn_server* server, long* batches, long* solved,
+                     size_t* bytes_in_use);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Runs `reps` launches of one kernel on the model's current state, on the
  * context stream, and returns the mean launch time (HIP events) in ms.

@@ -41,7 +41,10 @@ EXPORTS = [
     "pgn_batch_destroy", "pgn_batch_add_tet", "pgn_batch_add_shell", "pgn_batch_set_fixed",
     "pgn_batch_set_gravity", "pgn_batch_set_forces", "pgn_batch_build",
     "pgn_batch_num_instances", "pgn_batch_device_bytes", "pgn_batch_reset",
-    "pgn_batch_newton", "pgn_batch_get_x", "pgn_batch_get_energy", "pgn_rods_create", "pgn_rods_destroy",
+    "pgn_batch_newton", "pgn_batch_get_x", "pgn_batch_get_energy", "pgn_server_default_opts",
+    "pgn_server_create", "pgn_server_destroy", "pgn_server_submit_tet",
+    "pgn_server_submit_shell", "pgn_server_poll", "pgn_server_wait", "pgn_server_stats",
+    "pgn_rods_create", "pgn_rods_destroy",
     "pgn_rods_num_dofs", "pgn_rods_set_fixed", "pgn_rods_set_gravity",
     "pgn_rods_reset", "pgn_rods_set_x", "pgn_rods_get_x", "pgn_rods_get_energy",
     "pgn_rods_get_gradient", "pgn_rods_get_hessian_dense", "pgn_rods_get_frames",
@@ -78,6 +81,12 @@ class NewtonOpts(ctypes.Structure):
                 ("lsNumRegTrials", ctypes.c_int), ("precond", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("adaptive_tol", ctypes.c_int),
                 ("ls_noise_guard", ctypes.c_double)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("lanes", ctypes.c_int),
+                ("max_batch", ctypes.c_int), ("precision", ctypes.c_int),
+                ("newton", NewtonOpts)]
 
 
 class NewtonResult(ctypes.Structure):
@@ -164,6 +173,21 @@ def lib():
                                        ctypes.POINTER(NewtonResult)]
         L.pgn_batch_get_x.argtypes = [_vp, ctypes.c_int, _dp]
         L.pgn_batch_get_energy.argtypes = [_vp, _dp]
+        L.pgn_server_default_opts.argtypes = [ctypes.POINTER(ServerOpts)]
+        L.pgn_server_default_opts.restype = None
+        L.pgn_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerOpts),
+                                        ctypes.POINTER(_vp)]
+        L.pgn_server_destroy.argtypes = [_vp]
+        L.pgn_server_destroy.restype = None
+        L.pgn_server_submit_tet.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, _ip, _ucp,
+                                            _dp, _dp, _dp, _ucp, _dp, _dp, ctypes.c_int, _lp]
+        L.pgn_server_submit_shell.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp,
+                                              _dp, _dp, _dp, ctypes.c_int, _ip, _dp, _ucp, _dp,
+                                              _dp, ctypes.c_int, _lp]
+        L.pgn_server_poll.argtypes = [_vp, ctypes.c_long, _ip]
+        L.pgn_server_wait.argtypes = [_vp, ctypes.c_long, _dp, ctypes.POINTER(NewtonResult),
+                                      _dp, _dp]
+        L.pgn_server_stats.argtypes = [_vp, _lp, _lp, ctypes.POINTER(ctypes.c_size_t)]
         L.pgn_rods_create.argtypes = [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
                                       _dp, ctypes.c_int, ctypes.POINTER(_vp)]
         L.pgn_rods_destroy.argtypes = [_vp]
@@ -520,6 +544,88 @@ class Batch:
         e = np.zeros(self.K)
         _ok(lib().pgn_batch_get_energy(self._h, _p(e)))
         return e
+
+
+class Server:
+    """The request loop (pgn_server): submit scenes from any thread, wait for
+    their results.  Lanes run fused batches on their own streams; admission
+    is by estimated device footprint against hbm_budget_bytes."""
+
+    def __init__(self, device=0, hbm_budget_bytes=64 << 30, lanes=4, max_batch=64,
+                 precision=MIXED, **newton):
+        o = ServerOpts()
+        lib().pgn_server_default_opts(ctypes.byref(o))
+        o.hbm_budget_bytes = int(hbm_budget_bytes)
+        o.lanes = lanes
+        o.max_batch = max_batch
+        o.precision = precision
+        for k, v in newton.items():
+            setattr(o.newton, k, v)
+        h = _vp()
+        _ok(lib().pgn_server_create(device, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self._n = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pgn_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, scene):
+        """Queue a scenes.Scene (tets) or scenes.ShellScene; returns a ticket."""
+        t = ctypes.c_long()
+        X0 = _f64(scene.X0).reshape(-1)
+        n = X0.size // 3
+        fixed = np.ascontiguousarray(scene.fixed, dtype=np.uint8).reshape(-1)
+        grav = _f64(scene.gravity)
+        forces = getattr(scene, "forces", None)
+        fp = _p(_f64(forces).reshape(-1)) if forces is not None else None
+        if hasattr(scene, "hinges"):
+            F = np.ascontiguousarray(scene.F, dtype=np.int32).reshape(-1)
+            Hn = np.ascontiguousarray(scene.hinges, dtype=np.int32).reshape(-1)
+            nt, nh = F.size // 3, Hn.size // 4
+            b = lambda a, c: _f64(np.broadcast_to(a, (c,)))
+            _ok(lib().pgn_server_submit_shell(
+                self._h, n, _p(X0), nt, _p(F, _ip), _p(b(scene.lame1, nt)),
+                _p(b(scene.lame2, nt)), _p(b(scene.thickness, nt)), _p(b(scene.density, nt)),
+                nh, _p(Hn, _ip), _p(b(scene.bending_k, nh)), _p(fixed, _ucp), _p(grav), fp,
+                int(scene.stages), ctypes.byref(t)))
+        else:
+            T = np.ascontiguousarray(scene.T, dtype=np.int32).reshape(-1)
+            m = T.size // 4
+            b = lambda a: _f64(np.broadcast_to(a, (m,)))
+            mat = np.ascontiguousarray(np.broadcast_to(scene.material, (m,)), dtype=np.uint8)
+            _ok(lib().pgn_server_submit_tet(
+                self._h, n, _p(X0), m, _p(T, _ip), _p(mat, _ucp), _p(b(scene.lame1)),
+                _p(b(scene.lame2)), _p(b(scene.density)), _p(fixed, _ucp), _p(grav), fp,
+                int(scene.stages), ctypes.byref(t)))
+        self._n[t.value] = n
+        return t.value
+
+    def poll(self, ticket):
+        d = ctypes.c_int()
+        _ok(lib().pgn_server_poll(self._h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket):
+        """(x (n, 3), result dict, latency s, queueing s)"""
+        x = np.zeros(3 * self._n.pop(ticket))
+        r = NewtonResult()
+        lat = ctypes.c_double()
+        q = ctypes.c_double()
+        _ok(lib().pgn_server_wait(self._h, ticket, _p(x), ctypes.byref(r), ctypes.byref(lat),
+                                  ctypes.byref(q)))
+        return x.reshape(-1, 3), _result_dict(r), lat.value, q.value
+
+    def stats(self):
+        b = ctypes.c_long()
+        sv = ctypes.c_long()
+        u = ctypes.c_size_t()
+        _ok(lib().pgn_server_stats(self._h, ctypes.byref(b), ctypes.byref(sv), ctypes.byref(u)))
+        return {"batches": b.value, "solved": sv.value, "bytes_in_use": u.value}
 
 
 def host_bsr_pattern(n_nodes, T):
