@@ -121,6 +121,21 @@ def main():
                     help="for hardware-counter runs: at full load, one Hessian, one "
                          "PCG solve capped at 50 iterations, 5 E+g+H launches and 5 "
                          "calibration copies; no solve timing, no JSON line")
+    ap.add_argument("--workload", default="beam",
+                    choices=["beam", "batch", "rods", "serving"],
+                    help="beam (default): the 1.28M-tet beam; batch: 256 fused beams; "
+                         "rods: 4096 helices; serving: Poisson stream of beams and cloths")
+    ap.add_argument("--count", type=int, default=0,
+                    help="instances / rods / requests of the batch, rods and serving "
+                         "workloads (0 = the workload's default)")
+    ap.add_argument("--rate", type=float, default=100.0,
+                    help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--max-tets", type=int, default=500_000,
+                    help="serving: largest beam request (tets)")
+    ap.add_argument("--max-cloth", type=int, default=512,
+                    help="serving: largest cloth request (nodes per side)")
+    ap.add_argument("--rod-density", type=float, default=3.14e-6,
+                    help="rods: line density, kg per metre of rod")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and "
@@ -130,6 +145,15 @@ def main():
 
     import physim_amd as pg
     from physim_amd import scenes
+
+    if args.workload != "beam":
+        from physim_amd import workloads
+        line = getattr(workloads, args.workload)(args, log)
+        line.update(metric=METRIC, n_gpus=1, steps=line.get("steps", args.steps),
+                    warmup=line.get("warmup", args.warmup), higher_is_better=True,
+                    vs_baseline=None, data="synthetic")
+        print(json.dumps(line), flush=True)
+        return
 
     t0 = time.perf_counter()
     scene = scenes.config2()

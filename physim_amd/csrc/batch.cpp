@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <new>
 
 #include "solver.hpp"
@@ -187,6 +188,220 @@ void pgn_batch::reset() {
   M->hess_valid = false;
 }
 
+// ---- segmented PCG and linear solve -------------------------------------------
+
+void pgn_batch::seg_plan() {
+  const int Kn = K();
+  const int lpr = M->pcg_lpr;
+  const int rpw = seg_rows_per_wg(lpr);
+  std::vector<int> G(Kn), wg0(Kn), ws, wl;
+  std::vector<long> r0(Kn), r1(Kn);
+  int W = 0;
+  for (int k = 0; k < Kn; ++k) {
+    const long rows = hnoff[k + 1] - hnoff[k];
+    long g = (rows + rpw - 1) / rpw;
+    g = std::max(1L, std::min<long>(g, SEG_MAXG));
+    G[k] = (int)g;
+    wg0[k] = W;
+    r0[k] = hnoff[k];
+    r1[k] = hnoff[k + 1];
+    for (int j = 0; j < G[k]; ++j) {
+      ws.push_back(k);
+      wl.push_back(j);
+    }
+    W += G[k];
+  }
+  hipStream_t s = ctx->stream;
+  wg_seg.upload(ws, s);
+  wg_loc.upload(wl, s);
+  seg_G.upload(G, s);
+  seg_wg0.upload(wg0, s);
+  seg_row0.upload(r0, s);
+  seg_row1.upload(r1, s);
+  seg_part1.alloc(W);
+  seg_part2.alloc(4L * W);
+  seg_partb.alloc(W);
+  seg_sc.alloc(Kn);
+  seg_tol.alloc(Kn);
+  hseg_sc.resize(Kn);
+  seg_W = W;
+  seg_lpr = lpr;
+  PGN_HIP(hipStreamSynchronize(s));
+}
+
+SegPcgArgs pgn_batch::seg_args(int precond) const {
+  SegPcgArgs a;
+  a.K = K();
+  a.W = seg_W;
+  a.wg_seg = wg_seg.p;
+  a.wg_loc = wg_loc.p;
+  a.seg_G = seg_G.p;
+  a.seg_wg0 = seg_wg0.p;
+  a.seg_row0 = seg_row0.p;
+  a.seg_row1 = seg_row1.p;
+  a.tol = seg_tol.p;
+  a.rowptr = M->rowptr.p;
+  a.bcol = M->bcol.p;
+  a.bval = M->bval.p;
+  a.dinv = M->dinv.p;
+  a.binv = M->binv.p;
+  a.precond = precond;
+  a.lpr = seg_lpr;
+  const size_t v = 3L * M->n * M->tsize();
+  char* base = M->pv.p;
+  a.x = base;
+  a.r = base + v;
+  a.z = base + 2 * v;
+  a.p0 = base + 3 * v;
+  a.p1 = base + 4 * v;
+  a.q = base + 5 * v;
+  a.part1 = seg_part1.p;
+  a.part2 = seg_part2.p;
+  a.partb = seg_partb.p;
+  a.sc = seg_sc.p;
+  return a;
+}
+
+void pgn_batch::seg_pcg(const std::vector<double>& tol, int maxit, int precond, bool x_zero,
+                        std::vector<int>& iters) {
+  hipStream_t s = ctx->stream;
+  const int Kn = K();
+  const bool f32 = M->hfp32;
+  seg_tol.upload(tol, s);
+  const SegPcgArgs a = seg_args(precond);
+  launch_seg_init(s, f32, a, M->pb.p, x_zero ? 1 : 0);
+  int k = 0, chunk = 8;
+  for (;;) {
+    const int c = std::min(chunk, maxit - k);
+    for (int j = 0; j < c; ++j) {
+      launch_seg_k1(s, f32, a, k + j);
+      launch_seg_k2(s, f32, a, k + j);
+    }
+    k += c;
+    launch_seg_status(s, a, k, maxit);
+    seg_sc.download(hseg_sc.data(), Kn, s);
+    PGN_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    for (int q = 0; q < Kn; ++q) all &= (hseg_sc[q].done != 0);
+    if (all || k >= maxit) break;
+    chunk = std::min(2 * chunk, 64);
+  }
+  iters.resize(Kn);
+  for (int q = 0; q < Kn; ++q) iters[q] = hseg_sc[q].iters;
+}
+
+void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8_t>& active,
+                             const std::vector<double>& tol, const pgn_linear_opts& o,
+                             std::vector<int>& iters) {
+  hipStream_t s = ctx->stream;
+  const int Kn = K();
+  const long n = M->n, N = 3 * n;
+  const BatchArgs ba = args();
+  const double inf = std::numeric_limits<double>::infinity();
+  M->tune();
+  if (seg_lpr != M->pcg_lpr) seg_plan();
+  if (xprev.n < (size_t)N) xprev.alloc(N);
+  if (kvec.n < (size_t)Kn) kvec.alloc(Kn);
+  if (kvec2.n < (size_t)Kn) kvec2.alloc(Kn);
+  PGN_HIP(hipMemcpyAsync(xprev.p, x, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  iters.assign(Kn, 0);
+  std::vector<uint8_t> F = active;
+  std::vector<double> regT(Kn, -1.0), h(Kn), shift(Kn);
+  auto download_k = [&](const double* d, std::vector<double>& out) {
+    PGN_HIP(hipMemcpyAsync(out.data(), d, Kn * sizeof(double), hipMemcpyDeviceToHost, s));
+    PGN_HIP(hipStreamSynchronize(s));
+  };
+  const int regs = std::max(o.reg_iters, 1);
+  for (int it = 0; it < regs; ++it) {
+    bool any = false;
+    for (int k = 0; k < Kn; ++k) any |= F[k] != 0;
+    if (!any) break;
+    if (it != 0) {
+      // LinearSolver_EigenCG.cpp:73 + LinearSolver.cpp:57-66 per instance:
+      // A += 1e-6 |A_lower|_F 10^(it-1) I, cumulative over the tries
+      if (regT[0] < 0.0) {
+        launch_batch_frob(s, M->hfp32, ba, M->rowptr.p, M->brow.p, M->bcol.p, M->bval.p,
+                          kvec.p);
+        download_k(kvec.p, h);
+        for (int k = 0; k < Kn; ++k) regT[k] = 1e-6 * std::sqrt(h[k]);
+      }
+      for (int k = 0; k < Kn; ++k) shift[k] = F[k] ? regT[k] * std::pow(10.0, it - 1) : 0.0;
+      kvec2.upload(shift.data(), Kn, s);
+      launch_batch_add_diag(s, M->hfp32, ba, n, M->diag.p, M->bval.p, kvec2.p);
+      M->update_precond();
+    }
+    mask.upload(F, s);
+    launch_batch_copy(s, ba, n, mask.p, xprev.p, x);
+    // ---- one unregularized solve of the instances in F ----
+    if (!M->hfp32) {
+      std::vector<double> tv(Kn);
+      for (int k = 0; k < Kn; ++k) tv[k] = F[k] ? tol[k] : inf;
+      PGN_HIP(hipMemcpyAsync(M->pb.p, b, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      PGN_HIP(hipMemcpyAsync(M->pv.p, x, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      std::vector<int> itk;
+      seg_pcg(tv, o.max_iters, o.precond, false, itk);
+      PGN_HIP(hipMemcpyAsync(x, M->pv.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      for (int k = 0; k < Kn; ++k)
+        if (F[k]) iters[k] += itk[k];
+    } else {
+      // FP32 PCG corrections inside FP64 iterative refinement, per instance
+      launch_batch_dot(s, ba, b, b, kvec.p);
+      std::vector<double> bb(Kn), rr(Kn);
+      download_k(kvec.p, bb);
+      std::vector<uint8_t> act(Kn, 0), zero(Kn, 0);
+      bool anyz = false;
+      for (int k = 0; k < Kn; ++k) {
+        if (!F[k]) continue;
+        if (bb[k] == 0.0) {
+          zero[k] = 1;  // b = 0: x = 0
+          anyz = true;
+        } else {
+          act[k] = 1;
+        }
+      }
+      if (anyz) {
+        mask.upload(zero, s);
+        launch_batch_copy(s, ba, n, mask.p, b, x);
+      }
+      for (;;) {
+        launch_residual64(s, true, (int)n, M->rowptr.p, M->bcol.p, M->bval.p, x, b, M->r64.p,
+                          M->red64.p, M->scal.p + 4);
+        launch_batch_norm2(s, ba, M->r64.p, kvec.p);
+        download_k(kvec.p, rr);
+        bool any_act = false;
+        int min_it = o.max_iters;
+        std::vector<double> inner(Kn, inf);
+        for (int k = 0; k < Kn; ++k) {
+          if (!act[k]) continue;
+          const double thr = tol[k] * tol[k] * bb[k];
+          if (!(rr[k] >= thr) || iters[k] >= o.max_iters) {  // also stops on NaN
+            act[k] = 0;
+            continue;
+          }
+          inner[k] = std::max(o.inner_tol, tol[k] * std::sqrt(bb[k] / rr[k]));
+          min_it = std::min(min_it, iters[k]);
+          any_act = true;
+        }
+        if (!any_act) break;
+        launch_cast_f64_to_f32(s, N, M->r64.p, reinterpret_cast<float*>(M->pb.p));
+        std::vector<int> itk;
+        seg_pcg(inner, std::max(1, o.max_iters - min_it), o.precond, true, itk);
+        launch_axpy_f32_to_f64(s, N, reinterpret_cast<const float*>(M->pv.p), x);
+        for (int k = 0; k < Kn; ++k) {
+          if (!act[k]) continue;
+          iters[k] += itk[k];
+          if (itk[k] == 0) act[k] = 0;  // no progress possible in FP32
+        }
+      }
+    }
+    // indefinite (b.x < 0): try again with a larger shift
+    launch_batch_dot(s, ba, b, x, kvec.p);
+    download_k(kvec.p, h);
+    for (int k = 0; k < Kn; ++k)
+      if (F[k] && !(h[k] < 0.0)) F[k] = 0;
+  }
+}
+
 void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
@@ -230,7 +445,7 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
     d2d(xsave.p, M->x.p);
     eval_eg(E);
     bool need_h = false;
-    double tol_lin = 1.0;
+    std::vector<double> tol_lin(Kn, o.lsTolMaxError);
     for (int k = 0; k < Kn; ++k) {
       hmode[k] = 0;
       if (st[k] != ACTIVE) continue;
@@ -258,18 +473,19 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
           gprev[k] = opt[k];
           lt = std::max(o.lsTolMaxError, eta[k]);
         }
-        tol_lin = std::min(tol_lin, lt);
+        tol_lin[k] = lt;
       }
     }
     mode.upload(hmode, s);
     if (need_h) M->eval_hessian();
     launch_batch_rhs(s, ba, n, mode.p, M->g.p, xsave.p, v0.p, bvec.p, dx.p);
     if (need_h) {
-      lo.tol = tol_lin;
-      pgn_linear_result lr;
-      linear_solve(*M, bvec.p, dx.p, lo, lr);
+      std::vector<uint8_t> act(Kn);
+      for (int k = 0; k < Kn; ++k) act[k] = hmode[k] == 1;
+      std::vector<int> its_lin;
+      linear_solve(bvec.p, dx.p, act, tol_lin, lo, its_lin);
       for (int k = 0; k < Kn; ++k)
-        if (hmode[k] == 1) pcg[k] += lr.iterations;
+        if (hmode[k] == 1) pcg[k] += its_lin[k];
     }
     launch_batch_norm2(s, ba, dx.p, n2.p);
     n2.download(hn2.data(), Kn, s);

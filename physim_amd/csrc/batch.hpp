@@ -40,6 +40,13 @@ struct pgn_batch {
   pgn::DBuf<double> grav, falpha, fac, elemE, nodeE, nodeG2, out, n2;
   pgn::DBuf<uint8_t> mode, mask;
   pgn::DBuf<double> xsave, v0, dx, bvec;
+  // segmented PCG: workgroup table per instance (planned for a lanes-per-row)
+  int seg_lpr = 0, seg_W = 0;
+  pgn::DBuf<int> wg_seg, wg_loc, seg_G, seg_wg0;
+  pgn::DBuf<long> seg_row0, seg_row1;
+  pgn::DBuf<double> seg_tol, seg_part1, seg_part2, seg_partb, kvec, kvec2, xprev;
+  pgn::DBuf<pgn::PcgScalars> seg_sc;
+  std::vector<pgn::PcgScalars> hseg_sc;
 
   int K() const { return (int)inst.size(); }
   bool built() const { return (bool)M; }
@@ -51,5 +58,17 @@ struct pgn_batch {
   // per-instance {E, |g|^2, energy scale, elastic energy} at the current x
   void eval_eg(std::vector<double>& out4);
   void reset();
+  void seg_plan();
+  pgn::SegPcgArgs seg_args(int precond) const;
+  // PCG of every instance at once on the PCG rhs (model precision, M->pb) and
+  // solution (M->pv x); tol[k] = +inf leaves instance k untouched.  Returns
+  // per-instance iteration counts.
+  void seg_pcg(const std::vector<double>& tol, int maxit, int precond, bool x_zero,
+               std::vector<int>& iters);
+  // LinearSolver_EigenCG::SolveInternal per instance on the batch Hessian:
+  // active[k] selects the instances solved; tol per instance
+  void linear_solve(const double* b, double* x, const std::vector<uint8_t>& active,
+                    const std::vector<double>& tol, const pgn_linear_opts& o,
+                    std::vector<int>& iters);
   void newton(const pgn_newton_opts& o, pgn_newton_result* R);
 };

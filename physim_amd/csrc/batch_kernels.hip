@@ -135,3 +135,93 @@ void launch_batch_copy(hipStream_t s, const BatchArgs& a, long n_nodes, const ui
 }
 
 }  // namespace pgn
+
+namespace pgn {
+
+__global__ __launch_bounds__(BR_BLOCK) void batch_dot_kernel(BatchArgs A,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ y,
+                                                             double* __restrict__ out) {
+  __shared__ double red[BR_BLOCK / 64];
+  const int k = blockIdx.x;
+  const long d0 = 3 * A.noff[k], d1 = 3 * A.noff[k + 1];
+  double s = 0.0;
+  for (long d = d0 + threadIdx.x; d < d1; d += BR_BLOCK) s += x[d] * y[d];
+  s = block_sum<BR_BLOCK>(s, red);
+  if (threadIdx.x == 0) out[k] = s;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void batch_add_diag_kernel(long n, const int* __restrict__ inst,
+                                                             const long* __restrict__ diag,
+                                                             T* __restrict__ bval,
+                                                             const double* __restrict__ shift) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i >= n) return;
+  const double sh = shift[inst[i]];
+  if (sh == 0.0) return;
+  T* B = bval + 9 * diag[i];
+  B[0] += (T)sh;
+  B[4] += (T)sh;
+  B[8] += (T)sh;
+}
+
+void launch_batch_dot(hipStream_t s, const BatchArgs& a, const double* x, const double* y,
+                      double* out) {
+  if (a.K == 0) return;
+  hipLaunchKernelGGL(batch_dot_kernel, dim3(a.K), dim3(BR_BLOCK), 0, s, a, x, y, out);
+}
+
+void launch_batch_add_diag(hipStream_t s, bool fp32, const BatchArgs& a, long n_nodes,
+                           const long* diag, void* bval, const double* shift) {
+  if (n_nodes == 0) return;
+  const unsigned g = (unsigned)((n_nodes + 255) / 256);
+  if (fp32)
+    hipLaunchKernelGGL(batch_add_diag_kernel<float>, dim3(g), dim3(256), 0, s, n_nodes,
+                       a.node_inst, diag, reinterpret_cast<float*>(bval), shift);
+  else
+    hipLaunchKernelGGL(batch_add_diag_kernel<double>, dim3(g), dim3(256), 0, s, n_nodes,
+                       a.node_inst, diag, reinterpret_cast<double*>(bval), shift);
+}
+
+}  // namespace pgn
+
+namespace pgn {
+
+// |A_k|_F^2 over the stored lower entry set of each instance's rows (node
+// blocks with row >= col), one workgroup per instance
+template <class T>
+__global__ __launch_bounds__(BR_BLOCK) void batch_frob_kernel(BatchArgs A,
+                                                              const long* __restrict__ rowptr,
+                                                              const int* __restrict__ brow,
+                                                              const int* __restrict__ bcol,
+                                                              const T* __restrict__ bval,
+                                                              double* __restrict__ out) {
+  __shared__ double red[BR_BLOCK / 64];
+  const int k = blockIdx.x;
+  const long s0 = rowptr[A.noff[k]], s1 = rowptr[A.noff[k + 1]];
+  double acc = 0.0;
+  for (long s = s0 + threadIdx.x; s < s1; s += BR_BLOCK)
+    if (brow[s] >= bcol[s]) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        const double v = bval[s * 9 + q];
+        acc += v * v;
+      }
+    }
+  acc = block_sum<BR_BLOCK>(acc, red);
+  if (threadIdx.x == 0) out[k] = acc;
+}
+
+void launch_batch_frob(hipStream_t s, bool fp32, const BatchArgs& a, const long* rowptr,
+                       const int* brow, const int* bcol, const void* bval, double* out) {
+  if (a.K == 0) return;
+  if (fp32)
+    hipLaunchKernelGGL(batch_frob_kernel<float>, dim3(a.K), dim3(BR_BLOCK), 0, s, a, rowptr,
+                       brow, bcol, reinterpret_cast<const float*>(bval), out);
+  else
+    hipLaunchKernelGGL(batch_frob_kernel<double>, dim3(a.K), dim3(BR_BLOCK), 0, s, a, rowptr,
+                       brow, bcol, reinterpret_cast<const double*>(bval), out);
+}
+
+}  // namespace pgn

@@ -160,6 +160,47 @@ void launch_pcg_status(hipStream_t s, const PcgArgs& a, int k_next, int maxit);
 // plain device copy of n floats (bandwidth calibration)
 void launch_copy_f32(hipStream_t s, long n, const float* a, float* b);
 
+// ---- segmented PCG: one independent PCG per instance of a batch --------------
+// Segment k (an instance) owns block rows [row0[k], row1[k]) and workgroups
+// [wg0[k], wg0[k] + G[k]) of every launch (the WG table is wg_seg / wg_loc);
+// each segment has its own tolerance, scalars (alpha, beta from its own
+// partial sums) and convergence state, and stops on its own.
+struct SegPcgArgs {
+  int K;                 // segments
+  int W;                 // workgroups of every launch (sum of G)
+  const int* wg_seg;     // W: segment of each workgroup
+  const int* wg_loc;     // W: index of the workgroup within its segment
+  const int* seg_G;      // K
+  const int* seg_wg0;    // K
+  const long* seg_row0;  // K
+  const long* seg_row1;  // K
+  const double* tol;     // K relative tolerances
+  const long* rowptr;
+  const int* bcol;
+  const void* bval;
+  const void* dinv;
+  const void* binv;
+  int precond;
+  int lpr;
+  void* x;
+  void* r;
+  void* z;
+  void* p0;
+  void* p1;
+  void* q;
+  double* part1;         // W
+  double* part2;         // 2 parities x 2 x W
+  double* partb;         // W
+  PcgScalars* sc;        // K
+};
+constexpr int SEG_MAXG = 256;  // workgroups per segment at most
+void launch_seg_init(hipStream_t s, bool fp32, const SegPcgArgs& a, const void* b, int x_zero);
+void launch_seg_k1(hipStream_t s, bool fp32, const SegPcgArgs& a, int k);
+void launch_seg_k2(hipStream_t s, bool fp32, const SegPcgArgs& a, int k);
+void launch_seg_status(hipStream_t s, const SegPcgArgs& a, int k_next, int maxit);
+// block rows of one K1 workgroup pass for `lpr` lanes per row
+int seg_rows_per_wg(int lpr);
+
 // r64 = b64 - A x64 with A in precision T and FP64 accumulation; writes
 // |r|^2 into out[0].  partials: PCG_MAXP doubles + one unsigned ticket.
 void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
@@ -194,6 +235,15 @@ void launch_batch_step(hipStream_t s, const BatchArgs& a, long n_nodes, const do
 void launch_batch_rhs(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mode,
                       const double* g, const double* xsave, const double* v0, double* b,
                       double* dx);
+// out[k] = a_k . b_k over each instance's DoFs (FP64)
+void launch_batch_dot(hipStream_t s, const BatchArgs& a, const double* x, const double* y,
+                      double* out);
+// bval[diagonal block of node i] += shift[instance of i] I
+void launch_batch_add_diag(hipStream_t s, bool fp32, const BatchArgs& a, long n_nodes,
+                           const long* diag, void* bval, const double* shift);
+// out[k] = |A_k|_F^2 over the stored lower entry set of instance k's rows
+void launch_batch_frob(hipStream_t s, bool fp32, const BatchArgs& a, const long* rowptr,
+                       const int* brow, const int* bcol, const void* bval, double* out);
 // dst = src on instances with mask[k] != 0
 void launch_batch_copy(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mask,
                        const double* src, double* dst);

@@ -558,3 +558,287 @@ void launch_dot64(hipStream_t s, long n, const double* a, const double* b,
 }
 
 }  // namespace pgn
+
+// ---- segmented PCG -------------------------------------------------------------
+// Same iteration as above, one independent PCG per segment (instance).  A
+// workgroup serves one segment: it walks that segment's rows with the
+// segment's own stride (G workgroups), writes its partials at its slot, and
+// re-sums only its own segment's G partials.
+namespace pgn {
+
+constexpr int SEG_K2_BS = 256;
+
+int seg_rows_per_wg(int lpr) { return K1_BS / lpr; }
+
+__device__ __forceinline__ const double* seg_rr_slot(const SegPcgArgs& A, int k) {
+  return A.part2 + (long)(k & 1) * 2 * A.W;
+}
+
+template <class T, int L>
+__global__ __launch_bounds__(K1_BS) void seg_init_kernel(SegPcgArgs A, const T* __restrict__ b,
+                                                         int x_zero) {
+  __shared__ double red[3 * K1_BS / 64];
+  const int w = blockIdx.x, sg = A.wg_seg[w], j = A.wg_loc[w], G = A.seg_G[sg];
+  const long r0 = A.seg_row0[sg], r1 = A.seg_row1[sg];
+  const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
+  T* __restrict__ x = reinterpret_cast<T*>(A.x);
+  T* __restrict__ r = reinterpret_cast<T*>(A.r);
+  T* __restrict__ z = reinterpret_cast<T*>(A.z);
+  T* __restrict__ p0 = reinterpret_cast<T*>(A.p0);
+  T* __restrict__ p1 = reinterpret_cast<T*>(A.p1);
+  const int lane = threadIdx.x & (L - 1);
+  constexpr int RPW = K1_BS / L;
+  double bb = 0.0, rr = 0.0, rz = 0.0;
+  for (long row = r0 + (long)j * RPW + threadIdx.x / L; row < r1; row += (long)G * RPW) {
+    T y0 = 0, y1 = 0, y2 = 0;
+    if (!x_zero) {
+      const long e1 = A.rowptr[row + 1];
+      for (long s = A.rowptr[row] + lane; s < e1; s += L) {
+        const long c = A.bcol[s];
+        const T* B = bval + s * 9;
+        const T xj0 = x[3 * c], xj1 = x[3 * c + 1], xj2 = x[3 * c + 2];
+        y0 += B[0] * xj0 + B[1] * xj1 + B[2] * xj2;
+        y1 += B[3] * xj0 + B[4] * xj1 + B[5] * xj2;
+        y2 += B[6] * xj0 + B[7] * xj1 + B[8] * xj2;
+      }
+      y0 = group_sum<L>(y0);
+      y1 = group_sum<L>(y1);
+      y2 = group_sum<L>(y2);
+    }
+    if (lane == 0) {
+      const long i = row;
+      T ri[3] = {b[3 * i] - y0, b[3 * i + 1] - y1, b[3 * i + 2] - y2};
+      T zi[3];
+      apply_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
+                       reinterpret_cast<const T*>(A.binv), i, ri, zi);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        r[3 * i + p] = ri[p];
+        z[3 * i + p] = zi[p];
+        p0[3 * i + p] = T(0);
+        p1[3 * i + p] = T(0);
+        if (x_zero) x[3 * i + p] = T(0);
+        bb += (double)b[3 * i + p] * (double)b[3 * i + p];
+        rr += (double)ri[p] * (double)ri[p];
+        rz += (double)ri[p] * (double)zi[p];
+      }
+    }
+  }
+  const double sbb = block_sum<K1_BS>(bb, red);
+  block_sum2<K1_BS>(rr, rz, red);
+  if (threadIdx.x == 0) {
+    A.partb[w] = sbb;
+    double* o = A.part2 + 2L * A.W;  // parity slot of iteration -1
+    o[2 * w] = rr;
+    o[2 * w + 1] = rz;
+    if (j == 0) {
+      A.sc[sg].done = 0;
+      A.sc[sg].iters = 0;
+    }
+  }
+}
+
+template <class T, int L>
+__global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
+  __shared__ double red[2 * K1_BS / 64 + 2];
+  const int w = blockIdx.x, sg = A.wg_seg[w], j = A.wg_loc[w], G = A.seg_G[sg];
+  PcgScalars* sc = A.sc + sg;
+  if (sc->done) return;
+  const int g0 = A.seg_wg0[sg];
+  const double* prev = seg_rr_slot(A, k - 1) + 2L * g0;
+  const double rr = all_sum<K1_BS>(prev, G, 2, red);
+  const double rz = all_sum<K1_BS>(prev + 1, G, 2, red);
+  double thr, beta;
+  if (k == 0) {
+    const double bb = all_sum<K1_BS>(A.partb + g0, G, 1, red);
+    thr = A.tol[sg] * A.tol[sg] * bb;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+    if (j == 0 && threadIdx.x == 0) {
+      sc->thr = thr;
+      sc->bb = bb;
+      sc->rr = rr;
+    }
+    if (bb == 0.0 || rr < thr) {
+      if (j == 0 && threadIdx.x == 0) {
+        sc->done = 1;
+        sc->iters = 0;
+      }
+      return;
+    }
+    beta = 0.0;
+  } else {
+    thr = sc->thr;
+    if (rr < thr) {
+      if (j == 0 && threadIdx.x == 0) {
+        sc->done = 1;
+        sc->iters = k - 1;
+        sc->rr = rr;
+      }
+      return;
+    }
+    const double rzm = all_sum<K1_BS>(seg_rr_slot(A, k - 2) + 2L * g0 + 1, G, 2, red);
+    beta = rz / rzm;
+  }
+  const T tb = (T)beta;
+  const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
+  const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
+  T* __restrict__ pnew = reinterpret_cast<T*>((k & 1) ? A.p1 : A.p0);
+  const T* __restrict__ pold = reinterpret_cast<const T*>((k & 1) ? A.p0 : A.p1);
+  T* __restrict__ q = reinterpret_cast<T*>(A.q);
+  const int lane = threadIdx.x & (L - 1);
+  constexpr int RPW = K1_BS / L;
+  const long r0 = A.seg_row0[sg], r1 = A.seg_row1[sg];
+  double pq = 0.0;
+  for (long row = r0 + (long)j * RPW + threadIdx.x / L; row < r1; row += (long)G * RPW) {
+    T y0 = 0, y1 = 0, y2 = 0;
+    const long e1 = A.rowptr[row + 1];
+    for (long s = A.rowptr[row] + lane; s < e1; s += L) {
+      const long c = A.bcol[s];
+      const T* B = bval + s * 9;
+      const T pj0 = z[3 * c] + tb * pold[3 * c];
+      const T pj1 = z[3 * c + 1] + tb * pold[3 * c + 1];
+      const T pj2 = z[3 * c + 2] + tb * pold[3 * c + 2];
+      y0 += B[0] * pj0 + B[1] * pj1 + B[2] * pj2;
+      y1 += B[3] * pj0 + B[4] * pj1 + B[5] * pj2;
+      y2 += B[6] * pj0 + B[7] * pj1 + B[8] * pj2;
+    }
+    y0 = group_sum<L>(y0);
+    y1 = group_sum<L>(y1);
+    y2 = group_sum<L>(y2);
+    if (lane == 0) {
+      const long i = row;
+      const T pi0 = z[3 * i] + tb * pold[3 * i];
+      const T pi1 = z[3 * i + 1] + tb * pold[3 * i + 1];
+      const T pi2 = z[3 * i + 2] + tb * pold[3 * i + 2];
+      pnew[3 * i] = pi0;
+      pnew[3 * i + 1] = pi1;
+      pnew[3 * i + 2] = pi2;
+      q[3 * i] = y0;
+      q[3 * i + 1] = y1;
+      q[3 * i + 2] = y2;
+      pq += (double)pi0 * (double)y0 + (double)pi1 * (double)y1 + (double)pi2 * (double)y2;
+    }
+  }
+  const double s = block_sum<K1_BS>(pq, red);
+  if (threadIdx.x == 0) A.part1[w] = s;
+}
+
+template <class T>
+__global__ __launch_bounds__(SEG_K2_BS) void seg_k2_kernel(SegPcgArgs A, int k) {
+  __shared__ double red[2 * SEG_K2_BS / 64 + 2];
+  const int w = blockIdx.x, sg = A.wg_seg[w], j = A.wg_loc[w], G = A.seg_G[sg];
+  if (A.sc[sg].done) return;
+  const int g0 = A.seg_wg0[sg];
+  const double pq = all_sum<SEG_K2_BS>(A.part1 + g0, G, 1, red);
+  const double rz = all_sum<SEG_K2_BS>(seg_rr_slot(A, k - 1) + 2L * g0 + 1, G, 2, red);
+  const T ta = (T)(rz / pq);
+  const T* __restrict__ pk = reinterpret_cast<const T*>((k & 1) ? A.p1 : A.p0);
+  const T* __restrict__ q = reinterpret_cast<const T*>(A.q);
+  T* __restrict__ x = reinterpret_cast<T*>(A.x);
+  T* __restrict__ r = reinterpret_cast<T*>(A.r);
+  T* __restrict__ z = reinterpret_cast<T*>(A.z);
+  const long r0 = A.seg_row0[sg], r1 = A.seg_row1[sg];
+  double rr = 0.0, rzn = 0.0;
+  for (long i = r0 + (long)j * SEG_K2_BS + threadIdx.x; i < r1; i += (long)G * SEG_K2_BS) {
+    T ri[3], zi[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      x[3 * i + p] += ta * pk[3 * i + p];
+      ri[p] = r[3 * i + p] - ta * q[3 * i + p];
+      r[3 * i + p] = ri[p];
+    }
+    apply_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
+                     reinterpret_cast<const T*>(A.binv), i, ri, zi);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      z[3 * i + p] = zi[p];
+      rr += (double)ri[p] * (double)ri[p];
+      rzn += (double)ri[p] * (double)zi[p];
+    }
+  }
+  block_sum2<SEG_K2_BS>(rr, rzn, red);
+  if (threadIdx.x == 0) {
+    double* o = A.part2 + (long)(k & 1) * 2 * A.W;
+    o[2 * w] = rr;
+    o[2 * w + 1] = rzn;
+  }
+}
+
+// one workgroup per segment
+__global__ __launch_bounds__(256) void seg_status_kernel(SegPcgArgs A, int k_next, int maxit) {
+  __shared__ double red[2 * 256 / 64 + 2];
+  const int sg = blockIdx.x;
+  PcgScalars* sc = A.sc + sg;
+  if (sc->done) return;
+  const int k = k_next, g0 = A.seg_wg0[sg], G = A.seg_G[sg];
+  const double rr = all_sum<256>(seg_rr_slot(A, k - 1) + 2L * g0, G, 2, red);
+  double thr;
+  if (k == 0) {
+    const double bb = all_sum<256>(A.partb + g0, G, 1, red);
+    thr = A.tol[sg] * A.tol[sg] * bb;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+    if (threadIdx.x == 0) {
+      sc->bb = bb;
+      sc->thr = thr;
+    }
+  } else {
+    thr = sc->thr;
+  }
+  if (threadIdx.x == 0) {
+    sc->rr = rr;
+    if (rr < thr) {
+      sc->done = 1;
+      sc->iters = k > 0 ? k - 1 : 0;
+    } else if (k >= maxit) {
+      sc->done = 1;
+      sc->iters = maxit;
+    }
+  }
+}
+
+template <int L>
+static void seg_init_l(hipStream_t s, bool fp32, const SegPcgArgs& a, const void* b, int xz) {
+  if (fp32)
+    hipLaunchKernelGGL((seg_init_kernel<float, L>), dim3(a.W), dim3(K1_BS), 0, s, a,
+                       reinterpret_cast<const float*>(b), xz);
+  else
+    hipLaunchKernelGGL((seg_init_kernel<double, L>), dim3(a.W), dim3(K1_BS), 0, s, a,
+                       reinterpret_cast<const double*>(b), xz);
+}
+
+void launch_seg_init(hipStream_t s, bool fp32, const SegPcgArgs& a, const void* b, int x_zero) {
+  switch (a.lpr) {
+    case 4: seg_init_l<4>(s, fp32, a, b, x_zero); break;
+    case 16: seg_init_l<16>(s, fp32, a, b, x_zero); break;
+    default: seg_init_l<8>(s, fp32, a, b, x_zero); break;
+  }
+}
+
+template <int L>
+static void seg_k1_l(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {
+  if (fp32)
+    hipLaunchKernelGGL((seg_k1_kernel<float, L>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+  else
+    hipLaunchKernelGGL((seg_k1_kernel<double, L>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+}
+
+void launch_seg_k1(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {
+  switch (a.lpr) {
+    case 4: seg_k1_l<4>(s, fp32, a, k); break;
+    case 16: seg_k1_l<16>(s, fp32, a, k); break;
+    default: seg_k1_l<8>(s, fp32, a, k); break;
+  }
+}
+
+void launch_seg_k2(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {
+  if (fp32)
+    hipLaunchKernelGGL(seg_k2_kernel<float>, dim3(a.W), dim3(SEG_K2_BS), 0, s, a, k);
+  else
+    hipLaunchKernelGGL(seg_k2_kernel<double>, dim3(a.W), dim3(SEG_K2_BS), 0, s, a, k);
+}
+
+void launch_seg_status(hipStream_t s, const SegPcgArgs& a, int k_next, int maxit) {
+  hipLaunchKernelGGL(seg_status_kernel, dim3(a.K), dim3(256), 0, s, a, k_next, maxit);
+}
+
+}  // namespace pgn

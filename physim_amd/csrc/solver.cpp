@@ -15,8 +15,8 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
   hipStream_t s = M.stream();
   const PcgArgs a = M.pcg_args(tol, precond);
   launch_pcg_init(s, M.hfp32, a, M.pb.p, x_zero ? 1 : 0);
-  if (M.profiling && (int)M.prof_ev.size() < 3 * M.pcg_chunk_max) {
-    while ((int)M.prof_ev.size() < 3 * M.pcg_chunk_max) {
+  if (M.profiling && M.prof_ev.size() < 3) {
+    while (M.prof_ev.size() < 3) {
       hipEvent_t e;
       PGN_HIP(hipEventCreate(&e));
       M.prof_ev.push_back(e);
@@ -25,29 +25,31 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
   int k = 0, chunk = 8;
   for (;;) {
     const int c = std::min(chunk, maxit - k);
+    const int js = c / 2;  // sampled iteration of the chunk (mid-stream)
     for (int j = 0; j < c; ++j) {
-      if (M.profiling) PGN_HIP(hipEventRecord(M.prof_ev[3 * j], s));
+      // profiling brackets one K1 and K2 of every chunk with events: a
+      // sample of the launches, so the timed solve itself stays unperturbed
+      const bool ev = M.profiling && j == js;
+      if (ev) PGN_HIP(hipEventRecord(M.prof_ev[0], s));
       launch_pcg_k1(s, M.hfp32, a, k + j);
-      if (M.profiling) PGN_HIP(hipEventRecord(M.prof_ev[3 * j + 1], s));
+      if (ev) PGN_HIP(hipEventRecord(M.prof_ev[1], s));
       launch_pcg_k2(s, M.hfp32, a, k + j);
-      if (M.profiling) PGN_HIP(hipEventRecord(M.prof_ev[3 * j + 2], s));
+      if (ev) PGN_HIP(hipEventRecord(M.prof_ev[2], s));
     }
     k += c;
     launch_pcg_status(s, a, k, maxit);
     PGN_HIP(hipMemcpyAsync(M.hsc, M.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     M.sync();
-    if (M.profiling) {
-      // launches after convergence return at once: time only the live ones
-      const int live = M.hsc->done ? std::max(0, std::min(c, M.hsc->iters + 1 - (k - c))) : c;
-      for (int j = 0; j < live; ++j) {
-        float t1 = 0.f, t2 = 0.f;
-        PGN_HIP(hipEventElapsedTime(&t1, M.prof_ev[3 * j], M.prof_ev[3 * j + 1]));
-        PGN_HIP(hipEventElapsedTime(&t2, M.prof_ev[3 * j + 1], M.prof_ev[3 * j + 2]));
-        M.prof_ms[0] += t1;
-        M.prof_ms[1] += t2;
-      }
-      M.prof_n[0] += live;
-      M.prof_n[1] += live;
+    // launches after convergence return at once: the sample counts only if
+    // its iteration still did work
+    if (M.profiling && (!M.hsc->done || M.hsc->iters + 1 > k - c + js)) {
+      float t1 = 0.f, t2 = 0.f;
+      PGN_HIP(hipEventElapsedTime(&t1, M.prof_ev[0], M.prof_ev[1]));
+      PGN_HIP(hipEventElapsedTime(&t2, M.prof_ev[1], M.prof_ev[2]));
+      M.prof_ms[0] += t1;
+      M.prof_ms[1] += t2;
+      M.prof_n[0] += 1;
+      M.prof_n[1] += 1;
     }
     if (M.hsc->done || k >= maxit) break;
     chunk = std::min(2 * chunk, M.pcg_chunk_max);

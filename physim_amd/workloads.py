@@ -1,0 +1,203 @@
+"""Benchmark workloads beside the 1.28M-tet beam: the 256-instance fused
+beam batch, the 4096-rod batch and the serving stream.  Each returns the
+fields of one bench.py JSON line (bench.py adds the common ones).
+
+Workloads (all synthetic, seeded):
+  batch   -- 256 beams of scenes.batch_beams (hex dims U{4..24} x U{4..24} x
+             U{16..96} at 2.5 mm cells, CoNH/StVK 50/50, E lognormal
+             (ln 1e5, 0.5), nu U(0.25, 0.45), random clamp face, gravity +
+             U(0, 50) Pa traction), one fused pgn_batch solve from rest.
+  rods    -- 4096 helices (n U{50..500}, radius U(0.01, 0.05) m, pitch
+             U(0.005, 0.05) m, E 1e7, G = E/3, r 1e-3 m), first two vertices
+             and first edge angle clamped, gravity over 10 stages.
+  serving -- Poisson arrivals of a seeded 60/40 mix of beams (the 1.28M-tet
+             beam's generator at 2k..max tets) and cloths (n U{32..max}),
+             through pgn_server; latency = submit -> result on host.
+"""
+import threading
+import time
+
+import numpy as np
+
+import physim_amd as pg
+from physim_amd import scenes
+
+SOLVE_OPTS = dict(pg.CONVERGE_OPTS, precond=1, adaptive_tol=1)
+
+
+def _stats(vals):
+    v = np.asarray(vals, dtype=np.float64)
+    return {"min": float(v.min()), "p50": float(np.percentile(v, 50)),
+            "max": float(v.max())}
+
+
+def batch(args, log):
+    count = args.count or 256
+    t0 = time.perf_counter()
+    scs = scenes.batch_beams(count, seed=3)
+    ctx = pg.Context(0)
+    B = pg.Batch(ctx, pg.MIXED)
+    for s in scs:
+        B.add_scene(s)
+    B.build()
+    m = sum(s.m for s in scs)
+    n = sum(s.n for s in scs)
+    log(f"batch setup {time.perf_counter() - t0:.1f}s: {count} beams, {m} tets, {3 * n} DoF, "
+        f"{B.device_bytes() / 2**30:.2f} GiB on device")
+    opts = dict(SOLVE_OPTS, numMaxIters=200)
+    warm = min(args.warmup, 1)
+    for _ in range(warm):
+        B.reset()
+        B.newton(**opts)
+    steps = max(1, min(args.steps, 2))
+    times, res = [], None
+    for k in range(steps):
+        B.reset()
+        ctx.synchronize()
+        ts = time.perf_counter()
+        res = B.newton(**opts)
+        times.append(time.perf_counter() - ts)
+        ok = sum(r["result"] == "SUCCESS" for r in res)
+        log(f"batch step {k}: {times[-1]:.2f}s, {ok}/{count} converged")
+    xs = [B.get_x(k).copy() for k in (0, count // 2, count - 1)]
+    # the same instances solved alone (each its own model) must agree
+    solo, errs = [], []
+    for j, k in enumerate((0, count // 2, count - 1)):
+        s = scs[k]
+        G = pg.TetModel(ctx, s.X0, s.T, s.material, s.lame1, s.lame2, s.density, pg.MIXED)
+        s.apply(G)
+        ts = time.perf_counter()
+        r = G.newton(**opts)
+        solo.append(time.perf_counter() - ts)
+        errs.append(float(np.abs(G.get_x().reshape(-1, 3) - xs[j]).max() / s.bbox_diag()))
+        G.close()
+    el = float(np.mean(times))
+    B.close()
+    ctx.close()
+    return {
+        "value": count / el, "unit": "Newton solves/s", "ms_per_step": el * 1e3,
+        "steps": steps, "warmup": warm, "scaling": "weak",
+        "dtype": "f32 Hessian/PCG, f64 energy/gradient/residual",
+        "config": {"workload": f"batch_{count}_beams: {m} tets, {3 * n} DoF, one fused "
+                               "launch per kernel, per-instance Newton",
+                   "precision": "mixed", "precond": "block-jacobi",
+                   "linear_tol": "eisenstat-walker"},
+        "converged": sum(r["result"] == "SUCCESS" for r in res),
+        "instances": count,
+        "newton_iterations": _stats([r["iterations"] for r in res]),
+        "pcg_iterations_max": max(r["pcg_iterations"] for r in res),
+        "solo_check": {"instances": [0, count // 2, count - 1],
+                       "max_disp_diff_over_bbox_diag": max(errs),
+                       "matches_within_1e-6": bool(max(errs) <= 1e-6),
+                       "solo_solve_s": solo},
+    }
+
+
+def rods(args, log):
+    count = args.count or 4096
+    t0 = time.perf_counter()
+    sc = scenes.rod_batch(count, seed=4, density=args.rod_density)
+    ctx = pg.Context(0)
+    R = pg.Rods(ctx, sc.rods, sc.frame0_normals(), sc.young, sc.shear, sc.radius, sc.radius,
+                sc.density)
+    R.set_fixed(sc.fixed_mask())
+    R.set_gravity([0.0, -9.81, 0.0], sc.stages)
+    nv = int(sc.nverts.sum())
+    log(f"rods setup {time.perf_counter() - t0:.1f}s: {count} rods, {nv} vertices, "
+        f"{R.N} DoF")
+    opts = dict(tolMaxError=1e-8, numMaxIters=100, tolMinImpr=0.0, tolMinStep=0.0,
+                ls_noise_guard=1e-12)
+    warm = min(args.warmup, 1)
+    for _ in range(warm):
+        R.reset()
+        R.newton(**opts)
+    steps = max(1, min(args.steps, 3))
+    times, res = [], None
+    for k in range(steps):
+        R.reset()
+        ctx.synchronize()
+        ts = time.perf_counter()
+        res = R.newton(**opts)
+        times.append(time.perf_counter() - ts)
+        ok = sum(r["result"] == "SUCCESS" for r in res)
+        log(f"rods step {k}: {times[-1]:.2f}s, {ok}/{count} converged")
+    el = float(np.mean(times))
+    R.close()
+    ctx.close()
+    return {
+        "value": count / el, "unit": "Newton solves/s", "ms_per_step": el * 1e3,
+        "steps": steps, "warmup": warm, "scaling": "weak",
+        "dtype": "f64 elements and banded LDL^T",
+        "config": {"workload": f"rods_{count}_helices: {nv} vertices, line density "
+                               f"{args.rod_density:g} kg/m, 10 load stages",
+                   "linear_solver": "banded LDL^T per rod"},
+        "converged": sum(r["result"] == "SUCCESS" for r in res),
+        "instances": count,
+        "newton_iterations": _stats([r["iterations"] for r in res]),
+    }
+
+
+def _serving_mix(count, max_tets, max_cloth, seed=5):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        if rng.random() < 0.6:
+            m = float(np.exp(rng.uniform(np.log(2000.0), np.log(max_tets))))
+            d = max(2, int(round((m / 20.0) ** (1.0 / 3.0))))
+            out.append(scenes.cantilever(dims=(d, d, 4 * d), young_sigma=0.3, jitter=0.05,
+                                         seed=int(rng.integers(1 << 30)),
+                                         name=f"beam_{d}x{d}x{4 * d}"))
+        else:
+            n = int(rng.integers(32, max_cloth + 1))
+            out.append(scenes.cloth(n, stages=10, name=f"cloth_{n}"))
+    return out
+
+
+def serving(args, log):
+    count = args.count or 32
+    reqs = _serving_mix(count, args.max_tets, args.max_cloth)
+    log(f"serving: {count} requests at {args.rate:g}/s, "
+        f"{sum(hasattr(r, 'hinges') for r in reqs)} cloths")
+    S = pg.Server(hbm_budget_bytes=200 << 30, lanes=4, max_batch=16, precision=pg.MIXED,
+                  **dict(SOLVE_OPTS, numMaxIters=300))
+    rng = np.random.default_rng(6)
+    gaps = rng.exponential(1.0 / args.rate, size=count)
+    tickets = [None] * count
+    t_sub = [0.0] * count
+
+    def producer():
+        for i, s in enumerate(reqs):
+            time.sleep(float(gaps[i]))
+            t_sub[i] = time.perf_counter()
+            tickets[i] = S.submit(s)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    lat, queue, done_t, results = [], [], [], []
+    for i in range(count):
+        x, r, latency, q = S.wait(tickets[i])
+        lat.append(latency)
+        queue.append(q)
+        done_t.append(t_sub[i] + latency)
+        results.append(r)
+    span = max(done_t) - t0
+    st = S.stats()
+    S.close()
+    ok = sum(r["result"] == "SUCCESS" for r in results)
+    log(f"serving: {ok}/{count} converged in {span:.2f}s, batches {st['batches']}, "
+        f"p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
+    return {
+        "value": count / span, "unit": "Newton solves/s", "ms_per_step": span / count * 1e3,
+        "steps": count, "warmup": 0, "scaling": "weak",
+        "dtype": "f32 Hessian/PCG, f64 energy/gradient/residual",
+        "config": {"workload": f"serving_{count}_requests: Poisson {args.rate:g}/s, 60% beams "
+                               f"2k-{args.max_tets} tets, 40% cloths 32-{args.max_cloth}",
+                   "lanes": 4, "max_batch": 16},
+        "converged": ok,
+        "latency_ms_p50": float(np.percentile(lat, 50)) * 1e3,
+        "latency_ms_p99": float(np.percentile(lat, 99)) * 1e3,
+        "queue_ms_p50": float(np.percentile(queue, 50)) * 1e3,
+        "batches": st["batches"],
+    }
