@@ -190,9 +190,13 @@ void pgn_batch::reset() {
 
 // ---- segmented PCG and linear solve -------------------------------------------
 
-void pgn_batch::seg_plan() {
+// Workgroup table of the segmented PCG over the instances in `act` (the
+// others get no workgroups: launches then cost only what active instances
+// need, however many have already converged).
+void pgn_batch::seg_plan(const std::vector<uint8_t>& act) {
   const int Kn = K();
   const int lpr = M->pcg_lpr;
+  if (lpr == seg_lpr && act == seg_act) return;
   const int rpw = seg_rows_per_wg(lpr);
   std::vector<int> G(Kn), wg0(Kn), ws, wl;
   std::vector<long> r0(Kn), r1(Kn);
@@ -201,6 +205,7 @@ void pgn_batch::seg_plan() {
     const long rows = hnoff[k + 1] - hnoff[k];
     long g = (rows + rpw - 1) / rpw;
     g = std::max(1L, std::min<long>(g, SEG_MAXG));
+    if (!act[k]) g = 0;
     G[k] = (int)g;
     wg0[k] = W;
     r0[k] = hnoff[k];
@@ -218,14 +223,19 @@ void pgn_batch::seg_plan() {
   seg_wg0.upload(wg0, s);
   seg_row0.upload(r0, s);
   seg_row1.upload(r1, s);
-  seg_part1.alloc(W);
-  seg_part2.alloc(4L * W);
-  seg_partb.alloc(W);
-  seg_sc.alloc(Kn);
-  seg_tol.alloc(Kn);
+  if (seg_part1.n < (size_t)std::max(W, 1)) {
+    seg_part1.alloc(std::max(W, 1));
+    seg_part2.alloc(4L * std::max(W, 1));
+    seg_partb.alloc(std::max(W, 1));
+  }
+  if (seg_sc.n < (size_t)Kn) {
+    seg_sc.alloc(Kn);
+    seg_tol.alloc(Kn);
+  }
   hseg_sc.resize(Kn);
   seg_W = W;
   seg_lpr = lpr;
+  seg_act = act;
   PGN_HIP(hipStreamSynchronize(s));
 }
 
@@ -269,6 +279,10 @@ void pgn_batch::seg_pcg(const std::vector<double>& tol, int maxit, int precond, 
   const bool f32 = M->hfp32;
   seg_tol.upload(tol, s);
   const SegPcgArgs a = seg_args(precond);
+  if (a.W == 0) {
+    iters.assign(Kn, 0);
+    return;
+  }
   launch_seg_init(s, f32, a, M->pb.p, x_zero ? 1 : 0);
   int k = 0, chunk = 8;
   for (;;) {
@@ -282,12 +296,12 @@ void pgn_batch::seg_pcg(const std::vector<double>& tol, int maxit, int precond, 
     seg_sc.download(hseg_sc.data(), Kn, s);
     PGN_HIP(hipStreamSynchronize(s));
     bool all = true;
-    for (int q = 0; q < Kn; ++q) all &= (hseg_sc[q].done != 0);
+    for (int q = 0; q < Kn; ++q) all &= (!seg_act[q] || hseg_sc[q].done != 0);
     if (all || k >= maxit) break;
     chunk = std::min(2 * chunk, 64);
   }
   iters.resize(Kn);
-  for (int q = 0; q < Kn; ++q) iters[q] = hseg_sc[q].iters;
+  for (int q = 0; q < Kn; ++q) iters[q] = seg_act[q] ? hseg_sc[q].iters : 0;
 }
 
 void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8_t>& active,
@@ -299,7 +313,6 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
   const BatchArgs ba = args();
   const double inf = std::numeric_limits<double>::infinity();
   M->tune();
-  if (seg_lpr != M->pcg_lpr) seg_plan();
   if (xprev.n < (size_t)N) xprev.alloc(N);
   if (kvec.n < (size_t)Kn) kvec.alloc(Kn);
   if (kvec2.n < (size_t)Kn) kvec2.alloc(Kn);
@@ -332,6 +345,7 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
     }
     mask.upload(F, s);
     launch_batch_copy(s, ba, n, mask.p, xprev.p, x);
+    seg_plan(F);
     // ---- one unregularized solve of the instances in F ----
     if (!M->hfp32) {
       std::vector<double> tv(Kn);
@@ -385,8 +399,10 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
         if (!any_act) break;
         launch_cast_f64_to_f32(s, N, M->r64.p, reinterpret_cast<float*>(M->pb.p));
         std::vector<int> itk;
+        seg_plan(act);
         seg_pcg(inner, std::max(1, o.max_iters - min_it), o.precond, true, itk);
-        launch_axpy_f32_to_f64(s, N, reinterpret_cast<const float*>(M->pv.p), x);
+        mask.upload(act, s);
+        launch_batch_axpy_f32(s, ba, n, mask.p, reinterpret_cast<const float*>(M->pv.p), x);
         for (int k = 0; k < Kn; ++k) {
           if (!act[k]) continue;
           iters[k] += itk[k];

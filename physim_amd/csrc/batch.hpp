@@ -42,6 +42,7 @@ struct pgn_batch {
   pgn::DBuf<double> xsave, v0, dx, bvec;
   // segmented PCG: workgroup table per instance (planned for a lanes-per-row)
   int seg_lpr = 0, seg_W = 0;
+  std::vector<uint8_t> seg_act;  // instances the current table serves
   pgn::DBuf<int> wg_seg, wg_loc, seg_G, seg_wg0;
   pgn::DBuf<long> seg_row0, seg_row1;
   pgn::DBuf<double> seg_tol, seg_part1, seg_part2, seg_partb, kvec, kvec2, xprev;
@@ -58,7 +59,7 @@ struct pgn_batch {
   // per-instance {E, |g|^2, energy scale, elastic energy} at the current x
   void eval_eg(std::vector<double>& out4);
   void reset();
-  void seg_plan();
+  void seg_plan(const std::vector<uint8_t>& act);
   pgn::SegPcgArgs seg_args(int precond) const;
   // PCG of every instance at once on the PCG rhs (model precision, M->pb) and
   // solution (M->pv x); tol[k] = +inf leaves instance k untouched.  Returns

@@ -225,3 +225,24 @@ void launch_batch_frob(hipStream_t s, bool fp32, const BatchArgs& a, const long*
 }
 
 }  // namespace pgn
+
+namespace pgn {
+
+__global__ __launch_bounds__(256) void batch_axpy_f32_kernel(long nd, const int* __restrict__ inst,
+                                                             const uint8_t* __restrict__ mask,
+                                                             const float* __restrict__ d,
+                                                             double* __restrict__ x) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i >= nd) return;
+  if (mask[inst[i / 3]]) x[i] += (double)d[i];
+}
+
+void launch_batch_axpy_f32(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mask,
+                           const float* d, double* x) {
+  const long nd = 3 * n_nodes;
+  if (nd == 0) return;
+  hipLaunchKernelGGL(batch_axpy_f32_kernel, dim3(grid_of(nd, 256)), dim3(256), 0, s, nd,
+                     a.node_inst, mask, d, x);
+}
+
+}  // namespace pgn

@@ -244,6 +244,9 @@ void launch_batch_add_diag(hipStream_t s, bool fp32, const BatchArgs& a, long n_
 // out[k] = |A_k|_F^2 over the stored lower entry set of instance k's rows
 void launch_batch_frob(hipStream_t s, bool fp32, const BatchArgs& a, const long* rowptr,
                        const int* brow, const int* bcol, const void* bval, double* out);
+// x += d (FP32 correction) on instances with mask[k] != 0
+void launch_batch_axpy_f32(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mask,
+                           const float* d, double* x);
 // dst = src on instances with mask[k] != 0
 void launch_batch_copy(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mask,
                        const double* src, double* dst);

@@ -308,26 +308,33 @@ class ShellScene:
 
 
 def cloth(n, size=1.0, thickness=1e-3, young=1e6, poisson=0.3, density=0.2, stages=10,
-          jitter=0.0, seed=0, name=None):
+          jitter=0.0, seed=0, name=None, vertical=False):
     """n x n node sheet of side `size` in the horizontal xz-plane, pinned at
-    the two corners of its x = 0 edge, under gravity -9.81 y.  Membrane:
+    the two corners of its x = 0 edge, under gravity -9.81 y; or, vertical,
+    hanging in the xy-plane below its top edge (y = 0) from that edge's two
+    corners.  Membrane:
     2D StVK with the Lame parameters of (young, poisson); bending stiffness
     kB = young t^3 / (12 (1 - poisson^2)) per hinge (passed explicitly, as
     Param_BendingK); density in kg/m^2 (the reference lumps density x area,
-    with no thickness factor).  jitter > 0 moves nodes by U(-j, j) cells in
-    y (a non-flat rest shape)."""
+    with no thickness factor).  jitter > 0 moves nodes by U(-j, j) cells
+    along the sheet's normal (a non-flat rest shape)."""
     h = size / (n - 1)
     i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
-    X0 = np.stack([i.ravel() * h, np.zeros(n * n), j.ravel() * h], axis=1)
+    if vertical:
+        X0 = np.stack([i.ravel() * h, -j.ravel() * h, np.zeros(n * n)], axis=1)
+        normal = 2
+    else:
+        X0 = np.stack([i.ravel() * h, np.zeros(n * n), j.ravel() * h], axis=1)
+        normal = 1
     if jitter > 0:
         rng = np.random.default_rng(seed)
-        X0[:, 1] += rng.uniform(-jitter, jitter, n * n) * h
+        X0[:, normal] += rng.uniform(-jitter, jitter, n * n) * h
     F = grid_tri3(n, n)
     Hn = shell_hinges(F)
     l1, l2 = lame(young, poisson)
     kB = young * thickness ** 3 / (12.0 * (1.0 - poisson ** 2))
     fixed = np.zeros((n * n, 3), dtype=np.uint8)
-    fixed[0] = 1          # node (0, 0)
-    fixed[n - 1] = 1      # node (0, n-1)
+    fixed[0] = 1                                 # node (0, 0)
+    fixed[(n - 1) * n if vertical else n - 1] = 1  # the other end of the pinned edge
     return ShellScene(name or f"cloth_{n}x{n}", X0, F, Hn, float(l1), float(l2), thickness,
                       density, kB, fixed, G_EARTH, stages)

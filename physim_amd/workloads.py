@@ -9,9 +9,13 @@ Workloads (all synthetic, seeded):
              U(0, 50) Pa traction), one fused pgn_batch solve from rest.
   rods    -- 4096 helices (n U{50..500}, radius U(0.01, 0.05) m, pitch
              U(0.005, 0.05) m, E 1e7, G = E/3, r 1e-3 m), first two vertices
-             and first edge angle clamped, gravity over 10 stages.
+             and first edge angle clamped, gravity over 10 stages, line density
+             3.14e-6 kg/m (the configuration leaves it open; heavier rods
+             leave more of these very flexible helices unconverged within the
+             300-iteration cap of the reference's Newton).
   serving -- Poisson arrivals of a seeded 60/40 mix of beams (the 1.28M-tet
-             beam's generator at 2k..max tets) and cloths (n U{32..max}),
+             beam's generator at 2k..max tets) and cloths (n U{32..max}, 1 m
+             sheets hanging from the two corners of their top edge, 0.2 kg/m^2),
              through pgn_server; latency = submit -> result on host.
 """
 import threading
@@ -84,6 +88,9 @@ def batch(args, log):
                    "linear_tol": "eisenstat-walker"},
         "converged": sum(r["result"] == "SUCCESS" for r in res),
         "instances": count,
+        "not_converged": {scs[k].name: {"result": r["result"], "iterations": r["iterations"],
+                                        "optimality": r["optimality"]}
+                          for k, r in enumerate(res) if r["result"] != "SUCCESS"},
         "newton_iterations": _stats([r["iterations"] for r in res]),
         "pcg_iterations_max": max(r["pcg_iterations"] for r in res),
         "solo_check": {"instances": [0, count // 2, count - 1],
@@ -105,7 +112,7 @@ def rods(args, log):
     nv = int(sc.nverts.sum())
     log(f"rods setup {time.perf_counter() - t0:.1f}s: {count} rods, {nv} vertices, "
         f"{R.N} DoF")
-    opts = dict(tolMaxError=1e-8, numMaxIters=100, tolMinImpr=0.0, tolMinStep=0.0,
+    opts = dict(tolMaxError=1e-8, numMaxIters=300, tolMinImpr=0.0, tolMinStep=0.0,
                 ls_noise_guard=1e-12)
     warm = min(args.warmup, 1)
     for _ in range(warm):
@@ -133,6 +140,8 @@ def rods(args, log):
                    "linear_solver": "banded LDL^T per rod"},
         "converged": sum(r["result"] == "SUCCESS" for r in res),
         "instances": count,
+        "results": {o: sum(r["result"] == o for r in res)
+                    for o in sorted(set(r["result"] for r in res))},
         "newton_iterations": _stats([r["iterations"] for r in res]),
     }
 
@@ -149,7 +158,7 @@ def _serving_mix(count, max_tets, max_cloth, seed=5):
                                          name=f"beam_{d}x{d}x{4 * d}"))
         else:
             n = int(rng.integers(32, max_cloth + 1))
-            out.append(scenes.cloth(n, stages=10, name=f"cloth_{n}"))
+            out.append(scenes.cloth(n, stages=10, vertical=True, name=f"cloth_{n}"))
     return out
 
 
@@ -196,6 +205,8 @@ def serving(args, log):
                                f"2k-{args.max_tets} tets, 40% cloths 32-{args.max_cloth}",
                    "lanes": 4, "max_batch": 16},
         "converged": ok,
+        "not_converged": {f"{i}:{reqs[i].name}": results[i]["result"] for i in range(count)
+                          if results[i]["result"] != "SUCCESS"},
         "latency_ms_p50": float(np.percentile(lat, 50)) * 1e3,
         "latency_ms_p99": float(np.percentile(lat, 99)) * 1e3,
         "queue_ms_p50": float(np.percentile(queue, 50)) * 1e3,
