@@ -115,6 +115,10 @@ def main():
                     help="0 Jacobi, 1 3x3 block Jacobi (default)")
     ap.add_argument("--adaptive", type=int, default=1,
                     help="1 (default): Eisenstat-Walker linear tolerance")
+    ap.add_argument("--ls-max-iters", type=int, default=2000,
+                    help="PCG iteration cap per linear solve of the timed path "
+                         "(default 2000; the reference's lsNumMaxIters is 1000, kept "
+                         "by the plain FP64 path)")
     ap.add_argument("--sweep", action="store_true",
                     help="also time one solve per (precond, adaptive) variant")
     ap.add_argument("--kernels-only", action="store_true",
@@ -158,7 +162,8 @@ def main():
     t0 = time.perf_counter()
     scene = scenes.config2()
     ctx = pg.Context(0)
-    opts = dict(pg.CONVERGE_OPTS, precond=args.precond, adaptive_tol=args.adaptive)
+    opts = dict(pg.CONVERGE_OPTS, precond=args.precond, adaptive_tol=args.adaptive,
+                lsNumMaxIters=args.ls_max_iters)
     M = pg.TetModel(ctx, scene.X0, scene.T, scene.material, scene.lame1, scene.lame2,
                     scene.density, pg.MIXED)
     scene.apply(M)
@@ -272,7 +277,8 @@ def main():
                                "811,923 DoF, 10 load stages, Newton to |g|<=1e-8",
                    "precision": "mixed", "precond": "jacobi" if args.precond == 0
                    else "block-jacobi",
-                   "linear_tol": "eisenstat-walker" if args.adaptive else "1e-9"},
+                   "linear_tol": "eisenstat-walker" if args.adaptive else "1e-9",
+                   "pcg_max_iters": args.ls_max_iters},
         "converged": ok,
         "newton_iterations": int(np.mean([r["iterations"] for r in results])),
         "pcg_iterations": int(np.mean([r["pcg_iterations"] for r in results])),

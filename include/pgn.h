@@ -188,12 +188,15 @@ typedef struct {
   int precond;        /* 0 = Jacobi, 1 = 3x3 block Jacobi                  */
   double inner_tol;   /* MIXED: FP32 inner tolerance per refinement sweep */
   int adaptive_tol;   /* 0: every linear solve to lsTolMaxError (reference).
-                         1: Eisenstat-Walker forcing (choice 2, gamma 0.9,
-                         alpha 2, capped at 0.1, floored at lsTolMaxError)  */
+                         1: Eisenstat-Walker forcing (choice 2, gamma
+                         ew_gamma, alpha 2, capped at ew_max, floored at
+                         lsTolMaxError)                                      */
   double ls_noise_guard; /* 0 = reference line search.  > 0: also accept a
                          step whose energy rises by at most ls_noise_guard *
                          (|E_elastic| + |E_load|), i.e. within FP64 rounding
                          of the total, when |g| decreases                   */
+  double ew_gamma;    /* Eisenstat-Walker gamma (default 0.9)              */
+  double ew_max;      /* largest forcing term (default 0.1)                */
 } pgn_newton_opts;
 
 /* OSResult (BasicTypes.h:69-77) */

@@ -80,7 +80,8 @@ class NewtonOpts(ctypes.Structure):
                 ("lsNumMaxIters", ctypes.c_int), ("lsTolMaxError", ctypes.c_double),
                 ("lsNumRegTrials", ctypes.c_int), ("precond", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("adaptive_tol", ctypes.c_int),
-                ("ls_noise_guard", ctypes.c_double)]
+                ("ls_noise_guard", ctypes.c_double), ("ew_gamma", ctypes.c_double),
+                ("ew_max", ctypes.c_double)]
 
 
 class ServerOpts(ctypes.Structure):

@@ -436,7 +436,7 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
       nbis(Kn, 0), searching(Kn, 0);
   std::vector<uint8_t> hmode(Kn, 0), hmask(Kn, 0);
   std::vector<double> obj(Kn), opt(Kn), optStart(Kn), objNew(Kn), optNew(Kn), f(Kn), len(Kn),
-      clamp(Kn), stepLen(Kn), hfac(Kn), eta(Kn, 0.1), gprev(Kn, -1.0), hn2(Kn);
+      clamp(Kn), stepLen(Kn), hfac(Kn), eta(Kn, o.ew_max), gprev(Kn, -1.0), hn2(Kn);
   std::vector<double> E;
   auto d2d = [&](double* dst, const double* src) {
     PGN_HIP(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -481,10 +481,10 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
         if (o.adaptive_tol) {  // Eisenstat-Walker choice 2 per instance
           if (gprev[k] > 0.0) {
             const double r = opt[k] / gprev[k];
-            double e = 0.9 * r * r;
-            const double guard = 0.9 * eta[k] * eta[k];
+            double e = o.ew_gamma * r * r;
+            const double guard = o.ew_gamma * eta[k] * eta[k];
             if (guard > 0.1) e = std::max(e, guard);
-            eta[k] = std::min(0.1, e);
+            eta[k] = std::min(o.ew_max, e);
           }
           gprev[k] = opt[k];
           lt = std::max(o.lsTolMaxError, eta[k]);

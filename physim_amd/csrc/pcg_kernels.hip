@@ -26,14 +26,15 @@ constexpr int LPR = 8;       // lanes per row of the plain SpMV / residual kerne
 constexpr int PCG_BS = 256;
 constexpr int ROWS_PER_BLOCK = PCG_BS / LPR;
 
+static inline int round8(int g) { return g >= 8 ? g - g % 8 : g; }
 int pcg_grid1(int n, int lpr) {
   const int rows = K1_BS / lpr;
   int g = (n + rows - 1) / rows;
-  return g < 1 ? 1 : (g > PCG_MAXP ? PCG_MAXP : g);
+  return round8(g < 1 ? 1 : (g > PCG_MAXP ? PCG_MAXP : g));
 }
 int pcg_grid2(int n) {
   int g = (n + K2_BS - 1) / K2_BS;
-  return g < 1 ? 1 : (g > K2_MAXGRID ? K2_MAXGRID : g);
+  return round8(g < 1 ? 1 : (g > K2_MAXGRID ? K2_MAXGRID : g));
 }
 static inline int grid_rows(int n) {
   int g = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
@@ -50,6 +51,25 @@ __device__ __forceinline__ T group_sum(T v) {
 template <class T>
 __device__ __forceinline__ T group_sum8(T v) {
   return group_sum<8>(v);
+}
+
+// XCD-aware blocked partition of [0, n) rows over the grid.  Workgroups are
+// handed to the 8 XCDs round-robin by blockIdx.x, and each XCD has its own
+// L2: logical block (b % 8) * (G / 8) + b / 8 gives every XCD one contiguous
+// eighth of the rows, so its gathers of neighbouring rows' vector entries
+// stay inside its own L2 instead of every XCD fetching the whole vector.
+// Each logical block then takes one contiguous chunk of rows.
+struct RowRange {
+  long lo, hi;
+};
+__device__ __forceinline__ RowRange block_rows(long n) {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int lb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+  const long chunk = (n + G - 1) / G;
+  RowRange r;
+  r.lo = (long)lb * chunk;
+  r.hi = r.lo + chunk < n ? r.lo + chunk : n;
+  return r;
 }
 
 template <class T>
@@ -101,8 +121,8 @@ __global__ __launch_bounds__(K1_BS) void pcg_init_kernel(PcgArgs A, const T* __r
   T* __restrict__ p1 = reinterpret_cast<T*>(A.p1);
   const int lane = threadIdx.x & (L - 1);
   double bb = 0.0, rr = 0.0, rz = 0.0;
-  for (int row = (blockIdx.x * K1_BS + threadIdx.x) / L; row < A.n;
-       row += gridDim.x * (K1_BS / L)) {
+  const RowRange rg = block_rows(A.n);
+  for (long row = rg.lo + threadIdx.x / L; row < rg.hi; row += K1_BS / L) {
     T y0 = 0, y1 = 0, y2 = 0;
     if (!x_zero) {
       const long e1 = A.rowptr[row + 1];
@@ -200,8 +220,8 @@ __global__ __launch_bounds__(K1_BS) void pcg_k1_kernel(PcgArgs A, int k) {
   T* __restrict__ q = reinterpret_cast<T*>(A.q);
   const int lane = threadIdx.x & (L - 1);
   double pq = 0.0;
-  for (int row = (blockIdx.x * K1_BS + threadIdx.x) / L; row < A.n;
-       row += gridDim.x * (K1_BS / L)) {
+  const RowRange rg = block_rows(A.n);
+  for (long row = rg.lo + threadIdx.x / L; row < rg.hi; row += K1_BS / L) {
     T y0 = 0, y1 = 0, y2 = 0;
     const long e1 = A.rowptr[row + 1];
     for (long s = A.rowptr[row] + lane; s < e1; s += L) {
@@ -250,7 +270,8 @@ __global__ __launch_bounds__(K2_BS) void pcg_k2_kernel(PcgArgs A, int k) {
   T* __restrict__ r = reinterpret_cast<T*>(A.r);
   T* __restrict__ z = reinterpret_cast<T*>(A.z);
   double rr = 0.0, rzn = 0.0;
-  for (long i = blockIdx.x * K2_BS + threadIdx.x; i < A.n; i += (long)gridDim.x * K2_BS) {
+  const RowRange rg = block_rows(A.n);
+  for (long i = rg.lo + threadIdx.x; i < rg.hi; i += K2_BS) {
     T ri[3], zi[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -419,8 +440,8 @@ __global__ __launch_bounds__(K1_BS) void spmv_kernel(int n, const long* __restri
                                                     const T* __restrict__ x,
                                                     T* __restrict__ y) {
   const int lane = threadIdx.x & (L - 1);
-  for (int row = (blockIdx.x * K1_BS + threadIdx.x) / L; row < n;
-       row += gridDim.x * (K1_BS / L)) {
+  const RowRange rg = block_rows(n);
+  for (long row = rg.lo + threadIdx.x / L; row < rg.hi; row += K1_BS / L) {
     T y0 = 0, y1 = 0, y2 = 0;
     const long e1 = rowptr[row + 1];
     for (long s = rowptr[row] + lane; s < e1; s += L) {

@@ -5,6 +5,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 namespace pgn {
 
@@ -176,6 +178,8 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
     PGN_HIP(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyDeviceToDevice, s));
   };
 
+  // PGN_TRACE=1 prints one line per Newton step (diagnostics only)
+  const bool trace = std::getenv("PGN_TRACE") != nullptr;
   // OptimSolver::Init (OptimSolver.cpp:36-55)
   d2d(v0, x);
   double obj = 0.0, gg = 0.0;
@@ -189,7 +193,7 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
   int itCount = 0;
   // Eisenstat-Walker forcing (choice 2, gamma 0.9, alpha 2) when adaptive_tol:
   // the linear tolerance follows the Newton residual's own convergence rate.
-  double eta = 0.1, g_prev = -1.0;
+  double eta = o.ew_max, g_prev = -1.0;
 
   // OptimSolver_USQP_LS::SolveStepInternal (OptimSolver_USQP_LS.cpp:18-68)
   auto solve_step = [&](bool use_steepest) {
@@ -212,10 +216,10 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
       if (o.adaptive_tol) {
         if (g_prev > 0.0) {
           const double r = opt / g_prev;
-          double e = 0.9 * r * r;
-          const double guard = 0.9 * eta * eta;
+          double e = o.ew_gamma * r * r;
+          const double guard = o.ew_gamma * eta * eta;
           if (guard > 0.1) e = std::max(e, guard);
-          eta = std::min(0.1, e);
+          eta = std::min(o.ew_max, e);
         }
         lo.tol = std::max(o.lsTolMaxError, eta);
         g_prev = opt;
@@ -223,6 +227,10 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
       pgn_linear_result lr;
       linear_solve(M, bvec, dx, lo, lr);
       R.pcg_iterations += lr.iterations;
+      if (trace)
+        std::fprintf(stderr, "[pgn] newton %d |g| %.3e lin tol %.1e pcg %d refine %d reg %d relres %.2e\n",
+                     itCount, opt, lo.tol, lr.iterations, lr.refinements, lr.reg_used,
+                     lr.rel_residual);
     } else {
       launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, dx);
     }

@@ -26,7 +26,7 @@ import numpy as np
 import physim_amd as pg
 from physim_amd import scenes
 
-SOLVE_OPTS = dict(pg.CONVERGE_OPTS, precond=1, adaptive_tol=1)
+SOLVE_OPTS = dict(pg.CONVERGE_OPTS, precond=1, adaptive_tol=1, lsNumMaxIters=2000)
 
 
 def _stats(vals):
