@@ -135,6 +135,7 @@ struct PcgArgs {
   const void* binv;      // block Jacobi (9n)
   int precond;           // 0 Jacobi, 1 block Jacobi
   int lpr;               // lanes per block row in the SpMV (4, 8 or 16)
+  int split;             // K1 as two launches (p update, then plain SpMV)
   int g1, g2;            // workgroups of K1/init and of K2
   double tol;
   void* x;
@@ -152,7 +153,8 @@ int pcg_grid1(int n, int lpr);
 int pcg_grid2(int n);
 // r = b - A x (b in T); z = M^-1 r; p = 0.  x_zero: start from x = 0.
 void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, int x_zero);
-// iteration k: K1 (beta, p_k, q = A p_k, p.q) and K2 (alpha, x, r, z, r.r, r.z)
+// iteration k: K1 (beta, p_k, q = A p_k, p.q; one launch, or two when split)
+// and K2 (alpha, x, r, z, r.r, r.z)
 void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a, int k);
 void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k);
 // after iterations [0, k_next): convergence test, sets done / iters / rr

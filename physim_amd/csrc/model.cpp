@@ -418,6 +418,7 @@ BsrArgs pgn_model::bsr_args() const {
 PcgArgs pgn_model::pcg_args(double tol, int precond) const {
   PcgArgs a;
   a.lpr = pcg_lpr;
+  a.split = pcg_split ? 1 : 0;
   a.g1 = pcg_grid1(n, pcg_lpr);
   a.g2 = pcg_grid2(n);
   a.tol = tol;

@@ -1,6 +1,7 @@
 // model.hpp -- device-resident tet FEM model (the ISimulable of libpgn).
 #pragma once
 
+#include <cstdlib>
 #include <memory>
 
 #include "kernels.hpp"
@@ -70,6 +71,9 @@ struct pgn_model {
   pgn::DBuf<char> pb;     // rhs in Hessian precision
   pgn::DBuf<double> part1, part2, partb, red64;
   int pcg_lpr = 8;  // lanes per block row of the PCG SpMV (tuned per model)
+  // K1 split into a streaming p update + plain SpMV (default); PGN_PCG_FUSED
+  // in the environment selects the single fused launch instead
+  bool pcg_split = std::getenv("PGN_PCG_FUSED") == nullptr;
   bool tuned = false;
   double tune_us[3] = {0, 0, 0};  // SpMV time for 4, 8, 16 lanes per row
   pgn::DBuf<pgn::PcgScalars> sc;

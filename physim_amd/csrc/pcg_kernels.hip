@@ -101,6 +101,47 @@ __device__ __forceinline__ double all_sum(const double* __restrict__ a, int cnt,
   return r;
 }
 
+// Three fixed-order sums in one pass (one latency instead of three); each
+// result is bitwise what all_sum returns for it.
+template <int BS>
+__device__ __forceinline__ void all_sum3(const double* __restrict__ a, int ca, int sa,
+                                         const double* __restrict__ b, int cb, int sb,
+                                         const double* __restrict__ c, int cc, int scs,
+                                         double* lds, double& ra, double& rb, double& rc) {
+  double va = 0.0, vb = 0.0, vc = 0.0;
+  for (int i = threadIdx.x; i < ca; i += BS) va += a[(long)i * sa];
+  for (int i = threadIdx.x; i < cb; i += BS) vb += b[(long)i * sb];
+  for (int i = threadIdx.x; i < cc; i += BS) vc += c[(long)i * scs];
+  va = wave_sum(va);
+  vb = wave_sum(vb);
+  vc = wave_sum(vc);
+  constexpr int NW = BS / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[w] = va;
+    lds[NW + w] = vb;
+    lds[2 * NW + w] = vc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      x += lds[i];
+      y += lds[NW + i];
+      z += lds[2 * NW + i];
+    }
+    lds[3 * NW] = x;
+    lds[3 * NW + 1] = y;
+    lds[3 * NW + 2] = z;
+  }
+  __syncthreads();
+  ra = lds[3 * NW];
+  rb = lds[3 * NW + 1];
+  rc = lds[3 * NW + 2];
+  __syncthreads();
+}
+
 __device__ __forceinline__ const double* rr_slot(const PcgArgs& A, int k) {
   return A.part2 + (long)(k & 1) * 2 * PCG_MAXP;  // iteration k's {r.r, r.z}
 }
@@ -172,19 +213,18 @@ __global__ __launch_bounds__(K1_BS) void pcg_init_kernel(PcgArgs A, const T* __r
 }
 
 // ---- K1: convergence test of k-1; p_k = z + beta p_{k-1}; q = A p_k; p.q ----
-template <class T, int L>
-__global__ __launch_bounds__(K1_BS) void pcg_k1_kernel(PcgArgs A, int k) {
-  __shared__ double red[2 * K1_BS / 64 + 2];
+// convergence test of iteration k-1 and beta_k, shared by the fused K1 and
+// the split K1a; returns false when the solve is done (all threads agree)
+template <int BS>
+__device__ __forceinline__ bool pcg_beta(const PcgArgs& A, int k, double* red, double& beta) {
   PcgScalars* sc = A.sc;
-  if (sc->done) return;
   const double* prev = rr_slot(A, k - 1);
   const int cprev = rr_count(A, k - 1);
-  const double rr = all_sum<K1_BS>(prev, cprev, 2, red);
-  const double rz = all_sum<K1_BS>(prev + 1, cprev, 2, red);
-  double thr, beta;
+  double rr, rz, rzm = 1.0;
   if (k == 0) {
-    const double bb = all_sum<K1_BS>(A.partb, A.g1, 1, red);
-    thr = A.tol * A.tol * bb;
+    double bb;
+    all_sum3<BS>(prev, cprev, 2, prev + 1, cprev, 2, A.partb, A.g1, 1, red, rr, rz, bb);
+    double thr = A.tol * A.tol * bb;
     if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->thr = thr;
@@ -196,22 +236,32 @@ __global__ __launch_bounds__(K1_BS) void pcg_k1_kernel(PcgArgs A, int k) {
         sc->done = 1;
         sc->iters = 0;
       }
-      return;
+      return false;
     }
     beta = 0.0;
-  } else {
-    thr = sc->thr;
-    if (rr < thr) {  // iteration k-1 converged (Eigen breaks with i = k-1)
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        sc->done = 1;
-        sc->iters = k - 1;
-        sc->rr = rr;
-      }
-      return;
-    }
-    const double rzm = all_sum<K1_BS>(rr_slot(A, k - 2) + 1, rr_count(A, k - 2), 2, red);
-    beta = rz / rzm;
+    return true;
   }
+  all_sum3<BS>(prev, cprev, 2, prev + 1, cprev, 2, rr_slot(A, k - 2) + 1, rr_count(A, k - 2), 2,
+               red, rr, rz, rzm);
+  if (rr < sc->thr) {  // iteration k-1 converged (Eigen breaks with i = k-1)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->done = 1;
+      sc->iters = k - 1;
+      sc->rr = rr;
+    }
+    return false;
+  }
+  beta = rz / rzm;
+  return true;
+}
+
+template <class T, int L>
+__global__ __launch_bounds__(K1_BS) void pcg_k1_kernel(PcgArgs A, int k) {
+  __shared__ double red[3 * K1_BS / 64 + 4];
+  PcgScalars* sc = A.sc;
+  if (sc->done) return;
+  double beta;
+  if (!pcg_beta<K1_BS>(A, k, red, beta)) return;
   const T tb = (T)beta;
   const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
   const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
@@ -255,14 +305,72 @@ __global__ __launch_bounds__(K1_BS) void pcg_k1_kernel(PcgArgs A, int k) {
   if (threadIdx.x == 0) A.part1[blockIdx.x] = s;
 }
 
+// ---- split K1: K1a (beta, p = z + beta p_old on own rows, streaming) and K1b
+// (plain SpMV q = A p, p.q).  The plain SpMV streams the matrix at copy
+// bandwidth; folding the p update into its column gathers costs more than
+// the extra launch.
+template <class T>
+__global__ __launch_bounds__(K2_BS) void pcg_k1a_kernel(PcgArgs A, int k) {
+  __shared__ double red[3 * K2_BS / 64 + 4];
+  if (A.sc->done) return;
+  double beta;
+  if (!pcg_beta<K2_BS>(A, k, red, beta)) return;
+  const T tb = (T)beta;
+  const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
+  T* __restrict__ pnew = reinterpret_cast<T*>((k & 1) ? A.p1 : A.p0);
+  const T* __restrict__ pold = reinterpret_cast<const T*>((k & 1) ? A.p0 : A.p1);
+  const RowRange rg = block_rows(A.n);
+  for (long i = rg.lo + threadIdx.x; i < rg.hi; i += K2_BS)
+#pragma unroll
+    for (int p = 0; p < 3; ++p) pnew[3 * i + p] = z[3 * i + p] + tb * pold[3 * i + p];
+}
+
+template <class T, int L>
+__global__ __launch_bounds__(K1_BS) void pcg_k1b_kernel(PcgArgs A, int k) {
+  __shared__ double red[K1_BS / 64 + 2];
+  if (A.sc->done) return;
+  const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
+  const T* __restrict__ p = reinterpret_cast<const T*>((k & 1) ? A.p1 : A.p0);
+  T* __restrict__ q = reinterpret_cast<T*>(A.q);
+  const int lane = threadIdx.x & (L - 1);
+  double pq = 0.0;
+  const RowRange rg = block_rows(A.n);
+  for (long row = rg.lo + threadIdx.x / L; row < rg.hi; row += K1_BS / L) {
+    T y0 = 0, y1 = 0, y2 = 0;
+    const long e1 = A.rowptr[row + 1];
+    for (long s = A.rowptr[row] + lane; s < e1; s += L) {
+      const long j = A.bcol[s];
+      const T* B = bval + s * 9;
+      const T pj0 = p[3 * j], pj1 = p[3 * j + 1], pj2 = p[3 * j + 2];
+      y0 += B[0] * pj0 + B[1] * pj1 + B[2] * pj2;
+      y1 += B[3] * pj0 + B[4] * pj1 + B[5] * pj2;
+      y2 += B[6] * pj0 + B[7] * pj1 + B[8] * pj2;
+    }
+    y0 = group_sum<L>(y0);
+    y1 = group_sum<L>(y1);
+    y2 = group_sum<L>(y2);
+    if (lane == 0) {
+      const long i = row;
+      q[3 * i] = y0;
+      q[3 * i + 1] = y1;
+      q[3 * i + 2] = y2;
+      pq += (double)p[3 * i] * (double)y0 + (double)p[3 * i + 1] * (double)y1 +
+            (double)p[3 * i + 2] * (double)y2;
+    }
+  }
+  const double s = block_sum<K1_BS>(pq, red);
+  if (threadIdx.x == 0) A.part1[blockIdx.x] = s;
+}
+
 // ---- K2: x += alpha p; r -= alpha q; z = M^-1 r; r.r, r.z ---------------------
 template <class T>
 __global__ __launch_bounds__(K2_BS) void pcg_k2_kernel(PcgArgs A, int k) {
-  __shared__ double red[2 * K2_BS / 64 + 2];
+  __shared__ double red[3 * K2_BS / 64 + 4];
   PcgScalars* sc = A.sc;
   if (sc->done) return;
-  const double pq = all_sum<K2_BS>(A.part1, A.g1, 1, red);
-  const double rz = all_sum<K2_BS>(rr_slot(A, k - 1) + 1, rr_count(A, k - 1), 2, red);
+  double pq, rz, unused;
+  all_sum3<K2_BS>(A.part1, A.g1, 1, rr_slot(A, k - 1) + 1, rr_count(A, k - 1), 2, A.part1, 0, 1,
+                  red, pq, rz, unused);
   const T ta = (T)(rz / pq);
   const T* __restrict__ pk = reinterpret_cast<const T*>((k & 1) ? A.p1 : A.p0);
   const T* __restrict__ q = reinterpret_cast<const T*>(A.q);
@@ -347,6 +455,16 @@ void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, 
 
 template <int L>
 static void launch_k1_l(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
+  if (a.split) {
+    if (fp32) {
+      hipLaunchKernelGGL(pcg_k1a_kernel<float>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+      hipLaunchKernelGGL((pcg_k1b_kernel<float, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
+    } else {
+      hipLaunchKernelGGL(pcg_k1a_kernel<double>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+      hipLaunchKernelGGL((pcg_k1b_kernel<double, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
+    }
+    return;
+  }
   if (fp32)
     hipLaunchKernelGGL((pcg_k1_kernel<float, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
   else
