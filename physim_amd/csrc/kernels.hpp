@@ -136,6 +136,9 @@ struct PcgArgs {
   int precond;           // 0 Jacobi, 1 block Jacobi
   int lpr;               // lanes per block row in the SpMV (4, 8 or 16)
   int split;             // K1 as two launches (p update, then plain SpMV)
+  int cg1;               // Chronopoulos-Gear form: K1 = SpMV on z, K2 = every
+                         // vector update (p, q, x, r, z); overrides split
+  double* cgs;           // cg1 scalars: 2 parities x {alpha, r.z}
   int g1, g2;            // workgroups of K1/init and of K2
   double tol;
   void* x;

@@ -299,6 +299,7 @@ void pgn_model::setup(int n_nodes, const double* X0h, int n_elems, const int* Th
   partb.alloc(PCG_MAXP);
   red64.alloc(PCG_MAXP + 8);
   sc.alloc(1);
+  cgs.alloc(4);
   r64.alloc(3L * n);
   tmp64a.alloc(3L * n);
   tmp64b.alloc(3L * n);
@@ -419,6 +420,8 @@ PcgArgs pgn_model::pcg_args(double tol, int precond) const {
   PcgArgs a;
   a.lpr = pcg_lpr;
   a.split = pcg_split ? 1 : 0;
+  a.cg1 = pcg_cg1 ? 1 : 0;
+  a.cgs = cgs.p;
   a.g1 = pcg_grid1(n, pcg_lpr);
   a.g2 = pcg_grid2(n);
   a.tol = tol;

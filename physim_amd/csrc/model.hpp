@@ -74,6 +74,10 @@ struct pgn_model {
   // K1 split into a streaming p update + plain SpMV (default); PGN_PCG_FUSED
   // in the environment selects the single fused launch instead
   bool pcg_split = std::getenv("PGN_PCG_FUSED") == nullptr;
+  // Chronopoulos-Gear PCG (one SpMV + one streaming launch per iteration):
+  // PGN_PCG_CG1 in the environment selects it
+  bool pcg_cg1 = std::getenv("PGN_PCG_CG1") != nullptr;
+  pgn::DBuf<double> cgs;  // its 4 scalars
   bool tuned = false;
   double tune_us[3] = {0, 0, 0};  // SpMV time for 4, 8, 16 lanes per row
   pgn::DBuf<pgn::PcgScalars> sc;

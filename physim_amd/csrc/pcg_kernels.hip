@@ -160,6 +160,7 @@ __global__ __launch_bounds__(K1_BS) void pcg_init_kernel(PcgArgs A, const T* __r
   T* __restrict__ z = reinterpret_cast<T*>(A.z);
   T* __restrict__ p0 = reinterpret_cast<T*>(A.p0);
   T* __restrict__ p1 = reinterpret_cast<T*>(A.p1);
+  T* __restrict__ qv = reinterpret_cast<T*>(A.q);
   const int lane = threadIdx.x & (L - 1);
   double bb = 0.0, rr = 0.0, rz = 0.0;
   const RowRange rg = block_rows(A.n);
@@ -191,6 +192,7 @@ __global__ __launch_bounds__(K1_BS) void pcg_init_kernel(PcgArgs A, const T* __r
         z[3 * i + p] = zi[p];
         p0[3 * i + p] = T(0);
         p1[3 * i + p] = T(0);
+        qv[3 * i + p] = T(0);
         if (x_zero) x[3 * i + p] = T(0);
         bb += (double)b[3 * i + p] * (double)b[3 * i + p];
         rr += (double)ri[p] * (double)ri[p];
@@ -362,6 +364,142 @@ __global__ __launch_bounds__(K1_BS) void pcg_k1b_kernel(PcgArgs A, int k) {
   if (threadIdx.x == 0) A.part1[blockIdx.x] = s;
 }
 
+// ---- Chronopoulos-Gear PCG (cg1) ------------------------------------------------
+// (A. T. Chronopoulos, C. W. Gear, J. Comput. Appl. Math. 25 (1989)): with
+// w = A z, gamma = r.z and delta = w.z from the same iteration,
+//   beta_k = gamma_k / gamma_{k-1},
+//   alpha_k = gamma_k / (delta_k - beta_k gamma_k / alpha_{k-1}),
+//   p = z + beta p, q = w + beta q (= A p), x += alpha p, r -= alpha q, z = M^-1 r.
+// The SpMV reads z, which the previous launch completed, so each iteration is
+// one SpMV launch (w, delta partials) and one streaming launch (all vector
+// updates, gamma and r.r partials): no separate p update.  Vectors: z, p0 = p,
+// p1 = w, q = q.  Scalars alpha, gamma pass between iterations through cgs
+// (double-buffered by parity; block 0 writes them).
+template <class T, int L>
+__global__ __launch_bounds__(K1_BS) void cg1_a_kernel(PcgArgs A, int k) {
+  __shared__ double red[3 * K1_BS / 64 + 4];
+  PcgScalars* sc = A.sc;
+  if (sc->done) return;
+  const double* prev = rr_slot(A, k - 1);
+  const int cprev = rr_count(A, k - 1);
+  double rr, bb, unused;
+  all_sum3<K1_BS>(prev, cprev, 2, A.partb, k == 0 ? A.g1 : 0, 1, A.partb, 0, 1, red, rr, bb,
+                  unused);
+  if (k == 0) {
+    double thr = A.tol * A.tol * bb;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->thr = thr;
+      sc->bb = bb;
+      sc->rr = rr;
+    }
+    if (bb == 0.0 || rr < thr) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->done = 1;
+        sc->iters = 0;
+      }
+      return;
+    }
+  } else if (rr < sc->thr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->done = 1;
+      sc->iters = k - 1;
+      sc->rr = rr;
+    }
+    return;
+  }
+  const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
+  const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
+  T* __restrict__ w = reinterpret_cast<T*>(A.p1);
+  const int lane = threadIdx.x & (L - 1);
+  double dl = 0.0;
+  const RowRange rg = block_rows(A.n);
+  for (long row = rg.lo + threadIdx.x / L; row < rg.hi; row += K1_BS / L) {
+    T y0 = 0, y1 = 0, y2 = 0;
+    const long e1 = A.rowptr[row + 1];
+    for (long s = A.rowptr[row] + lane; s < e1; s += L) {
+      const long j = A.bcol[s];
+      const T* B = bval + s * 9;
+      const T zj0 = z[3 * j], zj1 = z[3 * j + 1], zj2 = z[3 * j + 2];
+      y0 += B[0] * zj0 + B[1] * zj1 + B[2] * zj2;
+      y1 += B[3] * zj0 + B[4] * zj1 + B[5] * zj2;
+      y2 += B[6] * zj0 + B[7] * zj1 + B[8] * zj2;
+    }
+    y0 = group_sum<L>(y0);
+    y1 = group_sum<L>(y1);
+    y2 = group_sum<L>(y2);
+    if (lane == 0) {
+      const long i = row;
+      w[3 * i] = y0;
+      w[3 * i + 1] = y1;
+      w[3 * i + 2] = y2;
+      dl += (double)z[3 * i] * (double)y0 + (double)z[3 * i + 1] * (double)y1 +
+            (double)z[3 * i + 2] * (double)y2;
+    }
+  }
+  const double s = block_sum<K1_BS>(dl, red);
+  if (threadIdx.x == 0) A.part1[blockIdx.x] = s;
+}
+
+template <class T>
+__global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
+  __shared__ double red[3 * K2_BS / 64 + 4];
+  if (A.sc->done) return;
+  double gam, del, unused;
+  all_sum3<K2_BS>(rr_slot(A, k - 1) + 1, rr_count(A, k - 1), 2, A.part1, A.g1, 1, A.part1, 0, 1,
+                  red, gam, del, unused);
+  double alpha, beta;
+  if (k == 0) {
+    beta = 0.0;
+    alpha = gam / del;
+  } else {
+    const double* prev = A.cgs + 2 * ((k - 1) & 1);
+    beta = gam / prev[1];
+    alpha = gam / (del - beta * gam / prev[0]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.cgs[2 * (k & 1)] = alpha;
+    A.cgs[2 * (k & 1) + 1] = gam;
+  }
+  const T ta = (T)alpha, tb = (T)beta;
+  T* __restrict__ p = reinterpret_cast<T*>(A.p0);
+  T* __restrict__ q = reinterpret_cast<T*>(A.q);
+  const T* __restrict__ w = reinterpret_cast<const T*>(A.p1);
+  T* __restrict__ x = reinterpret_cast<T*>(A.x);
+  T* __restrict__ r = reinterpret_cast<T*>(A.r);
+  T* __restrict__ z = reinterpret_cast<T*>(A.z);
+  double rr = 0.0, rz = 0.0;
+  const RowRange rg = block_rows(A.n);
+  for (long i = rg.lo + threadIdx.x; i < rg.hi; i += K2_BS) {
+    T ri[3], zi[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long d = 3 * i + c;
+      const T pd = z[d] + tb * p[d];
+      const T qd = w[d] + tb * q[d];
+      p[d] = pd;
+      q[d] = qd;
+      x[d] += ta * pd;
+      ri[c] = r[d] - ta * qd;
+      r[d] = ri[c];
+    }
+    apply_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
+                     reinterpret_cast<const T*>(A.binv), i, ri, zi);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      z[3 * i + c] = zi[c];
+      rr += (double)ri[c] * (double)ri[c];
+      rz += (double)ri[c] * (double)zi[c];
+    }
+  }
+  block_sum2<K2_BS>(rr, rz, red);
+  if (threadIdx.x == 0) {
+    double* o = A.part2 + (long)(k & 1) * 2 * PCG_MAXP;
+    o[2 * blockIdx.x] = rr;
+    o[2 * blockIdx.x + 1] = rz;
+  }
+}
+
 // ---- K2: x += alpha p; r -= alpha q; z = M^-1 r; r.r, r.z ---------------------
 template <class T>
 __global__ __launch_bounds__(K2_BS) void pcg_k2_kernel(PcgArgs A, int k) {
@@ -455,6 +593,13 @@ void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, 
 
 template <int L>
 static void launch_k1_l(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
+  if (a.cg1) {
+    if (fp32)
+      hipLaunchKernelGGL((cg1_a_kernel<float, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
+    else
+      hipLaunchKernelGGL((cg1_a_kernel<double, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
+    return;
+  }
   if (a.split) {
     if (fp32) {
       hipLaunchKernelGGL(pcg_k1a_kernel<float>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
@@ -480,6 +625,13 @@ void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
 }
 
 void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
+  if (a.cg1) {
+    if (fp32)
+      hipLaunchKernelGGL(cg1_b_kernel<float>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+    else
+      hipLaunchKernelGGL(cg1_b_kernel<double>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+    return;
+  }
   if (fp32)
     hipLaunchKernelGGL(pcg_k2_kernel<float>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
   else
