@@ -75,7 +75,7 @@ def measured_traffic(kernel):
     """HBM bytes per launch of `kernel`, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration
     recorded there), or None when no summary holds it."""
-    path = os.path.join(ROOT, "profiles", "pcg_k1_beam40x40x160_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "pcg_spmv_beam40x40x160_hbm_traffic.md")
     try:
         with open(path) as f:
             for line in f:
@@ -89,7 +89,7 @@ def measured_traffic(kernel):
 def kernels_only(M, scene, opts):
     """A short, fixed sequence of the hot kernels at the workload's real
     shape, for hardware-counter (rocprofv3 --pmc) runs: the PCG SpMV
-    (pcg_k1_kernel) launches of one capped solve, the fused element kernel,
+    (pcg_k1b_kernel) launches of one capped solve, the fused element kernel,
     and the 256 MiB calibration copy whose counters scale the others."""
     M.reset()
     while M.step_load():
@@ -289,11 +289,11 @@ def main():
         "copy_gbs": copy_gbs,
         **fp64,
         "pcg_k2_avg_us": k2_ms / max(k2_n, 1) * 1e3,
-        "roofline": {"bound": "hbm", "kernel": "pcg_k1_kernel<float> (BSR3 SpMV)",
+        "roofline": {"bound": "hbm", "kernel": "pcg_k1b_kernel<float> (BSR3 SpMV of the PCG)",
                      "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("pcg_k1_kernel"), "launches": k1_n, "avg_us": spmv_ms * 1e3,
+                     "traffic": measured_traffic("pcg_k1b_kernel"), "launches": k1_n, "avg_us": spmv_ms * 1e3,
                      "bytes_per_launch": spmv_bytes},
         "cpu_baseline": cpu,
         "spmv_autotune_us": {str(l): M.profile(2 + c)[1] for c, l in enumerate((4, 8, 16))},

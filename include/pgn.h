@@ -327,9 +327,9 @@ int pgn_time_kernel(pgn_model* model, int which, int reps, double* ms_mean);
 int pgn_kernel_bytes(pgn_model* model, int which, double* bytes);
 /* Live timing of the PCG launches inside linear / Newton solves: when on,
  * the middle SpMV launch (K1) and vector-update launch (K2) of every queued
- * chunk of PCG iterations (8 to 64 iterations) is bracketed by HIP events on
- * the context stream, a sample that leaves the solve's own timing
- * unperturbed.  Switching on resets the counters; switching off keeps them
+ * chunk of PCG iterations (8 to 64 iterations) carry HIP events stamped by
+ * their own dispatch (hipExtLaunchKernel start / stop events) on the context
+ * stream, a sample that leaves the solve's own timing unperturbed.  Switching on resets the counters; switching off keeps them
  * for pgn_model_get_profile. */
 int pgn_model_set_profiling(pgn_model* model, int on);
 /* which: 0 = K1, 1 = K2: sampled launches that did work and their summed ms;

@@ -560,9 +560,14 @@ int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
       case 7:  // calibration copy: 256 MiB read + 256 MiB written
         *bytes = 2.0 * 256.0 * 1024 * 1024;
         break;
-      case 8:  // PCG K1: SpMV values + block cols + row pointers, z and
-               // p_{k-1} read, p_k and q written
-        *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 4 * 3 * n * hs;
+      case 8:  // the PCG SpMV launch.  Fused K1: values + block cols + row
+               // pointers, z and p_{k-1} read, p_k and q written.  Split
+               // (default) and Chronopoulos-Gear: the plain SpMV, values +
+               // cols + row pointers, one vector read, q written.
+        if (M->pcg_split || M->pcg_cg1)
+          *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 2 * 3 * n * hs;
+        else
+          *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 4 * 3 * n * hs;
         break;
       case 9:  // PCG K2: p, q, x, r, dinv read; x, r, z written
         *bytes = 3 * n * hs * 8;

@@ -158,8 +158,13 @@ int pcg_grid2(int n);
 void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, int x_zero);
 // iteration k: K1 (beta, p_k, q = A p_k, p.q; one launch, or two when split)
 // and K2 (alpha, x, r, z, r.r, r.z)
-void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a, int k);
-void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k);
+// split mode: K1a (beta, p update) must precede K1 (then the plain SpMV)
+void launch_pcg_k1a(hipStream_t s, bool fp32, const PcgArgs& a, int k);
+// e0 / e1 (optional): events stamped with the launch's own start and end
+void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr);
+void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr);
 // after iterations [0, k_next): convergence test, sets done / iters / rr
 void launch_pcg_status(hipStream_t s, const PcgArgs& a, int k_next, int maxit);
 // plain device copy of n floats (bandwidth calibration)

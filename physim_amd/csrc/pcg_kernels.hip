@@ -14,6 +14,8 @@
 // partials in a fixed order (the same bits in every workgroup; the kernel
 // boundary makes them visible).  r.r / r.z partials are double-buffered by the
 // parity of the iteration that produced them; init counts as iteration -1.
+#include <hip/hip_ext.h>
+
 #include "kernels.hpp"
 #include "reduce.hpp"
 
@@ -591,51 +593,71 @@ void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, 
   }
 }
 
+// Launch with optional start / stop events taken from the dispatch itself
+// (hipExtLaunchKernelGGL), so a sampled launch is timed without the gap an
+// event-record command would add.
+template <class K>
+static void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0,
+                         hipEvent_t e1, const PcgArgs& a, int k) {
+  if (e0)
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, a, k);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, a, k);
+}
+
 template <int L>
-static void launch_k1_l(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
+static void launch_k1_l(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t e0,
+                        hipEvent_t e1) {
   if (a.cg1) {
     if (fp32)
-      hipLaunchKernelGGL((cg1_a_kernel<float, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
+      launch_timed(cg1_a_kernel<float, L>, dim3(a.g1), dim3(K1_BS), s, e0, e1, a, k);
     else
-      hipLaunchKernelGGL((cg1_a_kernel<double, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
+      launch_timed(cg1_a_kernel<double, L>, dim3(a.g1), dim3(K1_BS), s, e0, e1, a, k);
     return;
   }
   if (a.split) {
-    if (fp32) {
-      hipLaunchKernelGGL(pcg_k1a_kernel<float>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
-      hipLaunchKernelGGL((pcg_k1b_kernel<float, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
-    } else {
-      hipLaunchKernelGGL(pcg_k1a_kernel<double>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
-      hipLaunchKernelGGL((pcg_k1b_kernel<double, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
-    }
+    if (fp32)
+      launch_timed(pcg_k1b_kernel<float, L>, dim3(a.g1), dim3(K1_BS), s, e0, e1, a, k);
+    else
+      launch_timed(pcg_k1b_kernel<double, L>, dim3(a.g1), dim3(K1_BS), s, e0, e1, a, k);
     return;
   }
   if (fp32)
-    hipLaunchKernelGGL((pcg_k1_kernel<float, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
+    launch_timed(pcg_k1_kernel<float, L>, dim3(a.g1), dim3(K1_BS), s, e0, e1, a, k);
   else
-    hipLaunchKernelGGL((pcg_k1_kernel<double, L>), dim3(a.g1), dim3(K1_BS), 0, s, a, k);
+    launch_timed(pcg_k1_kernel<double, L>, dim3(a.g1), dim3(K1_BS), s, e0, e1, a, k);
 }
 
-void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
+void launch_pcg_k1a(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
+  if (!a.split || a.cg1) return;
+  if (fp32)
+    hipLaunchKernelGGL(pcg_k1a_kernel<float>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+  else
+    hipLaunchKernelGGL(pcg_k1a_kernel<double>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+}
+
+void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t e0,
+                   hipEvent_t e1) {
   switch (a.lpr) {
-    case 4: launch_k1_l<4>(s, fp32, a, k); break;
-    case 16: launch_k1_l<16>(s, fp32, a, k); break;
-    default: launch_k1_l<8>(s, fp32, a, k); break;
+    case 4: launch_k1_l<4>(s, fp32, a, k, e0, e1); break;
+    case 16: launch_k1_l<16>(s, fp32, a, k, e0, e1); break;
+    default: launch_k1_l<8>(s, fp32, a, k, e0, e1); break;
   }
 }
 
-void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k) {
+void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t e0,
+                   hipEvent_t e1) {
   if (a.cg1) {
     if (fp32)
-      hipLaunchKernelGGL(cg1_b_kernel<float>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+      launch_timed(cg1_b_kernel<float>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
     else
-      hipLaunchKernelGGL(cg1_b_kernel<double>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+      launch_timed(cg1_b_kernel<double>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
     return;
   }
   if (fp32)
-    hipLaunchKernelGGL(pcg_k2_kernel<float>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+    launch_timed(pcg_k2_kernel<float>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
   else
-    hipLaunchKernelGGL(pcg_k2_kernel<double>, dim3(a.g2), dim3(K2_BS), 0, s, a, k);
+    launch_timed(pcg_k2_kernel<double>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
 }
 
 void launch_pcg_status(hipStream_t s, const PcgArgs& a, int k_next, int maxit) {

@@ -17,8 +17,8 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
   hipStream_t s = M.stream();
   const PcgArgs a = M.pcg_args(tol, precond);
   launch_pcg_init(s, M.hfp32, a, M.pb.p, x_zero ? 1 : 0);
-  if (M.profiling && M.prof_ev.size() < 3) {
-    while (M.prof_ev.size() < 3) {
+  if (M.profiling && M.prof_ev.size() < 4) {
+    while (M.prof_ev.size() < 4) {
       hipEvent_t e;
       PGN_HIP(hipEventCreate(&e));
       M.prof_ev.push_back(e);
@@ -32,11 +32,11 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
       // profiling brackets one K1 and K2 of every chunk with events: a
       // sample of the launches, so the timed solve itself stays unperturbed
       const bool ev = M.profiling && j == js;
-      if (ev) PGN_HIP(hipEventRecord(M.prof_ev[0], s));
-      launch_pcg_k1(s, M.hfp32, a, k + j);
-      if (ev) PGN_HIP(hipEventRecord(M.prof_ev[1], s));
-      launch_pcg_k2(s, M.hfp32, a, k + j);
-      if (ev) PGN_HIP(hipEventRecord(M.prof_ev[2], s));
+      launch_pcg_k1a(s, M.hfp32, a, k + j);
+      launch_pcg_k1(s, M.hfp32, a, k + j, ev ? M.prof_ev[0] : nullptr,
+                    ev ? M.prof_ev[1] : nullptr);
+      launch_pcg_k2(s, M.hfp32, a, k + j, ev ? M.prof_ev[2] : nullptr,
+                    ev ? M.prof_ev[3] : nullptr);
     }
     k += c;
     launch_pcg_status(s, a, k, maxit);
@@ -47,7 +47,7 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
     if (M.profiling && (!M.hsc->done || M.hsc->iters + 1 > k - c + js)) {
       float t1 = 0.f, t2 = 0.f;
       PGN_HIP(hipEventElapsedTime(&t1, M.prof_ev[0], M.prof_ev[1]));
-      PGN_HIP(hipEventElapsedTime(&t2, M.prof_ev[1], M.prof_ev[2]));
+      PGN_HIP(hipEventElapsedTime(&t2, M.prof_ev[2], M.prof_ev[3]));
       M.prof_ms[0] += t1;
       M.prof_ms[1] += t2;
       M.prof_n[0] += 1;
