@@ -23,7 +23,7 @@ namespace pgn {
 
 constexpr int K1_BS = 512;   // SpMV workgroup
 constexpr int K2_BS = 256;   // vector-update workgroup
-constexpr int K2_MAXGRID = 512;
+constexpr int K2_NPT = 2;    // nodes per thread of one register-staged batch
 constexpr int LPR = 8;       // lanes per row of the plain SpMV / residual kernels
 constexpr int PCG_BS = 256;
 constexpr int ROWS_PER_BLOCK = PCG_BS / LPR;
@@ -34,9 +34,13 @@ int pcg_grid1(int n, int lpr) {
   int g = (n + rows - 1) / rows;
   return round8(g < 1 ? 1 : (g > PCG_MAXP ? PCG_MAXP : g));
 }
+// streaming launches: one register-staged batch of K2_NPT nodes per thread,
+// rounded up to whole XCD rounds
 int pcg_grid2(int n) {
-  int g = (n + K2_BS - 1) / K2_BS;
-  return round8(g < 1 ? 1 : (g > K2_MAXGRID ? K2_MAXGRID : g));
+  int g = (n + K2_BS * K2_NPT - 1) / (K2_BS * K2_NPT);
+  if (g < 1) g = 1;
+  if (g >= 8) g = (g + 7) / 8 * 8;
+  return g > PCG_MAXP ? PCG_MAXP : g;
 }
 static inline int grid_rows(int n) {
   int g = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
@@ -72,6 +76,60 @@ __device__ __forceinline__ RowRange block_rows(long n) {
   r.lo = (long)lb * chunk;
   r.hi = r.lo + chunk < n ? r.lo + chunk : n;
   return r;
+}
+
+// The three components of node i.  The streaming kernels load every operand
+// of a batch of nodes before they store anything: pointer arguments taken
+// from a struct carry no no-alias guarantee, so a store between two loads
+// would order them and cost one memory round trip per component.
+template <class T>
+struct N3 {
+  T c[3];
+};
+template <class T>
+__device__ __forceinline__ N3<T> ld3(const T* a, long i) {
+  N3<T> o;
+  o.c[0] = a[3 * i];
+  o.c[1] = a[3 * i + 1];
+  o.c[2] = a[3 * i + 2];
+  return o;
+}
+template <class T>
+__device__ __forceinline__ void st3(T* a, long i, const N3<T>& v) {
+  a[3 * i] = v.c[0];
+  a[3 * i + 1] = v.c[1];
+  a[3 * i + 2] = v.c[2];
+}
+
+// preconditioner operands of node i: the 3x3 block inverse (precond 1) or
+// the three Jacobi entries
+template <class T>
+struct PreM {
+  T m[9];
+};
+template <class T>
+__device__ __forceinline__ void ld_precond(int precond, const T* dinv, const T* binv, long i,
+                                           PreM<T>& M) {
+  if (precond == 1) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) M.m[e] = binv[9 * i + e];
+  } else {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) M.m[e] = dinv[3 * i + e];
+  }
+}
+template <class T>
+__device__ __forceinline__ N3<T> mul_precond(int precond, const PreM<T>& M, const N3<T>& r) {
+  N3<T> z;
+  if (precond == 1) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+      z.c[p] = M.m[3 * p] * r.c[0] + M.m[3 * p + 1] * r.c[1] + M.m[3 * p + 2] * r.c[2];
+  } else {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) z.c[p] = M.m[p] * r.c[p];
+  }
+  return z;
 }
 
 template <class T>
@@ -317,16 +375,46 @@ template <class T>
 __global__ __launch_bounds__(K2_BS) void pcg_k1a_kernel(PcgArgs A, int k) {
   __shared__ double red[3 * K2_BS / 64 + 4];
   if (A.sc->done) return;
+  const T* z = reinterpret_cast<const T*>(A.z);
+  T* pnew = reinterpret_cast<T*>((k & 1) ? A.p1 : A.p0);
+  const T* pold = reinterpret_cast<const T*>((k & 1) ? A.p0 : A.p1);
+  const RowRange rg = block_rows(A.n);
+  // the first batch is loaded before the partial sums: the two latencies overlap
+  N3<T> Z[K2_NPT], P[K2_NPT];
+  long i0 = rg.lo + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < K2_NPT; ++j) {
+    const long i = i0 + (long)j * K2_BS;
+    if (i < rg.hi) {
+      Z[j] = ld3(z, i);
+      P[j] = ld3(pold, i);
+    }
+  }
   double beta;
   if (!pcg_beta<K2_BS>(A, k, red, beta)) return;
   const T tb = (T)beta;
-  const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
-  T* __restrict__ pnew = reinterpret_cast<T*>((k & 1) ? A.p1 : A.p0);
-  const T* __restrict__ pold = reinterpret_cast<const T*>((k & 1) ? A.p0 : A.p1);
-  const RowRange rg = block_rows(A.n);
-  for (long i = rg.lo + threadIdx.x; i < rg.hi; i += K2_BS)
+  for (;;) {
 #pragma unroll
-    for (int p = 0; p < 3; ++p) pnew[3 * i + p] = z[3 * i + p] + tb * pold[3 * i + p];
+    for (int j = 0; j < K2_NPT; ++j) {
+      const long i = i0 + (long)j * K2_BS;
+      if (i < rg.hi) {
+        N3<T> v;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) v.c[p] = Z[j].c[p] + tb * P[j].c[p];
+        st3(pnew, i, v);
+      }
+    }
+    i0 += (long)K2_NPT * K2_BS;
+    if (i0 >= rg.hi) break;
+#pragma unroll
+    for (int j = 0; j < K2_NPT; ++j) {
+      const long i = i0 + (long)j * K2_BS;
+      if (i < rg.hi) {
+        Z[j] = ld3(z, i);
+        P[j] = ld3(pold, i);
+      }
+    }
+  }
 }
 
 template <class T, int L>
@@ -504,37 +592,72 @@ __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
 
 // ---- K2: x += alpha p; r -= alpha q; z = M^-1 r; r.r, r.z ---------------------
 template <class T>
+struct K2Batch {
+  N3<T> P[K2_NPT], Q[K2_NPT], R[K2_NPT], X[K2_NPT];
+  PreM<T> M[K2_NPT];
+};
+
+template <class T>
+__device__ __forceinline__ void k2_load(const PcgArgs& A, const T* pk, long i0, long hi,
+                                        K2Batch<T>& b) {
+#pragma unroll
+  for (int j = 0; j < K2_NPT; ++j) {
+    const long i = i0 + (long)j * K2_BS;
+    if (i < hi) {
+      b.P[j] = ld3(pk, i);
+      b.Q[j] = ld3(reinterpret_cast<const T*>(A.q), i);
+      b.R[j] = ld3(reinterpret_cast<const T*>(A.r), i);
+      b.X[j] = ld3(reinterpret_cast<const T*>(A.x), i);
+      ld_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
+                    reinterpret_cast<const T*>(A.binv), i, b.M[j]);
+    }
+  }
+}
+
+template <class T>
 __global__ __launch_bounds__(K2_BS) void pcg_k2_kernel(PcgArgs A, int k) {
   __shared__ double red[3 * K2_BS / 64 + 4];
   PcgScalars* sc = A.sc;
   if (sc->done) return;
+  const T* pk = reinterpret_cast<const T*>((k & 1) ? A.p1 : A.p0);
+  T* x = reinterpret_cast<T*>(A.x);
+  T* r = reinterpret_cast<T*>(A.r);
+  T* z = reinterpret_cast<T*>(A.z);
+  const RowRange rg = block_rows(A.n);
+  // the first batch is loaded before the partial sums: the two latencies overlap
+  K2Batch<T> b;
+  long i0 = rg.lo + threadIdx.x;
+  k2_load<T>(A, pk, i0, rg.hi, b);
   double pq, rz, unused;
   all_sum3<K2_BS>(A.part1, A.g1, 1, rr_slot(A, k - 1) + 1, rr_count(A, k - 1), 2, A.part1, 0, 1,
                   red, pq, rz, unused);
   const T ta = (T)(rz / pq);
-  const T* __restrict__ pk = reinterpret_cast<const T*>((k & 1) ? A.p1 : A.p0);
-  const T* __restrict__ q = reinterpret_cast<const T*>(A.q);
-  T* __restrict__ x = reinterpret_cast<T*>(A.x);
-  T* __restrict__ r = reinterpret_cast<T*>(A.r);
-  T* __restrict__ z = reinterpret_cast<T*>(A.z);
   double rr = 0.0, rzn = 0.0;
-  const RowRange rg = block_rows(A.n);
-  for (long i = rg.lo + threadIdx.x; i < rg.hi; i += K2_BS) {
-    T ri[3], zi[3];
+  for (;;) {
 #pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      x[3 * i + p] += ta * pk[3 * i + p];
-      ri[p] = r[3 * i + p] - ta * q[3 * i + p];
-      r[3 * i + p] = ri[p];
-    }
-    apply_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
-                     reinterpret_cast<const T*>(A.binv), i, ri, zi);
+    for (int j = 0; j < K2_NPT; ++j) {
+      const long i = i0 + (long)j * K2_BS;
+      if (i < rg.hi) {
+        N3<T> xi, ri;
 #pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      z[3 * i + p] = zi[p];
-      rr += (double)ri[p] * (double)ri[p];
-      rzn += (double)ri[p] * (double)zi[p];
+        for (int p = 0; p < 3; ++p) {
+          xi.c[p] = b.X[j].c[p] + ta * b.P[j].c[p];
+          ri.c[p] = b.R[j].c[p] - ta * b.Q[j].c[p];
+        }
+        const N3<T> zi = mul_precond<T>(A.precond, b.M[j], ri);
+        st3(x, i, xi);
+        st3(r, i, ri);
+        st3(z, i, zi);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          rr += (double)ri.c[p] * (double)ri.c[p];
+          rzn += (double)ri.c[p] * (double)zi.c[p];
+        }
+      }
     }
+    i0 += (long)K2_NPT * K2_BS;
+    if (i0 >= rg.hi) break;
+    k2_load<T>(A, pk, i0, rg.hi, b);
   }
   block_sum2<K2_BS>(rr, rzn, red);
   if (threadIdx.x == 0) {
@@ -1042,31 +1165,52 @@ __global__ __launch_bounds__(SEG_K2_BS) void seg_k2_kernel(SegPcgArgs A, int k) 
   const int w = blockIdx.x, sg = A.wg_seg[w], j = A.wg_loc[w], G = A.seg_G[sg];
   if (A.sc[sg].done) return;
   const int g0 = A.seg_wg0[sg];
+  const T* pk = reinterpret_cast<const T*>((k & 1) ? A.p1 : A.p0);
+  const T* q = reinterpret_cast<const T*>(A.q);
+  T* x = reinterpret_cast<T*>(A.x);
+  T* r = reinterpret_cast<T*>(A.r);
+  T* z = reinterpret_cast<T*>(A.z);
+  const T* dinv = reinterpret_cast<const T*>(A.dinv);
+  const T* binv = reinterpret_cast<const T*>(A.binv);
+  const long r1 = A.seg_row1[sg], stride = (long)G * SEG_K2_BS;
+  long i = A.seg_row0[sg] + (long)j * SEG_K2_BS + threadIdx.x;
+  // operands of the first node are loaded before the partial sums
+  N3<T> P, Q, R, X;
+  PreM<T> M;
+  if (i < r1) {
+    P = ld3(pk, i);
+    Q = ld3(q, i);
+    R = ld3<T>(r, i);
+    X = ld3<T>(x, i);
+    ld_precond<T>(A.precond, dinv, binv, i, M);
+  }
   const double pq = all_sum<SEG_K2_BS>(A.part1 + g0, G, 1, red);
   const double rz = all_sum<SEG_K2_BS>(seg_rr_slot(A, k - 1) + 2L * g0 + 1, G, 2, red);
   const T ta = (T)(rz / pq);
-  const T* __restrict__ pk = reinterpret_cast<const T*>((k & 1) ? A.p1 : A.p0);
-  const T* __restrict__ q = reinterpret_cast<const T*>(A.q);
-  T* __restrict__ x = reinterpret_cast<T*>(A.x);
-  T* __restrict__ r = reinterpret_cast<T*>(A.r);
-  T* __restrict__ z = reinterpret_cast<T*>(A.z);
-  const long r0 = A.seg_row0[sg], r1 = A.seg_row1[sg];
   double rr = 0.0, rzn = 0.0;
-  for (long i = r0 + (long)j * SEG_K2_BS + threadIdx.x; i < r1; i += (long)G * SEG_K2_BS) {
-    T ri[3], zi[3];
+  while (i < r1) {
+    N3<T> xi, ri;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-      x[3 * i + p] += ta * pk[3 * i + p];
-      ri[p] = r[3 * i + p] - ta * q[3 * i + p];
-      r[3 * i + p] = ri[p];
+      xi.c[p] = X.c[p] + ta * P.c[p];
+      ri.c[p] = R.c[p] - ta * Q.c[p];
     }
-    apply_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
-                     reinterpret_cast<const T*>(A.binv), i, ri, zi);
+    const N3<T> zi = mul_precond<T>(A.precond, M, ri);
+    st3(x, i, xi);
+    st3(r, i, ri);
+    st3(z, i, zi);
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-      z[3 * i + p] = zi[p];
-      rr += (double)ri[p] * (double)ri[p];
-      rzn += (double)ri[p] * (double)zi[p];
+      rr += (double)ri.c[p] * (double)ri.c[p];
+      rzn += (double)ri.c[p] * (double)zi.c[p];
+    }
+    i += stride;
+    if (i < r1) {
+      P = ld3(pk, i);
+      Q = ld3(q, i);
+      R = ld3<T>(r, i);
+      X = ld3<T>(x, i);
+      ld_precond<T>(A.precond, dinv, binv, i, M);
     }
   }
   block_sum2<SEG_K2_BS>(rr, rzn, red);
