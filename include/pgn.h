@@ -116,12 +116,16 @@ int pgn_model_set_forces(pgn_model* model, const double* f_ini,
 int pgn_model_step_load(pgn_model* model, int* stepped);
 /* Simulable::BoundaryConditionsLoaded (Simulable.cpp:1476) */
 int pgn_model_is_loaded(const pgn_model* model);
-/* x := X0 and every load stage := 0 */
+/* x := X0, velocity := 0 and every load stage := 0 */
 int pgn_model_reset(pgn_model* model);
 
 /* ISimulable::SetDOFVector / GetDOFVector (Position_X) */
 int pgn_model_set_x(pgn_model* model, const double* x);
 int pgn_model_get_x(pgn_model* model, double* x);
+/* ISimulable::SetDOFVector / GetDOFVector (Tag::Velocity, Simulable.cpp:
+ * 150-170); a model whose velocity was never set is at rest */
+int pgn_model_set_velocity(pgn_model* model, const double* v);
+int pgn_model_get_velocity(pgn_model* model, double* v);
 
 /* ---- evaluation (IOptimProblem / ISimulable) ---------------------------- */
 /* OptimProblem_BasicStatic::GetEnergy -> Simulable::AddEnergy (:321) */
@@ -227,6 +231,18 @@ void pgn_newton_default_opts(pgn_newton_opts* o);
  * OptimSolver_USQP_LS::SolveStepInternal (OptimSolver_USQP_LS.cpp:18-68),
  * Newton steps (ComputeStep_Newton :468) and LineSearch_Simple (:688). */
 int pgn_newton_solve(pgn_model* model, const pgn_newton_opts* opts,
+                     pgn_newton_result* res);
+/* One implicit (backward-Euler) time step: OptimProblem_BasicDynamic
+ * (OptimProblem_BasicDynamic.cpp:20-170) minimised by SolveFull.  The step
+ * keeps x0 = x and v0 = velocity, steps the boundary conditions once
+ * (OnSolveStart :133-146), starts from x0 + dt v0 and minimises
+ *   E(x) + dt^2/2 a.M a,  a = (x - x0)/dt^2 - v0/dt   (GetEnergy :68-86)
+ * with gradient g + M a (:88-107) and Hessian H + M/dt^2 (:109-121), M the
+ * lumped masses; every iteration is fully constrained (:127-129).  After the
+ * solve the velocity is (x - x0)/dt (PosPerformStep :159-169).  The
+ * solver's previous x (warm start of the linear solve) carries over from one
+ * step to the next until pgn_model_reset. */
+int pgn_dynamic_step(pgn_model* model, double dt, const pgn_newton_opts* opts,
                      pgn_newton_result* res);
 
 /* ---- fused batches of independent solves -------------------------------- */

@@ -961,7 +961,7 @@ void orc_newton_generic(orc_problem* P, const orc_newton_opts* o,
   double** bufs[] = {&S.vx, &S.v0, &S.vg, &S.dx, &S.dxo, &S.vgNew, &S.xsave, &S.b, &S.xt};
   for (int i = 0; i < 9; ++i) *bufs[i] = (double*)calloc(N, sizeof(double));
   P->get_x(P->p, S.vx);
-  memcpy(S.v0, S.vx, sizeof(double) * N);
+  memcpy(S.v0, P->v0_keep ? P->v0_keep : S.vx, sizeof(double) * N);
   S.obj = P->energy(P->p);
   P->gradient(P->p, S.vg);
   S.opt = norm2(N, S.vg);
@@ -1003,6 +1003,7 @@ void orc_newton_generic(orc_problem* P, const orc_newton_opts* o,
   R->line_search_bisections = S.bis;
   R->objective = S.obj;
   R->optimality = S.opt;
+  if (P->v0_keep) memcpy(P->v0_keep, S.v0, sizeof(double) * N);
   for (int i = 0; i < 9; ++i) free(*bufs[i]);
 }
 
@@ -1037,10 +1038,141 @@ void orc_newton_solve(orc_model* M, const orc_newton_opts* o, orc_newton_result*
   T.rp = (long*)malloc(sizeof(long) * (3 * M->n + 1));
   T.ci = (int*)malloc(sizeof(int) * M->nnz);
   T.val = (double*)malloc(sizeof(double) * M->nnz);
-  orc_problem P = {&T,       3 * M->n,  tp_energy, tp_escale, tp_gradient, tp_solve,
-                   tp_step,  tp_loaded, tp_get,    tp_set,    tp_finish};
+  orc_problem P = {&T,      3 * M->n,  tp_energy, tp_escale, tp_gradient, tp_solve,
+                   tp_step, tp_loaded, tp_get,    tp_set,    tp_finish, NULL};
   orc_newton_generic(&P, o, R);
   free(T.rp);
   free(T.ci);
   free(T.val);
+}
+
+/* ======================================================================= */
+/* Implicit dynamics (OptimProblem_BasicDynamic.cpp:20-170)                 */
+/* ======================================================================= */
+struct orc_dynamic {
+  orc_model* M;
+  int N;
+  double dt;
+  double *v, *x0, *v0, *mass, *v0_keep;
+  int v0_valid;
+  long* rp;
+  int* ci;
+  double* val;
+};
+
+orc_dynamic* orc_dynamic_create(orc_model* M) {
+  orc_dynamic* D = (orc_dynamic*)calloc(1, sizeof(orc_dynamic));
+  D->M = M;
+  D->N = 3 * M->n;
+  D->v = (double*)calloc(D->N, sizeof(double));
+  D->x0 = (double*)calloc(D->N, sizeof(double));
+  D->v0 = (double*)calloc(D->N, sizeof(double));
+  D->mass = (double*)calloc(D->N, sizeof(double));
+  D->v0_keep = (double*)calloc(D->N, sizeof(double));
+  D->rp = (long*)malloc(sizeof(long) * (D->N + 1));
+  D->ci = (int*)malloc(sizeof(int) * M->nnz);
+  D->val = (double*)malloc(sizeof(double) * M->nnz);
+  orc_model_get_mass(M, D->mass); /* GetInertia: lumped, per DoF */
+  return D;
+}
+
+void orc_dynamic_destroy(orc_dynamic* D) {
+  if (!D) return;
+  free(D->v);
+  free(D->x0);
+  free(D->v0);
+  free(D->mass);
+  free(D->v0_keep);
+  free(D->rp);
+  free(D->ci);
+  free(D->val);
+  free(D);
+}
+
+void orc_dynamic_set_velocity(orc_dynamic* D, const double* v) {
+  memcpy(D->v, v, sizeof(double) * D->N);
+}
+
+void orc_dynamic_get_velocity(const orc_dynamic* D, double* v) {
+  memcpy(v, D->v, sizeof(double) * D->N);
+}
+
+void orc_dynamic_begin(orc_dynamic* D, double dt) {
+  D->dt = dt;
+  if (!D->v0_valid) { /* OptimSolver::Init: m_v0 = x */
+    memcpy(D->v0_keep, D->M->x, sizeof(double) * D->N);
+    D->v0_valid = 1;
+  }
+  memcpy(D->x0, D->M->x, sizeof(double) * D->N);
+  memcpy(D->v0, D->v, sizeof(double) * D->N);
+  orc_model_step_load(D->M);
+  for (int k = 0; k < D->N; ++k) D->M->x[k] = D->x0[k] + dt * D->v0[k];
+}
+
+/* va = (x - x0) / dt^2 - v0 / dt */
+static double dyn_va(const orc_dynamic* D, int k) {
+  const double dt = D->dt;
+  return (D->M->x[k] - D->x0[k]) / (dt * dt) - D->v0[k] / dt;
+}
+
+double orc_dynamic_energy(orc_dynamic* D) {
+  double e = orc_model_energy(D->M);
+  double q = 0.0;
+  for (int k = 0; k < D->N; ++k) {
+    const double a = dyn_va(D, k);
+    q += a * D->mass[k] * a;
+  }
+  return e + D->dt * D->dt / 2.0 * q;
+}
+
+void orc_dynamic_gradient(orc_dynamic* D, double* g) {
+  orc_model_gradient(D->M, g);
+  for (int k = 0; k < D->N; ++k) g[k] += D->mass[k] * dyn_va(D, k); /* + DMvDt = 0 */
+  for (int k = 0; k < D->N; ++k)
+    if (D->M->fixed[k]) g[k] = 0.0; /* FixVectorDoFs */
+}
+
+void orc_dynamic_hessian_lower(orc_dynamic* D, long* rp, int* ci, double* val) {
+  orc_model_hessian_lower(D->M, rp, ci, val);
+  const double idt2 = 1.0 / (D->dt * D->dt);
+  for (int r = 0; r < D->N; ++r)
+    for (long p = rp[r]; p < rp[r + 1]; ++p)
+      if (ci[p] == r) val[p] += D->mass[r] * idt2;
+  for (int r = 0; r < D->N; ++r) /* FixMatrixDoFs */
+    for (long p = rp[r]; p < rp[r + 1]; ++p)
+      if (D->M->fixed[r] || D->M->fixed[ci[p]]) val[p] = (r == ci[p]) ? 1.0 : 0.0;
+}
+
+static double dp_energy(void* p) { return orc_dynamic_energy((orc_dynamic*)p); }
+static double dp_escale(void* p) { return orc_model_energy_scale(((orc_dynamic*)p)->M); }
+static void dp_gradient(void* p, double* g) { orc_dynamic_gradient((orc_dynamic*)p, g); }
+static int dp_solve(void* p, const double* b, double* dx, const orc_newton_opts* o) {
+  orc_dynamic* D = (orc_dynamic*)p;
+  orc_dynamic_hessian_lower(D, D->rp, D->ci, D->val);
+  int its = 0, reg = 0;
+  orc_linear_solve(D->N, D->rp, D->ci, D->val, b, dx, o->lsTolMaxError, o->lsNumMaxIters,
+                   o->lsNumRegTrials, &its, &reg);
+  return its;
+}
+static int dp_step(void* p) { (void)p; return 0; }   /* OnIterStart: no load step */
+static int dp_loaded(void* p) { (void)p; return 1; } /* IsFullyConstrained */
+static void dp_get(void* p, double* x) { orc_model_get_x(((orc_dynamic*)p)->M, x); }
+static void dp_set(void* p, const double* x) { orc_model_set_x(((orc_dynamic*)p)->M, x); }
+static void dp_finish(void* p) { (void)p; }
+
+void orc_dynamic_solve(orc_dynamic* D, const orc_newton_opts* o, orc_newton_result* R) {
+  orc_problem P = {D,       D->N,      dp_energy, dp_escale, dp_gradient, dp_solve,
+                   dp_step, dp_loaded, dp_get,    dp_set,    dp_finish,   D->v0_keep};
+  orc_newton_generic(&P, o, R);
+}
+
+void orc_dynamic_end(orc_dynamic* D) {
+  for (int k = 0; k < D->N; ++k) D->v[k] = (D->M->x[k] - D->x0[k]) / D->dt;
+}
+
+void orc_dynamic_step(orc_dynamic* D, double dt, const orc_newton_opts* o,
+                      orc_newton_result* R) {
+  orc_dynamic_begin(D, dt);
+  orc_dynamic_solve(D, o, R);
+  orc_dynamic_end(D);
 }

@@ -176,12 +176,39 @@ typedef struct {
   void (*get_x)(void*, double*);
   void (*set_x)(void*, const double*);
   void (*iter_finish)(void*);
+  /* optional (may be NULL): the solver's previous x kept across SolveFull
+   * calls of one solver object (time steps); used as the initial v0 and
+   * written back at the end */
+  double* v0_keep;
 } orc_problem;
 void orc_newton_generic(orc_problem* P, const orc_newton_opts* o,
                         orc_newton_result* r);
 /* OptimSolver::Init + SolveFull (OptimSolver.cpp:36-55,163-264) with
  * OptimSolver_USQP_LS::SolveStepInternal and OptimProblem_BasicStatic. */
 void orc_newton_solve(orc_model* m, const orc_newton_opts* o,
+                      orc_newton_result* r);
+
+/* Implicit dynamics: OptimProblem_BasicDynamic (OptimProblem_BasicDynamic.cpp:
+ * 20-170) over an orc_model, one backward-Euler step per orc_dynamic_step. */
+typedef struct orc_dynamic orc_dynamic;
+orc_dynamic* orc_dynamic_create(orc_model* m);
+void orc_dynamic_destroy(orc_dynamic* d);
+void orc_dynamic_set_velocity(orc_dynamic* d, const double* v);
+void orc_dynamic_get_velocity(const orc_dynamic* d, double* v);
+/* OnSolveStart (:133-146): x0 = x, v0 = v, loads stepped, x = x0 + dt v0 */
+void orc_dynamic_begin(orc_dynamic* d, double dt);
+/* the step's objective at the current x: GetEnergy (:68-86), GetGradient
+ * (:88-107, fixed DoFs zero), GetHessian (:109-121, lower CSR of the model's
+ * pattern, fixed rows / columns identity) */
+double orc_dynamic_energy(orc_dynamic* d);
+void orc_dynamic_gradient(orc_dynamic* d, double* g);
+void orc_dynamic_hessian_lower(orc_dynamic* d, long* rowptr, int* col, double* val);
+/* SolveFull on the step's objective (IsFullyConstrained :127-129) */
+void orc_dynamic_solve(orc_dynamic* d, const orc_newton_opts* o, orc_newton_result* r);
+/* PosPerformStep (:159-169): v = (x - x0) / dt */
+void orc_dynamic_end(orc_dynamic* d);
+/* begin + solve + end */
+void orc_dynamic_step(orc_dynamic* d, double dt, const orc_newton_opts* o,
                       orc_newton_result* r);
 
 /* ---- thin shells (shells.c) -------------------------------------------- */

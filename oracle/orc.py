@@ -139,6 +139,21 @@ def lib():
         L.orc_rod_update_kinematics.argtypes = [ctypes.c_void_p]
         L.orc_rod_newton.argtypes = [ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                      ctypes.POINTER(NewtonResult)]
+        L.orc_dynamic_create.restype = ctypes.c_void_p
+        L.orc_dynamic_create.argtypes = [ctypes.c_void_p]
+        L.orc_dynamic_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_dynamic_set_velocity.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_dynamic_get_velocity.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_dynamic_begin.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_dynamic_energy.restype = ctypes.c_double
+        L.orc_dynamic_energy.argtypes = [ctypes.c_void_p]
+        L.orc_dynamic_gradient.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_dynamic_hessian_lower.argtypes = [ctypes.c_void_p, _lp, _ip, _dp]
+        L.orc_dynamic_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
+                                        ctypes.POINTER(NewtonResult)]
+        L.orc_dynamic_end.argtypes = [ctypes.c_void_p]
+        L.orc_dynamic_step.argtypes = [ctypes.c_void_p, ctypes.c_double,
+                                       ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonResult)]
         L.orc_dense_ldlt_solve.restype = ctypes.c_int
         L.orc_dense_ldlt_solve.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
                                            ctypes.c_int, _ip]
@@ -312,6 +327,71 @@ class Model:
                 "cg_iterations": r.cg_iterations,
                 "bisections": r.line_search_bisections,
                 "objective": r.objective, "optimality": r.optimality}
+
+
+class Dynamic:
+    """OptimProblem_BasicDynamic over a Model: one backward-Euler time step
+    per step(); keeps the velocity and the solver's previous x."""
+
+    def __init__(self, model):
+        self.model = model  # keeps the model alive
+        self.N = model.N
+        self._h = lib().orc_dynamic_create(model._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_dynamic_destroy(h)
+            self._h = None
+
+    def set_velocity(self, v):
+        v = f64(v).reshape(-1)
+        lib().orc_dynamic_set_velocity(self._h, _p(v))
+
+    def get_velocity(self):
+        v = np.zeros(self.N)
+        lib().orc_dynamic_get_velocity(self._h, _p(v))
+        return v
+
+    def begin(self, dt):
+        lib().orc_dynamic_begin(self._h, float(dt))
+
+    def energy(self):
+        return lib().orc_dynamic_energy(self._h)
+
+    def gradient(self):
+        g = np.zeros(self.N)
+        lib().orc_dynamic_gradient(self._h, _p(g))
+        return g
+
+    def hessian_lower(self):
+        nnz = lib().orc_model_nnz_lower(self.model._h)
+        rp = np.zeros(self.N + 1, dtype=np.int64)
+        ci = np.zeros(nnz, dtype=np.int32)
+        v = np.zeros(nnz)
+        lib().orc_dynamic_hessian_lower(self._h, _p(rp, _lp), _p(ci, _ip), _p(v))
+        return rp, ci, v
+
+    def solve(self, **kw):
+        o = NewtonOpts()
+        lib().orc_newton_default_opts(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        r = NewtonResult()
+        lib().orc_dynamic_solve(self._h, ctypes.byref(o), ctypes.byref(r))
+        return {"result": OS_NAMES[r.result], "iterations": r.iterations,
+                "cg_iterations": r.cg_iterations,
+                "bisections": r.line_search_bisections,
+                "objective": r.objective, "optimality": r.optimality}
+
+    def end(self):
+        lib().orc_dynamic_end(self._h)
+
+    def step(self, dt, **kw):
+        self.begin(dt)
+        r = self.solve(**kw)
+        self.end()
+        return r
 
 
 def cg(rowptr, col, val, b, x0, tol, maxit):

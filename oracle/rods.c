@@ -636,7 +636,7 @@ static void rp_set(void* p, const double* x) { orc_rod_set_x((orc_rod*)p, x); }
 static void rp_finish(void* p) { orc_rod_update_kinematics((orc_rod*)p); }
 
 void orc_rod_newton(orc_rod* R, const orc_newton_opts* o, orc_newton_result* r) {
-  orc_problem P = {R,           R->N,     rp_energy, rp_escale, rp_gradient, rp_solve,
-                   rp_step,     rp_loaded, rp_get,    rp_set,    rp_finish};
+  orc_problem P = {R,       R->N,      rp_energy, rp_escale, rp_gradient, rp_solve,
+                   rp_step, rp_loaded, rp_get,    rp_set,    rp_finish, NULL};
   orc_newton_generic(&P, o, r);
 }

@@ -31,7 +31,8 @@ EXPORTS = [
     "pgn_model_num_dofs", "pgn_model_num_elements", "pgn_model_device_bytes",
     "pgn_model_set_fixed", "pgn_model_set_gravity", "pgn_model_set_forces",
     "pgn_model_step_load", "pgn_model_is_loaded", "pgn_model_reset",
-    "pgn_model_set_x", "pgn_model_get_x", "pgn_model_get_energy",
+    "pgn_model_set_x", "pgn_model_get_x", "pgn_model_set_velocity",
+    "pgn_model_get_velocity", "pgn_dynamic_step", "pgn_model_get_energy",
     "pgn_model_get_gradient", "pgn_model_hessian_nnz",
     "pgn_model_hessian_pattern", "pgn_model_get_hessian",
     "pgn_model_element_egh", "pgn_linear_default_opts", "pgn_linear_solve",
@@ -134,6 +135,8 @@ def lib():
         L.pgn_model_reset.argtypes = [_vp]
         L.pgn_model_set_x.argtypes = [_vp, _dp]
         L.pgn_model_get_x.argtypes = [_vp, _dp]
+        L.pgn_model_set_velocity.argtypes = [_vp, _dp]
+        L.pgn_model_get_velocity.argtypes = [_vp, _dp]
         L.pgn_model_get_energy.argtypes = [_vp, _dp]
         L.pgn_model_get_gradient.argtypes = [_vp, _dp]
         L.pgn_model_hessian_nnz.argtypes = [_vp, ctypes.c_int, _lp]
@@ -147,6 +150,8 @@ def lib():
         L.pgn_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
         L.pgn_newton_default_opts.restype = None
         L.pgn_newton_solve.argtypes = [_vp, ctypes.POINTER(NewtonOpts),
+                                       ctypes.POINTER(NewtonResult)]
+        L.pgn_dynamic_step.argtypes = [_vp, ctypes.c_double, ctypes.POINTER(NewtonOpts),
                                        ctypes.POINTER(NewtonResult)]
         L.pgn_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
         L.pgn_kernel_bytes.argtypes = [_vp, ctypes.c_int, _dp]
@@ -334,6 +339,15 @@ class TetModel:
         _ok(lib().pgn_model_get_x(self._h, _p(x)))
         return x
 
+    def set_velocity(self, v):
+        v = _f64(v).reshape(-1)
+        _ok(lib().pgn_model_set_velocity(self._h, _p(v)))
+
+    def get_velocity(self):
+        v = np.zeros(self.N)
+        _ok(lib().pgn_model_get_velocity(self._h, _p(v)))
+        return v
+
     def energy(self):
         e = ctypes.c_double()
         _ok(lib().pgn_model_get_energy(self._h, ctypes.byref(e)))
@@ -384,6 +398,22 @@ class TetModel:
             setattr(o, k, v)
         r = NewtonResult()
         _ok(lib().pgn_newton_solve(self._h, ctypes.byref(o), ctypes.byref(r)))
+        return {"result": OS_NAMES[r.result], "iterations": r.iterations,
+                "pcg_iterations": r.pcg_iterations,
+                "bisections": r.line_search_bisections,
+                "energy_evals": r.energy_evals, "hessian_evals": r.hessian_evals,
+                "objective": r.objective, "optimality": r.optimality,
+                "seconds": r.seconds}
+
+    def dynamic_step(self, dt, **kw):
+        """One backward-Euler time step of length dt (OptimProblem_BasicDynamic
+        minimised by SolveFull); updates x and the velocity."""
+        o = NewtonOpts()
+        lib().pgn_newton_default_opts(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        r = NewtonResult()
+        _ok(lib().pgn_dynamic_step(self._h, float(dt), ctypes.byref(o), ctypes.byref(r)))
         return {"result": OS_NAMES[r.result], "iterations": r.iterations,
                 "pcg_iterations": r.pcg_iterations,
                 "bisections": r.line_search_bisections,

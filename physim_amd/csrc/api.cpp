@@ -291,10 +291,13 @@ int pgn_model_reset(pgn_model* M) {
     use_device(M->ctx);
     PGN_HIP(hipMemcpyAsync(M->x.p, M->X0.p, M->X0.bytes(), hipMemcpyDeviceToDevice,
                            M->stream()));
+    if (M->vel.p)
+      PGN_HIP(hipMemsetAsync(M->vel.p, 0, M->vel.bytes(), M->stream()));
     M->sync();
     M->g_cur = 0;
     M->f_cur = 0;
     M->hess_valid = false;
+    M->dyn_v0_valid = false;
   });
 }
 
@@ -314,6 +317,29 @@ int pgn_model_get_x(pgn_model* M, double* x) {
     use_device(M->ctx);
     M->x.download(x, 3L * M->n, M->stream());
     M->sync();
+  });
+}
+
+int pgn_model_set_velocity(pgn_model* M, const double* v) {
+  return guard([&] {
+    PGN_CHECK(M && v, "null argument");
+    use_device(M->ctx);
+    M->vel.upload(v, 3L * M->n, M->stream());
+    M->sync();
+  });
+}
+
+int pgn_model_get_velocity(pgn_model* M, double* v) {
+  return guard([&] {
+    PGN_CHECK(M && v, "null argument");
+    use_device(M->ctx);
+    const long N = 3L * M->n;
+    if (M->vel.n == (size_t)N) {
+      M->vel.download(v, N, M->stream());
+      M->sync();
+    } else {
+      std::fill(v, v + N, 0.0);  // never set: at rest
+    }
   });
 }
 
@@ -503,6 +529,19 @@ int pgn_newton_solve(pgn_model* M, const pgn_newton_opts* opts, pgn_newton_resul
     pgn_newton_default_opts(&o);
     if (opts) o = *opts;
     newton_solve(*M, o, *res);
+  });
+}
+
+int pgn_dynamic_step(pgn_model* M, double dt, const pgn_newton_opts* opts,
+                     pgn_newton_result* res) {
+  return guard([&] {
+    PGN_CHECK(M && res, "null argument");
+    PGN_CHECK(dt > 0.0 && std::isfinite(dt), "time step must be positive and finite");
+    use_device(M->ctx);
+    pgn_newton_opts o;
+    pgn_newton_default_opts(&o);
+    if (opts) o = *opts;
+    dynamic_step(*M, dt, o, *res);
   });
 }
 

@@ -59,6 +59,16 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_gather_kernel(
         eN += xi[k] * f;
       }
     }
+    if (A.xhat) {
+      // OptimProblem_BasicDynamic.cpp:68-107: with a = (x - xhat) / dt^2,
+      // E += dt^2/2 a.M a and g += M a, before the fixed DoFs are zeroed
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double a = (xi[k] - A.xhat[3L * i + k]) * A.inv_dt2;
+        gr[k] += M * a;
+        eN += 0.5 / A.inv_dt2 * a * (M * a);
+      }
+    }
     const unsigned fm = A.fixmask[i];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -157,6 +167,13 @@ __global__ __launch_bounds__(256) void bsr_gather_kernel(BsrArgs A,
     }
   }
   const int bi = A.brow[s], bj = A.bcol[s];
+  if (A.mass && bi == bj) {
+    // OptimProblem_BasicDynamic.cpp:115-119: H += M / dt^2
+    const T md = (T)(A.mass[bi] * A.inv_dt2);
+    acc[0] += md;
+    acc[4] += md;
+    acc[8] += md;
+  }
   const unsigned fi = A.fixmask[bi], fj = A.fixmask[bj];
   if (fi | fj) {
 #pragma unroll

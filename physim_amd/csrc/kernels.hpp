@@ -74,6 +74,10 @@ struct NodeArgs {
   const double* falpha_inst = nullptr;
   double* nodeE = nullptr;
   double* nodeG2 = nullptr;
+  // implicit dynamics: inertia term m/(2 dt^2) |x_i - xhat_i|^2 of one
+  // backward-Euler step toward the predictor xhat (inv_dt2 = 1/dt^2)
+  const double* xhat = nullptr;
+  double inv_dt2 = 0.0;
 };
 void launch_node_gather(hipStream_t s, bool fp32_eg, const NodeArgs& a,
                         const void* eg, double* g, double* node_partials);
@@ -94,6 +98,9 @@ struct BsrArgs {
   const long* cptr;      // nb+1 contribution ranges
   const uint32_t* code;  // element*16 + 4*a + b
   const uint8_t* fixmask;
+  // implicit dynamics: m_i / dt^2 added to the diagonal of block (i, i)
+  const double* mass = nullptr;
+  double inv_dt2 = 0.0;
 };
 void launch_bsr_gather(hipStream_t s, bool fp32, const BsrArgs& a,
                        const void* eH, void* bval);

@@ -402,6 +402,10 @@ NodeArgs pgn_model::node_args() const {
   a.f_ini = has_f ? f_ini.p : nullptr;
   a.f_end = has_f ? f_end.p : nullptr;
   a.f_alpha = cur_force_alpha();
+  if (dyn_dt > 0.0) {
+    a.xhat = xhat.p;
+    a.inv_dt2 = 1.0 / (dyn_dt * dyn_dt);
+  }
   return a;
 }
 
@@ -413,6 +417,10 @@ BsrArgs pgn_model::bsr_args() const {
   a.cptr = cptr.p;
   a.code = code.p;
   a.fixmask = fixmask.p;
+  if (dyn_dt > 0.0) {
+    a.mass = mass.p;
+    a.inv_dt2 = 1.0 / (dyn_dt * dyn_dt);
+  }
   return a;
 }
 

@@ -83,6 +83,12 @@ struct pgn_model {
   pgn::DBuf<pgn::PcgScalars> sc;
   pgn::DBuf<double> r64, tmp64a, tmp64b;
   pgn::DBuf<double> nw_xsave, nw_v0, nw_dx, nw_b;  // Newton work vectors
+  // implicit dynamics (OptimProblem_BasicDynamic): while dyn_dt > 0 the
+  // objective carries the inertia term of one backward-Euler step of length
+  // dyn_dt toward the predictor xhat
+  double dyn_dt = 0.0;
+  pgn::DBuf<double> vel, xhat, xstep0;  // 3n: velocity, predictor, x at step start
+  bool dyn_v0_valid = false;  // nw_v0 carries the solver's previous x across steps
   double* hpin = nullptr;  // pinned host scalars
   pgn::PcgScalars* hsc = nullptr;
   bool hess_valid = false;

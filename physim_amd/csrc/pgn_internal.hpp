@@ -59,6 +59,10 @@ struct DBuf {
     }
     n = count;
   }
+  // keeps the allocation (and its contents) when the size already matches
+  void ensure(size_t count) {
+    if (count != n || (count && !p)) alloc(count);
+  }
   size_t bytes() const { return n * sizeof(T); }
   void upload(const T* h, size_t count, hipStream_t s) {
     if (count != n) alloc(count);

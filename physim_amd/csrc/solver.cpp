@@ -153,14 +153,14 @@ void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opt
 
 // ---- Newton -----------------------------------------------------------------
 
-void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) {
+void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, bool dynamic) {
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = M.stream();
   const long N = 3L * M.n;
-  M.nw_xsave.alloc(N);
-  M.nw_v0.alloc(N);
-  M.nw_dx.alloc(N);
-  M.nw_b.alloc(N);
+  M.nw_xsave.ensure(N);
+  M.nw_v0.ensure(N);
+  M.nw_dx.ensure(N);
+  M.nw_b.ensure(N);
   double* x = M.x.p;
   double* xsave = M.nw_xsave.p;
   double* v0 = M.nw_v0.p;
@@ -180,8 +180,10 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
 
   // PGN_TRACE=1 prints one line per Newton step (diagnostics only)
   const bool trace = std::getenv("PGN_TRACE") != nullptr;
-  // OptimSolver::Init (OptimSolver.cpp:36-55)
-  d2d(v0, x);
+  // OptimSolver::Init (OptimSolver.cpp:36-55); a time step keeps the previous
+  // x of the solver's last step instead (the solver is initialised once and
+  // SolveFull runs per time step)
+  if (!dynamic) d2d(v0, x);
   double obj = 0.0, gg = 0.0;
   M.eval_eg(obj, gg);
   R.energy_evals++;
@@ -197,7 +199,7 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
 
   // OptimSolver_USQP_LS::SolveStepInternal (OptimSolver_USQP_LS.cpp:18-68)
   auto solve_step = [&](bool use_steepest) {
-    M.step_load();                 // OnIterStart: StepBoundaryConditions
+    if (!dynamic) M.step_load();   // OnIterStart: StepBoundaryConditions (static)
     d2d(xsave, x);                 //              GetState
     M.eval_eg(obj, gg);            // GetEnergy + GetGradient
     R.energy_evals++;
@@ -301,7 +303,9 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
       g_valid = true;
     }
     opt = std::sqrt(gg);
-    if (M.is_loaded()) {
+    // IsFullyConstrained: the dynamic problem always is
+    // (OptimProblem_BasicDynamic.cpp:127-129)
+    if (dynamic || M.is_loaded()) {
       if (result == PGN_OS_NONDESC) {
         if (!o.trySteepest) break;
         if (steepest) break;
@@ -332,6 +336,45 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R) 
   R.objective = obj;
   R.optimality = opt;
   R.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ---- implicit dynamics ---------------------------------------------------------
+// One backward-Euler time step: OptimProblem_BasicDynamic
+// (OptimProblem_BasicDynamic.cpp:20-170) minimised by SolveFull.
+void dynamic_step(pgn_model& M, double dt, const pgn_newton_opts& o, pgn_newton_result& R) {
+  hipStream_t s = M.stream();
+  const long N = 3L * M.n;
+  if (M.vel.n != (size_t)N) {
+    M.vel.alloc(N);
+    PGN_HIP(hipMemsetAsync(M.vel.p, 0, N * sizeof(double), s));
+  }
+  M.xhat.ensure(N);
+  M.xstep0.ensure(N);
+  M.nw_v0.ensure(N);
+  if (!M.dyn_v0_valid) {  // OptimSolver::Init: m_v0 = x
+    PGN_HIP(hipMemcpyAsync(M.nw_v0.p, M.x.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    M.dyn_v0_valid = true;
+  }
+  // OnSolveStart (:133-146): x0, v0 kept, boundary conditions stepped, x set
+  // to the predictor x0 + dt v0
+  PGN_HIP(hipMemcpyAsync(M.xstep0.p, M.x.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  M.step_load();
+  launch_lincomb64(s, N, 1.0, M.xstep0.p, dt, M.vel.p, M.xhat.p);
+  PGN_HIP(hipMemcpyAsync(M.x.p, M.xhat.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  M.dyn_dt = dt;
+  M.hess_valid = false;
+  try {
+    newton_solve(M, o, R, true);
+  } catch (...) {
+    M.dyn_dt = 0.0;
+    M.hess_valid = false;
+    throw;
+  }
+  M.dyn_dt = 0.0;
+  M.hess_valid = false;
+  // PosPerformStep (:159-169): v = (x - x0) / dt
+  launch_lincomb64(s, N, 1.0 / dt, M.x.p, -1.0 / dt, M.xstep0.p, M.vel.p);
+  M.sync();
 }
 
 }  // namespace pgn
