@@ -126,12 +126,13 @@ def main():
                          "PCG solve capped at 50 iterations, 5 E+g+H launches and 5 "
                          "calibration copies; no solve timing, no JSON line")
     ap.add_argument("--workload", default="beam",
-                    choices=["beam", "batch", "rods", "serving"],
+                    choices=["beam", "batch", "rods", "serving", "dynamic"],
                     help="beam (default): the 1.28M-tet beam; batch: 256 fused beams; "
-                         "rods: 4096 helices; serving: Poisson stream of beams and cloths")
+                         "rods: 4096 helices; serving: Poisson stream of beams and cloths; "
+                         "dynamic: implicit time steps of the 1.28M-tet beam")
     ap.add_argument("--count", type=int, default=0,
-                    help="instances / rods / requests of the batch, rods and serving "
-                         "workloads (0 = the workload's default)")
+                    help="instances / rods / requests / time steps of the batch, rods, "
+                         "serving and dynamic workloads (0 = the workload's default)")
     ap.add_argument("--rate", type=float, default=100.0,
                     help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--max-tets", type=int, default=500_000,

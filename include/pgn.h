@@ -193,8 +193,9 @@ typedef struct {
   double inner_tol;   /* MIXED: FP32 inner tolerance per refinement sweep */
   int adaptive_tol;   /* 0: every linear solve to lsTolMaxError (reference).
                          1: Eisenstat-Walker forcing (choice 2, gamma
-                         ew_gamma, alpha 2, capped at ew_max, floored at
-                         lsTolMaxError)                                      */
+                         ew_gamma, alpha 2) with Kelley's safeguard
+                         (at least 0.5 tolMaxError / |g|), capped at
+                         ew_max, floored at lsTolMaxError                   */
   double ls_noise_guard; /* 0 = reference line search.  > 0: also accept a
                          step whose energy rises by at most ls_noise_guard *
                          (|E_elastic| + |E_load|), i.e. within FP64 rounding

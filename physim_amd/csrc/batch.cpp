@@ -484,8 +484,10 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
             double e = o.ew_gamma * r * r;
             const double guard = o.ew_gamma * eta[k] * eta[k];
             if (guard > 0.1) e = std::max(e, guard);
-            eta[k] = std::min(o.ew_max, e);
+            eta[k] = e;
           }
+          // Kelley's safeguard: no tighter than the absolute tolerance needs
+          eta[k] = std::min(o.ew_max, std::max(eta[k], 0.5 * o.tolMaxError / opt[k]));
           gprev[k] = opt[k];
           lt = std::max(o.lsTolMaxError, eta[k]);
         }

@@ -221,8 +221,13 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
           double e = o.ew_gamma * r * r;
           const double guard = o.ew_gamma * eta * eta;
           if (guard > 0.1) e = std::max(e, guard);
-          eta = std::min(o.ew_max, e);
+          eta = e;
         }
+        // Kelley's safeguard (Iterative Methods for Linear and Nonlinear
+        // Equations, SIAM 1995, sec. 6.3): a step whose linear residual is
+        // 0.5 tolMaxError already meets the Newton tolerance, so no linear
+        // solve is taken tighter than that
+        eta = std::min(o.ew_max, std::max(eta, 0.5 * o.tolMaxError / opt));
         lo.tol = std::max(o.lsTolMaxError, eta);
         g_prev = opt;
       }

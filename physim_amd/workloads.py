@@ -13,6 +13,12 @@ Workloads (all synthetic, seeded):
              3.14e-6 kg/m (the configuration leaves it open; heavier rods
              leave more of these very flexible helices unconverged within the
              300-iteration cap of the reference's Newton).
+  dynamic -- the 1.28M-tet beam (the quasi-static workload's generator)
+             released from rest and integrated by implicit (backward-Euler)
+             steps of 1/60 s, gravity ramped over the first 10 steps; one
+             step = one Newton solve of the BasicDynamic objective.  The plain
+             FP64 path (Jacobi PCG at the reference's tolerances) runs the same
+             steps for the speed-up and the displacement check.
   serving -- Poisson arrivals of a seeded 60/40 mix of beams (the 1.28M-tet
              beam's generator at 2k..max tets) and cloths (n U{32..max}, 1 m
              sheets hanging from the two corners of their top edge, 0.2 kg/m^2),
@@ -144,6 +150,61 @@ def rods(args, log):
                     for o in sorted(set(r["result"] for r in res))},
         "newton_iterations": _stats([r["iterations"] for r in res]),
     }
+
+
+def dynamic(args, log):
+    count = args.count or 20
+    dt = 1.0 / 60.0
+    t0 = time.perf_counter()
+    sc = scenes.config2()
+    ctx = pg.Context(0)
+    M = pg.TetModel(ctx, sc.X0, sc.T, sc.material, sc.lame1, sc.lame2, sc.density, pg.MIXED)
+    sc.apply(M)
+    log(f"dynamic setup {time.perf_counter() - t0:.1f}s: {sc.m} tets, {3 * sc.n} DoF")
+    opts = dict(SOLVE_OPTS)
+    if args.warmup:
+        M.dynamic_step(dt, **opts)
+        M.reset()
+    ctx.synchronize()
+    res = []
+    ts = time.perf_counter()
+    for k in range(count):
+        res.append(M.dynamic_step(dt, **opts))
+    el = time.perf_counter() - ts
+    x_fast = M.get_x()
+    log(f"dynamic: {count} steps in {el:.2f}s, newton {[r['iterations'] for r in res]}, "
+        f"pcg {[r['pcg_iterations'] for r in res]}")
+    out = {
+        "value": count / el, "unit": "implicit time steps/s (Newton solves/s)",
+        "ms_per_step": el / count * 1e3, "steps": count, "warmup": 1 if args.warmup else 0,
+        "scaling": "weak", "dtype": "f32 Hessian/PCG, f64 energy/gradient/residual",
+        "config": {"workload": f"dynamic_beam_40x40x160: {sc.m} tets, {3 * sc.n} DoF, "
+                               f"backward Euler dt 1/60 s, gravity over 10 steps",
+                   "precision": "mixed", "precond": "block-jacobi",
+                   "linear_tol": "eisenstat-walker"},
+        "converged": sum(r["result"] == "SUCCESS" for r in res),
+        "newton_iterations": _stats([r["iterations"] for r in res]),
+        "pcg_iterations_total": sum(r["pcg_iterations"] for r in res),
+    }
+    if not args.no_fp64:
+        F = pg.TetModel(ctx, sc.X0, sc.T, sc.material, sc.lame1, sc.lame2, sc.density,
+                        pg.FP64)
+        sc.apply(F)
+        ctx.synchronize()
+        ts = time.perf_counter()
+        rf = [F.dynamic_step(dt, **dict(pg.CONVERGE_OPTS, precond=0)) for _ in range(count)]
+        tf = time.perf_counter() - ts
+        err = float(np.abs(F.get_x() - x_fast).max())
+        out.update(fp64_step_s=tf / count,
+                   fp64_converged=sum(r["result"] == "SUCCESS" for r in rf),
+                   fp64_pcg_iterations_total=sum(r["pcg_iterations"] for r in rf),
+                   speedup_vs_fp64=tf / el, max_disp_diff_vs_fp64_m=err,
+                   matches_fp64=bool(err <= 1e-6 * sc.bbox_diag()))
+        log(f"dynamic fp64 plain path: {tf:.2f}s, |dx| {err:.2e}")
+        F.close()
+    M.close()
+    ctx.close()
+    return out
 
 
 def _serving_mix(count, max_tets, max_cloth, seed=5):
