@@ -14,6 +14,9 @@
 // cancellations of I1 - 3 and F - F^-T, so FP32 E and g stay accurate at
 // small strains.  The inversion test (Cell_Tetra.cpp:27-36, at Position_X)
 // is done in FP64 from the current node positions.
+#include <cstdlib>
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "reduce.hpp"
 
@@ -55,13 +58,37 @@ __device__ __forceinline__ double t_log1p<double>(double y) {
   return (y > -0.5) ? log1p(y) : log(fabs(1.0 + y));
 }
 
-template <class T, int FLAGS>
+// Copies nv values from LDS to HBM as 16-byte vectors (src and dst 16-byte
+// aligned), the remainder value by value.
+template <class T>
+__device__ __forceinline__ void copy_out(const T* src, T* dst, int nv) {
+  using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+  constexpr int W = 16 / sizeof(T);
+  const int nw = nv / W;
+  const V* s = reinterpret_cast<const V*>(src);
+  V* d = reinterpret_cast<V*>(dst);
+  for (int q = threadIdx.x; q < nw; q += TET_BLOCK) d[q] = s[q];
+  for (int q = W * nw + threadIdx.x; q < nv; q += TET_BLOCK) dst[q] = src[q];
+}
+
+// STAGE: the outputs of the workgroup's elements (packed Hessians, 78 values,
+// then gradients, 12 values) are written to LDS one element per lane and
+// copied out as one contiguous stretch each with 16-byte stores, instead of
+// every lane storing rows that lie 312 B (48 B) apart in HBM.  The output
+// offsets of a workgroup, TET_BLOCK x 78 (x 12) values, are 16-byte aligned.
+template <class T, int FLAGS, bool STAGE = false>
 __global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
     TetArgs A, double* __restrict__ blockE, double* __restrict__ elemE,
     T* __restrict__ eg, T* __restrict__ eH) {
   __shared__ double red[TET_BLOCK / 64];
+  constexpr int SH = (FLAGS & EVAL_H) ? TET_BLOCK * TET_H_PACK : 0;
+  constexpr int SG = (FLAGS & EVAL_G) ? TET_BLOCK * 12 : 0;
+  constexpr int HS = STAGE ? (SH > SG ? SH : SG) : 1;
+  static_assert(HS * sizeof(T) <= 96 * 1024, "staging buffer too large");
+  __shared__ __attribute__((aligned(16))) T hs[HS];
   const int e = blockIdx.x * TET_BLOCK + threadIdx.x;
   double Ee = 0.0;
+  T gv[12];
   if (e < A.m) {
     const long m = A.stride;
     const T* __restrict__ prm = reinterpret_cast<const T*>(A.prm);
@@ -172,7 +199,7 @@ __global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
 #pragma unroll
         for (int i = 0; i < 3; ++i) c[0][i] = -(c[1][i] + c[2][i] + c[3][i]);
         const T A0 = V0 * l2, A1 = V0 * (l2 - l1 * lnJ), A2 = V0 * l1;
-        T* __restrict__ h = eH + (long)e * TET_H_PACK;
+        T* h = STAGE ? hs + threadIdx.x * TET_H_PACK : eH + (long)e * TET_H_PACK;
 #pragma unroll
         for (int a = 1; a < 4; ++a)
 #pragma unroll
@@ -253,7 +280,7 @@ __global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
           for (int k = 0; k < 3; ++k)
             G[i][k] = F[i][0] * F[k][0] + F[i][1] * F[k][1] + F[i][2] * F[k][2];
         const T A0 = V0 * l2, A2 = V0 * l1;
-        T* __restrict__ h = eH + (long)e * TET_H_PACK;
+        T* h = STAGE ? hs + threadIdx.x * TET_H_PACK : eH + (long)e * TET_H_PACK;
 #pragma unroll
         for (int a = 1; a < 4; ++a)
 #pragma unroll
@@ -284,14 +311,35 @@ __global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
       }
     }
     if (FLAGS & EVAL_G) {
-      T* __restrict__ go = eg + (long)e * 12;
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-          go[3 * a + i] = V0 * (P[i][0] * b[a][0] + P[i][1] * b[a][1] + P[i][2] * b[a][2]);
+          gv[3 * a + i] = V0 * (P[i][0] * b[a][0] + P[i][1] * b[a][1] + P[i][2] * b[a][2]);
+      if constexpr (!STAGE) {
+        T* __restrict__ go = eg + (long)e * 12;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) go[k] = gv[k];
+      }
     }
     if ((FLAGS & EVAL_E) && elemE) elemE[e] = Ee;
+  }
+  if constexpr (STAGE) {
+    const long e0 = (long)blockIdx.x * TET_BLOCK;
+    const int ne = (int)((A.m - e0 < TET_BLOCK) ? A.m - e0 : TET_BLOCK);
+    if constexpr ((FLAGS & EVAL_H) != 0) {
+      __syncthreads();
+      copy_out<T>(hs, eH + e0 * TET_H_PACK, ne * TET_H_PACK);
+    }
+    if constexpr ((FLAGS & EVAL_G) != 0) {
+      __syncthreads();  // the Hessian copy has read hs
+      if (e < A.m) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) hs[threadIdx.x * 12 + k] = gv[k];
+      }
+      __syncthreads();
+      copy_out<T>(hs, eg + e0 * 12, ne * 12);
+    }
   }
   if ((FLAGS & EVAL_E) && blockE) {
     const double s = block_sum<TET_BLOCK>(Ee, red);
@@ -299,11 +347,29 @@ __global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
   }
 }
 
+// PGN_TET_STAGE=0 in the environment turns the LDS-staged output stores off
+static bool stage_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("PGN_TET_STAGE");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
 template <class T, int F>
 static void launch_t(hipStream_t s, const TetArgs& a, double* blockE,
                      double* elemE, void* eg, void* eH) {
   const int grid = (a.m + TET_BLOCK - 1) / TET_BLOCK;
   if (grid == 0) return;
+  // staging needs TET_BLOCK x 78 values of LDS for Hessians: FP32 only
+  if constexpr ((F & (EVAL_G | EVAL_H)) && (sizeof(T) == 4 || !(F & EVAL_H))) {
+    if (stage_enabled()) {
+      hipLaunchKernelGGL((tet_eval_kernel<T, F, true>), dim3(grid), dim3(TET_BLOCK), 0, s,
+                         a, blockE, elemE, reinterpret_cast<T*>(eg),
+                         reinterpret_cast<T*>(eH));
+      return;
+    }
+  }
   hipLaunchKernelGGL((tet_eval_kernel<T, F>), dim3(grid), dim3(TET_BLOCK), 0, s,
                      a, blockE, elemE, reinterpret_cast<T*>(eg),
                      reinterpret_cast<T*>(eH));

@@ -64,7 +64,7 @@ def test_model_without_elements(ctx):
     assert np.array_equal(G.get_x(), X0.reshape(-1))
     rp, ci, v = G.hessian(pg.LAYOUT_LOWER)
     # only the node blocks, all zero; AMatrixSd stores whole blocks with
-    # row >= col (AddStaticBlock, BasicTypes.cpp:60-75): 9 entries per node
+    # row >= col (AddStaticBlock, BasicTypes.cpp:64-77): 9 entries per node
     assert rp[-1] == 5 * 9 and np.all(v == 0.0)
     G.close()
 
