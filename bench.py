@@ -261,6 +261,7 @@ def main():
         log(f"cpu baseline {cpu}")
 
     lat = np.array([r["wall_s"] for r in results]) * 1e3
+    lpr = M.profile(2)[0]  # lanes per block row the SpMV autotuner chose
     line = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -290,15 +291,17 @@ def main():
         "copy_gbs": copy_gbs,
         **fp64,
         "pcg_k2_avg_us": k2_ms / max(k2_n, 1) * 1e3,
-        "roofline": {"bound": "hbm", "kernel": "pcg_k1b_kernel<float> (BSR3 SpMV of the PCG)",
+        "roofline": {"bound": "hbm",
+                     "kernel": f"pcg_k1b_kernel<float, {lpr}> (BSR3 SpMV of the PCG)",
                      "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("pcg_k1b_kernel"), "launches": k1_n, "avg_us": spmv_ms * 1e3,
+                     "traffic": measured_traffic(f"pcg_k1b_kernel<float, {lpr}>"),
+                     "launches": k1_n, "avg_us": spmv_ms * 1e3,
                      "bytes_per_launch": spmv_bytes},
         "cpu_baseline": cpu,
         "spmv_autotune_us": {str(l): M.profile(2 + c)[1] for c, l in enumerate((4, 8, 16))},
-        "spmv_lanes_per_row": M.profile(2)[0],
+        "spmv_lanes_per_row": lpr,
     }
     if sweep:
         line["sweep"] = [dict(precond=r["precond"], adaptive=r["adaptive"],

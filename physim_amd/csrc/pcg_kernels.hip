@@ -532,9 +532,38 @@ __global__ __launch_bounds__(K1_BS) void cg1_a_kernel(PcgArgs A, int k) {
 }
 
 template <class T>
+struct Cg1Batch {
+  N3<T> Z[K2_NPT], P[K2_NPT], W[K2_NPT], Q[K2_NPT], X[K2_NPT], R[K2_NPT];
+  PreM<T> M[K2_NPT];
+};
+
+template <class T>
+__device__ __forceinline__ void cg1_load(const PcgArgs& A, long i0, long hi, Cg1Batch<T>& b) {
+#pragma unroll
+  for (int j = 0; j < K2_NPT; ++j) {
+    const long i = i0 + (long)j * K2_BS;
+    if (i < hi) {
+      b.Z[j] = ld3(reinterpret_cast<const T*>(A.z), i);
+      b.P[j] = ld3(reinterpret_cast<const T*>(A.p0), i);
+      b.W[j] = ld3(reinterpret_cast<const T*>(A.p1), i);
+      b.Q[j] = ld3(reinterpret_cast<const T*>(A.q), i);
+      b.X[j] = ld3(reinterpret_cast<const T*>(A.x), i);
+      b.R[j] = ld3(reinterpret_cast<const T*>(A.r), i);
+      ld_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
+                    reinterpret_cast<const T*>(A.binv), i, b.M[j]);
+    }
+  }
+}
+
+template <class T>
 __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
   __shared__ double red[3 * K2_BS / 64 + 4];
   if (A.sc->done) return;
+  const RowRange rg = block_rows(A.n);
+  // the first batch is loaded before the partial sums: the two latencies overlap
+  Cg1Batch<T> b;
+  long i0 = rg.lo + threadIdx.x;
+  cg1_load<T>(A, i0, rg.hi, b);
   double gam, del, unused;
   all_sum3<K2_BS>(rr_slot(A, k - 1) + 1, rr_count(A, k - 1), 2, A.part1, A.g1, 1, A.part1, 0, 1,
                   red, gam, del, unused);
@@ -552,35 +581,41 @@ __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
     A.cgs[2 * (k & 1) + 1] = gam;
   }
   const T ta = (T)alpha, tb = (T)beta;
-  T* __restrict__ p = reinterpret_cast<T*>(A.p0);
-  T* __restrict__ q = reinterpret_cast<T*>(A.q);
-  const T* __restrict__ w = reinterpret_cast<const T*>(A.p1);
-  T* __restrict__ x = reinterpret_cast<T*>(A.x);
-  T* __restrict__ r = reinterpret_cast<T*>(A.r);
-  T* __restrict__ z = reinterpret_cast<T*>(A.z);
+  T* p = reinterpret_cast<T*>(A.p0);
+  T* q = reinterpret_cast<T*>(A.q);
+  T* x = reinterpret_cast<T*>(A.x);
+  T* r = reinterpret_cast<T*>(A.r);
+  T* z = reinterpret_cast<T*>(A.z);
   double rr = 0.0, rz = 0.0;
-  const RowRange rg = block_rows(A.n);
-  for (long i = rg.lo + threadIdx.x; i < rg.hi; i += K2_BS) {
-    T ri[3], zi[3];
+  for (;;) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const long d = 3 * i + c;
-      const T pd = z[d] + tb * p[d];
-      const T qd = w[d] + tb * q[d];
-      p[d] = pd;
-      q[d] = qd;
-      x[d] += ta * pd;
-      ri[c] = r[d] - ta * qd;
-      r[d] = ri[c];
-    }
-    apply_precond<T>(A.precond, reinterpret_cast<const T*>(A.dinv),
-                     reinterpret_cast<const T*>(A.binv), i, ri, zi);
+    for (int j = 0; j < K2_NPT; ++j) {
+      const long i = i0 + (long)j * K2_BS;
+      if (i < rg.hi) {
+        N3<T> pi, qi, xi, ri;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      z[3 * i + c] = zi[c];
-      rr += (double)ri[c] * (double)ri[c];
-      rz += (double)ri[c] * (double)zi[c];
+        for (int c = 0; c < 3; ++c) {
+          pi.c[c] = b.Z[j].c[c] + tb * b.P[j].c[c];
+          qi.c[c] = b.W[j].c[c] + tb * b.Q[j].c[c];
+          xi.c[c] = b.X[j].c[c] + ta * pi.c[c];
+          ri.c[c] = b.R[j].c[c] - ta * qi.c[c];
+        }
+        const N3<T> zi = mul_precond<T>(A.precond, b.M[j], ri);
+        st3(p, i, pi);
+        st3(q, i, qi);
+        st3(x, i, xi);
+        st3(r, i, ri);
+        st3(z, i, zi);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          rr += (double)ri.c[c] * (double)ri.c[c];
+          rz += (double)ri.c[c] * (double)zi.c[c];
+        }
+      }
     }
+    i0 += (long)K2_NPT * K2_BS;
+    if (i0 >= rg.hi) break;
+    cg1_load<T>(A, i0, rg.hi, b);
   }
   block_sum2<K2_BS>(rr, rz, red);
   if (threadIdx.x == 0) {
