@@ -261,7 +261,7 @@ def main():
         log(f"cpu baseline {cpu}")
 
     lat = np.array([r["wall_s"] for r in results]) * 1e3
-    lpr = M.profile(2)[0]  # lanes per block row the SpMV autotuner chose
+    lpr = M.profile(2)[0]  # lanes per block row of the SpMV
     line = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -300,7 +300,7 @@ def main():
                      "launches": k1_n, "avg_us": spmv_ms * 1e3,
                      "bytes_per_launch": spmv_bytes},
         "cpu_baseline": cpu,
-        "spmv_autotune_us": {str(l): M.profile(2 + c)[1] for c, l in enumerate((4, 8, 16))},
+        "spmv_width_us": {str(l): M.profile(2 + c)[1] for c, l in enumerate((4, 8, 16))},
         "spmv_lanes_per_row": lpr,
     }
     if sweep:

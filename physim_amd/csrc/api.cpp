@@ -563,7 +563,7 @@ int pgn_model_get_profile(pgn_model* M, int which, long* count, double* total_ms
     if (which <= 1) {
       *count = M->prof_n[which];
       *total_ms = M->prof_ms[which];
-    } else {  // autotuner: SpMV us with 4 / 8 / 16 lanes per row; count = choice
+    } else {  // SpMV us with 4 / 8 / 16 lanes per row; count = width in use
       *count = M->pcg_lpr;
       *total_ms = M->tune_us[which - 2];
     }

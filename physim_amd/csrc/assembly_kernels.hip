@@ -14,7 +14,18 @@ namespace pgn {
 constexpr int NODE_BLOCK = 256;
 constexpr int FINAL_BLOCK = 1024;
 
-int node_gather_blocks(int n) { return (n + NODE_BLOCK - 1) / NODE_BLOCK; }
+// Workgroups of the gathers: rounded up to whole rounds of the 8 XCDs, which
+// take workgroups round-robin by index.  xcd_block() maps workgroup b to
+// logical block (b % 8) * (G / 8) + b / 8, so each XCD walks one contiguous
+// eighth of the nodes (blocks) and the element records they share stay in
+// that XCD's L2 instead of being fetched by all eight.
+static inline int round_up8(long g) { return g >= 8 ? (int)((g + 7) / 8 * 8) : (int)g; }
+int node_gather_blocks(int n) { return round_up8((n + NODE_BLOCK - 1) / NODE_BLOCK); }
+
+__device__ __forceinline__ long xcd_block() {
+  const long G = gridDim.x, b = blockIdx.x;
+  return (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+}
 
 // Gradient = sum of element gradients (element order) - M_i g + f, fixed DoFs
 // zeroed (FixVectorDoFs, Simulable.cpp:735-744).  Node energy terms:
@@ -24,12 +35,35 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_gather_kernel(
     NodeArgs A, const T* __restrict__ eg, double* __restrict__ g,
     double* __restrict__ part) {
   __shared__ double red[2 * NODE_BLOCK / 64];
-  const int i = blockIdx.x * NODE_BLOCK + threadIdx.x;
+  const long i = xcd_block() * NODE_BLOCK + threadIdx.x;
   double eN = 0.0, gg = 0.0;
   if (i < A.n) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     const int p1 = A.inc_ptr[i + 1];
-    for (int p = A.inc_ptr[i]; p < p1; ++p) {
+    int p = A.inc_ptr[i];
+    // eight incidences per round: their indices, then their gradient rows,
+    // are all in flight before the sums (in incidence order) need them
+    constexpr int U = 8;
+    for (; p + U <= p1; p += U) {
+      int c[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) c[j] = A.inc[p + j];
+      T v[U][3];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const T* ge = eg + (long)(c[j] >> 2) * 12 + 3 * (c[j] & 3);
+        v[j][0] = ge[0];
+        v[j][1] = ge[1];
+        v[j][2] = ge[2];
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        s0 += (double)v[j][0];
+        s1 += (double)v[j][1];
+        s2 += (double)v[j][2];
+      }
+    }
+    for (; p < p1; ++p) {
       const int c = A.inc[p];
       const T* ge = eg + (long)(c >> 2) * 12 + 3 * (c & 3);
       s0 += (double)ge[0];
@@ -137,7 +171,7 @@ template <class T>
 __global__ __launch_bounds__(256) void bsr_gather_kernel(BsrArgs A,
                                                          const T* __restrict__ eH,
                                                          T* __restrict__ bval) {
-  const long s = blockIdx.x * 256L + threadIdx.x;
+  const long s = xcd_block() * 256L + threadIdx.x;
   if (s >= A.nb) return;
   T acc[9];
 #pragma unroll
@@ -190,7 +224,7 @@ __global__ __launch_bounds__(256) void bsr_gather_kernel(BsrArgs A,
 
 void launch_bsr_gather(hipStream_t s, bool fp32, const BsrArgs& a,
                        const void* eH, void* bval) {
-  const long grid = (a.nb + 255) / 256;
+  const long grid = round_up8((a.nb + 255) / 256);
   if (grid == 0) return;
   if (fp32)
     hipLaunchKernelGGL(bsr_gather_kernel<float>, dim3(grid), dim3(256), 0, s, a,
