@@ -139,6 +139,9 @@ def main():
                     help="serving: largest beam request (tets)")
     ap.add_argument("--max-cloth", type=int, default=512,
                     help="serving: largest cloth request (nodes per side)")
+    ap.add_argument("--group-by-size", type=int, default=1,
+                    help="serving: 1 (default) batches of similar footprint, cheapest "
+                         "first; 0 arrival order")
     ap.add_argument("--rod-density", type=float, default=3.14e-6,
                     help="rods: line density, kg per metre of rod")
     args = ap.parse_args()

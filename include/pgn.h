@@ -288,18 +288,26 @@ int pgn_batch_get_energy(pgn_batch* batch, double* energy);
 /* ---- request loop (serving) ---------------------------------------------- */
 /* A persistent loop that solves independent requests on one GPU.  Requests
  * are copied at submit and queued; each lane (a worker thread with its own
- * HIP stream) admits queued requests in arrival order while their estimated
- * device footprint fits hbm_budget_bytes, solves them as one fused batch
- * (pgn_batch semantics: every request keeps its own Newton state) and frees
- * their memory.  Requests arriving while batches run are taken by the next
- * free lane, so solves overlap on the device.  No reference counterpart:
- * PhySim solves one Simulable per OptimSolver call (Simulator.cpp:38-72). */
+ * HIP stream) admits queued requests while their estimated device footprint
+ * fits hbm_budget_bytes, solves them as one fused batch (pgn_batch
+ * semantics: every request keeps its own Newton state) and frees their
+ * memory.  A fused batch finishes with its slowest request, so by default
+ * (group_by_size) a lane starts its batch with the cheapest queued request
+ * and adds only requests of at most twice its footprint: small requests are
+ * not held up by large ones.  Requests arriving while batches run are taken
+ * by the next free lane, so solves overlap on the device.  No reference
+ * counterpart: PhySim solves one Simulable per OptimSolver call
+ * (Simulator.cpp:38-72). */
 typedef struct {
   size_t hbm_budget_bytes; /* device memory the resident requests may hold */
   int lanes;               /* concurrent batches (streams), 1..16          */
   int max_batch;           /* requests per batch                           */
   int precision;           /* PGN_FP64 or PGN_MIXED                        */
   pgn_newton_opts newton;  /* solver options of every request              */
+  int group_by_size;       /* 1 (default): batches of similar footprint,
+                              cheapest first; 0: arrival order              */
+  double max_wait_s;       /* group_by_size: a request queued this long is
+                              taken before cheaper ones (default 2 s)       */
 } pgn_server_opts;
 typedef struct pgn_server pgn_server;
 void pgn_server_default_opts(pgn_server_opts* o);

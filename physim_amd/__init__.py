@@ -88,7 +88,8 @@ class NewtonOpts(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("lanes", ctypes.c_int),
                 ("max_batch", ctypes.c_int), ("precision", ctypes.c_int),
-                ("newton", NewtonOpts)]
+                ("newton", NewtonOpts), ("group_by_size", ctypes.c_int),
+                ("max_wait_s", ctypes.c_double)]
 
 
 class NewtonResult(ctypes.Structure):
@@ -580,16 +581,19 @@ class Batch:
 class Server:
     """The request loop (pgn_server): submit scenes from any thread, wait for
     their results.  Lanes run fused batches on their own streams; admission
-    is by estimated device footprint against hbm_budget_bytes."""
+    is by estimated device footprint against hbm_budget_bytes, batches of
+    similar footprint cheapest first (group_by_size) or in arrival order."""
 
     def __init__(self, device=0, hbm_budget_bytes=64 << 30, lanes=4, max_batch=64,
-                 precision=MIXED, **newton):
+                 precision=MIXED, group_by_size=1, max_wait_s=2.0, **newton):
         o = ServerOpts()
         lib().pgn_server_default_opts(ctypes.byref(o))
         o.hbm_budget_bytes = int(hbm_budget_bytes)
         o.lanes = lanes
         o.max_batch = max_batch
         o.precision = precision
+        o.group_by_size = int(group_by_size)
+        o.max_wait_s = float(max_wait_s)
         for k, v in newton.items():
             setattr(o.newton, k, v)
         h = _vp()

@@ -3,15 +3,19 @@
 //
 // Requests (tet meshes and thin shells with their boundary conditions) are
 // copied into host staging at submit time and queued.  Each lane is a worker
-// thread with its own HIP stream; a free lane admits the queued requests in
-// arrival order while their estimated device footprint fits the server's
-// HBM budget (and at most max_batch of them), builds them into one fused
-// batch (pgn_batch: one launch per kernel covers all of them, every request
-// keeps its own Newton state), solves, hands the results back and releases
-// the memory.  Requests that arrive while batches run are taken by the next
+// thread with its own HIP stream; a free lane admits queued requests while
+// their estimated device footprint fits the server's HBM budget (and at most
+// max_batch of them), builds them into one fused batch (pgn_batch: one launch
+// per kernel covers all of them, every request keeps its own Newton state),
+// solves, hands the results back and releases the memory.  A fused batch ends
+// with its slowest request, so with group_by_size a lane starts from the
+// cheapest queued request (the oldest once it has waited max_wait_s) and
+// adds only requests of at most twice that footprint, cheapest first;
+// otherwise it takes requests in arrival order.  Requests that arrive while batches run are taken by the next
 // free lane, so solves overlap on the device across lanes, and each result
 // is the one the request would get alone (per-instance Newton, block-
 // diagonal linear solve).
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -59,7 +63,23 @@ struct pgn_server {
   long batches = 0, solved = 0;
 
   void lane_main(int lane);
+  // index in `pending` of the request a free lane starts with, or -1 when
+  // none may start now (caller holds mu)
+  long pick_head() const;
 };
+
+long pgn_server::pick_head() const {
+  if (pending.empty()) return -1;
+  auto fits = [&](long i) { return used + pending[i]->bytes <= opts.hbm_budget_bytes; };
+  if (!opts.group_by_size) return fits(0) ? 0 : -1;
+  const double waited =
+      std::chrono::duration<double>(Clock::now() - pending.front()->t_submit).count();
+  if (waited >= opts.max_wait_s) return fits(0) ? 0 : -1;  // no starvation
+  long best = 0;
+  for (long i = 1; i < (long)pending.size(); ++i)
+    if (pending[i]->bytes < pending[best]->bytes) best = i;
+  return fits(best) ? best : -1;
+}
 
 void pgn_server::lane_main(int lane) {
   pgn_ctx* ctx = ctxs[lane];
@@ -67,18 +87,47 @@ void pgn_server::lane_main(int lane) {
     std::vector<std::shared_ptr<Request>> take;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] {
+      long head = -1;
+      cv_work.wait_for(lk, std::chrono::milliseconds(50), [&] {
         if (stop) return true;
-        return !pending.empty() && used + pending.front()->bytes <= opts.hbm_budget_bytes;
+        head = pick_head();
+        return head >= 0;
       });
-      if (stop && pending.empty()) return;
       if (stop) return;
-      // admission by HBM footprint, arrival order
-      while (!pending.empty() && (int)take.size() < opts.max_batch &&
-             used + pending.front()->bytes <= opts.hbm_budget_bytes) {
-        used += pending.front()->bytes;
-        take.push_back(pending.front());
-        pending.pop_front();
+      if (head < 0) continue;  // re-check the aging bound
+      if (!opts.group_by_size) {
+        // arrival order while the footprint fits
+        while (!pending.empty() && (int)take.size() < opts.max_batch &&
+               used + pending.front()->bytes <= opts.hbm_budget_bytes) {
+          used += pending.front()->bytes;
+          take.push_back(pending.front());
+          pending.pop_front();
+        }
+      } else {
+        // the head, then requests of at most twice its footprint, cheapest
+        // first (arrival order among equals)
+        const size_t cap = 2 * pending[head]->bytes;
+        std::vector<long> order(pending.size());
+        for (long i = 0; i < (long)order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](long a, long b) {
+          return pending[a]->bytes < pending[b]->bytes;
+        });
+        std::vector<char> taken(pending.size(), 0);
+        used += pending[head]->bytes;
+        take.push_back(pending[head]);
+        taken[head] = 1;
+        for (long i : order) {
+          if ((int)take.size() >= opts.max_batch) break;
+          if (taken[i] || pending[i]->bytes > cap) continue;
+          if (used + pending[i]->bytes > opts.hbm_budget_bytes) break;
+          used += pending[i]->bytes;
+          take.push_back(pending[i]);
+          taken[i] = 1;
+        }
+        std::deque<std::shared_ptr<Request>> rest;
+        for (long i = 0; i < (long)pending.size(); ++i)
+          if (!taken[i]) rest.push_back(pending[i]);
+        pending.swap(rest);
       }
       ++batches;
     }
@@ -197,6 +246,8 @@ void pgn_server_default_opts(pgn_server_opts* o) {
   o->max_batch = 64;
   o->precision = PGN_MIXED;
   pgn_newton_default_opts(&o->newton);
+  o->group_by_size = 1;
+  o->max_wait_s = 2.0;
 }
 
 int pgn_server_create(int device, const pgn_server_opts* opts, pgn_server** out) {

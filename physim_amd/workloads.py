@@ -229,7 +229,7 @@ def serving(args, log):
     log(f"serving: {count} requests at {args.rate:g}/s, "
         f"{sum(hasattr(r, 'hinges') for r in reqs)} cloths")
     S = pg.Server(hbm_budget_bytes=200 << 30, lanes=4, max_batch=16, precision=pg.MIXED,
-                  **dict(SOLVE_OPTS, numMaxIters=300))
+                  group_by_size=args.group_by_size, **dict(SOLVE_OPTS, numMaxIters=300))
     rng = np.random.default_rng(6)
     gaps = rng.exponential(1.0 / args.rate, size=count)
     tickets = [None] * count
@@ -264,7 +264,7 @@ def serving(args, log):
         "dtype": "f32 Hessian/PCG, f64 energy/gradient/residual",
         "config": {"workload": f"serving_{count}_requests: Poisson {args.rate:g}/s, 60% beams "
                                f"2k-{args.max_tets} tets, 40% cloths 32-{args.max_cloth}",
-                   "lanes": 4, "max_batch": 16},
+                   "lanes": 4, "max_batch": 16, "group_by_size": args.group_by_size},
         "converged": ok,
         "not_converged": {f"{i}:{reqs[i].name}": results[i]["result"] for i in range(count)
                           if results[i]["result"] != "SUCCESS"},

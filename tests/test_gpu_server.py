@@ -47,11 +47,14 @@ def _requests(seed, count):
     return out
 
 
-def test_server_results_match_oracle():
-    _say("server: staggered submissions, 2 lanes, footprint admission")
+@pytest.mark.parametrize("group_by_size", [1, 0])
+def test_server_results_match_oracle(group_by_size):
+    _say(f"server: staggered submissions, 2 lanes, footprint admission, "
+         f"group_by_size={group_by_size}")
     reqs = _requests(11, 9)
     # a budget that holds two to five of these requests (0.06-0.5 MB each)
-    S = pg.Server(hbm_budget_bytes=1_000_000, lanes=2, max_batch=4, **OPTS)
+    S = pg.Server(hbm_budget_bytes=1_000_000, lanes=2, max_batch=4,
+                  group_by_size=group_by_size, **OPTS)
     tickets = [None] * len(reqs)
 
     def producer():
