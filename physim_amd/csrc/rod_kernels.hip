@@ -370,105 +370,153 @@ __global__ __launch_bounds__(ROD_BS) void rod_band_kernel(RodArgs A, const doubl
 }
 
 // ---- banded LDL^T per rod (LinearSolver_EigenLDLT::SolveInternal) -------------------
-// One lane per rod.  Factorization without pivoting of band + shift, singular
-// test |D_jj| < sinT, indefiniteness test b^.x^ < -regT, accuracy test, and
-// the cumulative regularization loop.  work: (ROD_BW+1) values per DoF.
-__global__ __launch_bounds__(64) void rod_ldlt_kernel(RodArgs A, const double* __restrict__ band,
-                                                       double* __restrict__ work,
-                                                       const double* __restrict__ b,
-                                                       double* __restrict__ x, int reg_iters,
-                                                       double max_error, int* __restrict__ status,
-                                                       const unsigned char* __restrict__ active) {
-  const int r = blockIdx.x * 64 + threadIdx.x;
+// One 64-lane workgroup per rod.  Factorization without pivoting of
+// band + shift, singular test |D_jj| < sinT, indefiniteness test
+// b^.x^ < -regT, accuracy test, and the cumulative regularization loop.
+// work: (ROD_BW+1) values per DoF (D on k = 0, L(i, i-k) on k = 1..ROD_BW).
+//
+// Left-looking factorization, one column j per step: D_j is a reduction over
+// the ROD_BW previous columns, and lane t (< ROD_BW) computes L(j+1+t, j) from
+// the rows kept in an LDS window of ROD_BW+1 rows (row i in slot i mod W).
+// The triangular solves keep the last ROD_BW values in lanes 1..ROD_BW and
+// slide them one lane per step.  Norms, the indefiniteness test and the
+// accuracy residual are sums over the rod's DoFs by the whole group.
+constexpr int LDLT_BS = 64;
+
+__device__ __forceinline__ double group_allsum(double v) {
+  v = wave_sum(v);
+  return __shfl(v, 0, 64);
+}
+
+__global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const double* __restrict__ band,
+                                                          double* __restrict__ work,
+                                                          const double* __restrict__ b,
+                                                          double* __restrict__ x, int reg_iters,
+                                                          double max_error, int* __restrict__ status,
+                                                          const unsigned char* __restrict__ active) {
+  constexpr int W = ROD_BW + 1;
+  __shared__ double Lw[W][W];  // Lw[i % W][k] = L(i, i - k), k = 1..ROD_BW
+  __shared__ double Dw[W];     // Dw[m % W] = D(m)
+  const int r = blockIdx.x;
   if (r >= A.nrods) return;
   if (active && !active[r]) return;
+  const int t = threadIdx.x;
   const long d0 = 4L * A.voff[r], d1 = 4L * A.voff[r + 1];
   const long N = d1 - d0;
-  constexpr int W = ROD_BW + 1;
   // |A| over the entry set AMatrixSd stores: lower triangle + the upper
   // entries of every vertex's 3x3 position block
-  double nA = 0.0;
-  for (long d = d0; d < d1; ++d) {
+  double nA = 0.0, nbb = 0.0;
+  for (long d = d0 + t; d < d1; d += LDLT_BS) {
+    const int comp = (int)(d & 3);
     for (int k = 0; k < W; ++k) {
       const double a = band[W * d + k];
       nA += a * a;
-      const int comp = (int)(d & 3);
       if (comp < 3 && k > 0 && k <= comp) nA += a * a;
     }
+    nbb += b[d] * b[d];
   }
-  nA = sqrt(nA);
+  nA = sqrt(group_allsum(nA));
+  const double nb = sqrt(group_allsum(nbb));
   double sinT = fmin(1e-6 * nA, 1e-6);
   sinT = fmax(sinT, 1e-9);
   const double regT = 1e-6 * nA;
   int st = 1;
-  double nb = 0.0;
-  for (long d = d0; d < d1; ++d) nb += b[d] * b[d];
-  nb = sqrt(nb);
   for (int it = 0; it < reg_iters && st != 0; ++it) {
     double cum = 0.0;  // LinearSolver_EigenLDLT.cpp:68: A += regT 10^(i-1) I, cumulative
     for (int q = 1; q <= it; ++q) cum += regT * pow(10.0, q - 1);
-    // factor: work holds L (unit lower, strictly-lower band) and D on k = 0
+    for (int q = t; q < W * W; q += LDLT_BS) (&Lw[0][0])[q] = 0.0;
+    if (t < W) Dw[t] = 0.0;
+    __syncthreads();
     bool bad = false;
-    for (long j = 0; j < N && !bad; ++j) {
+    for (long j = 0; j < N; ++j) {
       const long gj = d0 + j;
-      double dj = band[W * gj] + cum;
-      for (int k = 1; k < W && k <= j; ++k) {
-        const double l = work[W * gj + k];
-        dj -= l * l * work[W * (gj - k)];
+      const int sj = (int)(j % W);
+      double term = 0.0;
+      if (t >= 1 && t < W && t <= j) {
+        const double l = Lw[sj][t];
+        term = l * l * Dw[(j - t) % W];
       }
-      if (!(fabs(dj) >= sinT)) {
+      const double dj = band[W * gj] + cum - group_allsum(term);
+      if (!(fabs(dj) >= sinT)) {  // uniform: dj is the same in every lane
         bad = true;
         break;
       }
-      work[W * gj] = dj;
-      for (int i = 1; i < W && j + i < N; ++i) {
-        const long gi = gj + i;
-        // L(gi, gj) = (A(gi, gj) - sum_m L(gi, m) L(gj, m) D(m)) / dj
-        double s = band[W * gi + i];
-        for (int k = i + 1; k < W && (long)k <= j + i; ++k) {
-          // m = gi - k, needs gj - m = k - i < W
-          const long m = gi - k;
-          s -= work[W * gi + k] * work[W * gj + (k - i)] * work[W * m];
+      // L(i, j), i = j + 1 + t: (A(i, j) - sum_m L(i, m) L(j, m) D(m)) / D_j over
+      // the columns m < j within the band of both rows
+      const long i = j + 1 + t;
+      const bool row = t < ROD_BW && i < N;
+      double lij = 0.0;
+      if (row) {
+        const int o = t + 1;  // i - j
+        const int si = (int)(i % W);
+        double sacc = band[W * (d0 + i) + o];
+        for (int k = o + 1; k < W && (long)k <= i; ++k) {
+          const long m = i - k;
+          sacc -= Lw[si][k] * Lw[sj][k - o] * Dw[m % W];
         }
-        work[W * gi + i] = s / dj;
+        lij = sacc / dj;
       }
+      __syncthreads();  // the window of step j has been read
+      if (t == 0) {
+        Dw[j % W] = dj;
+        work[W * gj] = dj;
+      }
+      if (row) {
+        Lw[i % W][t + 1] = lij;
+        work[W * (d0 + i) + t + 1] = lij;
+      }
+      if (t >= 1 && t < W) Lw[sj][t] = 0.0;  // slot of row j becomes row j + W
+      __syncthreads();
     }
     if (bad) continue;
-    // solve L y = b, D z = y, L^T x = z
+    // L y = b: lane k in 1..ROD_BW holds y(j - k)
+    double win = 0.0;
     for (long j = 0; j < N; ++j) {
       const long gj = d0 + j;
-      double s = b[gj];
-      for (int k = 1; k < W && k <= j; ++k) s -= work[W * gj + k] * x[gj - k];
-      x[gj] = s;
+      const double term = (t >= 1 && t < W && t <= j) ? work[W * gj + t] * win : 0.0;
+      const double y = b[gj] - group_allsum(term);
+      if (t == 0) x[gj] = y;
+      win = __shfl_up(win, 1, 64);
+      if (t == 1) win = y;
     }
-    for (long j = 0; j < N; ++j) x[d0 + j] /= work[W * (d0 + j)];
+    __syncthreads();
+    // D z = y
+    for (long d = d0 + t; d < d1; d += LDLT_BS) x[d] /= work[W * d];
+    __syncthreads();
+    // L^T x = z: lane k in 1..ROD_BW holds x(j + k)
+    win = 0.0;
     for (long j = N - 1; j >= 0; --j) {
       const long gj = d0 + j;
-      double s = x[gj];
-      for (int k = 1; k < W && j + k < N; ++k) s -= work[W * (gj + k) + k] * x[gj + k];
-      x[gj] = s;
+      const double term =
+          (t >= 1 && t < W && j + t < N) ? work[W * (gj + t) + t] * win : 0.0;
+      const double xj = x[gj] - group_allsum(term);
+      if (t == 0) x[gj] = xj;
+      win = __shfl_up(win, 1, 64);
+      if (t == 1) win = xj;
     }
+    __syncthreads();
     // indefiniteness: normalized(b).normalized(x) < -regT
-    double nx = 0.0, bx = 0.0;
-    for (long d = d0; d < d1; ++d) {
-      nx += x[d] * x[d];
+    double nxx = 0.0, bx = 0.0;
+    for (long d = d0 + t; d < d1; d += LDLT_BS) {
+      nxx += x[d] * x[d];
       bx += b[d] * x[d];
     }
-    nx = sqrt(nx);
+    const double nx = sqrt(group_allsum(nxx));
+    bx = group_allsum(bx);
     if (nb > 0.0 && nx > 0.0 && bx / (nb * nx) < -regT) continue;
     // accuracy: |b - (A + shift) x| against maxError (abs and relative)
     double ae = 0.0;
-    for (long d = d0; d < d1; ++d) {
-      double s = (band[W * d] + cum) * x[d];
-      for (int k = 1; k < W && d - k >= d0; ++k) s += band[W * d + k] * x[d - k];
-      for (int k = 1; k < W && d + k < d1; ++k) s += band[W * (d + k) + k] * x[d + k];
-      ae += (b[d] - s) * (b[d] - s);
+    for (long d = d0 + t; d < d1; d += LDLT_BS) {
+      double sv = (band[W * d] + cum) * x[d];
+      for (int k = 1; k < W && d - k >= d0; ++k) sv += band[W * d + k] * x[d - k];
+      for (int k = 1; k < W && d + k < d1; ++k) sv += band[W * (d + k) + k] * x[d + k];
+      ae += (b[d] - sv) * (b[d] - sv);
     }
-    ae = sqrt(ae);
+    ae = sqrt(group_allsum(ae));
     if (ae > max_error && (nb > 0.0 ? ae / nb : ae) > max_error) continue;
     st = 0;
   }
-  status[r] = st;
+  if (t == 0) status[r] = st;
 }
 
 // ---- reference frames and twist (UpdateKinematics) -----------------------------------
@@ -578,7 +626,8 @@ void rod_launch_band(hipStream_t s, const RodArgs& a, bool fp32_hess, RodBufs& b
 
 void rod_launch_ldlt(hipStream_t s, const RodArgs& a, RodBufs& b, const double* rhs, double* x,
                      int reg_iters, double max_error, const unsigned char* active) {
-  hipLaunchKernelGGL(rod_ldlt_kernel, dim3(blocks(a.nrods, 64)), dim3(64), 0, s, a, b.band, b.work,
+  if (a.nrods == 0) return;
+  hipLaunchKernelGGL(rod_ldlt_kernel, dim3(a.nrods), dim3(LDLT_BS), 0, s, a, b.band, b.work,
                      rhs, x, reg_iters, max_error, b.status, active);
 }
 
