@@ -201,7 +201,7 @@ typedef struct {
                          (|E_elastic| + |E_load|), i.e. within FP64 rounding
                          of the total, when |g| decreases                   */
   double ew_gamma;    /* Eisenstat-Walker gamma (default 0.9)              */
-  double ew_max;      /* largest forcing term (default 0.1)                */
+  double ew_max;      /* largest forcing term (default 0.2)                */
 } pgn_newton_opts;
 
 /* OSResult (BasicTypes.h:69-77) */

@@ -518,7 +518,7 @@ void pgn_newton_default_opts(pgn_newton_opts* o) {
   o->adaptive_tol = 0;
   o->ls_noise_guard = 0.0;
   o->ew_gamma = 0.9;
-  o->ew_max = 0.1;
+  o->ew_max = 0.2;
 }
 
 int pgn_newton_solve(pgn_model* M, const pgn_newton_opts* opts, pgn_newton_result* res) {
