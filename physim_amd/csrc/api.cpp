@@ -600,16 +600,19 @@ int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
         *bytes = 2.0 * 256.0 * 1024 * 1024;
         break;
       case 8:  // the PCG SpMV launch.  Fused K1: values + block cols + row
-               // pointers, z and p_{k-1} read, p_k and q written.  Split
-               // (default) and Chronopoulos-Gear: the plain SpMV, values +
-               // cols + row pointers, one vector read, q written.
+               // pointers, z and p_{k-1} read, p_k and q written.
+               // Chronopoulos-Gear (default) and split: the plain SpMV,
+               // values + cols + row pointers, one vector read, one written.
         if (M->pcg_split || M->pcg_cg1)
           *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 2 * 3 * n * hs;
         else
           *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 4 * 3 * n * hs;
         break;
-      case 9:  // PCG K2: p, q, x, r, dinv read; x, r, z written
-        *bytes = 3 * n * hs * 8;
+      case 9:  // the PCG update launch with the block-Jacobi preconditioner
+               // (9 values per node).  Chronopoulos-Gear: z, p, w, q, x, r
+               // read, p, q, x, r, z written; split and fused K2: p, q, x, r
+               // read, x, r, z written.
+        *bytes = (M->pcg_cg1 ? 11.0 : 7.0) * 3 * n * hs + 9.0 * n * hs;
         break;
       default:
         throw Error(PGN_ERR_INVALID, "unknown kernel id");

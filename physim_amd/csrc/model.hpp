@@ -2,12 +2,21 @@
 #pragma once
 
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 
 #include "kernels.hpp"
 #include "pgn_internal.hpp"
 
 namespace pgn {
+
+// 0 = cg1, 1 = split, 2 = fused (from PGN_PCG; unset or unknown: cg1)
+inline int pcg_variant() {
+  const char* v = std::getenv("PGN_PCG");
+  if (v && std::strcmp(v, "split") == 0) return 1;
+  if (v && std::strcmp(v, "fused") == 0) return 2;
+  return 0;
+}
 
 // Host-side pattern of the BSR3 Hessian and the element->block slot map.
 struct BsrPattern {
@@ -71,12 +80,12 @@ struct pgn_model {
   pgn::DBuf<char> pb;     // rhs in Hessian precision
   pgn::DBuf<double> part1, part2, partb, red64;
   int pcg_lpr = 8;  // lanes per block row of the PCG SpMV (tuned per model)
-  // K1 split into a streaming p update + plain SpMV (default); PGN_PCG_FUSED
-  // in the environment selects the single fused launch instead
-  bool pcg_split = std::getenv("PGN_PCG_FUSED") == nullptr;
-  // Chronopoulos-Gear PCG (one SpMV + one streaming launch per iteration):
-  // PGN_PCG_CG1 in the environment selects it
-  bool pcg_cg1 = std::getenv("PGN_PCG_CG1") != nullptr;
+  // PCG launch structure, PGN_PCG in the environment: "cg1" (default:
+  // Chronopoulos-Gear, one SpMV on z + one streaming launch per iteration),
+  // "split" (streaming p update, plain SpMV, update launch) or "fused" (p
+  // update folded into the SpMV's column gathers, update launch)
+  bool pcg_cg1 = pgn::pcg_variant() == 0;
+  bool pcg_split = pgn::pcg_variant() != 2;
   pgn::DBuf<double> cgs;  // its 4 scalars
   bool tuned = false;
   double tune_us[3] = {0, 0, 0};  // SpMV time for 4, 8, 16 lanes per row
