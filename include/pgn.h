@@ -195,7 +195,11 @@ typedef struct {
                          1: Eisenstat-Walker forcing (choice 2, gamma
                          ew_gamma, alpha 2) with Kelley's safeguard
                          (at least 0.5 tolMaxError / |g|), capped at
-                         ew_max, floored at lsTolMaxError                   */
+                         ew_max, floored at lsTolMaxError; a step whose
+                         loosened solve gives a direction the line search
+                         cannot descend along is taken again at
+                         lsTolMaxError, as is every later step of the
+                         solve (per instance in a batch)                    */
   double ls_noise_guard; /* 0 = reference line search.  > 0: also accept a
                          step whose energy rises by at most ls_noise_guard *
                          (|E_elastic| + |E_load|), i.e. within FP64 rounding

@@ -437,6 +437,10 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
   std::vector<uint8_t> hmode(Kn, 0), hmask(Kn, 0);
   std::vector<double> obj(Kn), opt(Kn), optStart(Kn), objNew(Kn), optNew(Kn), f(Kn), len(Kn),
       clamp(Kn), stepLen(Kn), hfac(Kn), eta(Kn, o.ew_max), gprev(Kn, -1.0), hn2(Kn);
+  // per instance: fixed lsTolMaxError after an inexact step failed its line
+  // search (the retry rule of the single-model solver); inexact: this step's
+  // linear solve was looser than lsTolMaxError
+  std::vector<uint8_t> ew_off(Kn, 0), inexact(Kn, 0);
   std::vector<double> E;
   auto d2d = [&](double* dst, const double* src) {
     PGN_HIP(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -474,11 +478,12 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
         continue;
       }
       hmode[k] = steep[k] ? 2 : 1;
+      inexact[k] = 0;
       if (hmode[k] == 1) {
         need_h = true;
         hevals[k]++;
         double lt = o.lsTolMaxError;
-        if (o.adaptive_tol) {  // Eisenstat-Walker choice 2 per instance
+        if (o.adaptive_tol && !ew_off[k]) {  // Eisenstat-Walker choice 2 per instance
           if (gprev[k] > 0.0) {
             const double r = opt[k] / gprev[k];
             double e = o.ew_gamma * r * r;
@@ -492,6 +497,7 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
           lt = std::max(o.lsTolMaxError, eta[k]);
         }
         tol_lin[k] = lt;
+        inexact[k] = lt > o.lsTolMaxError;
       }
     }
     mode.upload(hmode, s);
@@ -593,6 +599,10 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
         opt[k] = optNew[k];
         hmask[k] = 1;  // v0 := the step start (OptimSolver.cpp:488 warm start)
         any_impr = true;
+      } else if (inexact[k]) {  // retry with the fixed tolerance
+        ew_off[k] = 1;
+        result[k] = PGN_OS_ONGOING;
+        opt[k] = optStart[k];
       } else {
         result[k] = PGN_OS_NONDESC;
         opt[k] = optStart[k];

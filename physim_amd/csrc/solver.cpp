@@ -196,6 +196,13 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
   // Eisenstat-Walker forcing (choice 2, gamma 0.9, alpha 2) when adaptive_tol:
   // the linear tolerance follows the Newton residual's own convergence rate.
   double eta = o.ew_max, g_prev = -1.0;
+  // A step whose loosened linear solve gave a direction the line search
+  // could not descend along is taken again with lsTolMaxError, and so is
+  // every later step of the solve: on buckling-prone models the inexact
+  // directions otherwise end the solve with NONDESC where the exact ones
+  // converge.
+  bool ew_off = false;
+  bool inexact = false;  // the current step's linear solve was looser than lsTolMaxError
 
   // OptimSolver_USQP_LS::SolveStepInternal (OptimSolver_USQP_LS.cpp:18-68)
   auto solve_step = [&](bool use_steepest) {
@@ -215,7 +222,8 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
       R.hessian_evals++;
       launch_lincomb64(s, N, 1.0, xsave, -1.0, v0, dx);  // dx0 = x - x_prev (:488)
       launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, bvec);
-      if (o.adaptive_tol) {
+      lo.tol = o.lsTolMaxError;
+      if (o.adaptive_tol && !ew_off) {
         if (g_prev > 0.0) {
           const double r = opt / g_prev;
           double e = o.ew_gamma * r * r;
@@ -231,6 +239,7 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
         lo.tol = std::max(o.lsTolMaxError, eta);
         g_prev = opt;
       }
+      inexact = lo.tol > o.lsTolMaxError;
       pgn_linear_result lr;
       linear_solve(M, bvec, dx, lo, lr);
       R.pcg_iterations += lr.iterations;
@@ -239,6 +248,7 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
                      itCount, opt, lo.tol, lr.iterations, lr.refinements, lr.reg_used,
                      lr.rel_residual);
     } else {
+      inexact = false;
       launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, dx);
     }
     double len = std::sqrt(M.dot(dx, dx, N));
@@ -291,6 +301,9 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
       d2d(v0, xsave);
       obj = objNew;
       opt = optNew;
+    } else if (inexact) {  // retry with the fixed tolerance
+      ew_off = true;
+      result = PGN_OS_ONGOING;
     } else {
       result = PGN_OS_NONDESC;
     }

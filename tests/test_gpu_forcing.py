@@ -1,0 +1,71 @@
+"""Inexact Newton (Eisenstat-Walker linear tolerances) against the reference's
+fixed linear tolerance on a model where they part ways: the 9x6x61 StVK beam
+of the 256-beam batch workload buckles under its load, and a step whose
+loosened linear solve gives a direction the line search cannot descend along
+is taken again with lsTolMaxError (and so is every later step).  Without that
+retry the inexact solve ended NONDESC after 14 Newton iterations while the
+fixed tolerance converges.
+
+Checked: both converge, alone and inside a fused batch, to the FP64
+fixed-tolerance solution within 1e-6 x bounding-box diagonal.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import physim_amd as pg
+from physim_amd import scenes
+from tests.helpers import pgn_model
+
+pytestmark = pytest.mark.gpu
+
+OPTS = dict(pg.CONVERGE_OPTS, precond=1, lsNumMaxIters=2000, numMaxIters=200)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def beams():
+    scs = scenes.batch_beams(33, seed=3)
+    assert scs[32].name == "beam(9, 6, 61)_stvk"
+    return scs[0], scs[32]
+
+
+@pytest.fixture(scope="module")
+def fixed_solution(ctx, beams):
+    s = beams[1]
+    F = pgn_model(ctx, s, pg.FP64)
+    r = F.newton(**dict(OPTS, adaptive_tol=0))
+    assert r["result"] == "SUCCESS", r
+    x = F.get_x().reshape(-1, 3)
+    F.close()
+    return x
+
+
+def test_inexact_newton_alone_matches_fixed_tolerance(ctx, beams, fixed_solution):
+    _say("forcing: 9x6x61 StVK beam alone")
+    s = beams[1]
+    M = pgn_model(ctx, s, pg.MIXED)
+    r = M.newton(**dict(OPTS, adaptive_tol=1))
+    _say(f"  {r}")
+    assert r["result"] == "SUCCESS", r
+    assert np.abs(M.get_x().reshape(-1, 3) - fixed_solution).max() <= 1e-6 * s.bbox_diag()
+    M.close()
+
+
+def test_inexact_newton_in_batch_matches_fixed_tolerance(ctx, beams, fixed_solution):
+    _say("forcing: 9x6x61 StVK beam in a batch")
+    B = pg.Batch(ctx, pg.MIXED)
+    for s in beams:
+        B.add_scene(s)
+    B.build()
+    res = B.newton(**dict(OPTS, adaptive_tol=1))
+    _say(f"  {res}")
+    assert all(r["result"] == "SUCCESS" for r in res), res
+    s = beams[1]
+    assert np.abs(B.get_x(1) - fixed_solution).max() <= 1e-6 * s.bbox_diag()
+    B.close()
