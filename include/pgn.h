@@ -38,6 +38,16 @@ extern "C" {
 #define PGN_ERR_NOMEM (-3)    /* device or host allocation failed */
 #define PGN_ERR_NODEVICE (-4) /* no HIP device / extension unusable */
 
+/* linear solvers, values of LSSolverType (BasicTypes.h:36-46) */
+enum pgn_ls_solver {
+  PGN_LS_EIGENCG = 1,   /* LinearSolver_EigenCG: (block-)Jacobi PCG (default here) */
+  PGN_LS_EIGENLDLT = 2  /* LinearSolver_EigenLDLT: FP64 LDL^T, the reference's
+                           default (OptimSolver.h:58); libpgn factors in a
+                           reverse Cuthill-McKee band order and holds a
+                           half-bandwidth of at most 139 DoFs (one workgroup's
+                           LDS): small models */
+};
+
 /* element material (MaterialModels.h:166 StVK, :235 CoNH) */
 enum pgn_material { PGN_CONH = 0, PGN_STVK = 1 };
 
@@ -156,19 +166,27 @@ typedef struct {
   int reg_iters;   /* LinearSolverOptions::regIters                       */
   int precond;     /* 0 = Jacobi (reference), 1 = 3x3 block Jacobi         */
   double inner_tol;/* MIXED only: FP32 inner relative tolerance per IR sweep */
+  int solver;      /* LinearSolverOptions::type: PGN_LS_EIGENCG (default) or
+                      PGN_LS_EIGENLDLT; with LDL^T, tol is maxError of the
+                      exactness check and max_iters, precond, inner_tol are
+                      not used                                             */
 } pgn_linear_opts;
 
 typedef struct {
   int status;      /* 0 = SUCCESS, 1 = FAILURE (LSResult)                  */
-  int iterations;  /* PCG iterations over all regularization tries         */
+  int iterations;  /* PCG iterations over all regularization tries (0: LDL^T) */
   int reg_used;    /* regularization try that succeeded                   */
   int refinements; /* MIXED: outer FP64 refinement sweeps                  */
   double rel_residual;
 } pgn_linear_result;
 
 void pgn_linear_default_opts(pgn_linear_opts* o);
-/* LinearSolver_EigenCG::SolveInternal (LinearSolver_EigenCG.cpp:51-125) on the
- * Hessian assembled at the current x: solves H x = b, x is the warm start. */
+/* LinearSolver_EigenCG::SolveInternal (LinearSolver_EigenCG.cpp:51-125) or,
+ * with solver = PGN_LS_EIGENLDLT, LinearSolver_EigenLDLT::SolveInternal
+ * (LinearSolver_EigenLDLT.cpp:49-153) on the Hessian assembled at the
+ * current x: solves H x = b, x is the warm start (CG) or kept until a
+ * factorization succeeds (LDL^T).  A model whose band exceeds the direct
+ * solver's limit gets PGN_ERR_INVALID. */
 int pgn_linear_solve(pgn_model* model, const double* b, double* x,
                      const pgn_linear_opts* opts, pgn_linear_result* res);
 
@@ -206,6 +224,10 @@ typedef struct {
                          of the total, when |g| decreases                   */
   double ew_gamma;    /* Eisenstat-Walker gamma (default 0.9)              */
   double ew_max;      /* largest forcing term (default 0.2)                */
+  /* OptimSolverOptions::lsSolverType (OptimSolver.h:40): PGN_LS_EIGENCG
+     (default here) or PGN_LS_EIGENLDLT (the reference's default; linear
+     solves are then exact and adaptive_tol is not used)                  */
+  int lsSolverType;
 } pgn_newton_opts;
 
 /* OSResult (BasicTypes.h:69-77) */

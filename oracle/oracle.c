@@ -841,6 +841,20 @@ int orc_linear_solve(int n, const long* rp, const int* ci, double* v,
   return status;
 }
 
+int orc_sparse_ldlt_solve(int n, const long* rp, const int* ci, const double* v,
+                          const double* b, double* x, double maxError, int regIters,
+                          int* reg_used) {
+  double nA = 0;
+  for (long k = 0; k < rp[n]; ++k) nA += v[k] * v[k];
+  double* A = (double*)calloc((size_t)n * n, sizeof(double));
+  for (int r0 = 0; r0 < n; ++r0)
+    for (long k = rp[r0]; k < rp[r0 + 1]; ++k)
+      if (ci[k] <= r0) A[(long)r0 * n + ci[k]] = v[k];
+  const int st = orc_ldlt_solve_norm(n, A, sqrt(nA), b, x, maxError, regIters, reg_used);
+  free(A);
+  return st;
+}
+
 /* ======================================================================= */
 /* Newton (OptimSolver + OptimSolver_USQP_LS + OptimProblem_BasicStatic)    */
 /* ======================================================================= */
@@ -861,6 +875,7 @@ void orc_newton_default_opts(orc_newton_opts* o) {
   o->lsearch_numTry = 32;
   o->lSearch_alpha = 0.5;
   o->ls_noise_guard = 0.0;
+  o->ls_solver = 1;
 }
 
 static double norm2(int n, const double* a) { return sqrt(dot(n, a, a)); }
@@ -1022,6 +1037,11 @@ static int tp_solve(void* p, const double* b, double* dx, const orc_newton_opts*
   tet_prob* T = (tet_prob*)p;
   orc_model_hessian_lower(T->M, T->rp, T->ci, T->val);
   int its = 0, reg = 0;
+  if (o->ls_solver == 2) {
+    orc_sparse_ldlt_solve(3 * T->M->n, T->rp, T->ci, T->val, b, dx, o->lsTolMaxError,
+                          o->lsNumRegTrials, &reg);
+    return 0;
+  }
   orc_linear_solve(3 * T->M->n, T->rp, T->ci, T->val, b, dx, o->lsTolMaxError,
                    o->lsNumMaxIters, o->lsNumRegTrials, &its, &reg);
   return its;
@@ -1150,6 +1170,11 @@ static int dp_solve(void* p, const double* b, double* dx, const orc_newton_opts*
   orc_dynamic* D = (orc_dynamic*)p;
   orc_dynamic_hessian_lower(D, D->rp, D->ci, D->val);
   int its = 0, reg = 0;
+  if (o->ls_solver == 2) {
+    orc_sparse_ldlt_solve(D->N, D->rp, D->ci, D->val, b, dx, o->lsTolMaxError,
+                          o->lsNumRegTrials, &reg);
+    return 0;
+  }
   orc_linear_solve(D->N, D->rp, D->ci, D->val, b, dx, o->lsTolMaxError, o->lsNumMaxIters,
                    o->lsNumRegTrials, &its, &reg);
   return its;

@@ -136,6 +136,10 @@ typedef struct {
    * energy| + |load energy|, the scale of the FP64 rounding of the total) when
    * the gradient norm decreases. */
   double ls_noise_guard;
+  /* OptimSolverOptions::lsSolverType (OptimSolver.h:40,58) with the values of
+   * LSSolverType (BasicTypes.h:36-46): 1 = EigenCG (Jacobi CG, the default
+   * here), 2 = EigenLDLT (the reference's own default) */
+  int ls_solver;
 } orc_newton_opts;
 
 /* OSResult (BasicTypes.h:69-77) */
@@ -263,6 +267,14 @@ void orc_rod_hessian(orc_rod* R, double* H); /* dense N x N */
 void orc_rod_update_kinematics(orc_rod* R);
 int orc_dense_ldlt_solve(int N, double* A, const double* b, double* x,
                          double maxError, int regIters, int* reg_used);
+int orc_ldlt_solve_norm(int N, double* A, double normA, const double* b, double* x,
+                        double maxError, int regIters, int* reg_used);
+/* LinearSolver_EigenLDLT on the lower CSR Hessian of orc_model_hessian_lower
+ * (normA over the stored entries, AMatrixSd's set; solved densely, small
+ * models only).  Returns 0 on SUCCESS. */
+int orc_sparse_ldlt_solve(int n, const long* rowptr, const int* col, const double* val,
+                          const double* b, double* x, double maxError, int regIters,
+                          int* reg_used);
 void orc_rod_newton(orc_rod* R, const orc_newton_opts* o, orc_newton_result* r);
 
 #ifdef __cplusplus

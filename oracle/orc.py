@@ -36,7 +36,8 @@ class NewtonOpts(ctypes.Structure):
                 ("lsNumMaxIters", ctypes.c_int),
                 ("lsTolMaxError", ctypes.c_double),
                 ("lsNumRegTrials", ctypes.c_int),
-                ("ls_noise_guard", ctypes.c_double)]
+                ("ls_noise_guard", ctypes.c_double),
+                ("ls_solver", ctypes.c_int)]
 
 
 class NewtonResult(ctypes.Structure):
@@ -157,6 +158,9 @@ def lib():
         L.orc_dense_ldlt_solve.restype = ctypes.c_int
         L.orc_dense_ldlt_solve.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
                                            ctypes.c_int, _ip]
+        L.orc_sparse_ldlt_solve.restype = ctypes.c_int
+        L.orc_sparse_ldlt_solve.argtypes = [ctypes.c_int, _lp, _ip, _dp, _dp, _dp,
+                                            ctypes.c_double, ctypes.c_int, _ip]
         _LIB = L
     return _LIB
 
@@ -518,6 +522,19 @@ def dense_ldlt_solve(A, b, x, max_error, reg_iters):
     reg = ctypes.c_int(0)
     st = lib().orc_dense_ldlt_solve(A.shape[0], _p(A), _p(f64(b)), _p(x), max_error,
                                     int(reg_iters), ctypes.byref(reg))
+    return st, reg.value
+
+
+def sparse_ldlt_solve(rowptr, col, val, b, x, max_error, reg_iters):
+    """LinearSolver_EigenLDLT::SolveInternal on a lower CSR matrix as
+    orc_model_hessian_lower stores it (normA over every stored entry).
+    Returns (status 0 = SUCCESS, regularization try used); x is written."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(col, dtype=np.int32)
+    reg = ctypes.c_int(0)
+    st = lib().orc_sparse_ldlt_solve(rp.size - 1, _p(rp, _lp), _p(ci, _ip), _p(f64(val)),
+                                     _p(f64(b)), _p(x), max_error, int(reg_iters),
+                                     ctypes.byref(reg))
     return st, reg.value
 
 

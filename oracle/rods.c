@@ -544,7 +544,19 @@ int orc_dense_ldlt_solve(int N, double* A, const double* b, double* x, double ma
   double nA = 0.0;
   for (int i = 0; i < N; ++i)
     for (int j = 0; j <= i; ++j) nA += A[(long)i * N + j] * A[(long)i * N + j];
-  nA = sqrt(nA);
+  return orc_ldlt_solve_norm(N, A, sqrt(nA), b, x, maxError, regIters, reg_used);
+}
+
+/* The same with normA given: LinearSolver::GetMatrixData (LinearSolver.cpp:
+ * 57-66) takes the Frobenius norm of the matrix as stored, sinT = max(min(
+ * 1e-6 normA, 1e-6), 1e-9), regT = 1e-6 normA; the tries of
+ * LinearSolver_EigenLDLT.cpp:63-151: cumulative shift regT 10^(i-1) I, a
+ * zero or |D_j| < sinT pivot (factorize failure, singular check) or
+ * normalized(b).normalized(x) < -regT (indefinite) or |b - A x| above
+ * maxError both absolutely and relatively (inexact) moves to the next try;
+ * x keeps its input until a factorization succeeds. */
+int orc_ldlt_solve_norm(int N, double* A, double nA, const double* b, double* x,
+                        double maxError, int regIters, int* reg_used) {
   double sinT = 1e-6 * nA;
   if (sinT > 1e-6) sinT = 1e-6;
   if (sinT < 1e-9) sinT = 1e-9;

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libpgn.so")
 
 CONH, STVK = 0, 1
 FP64, MIXED = 0, 1
+LS_EIGENCG, LS_EIGENLDLT = 1, 2  # LSSolverType values (BasicTypes.h:36-46)
 LAYOUT_LOWER, LAYOUT_FULL, LAYOUT_BSR3 = 0, 1, 2
 # Newton options of a solve "to residual 1e-8": the reference's MINIMPR /
 # MINSTEP exits and its energy-only line search would stop short of it
@@ -63,7 +64,7 @@ _vp = ctypes.c_void_p
 class LinearOpts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iters", ctypes.c_int),
                 ("reg_iters", ctypes.c_int), ("precond", ctypes.c_int),
-                ("inner_tol", ctypes.c_double)]
+                ("inner_tol", ctypes.c_double), ("solver", ctypes.c_int)]
 
 
 class LinearResult(ctypes.Structure):
@@ -82,7 +83,7 @@ class NewtonOpts(ctypes.Structure):
                 ("lsNumRegTrials", ctypes.c_int), ("precond", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("adaptive_tol", ctypes.c_int),
                 ("ls_noise_guard", ctypes.c_double), ("ew_gamma", ctypes.c_double),
-                ("ew_max", ctypes.c_double)]
+                ("ew_max", ctypes.c_double), ("lsSolverType", ctypes.c_int)]
 
 
 class ServerOpts(ctypes.Structure):

@@ -473,6 +473,7 @@ void pgn_linear_default_opts(pgn_linear_opts* o) {
   o->reg_iters = 10;    // LinearSolverOptions::regIters
   o->precond = 0;
   o->inner_tol = 1e-5;
+  o->solver = PGN_LS_EIGENCG;
 }
 
 int pgn_linear_solve(pgn_model* M, const double* b, double* x, const pgn_linear_opts* opts,
@@ -484,6 +485,8 @@ int pgn_linear_solve(pgn_model* M, const double* b, double* x, const pgn_linear_
     pgn_linear_default_opts(&o);
     if (opts) o = *opts;
     PGN_CHECK(o.max_iters >= 0 && o.tol >= 0.0, "bad linear options");
+    PGN_CHECK(o.solver == PGN_LS_EIGENCG || o.solver == PGN_LS_EIGENLDLT,
+              "linear solver: PGN_LS_EIGENCG or PGN_LS_EIGENLDLT");
     M->eval_hessian();
     const long N = 3L * M->n;
     M->nw_b.alloc(N);
@@ -519,6 +522,7 @@ void pgn_newton_default_opts(pgn_newton_opts* o) {
   o->ls_noise_guard = 0.0;
   o->ew_gamma = 0.9;
   o->ew_max = 0.2;
+  o->lsSolverType = PGN_LS_EIGENCG;
 }
 
 int pgn_newton_solve(pgn_model* M, const pgn_newton_opts* opts, pgn_newton_result* res) {

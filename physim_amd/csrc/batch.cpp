@@ -419,6 +419,7 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
 }
 
 void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
+  PGN_CHECK(o.lsSolverType == PGN_LS_EIGENCG, "batches solve with PGN_LS_EIGENCG");
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
   const int Kn = K();

@@ -282,4 +282,40 @@ void launch_widen(hipStream_t s, bool fp32, long n, const void* a, double* b);
 void launch_narrow_or_copy(hipStream_t s, bool fp32, long n, const double* a,
                            void* b);
 
+// ---- FP64 banded LDL^T (the EigenLDLT linear solver) ------------------------
+// One factorization per problem (a model, or one instance of a batch), in a
+// band order of its nodes: DoF I = 3 q + a for the node at position q.  Its
+// band holds N rows of bw + 1 doubles, row I = entries (I, I - d), d = 0..bw:
+// the lower band of the matrix before factorization, L below the diagonal
+// after (the diagonal keeps A; D is stored apart).
+constexpr int LDLT_MAXW = 140;  // bw + 1 at most: the LDS window is W x W doubles
+struct LdltProb {
+  long band0;  // offset of the band (doubles)
+  int dof0;    // offset of the problem's DoFs in D and the solve scratch
+  int N;       // DoFs
+  int bw;      // half-bandwidth in DoFs (3 x node half-bandwidth + 2)
+  int node0;   // first node of the problem in the model's node numbering
+};
+struct LdltArgs {
+  const LdltProb* prob;
+  const int* perm;     // node -> node0 + its band position
+  const int* iperm;    // node0 + band position -> node
+  const int* nprob;    // node -> problem
+  double* band;
+  double* D;           // pivots, band order
+  const double* sinT;  // per problem: pivots below it are singular
+  int* status;         // per problem {first zero pivot or -1, pivots below sinT}
+};
+// zero the bands of the listed problems, then copy the lower BSR3 entries of
+// problems with pmask set (all when pmask is null) into their bands
+void launch_ldlt_gather(hipStream_t s, bool fp32, const LdltArgs& a, const int* list,
+                        int nlist, long max_band, long nb, const int* brow, const int* bcol,
+                        const void* bval, const uint8_t* pmask);
+// factor the listed problems, one workgroup each
+void launch_ldlt_factor(hipStream_t s, const LdltArgs& a, const int* list, int nlist);
+// x = A^-1 b for the listed problems (b, x: the model's node-major DoF
+// vectors); y: scratch of the problems' DoF count
+void launch_ldlt_solve(hipStream_t s, const LdltArgs& a, const int* list, int nlist,
+                       const double* b, double* x, double* y);
+
 }  // namespace pgn

@@ -3,6 +3,8 @@
 // HBM and only scalars (energy, norms, PCG status) come back to the host.
 #include "solver.hpp"
 
+#include "ldlt.hpp"
+
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -118,6 +120,10 @@ static void solve_once(pgn_model& M, const double* b, double* x, const pgn_linea
 // regularize A += regT 10^(i-1) I (cumulative) while b.x < 0.
 void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opts& o,
                   pgn_linear_result& res) {
+  if (o.solver == PGN_LS_EIGENLDLT) {
+    ldlt_linear_solve(M, b, x, o, res);
+    return;
+  }
   PGN_CHECK(M.hess_valid, "Hessian not assembled");
   M.tune();
   hipStream_t s = M.stream();
@@ -172,6 +178,11 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
   lo.max_iters = o.lsNumMaxIters;
   lo.precond = o.precond;
   lo.inner_tol = o.inner_tol;
+  PGN_CHECK(o.lsSolverType == PGN_LS_EIGENCG || o.lsSolverType == PGN_LS_EIGENLDLT,
+            "lsSolverType: PGN_LS_EIGENCG or PGN_LS_EIGENLDLT");
+  lo.solver = o.lsSolverType;
+  const bool direct = o.lsSolverType == PGN_LS_EIGENLDLT;
+  if (direct) M.ldlt_prepare();  // a band beyond the limit fails before any step
   R = pgn_newton_result{};
 
   auto d2d = [&](double* dst, const double* src) {
@@ -223,7 +234,7 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
       launch_lincomb64(s, N, 1.0, xsave, -1.0, v0, dx);  // dx0 = x - x_prev (:488)
       launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, bvec);
       lo.tol = o.lsTolMaxError;
-      if (o.adaptive_tol && !ew_off) {
+      if (o.adaptive_tol && !ew_off && !direct) {
         if (g_prev > 0.0) {
           const double r = opt / g_prev;
           double e = o.ew_gamma * r * r;
