@@ -42,10 +42,11 @@ extern "C" {
 enum pgn_ls_solver {
   PGN_LS_EIGENCG = 1,   /* LinearSolver_EigenCG: (block-)Jacobi PCG (default here) */
   PGN_LS_EIGENLDLT = 2  /* LinearSolver_EigenLDLT: FP64 LDL^T, the reference's
-                           default (OptimSolver.h:58); libpgn factors in a
-                           reverse Cuthill-McKee band order and holds a
-                           half-bandwidth of at most 139 DoFs (one workgroup's
-                           LDS): small models */
+                           default (OptimSolver.h:58); libpgn factors in the
+                           narrowest of three node orders (as given, reverse
+                           Cuthill-McKee, sorted along the longest axis) and
+                           holds a half-bandwidth of at most 139 DoFs (one
+                           workgroup's LDS): small models */
 };
 
 /* element material (MaterialModels.h:166 StVK, :235 CoNH) */

@@ -17,6 +17,7 @@
 #include <limits>
 #include <new>
 
+#include "ldlt.hpp"
 #include "solver.hpp"
 
 using namespace pgn;
@@ -316,6 +317,11 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
   if (xprev.n < (size_t)N) xprev.alloc(N);
   if (kvec.n < (size_t)Kn) kvec.alloc(Kn);
   if (kvec2.n < (size_t)Kn) kvec2.alloc(Kn);
+  if (o.solver == PGN_LS_EIGENLDLT) {
+    iters.assign(Kn, 0);
+    ldlt_solve(b, x, active, tol, o);
+    return;
+  }
   PGN_HIP(hipMemcpyAsync(xprev.p, x, N * sizeof(double), hipMemcpyDeviceToDevice, s));
   iters.assign(Kn, 0);
   std::vector<uint8_t> F = active;
@@ -419,7 +425,10 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
 }
 
 void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
-  PGN_CHECK(o.lsSolverType == PGN_LS_EIGENCG, "batches solve with PGN_LS_EIGENCG");
+  PGN_CHECK(o.lsSolverType == PGN_LS_EIGENCG || o.lsSolverType == PGN_LS_EIGENLDLT,
+            "lsSolverType: PGN_LS_EIGENCG or PGN_LS_EIGENLDLT");
+  const bool direct = o.lsSolverType == PGN_LS_EIGENLDLT;
+  if (direct) ldlt_prepare();  // an instance beyond the band limit fails before any step
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
   const int Kn = K();
@@ -431,6 +440,7 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
   lo.max_iters = o.lsNumMaxIters;
   lo.precond = o.precond;
   lo.inner_tol = o.inner_tol;
+  lo.solver = o.lsSolverType;
   enum { ACTIVE = 0, DONE = 1 };
   std::vector<int> st(Kn, ACTIVE), result(Kn, PGN_OS_SUCCESS), minImpr(Kn, 0), minStep(Kn, 0),
       steep(Kn, 0), its(Kn, 0), bis(Kn, 0), evals(Kn, 0), hevals(Kn, 0), pcg(Kn, 0),
@@ -484,7 +494,7 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
         need_h = true;
         hevals[k]++;
         double lt = o.lsTolMaxError;
-        if (o.adaptive_tol && !ew_off[k]) {  // Eisenstat-Walker choice 2 per instance
+        if (o.adaptive_tol && !ew_off[k] && !direct) {  // Eisenstat-Walker choice 2
           if (gprev[k] > 0.0) {
             const double r = opt[k] / gprev[k];
             double e = o.ew_gamma * r * r;
@@ -657,6 +667,136 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
     R[k].objective = obj[k];
     R[k].optimality = opt[k];
     R[k].seconds = secs;
+  }
+}
+
+// ---- EigenLDLT per instance ------------------------------------------------------
+
+void pgn_batch::ldlt_prepare() {
+  if (ldlt_ready) return;
+  const int Kn = K();
+  const int n = M->n;
+  std::vector<int> perm(n), iperm(n);
+  std::vector<LdltProb> P(Kn);
+  long band0 = 0;
+  ldlt_maxband = 0;
+  for (int k = 0; k < Kn; ++k) {
+    const int n0 = (int)hnoff[k], n1 = (int)hnoff[k + 1];
+    const int bw = ldlt_plan(n0, n1, M->pat.rowptr, M->pat.bcol, M->hX0.data(), perm, iperm);
+    PGN_CHECK(bw + 1 <= LDLT_MAXW,
+              "EigenLDLT: instance " + std::to_string(k) + " has a half-bandwidth of " +
+                  std::to_string(bw) + " DoFs after reordering; the direct solver holds at most " +
+                  std::to_string(LDLT_MAXW - 1) + " (use EigenCG)");
+    P[k].band0 = band0;
+    P[k].dof0 = 3 * n0;
+    P[k].N = 3 * (n1 - n0);
+    P[k].bw = bw;
+    P[k].node0 = n0;
+    const long len = (long)P[k].N * (bw + 1);
+    band0 += len;
+    ldlt_maxband = std::max(ldlt_maxband, len);
+  }
+  hipStream_t s = ctx->stream;
+  ldlt_prob.upload(P, s);
+  ldlt_perm.upload(perm, s);
+  ldlt_iperm.upload(iperm, s);
+  ldlt_list.alloc(Kn);
+  ldlt_status.alloc(2L * Kn);
+  ldlt_band.alloc(std::max<long>(band0, 1));
+  ldlt_D.alloc(3L * n);
+  ldlt_y.alloc(3L * n);
+  ldlt_sinT.alloc(Kn);
+  PGN_HIP(hipStreamSynchronize(s));
+  ldlt_ready = true;
+}
+
+LdltArgs pgn_batch::ldlt_args() const {
+  LdltArgs a;
+  a.prob = ldlt_prob.p;
+  a.perm = ldlt_perm.p;
+  a.iperm = ldlt_iperm.p;
+  a.nprob = node_inst.p;
+  a.band = ldlt_band.p;
+  a.D = ldlt_D.p;
+  a.sinT = ldlt_sinT.p;
+  a.status = ldlt_status.p;
+  return a;
+}
+
+// The tries of ldlt_linear_solve (LinearSolver_EigenLDLT.cpp:49-153) for
+// every active instance at once: per-instance normA (its stored lower
+// entries), shifts, pivot checks, indefiniteness and exactness checks; the
+// factorizations of one try run side by side, one workgroup per instance.
+void pgn_batch::ldlt_solve(const double* b, double* x, const std::vector<uint8_t>& active,
+                           const std::vector<double>& tol, const pgn_linear_opts& o) {
+  ldlt_prepare();
+  hipStream_t s = ctx->stream;
+  const int Kn = K();
+  const long n = M->n;
+  const BatchArgs ba = args();
+  const LdltArgs la = ldlt_args();
+  if (kvec.n < (size_t)Kn) kvec.alloc(Kn);
+  if (kvec2.n < (size_t)Kn) kvec2.alloc(Kn);
+  auto download_k = [&](const double* d, std::vector<double>& out) {
+    out.resize(Kn);
+    PGN_HIP(hipMemcpyAsync(out.data(), d, Kn * sizeof(double), hipMemcpyDeviceToHost, s));
+    PGN_HIP(hipStreamSynchronize(s));
+  };
+  std::vector<double> h, bb, xx, bx, rr, sinT(Kn), regT(Kn), shift(Kn);
+  launch_batch_frob(s, M->hfp32, ba, M->rowptr.p, M->brow.p, M->bcol.p, M->bval.p, kvec.p);
+  download_k(kvec.p, h);
+  for (int k = 0; k < Kn; ++k) {
+    const double nA = std::sqrt(h[k]);
+    sinT[k] = std::max(std::min(1e-6 * nA, 1e-6), 1e-9);
+    regT[k] = 1e-6 * nA;
+  }
+  ldlt_sinT.upload(sinT.data(), Kn, s);
+  launch_batch_dot(s, ba, b, b, kvec.p);
+  download_k(kvec.p, bb);
+  std::vector<uint8_t> F = active;
+  std::vector<int> list, st(2 * Kn);
+  const int regs = std::max(o.reg_iters, 1);
+  for (int it = 0; it < regs; ++it) {
+    list.clear();
+    for (int k = 0; k < Kn; ++k)
+      if (F[k]) list.push_back(k);
+    if (list.empty()) break;
+    if (it != 0) {
+      for (int k = 0; k < Kn; ++k) shift[k] = F[k] ? regT[k] * std::pow(10.0, it - 1) : 0.0;
+      kvec2.upload(shift.data(), Kn, s);
+      launch_batch_add_diag(s, M->hfp32, ba, n, M->diag.p, M->bval.p, kvec2.p);
+    }
+    PGN_HIP(hipMemcpyAsync(ldlt_list.p, list.data(), list.size() * sizeof(int),
+                           hipMemcpyHostToDevice, s));
+    mask.upload(F, s);
+    launch_ldlt_gather(s, M->hfp32, la, ldlt_list.p, (int)list.size(), ldlt_maxband, M->pat.nb,
+                       M->brow.p, M->bcol.p, M->bval.p, mask.p);
+    launch_ldlt_factor(s, la, ldlt_list.p, (int)list.size());
+    ldlt_status.download(st.data(), 2L * Kn, s);
+    PGN_HIP(hipStreamSynchronize(s));
+    std::vector<int> ok;
+    for (int k : list)
+      if (st[2 * k] < 0 && st[2 * k + 1] == 0) ok.push_back(k);  // else: zero / singular pivot
+    if (ok.empty()) continue;
+    PGN_HIP(hipMemcpyAsync(ldlt_list.p, ok.data(), ok.size() * sizeof(int),
+                           hipMemcpyHostToDevice, s));
+    launch_ldlt_solve(s, la, ldlt_list.p, (int)ok.size(), b, x, ldlt_y.p);
+    launch_batch_dot(s, ba, x, x, kvec.p);
+    download_k(kvec.p, xx);
+    launch_batch_dot(s, ba, b, x, kvec.p);
+    download_k(kvec.p, bx);
+    launch_residual64(s, M->hfp32, (int)n, M->rowptr.p, M->bcol.p, M->bval.p, x, b, M->r64.p,
+                      M->red64.p, M->scal.p + 4);
+    launch_batch_norm2(s, ba, M->r64.p, kvec.p);
+    download_k(kvec.p, rr);
+    for (int k : ok) {
+      const double nb = std::sqrt(bb[k]), nx = std::sqrt(xx[k]);
+      if (nb > 0.0 && nx > 0.0 && bx[k] / (nb * nx) < -regT[k]) continue;  // indefinite
+      const double ae = std::sqrt(rr[k]);
+      const double re = nb > 0.0 ? ae / nb : std::numeric_limits<double>::quiet_NaN();
+      if (ae > tol[k] && re > tol[k]) continue;  // inexact
+      F[k] = 0;
+    }
   }
 }
 

@@ -72,4 +72,17 @@ struct pgn_batch {
                     const std::vector<double>& tol, const pgn_linear_opts& o,
                     std::vector<int>& iters);
   void newton(const pgn_newton_opts& o, pgn_newton_result* R);
+
+  // EigenLDLT: one banded LDL^T per instance, one workgroup each (band
+  // orders planned on first use)
+  bool ldlt_ready = false;
+  long ldlt_maxband = 0;
+  pgn::DBuf<pgn::LdltProb> ldlt_prob;
+  pgn::DBuf<int> ldlt_perm, ldlt_iperm, ldlt_list, ldlt_status;
+  pgn::DBuf<double> ldlt_band, ldlt_D, ldlt_sinT, ldlt_y;
+  void ldlt_prepare();
+  pgn::LdltArgs ldlt_args() const;
+  // LinearSolver_EigenLDLT::SolveInternal per active instance
+  void ldlt_solve(const double* b, double* x, const std::vector<uint8_t>& active,
+                  const std::vector<double>& tol, const pgn_linear_opts& o);
 };

@@ -95,17 +95,58 @@ void rcm_order(int n0, int n1, const std::vector<long>& rp, const std::vector<in
 }
 
 int ldlt_plan(int n0, int n1, const std::vector<long>& rp, const std::vector<int>& bc,
-              std::vector<int>& perm, std::vector<int>& iperm) {
-  std::vector<int> order;
-  rcm_order(n0, n1, rp, bc, order);
-  int bn = 0;
-  for (int q = 0; q < n1 - n0; ++q) {
-    perm[order[q]] = n0 + q;
-    iperm[n0 + q] = order[q];
+              const double* X, std::vector<int>& perm, std::vector<int>& iperm) {
+  const int n = n1 - n0;
+  // node half-bandwidth of an order (order[q] = node at band position q)
+  std::vector<int> pos(n);
+  auto band_of = [&](const std::vector<int>& order) {
+    for (int q = 0; q < n; ++q) pos[order[q] - n0] = q;
+    int bn = 0;
+    for (int i = n0; i < n1; ++i)
+      for (long e = rp[i]; e < rp[i + 1]; ++e)
+        bn = std::max(bn, std::abs(pos[i - n0] - pos[bc[e] - n0]));
+    return bn;
+  };
+  std::vector<std::vector<int>> cand;
+  // the given node order (mesh generators number nodes section by section)
+  cand.emplace_back(n);
+  for (int q = 0; q < n; ++q) cand[0][q] = n0 + q;
+  // reverse Cuthill-McKee
+  cand.emplace_back();
+  rcm_order(n0, n1, rp, bc, cand[1]);
+  // rest positions sorted along the longest box axis, then the others
+  if (X && n > 0) {
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) lo[c] = hi[c] = X[3L * n0 + c];
+    for (int i = n0; i < n1; ++i)
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::min(lo[c], X[3L * i + c]);
+        hi[c] = std::max(hi[c], X[3L * i + c]);
+      }
+    int ax[3] = {0, 1, 2};
+    std::sort(ax, ax + 3, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
+    std::vector<int> o(n);
+    for (int q = 0; q < n; ++q) o[q] = n0 + q;
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) {
+      for (int c : ax)
+        if (X[3L * a + c] != X[3L * b + c]) return X[3L * a + c] < X[3L * b + c];
+      return false;
+    });
+    cand.push_back(std::move(o));
   }
-  for (int i = n0; i < n1; ++i)
-    for (long e = rp[i]; e < rp[i + 1]; ++e) bn = std::max(bn, std::abs(perm[i] - perm[bc[e]]));
-  return 3 * bn + 2;
+  int best = 0, bbn = band_of(cand[0]);
+  for (size_t c = 1; c < cand.size(); ++c) {
+    const int bn = band_of(cand[c]);
+    if (bn < bbn) {
+      bbn = bn;
+      best = (int)c;
+    }
+  }
+  for (int q = 0; q < n; ++q) {
+    perm[cand[best][q]] = n0 + q;
+    iperm[n0 + q] = cand[best][q];
+  }
+  return 3 * bbn + 2;
 }
 
 }  // namespace pgn
@@ -115,7 +156,7 @@ using namespace pgn;
 void pgn_model::ldlt_prepare() {
   if (ldlt_ready) return;
   std::vector<int> perm(n), iperm(n);
-  const int bw = ldlt_plan(0, n, pat.rowptr, pat.bcol, perm, iperm);
+  const int bw = ldlt_plan(0, n, pat.rowptr, pat.bcol, hX0.data(), perm, iperm);
   PGN_CHECK(bw + 1 <= LDLT_MAXW,
             "EigenLDLT: the half-bandwidth after reordering is " + std::to_string(bw) +
                 " DoFs; the direct solver holds at most " + std::to_string(LDLT_MAXW - 1) +

@@ -25,6 +25,20 @@ namespace pgn {
 namespace {
 
 constexpr int LDLT_FBS = 1024;  // factorization workgroup
+
+// Workgroup barrier that orders LDS only: global loads issued before it stay
+// in flight (a full __syncthreads waits for them).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+// The same for a one-wavefront workgroup: LDS operations of a wavefront
+// complete in order, so only the compiler's ordering is needed.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+}
 // triangle pairs (ti >= tj) of a (W-1) x (W-1) lower triangle per thread
 constexpr int LDLT_Q = ((LDLT_MAXW - 1) * LDLT_MAXW / 2 + LDLT_FBS - 1) / LDLT_FBS;
 
@@ -93,6 +107,8 @@ __global__ __launch_bounds__(LDLT_FBS) void ldlt_factor_kernel(LdltArgs A,
     const int r = f / W;
     win[f] = r < N ? band[(long)f] : 0.0;
   }
+  // the row that refills the first retired slot, loaded ahead
+  double pre = (tid < W && W < N) ? band[(long)W * W + tid] : 0.0;
   __syncthreads();
   int zero_at = -1, nsing = 0, sk = 0;  // sk = k mod W
   for (int k = 0; k < N; ++k) {
@@ -113,7 +129,7 @@ __global__ __launch_bounds__(LDLT_FBS) void ldlt_factor_kernel(LdltArgs A,
       D[k] = d;
       if (!(fabs(d) >= sinT)) ++nsing;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < LDLT_Q; ++q) {
       const int ti = pk[q] >> 8, tj = pk[q] & 255;
@@ -122,13 +138,17 @@ __global__ __launch_bounds__(LDLT_FBS) void ldlt_factor_kernel(LdltArgs A,
       if (si >= W) si -= W;
       win[si * W + (ti - tj)] -= lv[ti] * d * lv[tj];
     }
-    __syncthreads();
-    // row k is done; its slot takes row k + W, which no earlier column touches
+    lds_barrier();
+    // row k is done; its slot takes row k + W, which no earlier column
+    // touches (its L entries are written from column k + 1 on); row k + W + 1
+    // is loaded for the next column
     const long rn = (long)k + W;
-    if (rn < N)
-      for (int t = tid; t < W; t += LDLT_FBS) win[sk * W + t] = band[rn * W + t];
+    if (rn < N && tid < W) {
+      win[sk * W + tid] = pre;
+      pre = rn + 1 < N ? band[(rn + 1) * W + tid] : 0.0;
+    }
     sk = sk + 1 == W ? 0 : sk + 1;
-    __syncthreads();
+    lds_barrier();
   }
   if (tid == 0) {
     A.status[2 * p] = zero_at;
@@ -152,20 +172,24 @@ __global__ __launch_bounds__(64) void ldlt_solve_kernel(LdltArgs A, const int* _
   const double* D = A.D + P.dof0;
   double* y = yscr + P.dof0;
   constexpr int S = (LDLT_MAXW + 63) / 64;  // band entries per lane and row
-  // forward: y_I = b_I - sum_{d=1..min(bw,I)} L(I, I-d) y_{I-d}; ring[I mod W] = y_I
+  // b in band order
+  for (int I = lane; I < N; I += 64) y[I] = b[ldlt_dof(A, P, I)];
+  __syncthreads();
+  // forward: y_I = b_I - sum_{d=1..min(bw,I)} L(I, I-d) y_{I-d}; ring[I mod W]
+  // = y_I.  Row I + 1's band entries and b value are loaded during row I.
   double cur[S];
 #pragma unroll
-  for (int q = 0; q < S; ++q) {
-    cur[q] = 0.0;  // row 0 has no subdiagonal entries
-  }
+  for (int q = 0; q < S; ++q) cur[q] = 0.0;  // row 0 has no subdiagonal entries
+  double bcur = y[0];
   int iw = 0;  // I mod W
   for (int I = 0; I < N; ++I) {
     double nxt[S];
 #pragma unroll
-    for (int q = 0; q < S; ++q) {  // the next row's band entries, loaded ahead
+    for (int q = 0; q < S; ++q) {
       const int d = 1 + lane + 64 * q;
       nxt[q] = (I + 1 < N && d <= bw && d <= I + 1) ? band[(long)(I + 1) * W + d] : 0.0;
     }
+    const double bnxt = I + 1 < N ? y[I + 1] : 0.0;
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < S; ++q) {
@@ -178,31 +202,41 @@ __global__ __launch_bounds__(64) void ldlt_solve_kernel(LdltArgs A, const int* _
     }
     acc = wave_sum(acc);
     if (lane == 0) {
-      const double yi = b[ldlt_dof(A, P, I)] - acc;
+      const double yi = bcur - acc;
       ring[iw] = yi;
       y[I] = yi;
     }
 #pragma unroll
     for (int q = 0; q < S; ++q) cur[q] = nxt[q];
+    bcur = bnxt;
     iw = iw + 1 == W ? 0 : iw + 1;
-    __syncthreads();
+    wave_lds_order();
   }
   // backward, right-looking: x_I = y_I / D_I - pend_I, then row I of L adds
-  // L(I, I-d) x_I to the pending sums of rows I-1 .. I-bw (ring slots)
+  // L(I, I-d) x_I to the pending sums of rows I-1 .. I-bw (ring slots); x
+  // goes to y[I] in band order
   for (int t = lane; t < W; t += 64) ring[t] = 0.0;
   __syncthreads();
   iw = (N - 1) % W;
+  double row[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    const int d = 1 + lane + 64 * q;
+    row[q] = (d <= bw && d <= N - 1) ? band[(long)(N - 1) * W + d] : 0.0;
+  }
+  double ycur = y[N - 1], dcur = D[N - 1];
   for (int I = N - 1; I >= 0; --I) {
-    double row[S];
+    double nrow[S];
 #pragma unroll
     for (int q = 0; q < S; ++q) {
       const int d = 1 + lane + 64 * q;
-      row[q] = (d <= bw && d <= I) ? band[(long)I * W + d] : 0.0;
+      nrow[q] = (I > 0 && d <= bw && d <= I - 1) ? band[(long)(I - 1) * W + d] : 0.0;
     }
-    const double xi = y[I] / D[I] - ring[iw];
-    __syncthreads();  // every lane has read ring[iw] before it is cleared
+    const double ynxt = I > 0 ? y[I - 1] : 0.0, dnxt = I > 0 ? D[I - 1] : 1.0;
+    const double xi = ycur / dcur - ring[iw];
+    wave_lds_order();  // every lane has read ring[iw] before it is cleared
     if (lane == 0) {
-      x[ldlt_dof(A, P, I)] = xi;
+      y[I] = xi;
       ring[iw] = 0.0;  // the slot now serves row I - W
     }
 #pragma unroll
@@ -214,9 +248,15 @@ __global__ __launch_bounds__(64) void ldlt_solve_kernel(LdltArgs A, const int* _
         ring[r] += row[q] * xi;
       }
     }
+#pragma unroll
+    for (int q = 0; q < S; ++q) row[q] = nrow[q];
+    ycur = ynxt;
+    dcur = dnxt;
     iw = iw == 0 ? W - 1 : iw - 1;
-    __syncthreads();
+    wave_lds_order();
   }
+  __syncthreads();
+  for (int I = lane; I < N; I += 64) x[ldlt_dof(A, P, I)] = y[I];
 }
 
 }  // namespace

@@ -98,8 +98,8 @@ struct pgn_model {
   double dyn_dt = 0.0;
   pgn::DBuf<double> vel, xhat, xstep0;  // 3n: velocity, predictor, x at step start
   bool dyn_v0_valid = false;  // nw_v0 carries the solver's previous x across steps
-  // EigenLDLT (FP64 banded LDL^T): band order of the nodes (reverse
-  // Cuthill-McKee), built on the first direct solve
+  // EigenLDLT (FP64 banded LDL^T): band order of the nodes, built on the
+  // first direct solve
   bool ldlt_ready = false;
   int ldlt_bw = 0;
   pgn::DBuf<pgn::LdltProb> ldlt_prob;

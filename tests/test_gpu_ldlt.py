@@ -159,3 +159,42 @@ def test_zero_hessian_fails_and_keeps_x(ctx):
     assert r["status"] == 1, r
     assert np.array_equal(x, x0)
     G.close()
+
+
+def test_batch_matches_single_and_oracle(ctx):
+    """A ragged batch (one tet, a beam, a cloth) solved with EigenLDLT, one
+    factorization workgroup per instance: each instance as alone and as the
+    oracle's EigenLDLT Newton."""
+    _say("ldlt batch")
+    rng = np.random.default_rng(7)
+    X0 = np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [0.0, 0.1, 0.0], [0.02, 0.03, 0.1]])
+    l1, l2 = orc.lame(1e5, 0.3)
+    fixed = np.zeros((4, 3), dtype=np.uint8)
+    fixed[:3] = 1
+    tet = scenes.Scene("one_tet", X0, np.array([[0, 1, 2, 3]], dtype=np.int32),
+                       np.array([0], dtype=np.uint8), np.array([l1]), np.array([l2]),
+                       np.array([1000.0]), fixed, scenes.G_EARTH, 5)
+    mix = [tet, scenes.batch_beam(rng, cell=0.01, dmin=(2, 2, 4), dmax=(3, 3, 6)),
+           scenes.cloth(5, size=0.2, stages=10)]
+    opts = dict(SOLVE, numMaxIters=200, lsSolverType=pg.LS_EIGENLDLT)
+    B = pg.Batch(ctx, pg.FP64)
+    for s in mix:
+        B.add_scene(s)
+    B.build()
+    res = B.newton(**opts)
+    _say(f"  {res}")
+    for k, s in enumerate(mix):
+        assert res[k]["result"] == "SUCCESS" and res[k]["pcg_iterations"] == 0, (k, res[k])
+        A = pg.Batch(ctx, pg.FP64)
+        A.add_scene(s)
+        A.build()
+        ra = A.newton(**opts)
+        assert ra[0]["result"] == "SUCCESS", (k, ra)
+        assert np.abs(A.get_x(0) - B.get_x(k)).max() <= 1e-9 * s.bbox_diag(), k
+        A.close()
+    for k in (0, 1):
+        O = oracle_model(mix[k])
+        ro = O.newton(**SOLVE, numMaxIters=200, ls_solver=2)
+        assert ro["result"] == "SUCCESS", (k, ro)
+        assert np.abs(B.get_x(k) - O.get_x().reshape(-1, 3)).max() <= 1e-6 * mix[k].bbox_diag()
+    B.close()
