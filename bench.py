@@ -126,7 +126,7 @@ def main():
                          "PCG solve capped at 50 iterations, 5 E+g+H launches and 5 "
                          "calibration copies; no solve timing, no JSON line")
     ap.add_argument("--workload", default="beam",
-                    choices=["beam", "batch", "rods", "serving", "dynamic"],
+                    choices=["beam", "batch", "rods", "serving", "dynamic", "tiny"],
                     help="beam (default): the 1.28M-tet beam; batch: 256 fused beams; "
                          "rods: 4096 helices; serving: Poisson stream of beams and cloths; "
                          "dynamic: implicit time steps of the 1.28M-tet beam")

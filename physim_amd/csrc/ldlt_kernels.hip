@@ -138,10 +138,10 @@ __global__ __launch_bounds__(LDLT_FBS) void ldlt_factor_kernel(LdltArgs A,
       if (si >= W) si -= W;
       win[si * W + (ti - tj)] -= lv[ti] * d * lv[tj];
     }
-    lds_barrier();
-    // row k is done; its slot takes row k + W, which no earlier column
-    // touches (its L entries are written from column k + 1 on); row k + W + 1
-    // is loaded for the next column
+    // row k is done (the update above touches rows k+1..k+m only): its slot
+    // takes row k + W, which no earlier column touches (its L entries are
+    // written from column k + 1 on); row k + W + 1 is loaded for the next
+    // column
     const long rn = (long)k + W;
     if (rn < N && tid < W) {
       win[sk * W + tid] = pre;
@@ -161,6 +161,102 @@ __device__ __forceinline__ long ldlt_dof(const LdltArgs& A, const LdltProb& P, i
   return 3L * A.iperm[P.node0 + I / 3] + I % 3;
 }
 
+constexpr int LDLT_S = (LDLT_MAXW + 63) / 64;  // band entries per lane and row
+
+// Row I's subdiagonal band entries L(I, I - d), d = 1 + lane + 64 q (zero
+// beyond the band or the matrix).  The loads are unconditional (clamped to
+// entry 0) so that the compiler can count them in flight.
+__device__ __forceinline__ void ldlt_load_row(const double* band, int W, int bw, int N, int I,
+                                              int lane, double (&v)[LDLT_S]) {
+#pragma unroll
+  for (int q = 0; q < LDLT_S; ++q) {
+    const int d = 1 + lane + 64 * q;
+    const bool ok = I >= 0 && I < N && d <= bw && d <= I;
+    const double t = band[ok ? (long)I * W + d : 0L];
+    v[q] = ok ? t : 0.0;
+  }
+}
+
+// a[i] for i clamped into [0, n)
+__device__ __forceinline__ double ldlt_ld(const double* a, int i, int n) {
+  return a[i < 0 ? 0 : (i >= n ? n - 1 : i)];
+}
+
+// DPP shift of a double within rows of 16 lanes (zero from outside the row)
+template <int CTRL>
+__device__ __forceinline__ double dpp_shr(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+// Sum over the wavefront in a fixed order, returned in every lane: shifted
+// partial sums within each row of 16 lanes (row_shr 1, 2, 4, 8), then the
+// four row totals.
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_shr<0x111>(v);
+  v += dpp_shr<0x112>(v);
+  v += dpp_shr<0x114>(v);
+  v += dpp_shr<0x118>(v);
+  return (lane_value(v, 15) + lane_value(v, 31)) + (lane_value(v, 47) + lane_value(v, 63));
+}
+
+// forward row I: y_I = b_I - sum_d L(I, I-d) y_{I-d} (ring holds the last W y)
+__device__ __forceinline__ void ldlt_fwd_row(int I, int& iw, int W, int bw, int lane,
+                                             const double (&row)[LDLT_S], double bI,
+                                             double* ring, double* y) {
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < LDLT_S; ++q) {
+    const int d = 1 + lane + 64 * q;
+    if (d <= bw && d <= I) {
+      int r = iw - d;
+      if (r < 0) r += W;
+      acc += row[q] * ring[r];
+    }
+  }
+  acc = wave_total(acc);
+  if (lane == 0) {
+    const double yi = bI - acc;
+    ring[iw] = yi;
+    y[I] = yi;
+  }
+  iw = iw + 1 == W ? 0 : iw + 1;
+  wave_lds_order();
+}
+
+// backward row I: x_I = y_I / D_I - pend_I; pend_{I-d} += L(I, I-d) x_I
+__device__ __forceinline__ void ldlt_bwd_row(int I, int& iw, int W, int bw, int lane,
+                                             const double (&row)[LDLT_S], double yI, double dI,
+                                             double* ring, double* y) {
+  const double xi = yI / dI - ring[iw];
+  wave_lds_order();  // every lane has read ring[iw] before it is cleared
+  if (lane == 0) {
+    y[I] = xi;
+    ring[iw] = 0.0;  // the slot now serves row I - W
+  }
+#pragma unroll
+  for (int q = 0; q < LDLT_S; ++q) {
+    const int d = 1 + lane + 64 * q;
+    if (d <= bw && d <= I) {
+      int r = iw - d;
+      if (r < 0) r += W;
+      ring[r] += row[q] * xi;
+    }
+  }
+  iw = iw == 0 ? W - 1 : iw - 1;
+  wave_lds_order();
+}
+
+// One wavefront per problem.  LDLT_P rows are in flight: each of LDLT_P
+// register sets is reloaded with the row LDLT_P ahead right after its row is
+// processed, so a row's loads have LDLT_P - 1 rows of work to arrive.
+constexpr int LDLT_P = 8;
+
 __global__ __launch_bounds__(64) void ldlt_solve_kernel(LdltArgs A, const int* __restrict__ list,
                                                         const double* __restrict__ b,
                                                         double* __restrict__ x,
@@ -171,89 +267,43 @@ __global__ __launch_bounds__(64) void ldlt_solve_kernel(LdltArgs A, const int* _
   const double* band = A.band + P.band0;
   const double* D = A.D + P.dof0;
   double* y = yscr + P.dof0;
-  constexpr int S = (LDLT_MAXW + 63) / 64;  // band entries per lane and row
   // b in band order
   for (int I = lane; I < N; I += 64) y[I] = b[ldlt_dof(A, P, I)];
   __syncthreads();
-  // forward: y_I = b_I - sum_{d=1..min(bw,I)} L(I, I-d) y_{I-d}; ring[I mod W]
-  // = y_I.  Row I + 1's band entries and b value are loaded during row I.
-  double cur[S];
+  double rs[LDLT_P][LDLT_S], vs[LDLT_P], ds[LDLT_P];
+  int iw = 0;
 #pragma unroll
-  for (int q = 0; q < S; ++q) cur[q] = 0.0;  // row 0 has no subdiagonal entries
-  double bcur = y[0];
-  int iw = 0;  // I mod W
-  for (int I = 0; I < N; ++I) {
-    double nxt[S];
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-      const int d = 1 + lane + 64 * q;
-      nxt[q] = (I + 1 < N && d <= bw && d <= I + 1) ? band[(long)(I + 1) * W + d] : 0.0;
-    }
-    const double bnxt = I + 1 < N ? y[I + 1] : 0.0;
-    double acc = 0.0;
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-      const int d = 1 + lane + 64 * q;
-      if (d <= bw && d <= I) {
-        int r = iw - d;
-        if (r < 0) r += W;
-        acc += cur[q] * ring[r];
-      }
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) {
-      const double yi = bcur - acc;
-      ring[iw] = yi;
-      y[I] = yi;
-    }
-#pragma unroll
-    for (int q = 0; q < S; ++q) cur[q] = nxt[q];
-    bcur = bnxt;
-    iw = iw + 1 == W ? 0 : iw + 1;
-    wave_lds_order();
+  for (int u = 0; u < LDLT_P; ++u) {
+    ldlt_load_row(band, W, bw, N, u, lane, rs[u]);
+    vs[u] = ldlt_ld(y, u, N);
   }
-  // backward, right-looking: x_I = y_I / D_I - pend_I, then row I of L adds
-  // L(I, I-d) x_I to the pending sums of rows I-1 .. I-bw (ring slots); x
-  // goes to y[I] in band order
+  for (int base = 0; base < N; base += LDLT_P) {
+#pragma unroll
+    for (int u = 0; u < LDLT_P; ++u) {
+      const int I = base + u;
+      if (I < N) ldlt_fwd_row(I, iw, W, bw, lane, rs[u], vs[u], ring, y);
+      ldlt_load_row(band, W, bw, N, I + LDLT_P, lane, rs[u]);
+      vs[u] = ldlt_ld(y, I + LDLT_P, N);
+    }
+  }
   for (int t = lane; t < W; t += 64) ring[t] = 0.0;
   __syncthreads();
   iw = (N - 1) % W;
-  double row[S];
 #pragma unroll
-  for (int q = 0; q < S; ++q) {
-    const int d = 1 + lane + 64 * q;
-    row[q] = (d <= bw && d <= N - 1) ? band[(long)(N - 1) * W + d] : 0.0;
+  for (int u = 0; u < LDLT_P; ++u) {
+    ldlt_load_row(band, W, bw, N, N - 1 - u, lane, rs[u]);
+    vs[u] = ldlt_ld(y, N - 1 - u, N);
+    ds[u] = ldlt_ld(D, N - 1 - u, N);
   }
-  double ycur = y[N - 1], dcur = D[N - 1];
-  for (int I = N - 1; I >= 0; --I) {
-    double nrow[S];
+  for (int base = N - 1; base >= 0; base -= LDLT_P) {
 #pragma unroll
-    for (int q = 0; q < S; ++q) {
-      const int d = 1 + lane + 64 * q;
-      nrow[q] = (I > 0 && d <= bw && d <= I - 1) ? band[(long)(I - 1) * W + d] : 0.0;
+    for (int u = 0; u < LDLT_P; ++u) {
+      const int I = base - u;
+      if (I >= 0) ldlt_bwd_row(I, iw, W, bw, lane, rs[u], vs[u], ds[u], ring, y);
+      ldlt_load_row(band, W, bw, N, I - LDLT_P, lane, rs[u]);
+      vs[u] = ldlt_ld(y, I - LDLT_P, N);
+      ds[u] = ldlt_ld(D, I - LDLT_P, N);
     }
-    const double ynxt = I > 0 ? y[I - 1] : 0.0, dnxt = I > 0 ? D[I - 1] : 1.0;
-    const double xi = ycur / dcur - ring[iw];
-    wave_lds_order();  // every lane has read ring[iw] before it is cleared
-    if (lane == 0) {
-      y[I] = xi;
-      ring[iw] = 0.0;  // the slot now serves row I - W
-    }
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-      const int d = 1 + lane + 64 * q;
-      if (d <= bw && d <= I) {
-        int r = iw - d;
-        if (r < 0) r += W;
-        ring[r] += row[q] * xi;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < S; ++q) row[q] = nrow[q];
-    ycur = ynxt;
-    dcur = dnxt;
-    iw = iw == 0 ? W - 1 : iw - 1;
-    wave_lds_order();
   }
   __syncthreads();
   for (int I = lane; I < N; I += 64) x[ldlt_dof(A, P, I)] = y[I];

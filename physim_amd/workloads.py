@@ -19,6 +19,12 @@ Workloads (all synthetic, seeded):
              step = one Newton solve of the BasicDynamic objective.  The plain
              FP64 path (Jacobi PCG at the reference's tolerances) runs the same
              steps for the speed-up and the displacement check.
+  tiny    -- 256 small beams (the batch generator at 1 cm cells, hex dims
+             U{2..4} x U{2..4} x U{8..32}) in one fused batch solved with the
+             EigenLDLT linear solver (the reference's default: FP64 banded
+             LDL^T, one factorization workgroup per instance); the same batch
+             with Jacobi-PCG (EigenCG at the reference's tolerances, FP64) is
+             timed beside it and the positions compared.
   serving -- Poisson arrivals of a seeded 60/40 mix of beams (the 1.28M-tet
              beam's generator at 2k..max tets) and cloths (n U{32..max}, 1 m
              sheets hanging from the two corners of their top edge, 0.2 kg/m^2),
@@ -104,6 +110,62 @@ def batch(args, log):
                        "matches_within_1e-6": bool(max(errs) <= 1e-6),
                        "solo_solve_s": solo},
     }
+
+
+def tiny(args, log):
+    count = args.count or 256
+    rng = np.random.default_rng(11)
+    scs = [scenes.batch_beam(rng, cell=0.01, dmin=(2, 2, 8), dmax=(4, 4, 32))
+           for _ in range(count)]
+    m = sum(s.m for s in scs)
+    n = sum(s.n for s in scs)
+    ctx = pg.Context(0)
+    base = dict(pg.CONVERGE_OPTS, lsNumMaxIters=2000, numMaxIters=200)
+    steps = max(1, min(args.steps, 3))
+    out = {}
+    for label, kw in (("ldlt", dict(lsSolverType=pg.LS_EIGENLDLT)),
+                      ("cg", dict(lsSolverType=pg.LS_EIGENCG, precond=0))):
+        if label == "cg" and args.no_fp64:
+            break
+        B = pg.Batch(ctx, pg.FP64)
+        for s in scs:
+            B.add_scene(s)
+        B.build()
+        for _ in range(min(args.warmup, 1)):
+            B.reset()
+            B.newton(**base, **kw)
+        times = []
+        for _ in range(steps if label == "ldlt" else 1):
+            B.reset()
+            ctx.synchronize()
+            t = time.perf_counter()
+            res = B.newton(**base, **kw)
+            times.append(time.perf_counter() - t)
+        out[label] = (float(np.mean(times)), res, [B.get_x(k).copy() for k in range(count)])
+        log(f"tiny {label}: {times[-1]:.3f}s, "
+            f"{sum(r['result'] == 'SUCCESS' for r in res)}/{count} converged")
+        B.close()
+    ctx.close()
+    el, res, xs = out["ldlt"]
+    its = [r["iterations"] for r in res]
+    line = {
+        "value": count / el, "unit": "Newton solves/s", "ms_per_step": el * 1e3,
+        "steps": steps, "warmup": min(args.warmup, 1), "scaling": "weak", "dtype": "f64",
+        "config": {"workload": f"tiny_{count}_beams: {m} tets, {3 * n} DoF, EigenLDLT "
+                               "(banded LDL^T per instance)", "precision": "fp64"},
+        "converged": sum(r["result"] == "SUCCESS" for r in res), "instances": count,
+        "newton_iterations": _stats(its),
+    }
+    if "cg" in out:
+        elc, resc, xc = out["cg"]
+        both = [k for k in range(count)
+                if res[k]["result"] == "SUCCESS" and resc[k]["result"] == "SUCCESS"]
+        diff = max((np.abs(xs[k] - xc[k]).max() / scs[k].bbox_diag() for k in both),
+                   default=0.0)
+        line["eigencg_fp64"] = {"s": elc, "converged":
+                                sum(r["result"] == "SUCCESS" for r in resc),
+                                "max_disp_diff_over_bbox_diag_both_converged": float(diff)}
+    return line
 
 
 def rods(args, log):
