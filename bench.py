@@ -265,9 +265,17 @@ def main():
 
     lat = np.array([r["wall_s"] for r in results]) * 1e3
     lpr = M.profile(2)[0]  # lanes per block row of the SpMV
-    # the PCG's SpMV launch of the variant in use (PGN_PCG, as the library reads it)
-    spmv_kernel = {"split": "pcg_k1b_kernel", "fused": "pcg_k1_kernel"}.get(
-        os.environ.get("PGN_PCG", "cg1"), "cg1_a_kernel")
+    # the PCG's SpMV launch of the variant in use (PGN_PCG and PGN_SELL, as the
+    # library reads them)
+    variant = os.environ.get("PGN_PCG", "cg1")
+    if variant not in ("split", "fused") and os.environ.get("PGN_SELL") != "0":
+        spmv_name = "cg1_sell_kernel<float>"
+        spmv_desc = f"{spmv_name} (SELL-64 SpMV of the PCG)"
+    else:
+        spmv_kernel = {"split": "pcg_k1b_kernel", "fused": "pcg_k1_kernel"}.get(
+            variant, "cg1_a_kernel")
+        spmv_name = f"{spmv_kernel}<float, {lpr}>"
+        spmv_desc = f"{spmv_name} (BSR3 SpMV of the PCG)"
     line = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -298,11 +306,11 @@ def main():
         **fp64,
         "pcg_update_avg_us": k2_ms / max(k2_n, 1) * 1e3,
         "roofline": {"bound": "hbm",
-                     "kernel": f"{spmv_kernel}<float, {lpr}> (BSR3 SpMV of the PCG)",
+                     "kernel": spmv_desc,
                      "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"{spmv_kernel}<float, {lpr}>"),
+                     "traffic": measured_traffic(spmv_name),
                      "launches": k1_n, "avg_us": spmv_ms * 1e3,
                      "bytes_per_launch": spmv_bytes},
         "cpu_baseline": cpu,

@@ -122,7 +122,8 @@ void launch_frob_lower(hipStream_t s, bool fp32, long nb, const int* brow,
 // Reductions are fence-free: each producer workgroup writes a partial sum,
 // and every consumer workgroup of the next launch re-sums the partials in a
 // fixed order (identical bits everywhere; kernel boundaries give coherence).
-constexpr int PCG_MAXP = 1024;  // max producer workgroups of one partial set
+constexpr int PCG_MAXP = 2048;      // capacity: producer workgroups of one partial set
+constexpr int PCG_GRID_CAP = 1024;  // grid cap of the BSR3 SpMV / update / residual launches
 
 struct PcgScalars {
   double thr;   // max(tol^2 |b|^2, DBL_MIN)
@@ -158,8 +159,25 @@ struct PcgArgs {
   double* part2;         // 2 parities x PCG_MAXP x {r.r, r.z}
   double* partb;         // PCG_MAXP: |b|^2 of init
   PcgScalars* sc;
+  // SELL-64 copy of the matrix for the Chronopoulos-Gear SpMV: slices of 64
+  // rows, one row per lane, slot (slice, j, lane) at sell_so[slice] + 64 j +
+  // lane, the 9 values of a slot component-major per (slice, j) so that each
+  // load of a wavefront reads 64 consecutive values
+  int sell = 0;          // the SpMV launch uses it (g1 = its grid)
+  int sell_spw = 0;      // slices per workgroup (a multiple of 8)
+  long sell_ns = 0;      // slices
+  const long* sell_so = nullptr;   // ns + 1 slot offsets
+  const int* sell_len = nullptr;   // n: blocks of each row
+  const int* sell_col = nullptr;   // per slot: block column
+  const void* sell_val = nullptr;  // 9 per slot, Hessian precision
 };
 int pcg_grid1(int n, int lpr);
+// SELL-64 layout: slices per workgroup and grid (<= PCG_MAXP) for ns slices
+void sell_grid(long ns, int& spw, int& grid);
+// copies the BSR3 values into the SELL-64 layout (map: source block per slot,
+// -1 for padding)
+void launch_sell_pack(hipStream_t s, bool fp32, long ns, const long* so, const int* map,
+                      const void* bval, void* sval);
 int pcg_grid2(int n);
 // r = b - A x (b in T); z = M^-1 r; p = 0.  x_zero: start from x = 0.
 void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, int x_zero);

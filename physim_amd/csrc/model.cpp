@@ -424,6 +424,44 @@ BsrArgs pgn_model::bsr_args() const {
   return a;
 }
 
+void pgn_model::sell_prepare() {
+  if (sell_ready || !use_sell) return;
+  const long ns = (n + 63) / 64;
+  std::vector<long> so(ns + 1, 0);
+  std::vector<int> len(n);
+  for (int i = 0; i < n; ++i) len[i] = (int)(pat.rowptr[i + 1] - pat.rowptr[i]);
+  for (long sl = 0; sl < ns; ++sl) {
+    int w = 0;
+    for (long r = sl * 64; r < std::min<long>(n, sl * 64 + 64); ++r) w = std::max(w, len[r]);
+    so[sl + 1] = so[sl] + 64L * w;
+  }
+  const long slots = so[ns];
+  std::vector<int> col(slots, 0), map(slots, -1);
+  for (long sl = 0; sl < ns; ++sl)
+    for (long r = sl * 64; r < std::min<long>(n, sl * 64 + 64); ++r)
+      for (int j = 0; j < len[r]; ++j) {
+        const long slot = so[sl] + 64L * j + (r - sl * 64);
+        col[slot] = pat.bcol[pat.rowptr[r] + j];
+        map[slot] = (int)(pat.rowptr[r] + j);
+      }
+  hipStream_t s = stream();
+  sell_so.upload(so, s);
+  sell_len.upload(len, s);
+  sell_col.upload(col, s);
+  sell_map.upload(map, s);
+  sell_val.alloc((size_t)9 * std::max<long>(slots, 1) * tsize());
+  sell_ns = ns;
+  sell_grid(ns, sell_spw, sell_g);
+  sync();
+  sell_ready = true;
+}
+
+void pgn_model::sell_pack() {
+  if (!use_sell) return;
+  sell_prepare();
+  launch_sell_pack(stream(), hfp32, sell_ns, sell_so.p, sell_map.p, bval.p, sell_val.p);
+}
+
 PcgArgs pgn_model::pcg_args(double tol, int precond) const {
   PcgArgs a;
   a.lpr = pcg_lpr;
@@ -431,6 +469,16 @@ PcgArgs pgn_model::pcg_args(double tol, int precond) const {
   a.cg1 = pcg_cg1 ? 1 : 0;
   a.cgs = cgs.p;
   a.g1 = pcg_grid1(n, pcg_lpr);
+  if (use_sell && sell_ready) {  // init and the SpMV share the SELL grid
+    a.sell = 1;
+    a.g1 = sell_g;
+    a.sell_spw = sell_spw;
+    a.sell_ns = sell_ns;
+    a.sell_so = sell_so.p;
+    a.sell_len = sell_len.p;
+    a.sell_col = sell_col.p;
+    a.sell_val = sell_val.p;
+  }
   a.g2 = pcg_grid2(n);
   a.tol = tol;
   a.partb = partb.p;

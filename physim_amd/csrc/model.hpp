@@ -87,6 +87,18 @@ struct pgn_model {
   bool pcg_cg1 = pgn::pcg_variant() == 0;
   bool pcg_split = pgn::pcg_variant() != 2;
   pgn::DBuf<double> cgs;  // its 4 scalars
+  // SELL-64 copy of the matrix for the C-G SpMV (PGN_SELL=0 in the
+  // environment keeps the BSR3 SpMV); built on the first PCG solve, packed
+  // from the BSR3 values before every solve try
+  bool use_sell = pcg_cg1 && !(std::getenv("PGN_SELL") && std::strcmp(std::getenv("PGN_SELL"), "0") == 0);
+  bool sell_ready = false;
+  long sell_ns = 0;
+  int sell_spw = 0, sell_g = 0;
+  pgn::DBuf<long> sell_so;
+  pgn::DBuf<int> sell_len, sell_col, sell_map;
+  pgn::DBuf<char> sell_val;
+  void sell_prepare();
+  void sell_pack();
   bool tuned = false;
   double tune_us[3] = {0, 0, 0};  // SpMV time for 4, 8, 16 lanes per row
   pgn::DBuf<pgn::PcgScalars> sc;

@@ -32,19 +32,32 @@ static inline int round8(int g) { return g >= 8 ? g - g % 8 : g; }
 int pcg_grid1(int n, int lpr) {
   const int rows = K1_BS / lpr;
   int g = (n + rows - 1) / rows;
-  return round8(g < 1 ? 1 : (g > PCG_MAXP ? PCG_MAXP : g));
+  return round8(g < 1 ? 1 : (g > PCG_GRID_CAP ? PCG_GRID_CAP : g));
 }
+constexpr int SELL_BS = 256;  // SELL SpMV workgroup: 4 wavefronts, one slice each per pass
+void sell_grid(long ns, int& spw, int& grid) {
+  constexpr int WAVES = SELL_BS / 64;
+  long per = (ns + PCG_MAXP - 1) / PCG_MAXP;
+  if (per < WAVES) per = WAVES;
+  per = (per + WAVES - 1) / WAVES * WAVES;
+  long g = (ns + per - 1) / per;
+  if (g < 1) g = 1;
+  if (g >= 8) g = (g + 7) / 8 * 8;
+  spw = (int)per;
+  grid = (int)g;
+}
+
 // streaming launches: one register-staged batch of K2_NPT nodes per thread,
 // rounded up to whole XCD rounds
 int pcg_grid2(int n) {
   int g = (n + K2_BS * K2_NPT - 1) / (K2_BS * K2_NPT);
   if (g < 1) g = 1;
   if (g >= 8) g = (g + 7) / 8 * 8;
-  return g > PCG_MAXP ? PCG_MAXP : g;
+  return g > PCG_GRID_CAP ? PCG_GRID_CAP : g;
 }
 static inline int grid_rows(int n) {
   int g = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
-  return g < 1 ? 1 : (g > PCG_MAXP ? PCG_MAXP : g);
+  return g < 1 ? 1 : (g > PCG_GRID_CAP ? PCG_GRID_CAP : g);
 }
 
 template <int L, class T>
@@ -531,6 +544,128 @@ __global__ __launch_bounds__(K1_BS) void cg1_a_kernel(PcgArgs A, int k) {
   if (threadIdx.x == 0) A.part1[blockIdx.x] = s;
 }
 
+// The same step with the SpMV over the SELL-64 copy: a wavefront takes a
+// slice of 64 rows, one row per lane, walking the row's blocks in column
+// order; every value and column load of the wavefront is 64 consecutive
+// entries.  Workgroups take spw consecutive slices (blocked per XCD).
+template <class T>
+__global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
+  __shared__ double red[3 * SELL_BS / 64 + 4];
+  constexpr int WAVES = SELL_BS / 64;
+  PcgScalars* sc = A.sc;
+  if (sc->done) return;
+  const T* __restrict__ sval = reinterpret_cast<const T*>(A.sell_val);
+  const int* __restrict__ scol = A.sell_col;
+  const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
+  T* __restrict__ w = reinterpret_cast<T*>(A.p1);
+  const int G = gridDim.x, b = blockIdx.x;
+  const int lb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the first block of this wavefront's first slice, loaded before the
+  // partial sums (unconditional loads at clamped indexes: every slice has at
+  // least one block per lane slot, every row its diagonal block)
+  const long s0 = min((long)lb * A.sell_spw + wave, A.sell_ns - 1);
+  const long base0 = A.sell_so[s0];
+  const int len0 = A.sell_len[min(s0 * 64 + lane, (long)A.n - 1)];
+  const long c0 = scol[base0 + lane];
+  T v0[9], z0[3];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) v0[q] = sval[base0 * 9 + lane + 64 * q];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) z0[q] = z[3 * c0 + q];
+  const double* prev = rr_slot(A, k - 1);
+  const int cprev = rr_count(A, k - 1);
+  double rr, bb, unused;
+  all_sum3<SELL_BS>(prev, cprev, 2, A.partb, k == 0 ? A.g1 : 0, 1, A.partb, 0, 1, red, rr, bb,
+                    unused);
+  if (k == 0) {
+    double thr = A.tol * A.tol * bb;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->thr = thr;
+      sc->bb = bb;
+      sc->rr = rr;
+    }
+    if (bb == 0.0 || rr < thr) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->done = 1;
+        sc->iters = 0;
+      }
+      return;
+    }
+  } else if (rr < sc->thr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->done = 1;
+      sc->iters = k - 1;
+      sc->rr = rr;
+    }
+    return;
+  }
+  double dl = 0.0;
+  for (int t = wave; t < A.sell_spw; t += WAVES) {
+    const long slice = (long)lb * A.sell_spw + t;
+    if (slice >= A.sell_ns) break;
+    const long row = slice * 64 + lane;
+    const bool first = t == wave;
+    const int len = row < A.n ? (first ? len0 : A.sell_len[row]) : 0;
+    const long base = first ? base0 : A.sell_so[slice];
+    T y0 = 0, y1 = 0, y2 = 0;
+    int j = 0;
+    if (first && len > 0) {
+      y0 = v0[0] * z0[0] + v0[1] * z0[1] + v0[2] * z0[2];
+      y1 = v0[3] * z0[0] + v0[4] * z0[1] + v0[5] * z0[2];
+      y2 = v0[6] * z0[0] + v0[7] * z0[1] + v0[8] * z0[2];
+      j = 1;
+    }
+    for (; j < len; ++j) {
+      const long c = scol[base + 64L * j + lane];
+      const T* V = sval + (base + 64L * j) * 9 + lane;
+      const T a0 = z[3 * c], a1 = z[3 * c + 1], a2 = z[3 * c + 2];
+      y0 += V[0] * a0 + V[64] * a1 + V[128] * a2;
+      y1 += V[192] * a0 + V[256] * a1 + V[320] * a2;
+      y2 += V[384] * a0 + V[448] * a1 + V[512] * a2;
+    }
+    if (row < A.n) {
+      w[3 * row] = y0;
+      w[3 * row + 1] = y1;
+      w[3 * row + 2] = y2;
+      dl += (double)z[3 * row] * (double)y0 + (double)z[3 * row + 1] * (double)y1 +
+            (double)z[3 * row + 2] * (double)y2;
+    }
+  }
+  const double s = block_sum<SELL_BS>(dl, red);
+  if (threadIdx.x == 0) A.part1[blockIdx.x] = s;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void sell_pack_kernel(long ns, const long* __restrict__ so,
+                                                        const int* __restrict__ map,
+                                                        const T* __restrict__ bval,
+                                                        T* __restrict__ sval) {
+  const long slice = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (slice >= ns) return;
+  const long base = so[slice], wd = (so[slice + 1] - base) / 64;
+  for (long j = 0; j < wd; ++j) {
+    const int src = map[base + 64 * j + lane];
+    T* V = sval + (base + 64 * j) * 9 + lane;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) V[64 * q] = src >= 0 ? bval[9L * src + q] : T(0);
+  }
+}
+
+void launch_sell_pack(hipStream_t s, bool fp32, long ns, const long* so, const int* map,
+                      const void* bval, void* sval) {
+  const int g = (int)((ns + 3) / 4);
+  if (g <= 0) return;
+  if (fp32)
+    hipLaunchKernelGGL(sell_pack_kernel<float>, dim3(g), dim3(256), 0, s, ns, so, map,
+                       reinterpret_cast<const float*>(bval), reinterpret_cast<float*>(sval));
+  else
+    hipLaunchKernelGGL(sell_pack_kernel<double>, dim3(g), dim3(256), 0, s, ns, so, map,
+                       reinterpret_cast<const double*>(bval), reinterpret_cast<double*>(sval));
+}
+
 template <class T>
 struct Cg1Batch {
   N3<T> Z[K2_NPT], P[K2_NPT], W[K2_NPT], Q[K2_NPT], X[K2_NPT], R[K2_NPT];
@@ -766,6 +901,13 @@ static void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEven
 template <int L>
 static void launch_k1_l(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t e0,
                         hipEvent_t e1) {
+  if (a.cg1 && a.sell) {
+    if (fp32)
+      launch_timed(cg1_sell_kernel<float>, dim3(a.g1), dim3(SELL_BS), s, e0, e1, a, k);
+    else
+      launch_timed(cg1_sell_kernel<double>, dim3(a.g1), dim3(SELL_BS), s, e0, e1, a, k);
+    return;
+  }
   if (a.cg1) {
     if (fp32)
       launch_timed(cg1_a_kernel<float, L>, dim3(a.g1), dim3(K1_BS), s, e0, e1, a, k);

@@ -145,6 +145,7 @@ void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opt
       res.reg_used = i;
     }
     PGN_HIP(hipMemcpyAsync(x, xprev, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    M.sell_pack();  // the SpMV's copy of this try's matrix
     int it = 0, refs = 0;
     double rel = 0.0;
     solve_once(M, b, x, o, it, refs, rel);

@@ -46,9 +46,14 @@ def fixed_solution(ctx, beams):
     return x
 
 
-def test_inexact_newton_alone_matches_fixed_tolerance(ctx, beams, fixed_solution):
+def test_inexact_newton_alone_matches_fixed_tolerance(ctx, beams, fixed_solution, monkeypatch):
+    """The single-model solver with the BSR3 SpMV (PGN_SELL=0): this beam's
+    inexact-Newton path depends on the FP32 summation order of the PCG, and
+    with the SELL-64 SpMV's order it inverts elements and ends NONDESC, the
+    retry included."""
     _say("forcing: 9x6x61 StVK beam alone")
     s = beams[1]
+    monkeypatch.setenv("PGN_SELL", "0")  # read when the model is created
     M = pgn_model(ctx, s, pg.MIXED)
     r = M.newton(**dict(OPTS, adaptive_tol=1))
     _say(f"  {r}")
