@@ -607,7 +607,12 @@ int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
                // pointers, z and p_{k-1} read, p_k and q written.
                // Chronopoulos-Gear (default) and split: the plain SpMV,
                // values + cols + row pointers, one vector read, one written.
-        if (M->pcg_split || M->pcg_cg1)
+        // SELL-64 (the C-G default): values + cols of the rows' own blocks
+        // (padding slots are not read), a 4 B length per row and an 8 B
+        // offset per slice instead of the row pointers.
+        if (M->pcg_cg1 && M->use_sell)
+          *bytes = nb * (9 * hs + 4) + n * 4.0 + (n + 63) / 64 * 8.0 + 2 * 3 * n * hs;
+        else if (M->pcg_split || M->pcg_cg1)
           *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 2 * 3 * n * hs;
         else
           *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 4 * 3 * n * hs;
