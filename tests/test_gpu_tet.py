@@ -179,6 +179,34 @@ def test_linear_solve_mixed_reaches_fp64_residual(ctx, precond):
 
 
 @pytest.mark.parametrize("precision", [pg.FP64, pg.MIXED])
+def test_sell_and_bsr_spmv_solve_the_same_system(ctx, precision, monkeypatch):
+    """The PCG over the SELL-64 copy of the matrix (default) and over BSR3
+    (PGN_SELL=0, read at model creation): both reach the FP64 residual of
+    the same assembled matrix (so their solutions differ by at most the
+    conditioning times 2e-10; checked loosely at 1e-4).  400 nodes (5 x 5
+    per section) fill six slices of 64 rows and a ragged seventh, with rows
+    of 4 to 19 blocks."""
+    _say(f"sell vs bsr precision={precision}")
+    s = small_beam(dims=(4, 4, 15), seed=61)
+    x = perturbed_x(s, 1e-3, seed=62)
+    b = np.random.default_rng(63).standard_normal(s.n * 3)
+    b[s.fixed.reshape(-1).astype(bool)] = 0.0
+    sols = []
+    for sell in ("1", "0"):
+        monkeypatch.setenv("PGN_SELL", sell)
+        G = pgn_model(ctx, s, precision)
+        G.set_x(x)
+        xg, info = G.linear_solve(b, tol=1e-10, max_iters=5000, precond=1)
+        assert info["status"] == 0, info
+        rp, ci, v = G.hessian(pg.LAYOUT_LOWER)
+        A = orc.lower_to_dense(rp, ci, v)
+        assert np.linalg.norm(A @ xg - b) <= 1e-10 * np.linalg.norm(b) * 1.01
+        sols.append(xg)
+        G.close()
+    assert np.linalg.norm(sols[0] - sols[1]) <= 1e-4 * np.linalg.norm(sols[1])
+
+
+@pytest.mark.parametrize("precision", [pg.FP64, pg.MIXED])
 def test_newton_small_beam_matches_oracle(ctx, precision):
     _say(f"newton small beam precision={precision}")
     s = scenes.cantilever(dims=(4, 4, 16), scale=(0.1, 0.1, 0.25), young_sigma=0.3,
