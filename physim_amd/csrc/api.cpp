@@ -611,17 +611,17 @@ int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
         // (padding slots are not read), a 4 B length per row and an 8 B
         // offset per slice instead of the row pointers.
         if (M->pcg_cg1 && M->use_sell)
-          *bytes = nb * (9 * hs + 4) + n * 4.0 + (n + 63) / 64 * 8.0 + 2 * 3 * n * hs;
+          *bytes = nb * (9 * hs + 4) + n * 4.0 + (double)(((long)n + 63) / 64) * 8.0 + 2 * 3 * n * hs;
         else if (M->pcg_split || M->pcg_cg1)
           *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 2 * 3 * n * hs;
         else
           *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 4 * 3 * n * hs;
         break;
       case 9:  // the PCG update launch with the block-Jacobi preconditioner
-               // (9 values per node).  Chronopoulos-Gear: z, p, w, q, x, r
-               // read, p, q, x, r, z written; split and fused K2: p, q, x, r
-               // read, x, r, z written.
-        *bytes = (M->pcg_cg1 ? 11.0 : 7.0) * 3 * n * hs + 9.0 * n * hs;
+               // (6 values per node: the symmetric inverse's upper triangle).
+               // Chronopoulos-Gear: z, p, w, q, x, r read, p, q, x, r, z
+               // written; split and fused K2: p, q, x, r read, x, r, z written.
+        *bytes = (M->pcg_cg1 ? 11.0 : 7.0) * 3 * n * hs + 6.0 * n * hs;
         break;
       default:
         throw Error(PGN_ERR_INVALID, "unknown kernel id");

@@ -236,7 +236,10 @@ void launch_bsr_gather(hipStream_t s, bool fp32, const BsrArgs& a,
 
 // Jacobi: Eigen's DiagonalPreconditioner (1/a_ii, or 1 where a_ii == 0).
 // Block Jacobi: inverse of each 3x3 diagonal node block (Jacobi where the
-// block is singular).
+// block is singular), stored as its upper triangle {m00, m01, m02, m11, m12,
+// m22}: diagonal blocks are assembled exactly symmetric, and the cofactor
+// of (i, j) then takes the same two products as that of (j, i), so the
+// inverse is exactly symmetric too.
 template <class T>
 __global__ __launch_bounds__(256) void precond_kernel(int n, const long* __restrict__ diag,
                                                      const T* __restrict__ bval,
@@ -254,23 +257,20 @@ __global__ __launch_bounds__(256) void precond_kernel(int n, const long* __restr
   const double c01 = (double)a[5] * a[6] - (double)a[3] * a[8];
   const double c02 = (double)a[3] * a[7] - (double)a[4] * a[6];
   const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
-  T* o = binv + 9L * i;
+  T* o = binv + 6L * i;
   if (det != 0.0 && isfinite(det)) {
     const double id = 1.0 / det;
     o[0] = (T)(c00 * id);
     o[1] = (T)(((double)a[2] * a[7] - (double)a[1] * a[8]) * id);
     o[2] = (T)(((double)a[1] * a[5] - (double)a[2] * a[4]) * id);
-    o[3] = (T)(c01 * id);
-    o[4] = (T)(((double)a[0] * a[8] - (double)a[2] * a[6]) * id);
-    o[5] = (T)(((double)a[2] * a[3] - (double)a[0] * a[5]) * id);
-    o[6] = (T)(c02 * id);
-    o[7] = (T)(((double)a[1] * a[6] - (double)a[0] * a[7]) * id);
-    o[8] = (T)(((double)a[0] * a[4] - (double)a[1] * a[3]) * id);
+    o[3] = (T)(((double)a[0] * a[8] - (double)a[2] * a[6]) * id);
+    o[4] = (T)(((double)a[2] * a[3] - (double)a[0] * a[5]) * id);
+    o[5] = (T)(((double)a[0] * a[4] - (double)a[1] * a[3]) * id);
   } else {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) o[q] = T(0);
-#pragma unroll
-    for (int p = 0; p < 3; ++p) o[4 * p] = dinv[3L * i + p];
+    o[0] = dinv[3L * i];
+    o[1] = o[2] = o[4] = T(0);
+    o[3] = dinv[3L * i + 1];
+    o[5] = dinv[3L * i + 2];
   }
 }
 

@@ -105,7 +105,7 @@ struct BsrArgs {
 void launch_bsr_gather(hipStream_t s, bool fp32, const BsrArgs& a,
                        const void* eH, void* bval);
 // diag: n block indexes of the diagonal blocks.  Writes dinv (3n, Jacobi) and
-// binv (9n, block-Jacobi inverse) in precision T.
+// binv (6n, block-Jacobi inverse as upper triangles) in precision T.
 void launch_precond(hipStream_t s, bool fp32, int n, const long* diag,
                     const void* bval, void* dinv, void* binv);
 // bval[diag blocks] += shift on the scalar diagonal
@@ -140,7 +140,7 @@ struct PcgArgs {
   const int* bcol;
   const void* bval;
   const void* dinv;      // Jacobi (3n)
-  const void* binv;      // block Jacobi (9n)
+  const void* binv;      // block Jacobi (6n: upper triangles)
   int precond;           // 0 Jacobi, 1 block Jacobi
   int lpr;               // lanes per block row in the SpMV (4, 8 or 16)
   int split;             // K1 as two launches (p update, then plain SpMV)

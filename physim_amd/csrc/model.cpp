@@ -285,7 +285,7 @@ void pgn_model::setup(int n_nodes, const double* X0h, int n_elems, const int* Th
   const size_t ts = tsize();
   bval.alloc(9 * pat.nb * ts);
   dinv.alloc(3L * n * ts);
-  binv.alloc(9L * n * ts);
+  binv.alloc(6L * n * ts);
   eg64.alloc(12L * std::max(m, 1));
   eH.alloc((size_t)TET_H_PACK * std::max(m, 1) * ts);
   blockE.alloc(elem_partials() + 1);

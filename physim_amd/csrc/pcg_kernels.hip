@@ -116,16 +116,22 @@ __device__ __forceinline__ void st3(T* a, long i, const N3<T>& v) {
 
 // preconditioner operands of node i: the 3x3 block inverse (precond 1) or
 // the three Jacobi entries
+// Block-Jacobi inverses are stored as upper triangles {m00, m01, m02, m11,
+// m12, m22} (exactly symmetric); row p of the full block is (m(p,0),
+// m(p,1), m(p,2)) with these indices
+__device__ __forceinline__ constexpr int sym3(int p, int q) {
+  return p <= q ? (p == 0 ? q : (p == 1 ? 2 + q : 5)) : (q == 0 ? p : (q == 1 ? 2 + p : 5));
+}
 template <class T>
 struct PreM {
-  T m[9];
+  T m[6];
 };
 template <class T>
 __device__ __forceinline__ void ld_precond(int precond, const T* dinv, const T* binv, long i,
                                            PreM<T>& M) {
   if (precond == 1) {
 #pragma unroll
-    for (int e = 0; e < 9; ++e) M.m[e] = binv[9 * i + e];
+    for (int e = 0; e < 6; ++e) M.m[e] = binv[6 * i + e];
   } else {
 #pragma unroll
     for (int e = 0; e < 3; ++e) M.m[e] = dinv[3 * i + e];
@@ -137,7 +143,7 @@ __device__ __forceinline__ N3<T> mul_precond(int precond, const PreM<T>& M, cons
   if (precond == 1) {
 #pragma unroll
     for (int p = 0; p < 3; ++p)
-      z.c[p] = M.m[3 * p] * r.c[0] + M.m[3 * p + 1] * r.c[1] + M.m[3 * p + 2] * r.c[2];
+      z.c[p] = M.m[sym3(p, 0)] * r.c[0] + M.m[sym3(p, 1)] * r.c[1] + M.m[sym3(p, 2)] * r.c[2];
   } else {
 #pragma unroll
     for (int p = 0; p < 3; ++p) z.c[p] = M.m[p] * r.c[p];
@@ -150,9 +156,10 @@ __device__ __forceinline__ void apply_precond(int precond, const T* __restrict__
                                               const T* __restrict__ binv, long i,
                                               const T r[3], T z[3]) {
   if (precond == 1) {
-    const T* M = binv + 9 * i;
+    const T* M = binv + 6 * i;
 #pragma unroll
-    for (int p = 0; p < 3; ++p) z[p] = M[3 * p] * r[0] + M[3 * p + 1] * r[1] + M[3 * p + 2] * r[2];
+    for (int p = 0; p < 3; ++p)
+      z[p] = M[sym3(p, 0)] * r[0] + M[sym3(p, 1)] * r[1] + M[sym3(p, 2)] * r[2];
   } else {
 #pragma unroll
     for (int p = 0; p < 3; ++p) z[p] = dinv[3 * i + p] * r[p];
