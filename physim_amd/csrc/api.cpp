@@ -609,9 +609,10 @@ int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
                // values + cols + row pointers, one vector read, one written.
         // SELL-64 (the C-G default): values + cols of the rows' own blocks
         // (padding slots are not read), a 4 B length per row and an 8 B
-        // offset per slice instead of the row pointers.
+        // offset per slice instead of the row pointers; z and q read, q
+        // (= A z + beta q) written.
         if (M->pcg_cg1 && M->use_sell)
-          *bytes = nb * (9 * hs + 4) + n * 4.0 + (double)(((long)n + 63) / 64) * 8.0 + 2 * 3 * n * hs;
+          *bytes = nb * (9 * hs + 4) + n * 4.0 + (double)(((long)n + 63) / 64) * 8.0 + 3 * 3 * n * hs;
         else if (M->pcg_split || M->pcg_cg1)
           *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 2 * 3 * n * hs;
         else
@@ -621,7 +622,9 @@ int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
                // (6 values per node: the symmetric inverse's upper triangle).
                // Chronopoulos-Gear: z, p, w, q, x, r read, p, q, x, r, z
                // written; split and fused K2: p, q, x, r read, x, r, z written.
-        *bytes = (M->pcg_cg1 ? 11.0 : 7.0) * 3 * n * hs + 6.0 * n * hs;
+        // With the SELL-64 SpMV forming q: z, p, q, x, r read, p, x, r, z
+        // written.
+        *bytes = (M->pcg_cg1 ? (M->use_sell ? 9.0 : 11.0) : 7.0) * 3 * n * hs + 6.0 * n * hs;
         break;
       default:
         throw Error(PGN_ERR_INVALID, "unknown kernel id");

@@ -146,7 +146,8 @@ struct PcgArgs {
   int split;             // K1 as two launches (p update, then plain SpMV)
   int cg1;               // Chronopoulos-Gear form: K1 = SpMV on z, K2 = every
                          // vector update (p, q, x, r, z); overrides split
-  double* cgs;           // cg1 scalars: 2 parities x {alpha, r.z}
+  double* cgs;           // cg1 scalars: 2 parities x {alpha, r.z}, then {beta, r.z}
+                         // of the current iteration (SELL form)
   int g1, g2;            // workgroups of K1/init and of K2
   double tol;
   void* x;
@@ -163,7 +164,9 @@ struct PcgArgs {
   // rows, one row per lane, slot (slice, j, lane) at sell_so[slice] + 64 j +
   // lane, the 9 values of a slot component-major per (slice, j) so that each
   // load of a wavefront reads 64 consecutive values
-  int sell = 0;          // the SpMV launch uses it (g1 = its grid)
+  int sell = 0;          // the SpMV launch uses it (g1 = its grid) and also
+                         // forms beta and q = w + beta q for its rows (cgs[4],
+                         // cgs[5] = beta, r.z for the update launch)
   int sell_spw = 0;      // slices per workgroup (a multiple of 8)
   long sell_ns = 0;      // slices
   const long* sell_so = nullptr;   // ns + 1 slot offsets

@@ -299,7 +299,7 @@ void pgn_model::setup(int n_nodes, const double* X0h, int n_elems, const int* Th
   partb.alloc(PCG_MAXP);
   red64.alloc(PCG_MAXP + 8);
   sc.alloc(1);
-  cgs.alloc(4);
+  cgs.alloc(8);
   r64.alloc(3L * n);
   tmp64a.alloc(3L * n);
   tmp64b.alloc(3L * n);

@@ -564,7 +564,6 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
   const T* __restrict__ sval = reinterpret_cast<const T*>(A.sell_val);
   const int* __restrict__ scol = A.sell_col;
   const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
-  T* __restrict__ w = reinterpret_cast<T*>(A.p1);
   const int G = gridDim.x, b = blockIdx.x;
   const int lb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -582,9 +581,9 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
   for (int q = 0; q < 3; ++q) z0[q] = z[3 * c0 + q];
   const double* prev = rr_slot(A, k - 1);
   const int cprev = rr_count(A, k - 1);
-  double rr, bb, unused;
-  all_sum3<SELL_BS>(prev, cprev, 2, A.partb, k == 0 ? A.g1 : 0, 1, A.partb, 0, 1, red, rr, bb,
-                    unused);
+  double rr, bb, gam;
+  all_sum3<SELL_BS>(prev, cprev, 2, A.partb, k == 0 ? A.g1 : 0, 1, prev + 1, cprev, 2, red, rr,
+                    bb, gam);
   if (k == 0) {
     double thr = A.tol * A.tol * bb;
     if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
@@ -608,6 +607,14 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
     }
     return;
   }
+  // beta of this iteration (Chronopoulos-Gear: r.z over the previous r.z)
+  const double beta = k == 0 ? 0.0 : gam / A.cgs[2 * ((k - 1) & 1) + 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.cgs[4] = beta;
+    A.cgs[5] = gam;
+  }
+  const T tb = (T)beta;
+  T* __restrict__ q = reinterpret_cast<T*>(A.q);
   double dl = 0.0;
   for (int t = wave; t < A.sell_spw; t += WAVES) {
     const long slice = (long)lb * A.sell_spw + t;
@@ -633,9 +640,10 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
       y2 += V[384] * a0 + V[448] * a1 + V[512] * a2;
     }
     if (row < A.n) {
-      w[3 * row] = y0;
-      w[3 * row + 1] = y1;
-      w[3 * row + 2] = y2;
+      const T q0 = q[3 * row], q1 = q[3 * row + 1], q2 = q[3 * row + 2];
+      q[3 * row] = y0 + tb * q0;
+      q[3 * row + 1] = y1 + tb * q1;
+      q[3 * row + 2] = y2 + tb * q2;
       dl += (double)z[3 * row] * (double)y0 + (double)z[3 * row + 1] * (double)y1 +
             (double)z[3 * row + 2] * (double)y2;
     }
@@ -679,7 +687,7 @@ struct Cg1Batch {
   PreM<T> M[K2_NPT];
 };
 
-template <class T>
+template <class T, bool QF>
 __device__ __forceinline__ void cg1_load(const PcgArgs& A, long i0, long hi, Cg1Batch<T>& b) {
 #pragma unroll
   for (int j = 0; j < K2_NPT; ++j) {
@@ -687,7 +695,7 @@ __device__ __forceinline__ void cg1_load(const PcgArgs& A, long i0, long hi, Cg1
     if (i < hi) {
       b.Z[j] = ld3(reinterpret_cast<const T*>(A.z), i);
       b.P[j] = ld3(reinterpret_cast<const T*>(A.p0), i);
-      b.W[j] = ld3(reinterpret_cast<const T*>(A.p1), i);
+      if (!QF) b.W[j] = ld3(reinterpret_cast<const T*>(A.p1), i);
       b.Q[j] = ld3(reinterpret_cast<const T*>(A.q), i);
       b.X[j] = ld3(reinterpret_cast<const T*>(A.x), i);
       b.R[j] = ld3(reinterpret_cast<const T*>(A.r), i);
@@ -697,7 +705,9 @@ __device__ __forceinline__ void cg1_load(const PcgArgs& A, long i0, long hi, Cg1
   }
 }
 
-template <class T>
+// QF: the SpMV launch has formed q = w + beta q already and left beta and
+// r.z in cgs[4], cgs[5]; w is not read and q not written here.
+template <class T, bool QF>
 __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
   __shared__ double red[3 * K2_BS / 64 + 4];
   if (A.sc->done) return;
@@ -705,17 +715,18 @@ __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
   // the first batch is loaded before the partial sums: the two latencies overlap
   Cg1Batch<T> b;
   long i0 = rg.lo + threadIdx.x;
-  cg1_load<T>(A, i0, rg.hi, b);
+  cg1_load<T, QF>(A, i0, rg.hi, b);
   double gam, del, unused;
-  all_sum3<K2_BS>(rr_slot(A, k - 1) + 1, rr_count(A, k - 1), 2, A.part1, A.g1, 1, A.part1, 0, 1,
-                  red, gam, del, unused);
+  all_sum3<K2_BS>(rr_slot(A, k - 1) + 1, QF ? 0 : rr_count(A, k - 1), 2, A.part1, A.g1, 1,
+                  A.part1, 0, 1, red, gam, del, unused);
   double alpha, beta;
+  if (QF) gam = A.cgs[5];
   if (k == 0) {
     beta = 0.0;
     alpha = gam / del;
   } else {
     const double* prev = A.cgs + 2 * ((k - 1) & 1);
-    beta = gam / prev[1];
+    beta = QF ? A.cgs[4] : gam / prev[1];
     alpha = gam / (del - beta * gam / prev[0]);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -738,13 +749,13 @@ __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           pi.c[c] = b.Z[j].c[c] + tb * b.P[j].c[c];
-          qi.c[c] = b.W[j].c[c] + tb * b.Q[j].c[c];
+          qi.c[c] = QF ? b.Q[j].c[c] : b.W[j].c[c] + tb * b.Q[j].c[c];
           xi.c[c] = b.X[j].c[c] + ta * pi.c[c];
           ri.c[c] = b.R[j].c[c] - ta * qi.c[c];
         }
         const N3<T> zi = mul_precond<T>(A.precond, b.M[j], ri);
         st3(p, i, pi);
-        st3(q, i, qi);
+        if (!QF) st3(q, i, qi);
         st3(x, i, xi);
         st3(r, i, ri);
         st3(z, i, zi);
@@ -757,7 +768,7 @@ __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
     }
     i0 += (long)K2_NPT * K2_BS;
     if (i0 >= rg.hi) break;
-    cg1_load<T>(A, i0, rg.hi, b);
+    cg1_load<T, QF>(A, i0, rg.hi, b);
   }
   block_sum2<K2_BS>(rr, rz, red);
   if (threadIdx.x == 0) {
@@ -955,10 +966,17 @@ void launch_pcg_k1(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t
 void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t e0,
                    hipEvent_t e1) {
   if (a.cg1) {
-    if (fp32)
-      launch_timed(cg1_b_kernel<float>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
-    else
-      launch_timed(cg1_b_kernel<double>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
+    if (a.sell) {
+      if (fp32)
+        launch_timed(cg1_b_kernel<float, true>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
+      else
+        launch_timed(cg1_b_kernel<double, true>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
+    } else {
+      if (fp32)
+        launch_timed(cg1_b_kernel<float, false>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
+      else
+        launch_timed(cg1_b_kernel<double, false>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
+    }
     return;
   }
   if (fp32)
