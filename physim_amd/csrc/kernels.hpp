@@ -177,10 +177,10 @@ struct PcgArgs {
 int pcg_grid1(int n, int lpr);
 // SELL-64 layout: slices per workgroup and grid (<= PCG_MAXP) for ns slices
 void sell_grid(long ns, int& spw, int& grid);
-// copies the BSR3 values into the SELL-64 layout (map: source block per slot,
-// -1 for padding)
-void launch_sell_pack(hipStream_t s, bool fp32, long ns, const long* so, const int* map,
-                      const void* bval, void* sval);
+// copies the BSR3 values into the SELL-64 layout (brow: block row of each
+// block; padding slots are left as they are)
+void launch_sell_pack(hipStream_t s, bool fp32, long nb, const int* brow, const long* rowptr,
+                      const long* so, const void* bval, void* sval);
 int pcg_grid2(int n);
 // r = b - A x (b in T); z = M^-1 r; p = 0.  x_zero: start from x = 0.
 void launch_pcg_init(hipStream_t s, bool fp32, const PcgArgs& a, const void* b, int x_zero);

@@ -436,19 +436,15 @@ void pgn_model::sell_prepare() {
     so[sl + 1] = so[sl] + 64L * w;
   }
   const long slots = so[ns];
-  std::vector<int> col(slots, 0), map(slots, -1);
+  std::vector<int> col(slots, 0);  // padding slots: column 0, never used
   for (long sl = 0; sl < ns; ++sl)
     for (long r = sl * 64; r < std::min<long>(n, sl * 64 + 64); ++r)
-      for (int j = 0; j < len[r]; ++j) {
-        const long slot = so[sl] + 64L * j + (r - sl * 64);
-        col[slot] = pat.bcol[pat.rowptr[r] + j];
-        map[slot] = (int)(pat.rowptr[r] + j);
-      }
+      for (int j = 0; j < len[r]; ++j)
+        col[so[sl] + 64L * j + (r - sl * 64)] = pat.bcol[pat.rowptr[r] + j];
   hipStream_t s = stream();
   sell_so.upload(so, s);
   sell_len.upload(len, s);
   sell_col.upload(col, s);
-  sell_map.upload(map, s);
   sell_val.alloc((size_t)9 * std::max<long>(slots, 1) * tsize());
   sell_ns = ns;
   sell_grid(ns, sell_spw, sell_g);
@@ -459,7 +455,7 @@ void pgn_model::sell_prepare() {
 void pgn_model::sell_pack() {
   if (!use_sell) return;
   sell_prepare();
-  launch_sell_pack(stream(), hfp32, sell_ns, sell_so.p, sell_map.p, bval.p, sell_val.p);
+  launch_sell_pack(stream(), hfp32, pat.nb, brow.p, rowptr.p, sell_so.p, bval.p, sell_val.p);
 }
 
 PcgArgs pgn_model::pcg_args(double tol, int precond) const {

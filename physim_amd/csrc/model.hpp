@@ -95,7 +95,7 @@ struct pgn_model {
   long sell_ns = 0;
   int sell_spw = 0, sell_g = 0;
   pgn::DBuf<long> sell_so;
-  pgn::DBuf<int> sell_len, sell_col, sell_map;
+  pgn::DBuf<int> sell_len, sell_col;
   pgn::DBuf<char> sell_val;
   void sell_prepare();
   void sell_pack();

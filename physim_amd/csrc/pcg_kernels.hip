@@ -16,6 +16,8 @@
 // parity of the iteration that produced them; init counts as iteration -1.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "reduce.hpp"
 
@@ -652,32 +654,36 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
   if (threadIdx.x == 0) A.part1[blockIdx.x] = s;
 }
 
+// One thread per source block, in BSR3 order (coalesced reads); a slice's
+// writes land together in L2.  Padding slots are never written: a lane
+// stops at its row's length and never reads them.
 template <class T>
-__global__ __launch_bounds__(256) void sell_pack_kernel(long ns, const long* __restrict__ so,
-                                                        const int* __restrict__ map,
+__global__ __launch_bounds__(256) void sell_pack_kernel(long nb, const int* __restrict__ brow,
+                                                        const long* __restrict__ rowptr,
+                                                        const long* __restrict__ so,
                                                         const T* __restrict__ bval,
                                                         T* __restrict__ sval) {
-  const long slice = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (slice >= ns) return;
-  const long base = so[slice], wd = (so[slice + 1] - base) / 64;
-  for (long j = 0; j < wd; ++j) {
-    const int src = map[base + 64 * j + lane];
-    T* V = sval + (base + 64 * j) * 9 + lane;
+  for (long b = (long)blockIdx.x * 256 + threadIdx.x; b < nb; b += (long)gridDim.x * 256) {
+    const int r = brow[b];
+    const long j = b - rowptr[r];
+    T* V = sval + (so[r >> 6] + 64 * j) * 9 + (r & 63);
+    T v[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) V[64 * q] = src >= 0 ? bval[9L * src + q] : T(0);
+    for (int q = 0; q < 9; ++q) v[q] = bval[9 * b + q];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) V[64 * q] = v[q];
   }
 }
 
-void launch_sell_pack(hipStream_t s, bool fp32, long ns, const long* so, const int* map,
-                      const void* bval, void* sval) {
-  const int g = (int)((ns + 3) / 4);
+void launch_sell_pack(hipStream_t s, bool fp32, long nb, const int* brow, const long* rowptr,
+                      const long* so, const void* bval, void* sval) {
+  const int g = (int)std::min<long>((nb + 255) / 256, 4096);
   if (g <= 0) return;
   if (fp32)
-    hipLaunchKernelGGL(sell_pack_kernel<float>, dim3(g), dim3(256), 0, s, ns, so, map,
+    hipLaunchKernelGGL(sell_pack_kernel<float>, dim3(g), dim3(256), 0, s, nb, brow, rowptr, so,
                        reinterpret_cast<const float*>(bval), reinterpret_cast<float*>(sval));
   else
-    hipLaunchKernelGGL(sell_pack_kernel<double>, dim3(g), dim3(256), 0, s, ns, so, map,
+    hipLaunchKernelGGL(sell_pack_kernel<double>, dim3(g), dim3(256), 0, s, nb, brow, rowptr, so,
                        reinterpret_cast<const double*>(bval), reinterpret_cast<double*>(sval));
 }
 
