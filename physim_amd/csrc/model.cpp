@@ -96,7 +96,6 @@ using namespace pgn;
 
 pgn_model::~pgn_model() {
   for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
-  if (pcg_graph) (void)hipGraphExecDestroy(pcg_graph);
   if (hpin) (void)hipHostFree(hpin);
   if (hsc) (void)hipHostFree(hsc);
 }

@@ -123,9 +123,6 @@ struct pgn_model {
   pgn::PcgScalars* hsc = nullptr;
   bool hess_valid = false;
   int pcg_chunk_max = 64;
-  hipGraphExec_t pcg_graph = nullptr;
-  int pcg_graph_len = 0;
-  bool use_graphs = true;
   // live per-kernel timing of the PCG launches (HIP events on the stream)
   bool profiling = false;
   std::vector<hipEvent_t> prof_ev;
