@@ -268,9 +268,10 @@ def main():
     # the PCG's SpMV launch of the variant in use (PGN_PCG and PGN_SELL, as the
     # library reads them)
     variant = os.environ.get("PGN_PCG", "cg1")
-    if variant not in ("split", "fused") and os.environ.get("PGN_SELL") != "0":
-        spmv_name = "cg1_sell_kernel<float>"
-        spmv_desc = f"{spmv_name} (SELL-64 SpMV of the PCG)"
+    col_bytes = M.profile(5)[0]  # SELL-64 column width in use (0: none)
+    if variant not in ("split", "fused") and col_bytes:
+        spmv_name = f"cg1_sell_kernel<float, {'true' if col_bytes == 2 else 'false'}>"
+        spmv_desc = f"{spmv_name} (SELL-64 SpMV of the PCG, {col_bytes} B columns)"
     else:
         spmv_kernel = {"split": "pcg_k1b_kernel", "fused": "pcg_k1_kernel"}.get(
             variant, "cg1_a_kernel")

@@ -387,7 +387,9 @@ int pgn_model_set_profiling(pgn_model* model, int on);
 /* which: 0 = K1, 1 = K2: sampled launches that did work and their summed ms;
  * 2, 3, 4: the SpMV time (us, in total_ms) with 4, 8, 16 lanes per block
  * row, timed once per model, and in count the lanes per row in use (16 from
- * 12 mean blocks per row, 8 from 6, else 4; PGN_PCG_LPR overrides) */
+ * 12 mean blocks per row, 8 from 6, else 4; PGN_PCG_LPR overrides);
+ * 5: in count the bytes per block column of the SELL-64 copy (2: 16-bit
+ * offsets from the row, 4: 32-bit columns, 0: no SELL-64 copy) */
 int pgn_model_get_profile(pgn_model* model, int which, long* count,
                           double* total_ms);
 

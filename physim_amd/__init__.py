@@ -432,7 +432,9 @@ class TetModel:
         _ok(lib().pgn_model_set_profiling(self._h, 1 if on else 0))
 
     def profile(self, which):
-        """(launches, total ms) of PCG kernel `which` (0 = SpMV, 1 = update)."""
+        """(launches, total ms) of PCG kernel `which` (0 = SpMV, 1 = update);
+        2-4: (lanes per row in use, SpMV us at 4 / 8 / 16 lanes); 5: (bytes per
+        SELL-64 block column, 0 without the SELL-64 copy; 0)."""
         c = ctypes.c_long()
         ms = ctypes.c_double()
         _ok(lib().pgn_model_get_profile(self._h, which, ctypes.byref(c), ctypes.byref(ms)))

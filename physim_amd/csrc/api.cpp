@@ -563,10 +563,14 @@ int pgn_model_set_profiling(pgn_model* M, int on) {
 int pgn_model_get_profile(pgn_model* M, int which, long* count, double* total_ms) {
   return guard([&] {
     PGN_CHECK(M && count && total_ms, "null argument");
-    PGN_CHECK(which >= 0 && which <= 4, "unknown profile slot");
+    PGN_CHECK(which >= 0 && which <= 5, "unknown profile slot");
     if (which <= 1) {
       *count = M->prof_n[which];
       *total_ms = M->prof_ms[which];
+    } else if (which == 5) {  // bytes per SELL-64 block column, 0 without SELL
+      if (M->pcg_cg1 && M->use_sell) M->sell_prepare();
+      *count = !(M->pcg_cg1 && M->use_sell) ? 0 : (M->sell_c16.p ? 2 : 4);
+      *total_ms = 0.0;
     } else {  // SpMV us with 4 / 8 / 16 lanes per row; count = width in use
       *count = M->pcg_lpr;
       *total_ms = M->tune_us[which - 2];
@@ -610,9 +614,14 @@ int pgn_kernel_bytes(pgn_model* M, int which, double* bytes) {
         // SELL-64 (the C-G default): values + cols of the rows' own blocks
         // (padding slots are not read), a 4 B length per row and an 8 B
         // offset per slice instead of the row pointers; z and q read, q
-        // (= A z + beta q) written.
-        if (M->pcg_cg1 && M->use_sell)
-          *bytes = nb * (9 * hs + 4) + n * 4.0 + (double)(((long)n + 63) / 64) * 8.0 + 3 * 3 * n * hs;
+        // (= A z + beta q) written.  Columns take 2 B where the layout
+        // stores them as 16-bit offsets from the row.
+        if (M->pcg_cg1 && M->use_sell) {
+          M->sell_prepare();
+          const double cb = M->sell_c16.p ? 2.0 : 4.0;
+          *bytes = nb * (9 * hs + cb) + n * 4.0 + (double)(((long)n + 63) / 64) * 8.0 +
+                   3 * 3 * n * hs;
+        }
         else if (M->pcg_split || M->pcg_cg1)
           *bytes = nb * (9 * hs + 4) + (n + 1) * 8 + 2 * 3 * n * hs;
         else

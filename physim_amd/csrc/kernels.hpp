@@ -171,7 +171,8 @@ struct PcgArgs {
   long sell_ns = 0;      // slices
   const long* sell_so = nullptr;   // ns + 1 slot offsets
   const int* sell_len = nullptr;   // n: blocks of each row
-  const int* sell_col = nullptr;   // per slot: block column
+  const int* sell_col = nullptr;   // per slot: block column (null when sell_c16 is set)
+  const short* sell_c16 = nullptr; // per slot: block column minus the slot's row
   const void* sell_val = nullptr;  // 9 per slot, Hessian precision
 };
 int pcg_grid1(int n, int lpr);

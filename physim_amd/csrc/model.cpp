@@ -435,15 +435,43 @@ void pgn_model::sell_prepare() {
     so[sl + 1] = so[sl] + 64L * w;
   }
   const long slots = so[ns];
-  std::vector<int> col(slots, 0);  // padding slots: column 0, never used
-  for (long sl = 0; sl < ns; ++sl)
-    for (long r = sl * 64; r < std::min<long>(n, sl * 64 + 64); ++r)
-      for (int j = 0; j < len[r]; ++j)
-        col[so[sl] + 64L * j + (r - sl * 64)] = pat.bcol[pat.rowptr[r] + j];
+  // Columns as 16-bit offsets from the slot's row when every block of the
+  // pattern lies within 32767 nodes of the diagonal (PGN_SELL_C16=0 keeps
+  // 32-bit columns).  Padding slots hold the row itself, or, for the lanes
+  // past the last row, an offset to the last row, so that the kernel's
+  // unconditional first-block loads stay in bounds.
+  bool c16 = !(std::getenv("PGN_SELL_C16") && std::strcmp(std::getenv("PGN_SELL_C16"), "0") == 0);
+  for (int r = 0; r < n && c16; ++r)
+    for (long e = pat.rowptr[r]; e < pat.rowptr[r + 1]; ++e)
+      if (std::abs((long)pat.bcol[e] - r) > 32767) {
+        c16 = false;
+        break;
+      }
   hipStream_t s = stream();
+  if (c16) {
+    std::vector<short> col(slots, 0);
+    for (long sl = 0; sl < ns; ++sl)
+      for (long q = 0; q < 64; ++q) {
+        const long r = sl * 64 + q;
+        if (r >= n) {
+          const long w = (so[sl + 1] - so[sl]) / 64;
+          for (long j = 0; j < w; ++j) col[so[sl] + 64L * j + q] = (short)(n - 1 - r);
+          continue;
+        }
+        for (int j = 0; j < len[r]; ++j)
+          col[so[sl] + 64L * j + q] = (short)(pat.bcol[pat.rowptr[r] + j] - r);
+      }
+    sell_c16.upload(col, s);
+  } else {
+    std::vector<int> col(slots, 0);  // padding slots: column 0, never used
+    for (long sl = 0; sl < ns; ++sl)
+      for (long r = sl * 64; r < std::min<long>(n, sl * 64 + 64); ++r)
+        for (int j = 0; j < len[r]; ++j)
+          col[so[sl] + 64L * j + (r - sl * 64)] = pat.bcol[pat.rowptr[r] + j];
+    sell_col.upload(col, s);
+  }
   sell_so.upload(so, s);
   sell_len.upload(len, s);
-  sell_col.upload(col, s);
   sell_val.alloc((size_t)9 * std::max<long>(slots, 1) * tsize());
   sell_ns = ns;
   sell_grid(ns, sell_spw, sell_g);
@@ -472,6 +500,7 @@ PcgArgs pgn_model::pcg_args(double tol, int precond) const {
     a.sell_so = sell_so.p;
     a.sell_len = sell_len.p;
     a.sell_col = sell_col.p;
+    a.sell_c16 = sell_c16.p;
     a.sell_val = sell_val.p;
   }
   a.g2 = pcg_grid2(n);

@@ -96,6 +96,7 @@ struct pgn_model {
   int sell_spw = 0, sell_g = 0;
   pgn::DBuf<long> sell_so;
   pgn::DBuf<int> sell_len, sell_col;
+  pgn::DBuf<short> sell_c16;  // 16-bit column deltas, used instead of sell_col when all fit
   pgn::DBuf<char> sell_val;
   void sell_prepare();
   void sell_pack();

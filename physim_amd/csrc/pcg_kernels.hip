@@ -557,7 +557,8 @@ __global__ __launch_bounds__(K1_BS) void cg1_a_kernel(PcgArgs A, int k) {
 // slice of 64 rows, one row per lane, walking the row's blocks in column
 // order; every value and column load of the wavefront is 64 consecutive
 // entries.  Workgroups take spw consecutive slices (blocked per XCD).
-template <class T>
+// C16: block columns stored as 16-bit offsets from the slot's row.
+template <class T, bool C16>
 __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
   __shared__ double red[3 * SELL_BS / 64 + 4];
   constexpr int WAVES = SELL_BS / 64;
@@ -565,6 +566,7 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
   if (sc->done) return;
   const T* __restrict__ sval = reinterpret_cast<const T*>(A.sell_val);
   const int* __restrict__ scol = A.sell_col;
+  const short* __restrict__ s16 = A.sell_c16;
   const T* __restrict__ z = reinterpret_cast<const T*>(A.z);
   const int G = gridDim.x, b = blockIdx.x;
   const int lb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
@@ -575,7 +577,7 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
   const long s0 = min((long)lb * A.sell_spw + wave, A.sell_ns - 1);
   const long base0 = A.sell_so[s0];
   const int len0 = A.sell_len[min(s0 * 64 + lane, (long)A.n - 1)];
-  const long c0 = scol[base0 + lane];
+  const long c0 = C16 ? s0 * 64 + lane + s16[base0 + lane] : scol[base0 + lane];
   T v0[9], z0[3];
 #pragma unroll
   for (int q = 0; q < 9; ++q) v0[q] = sval[base0 * 9 + lane + 64 * q];
@@ -634,7 +636,7 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
       j = 1;
     }
     for (; j < len; ++j) {
-      const long c = scol[base + 64L * j + lane];
+      const long c = C16 ? row + s16[base + 64L * j + lane] : scol[base + 64L * j + lane];
       const T* V = sval + (base + 64L * j) * 9 + lane;
       const T a0 = z[3 * c], a1 = z[3 * c + 1], a2 = z[3 * c + 2];
       y0 += V[0] * a0 + V[64] * a1 + V[128] * a2;
@@ -926,10 +928,18 @@ template <int L>
 static void launch_k1_l(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t e0,
                         hipEvent_t e1) {
   if (a.cg1 && a.sell) {
-    if (fp32)
-      launch_timed(cg1_sell_kernel<float>, dim3(a.g1), dim3(SELL_BS), s, e0, e1, a, k);
-    else
-      launch_timed(cg1_sell_kernel<double>, dim3(a.g1), dim3(SELL_BS), s, e0, e1, a, k);
+    if (a.sell_c16) {
+      if (fp32)
+        launch_timed(cg1_sell_kernel<float, true>, dim3(a.g1), dim3(SELL_BS), s, e0, e1, a, k);
+      else
+        launch_timed(cg1_sell_kernel<double, true>, dim3(a.g1), dim3(SELL_BS), s, e0, e1, a, k);
+    } else {
+      if (fp32)
+        launch_timed(cg1_sell_kernel<float, false>, dim3(a.g1), dim3(SELL_BS), s, e0, e1, a, k);
+      else
+        launch_timed(cg1_sell_kernel<double, false>, dim3(a.g1), dim3(SELL_BS), s, e0, e1, a,
+                     k);
+    }
     return;
   }
   if (a.cg1) {

@@ -10,6 +10,7 @@ import sys
 
 import numpy as np
 import pytest
+import scipy.sparse
 
 import physim_amd as pg
 from oracle import orc
@@ -204,6 +205,60 @@ def test_sell_and_bsr_spmv_solve_the_same_system(ctx, precision, monkeypatch):
         sols.append(xg)
         G.close()
     assert np.linalg.norm(sols[0] - sols[1]) <= 1e-4 * np.linalg.norm(sols[1])
+
+
+@pytest.mark.parametrize("precision", [pg.FP64, pg.MIXED])
+def test_sell_16bit_columns_match_32bit(ctx, precision, monkeypatch):
+    """The SELL-64 copy with block columns as 16-bit offsets from the row
+    (default where they fit) and as 32-bit columns (PGN_SELL_C16=0, read when
+    the copy is first built): the same products in the same order, so the
+    same iterations and bit-identical solutions.  The 400-node beam's ragged
+    seventh slice has 48 lanes past the last row, whose padding offsets the
+    first-block loads read."""
+    _say(f"sell 16-bit columns precision={precision}")
+    s = small_beam(dims=(4, 4, 15), seed=61)
+    x = perturbed_x(s, 1e-3, seed=62)
+    b = np.random.default_rng(64).standard_normal(s.n * 3)
+    b[s.fixed.reshape(-1).astype(bool)] = 0.0
+    out = []
+    for c16, width in (("1", 2), ("0", 4)):
+        monkeypatch.setenv("PGN_SELL_C16", c16)
+        G = pgn_model(ctx, s, precision)
+        G.set_x(x)
+        xg, info = G.linear_solve(b, tol=1e-10, max_iters=5000, precond=1)
+        assert info["status"] == 0, info
+        assert G.profile(5)[0] == width
+        out.append((xg, info["iterations"]))
+        G.close()
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
+
+
+def test_sell_wide_band_keeps_32bit_columns(ctx):
+    """A 36,421-node beam with its node ids shuffled: blocks lie up to ~36,000
+    nodes off the diagonal, so the SELL-64 copy keeps 32-bit columns by
+    itself, and the solve reaches the FP64 residual of the assembled matrix
+    (checked with a sparse product of the exported BSR3 matrix)."""
+    _say("sell wide band")
+    s = small_beam(dims=(10, 10, 300), seed=65)
+    perm = np.random.default_rng(66).permutation(s.n)
+    X0 = np.empty_like(s.X0)
+    X0[perm] = s.X0
+    fixed = np.empty_like(s.fixed)
+    fixed[perm] = s.fixed
+    sp_ = scenes.Scene("shuffled beam", X0, perm[s.T].astype(np.int32), s.material, s.lame1,
+                       s.lame2, s.density, fixed, s.gravity, s.stages)
+    G = pgn_model(ctx, sp_, pg.MIXED)
+    b = np.random.default_rng(67).standard_normal(s.n * 3)
+    b[fixed.reshape(-1).astype(bool)] = 0.0
+    xg, info = G.linear_solve(b, tol=1e-8, max_iters=20000, precond=1)
+    _say(f"  {info}")
+    assert info["status"] == 0, info
+    assert G.profile(5)[0] == 4
+    rp, ci, v = G.hessian(pg.LAYOUT_BSR3)
+    A = scipy.sparse.bsr_matrix((v.reshape(-1, 3, 3), ci, rp), shape=(G.N, G.N))
+    assert np.linalg.norm(A @ xg - b) <= 1e-8 * np.linalg.norm(b) * 1.01
+    G.close()
 
 
 @pytest.mark.parametrize("precision", [pg.FP64, pg.MIXED])
