@@ -383,6 +383,7 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
         mask.upload(zero, s);
         launch_batch_copy(s, ba, n, mask.p, b, x);
       }
+      std::vector<int> nref(Kn, 0);
       for (;;) {
         launch_residual64(s, true, (int)n, M->rowptr.p, M->bcol.p, M->bval.p, x, b, M->r64.p,
                           M->red64.p, M->scal.p + 4);
@@ -394,10 +395,12 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
         for (int k = 0; k < Kn; ++k) {
           if (!act[k]) continue;
           const double thr = tol[k] * tol[k] * bb[k];
-          if (!(rr[k] >= thr) || iters[k] >= o.max_iters) {  // also stops on NaN
+          // also stops on NaN
+          if (!(rr[k] >= thr) || iters[k] >= o.max_iters || nref[k] >= MAX_REFINEMENTS) {
             act[k] = 0;
             continue;
           }
+          ++nref[k];
           inner[k] = std::max(o.inner_tol, tol[k] * std::sqrt(bb[k] / rr[k]));
           min_it = std::min(min_it, iters[k]);
           any_act = true;
@@ -412,7 +415,9 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
         for (int k = 0; k < Kn; ++k) {
           if (!act[k]) continue;
           iters[k] += itk[k];
-          if (itk[k] == 0) act[k] = 0;  // no progress possible in FP32
+          // the residual rounded to zero in FP32: no FP32 correction can
+          // follow (a count of 0 alone can still be one iteration taken)
+          if (hseg_sc[k].bb == 0.0) act[k] = 0;
         }
       }
     }

@@ -123,6 +123,8 @@ void launch_frob_lower(hipStream_t s, bool fp32, long nb, const int* brow,
 // and every consumer workgroup of the next launch re-sums the partials in a
 // fixed order (identical bits everywhere; kernel boundaries give coherence).
 constexpr int PCG_MAXP = 2048;      // capacity: producer workgroups of one partial set
+// cap on FP32 corrections per mixed-precision linear solve
+constexpr int MAX_REFINEMENTS = 64;
 constexpr int PCG_GRID_CAP = 1024;  // grid cap of the BSR3 SpMV / update / residual launches
 
 struct PcgScalars {

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <limits>
 
 namespace pgn {
 
@@ -103,7 +104,8 @@ static void solve_once(pgn_model& M, const double* b, double* x, const pgn_linea
     PGN_HIP(hipMemcpyAsync(M.hpin + 4, M.scal.p + 4, sizeof(double), hipMemcpyDeviceToHost, s));
     M.sync();
     rr = M.hpin[4];
-    if (!(rr >= thr) || iters >= o.max_iters) break;  // also stops on NaN
+    // also stops on NaN
+    if (!(rr >= thr) || iters >= o.max_iters || refinements >= MAX_REFINEMENTS) break;
     const double inner = std::max(o.inner_tol, o.tol * std::sqrt(bb / rr));
     launch_cast_f64_to_f32(s, N, M.r64.p, reinterpret_cast<float*>(M.pb.p));
     double irr = 0, ibb = 0;
@@ -111,7 +113,11 @@ static void solve_once(pgn_model& M, const double* b, double* x, const pgn_linea
     launch_axpy_f32_to_f64(s, N, reinterpret_cast<const float*>(px), x);
     iters += it;
     ++refinements;
-    if (it == 0) break;  // no progress possible in FP32
+    // The residual rounded to zero in FP32: no FP32 correction can follow.
+    // From x = 0 the PCG otherwise always takes a step (inner <= 1 here), even
+    // when Eigen's count says 0: that count is 0 also when one iteration ran
+    // and met the tolerance.
+    if (ibb == 0.0) break;
   }
   relres = std::sqrt(rr / bb);
 }

@@ -234,6 +234,39 @@ def test_sell_16bit_columns_match_32bit(ctx, precision, monkeypatch):
     assert np.array_equal(out[0][0], out[1][0])
 
 
+@pytest.mark.parametrize("dims", [None, (1, 1, 2)])
+def test_sell_models_smaller_than_a_slice(ctx, dims):
+    """Models of 4 and 12 nodes: one SELL-64 slice with 60 and 52 lanes past
+    the last row.  The PCG solve reaches the FP64 residual of the exported
+    matrix, and equals numpy's dense solve of it within its conditioning."""
+    _say(f"sell tiny model dims={dims}")
+    if dims is None:
+        X0 = np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [0.0, 0.1, 0.0], [0.02, 0.03, 0.1]])
+        l1, l2 = orc.lame(1e5, 0.3)
+        fixed = np.zeros((4, 3), dtype=np.uint8)
+        fixed[:3] = 1
+        s = scenes.Scene("one_tet", X0, np.array([[0, 1, 2, 3]], dtype=np.int32),
+                         np.array([0], dtype=np.uint8), np.array([l1]), np.array([l2]),
+                         np.array([1000.0]), fixed, scenes.G_EARTH, 1)
+    else:
+        s = small_beam(dims=dims, seed=68)
+    assert s.n < 64
+    for precision in (pg.FP64, pg.MIXED):
+        G = pgn_model(ctx, s, precision)
+        G.set_x(perturbed_x(s, 1e-3, seed=69))
+        b = np.random.default_rng(70).standard_normal(s.n * 3)
+        b[s.fixed.reshape(-1).astype(bool)] = 0.0
+        xg, info = G.linear_solve(b, tol=1e-10, max_iters=1000, precond=1)
+        assert info["status"] == 0, info
+        assert G.profile(5)[0] == 2
+        rp, ci, v = G.hessian(pg.LAYOUT_LOWER)
+        A = orc.lower_to_dense(rp, ci, v)
+        assert np.linalg.norm(A @ xg - b) <= 1e-10 * np.linalg.norm(b) * 1.01
+        xd = np.linalg.solve(A, b)
+        assert np.linalg.norm(xg - xd) <= 1e-10 * np.linalg.cond(A) * np.linalg.norm(xd) * 1.01
+        G.close()
+
+
 def test_sell_wide_band_keeps_32bit_columns(ctx):
     """A 36,421-node beam with its node ids shuffled: blocks lie up to ~36,000
     nodes off the diagonal, so the SELL-64 copy keeps 32-bit columns by
