@@ -115,9 +115,9 @@ def main():
                     help="0 Jacobi, 1 3x3 block Jacobi (default)")
     ap.add_argument("--adaptive", type=int, default=1,
                     help="1 (default): Eisenstat-Walker linear tolerance")
-    ap.add_argument("--ls-max-iters", type=int, default=2000,
+    ap.add_argument("--ls-max-iters", type=int, default=2500,
                     help="PCG iteration cap per linear solve of the timed path "
-                         "(default 2000; the reference's lsNumMaxIters is 1000, kept "
+                         "(default 2500; the reference's lsNumMaxIters is 1000, kept "
                          "by the plain FP64 path)")
     ap.add_argument("--sweep", action="store_true",
                     help="also time one solve per (precond, adaptive) variant")

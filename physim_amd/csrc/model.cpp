@@ -504,6 +504,10 @@ PcgArgs pgn_model::pcg_args(double tol, int precond) const {
     a.sell_val = sell_val.p;
   }
   a.g2 = pcg_grid2(n);
+  // PGN_K2_GRID overrides the update launch's grid (diagnostics: it changes
+  // only the summation order of the r.r / r.z partials)
+  if (const char* e = std::getenv("PGN_K2_GRID"))
+    a.g2 = std::max(1, std::min(PCG_GRID_CAP, std::atoi(e)));
   a.tol = tol;
   a.partb = partb.p;
   const size_t v = 3L * n * tsize();
