@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <limits>
 #include <new>
 
@@ -234,6 +236,30 @@ void pgn_batch::seg_plan(const std::vector<uint8_t>& act) {
     seg_tol.alloc(Kn);
   }
   hseg_sc.resize(Kn);
+  if (!bcol16_done) {
+    // block columns as 16-bit offsets from their rows when every block lies
+    // within 32767 nodes of the diagonal (instances never couple, so the
+    // widest instance band decides); PGN_SELL_C16=0 keeps 32-bit columns
+    bcol16_done = true;
+    const char* e = std::getenv("PGN_SELL_C16");
+    bool fit = !(e && std::strcmp(e, "0") == 0);
+    const auto& P = M->pat;
+    std::vector<short> c16;
+    if (fit) c16.resize(P.nb);
+    for (int r = 0; r < P.n && fit; ++r)
+      for (long q = P.rowptr[r]; q < P.rowptr[r + 1]; ++q) {
+        const long d = (long)P.bcol[q] - r;
+        if (d > 32767 || d < -32767) {
+          fit = false;
+          break;
+        }
+        c16[q] = (short)d;
+      }
+    if (fit && P.nb > 0) {
+      bcol16.upload(c16, s);
+      PGN_HIP(hipStreamSynchronize(s));  // c16 is released at the end of this block
+    }
+  }
   seg_W = W;
   seg_lpr = lpr;
   seg_act = act;
@@ -253,6 +279,7 @@ SegPcgArgs pgn_batch::seg_args(int precond) const {
   a.tol = seg_tol.p;
   a.rowptr = M->rowptr.p;
   a.bcol = M->bcol.p;
+  a.bcol16 = bcol16.p;
   a.bval = M->bval.p;
   a.dinv = M->dinv.p;
   a.binv = M->binv.p;

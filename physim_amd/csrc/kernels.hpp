@@ -218,6 +218,7 @@ struct SegPcgArgs {
   const double* tol;     // K relative tolerances
   const long* rowptr;
   const int* bcol;
+  const short* bcol16 = nullptr;  // block columns minus their rows, when all fit in 16 bits
   const void* bval;
   const void* dinv;
   const void* binv;

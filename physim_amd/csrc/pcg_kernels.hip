@@ -1292,7 +1292,7 @@ __global__ __launch_bounds__(K1_BS) void seg_init_kernel(SegPcgArgs A, const T* 
   }
 }
 
-template <class T, int L>
+template <class T, int L, bool C16>
 __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
   __shared__ double red[2 * K1_BS / 64 + 2];
   const int w = blockIdx.x, sg = A.wg_seg[w], j = A.wg_loc[w], G = A.seg_G[sg];
@@ -1347,7 +1347,7 @@ __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
     T y0 = 0, y1 = 0, y2 = 0;
     const long e1 = A.rowptr[row + 1];
     for (long s = A.rowptr[row] + lane; s < e1; s += L) {
-      const long c = A.bcol[s];
+      const long c = C16 ? row + A.bcol16[s] : A.bcol[s];
       const T* B = bval + s * 9;
       const T pj0 = z[3 * c] + tb * pold[3 * c];
       const T pj1 = z[3 * c + 1] + tb * pold[3 * c + 1];
@@ -1491,10 +1491,17 @@ void launch_seg_init(hipStream_t s, bool fp32, const SegPcgArgs& a, const void* 
 
 template <int L>
 static void seg_k1_l(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {
-  if (fp32)
-    hipLaunchKernelGGL((seg_k1_kernel<float, L>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
-  else
-    hipLaunchKernelGGL((seg_k1_kernel<double, L>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+  if (a.bcol16) {
+    if (fp32)
+      hipLaunchKernelGGL((seg_k1_kernel<float, L, true>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+    else
+      hipLaunchKernelGGL((seg_k1_kernel<double, L, true>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+  } else {
+    if (fp32)
+      hipLaunchKernelGGL((seg_k1_kernel<float, L, false>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+    else
+      hipLaunchKernelGGL((seg_k1_kernel<double, L, false>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+  }
 }
 
 void launch_seg_k1(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {

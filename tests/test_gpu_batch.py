@@ -85,6 +85,31 @@ def test_batch_newton_matches_oracle(ctx, precision):
         assert err <= 1e-6 * s.bbox_diag(), (k, err)
 
 
+@pytest.mark.parametrize("precision", [pg.FP64, pg.MIXED])
+def test_batch_16bit_columns_match_32bit(ctx, precision, monkeypatch):
+    """The segmented SpMV with block columns as 16-bit offsets from their
+    rows (default where they fit) and as 32-bit columns (PGN_SELL_C16=0,
+    read at a batch's first solve): the same products in the same order, so
+    the same iteration counts and bit-identical positions."""
+    _say(f"batch 16-bit columns precision={precision}")
+    sc = _small_mix(4)
+    out = []
+    for c16 in ("1", "0"):
+        monkeypatch.setenv("PGN_SELL_C16", c16)
+        B = pg.Batch(ctx, precision)
+        for s in sc:
+            B.add_scene(s)
+        B.build()
+        res = B.newton(**OPTS)
+        assert all(r["result"] == "SUCCESS" for r in res), res
+        out.append(([(r["iterations"], r["pcg_iterations"]) for r in res],
+                    [B.get_x(k).copy() for k in range(len(sc))]))
+        B.close()
+    assert out[0][0] == out[1][0]
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+
+
 def test_batch_result_same_alone_and_in_mix(ctx):
     _say("batch alone vs mix")
     sc = _small_mix(3)
