@@ -915,7 +915,7 @@ static void solve_step(orc_problem* P, const orc_newton_opts* o, nstate* S,
   if (len > o->maxStepSize)
     for (int k = 0; k < N; ++k) S->dx[k] = S->dx[k] / len * o->maxStepSize;
   double factor = 1.0, objNew = 0;
-  int numBis = -1;
+  int numBis = -1, guarded = 0;
   do {
     numBis++;
     for (int k = 0; k < N; ++k) S->dxo[k] = factor * S->dx[k];
@@ -928,17 +928,18 @@ static void solve_step(orc_problem* P, const orc_newton_opts* o, nstate* S,
         objNew <= S->obj + o->ls_noise_guard * P->energy_scale(P->p) &&
         norm2(N, S->vgNew) < S->opt) {
       /* energy change below rounding: accept on the gradient norm and book
-       * it as the smallest positive improvement */
-      objNew = S->obj - 1e-300;
+       * it as an improvement of zero (the decrease branch below) */
+      guarded = 1;
       break;
     }
     factor *= o->lSearch_alpha;
     P->set_x(P->p, S->xsave); /* PrePerformStep */
   } while (norm2(N, S->dxo) > o->tolMinStep && numBis <= o->lsearch_numTry);
   S->bis += numBis;
-  const double impr = S->obj - objNew;
+  double impr = S->obj - objNew;
   const double step = norm2(N, S->dxo);
-  if (impr > 0) {
+  if (guarded && impr < 0) impr = 0;
+  if (impr > 0 || guarded) {
     const double optNew = norm2(N, S->vgNew);
     if (impr < o->tolMinImpr) {
       S->minImprCount++;

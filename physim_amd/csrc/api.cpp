@@ -220,17 +220,18 @@ int pgn_model_num_elements(const pgn_model* model) { return model ? model->m : 0
 
 size_t pgn_model_device_bytes(const pgn_model* M) {
   if (!M) return 0;
-  return M->X0.bytes() + M->x.bytes() + M->tets.bytes() + M->kind.bytes() +
-         M->prm64.bytes() + M->prm32.bytes() + M->mass.bytes() + M->fixmask.bytes() +
-         M->inc_ptr.bytes() + M->inc.bytes() + M->rowptr.bytes() + M->cptr.bytes() +
-         M->diag.bytes() + M->bcol.bytes() + M->brow.bytes() + M->code.bytes() +
-         M->bval.bytes() + M->dinv.bytes() + M->binv.bytes() + M->eg64.bytes() +
-         M->eg32.bytes() + M->eH.bytes() + M->eH_alt.bytes() + M->blockE.bytes() +
-         M->nodeP.bytes() + M->scal.bytes() + M->g.bytes() + M->f_ini.bytes() +
-         M->f_end.bytes() + M->pv.bytes() + M->pb.bytes() + M->part1.bytes() +
-         M->part2.bytes() + M->red64.bytes() + M->sc.bytes() + M->r64.bytes() +
-         M->tmp64a.bytes() + M->tmp64b.bytes() + M->nw_xsave.bytes() +
-         M->nw_v0.bytes() + M->nw_dx.bytes() + M->nw_b.bytes();
+  // every device allocation the model holds (its matrix copies, PCG, Newton,
+  // time-step and direct-solver buffers included)
+  return sum_bytes(M->X0, M->x, M->tets, M->kind, M->prm64, M->prm32, M->mass, M->fixmask,
+                   M->inc_ptr, M->inc, M->rowptr, M->cptr, M->diag, M->bcol, M->brow,
+                   M->code, M->bval, M->dinv, M->binv, M->eg64, M->eg32, M->eH, M->eH_alt,
+                   M->blockE, M->nodeP, M->scal, M->g, M->f_ini, M->f_end, M->pv, M->pb,
+                   M->part1, M->part2, M->partb, M->red64, M->cgs, M->sell_so, M->sell_len,
+                   M->sell_col, M->sell_c16, M->sell_val, M->sc, M->r64, M->tmp64a,
+                   M->tmp64b, M->nw_xsave, M->nw_v0, M->nw_dx, M->nw_b, M->vel, M->xhat,
+                   M->xstep0, M->ldlt_prob, M->ldlt_perm, M->ldlt_iperm, M->ldlt_nprob,
+                   M->ldlt_list, M->ldlt_status, M->ldlt_band, M->ldlt_D, M->ldlt_sinT,
+                   M->ldlt_y);
 }
 
 int pgn_model_set_fixed(pgn_model* M, const unsigned char* mask) {

@@ -476,7 +476,7 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
   enum { ACTIVE = 0, DONE = 1 };
   std::vector<int> st(Kn, ACTIVE), result(Kn, PGN_OS_SUCCESS), minImpr(Kn, 0), minStep(Kn, 0),
       steep(Kn, 0), its(Kn, 0), bis(Kn, 0), evals(Kn, 0), hevals(Kn, 0), pcg(Kn, 0),
-      nbis(Kn, 0), searching(Kn, 0);
+      nbis(Kn, 0), searching(Kn, 0), guarded(Kn, 0);
   std::vector<uint8_t> hmode(Kn, 0), hmask(Kn, 0);
   std::vector<double> obj(Kn), opt(Kn), optStart(Kn), objNew(Kn), optNew(Kn), f(Kn), len(Kn),
       clamp(Kn), stepLen(Kn), hfac(Kn), eta(Kn, o.ew_max), gprev(Kn, -1.0), hn2(Kn);
@@ -569,6 +569,7 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
       f[k] = 1.0;
       nbis[k] = -1;
       searching[k] = 1;
+      guarded[k] = 0;
     }
     // LineSearch_Simple (OptimSolver.cpp:688-736), instances in lockstep
     for (;;) {
@@ -600,7 +601,7 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
         }
         if (o.ls_noise_guard > 0.0 && en <= obj[k] + o.ls_noise_guard * E[4 * k + 2] &&
             gn < opt[k]) {
-          objNew[k] = obj[k] - 1e-300;
+          guarded[k] = 1;  // booked as an improvement of zero
           searching[k] = 0;
           continue;
         }
@@ -623,8 +624,9 @@ void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
       hmask[k] = 0;
       if (!hmode[k]) continue;
       bis[k] += nbis[k];
-      const double impr = obj[k] - objNew[k];
-      if (impr > 0) {
+      double impr = obj[k] - objNew[k];
+      if (guarded[k] && impr < 0) impr = 0;
+      if (impr > 0 || guarded[k]) {
         if (impr < o.tolMinImpr) {
           minImpr[k]++;
           result[k] = PGN_OS_MINIMPR;
@@ -991,9 +993,14 @@ int pgn_batch_num_instances(const pgn_batch* B) { return B ? B->K() : 0; }
 
 size_t pgn_batch_device_bytes(const pgn_batch* B) {
   if (!B || !B->built()) return 0;
-  return pgn_model_device_bytes(B->M.get()) + B->noff.bytes() + B->eoff.bytes() +
-         B->node_inst.bytes() + B->elemE.bytes() + B->nodeE.bytes() + B->nodeG2.bytes() +
-         B->xsave.bytes() + B->v0.bytes() + B->dx.bytes() + B->bvec.bytes();
+  return pgn_model_device_bytes(B->M.get()) +
+         sum_bytes(B->noff, B->eoff, B->node_inst, B->grav, B->falpha, B->fac, B->elemE,
+                   B->nodeE, B->nodeG2, B->out, B->n2, B->mode, B->mask, B->xsave, B->v0,
+                   B->dx, B->bvec, B->wg_seg, B->wg_loc, B->seg_G, B->seg_wg0, B->seg_row0,
+                   B->seg_row1, B->seg_tol, B->seg_part1, B->seg_part2, B->seg_partb,
+                   B->kvec, B->kvec2, B->xprev, B->seg_sc, B->bcol16, B->ldlt_prob,
+                   B->ldlt_perm, B->ldlt_iperm, B->ldlt_list, B->ldlt_status, B->ldlt_band,
+                   B->ldlt_D, B->ldlt_sinT, B->ldlt_y);
 }
 
 int pgn_batch_reset(pgn_batch* B) {

@@ -76,6 +76,12 @@ struct DBuf {
   }
 };
 
+// total size of a set of device buffers
+template <class... B>
+size_t sum_bytes(const B&... b) {
+  return (size_t(0) + ... + b.bytes());
+}
+
 }  // namespace pgn
 
 struct pgn_ctx {

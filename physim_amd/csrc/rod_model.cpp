@@ -279,7 +279,8 @@ void pgn_rods::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
   enum { ACTIVE = 0, DONE = 1 };
   std::vector<int> st(nrods, ACTIVE), result(nrods, PGN_OS_SUCCESS), minImpr(nrods, 0),
       minStep(nrods, 0), steep(nrods, 0), its(nrods, 0), bis(nrods, 0), evals(nrods, 0),
-      hevals(nrods, 0), stepping(nrods, 0), nbis(nrods, 0), searching(nrods, 0);
+      hevals(nrods, 0), stepping(nrods, 0), nbis(nrods, 0), searching(nrods, 0),
+      guarded(nrods, 0);
   std::vector<double> obj(nrods), opt(nrods), objNew(nrods), optNew(nrods), f(nrods),
       len(nrods), stepLen(nrods), escale(nrods);
   std::vector<unsigned char> hsel(nrods), hact(nrods);
@@ -356,6 +357,7 @@ void pgn_rods::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
       f[r] = 1.0;
       nbis[r] = -1;
       searching[r] = 1;
+      guarded[r] = 0;
     }
     // LineSearch_Simple, all rods in lockstep
     for (;;) {
@@ -388,7 +390,7 @@ void pgn_rods::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
         }
         if (o.ls_noise_guard > 0.0 && en <= obj[r] + o.ls_noise_guard * E[4 * r + 2] &&
             gn < opt[r]) {
-          objNew[r] = obj[r] - 1e-300;
+          guarded[r] = 1;  // booked as an improvement of zero
           searching[r] = 0;
           continue;
         }
@@ -408,8 +410,9 @@ void pgn_rods::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
     for (int r = 0; r < nrods; ++r) {
       if (!stepping[r]) continue;
       bis[r] += nbis[r];
-      const double impr = obj[r] - objNew[r];
-      if (impr > 0) {
+      double impr = obj[r] - objNew[r];
+      if (guarded[r] && impr < 0) impr = 0;
+      if (impr > 0 || guarded[r]) {
         if (impr < o.tolMinImpr) {
           minImpr[r]++;
           result[r] = PGN_OS_MINIMPR;

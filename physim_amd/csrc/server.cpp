@@ -58,6 +58,7 @@ struct pgn_server {
   size_t used = 0;
   long next_ticket = 1;
   bool stop = false;
+  int waiters = 0;  // threads inside pgn_server_wait
   std::vector<std::thread> lanes;
   std::vector<pgn_ctx*> ctxs;
   long batches = 0, solved = 0;
@@ -285,7 +286,21 @@ void pgn_server_destroy(pgn_server* S) {
     S->stop = true;
   }
   S->cv_work.notify_all();
-  for (auto& t : S->lanes) t.join();
+  for (auto& t : S->lanes) t.join();  // running batches finish first
+  {
+    // requests never started end with an error; waiters wake, read it and
+    // leave before the server (its mutex and condition variables) goes away
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (auto& r : S->pending) {
+      r->status = PGN_ERR_INVALID;
+      r->error = "server destroyed before the request started";
+      r->done = true;
+      r->inst = pgn_batch_inst{};
+    }
+    S->pending.clear();
+    S->cv_done.notify_all();
+    S->cv_done.wait(lk, [&] { return S->waiters == 0; });
+  }
   for (pgn_ctx* c : S->ctxs) pgn_ctx_destroy(c);
   delete S;
 }
@@ -379,8 +394,13 @@ int pgn_server_wait(pgn_server* S, long ticket, double* x, pgn_newton_result* re
       auto it = S->all.find(ticket);
       PGN_CHECK(it != S->all.end(), "unknown ticket");
       r = it->second;
+      ++S->waiters;
       S->cv_done.wait(lk, [&] { return r->done; });
-      S->all.erase(it);
+      --S->waiters;
+      // another wait on the same ticket may have removed it meanwhile
+      auto again = S->all.find(ticket);
+      if (again != S->all.end() && again->second == r) S->all.erase(again);
+      if (S->waiters == 0) S->cv_done.notify_all();
     }
     if (r->status != PGN_OK) throw Error(r->status, "request failed: " + r->error);
     if (x) std::copy(r->x.begin(), r->x.end(), x);

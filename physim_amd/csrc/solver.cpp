@@ -277,6 +277,7 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
     // LineSearch_Simple (OptimSolver.cpp:688-736)
     double factor = 1.0, objNew = 0.0, ggNew = 0.0, stepLen = 0.0, escale = 0.0;
     int numBis = -1;
+    bool guarded = false;
     do {
       ++numBis;
       stepLen = factor * len;
@@ -288,8 +289,8 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
       if (o.ls_noise_guard > 0.0 && objNew <= obj + o.ls_noise_guard * escale &&
           ggNew < gg) {
         // energy change below FP64 rounding: accept on the gradient norm and
-        // book it as the smallest positive improvement
-        objNew = obj - 1e-300;
+        // book it as an improvement of zero (the decrease branch below)
+        guarded = true;
         break;
       }
       factor *= o.lSearch_alpha;
@@ -298,8 +299,9 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
     } while (stepLen > o.tolMinStep && numBis <= o.lsearch_numTry);
     R.line_search_bisections += numBis;
     // UpdateStepResult (OptimSolver.cpp:402-466)
-    const double impr = obj - objNew;
-    if (impr > 0) {
+    double impr = obj - objNew;
+    if (guarded && impr < 0) impr = 0;
+    if (impr > 0 || guarded) {
       const double optNew = std::sqrt(ggNew);
       if (impr < o.tolMinImpr) {
         minImpr++;
