@@ -191,6 +191,30 @@ void pgn_linear_default_opts(pgn_linear_opts* o);
 int pgn_linear_solve(pgn_model* model, const double* b, double* x,
                      const pgn_linear_opts* opts, pgn_linear_result* res);
 
+/* ---- linear solve on a caller's matrix (ILinearSolver::Solve) ----------- */
+/* ILinearSolver::Solve(MatrixSd& A, const VectorXd& b, VectorXd& x)
+ * (LinearSolver.h:38-52) -> LinearSolver_EigenCG::SolveInternal
+ * (LinearSolver_EigenCG.cpp:51-125) on a matrix the caller assembled, with
+ * no device model: FP64 Jacobi PCG (Eigen's ConjugateGradient with
+ * DiagonalPreconditioner: 1 / a_ii, 1 where a_ii is 0; relative residual
+ * |r| <= tol |b|; warm start x; Eigen's iteration count) and the
+ * regularization loop (b.x < 0 -> A += regT 10^(i-1) I, cumulative,
+ * regT = 1e-6 |A|_F over the stored entries as given, as GetMatrixData
+ * computes it, LinearSolver.cpp:57-66).  A is symmetric, given as row-major
+ * CSR over n rows (rowptr: n+1 entries, nnz = rowptr[n]) either as its lower
+ * part (PGN_LAYOUT_LOWER: AMatrixSd's entry set, whose whole 3x3 diagonal
+ * node blocks put some entries above the diagonal; as with Eigen's
+ * selfadjointView<Lower> only entries with col <= row enter the product) or
+ * in full (PGN_LAYOUT_FULL); every row stores its diagonal entry.  Like the
+ * reference (which adds the shifts to mA), the diagonal entries of val keep
+ * the shifts of the tries that ran.  opts: tol, max_iters, reg_iters;
+ * precond must be 0 (Jacobi) and solver PGN_LS_EIGENCG.  res->iterations
+ * counts the PCG iterations of every try.  Host arrays. */
+int pgn_csr_linear_solve(pgn_ctx* ctx, int n, int layout, const long* rowptr,
+                         const int* col, double* val, const double* b,
+                         double* x, const pgn_linear_opts* opts,
+                         pgn_linear_result* res);
+
 /* ---- Newton solve (OptimSolver::SolveFull) ------------------------------ */
 typedef struct {
   /* OptimSolverOptions (OptimSolver.h:23-71) */

@@ -51,7 +51,7 @@ EXPORTS = [
     "pgn_rods_reset", "pgn_rods_set_x", "pgn_rods_get_x", "pgn_rods_get_energy",
     "pgn_rods_get_gradient", "pgn_rods_get_hessian_dense", "pgn_rods_get_frames",
     "pgn_rods_set_frames", "pgn_rods_update_kinematics", "pgn_rods_element_bt",
-    "pgn_rods_newton",
+    "pgn_rods_newton", "pgn_csr_linear_solve",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -149,6 +149,9 @@ def lib():
         L.pgn_linear_default_opts.restype = None
         L.pgn_linear_solve.argtypes = [_vp, _dp, _dp, ctypes.POINTER(LinearOpts),
                                        ctypes.POINTER(LinearResult)]
+        L.pgn_csr_linear_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _lp, _ip, _dp, _dp,
+                                           _dp, ctypes.POINTER(LinearOpts),
+                                           ctypes.POINTER(LinearResult)]
         L.pgn_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
         L.pgn_newton_default_opts.restype = None
         L.pgn_newton_solve.argtypes = [_vp, ctypes.POINTER(NewtonOpts),
@@ -219,6 +222,41 @@ def lib():
         L.pgn_model_get_profile.argtypes = [_vp, ctypes.c_int, _lp, _dp]
         _LIB = L
     return _LIB
+
+
+def _linear_opts(kw):
+    o = LinearOpts()
+    lib().pgn_linear_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"pgn_linear_opts has no field {k}")
+        setattr(o, k, v)
+    return o
+
+
+def _linear_result(r):
+    return {"status": r.status, "iterations": r.iterations, "reg_used": r.reg_used,
+            "refinements": r.refinements, "rel_residual": r.rel_residual}
+
+
+def csr_linear_solve(ctx, rowptr, col, val, b, x0=None, layout=None, **opts):
+    """ILinearSolver::Solve(A, b, x) on a caller's CSR matrix (host arrays):
+    pgn_csr_linear_solve.  layout: LAYOUT_LOWER (default) or LAYOUT_FULL.
+    val is updated in place with the regularization shifts, as the reference
+    updates mA.  Returns (x, result dict)."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(col, dtype=np.int32)
+    if not (isinstance(val, np.ndarray) and val.dtype == np.float64 and val.flags.c_contiguous):
+        raise TypeError("val must be a C-contiguous float64 array (it is updated in place)")
+    n = rp.size - 1
+    b = _f64(b).reshape(-1)
+    x = np.zeros(n) if x0 is None else _f64(x0).reshape(-1).copy()
+    o = _linear_opts(opts)
+    r = LinearResult()
+    _ok(lib().pgn_csr_linear_solve(ctx.handle, n, LAYOUT_LOWER if layout is None else layout,
+                                   _p(rp, _lp), _p(ci, _ip), _p(val), _p(b), _p(x),
+                                   ctypes.byref(o), ctypes.byref(r)))
+    return x, _linear_result(r)
 
 
 class PgnError(RuntimeError):
