@@ -215,6 +215,25 @@ int pgn_csr_linear_solve(pgn_ctx* ctx, int n, int layout, const long* rowptr,
                          double* x, const pgn_linear_opts* opts,
                          pgn_linear_result* res);
 
+/* Most rows pgn_csr_linear_solve_noalloc accepts. */
+#define PGN_CSR_NOALLOC_MAX_ROWS 262144
+/* No counterpart in the reference: pgn_csr_linear_solve for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same computation and kernels,
+ * same results bit for bit, but every array is a DEVICE pointer on the
+ * context's device, A is given in full (PGN_LAYOUT_FULL only), n is at most
+ * PGN_CSR_NOALLOC_MAX_ROWS, and the work space (FP64 vectors, partial sums,
+ * state) is a static device array of 14.7 MB per device, held until the
+ * process ends, shared by every call on that device: calls that overlap on
+ * one GPU (from several threads or contexts) race on it and must be
+ * serialized by the caller.  Calls no hipMalloc / hipFree; reads a few state
+ * words back to the host between kernel chunks.  val's diagonal keeps the
+ * regularization shifts, as in pgn_csr_linear_solve. */
+int pgn_csr_linear_solve_noalloc(pgn_ctx* ctx, int n, const long* rowptr,
+                                 const int* col, double* val, const double* b,
+                                 double* x, const pgn_linear_opts* opts,
+                                 pgn_linear_result* res);
+
 /* ---- Newton solve (OptimSolver::SolveFull) ------------------------------ */
 typedef struct {
   /* OptimSolverOptions (OptimSolver.h:23-71) */

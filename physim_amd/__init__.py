@@ -51,7 +51,7 @@ EXPORTS = [
     "pgn_rods_reset", "pgn_rods_set_x", "pgn_rods_get_x", "pgn_rods_get_energy",
     "pgn_rods_get_gradient", "pgn_rods_get_hessian_dense", "pgn_rods_get_frames",
     "pgn_rods_set_frames", "pgn_rods_update_kinematics", "pgn_rods_element_bt",
-    "pgn_rods_newton", "pgn_csr_linear_solve",
+    "pgn_rods_newton", "pgn_csr_linear_solve", "pgn_csr_linear_solve_noalloc",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -152,6 +152,9 @@ def lib():
         L.pgn_csr_linear_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _lp, _ip, _dp, _dp,
                                            _dp, ctypes.POINTER(LinearOpts),
                                            ctypes.POINTER(LinearResult)]
+        L.pgn_csr_linear_solve_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                                   ctypes.POINTER(LinearOpts),
+                                                   ctypes.POINTER(LinearResult)]
         L.pgn_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
         L.pgn_newton_default_opts.restype = None
         L.pgn_newton_solve.argtypes = [_vp, ctypes.POINTER(NewtonOpts),
@@ -257,6 +260,16 @@ def csr_linear_solve(ctx, rowptr, col, val, b, x0=None, layout=None, **opts):
                                    _p(rp, _lp), _p(ci, _ip), _p(val), _p(b), _p(x),
                                    ctypes.byref(o), ctypes.byref(r)))
     return x, _linear_result(r)
+
+
+def csr_linear_solve_noalloc(ctx, n, rowptr_ptr, col_ptr, val_ptr, b_ptr, x_ptr, **opts):
+    """pgn_csr_linear_solve_noalloc: every array is a device address (int) on
+    the context's device, the matrix in full; x is updated on the device."""
+    o = _linear_opts(opts)
+    r = LinearResult()
+    _ok(lib().pgn_csr_linear_solve_noalloc(ctx.handle, int(n), rowptr_ptr, col_ptr, val_ptr,
+                                           b_ptr, x_ptr, ctypes.byref(o), ctypes.byref(r)))
+    return _linear_result(r)
 
 
 class PgnError(RuntimeError):

@@ -36,6 +36,7 @@ void use_device(const pgn_ctx* c) { PGN_HIP(hipSetDevice(c->device)); }
 }  // namespace
 
 namespace pgn {
+void csr_scratch_setup(int device);  // csr_solve.hip
 // error reporting shared by every extern "C" translation unit
 int api_guard_set(const char* msg, int code) {
   g_err = msg;
@@ -112,6 +113,7 @@ int pgn_ctx_create(int device, pgn_ctx** out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
       throw Error(PGN_ERR_NODEVICE, std::string("libpgn is built for gfx950, device is ") +
                                         prop.gcnArchName);
+    csr_scratch_setup(device);  // the allocation-free CSR solve's work space
     auto* c = new pgn_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -24,13 +24,16 @@
 // convergence test and p = z + beta p); the host queues them in chunks and
 // reads the state between chunks.
 //
-// pgn_csr_linear_solve takes host arrays and allocates its device copies and
-// work space.
+// Two entry points share the kernels: pgn_csr_linear_solve takes host arrays
+// and allocates its device copies and work space; pgn_csr_linear_solve_noalloc
+// takes device arrays and keeps its work space in a static device array
+// (`scratch`) of a fixed maximum size, allocating nothing.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pgn_internal.hpp"
@@ -46,6 +49,7 @@ namespace {
 
 constexpr int CSR_BS = 256;     // threads per workgroup
 constexpr int CSR_GMAX = 1024;  // workgroups per launch = partials per reduction
+constexpr int CSR_MAX_DEVICES = 64;
 
 struct CsrState {
   double abs_new[2];  // r.z by iteration parity (Eigen's absNew)
@@ -98,6 +102,19 @@ CsrWork csr_carve(char* base, long n) {
   w.st = (CsrState*)take(sizeof(CsrState));
   return w;
 }
+
+// The allocation-free entry point's work space: rows up to
+// PGN_CSR_NOALLOC_MAX_ROWS (14.7 MB on every device the library runs on,
+// held until the process ends).  Host code never passes `scratch` itself to
+// a kernel: its address on each device comes from hipGetSymbolAddress.
+constexpr long CSR_SCRATCH_BYTES =
+    6 * ((8L * PGN_CSR_NOALLOC_MAX_ROWS + 255) & ~255L) +
+    ((8L * PGN_CSR_NOALLOC_MAX_ROWS + 255) & ~255L) + 3 * ((8L * CSR_GMAX + 255) & ~255L) +
+    ((long(sizeof(CsrState)) + 255) & ~255L);
+static __device__ __align__(256) char scratch[CSR_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[CSR_MAX_DEVICES] = {};
 
 // workgroup -> contiguous row range, consecutive ranges on one XCD
 __device__ __forceinline__ int csr_chunk(int b, int G) {
@@ -524,6 +541,20 @@ void check_opts(const pgn_linear_opts& o) {
 
 }  // namespace
 
+namespace pgn {
+// Once per device, with the device current: the address of that device's
+// copy of `scratch` (called from pgn_ctx_create).
+void csr_scratch_setup(int device) {
+  PGN_CHECK(device >= 0 && device < CSR_MAX_DEVICES, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  PGN_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  PGN_CHECK(ptr != nullptr, "no device address for the CSR scratch array");
+  g_scratch_ptr[device] = ptr;
+}
+}  // namespace pgn
+
 extern "C" {
 
 int pgn_csr_linear_solve(pgn_ctx* ctx, int n, int layout, const long* rowptr, const int* col,
@@ -625,6 +656,36 @@ int pgn_csr_linear_solve(pgn_ctx* ctx, int n, int layout, const long* rowptr, co
     // the shifts stay in the caller's matrix, added in the same order
     for (double sh : shifts)
       for (int i = 0; i < n; ++i) val[dpos[i]] += sh;
+    *res = r;
+  });
+}
+
+int pgn_csr_linear_solve_noalloc(pgn_ctx* ctx, int n, const long* rowptr, const int* col,
+                                 double* val, const double* b, double* x,
+                                 const pgn_linear_opts* opts, pgn_linear_result* res) {
+  return cguard([&] {
+    PGN_CHECK(ctx && rowptr && col && val && b && x && res, "null argument");
+    PGN_CHECK(n >= 0, "n must be >= 0");
+    PGN_CHECK(n <= PGN_CSR_NOALLOC_MAX_ROWS,
+              "pgn_csr_linear_solve_noalloc: n exceeds PGN_CSR_NOALLOC_MAX_ROWS (262144 rows)");
+    pgn_linear_opts o;
+    pgn_linear_default_opts(&o);
+    if (opts) o = *opts;
+    check_opts(o);
+    PGN_HIP(hipSetDevice(ctx->device));
+    void* base = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(g_scratch_mu);
+      base = ctx->device < CSR_MAX_DEVICES ? g_scratch_ptr[ctx->device] : nullptr;
+    }
+    PGN_CHECK(base, "no scratch address for this device (context not created by pgn_ctx_create)");
+    hipStream_t s = ctx->stream;
+    long nnz = 0;
+    PGN_HIP(hipMemcpyAsync(&nnz, rowptr + n, sizeof(long), hipMemcpyDeviceToHost, s));
+    PGN_HIP(hipStreamSynchronize(s));
+    PGN_CHECK(nnz >= 0, "malformed CSR matrix: negative nnz");
+    pgn_linear_result r;
+    csr_solve_device(s, n, nnz, rowptr, col, val, b, x, o, -1.0, (char*)base, r, nullptr);
     *res = r;
   });
 }
