@@ -268,6 +268,7 @@ void pgn_batch::seg_plan(const std::vector<uint8_t>& act) {
 
 SegPcgArgs pgn_batch::seg_args(int precond) const {
   SegPcgArgs a;
+  std::memset(&a, 0, sizeof(a));  // padding too: chunk graphs compare argument bytes
   a.K = K();
   a.W = seg_W;
   a.wg_seg = wg_seg.p;
@@ -312,15 +313,36 @@ void pgn_batch::seg_pcg(const std::vector<double>& tol, int maxit, int precond, 
     return;
   }
   launch_seg_init(s, f32, a, M->pb.p, x_zero ? 1 : 0);
+  // chunks after the first replay a captured graph when they have the full
+  // length (launch iterations relative to the chunk, read from kctl)
+  SegPcgArgs ga = a;
+  ga.kctl = M->kctl.p;
   int k = 0, chunk = 8;
   for (;;) {
     const int c = std::min(chunk, maxit - k);
-    for (int j = 0; j < c; ++j) {
-      launch_seg_k1(s, f32, a, k + j);
-      launch_seg_k2(s, f32, a, k + j);
+    hipGraphExec_t ex = nullptr;
+    if (M->use_graphs >= 2 && k > 0 && c == chunk)
+      ex = seg_graphs.get(s, c, k & 1, ga, [&](hipStream_t cs) {
+        for (int j = 0; j < c; ++j) {
+          launch_seg_k1(cs, f32, ga, j);
+          launch_seg_k2(cs, f32, ga, j);
+        }
+        launch_seg_status(cs, ga, c, 0);
+      });
+    if (ex) {
+      M->hkctl[0] = k;
+      M->hkctl[1] = maxit;
+      PGN_HIP(hipMemcpyAsync(M->kctl.p, M->hkctl, 2 * sizeof(int), hipMemcpyHostToDevice, s));
+      PGN_HIP(hipGraphLaunch(ex, s));
+      k += c;
+    } else {
+      for (int j = 0; j < c; ++j) {
+        launch_seg_k1(s, f32, a, k + j);
+        launch_seg_k2(s, f32, a, k + j);
+      }
+      k += c;
+      launch_seg_status(s, a, k, maxit);
     }
-    k += c;
-    launch_seg_status(s, a, k, maxit);
     seg_sc.download(hseg_sc.data(), Kn, s);
     PGN_HIP(hipStreamSynchronize(s));
     bool all = true;

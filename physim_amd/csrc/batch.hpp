@@ -47,6 +47,7 @@ struct pgn_batch {
   pgn::DBuf<long> seg_row0, seg_row1;
   pgn::DBuf<double> seg_tol, seg_part1, seg_part2, seg_partb, kvec, kvec2, xprev;
   pgn::DBuf<pgn::PcgScalars> seg_sc;
+  pgn::ChunkGraphs<pgn::SegPcgArgs> seg_graphs;  // replayable segmented PCG chunks
   pgn::DBuf<short> bcol16;  // 16-bit column offsets for the segmented SpMV (empty: 32-bit)
   bool bcol16_done = false;
   std::vector<pgn::PcgScalars> hseg_sc;

@@ -176,6 +176,11 @@ struct PcgArgs {
   const int* sell_col = nullptr;   // per slot: block column (null when sell_c16 is set)
   const short* sell_c16 = nullptr; // per slot: block column minus the slot's row
   const void* sell_val = nullptr;  // 9 per slot, Hessian precision
+  // Launches inside a captured chunk graph take their iteration relative to
+  // the chunk: k = kctl[0] + argument, and the status launch's iteration
+  // cap is kctl[1] (both written by the host before each replay).  Null:
+  // the arguments are absolute.
+  const int* kctl = nullptr;
 };
 int pcg_grid1(int n, int lpr);
 // SELL-64 layout: slices per workgroup and grid (<= PCG_MAXP) for ns slices
@@ -234,6 +239,7 @@ struct SegPcgArgs {
   double* part2;         // 2 parities x 2 x W
   double* partb;         // W
   PcgScalars* sc;        // K
+  const int* kctl = nullptr;  // as PcgArgs::kctl
 };
 constexpr int SEG_MAXG = 256;  // workgroups per segment at most
 void launch_seg_init(hipStream_t s, bool fp32, const SegPcgArgs& a, const void* b, int x_zero);

@@ -95,6 +95,8 @@ void build_bsr_pattern(int n, int m, const int* T, BsrPattern& P) {
 using namespace pgn;
 
 pgn_model::~pgn_model() {
+  pcg_graphs.clear();
+  if (hkctl) (void)hipHostFree(hkctl);
   for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
   if (hpin) (void)hipHostFree(hpin);
   if (hsc) (void)hipHostFree(hsc);
@@ -309,6 +311,8 @@ void pgn_model::setup(int n_nodes, const double* X0h, int n_elems, const int* Th
   PGN_HIP(hipMemsetAsync(pv.p, 0, pv.bytes(), s));
   PGN_HIP(hipHostMalloc((void**)&hpin, 16 * sizeof(double), hipHostMallocDefault));
   PGN_HIP(hipHostMalloc((void**)&hsc, sizeof(PcgScalars), hipHostMallocDefault));
+  PGN_HIP(hipHostMalloc((void**)&hkctl, 2 * sizeof(int), hipHostMallocDefault));
+  kctl.alloc(2);
   sync();
 }
 
@@ -487,6 +491,7 @@ void pgn_model::sell_pack() {
 
 PcgArgs pgn_model::pcg_args(double tol, int precond) const {
   PcgArgs a;
+  std::memset(&a, 0, sizeof(a));  // padding too: chunk graphs compare argument bytes
   a.lpr = pcg_lpr;
   a.split = pcg_split ? 1 : 0;
   a.cg1 = pcg_cg1 ? 1 : 0;

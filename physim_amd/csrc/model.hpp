@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 
+#include "graph.hpp"
 #include "kernels.hpp"
 #include "pgn_internal.hpp"
 
@@ -124,6 +125,12 @@ struct pgn_model {
   pgn::PcgScalars* hsc = nullptr;
   bool hess_valid = false;
   int pcg_chunk_max = 64;
+  // PCG chunks after the first replay captured graphs (PGN_GRAPH=0: direct
+  // launches); kctl = {first iteration of the chunk, iteration cap}
+  int use_graphs = pgn::graphs_level();
+  pgn::DBuf<int> kctl;
+  int* hkctl = nullptr;  // pinned host copy
+  pgn::ChunkGraphs<pgn::PcgArgs> pcg_graphs;
   // live per-kernel timing of the PCG launches (HIP events on the stream)
   bool profiling = false;
   std::vector<hipEvent_t> prof_ev;

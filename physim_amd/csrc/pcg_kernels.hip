@@ -224,6 +224,12 @@ __device__ __forceinline__ void all_sum3(const double* __restrict__ a, int ca, i
   __syncthreads();
 }
 
+// the absolute iteration of a launch (relative to its chunk in a graph)
+template <class Args>
+__device__ __forceinline__ int chunk_k(const Args& A, int k) {
+  return A.kctl ? A.kctl[0] + k : k;
+}
+
 __device__ __forceinline__ const double* rr_slot(const PcgArgs& A, int k) {
   return A.part2 + (long)(k & 1) * 2 * PCG_MAXP;  // iteration k's {r.r, r.z}
 }
@@ -341,6 +347,7 @@ __device__ __forceinline__ bool pcg_beta(const PcgArgs& A, int k, double* red, d
 
 template <class T, int L>
 __global__ __launch_bounds__(K1_BS) void pcg_k1_kernel(PcgArgs A, int k) {
+  k = chunk_k(A, k);
   __shared__ double red[3 * K1_BS / 64 + 4];
   PcgScalars* sc = A.sc;
   if (sc->done) return;
@@ -395,6 +402,7 @@ __global__ __launch_bounds__(K1_BS) void pcg_k1_kernel(PcgArgs A, int k) {
 // the extra launch.
 template <class T>
 __global__ __launch_bounds__(K2_BS) void pcg_k1a_kernel(PcgArgs A, int k) {
+  k = chunk_k(A, k);
   __shared__ double red[3 * K2_BS / 64 + 4];
   if (A.sc->done) return;
   const T* z = reinterpret_cast<const T*>(A.z);
@@ -441,6 +449,7 @@ __global__ __launch_bounds__(K2_BS) void pcg_k1a_kernel(PcgArgs A, int k) {
 
 template <class T, int L>
 __global__ __launch_bounds__(K1_BS) void pcg_k1b_kernel(PcgArgs A, int k) {
+  k = chunk_k(A, k);
   __shared__ double red[K1_BS / 64 + 2];
   if (A.sc->done) return;
   const T* __restrict__ bval = reinterpret_cast<const T*>(A.bval);
@@ -489,6 +498,7 @@ __global__ __launch_bounds__(K1_BS) void pcg_k1b_kernel(PcgArgs A, int k) {
 // (double-buffered by parity; block 0 writes them).
 template <class T, int L>
 __global__ __launch_bounds__(K1_BS) void cg1_a_kernel(PcgArgs A, int k) {
+  k = chunk_k(A, k);
   __shared__ double red[3 * K1_BS / 64 + 4];
   PcgScalars* sc = A.sc;
   if (sc->done) return;
@@ -560,6 +570,7 @@ __global__ __launch_bounds__(K1_BS) void cg1_a_kernel(PcgArgs A, int k) {
 // C16: block columns stored as 16-bit offsets from the slot's row.
 template <class T, bool C16>
 __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
+  k = chunk_k(A, k);
   __shared__ double red[3 * SELL_BS / 64 + 4];
   constexpr int WAVES = SELL_BS / 64;
   PcgScalars* sc = A.sc;
@@ -717,6 +728,7 @@ __device__ __forceinline__ void cg1_load(const PcgArgs& A, long i0, long hi, Cg1
 // r.z in cgs[4], cgs[5]; w is not read and q not written here.
 template <class T, bool QF>
 __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
+  k = chunk_k(A, k);
   __shared__ double red[3 * K2_BS / 64 + 4];
   if (A.sc->done) return;
   const RowRange rg = block_rows(A.n);
@@ -812,6 +824,7 @@ __device__ __forceinline__ void k2_load(const PcgArgs& A, const T* pk, long i0, 
 
 template <class T>
 __global__ __launch_bounds__(K2_BS) void pcg_k2_kernel(PcgArgs A, int k) {
+  k = chunk_k(A, k);
   __shared__ double red[3 * K2_BS / 64 + 4];
   PcgScalars* sc = A.sc;
   if (sc->done) return;
@@ -865,6 +878,10 @@ __global__ __launch_bounds__(K2_BS) void pcg_k2_kernel(PcgArgs A, int k) {
 
 // ---- status after iterations [0, k_next) ---------------------------------------
 __global__ __launch_bounds__(256) void pcg_status_kernel(PcgArgs A, int k_next, int maxit) {
+  if (A.kctl) {
+    k_next += A.kctl[0];
+    maxit = A.kctl[1];
+  }
   __shared__ double red[2 * 256 / 64 + 2];
   PcgScalars* sc = A.sc;
   if (sc->done) return;
@@ -1294,6 +1311,7 @@ __global__ __launch_bounds__(K1_BS) void seg_init_kernel(SegPcgArgs A, const T* 
 
 template <class T, int L, bool C16>
 __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
+  k = chunk_k(A, k);
   __shared__ double red[2 * K1_BS / 64 + 2];
   const int w = blockIdx.x, sg = A.wg_seg[w], j = A.wg_loc[w], G = A.seg_G[sg];
   PcgScalars* sc = A.sc + sg;
@@ -1379,6 +1397,7 @@ __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
 
 template <class T>
 __global__ __launch_bounds__(SEG_K2_BS) void seg_k2_kernel(SegPcgArgs A, int k) {
+  k = chunk_k(A, k);
   __shared__ double red[2 * SEG_K2_BS / 64 + 2];
   const int w = blockIdx.x, sg = A.wg_seg[w], j = A.wg_loc[w], G = A.seg_G[sg];
   if (A.sc[sg].done) return;
@@ -1441,6 +1460,10 @@ __global__ __launch_bounds__(SEG_K2_BS) void seg_k2_kernel(SegPcgArgs A, int k) 
 
 // one workgroup per segment
 __global__ __launch_bounds__(256) void seg_status_kernel(SegPcgArgs A, int k_next, int maxit) {
+  if (A.kctl) {
+    k_next += A.kctl[0];
+    maxit = A.kctl[1];
+  }
   __shared__ double red[2 * 256 / 64 + 2];
   const int sg = blockIdx.x;
   PcgScalars* sc = A.sc + sg;

@@ -27,22 +27,46 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
       M.prof_ev.push_back(e);
     }
   }
+  // chunks after the first (whose iteration 0 starts the recurrences) replay
+  // a captured graph of their launches when they have the full length
+  PcgArgs ga = a;
+  ga.tol = 0.0;  // read at iteration 0 only
+  ga.kctl = M.kctl.p;
+  const bool graphs = M.use_graphs >= 1 && !M.profiling;
   int k = 0, chunk = 8;
   for (;;) {
     const int c = std::min(chunk, maxit - k);
     const int js = c / 2;  // sampled iteration of the chunk (mid-stream)
-    for (int j = 0; j < c; ++j) {
-      // profiling brackets one K1 and K2 of every chunk with events: a
-      // sample of the launches, so the timed solve itself stays unperturbed
-      const bool ev = M.profiling && j == js;
-      launch_pcg_k1a(s, M.hfp32, a, k + j);
-      launch_pcg_k1(s, M.hfp32, a, k + j, ev ? M.prof_ev[0] : nullptr,
-                    ev ? M.prof_ev[1] : nullptr);
-      launch_pcg_k2(s, M.hfp32, a, k + j, ev ? M.prof_ev[2] : nullptr,
-                    ev ? M.prof_ev[3] : nullptr);
+    hipGraphExec_t ex = nullptr;
+    if (graphs && k > 0 && c == chunk)
+      ex = M.pcg_graphs.get(s, c, k & 1, ga, [&](hipStream_t cs) {
+        for (int j = 0; j < c; ++j) {
+          launch_pcg_k1a(cs, M.hfp32, ga, j);
+          launch_pcg_k1(cs, M.hfp32, ga, j, nullptr, nullptr);
+          launch_pcg_k2(cs, M.hfp32, ga, j, nullptr, nullptr);
+        }
+        launch_pcg_status(cs, ga, c, 0);
+      });
+    if (ex) {
+      M.hkctl[0] = k;
+      M.hkctl[1] = maxit;
+      PGN_HIP(hipMemcpyAsync(M.kctl.p, M.hkctl, 2 * sizeof(int), hipMemcpyHostToDevice, s));
+      PGN_HIP(hipGraphLaunch(ex, s));
+      k += c;
+    } else {
+      for (int j = 0; j < c; ++j) {
+        // profiling brackets one K1 and K2 of every chunk with events: a
+        // sample of the launches, so the timed solve itself stays unperturbed
+        const bool ev = M.profiling && j == js;
+        launch_pcg_k1a(s, M.hfp32, a, k + j);
+        launch_pcg_k1(s, M.hfp32, a, k + j, ev ? M.prof_ev[0] : nullptr,
+                      ev ? M.prof_ev[1] : nullptr);
+        launch_pcg_k2(s, M.hfp32, a, k + j, ev ? M.prof_ev[2] : nullptr,
+                      ev ? M.prof_ev[3] : nullptr);
+      }
+      k += c;
+      launch_pcg_status(s, a, k, maxit);
     }
-    k += c;
-    launch_pcg_status(s, a, k, maxit);
     PGN_HIP(hipMemcpyAsync(M.hsc, M.sc.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     M.sync();
     // launches after convergence return at once: the sample counts only if
