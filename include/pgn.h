@@ -432,7 +432,13 @@ int pgn_model_set_profiling(pgn_model* model, int on);
  * row, timed once per model, and in count the lanes per row in use (16 from
  * 12 mean blocks per row, 8 from 6, else 4; PGN_PCG_LPR overrides);
  * 5: in count the bytes per block column of the SELL-64 copy (2: 16-bit
- * offsets from the row, 4: 32-bit columns, 0: no SELL-64 copy) */
+ * offsets from the row, 4: 32-bit columns, 0: no SELL-64 copy);
+ * 6, 7, 8: in count the tile of the tet element kernel the in-library
+ * autotuner chose (timed on the model's own elements at the first launch;
+ * -1 before it; 0..5 = workgroup size x elements per lane: 256x1, 128x1,
+ * 64x1, 256x2, 128x2, 64x2) for the Hessian (model precision), FP64 E+g
+ * and E+g+H (model precision), in total_ms its time per launch in us;
+ * 9 + t: the Hessian launch's time (us) with tile t (0 if not timed) */
 int pgn_model_get_profile(pgn_model* model, int which, long* count,
                           double* total_ms);
 

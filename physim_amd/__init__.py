@@ -485,7 +485,9 @@ class TetModel:
     def profile(self, which):
         """(launches, total ms) of PCG kernel `which` (0 = SpMV, 1 = update);
         2-4: (lanes per row in use, SpMV us at 4 / 8 / 16 lanes); 5: (bytes per
-        SELL-64 block column, 0 without the SELL-64 copy; 0)."""
+        SELL-64 block column, 0 without the SELL-64 copy; 0); 6-8: (tuned tet
+        element tile, its us) of the Hessian, FP64 E+g and E+g+H launches;
+        9 + t: (t, the Hessian launch's us with tile t)."""
         c = ctypes.c_long()
         ms = ctypes.c_double()
         _ok(lib().pgn_model_get_profile(self._h, which, ctypes.byref(c), ctypes.byref(ms)))

@@ -566,8 +566,21 @@ int pgn_model_set_profiling(pgn_model* M, int on) {
 int pgn_model_get_profile(pgn_model* M, int which, long* count, double* total_ms) {
   return guard([&] {
     PGN_CHECK(M && count && total_ms, "null argument");
-    PGN_CHECK(which >= 0 && which <= 5, "unknown profile slot");
-    if (which <= 1) {
+    PGN_CHECK(which >= 0 && which <= 8 + TET_TILES, "unknown profile slot");
+    if (which >= 6) {  // element kernel tiles: 6 = Hessian (model precision),
+                       // 7 = FP64 E+g, 8 = E+g+H (model precision); 9.. = the
+                       // timings (us) of the Hessian's tiles
+      const int fp = M->hfp32 ? 1 : 0;
+      if (which >= 9) {
+        *count = which - 9;
+        *total_ms = M->tet_tile_us[fp][EVAL_H][which - 9];
+        return;
+      }
+      const int slot[3][2] = {{fp, EVAL_H}, {0, EVAL_E | EVAL_G}, {fp, EVAL_E | EVAL_G | EVAL_H}};
+      const int t = M->tet_tile_choice[slot[which - 6][0]][slot[which - 6][1]];
+      *count = t;  // -1: not launched yet
+      *total_ms = t < 0 ? 0.0 : M->tet_tile_us[slot[which - 6][0]][slot[which - 6][1]][t];
+    } else if (which <= 1) {
       *count = M->prof_n[which];
       *total_ms = M->prof_ms[which];
     } else if (which == 5) {  // bytes per SELL-64 block column, 0 without SELL
@@ -669,15 +682,13 @@ int pgn_time_kernel(pgn_model* M, int which, int reps, double* ms_mean) {
         void* eg = f ? ensure_eg32() : (void*)M->eg64.p;
         void* eH = M->eH.p;
         launch = [=] {
-          launch_tet_eval(s, f, EVAL_E | EVAL_G | EVAL_H, M->tet_args(f), M->blockE.p, nullptr,
-                          eg, eH);
+          M->launch_tet(f, EVAL_E | EVAL_G | EVAL_H, M->blockE.p, nullptr, eg, eH);
         };
         break;
       }
       case 1:
         launch = [=] {
-          launch_tet_eval(s, false, EVAL_E | EVAL_G, M->tet_args(false), M->blockE.p, nullptr,
-                          M->eg64.p, nullptr);
+          M->launch_tet(false, EVAL_E | EVAL_G, M->blockE.p, nullptr, M->eg64.p, nullptr);
         };
         break;
       case 2:
@@ -697,8 +708,7 @@ int pgn_time_kernel(pgn_model* M, int which, int reps, double* ms_mean) {
       case 4: {
         void* eH = ensure_eH(false);
         launch = [=] {
-          launch_tet_eval(s, false, EVAL_E | EVAL_G | EVAL_H, M->tet_args(false), M->blockE.p,
-                          nullptr, M->eg64.p, eH);
+          M->launch_tet(false, EVAL_E | EVAL_G | EVAL_H, M->blockE.p, nullptr, M->eg64.p, eH);
         };
         break;
       }
@@ -706,8 +716,7 @@ int pgn_time_kernel(pgn_model* M, int which, int reps, double* ms_mean) {
         void* eH = ensure_eH(true);
         void* eg = ensure_eg32();
         launch = [=] {
-          launch_tet_eval(s, true, EVAL_E | EVAL_G | EVAL_H, M->tet_args(true), M->blockE.p,
-                          nullptr, eg, eH);
+          M->launch_tet(true, EVAL_E | EVAL_G | EVAL_H, M->blockE.p, nullptr, eg, eH);
         };
         break;
       }

@@ -27,11 +27,16 @@ struct TetArgs {
   const double* X0;      // 3n rest positions
 };
 
-// Element evaluation.  blockE: one FP64 partial energy per workgroup
-// (ceil(m / TET_BLOCK) entries) or null; elemE: per-element FP64 energy or
+// Element evaluation.  blockE: one FP64 partial energy per 64 consecutive
+// elements (ceil(m / 64) entries) or null; elemE: per-element FP64 energy or
 // null; eg: 12 values per element (precision T); eH: TET_H_PACK per element.
+// tile: workgroup size x elements per lane, TET_TILES of them (0 = 256 x 1);
+// every tile gives the same results bit for bit.
+constexpr int TET_TILES = 6;
+constexpr int TET_TILE_BS[TET_TILES] = {256, 128, 64, 256, 128, 64};
+constexpr int TET_TILE_EPL[TET_TILES] = {1, 1, 1, 2, 2, 2};
 void launch_tet_eval(hipStream_t s, bool fp32, int flags, const TetArgs& a,
-                     double* blockE, double* elemE, void* eg, void* eH);
+                     double* blockE, double* elemE, void* eg, void* eH, int tile = 0);
 
 // Thin-shell elements (membrane triangles, hinges) evaluated into the same
 // per-element outputs as tets.  Elements [first, first + count) of the model;

@@ -378,14 +378,54 @@ ShellArgs pgn_model::shell_args(bool fp32, int k) const {
 }
 
 static int parts(int count) { return (count + TET_BLOCK - 1) / TET_BLOCK; }
+static int tet_parts(int count) { return (count + 63) / 64; }  // one per wavefront
 
-int pgn_model::elem_partials() const { return parts(m_tet) + parts(m_mem) + parts(m_hinge); }
+int pgn_model::elem_partials() const {
+  return tet_parts(m_tet) + parts(m_mem) + parts(m_hinge);
+}
+
+void pgn_model::launch_tet(bool fp32, int flags, double* bE, double* elemE, void* eg,
+                           void* eHp) {
+  hipStream_t s = stream();
+  const TetArgs ta = tet_args(fp32);
+  const bool tunable = flags == (EVAL_E | EVAL_G) || flags == EVAL_H ||
+                       flags == (EVAL_E | EVAL_G | EVAL_H);
+  int& tile = tet_tile_choice[fp32 ? 1 : 0][flags & 7];
+  if (tile < 0) {
+    tile = 0;
+    if (const char* e = std::getenv("PGN_TET_TILE")) {
+      tile = std::max(0, std::min(TET_TILES - 1, std::atoi(e)));
+    } else if (tunable && m_tet >= TUNE_MIN_TETS) {
+      hipEvent_t e0, e1;
+      PGN_HIP(hipEventCreate(&e0));
+      PGN_HIP(hipEventCreate(&e1));
+      float best = 0.0f;
+      for (int t = 0; t < TET_TILES; ++t) {
+        launch_tet_eval(s, fp32, flags, ta, bE, elemE, eg, eHp, t);  // warm-up
+        PGN_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) launch_tet_eval(s, fp32, flags, ta, bE, elemE, eg, eHp, t);
+        PGN_HIP(hipEventRecord(e1, s));
+        PGN_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        PGN_HIP(hipEventElapsedTime(&ms, e0, e1));
+        tet_tile_us[fp32 ? 1 : 0][flags & 7][t] = 1e3f * ms / 3.0f;
+        if (t == 0 || ms < best) {
+          best = ms;
+          tile = t;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+  }
+  launch_tet_eval(s, fp32, flags, ta, bE, elemE, eg, eHp, tile);
+}
 
 void pgn_model::launch_elements(bool fp32, int flags, double* bE, double* elemE, void* eg,
                                 void* eHp) {
   hipStream_t s = stream();
-  launch_tet_eval(s, fp32, flags, tet_args(fp32), bE, elemE, eg, eHp);
-  double* b2 = bE ? bE + parts(m_tet) : nullptr;
+  launch_tet(fp32, flags, bE, elemE, eg, eHp);
+  double* b2 = bE ? bE + tet_parts(m_tet) : nullptr;
   launch_shell_eval(s, fp32, SHELL_MEMBRANE, flags, shell_args(fp32, SHELL_MEMBRANE), b2, elemE,
                     eg, eHp);
   double* b3 = bE ? b2 + parts(m_mem) : nullptr;

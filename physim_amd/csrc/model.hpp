@@ -160,6 +160,15 @@ struct pgn_model {
   // given flags; blockE receives elem_partials() FP64 energy partials
   void launch_elements(bool fp32, int flags, double* blockE, double* elemE, void* eg,
                        void* eH);
+  // the tet elements alone, in the tile tuned for (precision, flags): the
+  // first launch of a flag set of the Newton path on a model of at least
+  // TUNE_MIN_TETS tets times every tile (the real outputs, written again by
+  // each) and keeps the fastest; PGN_TET_TILE in the environment fixes it
+  void launch_tet(bool fp32, int flags, double* blockE, double* elemE, void* eg, void* eH);
+  int tet_tile_choice[2][8] = {{-1, -1, -1, -1, -1, -1, -1, -1},
+                               {-1, -1, -1, -1, -1, -1, -1, -1}};
+  float tet_tile_us[2][8][pgn::TET_TILES] = {};  // the timings of the choice
+  static constexpr int TUNE_MIN_TETS = 16384;
   int elem_partials() const;
   pgn::NodeArgs node_args() const;
   pgn::BsrArgs bsr_args() const;

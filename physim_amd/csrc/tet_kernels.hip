@@ -1,6 +1,7 @@
 // tet_kernels.hip -- Tet4 element energy / gradient / Hessian for gfx950.
 //
-// One element per lane, 256-lane workgroups.  The per-element arithmetic is
+// One element per lane, workgroups of 64-256 lanes (tiles chosen per model
+// by timing).  The per-element arithmetic is
 // the reference's (EnergyElement_FEM.cpp:125-222: E = V0 Psi(F),
 // g = V0 DFDx^T P, H = V0 DFDx^T dP/dF DFDx with 1-point Tet1 quadrature),
 // re-derived so that no 9x12 DFDx or 9x9 dP/dF is ever formed:
@@ -60,35 +61,23 @@ __device__ __forceinline__ double t_log1p<double>(double y) {
 
 // Copies nv values from LDS to HBM as 16-byte vectors (src and dst 16-byte
 // aligned), the remainder value by value.
-template <class T>
+template <class T, int BS>
 __device__ __forceinline__ void copy_out(const T* src, T* dst, int nv) {
   using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
   constexpr int W = 16 / sizeof(T);
   const int nw = nv / W;
   const V* s = reinterpret_cast<const V*>(src);
   V* d = reinterpret_cast<V*>(dst);
-  for (int q = threadIdx.x; q < nw; q += TET_BLOCK) d[q] = s[q];
-  for (int q = W * nw + threadIdx.x; q < nv; q += TET_BLOCK) dst[q] = src[q];
+  for (int q = threadIdx.x; q < nw; q += BS) d[q] = s[q];
+  for (int q = W * nw + threadIdx.x; q < nv; q += BS) dst[q] = src[q];
 }
 
-// STAGE: the outputs of the workgroup's elements (packed Hessians, 78 values,
-// then gradients, 12 values) are written to LDS one element per lane and
-// copied out as one contiguous stretch each with 16-byte stores, instead of
-// every lane storing rows that lie 312 B (48 B) apart in HBM.  The output
-// offsets of a workgroup, TET_BLOCK x 78 (x 12) values, are 16-byte aligned.
-template <class T, int FLAGS, bool STAGE = false>
-__global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
-    TetArgs A, double* __restrict__ blockE, double* __restrict__ elemE,
-    T* __restrict__ eg, T* __restrict__ eH) {
-  __shared__ double red[TET_BLOCK / 64];
-  constexpr int SH = (FLAGS & EVAL_H) ? TET_BLOCK * TET_H_PACK : 0;
-  constexpr int SG = (FLAGS & EVAL_G) ? TET_BLOCK * 12 : 0;
-  constexpr int HS = STAGE ? (SH > SG ? SH : SG) : 1;
-  static_assert(HS * sizeof(T) <= 96 * 1024, "staging buffer too large");
-  __shared__ __attribute__((aligned(16))) T hs[HS];
-  const int e = blockIdx.x * TET_BLOCK + threadIdx.x;
-  double Ee = 0.0;
-  T gv[12];
+// One element: E (into Ee), g (into gv, and stored unless staged) and the
+// packed Hessian (into hrow when staged, else straight to eH).
+template <class T, int FLAGS, bool STAGE>
+__device__ __forceinline__ void tet_element(const TetArgs& A, long e, T* __restrict__ hrow,
+                                            double* __restrict__ elemE, T* __restrict__ eg,
+                                            T* __restrict__ eH, double& Ee, T (&gv)[12]) {
   if (e < A.m) {
     const long m = A.stride;
     const T* __restrict__ prm = reinterpret_cast<const T*>(A.prm);
@@ -199,7 +188,7 @@ __global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
 #pragma unroll
         for (int i = 0; i < 3; ++i) c[0][i] = -(c[1][i] + c[2][i] + c[3][i]);
         const T A0 = V0 * l2, A1 = V0 * (l2 - l1 * lnJ), A2 = V0 * l1;
-        T* h = STAGE ? hs + threadIdx.x * TET_H_PACK : eH + (long)e * TET_H_PACK;
+        T* h = STAGE ? hrow : eH + e * TET_H_PACK;
 #pragma unroll
         for (int a = 1; a < 4; ++a)
 #pragma unroll
@@ -280,7 +269,7 @@ __global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
           for (int k = 0; k < 3; ++k)
             G[i][k] = F[i][0] * F[k][0] + F[i][1] * F[k][1] + F[i][2] * F[k][2];
         const T A0 = V0 * l2, A2 = V0 * l1;
-        T* h = STAGE ? hs + threadIdx.x * TET_H_PACK : eH + (long)e * TET_H_PACK;
+        T* h = STAGE ? hrow : eH + e * TET_H_PACK;
 #pragma unroll
         for (int a = 1; a < 4; ++a)
 #pragma unroll
@@ -324,26 +313,57 @@ __global__ __launch_bounds__(TET_BLOCK) void tet_eval_kernel(
     }
     if ((FLAGS & EVAL_E) && elemE) elemE[e] = Ee;
   }
-  if constexpr (STAGE) {
-    const long e0 = (long)blockIdx.x * TET_BLOCK;
-    const int ne = (int)((A.m - e0 < TET_BLOCK) ? A.m - e0 : TET_BLOCK);
-    if constexpr ((FLAGS & EVAL_H) != 0) {
-      __syncthreads();
-      copy_out<T>(hs, eH + e0 * TET_H_PACK, ne * TET_H_PACK);
-    }
-    if constexpr ((FLAGS & EVAL_G) != 0) {
-      __syncthreads();  // the Hessian copy has read hs
-      if (e < A.m) {
-#pragma unroll
-        for (int k = 0; k < 12; ++k) hs[threadIdx.x * 12 + k] = gv[k];
+}
+
+// BS elements per workgroup pass, EPL passes per workgroup (one element per
+// lane per pass: 64 x EPL elements per wavefront).  STAGE: the outputs of a
+// pass (packed Hessians, 78 values, then gradients, 12 values) are written to
+// LDS one element per lane and copied out as one contiguous stretch each
+// with 16-byte stores, instead of every lane storing rows that lie 312 B
+// (48 B) apart in HBM; a pass's output offsets, BS x 78 (x 12) values, are
+// 16-byte aligned.  blockE receives one FP64 energy partial per 64
+// consecutive elements (ceil(m / 64) entries), whatever the tile, so every
+// tile gives the same bits.
+template <class T, int FLAGS, bool STAGE, int BS, int EPL>
+__global__ __launch_bounds__(BS) void tet_eval_kernel(TetArgs A, double* __restrict__ blockE,
+                                                      double* __restrict__ elemE,
+                                                      T* __restrict__ eg, T* __restrict__ eH) {
+  constexpr int SH = (FLAGS & EVAL_H) ? BS * TET_H_PACK : 0;
+  constexpr int SG = (FLAGS & EVAL_G) ? BS * 12 : 0;
+  constexpr int HS = STAGE ? (SH > SG ? SH : SG) : 1;
+  static_assert(HS * sizeof(T) <= 96 * 1024, "staging buffer too large");
+  __shared__ __attribute__((aligned(16))) T hs[HS];
+#pragma unroll 1
+  for (int q = 0; q < EPL; ++q) {
+    const long e0 = ((long)blockIdx.x * EPL + q) * BS;
+    if (e0 >= A.m) break;  // the same for the whole workgroup
+    const long e = e0 + threadIdx.x;
+    double Ee = 0.0;
+    T gv[12];
+    tet_element<T, FLAGS, STAGE>(A, e, hs + (STAGE ? threadIdx.x * TET_H_PACK : 0), elemE, eg,
+                                 eH, Ee, gv);
+    if constexpr (STAGE) {
+      const int ne = (int)((A.m - e0 < BS) ? A.m - e0 : BS);
+      if constexpr ((FLAGS & EVAL_H) != 0) {
+        __syncthreads();
+        copy_out<T, BS>(hs, eH + e0 * TET_H_PACK, ne * TET_H_PACK);
       }
-      __syncthreads();
-      copy_out<T>(hs, eg + e0 * 12, ne * 12);
+      if constexpr ((FLAGS & EVAL_G) != 0) {
+        __syncthreads();  // the Hessian copy has read hs
+        if (e < A.m) {
+#pragma unroll
+          for (int k = 0; k < 12; ++k) hs[threadIdx.x * 12 + k] = gv[k];
+        }
+        __syncthreads();
+        copy_out<T, BS>(hs, eg + e0 * 12, ne * 12);
+      }
+      __syncthreads();  // the next pass writes hs again
     }
-  }
-  if ((FLAGS & EVAL_E) && blockE) {
-    const double s = block_sum<TET_BLOCK>(Ee, red);
-    if (threadIdx.x == 0) blockE[blockIdx.x] = s;
+    if ((FLAGS & EVAL_E) && blockE) {
+      const double w = wave_sum(Ee);
+      const long we = e0 + (threadIdx.x & ~63);
+      if ((threadIdx.x & 63) == 0 && we < A.m) blockE[we >> 6] = w;
+    }
   }
 }
 
@@ -356,48 +376,72 @@ static bool stage_enabled() {
   return on;
 }
 
-template <class T, int F>
-static void launch_t(hipStream_t s, const TetArgs& a, double* blockE,
-                     double* elemE, void* eg, void* eH) {
-  const int grid = (a.m + TET_BLOCK - 1) / TET_BLOCK;
+template <class T, int F, bool STAGE, int BS, int EPL>
+static void launch_k(hipStream_t s, const TetArgs& a, double* blockE, double* elemE, void* eg,
+                     void* eH) {
+  const long per = (long)BS * EPL;
+  const int grid = (int)((a.m + per - 1) / per);
   if (grid == 0) return;
-  // staging needs TET_BLOCK x 78 values of LDS for Hessians: FP32 only
-  if constexpr ((F & (EVAL_G | EVAL_H)) && (sizeof(T) == 4 || !(F & EVAL_H))) {
+  hipLaunchKernelGGL((tet_eval_kernel<T, F, STAGE, BS, EPL>), dim3(grid), dim3(BS), 0, s, a,
+                     blockE, elemE, reinterpret_cast<T*>(eg), reinterpret_cast<T*>(eH));
+}
+
+// staging needs BS x 78 values of LDS for Hessians: FP32 only
+template <class T, int F>
+constexpr bool can_stage() {
+  return (F & (EVAL_G | EVAL_H)) != 0 && (sizeof(T) == 4 || (F & EVAL_H) == 0);
+}
+
+// Tiles (workgroup size x elements per lane): 0 = 256 x 1 (the default),
+// 1 = 128 x 1, 2 = 64 x 1, 3 = 256 x 2, 4 = 128 x 2, 5 = 64 x 2.  Only the
+// flag sets of the Newton path (E+g, H, E+g+H) have the tuned tiles.
+template <class T, int F>
+static void launch_t(hipStream_t s, const TetArgs& a, double* blockE, double* elemE, void* eg,
+                     void* eH, int tile) {
+  constexpr bool ST = can_stage<T, F>();
+  constexpr bool tuned = F == (EVAL_E | EVAL_G) || F == EVAL_H || F == (EVAL_E | EVAL_G | EVAL_H);
+  if constexpr (tuned) {
+    switch (tile) {
+      case 1: launch_k<T, F, ST, 128, 1>(s, a, blockE, elemE, eg, eH); return;
+      case 2: launch_k<T, F, ST, 64, 1>(s, a, blockE, elemE, eg, eH); return;
+      case 3: launch_k<T, F, ST, 256, 2>(s, a, blockE, elemE, eg, eH); return;
+      case 4: launch_k<T, F, ST, 128, 2>(s, a, blockE, elemE, eg, eH); return;
+      case 5: launch_k<T, F, ST, 64, 2>(s, a, blockE, elemE, eg, eH); return;
+      default: break;
+    }
+  }
+  if constexpr (ST) {
     if (stage_enabled()) {
-      hipLaunchKernelGGL((tet_eval_kernel<T, F, true>), dim3(grid), dim3(TET_BLOCK), 0, s,
-                         a, blockE, elemE, reinterpret_cast<T*>(eg),
-                         reinterpret_cast<T*>(eH));
+      launch_k<T, F, true, 256, 1>(s, a, blockE, elemE, eg, eH);
       return;
     }
   }
-  hipLaunchKernelGGL((tet_eval_kernel<T, F>), dim3(grid), dim3(TET_BLOCK), 0, s,
-                     a, blockE, elemE, reinterpret_cast<T*>(eg),
-                     reinterpret_cast<T*>(eH));
+  launch_k<T, F, false, 256, 1>(s, a, blockE, elemE, eg, eH);
 }
 
 template <class T>
-static void launch_flags(hipStream_t s, int flags, const TetArgs& a,
-                         double* blockE, double* elemE, void* eg, void* eH) {
+static void launch_flags(hipStream_t s, int flags, const TetArgs& a, double* blockE,
+                         double* elemE, void* eg, void* eH, int tile) {
   switch (flags) {
-    case EVAL_E: launch_t<T, EVAL_E>(s, a, blockE, elemE, eg, eH); break;
-    case EVAL_E | EVAL_G: launch_t<T, EVAL_E | EVAL_G>(s, a, blockE, elemE, eg, eH); break;
-    case EVAL_H: launch_t<T, EVAL_H>(s, a, blockE, elemE, eg, eH); break;
+    case EVAL_E: launch_t<T, EVAL_E>(s, a, blockE, elemE, eg, eH, tile); break;
+    case EVAL_E | EVAL_G: launch_t<T, EVAL_E | EVAL_G>(s, a, blockE, elemE, eg, eH, tile); break;
+    case EVAL_H: launch_t<T, EVAL_H>(s, a, blockE, elemE, eg, eH, tile); break;
     case EVAL_E | EVAL_G | EVAL_H:
-      launch_t<T, EVAL_E | EVAL_G | EVAL_H>(s, a, blockE, elemE, eg, eH);
+      launch_t<T, EVAL_E | EVAL_G | EVAL_H>(s, a, blockE, elemE, eg, eH, tile);
       break;
-    case EVAL_G: launch_t<T, EVAL_G>(s, a, blockE, elemE, eg, eH); break;
-    case EVAL_G | EVAL_H: launch_t<T, EVAL_G | EVAL_H>(s, a, blockE, elemE, eg, eH); break;
-    case EVAL_E | EVAL_H: launch_t<T, EVAL_E | EVAL_H>(s, a, blockE, elemE, eg, eH); break;
+    case EVAL_G: launch_t<T, EVAL_G>(s, a, blockE, elemE, eg, eH, tile); break;
+    case EVAL_G | EVAL_H: launch_t<T, EVAL_G | EVAL_H>(s, a, blockE, elemE, eg, eH, tile); break;
+    case EVAL_E | EVAL_H: launch_t<T, EVAL_E | EVAL_H>(s, a, blockE, elemE, eg, eH, tile); break;
     default: break;
   }
 }
 
 void launch_tet_eval(hipStream_t s, bool fp32, int flags, const TetArgs& a,
-                     double* blockE, double* elemE, void* eg, void* eH) {
+                     double* blockE, double* elemE, void* eg, void* eH, int tile) {
   if (fp32)
-    launch_flags<float>(s, flags, a, blockE, elemE, eg, eH);
+    launch_flags<float>(s, flags, a, blockE, elemE, eg, eH, tile);
   else
-    launch_flags<double>(s, flags, a, blockE, elemE, eg, eH);
+    launch_flags<double>(s, flags, a, blockE, elemE, eg, eH, tile);
 }
 
 }  // namespace pgn
