@@ -272,7 +272,20 @@ typedef struct {
      (default here) or PGN_LS_EIGENLDLT (the reference's default; linear
      solves are then exact and adaptive_tol is not used)                  */
   int lsSolverType;
+  /* OptimSolverOptions::qpSolverType (OptimSolver.h:30), QPSolverType values
+     (BasicTypes.h:48-55): PGN_QP_NEWTON (default), PGN_QP_STEEPEST or
+     PGN_QP_LBFGS (single models only; ComputeStep_LBFGS,
+     OptimSolver.cpp:522-591, with a history of 3 pairs, corrected: the
+     previous x and g are updated at every step, the second loop wraps
+     modulo the history length and the pair count saturates at it, where
+     the reference takes them once, wraps modulo the problem size and falls
+     back to one pair when its ring wraps; the history starts afresh with
+     every solve)                                                         */
+  int qpSolverType;
 } pgn_newton_opts;
+
+/* QPSolverType (BasicTypes.h:48-55) */
+enum pgn_qp_solver { PGN_QP_STEEPEST = 0, PGN_QP_NEWTON = 1, PGN_QP_LBFGS = 2 };
 
 /* OSResult (BasicTypes.h:69-77) */
 enum pgn_os_result {

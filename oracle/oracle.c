@@ -876,6 +876,37 @@ void orc_newton_default_opts(orc_newton_opts* o) {
   o->lSearch_alpha = 0.5;
   o->ls_noise_guard = 0.0;
   o->ls_solver = 1;
+  o->qpSolverType = 1; /* QPSolverType::Newton */
+}
+
+void orc_lbfgs_direction(int N, int M, int count, int newest, const double* S,
+                         const double* Y, const double* g, double* dx) {
+  double a[64];
+  for (int k = 0; k < N; ++k) dx[k] = -g[k];
+  if (count <= 0) return;
+  /* newest to oldest */
+  int j = newest;
+  for (int i = 0; i < count; ++i) {
+    const double* s = S + (size_t)j * N;
+    const double* y = Y + (size_t)j * N;
+    a[j] = dot(N, s, dx) / dot(N, y, s);
+    for (int k = 0; k < N; ++k) dx[k] -= a[j] * y[k];
+    if (i + 1 < count) j = (j + M - 1) % M;
+  }
+  {
+    const double* s = S + (size_t)newest * N;
+    const double* y = Y + (size_t)newest * N;
+    const double gam = dot(N, s, y) / dot(N, y, y);
+    for (int k = 0; k < N; ++k) dx[k] *= gam;
+  }
+  /* oldest to newest */
+  for (int i = 0; i < count; ++i) {
+    const double* s = S + (size_t)j * N;
+    const double* y = Y + (size_t)j * N;
+    const double beta = dot(N, y, dx) / dot(N, y, s);
+    for (int k = 0; k < N; ++k) dx[k] += (a[j] - beta) * s[k];
+    j = (j + 1) % M;
+  }
 }
 
 static double norm2(int n, const double* a) { return sqrt(dot(n, a, a)); }
@@ -883,12 +914,16 @@ static double norm2(int n, const double* a) { return sqrt(dot(n, a, a)); }
 /* OptimSolver_USQP_LS::SolveStepInternal (OptimSolver_USQP_LS.cpp:18-68),
  * ComputeStep (OptimSolver.cpp:313-352), LineSearch_Simple (:688-736) and
  * UpdateStepResult (:402-466). */
+#define ORC_LBFGS_M 3 /* the reference's history length (OptimSolver.cpp:527) */
 typedef struct {
   int N;
   double *vx, *v0, *vg, *dx, *dxo, *vgNew, *xsave, *b, *xt;
   double obj, opt;
   int result, minImprCount, minStepCount;
   int lin_its, bis;
+  /* LBFGS: previous x and g of the last LBFGS step, ring of pairs */
+  int bfgs_init, bfgs_count, bfgs_newest;
+  double *bx_p, *bg_p, *bS, *bY;
 } nstate;
 
 static void solve_step(orc_problem* P, const orc_newton_opts* o, nstate* S,
@@ -904,7 +939,28 @@ static void solve_step(orc_problem* P, const orc_newton_opts* o, nstate* S,
     S->result = ORC_OS_SUCCESS;
     return;
   }
-  if (!steepest) {
+  if (!steepest && o->qpSolverType == 2) {
+    /* ComputeStep_LBFGS (OptimSolver.cpp:522-591), corrected */
+    if (!S->bfgs_init) {
+      memcpy(S->bx_p, S->vx, sizeof(double) * N);
+      memcpy(S->bg_p, S->vg, sizeof(double) * N);
+      for (int k = 0; k < N; ++k) S->dx[k] = -S->vg[k];
+      S->bfgs_init = 1;
+    } else {
+      const int slot = S->bfgs_count == 0 ? 0 : (S->bfgs_newest + 1) % ORC_LBFGS_M;
+      double* sv = S->bS + (size_t)slot * N;
+      double* yv = S->bY + (size_t)slot * N;
+      for (int k = 0; k < N; ++k) {
+        sv[k] = S->vx[k] - S->bx_p[k];
+        yv[k] = S->vg[k] - S->bg_p[k];
+      }
+      memcpy(S->bx_p, S->vx, sizeof(double) * N);
+      memcpy(S->bg_p, S->vg, sizeof(double) * N);
+      S->bfgs_newest = slot;
+      if (S->bfgs_count < ORC_LBFGS_M) S->bfgs_count++;
+      orc_lbfgs_direction(N, ORC_LBFGS_M, S->bfgs_count, slot, S->bS, S->bY, S->vg, S->dx);
+    }
+  } else if (!steepest && o->qpSolverType != 0) {
     for (int k = 0; k < N; ++k) S->dx[k] = S->vx[k] - S->v0[k]; /* :488 */
     for (int k = 0; k < N; ++k) S->b[k] = -S->vg[k];
     S->lin_its += P->solve(P->p, S->b, S->dx, o);
@@ -976,6 +1032,10 @@ void orc_newton_generic(orc_problem* P, const orc_newton_opts* o,
   S.N = N;
   double** bufs[] = {&S.vx, &S.v0, &S.vg, &S.dx, &S.dxo, &S.vgNew, &S.xsave, &S.b, &S.xt};
   for (int i = 0; i < 9; ++i) *bufs[i] = (double*)calloc(N, sizeof(double));
+  S.bx_p = (double*)calloc(N, sizeof(double));
+  S.bg_p = (double*)calloc(N, sizeof(double));
+  S.bS = (double*)calloc((size_t)ORC_LBFGS_M * N, sizeof(double));
+  S.bY = (double*)calloc((size_t)ORC_LBFGS_M * N, sizeof(double));
   P->get_x(P->p, S.vx);
   memcpy(S.v0, P->v0_keep ? P->v0_keep : S.vx, sizeof(double) * N);
   S.obj = P->energy(P->p);
@@ -1021,6 +1081,10 @@ void orc_newton_generic(orc_problem* P, const orc_newton_opts* o,
   R->optimality = S.opt;
   if (P->v0_keep) memcpy(P->v0_keep, S.v0, sizeof(double) * N);
   for (int i = 0; i < 9; ++i) free(*bufs[i]);
+  free(S.bx_p);
+  free(S.bg_p);
+  free(S.bS);
+  free(S.bY);
 }
 
 /* tet model as a problem: Hessian in lower CSR + Jacobi CG with the

@@ -140,7 +140,25 @@ typedef struct {
    * LSSolverType (BasicTypes.h:36-46): 1 = EigenCG (Jacobi CG, the default
    * here), 2 = EigenLDLT (the reference's own default) */
   int ls_solver;
+  /* OptimSolverOptions::qpSolverType (OptimSolver.h:30) with the values of
+   * QPSolverType (BasicTypes.h:48-55): 0 = Steepest, 1 = Newton (default),
+   * 2 = LBFGS (ComputeStep_LBFGS, OptimSolver.cpp:522-591, corrected: see
+   * orc_lbfgs_direction) */
+  int qpSolverType;
 } orc_newton_opts;
+
+/* The LBFGS direction dx = -H g by the two-loop recursion (Nocedal & Wright,
+ * Numerical Optimization, 2nd ed., Alg. 7.4) over the `count` newest of the
+ * pairs (s_i, y_i) held in a ring of M columns (column-major N x M arrays S,
+ * Y), the newest in column `newest`; H0 = (s.y / y.y) I of the newest pair,
+ * as ComputeStep_LBFGS scales it (OptimSolver.cpp:568-580).  The reference's
+ * own recursion differs in three places, corrected here: its previous x and
+ * g are taken once (OptimSolver.cpp:524-525) instead of at every step, its
+ * second loop advances the ring index modulo the problem size (:587)
+ * instead of M, and its pair count min(M, K + 1) uses the ring slot K, so
+ * that it falls back to one pair whenever the ring wraps (:567). */
+void orc_lbfgs_direction(int N, int M, int count, int newest, const double* S,
+                         const double* Y, const double* g, double* dx);
 
 /* OSResult (BasicTypes.h:69-77) */
 enum {

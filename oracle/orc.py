@@ -37,7 +37,8 @@ class NewtonOpts(ctypes.Structure):
                 ("lsTolMaxError", ctypes.c_double),
                 ("lsNumRegTrials", ctypes.c_int),
                 ("ls_noise_guard", ctypes.c_double),
-                ("ls_solver", ctypes.c_int)]
+                ("ls_solver", ctypes.c_int),
+                ("qpSolverType", ctypes.c_int)]
 
 
 class NewtonResult(ctypes.Structure):
@@ -85,6 +86,9 @@ def lib():
         L.orc_model_nnz_lower.argtypes = [ctypes.c_void_p]
         L.orc_model_hessian_lower.argtypes = [ctypes.c_void_p, _lp, _ip, _dp]
         L.orc_model_element_egh.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_lbfgs_direction.restype = None
+        L.orc_lbfgs_direction.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_cg.restype = ctypes.c_int
         L.orc_cg.argtypes = [ctypes.c_int, _lp, _ip, _dp, _dp, _dp,
                              ctypes.c_double, ctypes.c_int, _dp]
@@ -512,6 +516,18 @@ class Rod:
         return {"result": OS_NAMES[r.result], "iterations": r.iterations,
                 "bisections": r.line_search_bisections, "objective": r.objective,
                 "optimality": r.optimality}
+
+
+def lbfgs_direction(S, Y, g, count, newest):
+    """orc_lbfgs_direction: S, Y are (M, N) arrays (pair i in row i), the
+    `count` newest pairs ending at row `newest` are used; returns -H g."""
+    S = f64(S)
+    Y = f64(Y)
+    M, N = S.shape
+    dx = np.zeros(N)
+    lib().orc_lbfgs_direction(N, M, int(count), int(newest), _p(S), _p(Y), _p(f64(g)),
+                              _p(dx))
+    return dx
 
 
 def dense_ldlt_solve(A, b, x, max_error, reg_iters):

@@ -20,6 +20,7 @@ LS_EIGENCG, LS_EIGENLDLT = 1, 2  # LSSolverType values (BasicTypes.h:36-46)
 LAYOUT_LOWER, LAYOUT_FULL, LAYOUT_BSR3 = 0, 1, 2
 # Newton options of a solve "to residual 1e-8": the reference's MINIMPR /
 # MINSTEP exits and its energy-only line search would stop short of it
+QP_STEEPEST, QP_NEWTON, QP_LBFGS = 0, 1, 2  # QPSolverType (BasicTypes.h:48-55)
 CONVERGE_OPTS = dict(tolMaxError=1e-8, tolMinImpr=0.0, tolMinStep=0.0,
                      ls_noise_guard=1e-12)
 OS_NAMES = ["ONGOING", "SUCCESS", "MINSTEP", "MINIMPR", "FAILURE", "MAXITER",
@@ -83,7 +84,8 @@ class NewtonOpts(ctypes.Structure):
                 ("lsNumRegTrials", ctypes.c_int), ("precond", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("adaptive_tol", ctypes.c_int),
                 ("ls_noise_guard", ctypes.c_double), ("ew_gamma", ctypes.c_double),
-                ("ew_max", ctypes.c_double), ("lsSolverType", ctypes.c_int)]
+                ("ew_max", ctypes.c_double), ("lsSolverType", ctypes.c_int),
+                ("qpSolverType", ctypes.c_int)]
 
 
 class ServerOpts(ctypes.Structure):
@@ -227,11 +229,21 @@ def lib():
     return _LIB
 
 
+def _newton_opts(kw):
+    o = NewtonOpts()
+    lib().pgn_newton_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        if k not in dict(NewtonOpts._fields_):
+            raise AttributeError(f"pgn_newton_opts has no field {k}")
+        setattr(o, k, v)
+    return o
+
+
 def _linear_opts(kw):
     o = LinearOpts()
     lib().pgn_linear_default_opts(ctypes.byref(o))
     for k, v in kw.items():
-        if not hasattr(o, k):
+        if k not in dict(LinearOpts._fields_):
             raise AttributeError(f"pgn_linear_opts has no field {k}")
         setattr(o, k, v)
     return o
@@ -432,10 +444,7 @@ class TetModel:
         return E, g, H
 
     def linear_solve(self, b, x0=None, **kw):
-        o = LinearOpts()
-        lib().pgn_linear_default_opts(ctypes.byref(o))
-        for k, v in kw.items():
-            setattr(o, k, v)
+        o = _linear_opts(kw)
         b = _f64(b).reshape(-1)
         x = np.zeros(self.N) if x0 is None else _f64(x0).reshape(-1).copy()
         r = LinearResult()
@@ -445,10 +454,7 @@ class TetModel:
                    "rel_residual": r.rel_residual}
 
     def newton(self, **kw):
-        o = NewtonOpts()
-        lib().pgn_newton_default_opts(ctypes.byref(o))
-        for k, v in kw.items():
-            setattr(o, k, v)
+        o = _newton_opts(kw)
         r = NewtonResult()
         _ok(lib().pgn_newton_solve(self._h, ctypes.byref(o), ctypes.byref(r)))
         return {"result": OS_NAMES[r.result], "iterations": r.iterations,
@@ -461,10 +467,7 @@ class TetModel:
     def dynamic_step(self, dt, **kw):
         """One backward-Euler time step of length dt (OptimProblem_BasicDynamic
         minimised by SolveFull); updates x and the velocity."""
-        o = NewtonOpts()
-        lib().pgn_newton_default_opts(ctypes.byref(o))
-        for k, v in kw.items():
-            setattr(o, k, v)
+        o = _newton_opts(kw)
         r = NewtonResult()
         _ok(lib().pgn_dynamic_step(self._h, float(dt), ctypes.byref(o), ctypes.byref(r)))
         return {"result": OS_NAMES[r.result], "iterations": r.iterations,
@@ -615,10 +618,7 @@ class Batch:
         _ok(lib().pgn_batch_reset(self._h))
 
     def newton(self, **kw):
-        o = NewtonOpts()
-        lib().pgn_newton_default_opts(ctypes.byref(o))
-        for k, v in kw.items():
-            setattr(o, k, v)
+        o = _newton_opts(kw)
         res = (NewtonResult * self.K)()
         _ok(lib().pgn_batch_newton(self._h, ctypes.byref(o), res))
         return [_result_dict(r) for r in res]
@@ -650,8 +650,7 @@ class Server:
         o.precision = precision
         o.group_by_size = int(group_by_size)
         o.max_wait_s = float(max_wait_s)
-        for k, v in newton.items():
-            setattr(o.newton, k, v)
+        o.newton = _newton_opts(newton)
         h = _vp()
         _ok(lib().pgn_server_create(device, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -822,10 +821,7 @@ class Rods:
         return E, g, H
 
     def newton(self, **kw):
-        o = NewtonOpts()
-        lib().pgn_newton_default_opts(ctypes.byref(o))
-        for k, v in kw.items():
-            setattr(o, k, v)
+        o = _newton_opts(kw)
         res = (NewtonResult * self.R)()
         _ok(lib().pgn_rods_newton(self._h, ctypes.byref(o), res))
         return [{"result": OS_NAMES[r.result], "iterations": r.iterations,

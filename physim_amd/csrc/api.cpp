@@ -230,7 +230,8 @@ size_t pgn_model_device_bytes(const pgn_model* M) {
                    M->blockE, M->nodeP, M->scal, M->g, M->f_ini, M->f_end, M->pv, M->pb,
                    M->part1, M->part2, M->partb, M->red64, M->cgs, M->sell_so, M->sell_len,
                    M->sell_col, M->sell_c16, M->sell_val, M->sc, M->r64, M->tmp64a,
-                   M->tmp64b, M->nw_xsave, M->nw_v0, M->nw_dx, M->nw_b, M->vel, M->xhat,
+                   M->tmp64b, M->nw_xsave, M->nw_v0, M->nw_dx, M->nw_b, M->lb_xp, M->lb_gp,
+                   M->lb_S, M->lb_Y, M->vel, M->xhat,
                    M->xstep0, M->ldlt_prob, M->ldlt_perm, M->ldlt_iperm, M->ldlt_nprob,
                    M->ldlt_list, M->ldlt_status, M->ldlt_band, M->ldlt_D, M->ldlt_sinT,
                    M->ldlt_y);
@@ -526,6 +527,7 @@ void pgn_newton_default_opts(pgn_newton_opts* o) {
   o->ew_gamma = 0.9;
   o->ew_max = 0.2;
   o->lsSolverType = PGN_LS_EIGENCG;
+  o->qpSolverType = PGN_QP_NEWTON;
 }
 
 int pgn_newton_solve(pgn_model* M, const pgn_newton_opts* opts, pgn_newton_result* res) {

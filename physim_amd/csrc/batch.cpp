@@ -479,6 +479,7 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
 }
 
 void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
+  PGN_CHECK(o.qpSolverType == PGN_QP_NEWTON, "fused batches take Newton steps (PGN_QP_NEWTON)");
   PGN_CHECK(o.lsSolverType == PGN_LS_EIGENCG || o.lsSolverType == PGN_LS_EIGENLDLT,
             "lsSolverType: PGN_LS_EIGENCG or PGN_LS_EIGENLDLT");
   const bool direct = o.lsSolverType == PGN_LS_EIGENLDLT;

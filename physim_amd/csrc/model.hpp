@@ -106,6 +106,10 @@ struct pgn_model {
   pgn::DBuf<pgn::PcgScalars> sc;
   pgn::DBuf<double> r64, tmp64a, tmp64b;
   pgn::DBuf<double> nw_xsave, nw_v0, nw_dx, nw_b;  // Newton work vectors
+  // LBFGS (QPSolverType::LBFGS): x and g of the last LBFGS step, a ring of
+  // LBFGS_M pairs (s, y), 3n each
+  static constexpr int LBFGS_M = 3;
+  pgn::DBuf<double> lb_xp, lb_gp, lb_S, lb_Y;
   // implicit dynamics (OptimProblem_BasicDynamic): while dyn_dt > 0 the
   // objective carries the inertia term of one backward-Euler step of length
   // dyn_dt toward the predictor xhat

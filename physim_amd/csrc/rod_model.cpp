@@ -271,6 +271,7 @@ long pgn_rods::ref_index(int r, long i4) const {
 // ---- batched Newton -----------------------------------------------------------------
 
 void pgn_rods::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
+  PGN_CHECK(o.qpSolverType == PGN_QP_NEWTON, "rod batches take Newton steps (PGN_QP_NEWTON)");
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = stream();
   const long nd = 4 * nv;

@@ -212,8 +212,21 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
   PGN_CHECK(o.lsSolverType == PGN_LS_EIGENCG || o.lsSolverType == PGN_LS_EIGENLDLT,
             "lsSolverType: PGN_LS_EIGENCG or PGN_LS_EIGENLDLT");
   lo.solver = o.lsSolverType;
-  const bool direct = o.lsSolverType == PGN_LS_EIGENLDLT;
+  PGN_CHECK(o.qpSolverType == PGN_QP_NEWTON || o.qpSolverType == PGN_QP_STEEPEST ||
+                o.qpSolverType == PGN_QP_LBFGS,
+            "qpSolverType: PGN_QP_STEEPEST, PGN_QP_NEWTON or PGN_QP_LBFGS");
+  const bool direct = o.lsSolverType == PGN_LS_EIGENLDLT && o.qpSolverType == PGN_QP_NEWTON;
   if (direct) M.ldlt_prepare();  // a band beyond the limit fails before any step
+  // LBFGS history (ComputeStep_LBFGS, OptimSolver.cpp:522-591), fresh per solve
+  const bool lbfgs = o.qpSolverType == PGN_QP_LBFGS;
+  bool lb_init = false;
+  int lb_count = 0, lb_newest = 0;
+  if (lbfgs) {
+    M.lb_xp.ensure(N);
+    M.lb_gp.ensure(N);
+    M.lb_S.ensure((size_t)pgn_model::LBFGS_M * N);
+    M.lb_Y.ensure((size_t)pgn_model::LBFGS_M * N);
+  }
   R = pgn_newton_result{};
 
   auto d2d = [&](double* dst, const double* src) {
@@ -259,7 +272,50 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
       return;
     }
     // ComputeStep (OptimSolver.cpp:313-352); trial 0 always succeeds
-    if (!use_steepest) {
+    if (!use_steepest && lbfgs) {
+      // ComputeStep_LBFGS, corrected: the newest pair is (x - x_prev,
+      // g - g_prev) of the last LBFGS step, then the two-loop recursion
+      // (Nocedal & Wright, Numerical Optimization, Alg. 7.4)
+      inexact = false;
+      if (!lb_init) {
+        d2d(M.lb_xp.p, xsave);
+        d2d(M.lb_gp.p, M.g.p);
+        launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, dx);  // steepest first
+        lb_init = true;
+      } else {
+        const int Mh = pgn_model::LBFGS_M;
+        const int slot = lb_count == 0 ? 0 : (lb_newest + 1) % Mh;
+        double* S = M.lb_S.p;
+        double* Y = M.lb_Y.p;
+        launch_lincomb64(s, N, 1.0, xsave, -1.0, M.lb_xp.p, S + slot * N);
+        launch_lincomb64(s, N, 1.0, M.g.p, -1.0, M.lb_gp.p, Y + slot * N);
+        d2d(M.lb_xp.p, xsave);
+        d2d(M.lb_gp.p, M.g.p);
+        lb_newest = slot;
+        if (lb_count < Mh) lb_count++;
+        double a[pgn_model::LBFGS_M];
+        launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, dx);
+        int j = lb_newest;
+        for (int i = 0; i < lb_count; ++i) {  // newest to oldest
+          const double* sj = S + j * N;
+          const double* yj = Y + j * N;
+          a[j] = M.dot(sj, dx, N) / M.dot(yj, sj, N);
+          launch_lincomb64(s, N, 1.0, dx, -a[j], yj, dx);
+          if (i + 1 < lb_count) j = (j + Mh - 1) % Mh;
+        }
+        const double* sn = S + lb_newest * N;
+        const double* yn = Y + lb_newest * N;
+        const double gam = M.dot(sn, yn, N) / M.dot(yn, yn, N);
+        launch_lincomb64(s, N, gam, dx, 0.0, dx, dx);
+        for (int i = 0; i < lb_count; ++i) {  // oldest to newest
+          const double* sj = S + j * N;
+          const double* yj = Y + j * N;
+          const double beta = M.dot(yj, dx, N) / M.dot(yj, sj, N);
+          launch_lincomb64(s, N, 1.0, dx, a[j] - beta, sj, dx);
+          j = (j + 1) % Mh;
+        }
+      }
+    } else if (!use_steepest && o.qpSolverType == PGN_QP_NEWTON) {
       M.eval_hessian();
       R.hessian_evals++;
       launch_lincomb64(s, N, 1.0, xsave, -1.0, v0, dx);  // dx0 = x - x_prev (:488)
