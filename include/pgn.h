@@ -151,6 +151,14 @@ int pgn_model_hessian_pattern(pgn_model* model, int layout, long* rowptr,
 /* OptimProblem_BasicStatic::GetHessian -> GetHessian + FixMatrixDoFs.
  * Values in the model's Hessian precision, widened to FP64 (nnz*9 for BSR3) */
 int pgn_model_get_hessian(pgn_model* model, int layout, double* values);
+/* y = H p with the Hessian of the current x (assembled first, fixed rows and
+ * columns projected; in the model's Hessian precision, p and y FP64 at the
+ * boundary): the product of each CG iteration (LinearSolver_EigenCG.cpp:89,
+ * Eigen's ConjugateGradient).  matrix_free = 0: the assembled BSR3 matrix;
+ * 1: the sum of the element products over the stored element Hessians
+ * (tet models), as the PCG forms it under PGN_MATFREE=1. */
+int pgn_model_hessian_product(pgn_model* model, int matrix_free, const double* p,
+                              double* y);
 /* IEnergyElement::GetLocalEnergy/Gradient/Hessian for every element at the
  * current x, from the fused E+g+H element kernels of the given precision
  * (PGN_FP64 or PGN_MIXED = all-FP32 kernels).  Any pointer may be NULL.

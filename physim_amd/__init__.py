@@ -38,6 +38,7 @@ EXPORTS = [
     "pgn_model_get_gradient", "pgn_model_hessian_nnz",
     "pgn_model_hessian_pattern", "pgn_model_get_hessian",
     "pgn_model_element_egh", "pgn_linear_default_opts", "pgn_linear_solve",
+    "pgn_model_hessian_product",
     "pgn_newton_default_opts", "pgn_newton_solve", "pgn_time_kernel",
     "pgn_kernel_bytes", "pgn_host_bsr_pattern", "pgn_model_set_profiling",
     "pgn_model_get_profile", "pgn_model_create_shell", "pgn_batch_create",
@@ -147,6 +148,7 @@ def lib():
         L.pgn_model_hessian_pattern.argtypes = [_vp, ctypes.c_int, _lp, _ip]
         L.pgn_model_get_hessian.argtypes = [_vp, ctypes.c_int, _dp]
         L.pgn_model_element_egh.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp]
+        L.pgn_model_hessian_product.argtypes = [_vp, ctypes.c_int, _dp, _dp]
         L.pgn_linear_default_opts.argtypes = [ctypes.POINTER(LinearOpts)]
         L.pgn_linear_default_opts.restype = None
         L.pgn_linear_solve.argtypes = [_vp, _dp, _dp, ctypes.POINTER(LinearOpts),
@@ -442,6 +444,13 @@ class TetModel:
         H = np.zeros((self.m, 12, 12))
         _ok(lib().pgn_model_element_egh(self._h, prec, _p(E), _p(g), _p(H)))
         return E, g, H
+
+    def hessian_product(self, p, matrix_free=False):
+        """y = H p at the current x (pgn_model_hessian_product)."""
+        p = _f64(p).reshape(-1)
+        y = np.zeros(self.N)
+        _ok(lib().pgn_model_hessian_product(self._h, 1 if matrix_free else 0, _p(p), _p(y)))
+        return y
 
     def linear_solve(self, b, x0=None, **kw):
         o = _linear_opts(kw)

@@ -421,6 +421,42 @@ int pgn_model_get_hessian(pgn_model* M, int layout, double* values) {
   });
 }
 
+int pgn_model_hessian_product(pgn_model* M, int matrix_free, const double* p, double* y) {
+  return guard([&] {
+    PGN_CHECK(M && p && y, "null argument");
+    PGN_CHECK(!matrix_free || (M->m_mem == 0 && M->m_hinge == 0),
+              "the matrix-free product covers tet models");
+    use_device(M->ctx);
+    M->eval_hessian();
+    hipStream_t s = M->stream();
+    const long N = 3L * M->n;
+    const size_t ts = M->tsize();
+    char* pin = M->pv.p + 3 * N * ts;   // the PCG's p0 and q slots
+    char* qout = M->pv.p + 5 * N * ts;
+    if (M->hfp32) {
+      PGN_HIP(hipMemcpyAsync(M->tmp64a.p, p, N * sizeof(double), hipMemcpyHostToDevice, s));
+      launch_cast_f64_to_f32(s, N, M->tmp64a.p, reinterpret_cast<float*>(pin));
+    } else {
+      PGN_HIP(hipMemcpyAsync(pin, p, N * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    if (matrix_free)
+      launch_matfree_product(s, M->hfp32, M->mf_args(), pin, qout, nullptr,
+                             pcg_grid1(M->n, M->pcg_lpr));
+    else
+      launch_spmv(s, M->hfp32, M->n, M->rowptr.p, M->bcol.p, M->bval.p, pin, qout, M->pcg_lpr);
+    PGN_HIP(hipGetLastError());
+    if (M->hfp32) {
+      std::vector<float> h(N);
+      PGN_HIP(hipMemcpyAsync(h.data(), qout, N * sizeof(float), hipMemcpyDeviceToHost, s));
+      M->sync();
+      for (long k = 0; k < N; ++k) y[k] = h[k];
+    } else {
+      PGN_HIP(hipMemcpyAsync(y, qout, N * sizeof(double), hipMemcpyDeviceToHost, s));
+      M->sync();
+    }
+  });
+}
+
 int pgn_model_element_egh(pgn_model* M, int precision, double* E, double* g, double* H) {
   return guard([&] {
     PGN_CHECK(M, "null model");

@@ -141,6 +141,27 @@ struct PcgScalars {
   int pad[2];
 };
 
+// Matrix-free Hessian product over the stored packed element Hessians (tet
+// models): y_e = H_e p_e per tet into ye (12 per tet), then per node the
+// sum over its incidences with the fixed rows' identity and the diagonal
+// shift (matfree_kernels.hip).
+struct MatFreeArgs {
+  long m;                // tets
+  int n;                 // nodes
+  const int4* tets;
+  const void* eH;        // TET_H_PACK per tet, Hessian precision
+  void* ye;              // 12 per tet, Hessian precision
+  const int* inc_ptr;
+  const int* inc;
+  const uint8_t* fixmask;
+  double shift;          // regularization shift on every diagonal entry
+};
+// q = H p (p, q: 3n in the Hessian precision); part: one p.q partial per
+// workgroup of the node launch (grid of them, rows blocked per XCD as the
+// PCG SpMV) or null
+void launch_matfree_product(hipStream_t s, bool fp32, const MatFreeArgs& a, const void* p,
+                            void* q, double* part, int grid);
+
 struct PcgArgs {
   int n;                 // block rows (nodes)
   const long* rowptr;    // n+1
@@ -186,6 +207,9 @@ struct PcgArgs {
   // cap is kctl[1] (both written by the host before each replay).  Null:
   // the arguments are absolute.
   const int* kctl = nullptr;
+  // split form only: the SpMV launch replaced by the matrix-free product
+  int mf = 0;
+  MatFreeArgs mfa;
 };
 int pcg_grid1(int n, int lpr);
 // SELL-64 layout: slices per workgroup and grid (<= PCG_MAXP) for ns slices

@@ -196,6 +196,15 @@ void pgn_model::setup(int n_nodes, const double* X0h, int n_elems, const int* Th
     else ++m_hinge;
   }
   PGN_CHECK(!(m_mem + m_hinge) || extra, "shell elements need thickness / kB values");
+  {
+    const char* mf = std::getenv("PGN_MATFREE");
+    use_matfree = mf && std::strcmp(mf, "1") == 0 && m_mem == 0 && m_hinge == 0 && m > 0;
+    if (use_matfree) {  // the textbook split form with direct launches
+      pcg_cg1 = false;
+      pcg_split = true;
+      use_graphs = 0;
+    }
+  }
   build_bsr_pattern(n, m, Th, pat);
   hX0.assign(X0h, X0h + 3L * n);
   hfix.assign(3L * n, 0);
@@ -573,6 +582,20 @@ PcgArgs pgn_model::pcg_args(double tol, int precond) const {
   a.part1 = part1.p;
   a.part2 = part2.p;
   a.sc = sc.p;
+  // the matrix-free product for static solves (the inertia of a time step
+  // lives in the assembled diagonal only)
+  if (use_matfree && dyn_dt == 0.0 && mf_ye.p) {
+    a.mf = 1;
+    a.mfa.m = m_tet;
+    a.mfa.n = n;
+    a.mfa.tets = tets.p;
+    a.mfa.eH = eH.p;
+    a.mfa.ye = mf_ye.p;
+    a.mfa.inc_ptr = inc_ptr.p;
+    a.mfa.inc = inc.p;
+    a.mfa.fixmask = fixmask.p;
+    a.mfa.shift = mf_shift;
+  }
   return a;
 }
 
@@ -602,7 +625,24 @@ void pgn_model::eval_hessian() {
   launch_elements(hfp32, EVAL_H, nullptr, nullptr, nullptr, eH.p);
   launch_bsr_gather(s, hfp32, bsr_args(), eH.p, bval.p);
   update_precond();
+  mf_shift = 0.0;
   hess_valid = true;
+}
+
+MatFreeArgs pgn_model::mf_args() {
+  MatFreeArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.m = m_tet;
+  a.n = n;
+  a.tets = tets.p;
+  a.eH = eH.p;
+  mf_ye.ensure(12L * std::max<long>(m_tet, 1) * tsize());
+  a.ye = mf_ye.p;
+  a.inc_ptr = inc_ptr.p;
+  a.inc = inc.p;
+  a.fixmask = fixmask.p;
+  a.shift = mf_shift;
+  return a;
 }
 
 double pgn_model::frob_lower() {

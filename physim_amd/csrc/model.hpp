@@ -87,6 +87,15 @@ struct pgn_model {
   // update folded into the SpMV's column gathers, update launch)
   bool pcg_cg1 = pgn::pcg_variant() == 0;
   bool pcg_split = pgn::pcg_variant() != 2;
+  // PGN_MATFREE=1 in the environment: the PCG product of a tet model's
+  // static solves is the matrix-free one over the stored element Hessians
+  // (textbook split form, direct launches; the assembled matrix still
+  // supplies the preconditioner); mf_shift tracks the regularization shift
+  // the assembled diagonal received
+  bool use_matfree = false;
+  double mf_shift = 0.0;
+  pgn::DBuf<char> mf_ye;
+  pgn::MatFreeArgs mf_args();
   pgn::DBuf<double> cgs;  // its 4 scalars
   // SELL-64 copy of the matrix for the C-G SpMV (PGN_SELL=0 in the
   // environment keeps the BSR3 SpMV); built on the first PCG solve, packed

@@ -19,6 +19,7 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "pgn_internal.hpp"
 #include "reduce.hpp"
 
 namespace pgn {
@@ -967,6 +968,12 @@ static void launch_k1_l(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEv
     return;
   }
   if (a.split) {
+    if (a.mf) {  // matrix-free product on this iteration's p (direct launches only)
+      if (e0) PGN_HIP(hipEventRecord(e0, s));
+      launch_matfree_product(s, fp32, a.mfa, (k & 1) ? a.p1 : a.p0, a.q, a.part1, a.g1);
+      if (e1) PGN_HIP(hipEventRecord(e1, s));
+      return;
+    }
     if (fp32)
       launch_timed(pcg_k1b_kernel<float, L>, dim3(a.g1), dim3(K1_BS), s, e0, e1, a, k);
     else

@@ -156,6 +156,7 @@ void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opt
   }
   PGN_CHECK(M.hess_valid, "Hessian not assembled");
   M.tune();
+  if (M.use_matfree) M.mf_args();  // its element product buffer
   hipStream_t s = M.stream();
   const long N = 3L * M.n;
   double* xprev = M.tmp64b.p;
@@ -171,6 +172,7 @@ void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opt
     if (i != 0) {
       if (regT < 0.0) regT = 1e-6 * M.frob_lower();  // LinearSolver.cpp:57-66
       launch_add_diag(s, M.hfp32, M.n, M.diag.p, M.bval.p, regT * std::pow(10.0, i - 1));
+      M.mf_shift += regT * std::pow(10.0, i - 1);
       M.update_precond();
       res.reg_used = i;
     }
