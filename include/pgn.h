@@ -463,6 +463,53 @@ int pgn_model_set_profiling(pgn_model* model, int on);
 int pgn_model_get_profile(pgn_model* model, int which, long* count,
                           double* total_ms);
 
+/* ---- scene ingestion ----------------------------------------------------- */
+/* TetGen volume meshes, the format of Serializer_Mesh::ReadVolume_TET
+ * (Serializer_Mesh.h:46-52; declared, never implemented in the reference):
+ * base.node and base.ele (base may carry either extension), Tet4 only,
+ * indices from 0 or 1; a tet listed with negative volume gets its last two
+ * corners swapped.  X: 3 per node; T: 4 per tet, 0-based. */
+int pgn_tetgen_sizes(const char* base, int* n_nodes, int* n_tets);
+int pgn_tetgen_read(const char* base, double* X, int* T);
+
+/* A scene document in the reference's XML vocabulary: a <scene> root whose
+ * children are the elements Serializer_XML reads (<simulable> as
+ * LoadSimulable_VolumeFEM, Serializer_XML.cpp:1199-1325; <boundary> with
+ * <bc> children as LoadBoundary, :1507-1647; <problem> as
+ * LoadOptimProblem, :1327-1354; <solver> as LoadOptimSolver, :1356-1505).
+ * Volume Tet4 models with CONH / STVK materials, Gravity and hard
+ * FixPosition conditions, Static / Dynamic problems, Newton / Steepest /
+ * LBFGS steps with EigenCG / EigenLDLT; anything else fails with
+ * PGN_ERR_INVALID naming it.  meshInit FILEVOLUME reads its `path`
+ * attribute (relative to the document; the reference reads `mesh`), GRID
+ * builds CreateTetMesh_Grid's 5-tet split (a TODO in the reference's tet
+ * reader).  Host only: no device is used until pgn_scene_create_model. */
+typedef struct pgn_scene pgn_scene;
+int pgn_scene_load(const char* xml_path, pgn_scene** out);
+void pgn_scene_destroy(pgn_scene* scene);
+int pgn_scene_sizes(const pgn_scene* scene, int* n_nodes, int* n_tets);
+/* X0: rest positions (meshRest, else meshInit), X: initial positions
+ * (meshInit), 3 per node; T: 4 per tet; per tet material kind (pgn_material),
+ * Lame parameters (from YOUNG / POISSON by ParameterSet::
+ * InitLinearFromYoungPoisson, ParameterSet.cpp:54-70) and density.  Any
+ * pointer may be NULL. */
+int pgn_scene_mesh(const pgn_scene* scene, double* X0, double* X, int* T,
+                   unsigned char* material, double* lame1, double* lame2,
+                   double* density);
+/* fixed: 3 per node; gravity as BC_Gravity without an animator applies it:
+ * the same value at every stage (g_ini = g_end), stages = numStep */
+int pgn_scene_boundary(const pgn_scene* scene, unsigned char* fixed,
+                       double g_ini[3], double g_end[3], int* stages,
+                       int* has_gravity);
+/* the solver options of <solver> over pgn_newton_default_opts; dynamic and
+ * its time step from <problem> */
+int pgn_scene_solver(const pgn_scene* scene, pgn_newton_opts* opts, int* dynamic,
+                     double* dt);
+/* pgn_model_create_tet + pgn_model_set_fixed / _set_gravity (+ set_x when
+ * meshInit differs from meshRest) */
+int pgn_scene_create_model(pgn_ctx* ctx, const pgn_scene* scene, int precision,
+                           pgn_model** out);
+
 /* ---- discrete elastic rods (Simulable_DER) ------------------------------ */
 /* A batch of open rods solved together (every rod keeps its own Newton
  * state).  Reference DoF layout per rod: 3n positions then n-1 edge angles

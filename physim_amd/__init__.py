@@ -54,6 +54,9 @@ EXPORTS = [
     "pgn_rods_get_gradient", "pgn_rods_get_hessian_dense", "pgn_rods_get_frames",
     "pgn_rods_set_frames", "pgn_rods_update_kinematics", "pgn_rods_element_bt",
     "pgn_rods_newton", "pgn_csr_linear_solve", "pgn_csr_linear_solve_noalloc",
+    "pgn_tetgen_sizes", "pgn_tetgen_read", "pgn_scene_load", "pgn_scene_destroy",
+    "pgn_scene_sizes", "pgn_scene_mesh", "pgn_scene_boundary", "pgn_scene_solver",
+    "pgn_scene_create_model",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -159,6 +162,17 @@ def lib():
         L.pgn_csr_linear_solve_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
                                                    ctypes.POINTER(LinearOpts),
                                                    ctypes.POINTER(LinearResult)]
+        _cp = ctypes.c_char_p
+        L.pgn_tetgen_sizes.argtypes = [_cp, _ip, _ip]
+        L.pgn_tetgen_read.argtypes = [_cp, _dp, _ip]
+        L.pgn_scene_load.argtypes = [_cp, ctypes.POINTER(_vp)]
+        L.pgn_scene_destroy.argtypes = [_vp]
+        L.pgn_scene_destroy.restype = None
+        L.pgn_scene_sizes.argtypes = [_vp, _ip, _ip]
+        L.pgn_scene_mesh.argtypes = [_vp, _dp, _dp, _ip, _ucp, _dp, _dp, _dp]
+        L.pgn_scene_boundary.argtypes = [_vp, _ucp, _dp, _dp, _ip, _ip]
+        L.pgn_scene_solver.argtypes = [_vp, ctypes.POINTER(NewtonOpts), _ip, _dp]
+        L.pgn_scene_create_model.argtypes = [_vp, _vp, ctypes.c_int, ctypes.POINTER(_vp)]
         L.pgn_newton_default_opts.argtypes = [ctypes.POINTER(NewtonOpts)]
         L.pgn_newton_default_opts.restype = None
         L.pgn_newton_solve.argtypes = [_vp, ctypes.POINTER(NewtonOpts),
@@ -286,6 +300,68 @@ def csr_linear_solve_noalloc(ctx, n, rowptr_ptr, col_ptr, val_ptr, b_ptr, x_ptr,
     return _linear_result(r)
 
 
+def read_tetgen(base):
+    """TetGen base.node / base.ele -> (X (n, 3), T (m, 4)) (pgn_tetgen_read)."""
+    b = str(base).encode()
+    n, m = ctypes.c_int(), ctypes.c_int()
+    _ok(lib().pgn_tetgen_sizes(b, ctypes.byref(n), ctypes.byref(m)))
+    X = np.zeros((n.value, 3))
+    T = np.zeros((m.value, 4), dtype=np.int32)
+    _ok(lib().pgn_tetgen_read(b, _p(X), _p(T, _ip)))
+    return X, T
+
+
+class Scene:
+    """A scene document in the reference's XML vocabulary (pgn_scene_load):
+    mesh, materials, boundary conditions, problem and solver options."""
+
+    def __init__(self, path):
+        h = _vp()
+        _ok(lib().pgn_scene_load(str(path).encode(), ctypes.byref(h)))
+        self._h = h
+        n, m = ctypes.c_int(), ctypes.c_int()
+        _ok(lib().pgn_scene_sizes(h, ctypes.byref(n), ctypes.byref(m)))
+        self.n, self.m = n.value, m.value
+        self.X0 = np.zeros((self.n, 3))
+        self.X = np.zeros((self.n, 3))
+        self.T = np.zeros((self.m, 4), dtype=np.int32)
+        self.material = np.zeros(self.m, dtype=np.uint8)
+        self.lame1 = np.zeros(self.m)
+        self.lame2 = np.zeros(self.m)
+        self.density = np.zeros(self.m)
+        _ok(lib().pgn_scene_mesh(h, _p(self.X0), _p(self.X), _p(self.T, _ip),
+                                 _p(self.material, _ucp), _p(self.lame1), _p(self.lame2),
+                                 _p(self.density)))
+        self.fixed = np.zeros((self.n, 3), dtype=np.uint8)
+        gi, ge = np.zeros(3), np.zeros(3)
+        st, hg = ctypes.c_int(), ctypes.c_int()
+        _ok(lib().pgn_scene_boundary(h, _p(self.fixed, _ucp), _p(gi), _p(ge), ctypes.byref(st),
+                                     ctypes.byref(hg)))
+        self.gravity = ge if hg.value else None
+        self.gravity_ini = gi
+        self.stages = st.value
+        o = NewtonOpts()
+        dyn, dt = ctypes.c_int(), ctypes.c_double()
+        _ok(lib().pgn_scene_solver(h, ctypes.byref(o), ctypes.byref(dyn), ctypes.byref(dt)))
+        self.opts = {k: getattr(o, k) for k, _ in NewtonOpts._fields_}
+        self.dynamic, self.dt = bool(dyn.value), dt.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pgn_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def create_model(self, ctx, precision=MIXED):
+        """pgn_scene_create_model: a TetModel with the scene's boundary
+        conditions applied."""
+        h = _vp()
+        _ok(lib().pgn_scene_create_model(ctx.handle, self._h, precision, ctypes.byref(h)))
+        return TetModel._adopt_handle(ctx, h, self.n, self.m, precision)
+
+
 class PgnError(RuntimeError):
     pass
 
@@ -360,6 +436,17 @@ class TetModel:
                                        precision, ctypes.byref(h)))
         self._h = h
         ctx._adopt(self)
+
+    @classmethod
+    def _adopt_handle(cls, ctx, h, n, m, precision):
+        """Wraps a model the library created (e.g. pgn_scene_create_model)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.n, self.m, self.N = n, m, 3 * n
+        self.precision = precision
+        self._h = h
+        ctx._adopt(self)
+        return self
 
     def close(self):
         if getattr(self, "_h", None):
