@@ -135,6 +135,12 @@ def main():
                          "serving and dynamic workloads (0 = the workload's default)")
     ap.add_argument("--rate", type=float, default=100.0,
                     help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--rates", default="",
+                    help="serving: comma-separated arrival rates, one JSON line each "
+                         "(e.g. 50,100,200,500,1000,2000)")
+    ap.add_argument("--lanes", type=int, default=4, help="serving: request-loop lanes")
+    ap.add_argument("--cohorts", type=int, default=4,
+                    help="serving: fused batches resident per lane")
     ap.add_argument("--max-tets", type=int, default=500_000,
                     help="serving: largest beam request (tets)")
     ap.add_argument("--max-cloth", type=int, default=512,
@@ -156,11 +162,12 @@ def main():
 
     if args.workload != "beam":
         from physim_amd import workloads
-        line = getattr(workloads, args.workload)(args, log)
-        line.update(metric=METRIC, n_gpus=1, steps=line.get("steps", args.steps),
-                    warmup=line.get("warmup", args.warmup), higher_is_better=True,
-                    vs_baseline=None, data="synthetic")
-        print(json.dumps(line), flush=True)
+        out = getattr(workloads, args.workload)(args, log)
+        for line in out if isinstance(out, list) else [out]:
+            line.update(metric=METRIC, n_gpus=1, steps=line.get("steps", args.steps),
+                        warmup=line.get("warmup", args.warmup), higher_is_better=True,
+                        vs_baseline=None, data="synthetic")
+            print(json.dumps(line), flush=True)
         return
 
     t0 = time.perf_counter()

@@ -377,18 +377,20 @@ int pgn_batch_get_x(pgn_batch* batch, int instance, double* x);
 int pgn_batch_get_energy(pgn_batch* batch, double* energy);
 
 /* ---- request loop (serving) ---------------------------------------------- */
-/* A persistent loop that solves independent requests on one GPU.  Requests
- * are copied at submit and queued; each lane (a worker thread with its own
- * HIP stream) admits queued requests while their estimated device footprint
- * fits hbm_budget_bytes, solves them as one fused batch (pgn_batch
- * semantics: every request keeps its own Newton state) and frees their
- * memory.  A fused batch finishes with its slowest request, so by default
- * (group_by_size) a lane starts its batch with the cheapest queued request
- * and adds only requests of at most twice its footprint: small requests are
- * not held up by large ones.  Requests arriving while batches run are taken
- * by the next free lane, so solves overlap on the device.  No reference
- * counterpart: PhySim solves one Simulable per OptimSolver call
- * (Simulator.cpp:38-72). */
+/* A persistent loop that solves independent requests on one GPU with
+ * continuous batching.  Requests are copied at submit and queued.  Each lane
+ * (a worker thread with its own HIP stream) keeps up to cohorts_per_lane
+ * fused batches resident (pgn_batch semantics: one launch per kernel covers
+ * a batch, every request keeps its own Newton state) and advances them one
+ * Newton iteration at a time; between iterations it admits queued requests
+ * as a new batch while their estimated device footprint fits
+ * hbm_budget_bytes (by default, group_by_size, the cheapest queued request
+ * and requests of at most twice its footprint), hands each request back the
+ * moment its own solve ends, and frees a batch's memory when its last
+ * request has ended.  A request arriving while others run starts within one
+ * Newton iteration of a lane; solves overlap on the device across lanes.
+ * No reference counterpart: PhySim solves one Simulable per OptimSolver
+ * call (Simulator.cpp:38-72). */
 typedef struct {
   size_t hbm_budget_bytes; /* device memory the resident requests may hold */
   int lanes;               /* concurrent batches (streams), 1..16          */
@@ -399,11 +401,15 @@ typedef struct {
                               cheapest first; 0: arrival order              */
   double max_wait_s;       /* group_by_size: a request queued this long is
                               taken before cheaper ones (default 2 s)       */
+  int cohorts_per_lane;    /* fused batches a lane keeps resident at once
+                              (default 4; 1: a lane's batch ends before the
+                              lane admits more)                             */
 } pgn_server_opts;
 typedef struct pgn_server pgn_server;
 void pgn_server_default_opts(pgn_server_opts* o);
 int pgn_server_create(int device, const pgn_server_opts* opts, pgn_server** out);
-/* stops the lanes after their current batch; queued requests are dropped */
+/* stops the lanes after their current Newton iteration; requests not yet
+ * solved end with PGN_ERR_INVALID ("server destroyed"), their waiters wake */
 void pgn_server_destroy(pgn_server* server);
 /* submit a tet or thin-shell request (mesh arguments as pgn_model_create_*);
  * fixed: 3 per node or NULL; gravity: 3 or NULL; forces: 3 per node or NULL;
@@ -423,7 +429,8 @@ int pgn_server_submit_shell(pgn_server* server, int n_nodes, const double* X0,
                             long* ticket);
 int pgn_server_poll(pgn_server* server, long ticket, int* done);
 /* blocks until the request is solved; x: 3 per node (or NULL); latency_s:
- * submit -> result on host; queue_s: submit -> its batch started */
+ * submit -> result on host; queue_s: submit -> its batch started;
+ * res->seconds: its batch start -> its own solve's end */
 int pgn_server_wait(pgn_server* server, long ticket, double* x, pgn_newton_result* res,
                     double* latency_s, double* queue_s);
 int pgn_server_stats(pgn_server* server, long* batches, long* solved,

@@ -96,7 +96,7 @@ class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("lanes", ctypes.c_int),
                 ("max_batch", ctypes.c_int), ("precision", ctypes.c_int),
                 ("newton", NewtonOpts), ("group_by_size", ctypes.c_int),
-                ("max_wait_s", ctypes.c_double)]
+                ("max_wait_s", ctypes.c_double), ("cohorts_per_lane", ctypes.c_int)]
 
 
 class NewtonResult(ctypes.Structure):
@@ -732,12 +732,16 @@ class Batch:
 
 class Server:
     """The request loop (pgn_server): submit scenes from any thread, wait for
-    their results.  Lanes run fused batches on their own streams; admission
-    is by estimated device footprint against hbm_budget_bytes, batches of
-    similar footprint cheapest first (group_by_size) or in arrival order."""
+    their results.  Continuous batching: each lane keeps up to
+    cohorts_per_lane fused batches resident on its own stream and advances
+    them one Newton iteration at a time; between iterations it admits queued
+    requests (by estimated device footprint against hbm_budget_bytes, similar
+    footprints cheapest first under group_by_size, else arrival order) and
+    hands back each request as soon as its own solve ends."""
 
     def __init__(self, device=0, hbm_budget_bytes=64 << 30, lanes=4, max_batch=64,
-                 precision=MIXED, group_by_size=1, max_wait_s=2.0, **newton):
+                 precision=MIXED, group_by_size=1, max_wait_s=2.0, cohorts_per_lane=4,
+                 **newton):
         o = ServerOpts()
         lib().pgn_server_default_opts(ctypes.byref(o))
         o.hbm_budget_bytes = int(hbm_budget_bytes)
@@ -746,6 +750,7 @@ class Server:
         o.precision = precision
         o.group_by_size = int(group_by_size)
         o.max_wait_s = float(max_wait_s)
+        o.cohorts_per_lane = int(cohorts_per_lane)
         o.newton = _newton_opts(newton)
         h = _vp()
         _ok(lib().pgn_server_create(device, ctypes.byref(o), ctypes.byref(h)))

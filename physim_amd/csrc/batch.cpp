@@ -478,251 +478,345 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
   }
 }
 
-void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
+// One SolveFull run over the batch (OptimSolver.cpp:163-264 per instance),
+// resumable one Newton iteration at a time so that a caller can hand back
+// the instances that finish while the others go on.
+struct pgn_batch::NewtonRun {
+  pgn_newton_opts o{};
+  bool direct = false;
+  pgn_linear_opts lo{};
+  std::chrono::steady_clock::time_point t0;
+  std::vector<int> st, result, minImpr, minStep, steep, its, bis, evals, hevals, pcg, nbis,
+      searching, guarded;
+  std::vector<uint8_t> hmode, hmask;
+  std::vector<double> obj, opt, optStart, objNew, optNew, f, len, clamp, stepLen, hfac, eta,
+      gprev, hn2, secs;
+  // per instance: fixed lsTolMaxError after an inexact step failed its line
+  // search (the retry rule of the single-model solver); inexact: this step's
+  // linear solve was looser than lsTolMaxError
+  std::vector<uint8_t> ew_off, inexact;
+  std::vector<double> E;
+};
+
+pgn_batch::pgn_batch() = default;
+pgn_batch::~pgn_batch() = default;
+
+void pgn_batch::newton_begin(const pgn_newton_opts& o) {
   PGN_CHECK(o.qpSolverType == PGN_QP_NEWTON, "fused batches take Newton steps (PGN_QP_NEWTON)");
   PGN_CHECK(o.lsSolverType == PGN_LS_EIGENCG || o.lsSolverType == PGN_LS_EIGENLDLT,
             "lsSolverType: PGN_LS_EIGENCG or PGN_LS_EIGENLDLT");
   const bool direct = o.lsSolverType == PGN_LS_EIGENLDLT;
   if (direct) ldlt_prepare();  // an instance beyond the band limit fails before any step
-  const auto t0 = std::chrono::steady_clock::now();
+  run = std::make_unique<NewtonRun>();
+  NewtonRun& r = *run;
+  r.o = o;
+  r.direct = direct;
+  r.t0 = std::chrono::steady_clock::now();
+  hipStream_t s = ctx->stream;
+  const int Kn = K();
+  const long N = 3L * M->n;
+  pgn_linear_default_opts(&r.lo);
+  r.lo.tol = o.lsTolMaxError;
+  r.lo.max_iters = o.lsNumMaxIters;
+  r.lo.precond = o.precond;
+  r.lo.inner_tol = o.inner_tol;
+  r.lo.solver = o.lsSolverType;
+  for (auto* v : {&r.st, &r.minImpr, &r.minStep, &r.steep, &r.its, &r.bis, &r.evals, &r.hevals,
+                  &r.pcg, &r.nbis, &r.searching, &r.guarded})
+    v->assign(Kn, 0);
+  r.result.assign(Kn, PGN_OS_SUCCESS);
+  r.hmode.assign(Kn, 0);
+  r.hmask.assign(Kn, 0);
+  for (auto* v : {&r.obj, &r.opt, &r.optStart, &r.objNew, &r.optNew, &r.f, &r.len, &r.clamp,
+                  &r.stepLen, &r.hfac, &r.hn2, &r.secs})
+    v->assign(Kn, 0.0);
+  r.eta.assign(Kn, o.ew_max);
+  r.gprev.assign(Kn, -1.0);
+  r.ew_off.assign(Kn, 0);
+  r.inexact.assign(Kn, 0);
+  // OptimSolver::Init (OptimSolver.cpp:36-55)
+  PGN_HIP(hipMemcpyAsync(v0.p, M->x.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  upload_loads();
+  eval_eg(r.E);
+  for (int k = 0; k < Kn; ++k) {
+    r.obj[k] = r.E[4 * k];
+    r.opt[k] = std::sqrt(r.E[4 * k + 1]);
+    r.evals[k]++;
+  }
+}
+
+bool pgn_batch::newton_active() const {
+  if (!run) return false;
+  for (int v : run->st)
+    if (v == 0) return true;
+  return false;
+}
+
+void pgn_batch::newton_result(int k, pgn_newton_result& R) const {
+  PGN_CHECK(run && k >= 0 && k < K(), "no Newton run / instance out of range");
+  const NewtonRun& r = *run;
+  R = pgn_newton_result{};
+  R.result = r.result[k];
+  R.iterations = r.its[k];
+  R.pcg_iterations = r.pcg[k];
+  R.line_search_bisections = r.bis[k];
+  R.energy_evals = r.evals[k];
+  R.hessian_evals = r.hevals[k];
+  R.objective = r.obj[k];
+  R.optimality = r.opt[k];
+  R.seconds = r.secs[k];
+}
+
+std::vector<int> pgn_batch::newton_step() {
+  std::vector<int> finished;
+  PGN_CHECK(run, "newton_step before newton_begin");
+  NewtonRun& r = *run;
+  const pgn_newton_opts& o = r.o;
+  const bool direct = r.direct;
+  const pgn_linear_opts& lo = r.lo;
   hipStream_t s = ctx->stream;
   const int Kn = K();
   const long n = M->n, N = 3 * n;
   const BatchArgs ba = args();
-  pgn_linear_opts lo;
-  pgn_linear_default_opts(&lo);
-  lo.tol = o.lsTolMaxError;
-  lo.max_iters = o.lsNumMaxIters;
-  lo.precond = o.precond;
-  lo.inner_tol = o.inner_tol;
-  lo.solver = o.lsSolverType;
   enum { ACTIVE = 0, DONE = 1 };
-  std::vector<int> st(Kn, ACTIVE), result(Kn, PGN_OS_SUCCESS), minImpr(Kn, 0), minStep(Kn, 0),
-      steep(Kn, 0), its(Kn, 0), bis(Kn, 0), evals(Kn, 0), hevals(Kn, 0), pcg(Kn, 0),
-      nbis(Kn, 0), searching(Kn, 0), guarded(Kn, 0);
-  std::vector<uint8_t> hmode(Kn, 0), hmask(Kn, 0);
-  std::vector<double> obj(Kn), opt(Kn), optStart(Kn), objNew(Kn), optNew(Kn), f(Kn), len(Kn),
-      clamp(Kn), stepLen(Kn), hfac(Kn), eta(Kn, o.ew_max), gprev(Kn, -1.0), hn2(Kn);
-  // per instance: fixed lsTolMaxError after an inexact step failed its line
-  // search (the retry rule of the single-model solver); inexact: this step's
-  // linear solve was looser than lsTolMaxError
-  std::vector<uint8_t> ew_off(Kn, 0), inexact(Kn, 0);
-  std::vector<double> E;
+  auto& st = r.st;
+  auto& result = r.result;
+  auto& minImpr = r.minImpr;
+  auto& minStep = r.minStep;
+  auto& steep = r.steep;
+  auto& its = r.its;
+  auto& bis = r.bis;
+  auto& evals = r.evals;
+  auto& hevals = r.hevals;
+  auto& pcg = r.pcg;
+  auto& nbis = r.nbis;
+  auto& searching = r.searching;
+  auto& guarded = r.guarded;
+  auto& hmode = r.hmode;
+  auto& hmask = r.hmask;
+  auto& obj = r.obj;
+  auto& opt = r.opt;
+  auto& optStart = r.optStart;
+  auto& objNew = r.objNew;
+  auto& optNew = r.optNew;
+  auto& f = r.f;
+  auto& len = r.len;
+  auto& clamp = r.clamp;
+  auto& stepLen = r.stepLen;
+  auto& hfac = r.hfac;
+  auto& eta = r.eta;
+  auto& gprev = r.gprev;
+  auto& hn2 = r.hn2;
+  auto& ew_off = r.ew_off;
+  auto& inexact = r.inexact;
+  auto& E = r.E;
   auto d2d = [&](double* dst, const double* src) {
     PGN_HIP(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyDeviceToDevice, s));
   };
-  // OptimSolver::Init (OptimSolver.cpp:36-55)
-  PGN_HIP(hipMemcpyAsync(v0.p, M->x.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+  bool any = false;
+  for (int k = 0; k < Kn; ++k) any |= (st[k] == ACTIVE);
+  if (!any) return finished;
+  // OnIterStart: StepBoundaryConditions of every active instance
+  for (int k = 0; k < Kn; ++k)
+    if (st[k] == ACTIVE) step_load(k);
   upload_loads();
+  d2d(xsave.p, M->x.p);
   eval_eg(E);
+  bool need_h = false;
+  std::vector<double> tol_lin(Kn, o.lsTolMaxError);
   for (int k = 0; k < Kn; ++k) {
-    obj[k] = E[4 * k];
-    opt[k] = std::sqrt(E[4 * k + 1]);
+    hmode[k] = 0;
+    if (st[k] != ACTIVE) continue;
+    its[k]++;
     evals[k]++;
+    obj[k] = E[4 * k];
+    opt[k] = optStart[k] = std::sqrt(E[4 * k + 1]);
+    if (opt[k] < o.tolMaxError) {
+      result[k] = PGN_OS_SUCCESS;
+      continue;
+    }
+    hmode[k] = steep[k] ? 2 : 1;
+    inexact[k] = 0;
+    if (hmode[k] == 1) {
+      need_h = true;
+      hevals[k]++;
+      double lt = o.lsTolMaxError;
+      if (o.adaptive_tol && !ew_off[k] && !direct) {  // Eisenstat-Walker choice 2
+        if (gprev[k] > 0.0) {
+          const double r = opt[k] / gprev[k];
+          double e = o.ew_gamma * r * r;
+          const double guard = o.ew_gamma * eta[k] * eta[k];
+          if (guard > 0.1) e = std::max(e, guard);
+          eta[k] = e;
+        }
+        // Kelley's safeguard: no tighter than the absolute tolerance needs
+        eta[k] = std::min(o.ew_max, std::max(eta[k], 0.5 * o.tolMaxError / opt[k]));
+        gprev[k] = opt[k];
+        lt = std::max(o.lsTolMaxError, eta[k]);
+      }
+      tol_lin[k] = lt;
+      inexact[k] = lt > o.lsTolMaxError;
+    }
   }
-  for (;;) {
-    bool any = false;
-    for (int k = 0; k < Kn; ++k) any |= (st[k] == ACTIVE);
-    if (!any) break;
-    // OnIterStart: StepBoundaryConditions of every active instance
+  mode.upload(hmode, s);
+  if (need_h) M->eval_hessian();
+  launch_batch_rhs(s, ba, n, mode.p, M->g.p, xsave.p, v0.p, bvec.p, dx.p);
+  if (need_h) {
+    std::vector<uint8_t> act(Kn);
+    for (int k = 0; k < Kn; ++k) act[k] = hmode[k] == 1;
+    std::vector<int> its_lin;
+    linear_solve(bvec.p, dx.p, act, tol_lin, lo, its_lin);
     for (int k = 0; k < Kn; ++k)
-      if (st[k] == ACTIVE) step_load(k);
-    upload_loads();
-    d2d(xsave.p, M->x.p);
-    eval_eg(E);
-    bool need_h = false;
-    std::vector<double> tol_lin(Kn, o.lsTolMaxError);
+      if (hmode[k] == 1) pcg[k] += its_lin[k];
+  }
+  launch_batch_norm2(s, ba, dx.p, n2.p);
+  n2.download(hn2.data(), Kn, s);
+  PGN_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < Kn; ++k) {
+    searching[k] = 0;
+    if (!hmode[k]) continue;
+    len[k] = std::sqrt(hn2[k]);
+    clamp[k] = 1.0;
+    if (len[k] > o.maxStepSize) {  // OptimSolver.cpp:346
+      clamp[k] = o.maxStepSize / len[k];
+      len[k] = o.maxStepSize;
+    }
+    f[k] = 1.0;
+    nbis[k] = -1;
+    searching[k] = 1;
+    guarded[k] = 0;
+  }
+  // LineSearch_Simple (OptimSolver.cpp:688-736), instances in lockstep
+  for (;;) {
+    bool anys = false;
     for (int k = 0; k < Kn; ++k) {
-      hmode[k] = 0;
-      if (st[k] != ACTIVE) continue;
-      its[k]++;
+      hfac[k] = -1.0;
+      if (searching[k]) {
+        nbis[k]++;
+        stepLen[k] = f[k] * len[k];
+        hfac[k] = f[k] * clamp[k];
+        anys = true;
+      }
+    }
+    if (!anys) break;
+    fac.upload(hfac, s);
+    launch_batch_step(s, ba, n, fac.p, xsave.p, dx.p, M->x.p);
+    eval_eg(E);
+    bool restore = false;
+    for (int k = 0; k < Kn; ++k) {
+      hfac[k] = -1.0;
+      if (!searching[k]) continue;
       evals[k]++;
-      obj[k] = E[4 * k];
-      opt[k] = optStart[k] = std::sqrt(E[4 * k + 1]);
-      if (opt[k] < o.tolMaxError) {
-        result[k] = PGN_OS_SUCCESS;
+      const double en = E[4 * k], gn = std::sqrt(E[4 * k + 1]);
+      objNew[k] = en;
+      optNew[k] = gn;
+      if (en < obj[k]) {
+        searching[k] = 0;
         continue;
       }
-      hmode[k] = steep[k] ? 2 : 1;
-      inexact[k] = 0;
-      if (hmode[k] == 1) {
-        need_h = true;
-        hevals[k]++;
-        double lt = o.lsTolMaxError;
-        if (o.adaptive_tol && !ew_off[k] && !direct) {  // Eisenstat-Walker choice 2
-          if (gprev[k] > 0.0) {
-            const double r = opt[k] / gprev[k];
-            double e = o.ew_gamma * r * r;
-            const double guard = o.ew_gamma * eta[k] * eta[k];
-            if (guard > 0.1) e = std::max(e, guard);
-            eta[k] = e;
-          }
-          // Kelley's safeguard: no tighter than the absolute tolerance needs
-          eta[k] = std::min(o.ew_max, std::max(eta[k], 0.5 * o.tolMaxError / opt[k]));
-          gprev[k] = opt[k];
-          lt = std::max(o.lsTolMaxError, eta[k]);
-        }
-        tol_lin[k] = lt;
-        inexact[k] = lt > o.lsTolMaxError;
+      if (o.ls_noise_guard > 0.0 && en <= obj[k] + o.ls_noise_guard * E[4 * k + 2] &&
+          gn < opt[k]) {
+        guarded[k] = 1;  // booked as an improvement of zero
+        searching[k] = 0;
+        continue;
+      }
+      f[k] *= o.lSearch_alpha;
+      if (!(stepLen[k] > o.tolMinStep && nbis[k] <= o.lsearch_numTry)) {
+        searching[k] = 0;  // exhausted: back to the step start
+        hfac[k] = 0.0;
+        optNew[k] = optStart[k];
+        restore = true;
       }
     }
-    mode.upload(hmode, s);
-    if (need_h) M->eval_hessian();
-    launch_batch_rhs(s, ba, n, mode.p, M->g.p, xsave.p, v0.p, bvec.p, dx.p);
-    if (need_h) {
-      std::vector<uint8_t> act(Kn);
-      for (int k = 0; k < Kn; ++k) act[k] = hmode[k] == 1;
-      std::vector<int> its_lin;
-      linear_solve(bvec.p, dx.p, act, tol_lin, lo, its_lin);
-      for (int k = 0; k < Kn; ++k)
-        if (hmode[k] == 1) pcg[k] += its_lin[k];
-    }
-    launch_batch_norm2(s, ba, dx.p, n2.p);
-    n2.download(hn2.data(), Kn, s);
-    PGN_HIP(hipStreamSynchronize(s));
-    for (int k = 0; k < Kn; ++k) {
-      searching[k] = 0;
-      if (!hmode[k]) continue;
-      len[k] = std::sqrt(hn2[k]);
-      clamp[k] = 1.0;
-      if (len[k] > o.maxStepSize) {  // OptimSolver.cpp:346
-        clamp[k] = o.maxStepSize / len[k];
-        len[k] = o.maxStepSize;
-      }
-      f[k] = 1.0;
-      nbis[k] = -1;
-      searching[k] = 1;
-      guarded[k] = 0;
-    }
-    // LineSearch_Simple (OptimSolver.cpp:688-736), instances in lockstep
-    for (;;) {
-      bool anys = false;
-      for (int k = 0; k < Kn; ++k) {
-        hfac[k] = -1.0;
-        if (searching[k]) {
-          nbis[k]++;
-          stepLen[k] = f[k] * len[k];
-          hfac[k] = f[k] * clamp[k];
-          anys = true;
-        }
-      }
-      if (!anys) break;
+    if (restore) {
       fac.upload(hfac, s);
       launch_batch_step(s, ba, n, fac.p, xsave.p, dx.p, M->x.p);
-      eval_eg(E);
-      bool restore = false;
-      for (int k = 0; k < Kn; ++k) {
-        hfac[k] = -1.0;
-        if (!searching[k]) continue;
-        evals[k]++;
-        const double en = E[4 * k], gn = std::sqrt(E[4 * k + 1]);
-        objNew[k] = en;
-        optNew[k] = gn;
-        if (en < obj[k]) {
-          searching[k] = 0;
-          continue;
-        }
-        if (o.ls_noise_guard > 0.0 && en <= obj[k] + o.ls_noise_guard * E[4 * k + 2] &&
-            gn < opt[k]) {
-          guarded[k] = 1;  // booked as an improvement of zero
-          searching[k] = 0;
-          continue;
-        }
-        f[k] *= o.lSearch_alpha;
-        if (!(stepLen[k] > o.tolMinStep && nbis[k] <= o.lsearch_numTry)) {
-          searching[k] = 0;  // exhausted: back to the step start
-          hfac[k] = 0.0;
-          optNew[k] = optStart[k];
-          restore = true;
-        }
-      }
-      if (restore) {
-        fac.upload(hfac, s);
-        launch_batch_step(s, ba, n, fac.p, xsave.p, dx.p, M->x.p);
-      }
-    }
-    // UpdateStepResult (OptimSolver.cpp:402-466)
-    bool any_impr = false;
-    for (int k = 0; k < Kn; ++k) {
-      hmask[k] = 0;
-      if (!hmode[k]) continue;
-      bis[k] += nbis[k];
-      double impr = obj[k] - objNew[k];
-      if (guarded[k] && impr < 0) impr = 0;
-      if (impr > 0 || guarded[k]) {
-        if (impr < o.tolMinImpr) {
-          minImpr[k]++;
-          result[k] = PGN_OS_MINIMPR;
-        } else if (stepLen[k] < o.tolMinStep) {
-          minStep[k]++;
-          result[k] = PGN_OS_MINSTEP;
-        } else if (optNew[k] <= o.tolMaxError) {
-          result[k] = PGN_OS_SUCCESS;
-          minImpr[k] = minStep[k] = 0;
-        } else {
-          result[k] = PGN_OS_ONGOING;
-          minImpr[k] = minStep[k] = 0;
-        }
-        obj[k] = objNew[k];
-        opt[k] = optNew[k];
-        hmask[k] = 1;  // v0 := the step start (OptimSolver.cpp:488 warm start)
-        any_impr = true;
-      } else if (inexact[k]) {  // retry with the fixed tolerance
-        ew_off[k] = 1;
-        result[k] = PGN_OS_ONGOING;
-        opt[k] = optStart[k];
-      } else {
-        result[k] = PGN_OS_NONDESC;
-        opt[k] = optStart[k];
-      }
-    }
-    if (any_impr) {
-      mask.upload(hmask, s);
-      launch_batch_copy(s, ba, n, mask.p, xsave.p, v0.p);
-    }
-    // SolveFull status machine per instance
-    for (int k = 0; k < Kn; ++k) {
-      if (st[k] != ACTIVE) continue;
-      bool stop = false;
-      if (is_loaded(k)) {
-        int& res = result[k];
-        if (res == PGN_OS_NONDESC) {
-          if (!o.trySteepest || steep[k]) stop = true;
-          else steep[k] = 1;
-        }
-        if (!stop && res == PGN_OS_MINIMPR) {
-          if (minImpr[k] == o.minImprMaxCount) stop = true;
-          else if (o.trySteepest) steep[k] = !steep[k];
-        }
-        if (!stop && res == PGN_OS_MINSTEP) {
-          if (minStep[k] == o.minStepMaxCount) stop = true;
-          else if (o.trySteepest) steep[k] = !steep[k];
-        }
-        if (!stop && res == PGN_OS_SUCCESS) {
-          if (opt[k] <= o.tolMaxError) stop = true;
-          else res = PGN_OS_ONGOING;
-        }
-        if (!stop && res == PGN_OS_ONGOING) steep[k] = 0;
-      }
-      if (!stop && its[k] + 1 > o.numMaxIters) {
-        result[k] = PGN_OS_MAXITER;
-        stop = true;
-      }
-      if (stop) st[k] = DONE;
     }
   }
-  PGN_HIP(hipStreamSynchronize(s));
-  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  // UpdateStepResult (OptimSolver.cpp:402-466)
+  bool any_impr = false;
   for (int k = 0; k < Kn; ++k) {
-    R[k] = pgn_newton_result{};
-    R[k].result = result[k];
-    R[k].iterations = its[k];
-    R[k].pcg_iterations = pcg[k];
-    R[k].line_search_bisections = bis[k];
-    R[k].energy_evals = evals[k];
-    R[k].hessian_evals = hevals[k];
-    R[k].objective = obj[k];
-    R[k].optimality = opt[k];
+    hmask[k] = 0;
+    if (!hmode[k]) continue;
+    bis[k] += nbis[k];
+    double impr = obj[k] - objNew[k];
+    if (guarded[k] && impr < 0) impr = 0;
+    if (impr > 0 || guarded[k]) {
+      if (impr < o.tolMinImpr) {
+        minImpr[k]++;
+        result[k] = PGN_OS_MINIMPR;
+      } else if (stepLen[k] < o.tolMinStep) {
+        minStep[k]++;
+        result[k] = PGN_OS_MINSTEP;
+      } else if (optNew[k] <= o.tolMaxError) {
+        result[k] = PGN_OS_SUCCESS;
+        minImpr[k] = minStep[k] = 0;
+      } else {
+        result[k] = PGN_OS_ONGOING;
+        minImpr[k] = minStep[k] = 0;
+      }
+      obj[k] = objNew[k];
+      opt[k] = optNew[k];
+      hmask[k] = 1;  // v0 := the step start (OptimSolver.cpp:488 warm start)
+      any_impr = true;
+    } else if (inexact[k]) {  // retry with the fixed tolerance
+      ew_off[k] = 1;
+      result[k] = PGN_OS_ONGOING;
+      opt[k] = optStart[k];
+    } else {
+      result[k] = PGN_OS_NONDESC;
+      opt[k] = optStart[k];
+    }
+  }
+  if (any_impr) {
+    mask.upload(hmask, s);
+    launch_batch_copy(s, ba, n, mask.p, xsave.p, v0.p);
+  }
+  // SolveFull status machine per instance
+  for (int k = 0; k < Kn; ++k) {
+    if (st[k] != ACTIVE) continue;
+    bool stop = false;
+    if (is_loaded(k)) {
+      int& res = result[k];
+      if (res == PGN_OS_NONDESC) {
+        if (!o.trySteepest || steep[k]) stop = true;
+        else steep[k] = 1;
+      }
+      if (!stop && res == PGN_OS_MINIMPR) {
+        if (minImpr[k] == o.minImprMaxCount) stop = true;
+        else if (o.trySteepest) steep[k] = !steep[k];
+      }
+      if (!stop && res == PGN_OS_MINSTEP) {
+        if (minStep[k] == o.minStepMaxCount) stop = true;
+        else if (o.trySteepest) steep[k] = !steep[k];
+      }
+      if (!stop && res == PGN_OS_SUCCESS) {
+        if (opt[k] <= o.tolMaxError) stop = true;
+        else res = PGN_OS_ONGOING;
+      }
+      if (!stop && res == PGN_OS_ONGOING) steep[k] = 0;
+    }
+    if (!stop && its[k] + 1 > o.numMaxIters) {
+      result[k] = PGN_OS_MAXITER;
+      stop = true;
+    }
+    if (stop) {
+      st[k] = DONE;
+      r.secs[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - r.t0).count();
+      finished.push_back(k);
+    }
+  }
+  return finished;
+}
+
+void pgn_batch::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
+  newton_begin(o);
+  while (newton_active()) newton_step();
+  PGN_HIP(hipStreamSynchronize(ctx->stream));
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - run->t0).count();
+  for (int k = 0; k < K(); ++k) {
+    newton_result(k, R[k]);
     R[k].seconds = secs;
   }
 }

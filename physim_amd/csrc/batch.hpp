@@ -75,6 +75,18 @@ struct pgn_batch {
                     const std::vector<double>& tol, const pgn_linear_opts& o,
                     std::vector<int>& iters);
   void newton(const pgn_newton_opts& o, pgn_newton_result* R);
+  // The same run one Newton iteration at a time: newton_begin starts it
+  // (OptimSolver::Init), newton_step advances every active instance by one
+  // iteration and returns the instances that finished in it, whose results
+  // (newton_result) and positions are then final while the others go on.
+  struct NewtonRun;
+  std::unique_ptr<NewtonRun> run;
+  void newton_begin(const pgn_newton_opts& o);
+  std::vector<int> newton_step();
+  bool newton_active() const;
+  void newton_result(int k, pgn_newton_result& r) const;
+  pgn_batch();
+  ~pgn_batch();
 
   // EigenLDLT: one banded LDL^T per instance, one workgroup each (band
   // orders planned on first use)

@@ -1,20 +1,25 @@
 // server.cpp -- request loop for many independent solves on one GPU
-// (pgn_server_*).
+// (pgn_server_*), with continuous batching.
 //
 // Requests (tet meshes and thin shells with their boundary conditions) are
 // copied into host staging at submit time and queued.  Each lane is a worker
-// thread with its own HIP stream; a free lane admits queued requests while
-// their estimated device footprint fits the server's HBM budget (and at most
-// max_batch of them), builds them into one fused batch (pgn_batch: one launch
-// per kernel covers all of them, every request keeps its own Newton state),
-// solves, hands the results back and releases the memory.  A fused batch ends
-// with its slowest request, so with group_by_size a lane starts from the
-// cheapest queued request (the oldest once it has waited max_wait_s) and
-// adds only requests of at most twice that footprint, cheapest first;
-// otherwise it takes requests in arrival order.  Requests that arrive while batches run are taken by the next
-// free lane, so solves overlap on the device across lanes, and each result
-// is the one the request would get alone (per-instance Newton, block-
-// diagonal linear solve).
+// thread with its own HIP stream that keeps up to cohorts_per_lane fused
+// batches ("cohorts": pgn_batch, one launch per kernel covers all of their
+// requests, every request keeps its own Newton state) resident and advances
+// them one Newton iteration at a time.  Between iterations it
+//   - admits queued requests as a new cohort when their estimated device
+//     footprint fits the server's HBM budget (at most max_batch of them):
+//     with group_by_size the cheapest queued request (the oldest once it has
+//     waited max_wait_s) and requests of at most twice its footprint,
+//     otherwise arrival order;
+//   - hands every request back the moment its own solve ends (its positions
+//     downloaded, its waiter woken), while its cohort-mates go on;
+//   - releases a cohort's memory when its last request has ended.
+// So a request that arrives while others are on the GPU starts within one
+// Newton iteration of some lane, and finished requests never wait for
+// their batch-mates.  Lanes overlap on the device through their streams.
+// Each result is the one the request gets alone (per-instance Newton,
+// block-diagonal linear solve).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -67,6 +72,8 @@ struct pgn_server {
   // index in `pending` of the request a free lane starts with, or -1 when
   // none may start now (caller holds mu)
   long pick_head() const;
+  std::vector<std::shared_ptr<Request>> admit();
+  void finish(const std::shared_ptr<Request>& r, int code, const std::string& err);
 };
 
 long pgn_server::pick_head() const {
@@ -82,104 +89,186 @@ long pgn_server::pick_head() const {
   return fits(best) ? best : -1;
 }
 
+// Takes the requests of a new cohort from `pending` (caller holds mu);
+// empty when none may start now.
+std::vector<std::shared_ptr<Request>> pgn_server::admit() {
+  std::vector<std::shared_ptr<Request>> take;
+  const long head = pick_head();
+  if (head < 0) return take;
+  if (!opts.group_by_size) {
+    // arrival order while the footprint fits
+    while (!pending.empty() && (int)take.size() < opts.max_batch &&
+           used + pending.front()->bytes <= opts.hbm_budget_bytes) {
+      used += pending.front()->bytes;
+      take.push_back(pending.front());
+      pending.pop_front();
+    }
+    return take;
+  }
+  // the head, then requests of at most twice its footprint, cheapest first
+  // (arrival order among equals)
+  const size_t cap = 2 * pending[head]->bytes;
+  std::vector<long> order(pending.size());
+  for (long i = 0; i < (long)order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](long a, long b) { return pending[a]->bytes < pending[b]->bytes; });
+  std::vector<char> taken(pending.size(), 0);
+  used += pending[head]->bytes;
+  take.push_back(pending[head]);
+  taken[head] = 1;
+  for (long i : order) {
+    if ((int)take.size() >= opts.max_batch) break;
+    if (taken[i] || pending[i]->bytes > cap) continue;
+    if (used + pending[i]->bytes > opts.hbm_budget_bytes) break;
+    used += pending[i]->bytes;
+    take.push_back(pending[i]);
+    taken[i] = 1;
+  }
+  std::deque<std::shared_ptr<Request>> rest;
+  for (long i = 0; i < (long)pending.size(); ++i)
+    if (!taken[i]) rest.push_back(pending[i]);
+  pending.swap(rest);
+  return take;
+}
+
+namespace {
+// a fused batch resident on a lane
+struct Cohort {
+  std::unique_ptr<pgn_batch> B;
+  std::vector<std::shared_ptr<Request>> req;
+  std::vector<char> ended;
+  size_t bytes = 0;
+};
+}  // namespace
+
+void pgn_server::finish(const std::shared_ptr<Request>& r, int code, const std::string& err) {
+  // caller holds mu
+  r->status = code;
+  r->error = err;
+  r->t_done = Clock::now();
+  r->done = true;
+  r->inst = pgn_batch_inst{};  // free the staged inputs
+  if (code == PGN_OK) ++solved;
+}
+
 void pgn_server::lane_main(int lane) {
   pgn_ctx* ctx = ctxs[lane];
+  std::vector<Cohort> cohorts;
+  auto fail_cohort = [&](Cohort& c, int code, const std::string& err) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (size_t k = 0; k < c.req.size(); ++k)
+      if (!c.ended[k]) {
+        c.ended[k] = 1;
+        finish(c.req[k], code, err);
+      }
+    used -= c.bytes;
+    c.bytes = 0;
+  };
   for (;;) {
+    // ---- admission (between Newton iterations) ----
     std::vector<std::shared_ptr<Request>> take;
     {
       std::unique_lock<std::mutex> lk(mu);
-      long head = -1;
-      cv_work.wait_for(lk, std::chrono::milliseconds(50), [&] {
-        if (stop) return true;
-        head = pick_head();
-        return head >= 0;
-      });
-      if (stop) return;
-      if (head < 0) continue;  // re-check the aging bound
-      if (!opts.group_by_size) {
-        // arrival order while the footprint fits
-        while (!pending.empty() && (int)take.size() < opts.max_batch &&
-               used + pending.front()->bytes <= opts.hbm_budget_bytes) {
-          used += pending.front()->bytes;
-          take.push_back(pending.front());
-          pending.pop_front();
-        }
-      } else {
-        // the head, then requests of at most twice its footprint, cheapest
-        // first (arrival order among equals)
-        const size_t cap = 2 * pending[head]->bytes;
-        std::vector<long> order(pending.size());
-        for (long i = 0; i < (long)order.size(); ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](long a, long b) {
-          return pending[a]->bytes < pending[b]->bytes;
+      if (cohorts.empty())
+        cv_work.wait_for(lk, std::chrono::milliseconds(50), [&] {
+          return stop || pick_head() >= 0;
         });
-        std::vector<char> taken(pending.size(), 0);
-        used += pending[head]->bytes;
-        take.push_back(pending[head]);
-        taken[head] = 1;
-        for (long i : order) {
-          if ((int)take.size() >= opts.max_batch) break;
-          if (taken[i] || pending[i]->bytes > cap) continue;
-          if (used + pending[i]->bytes > opts.hbm_budget_bytes) break;
-          used += pending[i]->bytes;
-          take.push_back(pending[i]);
-          taken[i] = 1;
+      if (stop) {
+        lk.unlock();
+        for (auto& c : cohorts) fail_cohort(c, PGN_ERR_INVALID, "server destroyed during the solve");
+        cv_done.notify_all();
+        return;
+      }
+      if ((int)cohorts.size() < opts.cohorts_per_lane) {
+        take = admit();
+        if (!take.empty()) ++batches;
+      }
+    }
+    if (!take.empty()) {
+      Cohort c;
+      c.req = take;
+      c.ended.assign(take.size(), 0);
+      for (auto& r : take) c.bytes += r->bytes;
+      const auto t_start = Clock::now();
+      try {
+        PGN_HIP(hipSetDevice(device));
+        c.B = std::make_unique<pgn_batch>();
+        c.B->ctx = ctx;
+        c.B->prec = opts.precision;
+        for (auto& r : take) {
+          r->t_start = t_start;
+          c.B->inst.push_back(r->inst);
         }
-        std::deque<std::shared_ptr<Request>> rest;
-        for (long i = 0; i < (long)pending.size(); ++i)
-          if (!taken[i]) rest.push_back(pending[i]);
-        pending.swap(rest);
-      }
-      ++batches;
-    }
-    const auto t_start = Clock::now();
-    std::string err;
-    int code = PGN_OK;
-    std::vector<pgn_newton_result> res(take.size());
-    std::vector<std::vector<double>> xs(take.size());
-    try {
-      PGN_HIP(hipSetDevice(device));
-      pgn_batch B;
-      B.ctx = ctx;
-      B.prec = opts.precision;
-      for (auto& r : take) {
-        r->t_start = t_start;
-        B.inst.push_back(r->inst);
-      }
-      B.build();
-      B.newton(opts.newton, res.data());
-      for (size_t k = 0; k < take.size(); ++k) {
-        const long n0 = B.hnoff[k], nk = B.hnoff[k + 1] - n0;
-        xs[k].resize(3 * nk);
-        PGN_HIP(hipMemcpyAsync(xs[k].data(), B.M->x.p + 3 * n0, 3 * nk * sizeof(double),
-                               hipMemcpyDeviceToHost, ctx->stream));
-      }
-      PGN_HIP(hipStreamSynchronize(ctx->stream));
-    } catch (const Error& e) {
-      err = e.what();
-      code = e.code;
-    } catch (const std::exception& e) {
-      err = e.what();
-      code = PGN_ERR_INVALID;
-    }
-    const auto t_done = Clock::now();
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      for (size_t k = 0; k < take.size(); ++k) {
-        auto& r = take[k];
-        used -= r->bytes;
-        r->status = code;
-        r->error = err;
-        r->res = res[k];
-        r->x = std::move(xs[k]);
-        r->t_done = t_done;
-        r->done = true;
-        r->inst = pgn_batch_inst{};  // free the staged inputs
-        ++solved;
+        c.B->build();
+        c.B->newton_begin(opts.newton);
+        cohorts.push_back(std::move(c));
+      } catch (const Error& e) {
+        fail_cohort(c, e.code, e.what());
+        cv_done.notify_all();
+        cv_work.notify_all();
+      } catch (const std::exception& e) {
+        fail_cohort(c, PGN_ERR_INVALID, e.what());
+        cv_done.notify_all();
+        cv_work.notify_all();
       }
     }
-    cv_done.notify_all();
-    cv_work.notify_all();
+    if (cohorts.empty()) continue;
+    // ---- one Newton iteration of every resident cohort ----
+    for (auto& c : cohorts) {
+      std::vector<int> fin;
+      std::vector<std::vector<double>> xs;
+      std::vector<pgn_newton_result> rs;
+      try {
+        PGN_HIP(hipSetDevice(device));
+        fin = c.B->newton_step();
+        xs.resize(fin.size());
+        rs.resize(fin.size());
+        for (size_t q = 0; q < fin.size(); ++q) {
+          const int k = fin[q];
+          const long n0 = c.B->hnoff[k], nk = c.B->hnoff[k + 1] - n0;
+          xs[q].resize(3 * nk);
+          PGN_HIP(hipMemcpyAsync(xs[q].data(), c.B->M->x.p + 3 * n0, 3 * nk * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+          c.B->newton_result(k, rs[q]);
+        }
+        if (!fin.empty()) PGN_HIP(hipStreamSynchronize(ctx->stream));
+      } catch (const Error& e) {
+        fail_cohort(c, e.code, e.what());
+        c.B.reset();
+      } catch (const std::exception& e) {
+        fail_cohort(c, PGN_ERR_INVALID, e.what());
+        c.B.reset();
+      }
+      if (!fin.empty() && c.B) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t q = 0; q < fin.size(); ++q) {
+          const int k = fin[q];
+          auto& r = c.req[k];
+          r->res = rs[q];
+          r->x = std::move(xs[q]);
+          c.ended[k] = 1;
+          finish(r, PGN_OK, std::string());
+        }
+      }
+      if (!fin.empty() || !c.B) cv_done.notify_all();
+    }
+    // ---- release the cohorts whose requests have all ended ----
+    bool freed = false;
+    for (auto it = cohorts.begin(); it != cohorts.end();) {
+      const bool all = std::all_of(it->ended.begin(), it->ended.end(), [](char e) { return e; });
+      if (all || !it->B) {
+        it->B.reset();  // device memory back
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          used -= it->bytes;
+        }
+        it = cohorts.erase(it);
+        freed = true;
+      } else {
+        ++it;
+      }
+    }
+    if (freed) cv_work.notify_all();
   }
 }
 
@@ -249,6 +338,7 @@ void pgn_server_default_opts(pgn_server_opts* o) {
   pgn_newton_default_opts(&o->newton);
   o->group_by_size = 1;
   o->max_wait_s = 2.0;
+  o->cohorts_per_lane = 4;
 }
 
 int pgn_server_create(int device, const pgn_server_opts* opts, pgn_server** out) {
@@ -261,6 +351,8 @@ int pgn_server_create(int device, const pgn_server_opts* opts, pgn_server** out)
     if (opts) S->opts = *opts;
     PGN_CHECK(S->opts.lanes >= 1 && S->opts.lanes <= 16, "lanes must be 1..16");
     PGN_CHECK(S->opts.max_batch >= 1, "max_batch must be >= 1");
+    PGN_CHECK(S->opts.cohorts_per_lane >= 1 && S->opts.cohorts_per_lane <= 64,
+              "cohorts_per_lane must be 1..64");
     PGN_CHECK(S->opts.precision == PGN_FP64 || S->opts.precision == PGN_MIXED,
               "unknown precision");
     for (int l = 0; l < S->opts.lanes; ++l) {

@@ -285,21 +285,23 @@ def _serving_mix(count, max_tets, max_cloth, seed=5):
     return out
 
 
-def serving(args, log):
-    count = args.count or 32
-    reqs = _serving_mix(count, args.max_tets, args.max_cloth)
-    log(f"serving: {count} requests at {args.rate:g}/s, "
-        f"{sum(hasattr(r, 'hinges') for r in reqs)} cloths")
-    S = pg.Server(hbm_budget_bytes=200 << 30, lanes=4, max_batch=16, precision=pg.MIXED,
-                  group_by_size=args.group_by_size, **dict(SOLVE_OPTS, numMaxIters=300))
+def _serve_once(reqs, rate, args, log):
+    S = pg.Server(hbm_budget_bytes=200 << 30, lanes=args.lanes, max_batch=16,
+                  precision=pg.MIXED, group_by_size=args.group_by_size,
+                  cohorts_per_lane=args.cohorts, **dict(SOLVE_OPTS, numMaxIters=300))
+    count = len(reqs)
     rng = np.random.default_rng(6)
-    gaps = rng.exponential(1.0 / args.rate, size=count)
+    gaps = rng.exponential(1.0 / rate, size=count)
     tickets = [None] * count
     t_sub = [0.0] * count
 
     def producer():
+        t = time.perf_counter()
         for i, s in enumerate(reqs):
-            time.sleep(float(gaps[i]))
+            t += float(gaps[i])  # the arrival schedule, not the sleep's drift
+            d = t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
             t_sub[i] = time.perf_counter()
             tickets[i] = S.submit(s)
 
@@ -315,23 +317,45 @@ def serving(args, log):
         done_t.append(t_sub[i] + latency)
         results.append(r)
     span = max(done_t) - t0
+    offered = max(t_sub) - t0
     st = S.stats()
     S.close()
     ok = sum(r["result"] == "SUCCESS" for r in results)
-    log(f"serving: {ok}/{count} converged in {span:.2f}s, batches {st['batches']}, "
-        f"p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
+    thr = count / span
+    # saturated: the loop no longer keeps up with the arrivals
+    saturated = bool(thr < 0.9 * count / max(offered, 1e-9))
+    log(f"serving {rate:g}/s: {ok}/{count} converged in {span:.2f}s ({thr:.1f}/s, "
+        f"offered {count / max(offered, 1e-9):.1f}/s{', SATURATED' if saturated else ''}), "
+        f"batches {st['batches']}, p50 {np.percentile(lat, 50) * 1e3:.1f} ms "
+        f"p99 {np.percentile(lat, 99) * 1e3:.1f} ms")
     return {
-        "value": count / span, "unit": "Newton solves/s", "ms_per_step": span / count * 1e3,
+        "value": thr, "unit": "Newton solves/s", "ms_per_step": span / count * 1e3,
         "steps": count, "warmup": 0, "scaling": "weak",
         "dtype": "f32 Hessian/PCG, f64 energy/gradient/residual",
-        "config": {"workload": f"serving_{count}_requests: Poisson {args.rate:g}/s, 60% beams "
+        "config": {"workload": f"serving_{count}_requests: Poisson {rate:g}/s, 60% beams "
                                f"2k-{args.max_tets} tets, 40% cloths 32-{args.max_cloth}",
-                   "lanes": 4, "max_batch": 16, "group_by_size": args.group_by_size},
+                   "rate": rate, "lanes": args.lanes, "max_batch": 16,
+                   "cohorts_per_lane": args.cohorts, "group_by_size": args.group_by_size},
+        "offered_per_s": count / max(offered, 1e-9),
+        "saturated": saturated,
         "converged": ok,
         "not_converged": {f"{i}:{reqs[i].name}": results[i]["result"] for i in range(count)
                           if results[i]["result"] != "SUCCESS"},
         "latency_ms_p50": float(np.percentile(lat, 50)) * 1e3,
         "latency_ms_p99": float(np.percentile(lat, 99)) * 1e3,
         "queue_ms_p50": float(np.percentile(queue, 50)) * 1e3,
+        "queue_ms_p99": float(np.percentile(queue, 99)) * 1e3,
         "batches": st["batches"],
     }
+
+
+def serving(args, log):
+    """Poisson arrivals of the serving mix into the request loop; one result
+    per arrival rate of args.rates (comma-separated, else args.rate)."""
+    count = args.count or 32
+    reqs = _serving_mix(count, args.max_tets, args.max_cloth)
+    log(f"serving: {count} requests, {sum(hasattr(r, 'hinges') for r in reqs)} cloths, "
+        f"{sum(getattr(r, 'm', 0) for r in reqs)} tets in all")
+    rates = [float(v) for v in args.rates.split(",")] if args.rates else [args.rate]
+    lines = [_serve_once(reqs, rate, args, log) for rate in rates]
+    return lines if args.rates else lines[0]

@@ -36,6 +36,17 @@ def _oracle_x(scene):
     return O.get_x().reshape(-1, 3)
 
 
+def _stats_drained(S):
+    """stats once the lanes have released every finished batch: a request is
+    handed back before its batch's memory is freed"""
+    t0 = time.time()
+    while True:
+        st = S.stats()
+        if st["bytes_in_use"] == 0 or time.time() - t0 > 10:
+            return st
+        time.sleep(0.001)
+
+
 def _requests(seed, count):
     rng = np.random.default_rng(seed)
     out = []
@@ -74,7 +85,7 @@ def test_server_results_match_oracle(group_by_size):
         assert err <= 1e-6 * s.bbox_diag(), (i, err)
         assert 0.0 <= queue <= latency
         lat.append(latency)
-    st = S.stats()
+    st = _stats_drained(S)
     _say(f"  stats {st}, latency p50 {np.percentile(lat, 50):.3f}s "
          f"p99 {np.percentile(lat, 99):.3f}s")
     assert st["solved"] == len(reqs) and st["bytes_in_use"] == 0
@@ -89,3 +100,84 @@ def test_server_rejects_request_over_budget():
     with pytest.raises(pg.PgnError, match="budget"):
         S.submit(s)
     S.close()
+
+
+def _gpu_solo(ctx, scene):
+    B = pg.Batch(ctx, pg.MIXED)
+    B.add_scene(scene)
+    B.build()
+    r = B.newton(**OPTS)
+    assert r[0]["result"] == "SUCCESS"
+    x = B.get_x(0).copy()
+    B.close()
+    return x
+
+
+def test_server_admits_mid_run_and_returns_early(ctx):
+    """Continuous batching: with one lane busy on a long solve, a small
+    request submitted meanwhile is admitted between Newton iterations, ends
+    (and is handed back) while the long one still runs, and equals its solo
+    solve within 1e-6 x its bounding-box diagonal."""
+    _say("server: a small request admitted while a long one runs (1 lane)")
+    big = scenes.config1()  # 20,000 tets over 10 load stages
+    small = scenes.cantilever(dims=(2, 2, 4), scale=(0.02, 0.02, 0.04), stages=2)
+    S = pg.Server(lanes=1, max_batch=4, cohorts_per_lane=4, **OPTS)
+    tb = S.submit(big)
+    t0 = time.time()
+    while S.stats()["batches"] < 1:  # the long request is on the GPU
+        assert time.time() - t0 < 60
+        time.sleep(0.001)
+    ts = S.submit(small)
+    xs, rs, lat_s, _ = S.wait(ts)
+    big_done_then = S.poll(tb)
+    xb, rb, lat_b, _ = S.wait(tb)
+    st = _stats_drained(S)
+    S.close()
+    _say(f"  small {rs['iterations']} its {lat_s * 1e3:.1f} ms, big {rb['iterations']} its "
+         f"{lat_b * 1e3:.1f} ms, big done when small returned: {big_done_then}, {st}")
+    assert rs["result"] == "SUCCESS" and rb["result"] == "SUCCESS"
+    assert not big_done_then
+    assert st["batches"] == 2 and st["solved"] == 2 and st["bytes_in_use"] == 0
+    d = small.bbox_diag()
+    assert np.abs(xs - _gpu_solo(ctx, small)).max() <= 1e-6 * d
+    assert np.abs(xs - _oracle_x(small)).max() <= 1e-6 * d
+    assert np.abs(xb - _gpu_solo(ctx, big)).max() <= 1e-6 * big.bbox_diag()
+
+
+@pytest.mark.parametrize("cohorts", [1, 3])
+def test_server_cohort_limit(cohorts):
+    """cohorts_per_lane bounds the fused batches resident on a lane; results
+    do not depend on it."""
+    _say(f"server: cohorts_per_lane={cohorts}")
+    reqs = _requests(17, 6)
+    S = pg.Server(lanes=1, max_batch=1, cohorts_per_lane=cohorts, **OPTS)
+    tickets = [S.submit(s) for s in reqs]
+    for s, t in zip(reqs, tickets):
+        x, r, _, _ = S.wait(t)
+        assert r["result"] == "SUCCESS"
+        assert np.abs(x - _oracle_x(s)).max() <= 1e-6 * s.bbox_diag()
+    st = _stats_drained(S)
+    S.close()
+    assert st["batches"] == len(reqs) and st["solved"] == len(reqs)
+
+
+def test_server_destroy_during_a_solve_ends_promptly():
+    """Destroying the server stops its lanes after their current Newton
+    iteration instead of running the resident solves to the end."""
+    _say("server: destroy while a long solve runs")
+    big = scenes.cantilever(dims=(16, 16, 64), scale=(0.1, 0.1, 0.4), stages=20)  # 81,920 tets
+    S = pg.Server(lanes=1, **OPTS)
+    t0 = time.perf_counter()
+    _, r, _, _ = S.wait(S.submit(big))
+    t_full = time.perf_counter() - t0
+    S.close()
+    assert r["result"] == "SUCCESS"
+    S = pg.Server(lanes=1, **OPTS)
+    S.submit(big)
+    time.sleep(0.3 * t_full)
+    t1 = time.perf_counter()
+    S.close()
+    dt = time.perf_counter() - t1
+    _say(f"  full solve {t_full * 1e3:.0f} ms ({r['iterations']} its), destroy after "
+         f"{0.3 * t_full * 1e3:.0f} ms took {dt * 1e3:.1f} ms")
+    assert dt < 0.3 * t_full
