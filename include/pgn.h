@@ -385,7 +385,8 @@ int pgn_batch_get_energy(pgn_batch* batch, double* energy);
  * Newton iteration at a time; between iterations it admits queued requests
  * as a new batch while their estimated device footprint fits
  * hbm_budget_bytes (by default, group_by_size, the cheapest queued request
- * and requests of at most twice its footprint), hands each request back the
+ * and requests of at most twice its footprint; a request of solo_nodes or
+ * more alone, solved by the single-model PCG), hands each request back the
  * moment its own solve ends, and frees a batch's memory when its last
  * request has ended.  A request arriving while others run starts within one
  * Newton iteration of a lane; solves overlap on the device across lanes.
@@ -404,6 +405,9 @@ typedef struct {
   int cohorts_per_lane;    /* fused batches a lane keeps resident at once
                               (default 4; 1: a lane's batch ends before the
                               lane admits more)                             */
+  long solo_nodes;         /* requests of at least this many nodes run as a
+                              batch of their own, solved by the single-model
+                              PCG (default 32768)                           */
 } pgn_server_opts;
 typedef struct pgn_server pgn_server;
 void pgn_server_default_opts(pgn_server_opts* o);

@@ -96,7 +96,8 @@ class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("lanes", ctypes.c_int),
                 ("max_batch", ctypes.c_int), ("precision", ctypes.c_int),
                 ("newton", NewtonOpts), ("group_by_size", ctypes.c_int),
-                ("max_wait_s", ctypes.c_double), ("cohorts_per_lane", ctypes.c_int)]
+                ("max_wait_s", ctypes.c_double), ("cohorts_per_lane", ctypes.c_int),
+                ("solo_nodes", ctypes.c_long)]
 
 
 class NewtonResult(ctypes.Structure):
@@ -741,7 +742,7 @@ class Server:
 
     def __init__(self, device=0, hbm_budget_bytes=64 << 30, lanes=4, max_batch=64,
                  precision=MIXED, group_by_size=1, max_wait_s=2.0, cohorts_per_lane=4,
-                 **newton):
+                 solo_nodes=32768, **newton):
         o = ServerOpts()
         lib().pgn_server_default_opts(ctypes.byref(o))
         o.hbm_budget_bytes = int(hbm_budget_bytes)
@@ -751,6 +752,7 @@ class Server:
         o.group_by_size = int(group_by_size)
         o.max_wait_s = float(max_wait_s)
         o.cohorts_per_lane = int(cohorts_per_lane)
+        o.solo_nodes = int(solo_nodes)
         o.newton = _newton_opts(newton)
         h = _vp()
         _ok(lib().pgn_server_create(device, ctypes.byref(o), ctypes.byref(h)))

@@ -31,11 +31,12 @@ int api_guard_set(const char* msg, int code);
 size_t pgn_batch_inst::footprint() const {
   // per node: x, X0, g, PCG and Newton vectors (~30 x 8 B); per element:
   // connectivity + parameters + outputs (~16 + 96 + 96 + 78 x 8 B); per
-  // block (~15 per node): values + indices (9 x 8 + 24 B)
+  // block (~15 per node): values + indices (9 x 8 + 24 B), and the SpMV's
+  // sliced copy of a batch of one (9 x 8 + 4 B)
   long m = 0;
   for (int c = 0; c < 3; ++c) m += (long)T[c].size() / 4;
   const double ts = 8.0;
-  return (size_t)(n * (30.0 * ts + 15.0 * (9 * ts + 24)) + m * (16 + 24 * ts + 78 * ts + 40));
+  return (size_t)(n * (30.0 * ts + 15.0 * (18 * ts + 28)) + m * (16 + 24 * ts + 78 * ts + 40));
 }
 
 void pgn_batch::build() {
@@ -369,6 +370,18 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
   if (o.solver == PGN_LS_EIGENLDLT) {
     iters.assign(Kn, 0);
     ldlt_solve(b, x, active, tol, o);
+    return;
+  }
+  if (Kn == 1) {
+    // one instance: the single-model PCG (SELL-64 SpMV, captured chunks)
+    // solves the same system with the same regularization tries
+    iters.assign(1, 0);
+    if (!active[0]) return;
+    pgn_linear_opts o1 = o;
+    o1.tol = tol[0];
+    pgn_linear_result r1{};
+    pgn::linear_solve(*M, b, x, o1, r1);
+    iters[0] = r1.iterations;
     return;
   }
   PGN_HIP(hipMemcpyAsync(xprev.p, x, N * sizeof(double), hipMemcpyDeviceToDevice, s));

@@ -11,7 +11,8 @@
 //     footprint fits the server's HBM budget (at most max_batch of them):
 //     with group_by_size the cheapest queued request (the oldest once it has
 //     waited max_wait_s) and requests of at most twice its footprint,
-//     otherwise arrival order;
+//     otherwise arrival order; a request of at least solo_nodes nodes
+//     fills the device alone and runs as a batch of one;
 //   - hands every request back the moment its own solve ends (its positions
 //     downloaded, its waiter woken), while its cohort-mates go on;
 //   - releases a cohort's memory when its last request has ended.
@@ -95,9 +96,18 @@ std::vector<std::shared_ptr<Request>> pgn_server::admit() {
   std::vector<std::shared_ptr<Request>> take;
   const long head = pick_head();
   if (head < 0) return take;
+  auto solo = [&](long i) { return pending[i]->inst.n >= opts.solo_nodes; };
+  if (solo(head)) {
+    // a request that fills the device alone runs as a batch of one (the
+    // single-model PCG)
+    used += pending[head]->bytes;
+    take.push_back(pending[head]);
+    pending.erase(pending.begin() + head);
+    return take;
+  }
   if (!opts.group_by_size) {
     // arrival order while the footprint fits
-    while (!pending.empty() && (int)take.size() < opts.max_batch &&
+    while (!pending.empty() && (int)take.size() < opts.max_batch && !solo(0) &&
            used + pending.front()->bytes <= opts.hbm_budget_bytes) {
       used += pending.front()->bytes;
       take.push_back(pending.front());
@@ -118,7 +128,7 @@ std::vector<std::shared_ptr<Request>> pgn_server::admit() {
   taken[head] = 1;
   for (long i : order) {
     if ((int)take.size() >= opts.max_batch) break;
-    if (taken[i] || pending[i]->bytes > cap) continue;
+    if (taken[i] || pending[i]->bytes > cap || solo(i)) continue;
     if (used + pending[i]->bytes > opts.hbm_budget_bytes) break;
     used += pending[i]->bytes;
     take.push_back(pending[i]);
@@ -339,6 +349,7 @@ void pgn_server_default_opts(pgn_server_opts* o) {
   o->group_by_size = 1;
   o->max_wait_s = 2.0;
   o->cohorts_per_lane = 4;
+  o->solo_nodes = 32768;
 }
 
 int pgn_server_create(int device, const pgn_server_opts* opts, pgn_server** out) {
