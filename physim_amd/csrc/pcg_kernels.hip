@@ -575,7 +575,6 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
   __shared__ double red[3 * SELL_BS / 64 + 4];
   constexpr int WAVES = SELL_BS / 64;
   PcgScalars* sc = A.sc;
-  if (sc->done) return;
   const T* __restrict__ sval = reinterpret_cast<const T*>(A.sell_val);
   const int* __restrict__ scol = A.sell_col;
   const short* __restrict__ s16 = A.sell_c16;
@@ -595,11 +594,15 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
   for (int q = 0; q < 9; ++q) v0[q] = sval[base0 * 9 + lane + 64 * q];
 #pragma unroll
   for (int q = 0; q < 3; ++q) z0[q] = z[3 * c0 + q];
+  // the flag is read beside the loads above and the partial sums (all in
+  // bounds whether or not the solve has ended): their latencies overlap
+  const int done = sc->done;
   const double* prev = rr_slot(A, k - 1);
   const int cprev = rr_count(A, k - 1);
   double rr, bb, gam;
   all_sum3<SELL_BS>(prev, cprev, 2, A.partb, k == 0 ? A.g1 : 0, 1, prev + 1, cprev, 2, red, rr,
                     bb, gam);
+  if (done) return;
   if (k == 0) {
     double thr = A.tol * A.tol * bb;
     if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
@@ -701,16 +704,17 @@ void launch_sell_pack(hipStream_t s, bool fp32, long nb, const int* brow, const 
                        reinterpret_cast<const double*>(bval), reinterpret_cast<double*>(sval));
 }
 
-template <class T>
+template <class T, int NPT>
 struct Cg1Batch {
-  N3<T> Z[K2_NPT], P[K2_NPT], W[K2_NPT], Q[K2_NPT], X[K2_NPT], R[K2_NPT];
-  PreM<T> M[K2_NPT];
+  N3<T> Z[NPT], P[NPT], W[NPT], Q[NPT], X[NPT], R[NPT];
+  PreM<T> M[NPT];
 };
 
-template <class T, bool QF>
-__device__ __forceinline__ void cg1_load(const PcgArgs& A, long i0, long hi, Cg1Batch<T>& b) {
+template <class T, bool QF, int NPT>
+__device__ __forceinline__ void cg1_load(const PcgArgs& A, long i0, long hi,
+                                         Cg1Batch<T, NPT>& b) {
 #pragma unroll
-  for (int j = 0; j < K2_NPT; ++j) {
+  for (int j = 0; j < NPT; ++j) {
     const long i = i0 + (long)j * K2_BS;
     if (i < hi) {
       b.Z[j] = ld3(reinterpret_cast<const T*>(A.z), i);
@@ -727,19 +731,21 @@ __device__ __forceinline__ void cg1_load(const PcgArgs& A, long i0, long hi, Cg1
 
 // QF: the SpMV launch has formed q = w + beta q already and left beta and
 // r.z in cgs[4], cgs[5]; w is not read and q not written here.
-template <class T, bool QF>
+template <class T, bool QF, int NPT>
 __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
   k = chunk_k(A, k);
   __shared__ double red[3 * K2_BS / 64 + 4];
-  if (A.sc->done) return;
   const RowRange rg = block_rows(A.n);
-  // the first batch is loaded before the partial sums: the two latencies overlap
-  Cg1Batch<T> b;
+  // the first batch, the flag and the partial sums are loaded together
+  // (all in bounds whether or not the solve has ended): the latencies overlap
+  Cg1Batch<T, NPT> b;
   long i0 = rg.lo + threadIdx.x;
-  cg1_load<T, QF>(A, i0, rg.hi, b);
+  cg1_load<T, QF, NPT>(A, i0, rg.hi, b);
+  const int done = A.sc->done;
   double gam, del, unused;
   all_sum3<K2_BS>(rr_slot(A, k - 1) + 1, QF ? 0 : rr_count(A, k - 1), 2, A.part1, A.g1, 1,
                   A.part1, 0, 1, red, gam, del, unused);
+  if (done) return;
   double alpha, beta;
   if (QF) gam = A.cgs[5];
   if (k == 0) {
@@ -763,7 +769,7 @@ __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
   double rr = 0.0, rz = 0.0;
   for (;;) {
 #pragma unroll
-    for (int j = 0; j < K2_NPT; ++j) {
+    for (int j = 0; j < NPT; ++j) {
       const long i = i0 + (long)j * K2_BS;
       if (i < rg.hi) {
         N3<T> pi, qi, xi, ri;
@@ -787,9 +793,9 @@ __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
         }
       }
     }
-    i0 += (long)K2_NPT * K2_BS;
+    i0 += (long)NPT * K2_BS;
     if (i0 >= rg.hi) break;
-    cg1_load<T, QF>(A, i0, rg.hi, b);
+    cg1_load<T, QF, NPT>(A, i0, rg.hi, b);
   }
   block_sum2<K2_BS>(rr, rz, red);
   if (threadIdx.x == 0) {
@@ -1008,14 +1014,18 @@ void launch_pcg_k2(hipStream_t s, bool fp32, const PcgArgs& a, int k, hipEvent_t
   if (a.cg1) {
     if (a.sell) {
       if (fp32)
-        launch_timed(cg1_b_kernel<float, true>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
+        launch_timed(cg1_b_kernel<float, true, K2_NPT>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a,
+                     k);
       else
-        launch_timed(cg1_b_kernel<double, true>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
+        launch_timed(cg1_b_kernel<double, true, K2_NPT>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a,
+                     k);
     } else {
       if (fp32)
-        launch_timed(cg1_b_kernel<float, false>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
+        launch_timed(cg1_b_kernel<float, false, K2_NPT>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a,
+                     k);
       else
-        launch_timed(cg1_b_kernel<double, false>, dim3(a.g2), dim3(K2_BS), s, e0, e1, a, k);
+        launch_timed(cg1_b_kernel<double, false, K2_NPT>, dim3(a.g2), dim3(K2_BS), s, e0, e1,
+                     a, k);
     }
     return;
   }
