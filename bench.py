@@ -36,11 +36,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, counts):
-    """Oracle (FP64 CPU restatement, OpenMP element loops, serial assembly and
-    CG like the reference) timed on a bounded sample of the same workload:
-    one E+g evaluation, one Hessian evaluation + assembly and 20 CG iterations
-    of the config, scaled by the GPU solve's counts of each."""
+def cpu_baseline(scene, counts, counts_from):
+    """Oracle (FP64 CPU restatement of the reference algorithm: OpenMP element
+    loops, serial assembly and Jacobi-CG like the reference) on a bounded
+    sample of the same workload: one E+g evaluation, one Hessian evaluation +
+    assembly and 20 CG iterations of the 1.28M-tet config, scaled by the
+    counts of the reference algorithm as the plain FP64 path ran it (Jacobi
+    PCG to 1e-9, cap 1000: counts_from says which solve the counts are
+    from); plus the 10x10x40 beam's full oracle solve, timed once."""
     from oracle import orc
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     M = orc.Model(scene.X0, scene.T, scene.material, scene.lame1, scene.lame2,
@@ -59,16 +62,71 @@ def cpu_baseline(scene, counts):
     t0 = time.perf_counter()
     orc.cg(rp, ci, v, -g, np.zeros_like(g), 1e-30, its)
     t_cg = (time.perf_counter() - t0) / its
+    del M, rp, ci, v
     t_solve = (counts["energy_evals"] * t_eg + counts["hessian_evals"] * t_h
                + counts["pcg_iterations"] * t_cg)
+    c1, r1 = _cpu_full_solve("config1")
     return {"value": 1.0 / t_solve, "unit": "Newton solves/s", "cores": threads,
             "kind": "port",
             "sample": (f"oracle on the same 1.28M-tet config: E+g {t_eg:.3f}s, "
                        f"Hessian+assembly {t_h:.3f}s, CG iteration {t_cg * 1e3:.1f}ms "
-                       f"(20 timed), scaled by the GPU solve's "
+                       f"(20 timed), scaled by the {counts_from}'s "
                        f"{counts['energy_evals']} E+g / {counts['hessian_evals']} H / "
-                       f"{counts['pcg_iterations']} PCG iterations -> "
-                       f"{t_solve:.1f}s per solve")}
+                       f"{counts['pcg_iterations']} CG iterations -> "
+                       f"{t_solve:.1f}s per solve"),
+            "config1_full_solve_s": c1, "config1_newton_iterations": r1["iterations"],
+            "config1_cg_iterations": r1["cg_iterations"]}
+
+
+def _cpu_full_solve(which):
+    """One full oracle Newton solve of the 10x10x40 beam ("config1") or the
+    1.28M-tet beam ("config2") with the reference algorithm's linear solver
+    (Jacobi CG to 1e-9, cap 1000) and the benchmark's Newton tolerance."""
+    import physim_amd as pg
+    from oracle import orc
+    from physim_amd import scenes
+    sc = scenes.config1() if which == "config1" else scenes.config2()
+    O = orc.Model(sc.X0, sc.T, sc.material, sc.lame1, sc.lame2, sc.density)
+    sc.apply(O)
+    t0 = time.perf_counter()
+    r = O.newton(**pg.CONVERGE_OPTS)
+    return time.perf_counter() - t0, r
+
+
+def cpu_study(args):
+    """The oracle's full solves on the host: the 10x10x40 beam with all host
+    threads and with one, median of 5 after a warm-up (each in a child
+    process, so that OMP_NUM_THREADS takes effect), and with --cpu-config2
+    the 1.28M-tet beam once with all threads.  No GPU is used."""
+    import platform
+    import subprocess
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"cpu": platform.processor() or platform.machine(), "host_cpus": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    out["cpu"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    jobs = [("config1", nthreads, 6), ("config1", 1, 6)]
+    if args.cpu_config2:
+        jobs.append(("config2", nthreads, 1))
+    for which, th, reps in jobs:
+        env = dict(os.environ, OMP_NUM_THREADS=str(th))
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-solve", which,
+                            "--reps", str(reps)], env=env, capture_output=True, text=True,
+                           check=True)
+        res = json.loads(p.stdout.strip().splitlines()[-1])
+        t = res["times"]
+        key = f"{which}_{'1thread' if th == 1 else f'{th}threads'}"
+        out[key] = {"threads": th, "runs_s": t,
+                    "median_s": float(np.median(t[1:] if len(t) > 1 else t)),
+                    "newton_iterations": res["iterations"],
+                    "cg_iterations": res["cg_iterations"], "result": res["result"]}
+        log(f"cpu {key}: {out[key]}")
+    print(json.dumps({"cpu_study": out}), flush=True)
 
 
 def measured_traffic(kernel):
@@ -150,7 +208,25 @@ def main():
                          "first; 0 arrival order")
     ap.add_argument("--rod-density", type=float, default=3.14e-6,
                     help="rods: line density, kg per metre of rod")
+    ap.add_argument("--cpu-study", action="store_true",
+                    help="time the oracle's full solves on the host (10x10x40 beam with "
+                         "all threads and with one, median of 5) and exit; no GPU")
+    ap.add_argument("--cpu-config2", action="store_true",
+                    help="--cpu-study: also the 1.28M-tet beam's full solve, once")
+    ap.add_argument("--cpu-solve", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--reps", type=int, default=1, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_solve:
+        ts, r = [], None
+        for _ in range(args.reps):
+            t, r = _cpu_full_solve(args.cpu_solve)
+            ts.append(t)
+        print(json.dumps({"times": ts, "iterations": r["iterations"],
+                          "cg_iterations": r["cg_iterations"], "result": r["result"]}))
+        return
+    if args.cpu_study:
+        cpu_study(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and "
               f"has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -248,6 +324,7 @@ def main():
         ts = time.perf_counter()
         rs = F.newton(**opts)
         ts = time.perf_counter() - ts
+        fp64_counts = rf
         fp64 = {"fp64_solve_s": tf, "fp64_result": rf["result"],
                 "fp64_newton_iterations": rf["iterations"],
                 "fp64_pcg_iterations": rf["pcg_iterations"],
@@ -265,9 +342,14 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        counts = {k: int(np.mean([r[k] for r in results]))
-                  for k in ("energy_evals", "hessian_evals", "pcg_iterations")}
-        cpu = cpu_baseline(scene, counts)
+        keys = ("energy_evals", "hessian_evals", "pcg_iterations")
+        if fp64:
+            counts = {k: int(fp64_counts[k]) for k in keys}
+            src = "plain FP64 path (the reference algorithm)"
+        else:
+            counts = {k: int(np.mean([r[k] for r in results])) for k in keys}
+            src = "mixed path (no FP64 run)"
+        cpu = cpu_baseline(scene, counts, src)
         log(f"cpu baseline {cpu}")
 
     lat = np.array([r["wall_s"] for r in results]) * 1e3
