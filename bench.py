@@ -169,8 +169,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--precond", type=int, default=1,
-                    help="0 Jacobi, 1 3x3 block Jacobi (default)")
+    ap.add_argument("--precond", type=int, default=2,
+                    help="0 Jacobi, 1 3x3 block Jacobi, 2 two-level (default: block "
+                         "Jacobi + aggregated rigid-body coarse space)")
     ap.add_argument("--adaptive", type=int, default=1,
                     help="1 (default): Eisenstat-Walker linear tolerance")
     ap.add_argument("--ls-max-iters", type=int, default=2500,
@@ -270,7 +271,7 @@ def main():
 
     sweep = []
     if args.sweep:
-        for pc in (0, 1):
+        for pc in (0, 1, 2):
             for ad in (0, 1):
                 M.reset()
                 r = M.newton(**dict(pg.CONVERGE_OPTS, precond=pc, adaptive_tol=ad))
@@ -381,8 +382,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "beam_40x40x160_conh_lognormal: 1,280,000 tets, "
                                "811,923 DoF, 10 load stages, Newton to |g|<=1e-8",
-                   "precision": "mixed", "precond": "jacobi" if args.precond == 0
-                   else "block-jacobi",
+                   "precision": "mixed",
+                   "precond": ["jacobi", "block-jacobi", "two-level"][args.precond],
                    "linear_tol": "eisenstat-walker" if args.adaptive else "1e-9",
                    "pcg_max_iters": args.ls_max_iters},
         "converged": ok,

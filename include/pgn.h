@@ -159,6 +159,30 @@ int pgn_model_get_hessian(pgn_model* model, int layout, double* values);
  * (tet models), as the PCG forms it under PGN_MATFREE=1. */
 int pgn_model_hessian_product(pgn_model* model, int matrix_free, const double* p,
                               double* y);
+/* The preconditioner of LinearSolver_EigenCG's ConjugateGradient
+ * (LinearSolver_EigenCG.cpp:68-89: Eigen's DiagonalPreconditioner), and the
+ * two MI355X alternatives, applied to r with the Hessian of the current x:
+ * z = M^-1 r (FP64 at the boundary, the model's Hessian precision inside).
+ * precond 0: Jacobi; 1: 3x3 block Jacobi B; 2: two-level,
+ * z = B^-1 r + P Ac^-1 P^T r (see pgn_model_coarse_space). */
+int pgn_model_precond_apply(pgn_model* model, int precond, const double* r,
+                            double* z);
+/* The coarse space of the two-level preconditioner (no reference
+ * counterpart; it replaces the diagonal preconditioner of the CG above when
+ * precond = 2).  Nodes are aggregated by their rest positions into a grid
+ * of boxes over the bounding box (about 128 of them, shaped to its extents;
+ * PGN_COARSE_AGGS in the environment sets the count, at most 341); each
+ * aggregate spans the three translations and the three infinitesimal
+ * rotations about its centroid over its free DoFs (columns dependent on the
+ * kept ones are dropped).  n_agg: aggregates; agg_of (n, may be NULL): the
+ * aggregate of each node; cmask (n_agg, may be NULL): kept columns (bits 0-2
+ * translations x y z, 3-5 rotations about x y z); coarse and coarse_inv
+ * (36 n_agg^2 each, may be NULL): Ac = P^T H P for the Hessian of the
+ * current x, dense row-major over coarse DoFs 6a + c with identity rows for
+ * dropped columns, and the inverse the PCG applies. */
+int pgn_model_coarse_space(pgn_model* model, int* n_agg, int* agg_of,
+                           unsigned char* cmask, double* coarse,
+                           double* coarse_inv);
 /* IEnergyElement::GetLocalEnergy/Gradient/Hessian for every element at the
  * current x, from the fused E+g+H element kernels of the given precision
  * (PGN_FP64 or PGN_MIXED = all-FP32 kernels).  Any pointer may be NULL.
@@ -173,7 +197,10 @@ typedef struct {
   double tol;      /* relative residual |r| <= tol |b| (lsTolMaxError)   */
   int max_iters;   /* lsNumMaxIters                                       */
   int reg_iters;   /* LinearSolverOptions::regIters                       */
-  int precond;     /* 0 = Jacobi (reference), 1 = 3x3 block Jacobi         */
+  int precond;     /* 0 = Jacobi (reference), 1 = 3x3 block Jacobi, 2 =
+                      two-level (block Jacobi + aggregated rigid-body coarse
+                      space, pgn_model_coarse_space; Chronopoulos-Gear PCG
+                      over the assembled matrix only)                    */
   double inner_tol;/* MIXED only: FP32 inner relative tolerance per IR sweep */
   int solver;      /* LinearSolverOptions::type: PGN_LS_EIGENCG (default) or
                       PGN_LS_EIGENLDLT; with LDL^T, tol is maxError of the
@@ -259,7 +286,8 @@ typedef struct {
   double lsTolMaxError;
   int lsNumRegTrials;
   /* MI355X path controls (no reference counterpart) */
-  int precond;        /* 0 = Jacobi, 1 = 3x3 block Jacobi                  */
+  int precond;        /* 0 = Jacobi, 1 = 3x3 block Jacobi, 2 = two-level
+                         (single models; batches take 2 as 1)              */
   double inner_tol;   /* MIXED: FP32 inner tolerance per refinement sweep */
   int adaptive_tol;   /* 0: every linear solve to lsTolMaxError (reference).
                          1: Eisenstat-Walker forcing (choice 2, gamma

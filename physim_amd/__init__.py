@@ -38,7 +38,7 @@ EXPORTS = [
     "pgn_model_get_gradient", "pgn_model_hessian_nnz",
     "pgn_model_hessian_pattern", "pgn_model_get_hessian",
     "pgn_model_element_egh", "pgn_linear_default_opts", "pgn_linear_solve",
-    "pgn_model_hessian_product",
+    "pgn_model_hessian_product", "pgn_model_precond_apply", "pgn_model_coarse_space",
     "pgn_newton_default_opts", "pgn_newton_solve", "pgn_time_kernel",
     "pgn_kernel_bytes", "pgn_host_bsr_pattern", "pgn_model_set_profiling",
     "pgn_model_get_profile", "pgn_model_create_shell", "pgn_batch_create",
@@ -153,6 +153,8 @@ def lib():
         L.pgn_model_get_hessian.argtypes = [_vp, ctypes.c_int, _dp]
         L.pgn_model_element_egh.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp]
         L.pgn_model_hessian_product.argtypes = [_vp, ctypes.c_int, _dp, _dp]
+        L.pgn_model_precond_apply.argtypes = [_vp, ctypes.c_int, _dp, _dp]
+        L.pgn_model_coarse_space.argtypes = [_vp, _ip, _ip, _ucp, _dp, _dp]
         L.pgn_linear_default_opts.argtypes = [ctypes.POINTER(LinearOpts)]
         L.pgn_linear_default_opts.restype = None
         L.pgn_linear_solve.argtypes = [_vp, _dp, _dp, ctypes.POINTER(LinearOpts),
@@ -539,6 +541,33 @@ class TetModel:
         y = np.zeros(self.N)
         _ok(lib().pgn_model_hessian_product(self._h, 1 if matrix_free else 0, _p(p), _p(y)))
         return y
+
+    def precond_apply(self, r, precond):
+        """z = M^-1 r with the preconditioner of the Hessian at the current x
+        (pgn_model_precond_apply: 0 Jacobi, 1 block Jacobi, 2 two-level)."""
+        r = _f64(r).reshape(-1)
+        z = np.zeros(self.N)
+        _ok(lib().pgn_model_precond_apply(self._h, int(precond), _p(r), _p(z)))
+        return z
+
+    def coarse_space(self, matrices=False):
+        """The two-level preconditioner's aggregation (pgn_model_coarse_space):
+        (agg_of, cmask) and, with matrices, also (Ac, Ac^-1) of the Hessian at
+        the current x."""
+        na = ctypes.c_int(0)
+        _ok(lib().pgn_model_coarse_space(self._h, ctypes.byref(na), None, None, None, None))
+        k = na.value
+        agg = np.zeros(self.N // 3, dtype=np.int32)
+        cm = np.zeros(max(k, 1), dtype=np.uint8)
+        if not matrices:
+            _ok(lib().pgn_model_coarse_space(self._h, ctypes.byref(na), _p(agg, _ip),
+                                             _p(cm, _ucp), None, None))
+            return agg, cm[:k]
+        Ac = np.zeros((6 * k, 6 * k))
+        Ai = np.zeros((6 * k, 6 * k))
+        _ok(lib().pgn_model_coarse_space(self._h, ctypes.byref(na), _p(agg, _ip), _p(cm, _ucp),
+                                         _p(Ac), _p(Ai)))
+        return agg, cm[:k], Ac, Ai
 
     def linear_solve(self, b, x0=None, **kw):
         o = _linear_opts(kw)

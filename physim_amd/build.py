@@ -17,7 +17,7 @@ OBJ = os.path.join(HERE, "_build")
 SOURCES = ["rod_kernels.hip", "tet_kernels.hip", "shell_kernels.hip", "assembly_kernels.hip",
            "pcg_kernels.hip", "batch_kernels.hip", "model.cpp", "solver.cpp", "api.cpp",
            "rod_model.cpp", "rod_api.cpp", "batch.cpp", "server.cpp", "ldlt_kernels.hip",
-           "ldlt.cpp", "csr_solve.hip", "matfree_kernels.hip", "scene.cpp"]
+           "ldlt.cpp", "csr_solve.hip", "matfree_kernels.hip", "scene.cpp", "coarse_kernels.hip", "coarse.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

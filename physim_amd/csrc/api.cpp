@@ -234,7 +234,10 @@ size_t pgn_model_device_bytes(const pgn_model* M) {
                    M->lb_S, M->lb_Y, M->vel, M->xhat,
                    M->xstep0, M->ldlt_prob, M->ldlt_perm, M->ldlt_iperm, M->ldlt_nprob,
                    M->ldlt_list, M->ldlt_status, M->ldlt_band, M->ldlt_D, M->ldlt_sinT,
-                   M->ldlt_y);
+                   M->ldlt_y, M->cg_seg_agg, M->cg_seg_lo, M->cg_agg_seg0, M->cg_agg_nodes,
+                   M->cg_agg_of, M->cg_cb_a, M->cg_cb_b, M->cg_cb_ptr, M->cg_cb_fine,
+                   M->cg_rel, M->cg_cmask, M->cg_A, M->cg_D, M->cg_spart, M->cg_uc,
+                   M->cg_cpart, M->cg_cdot, M->cg_ok);
 }
 
 int pgn_model_set_fixed(pgn_model* M, const unsigned char* mask) {
@@ -248,6 +251,7 @@ int pgn_model_set_fixed(pgn_model* M, const unsigned char* mask) {
     M->fixmask.upload(fm, M->stream());
     M->sync();
     M->hess_valid = false;
+    M->coarse_ready = false;  // the coarse columns depend on the free DoFs
   });
 }
 
@@ -454,6 +458,68 @@ int pgn_model_hessian_product(pgn_model* M, int matrix_free, const double* p, do
       PGN_HIP(hipMemcpyAsync(y, qout, N * sizeof(double), hipMemcpyDeviceToHost, s));
       M->sync();
     }
+  });
+}
+
+int pgn_model_coarse_space(pgn_model* M, int* n_agg, int* agg_of, unsigned char* cmask,
+                           double* coarse, double* coarse_inv) {
+  return guard([&] {
+    PGN_CHECK(M && n_agg, "null argument");
+    use_device(M->ctx);
+    M->coarse_prepare();
+    const int na = M->cga.na;
+    *n_agg = na;
+    hipStream_t s = M->stream();
+    if (agg_of) std::copy(M->h_agg_of.begin(), M->h_agg_of.end(), agg_of);
+    if (cmask) {
+      M->cg_cmask.download(cmask, na, s);
+      M->sync();
+    }
+    if (!coarse && !coarse_inv) return;
+    M->eval_hessian();
+    const int nc = 6 * na, np = M->cga.npad;
+    std::vector<double> h((size_t)np * np);
+    auto take = [&](const double* dev, double* out, double sign) {
+      PGN_HIP(hipMemcpyAsync(h.data(), dev, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      M->sync();
+      for (int i = 0; i < nc; ++i)
+        for (int j = 0; j < nc; ++j) out[(size_t)i * nc + j] = sign * h[(size_t)i * np + j];
+    };
+    if (coarse) take(M->coarse_update(true), coarse, 1.0);
+    if (coarse_inv) {
+      const double* R = M->coarse_update(false);
+      int ok = 0;
+      PGN_HIP(hipMemcpyAsync(&ok, M->cg_ok.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      M->sync();
+      PGN_CHECK(ok, "coarse operator not positive definite");
+      take(R, coarse_inv, -1.0);  // R holds -Ac^-1
+    }
+  });
+}
+
+int pgn_model_precond_apply(pgn_model* M, int precond, const double* r, double* z) {
+  return guard([&] {
+    PGN_CHECK(M && r && z, "null argument");
+    PGN_CHECK(precond >= 0 && precond <= 2, "precond: 0, 1 or 2");
+    use_device(M->ctx);
+    M->eval_hessian();
+    if (precond == 2) {
+      M->coarse_prepare();
+      M->coarse_update();
+    }
+    hipStream_t s = M->stream();
+    const long N = 3L * M->n;
+    const size_t ts = M->tsize();
+    PGN_HIP(hipMemcpyAsync(M->tmp64a.p, r, N * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_narrow_or_copy(s, M->hfp32, N, M->tmp64a.p, M->pb.p);
+    // the PCG's init from x = 0: r_pcg = r, z = B^-1 r, then the coarse part
+    const PcgArgs a = M->pcg_args(0.0, precond);
+    launch_pcg_init(s, M->hfp32, a, M->pb.p, 1);
+    if (a.coarse) launch_coarse_apply(s, M->hfp32, a);
+    PGN_HIP(hipGetLastError());
+    launch_widen(s, M->hfp32, N, M->pv.p + 2 * N * ts, M->tmp64b.p);
+    PGN_HIP(hipMemcpyAsync(z, M->tmp64b.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+    M->sync();
   });
 }
 

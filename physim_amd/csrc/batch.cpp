@@ -285,7 +285,7 @@ SegPcgArgs pgn_batch::seg_args(int precond) const {
   a.bval = M->bval.p;
   a.dinv = M->dinv.p;
   a.binv = M->binv.p;
-  a.precond = precond;
+  a.precond = precond >= 1 ? 1 : 0;  // the two-level preconditioner is for single models
   a.lpr = seg_lpr;
   const size_t v = 3L * M->n * M->tsize();
   char* base = M->pv.p;

@@ -1,0 +1,250 @@
+// coarse.cpp -- host setup of the two-level preconditioner's coarse space.
+//
+// Aggregates: the rest positions binned into a grid of boxes over the model's
+// bounding box, about coarse_target boxes shaped to its extents (empty boxes
+// dropped, ids in box order).  Each aggregate spans the translations and the
+// infinitesimal rotations about its centroid over its free DoFs; a column is
+// kept only where it is independent of the kept ones (an aggregate whose
+// free nodes are all fixed, collinear or too few keeps fewer than six).
+// PGN_COARSE_AGGS in the environment sets the target (at most 341, the
+// coarse dense inverse's capacity).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "model.hpp"
+
+namespace pgn {
+
+namespace {
+
+// keeps column c of the Gram matrix G (6x6) when its pivot, after the kept
+// columns, is a fair part of its own diagonal (incremental Cholesky)
+uint8_t independent_columns(const double G[36]) {
+  double L[36] = {0};
+  int kept[6], nk = 0;
+  uint8_t mask = 0;
+  for (int c = 0; c < 6; ++c) {
+    const double gcc = G[6 * c + c];
+    if (!(gcc > 0.0)) continue;
+    double y[6];
+    double d = gcc;
+    for (int k = 0; k < nk; ++k) {
+      double v = G[6 * kept[k] + c];
+      for (int m = 0; m < k; ++m) v -= L[6 * k + m] * y[m];
+      y[k] = v / L[6 * k + k];
+      d -= y[k] * y[k];
+    }
+    if (d > 1e-8 * gcc) {
+      for (int m = 0; m < nk; ++m) L[6 * nk + m] = y[m];
+      L[6 * nk + nk] = std::sqrt(d);
+      kept[nk++] = c;
+      mask |= (uint8_t)(1u << c);
+    }
+  }
+  return mask;
+}
+
+}  // namespace
+
+}  // namespace pgn
+
+using namespace pgn;
+
+void pgn_model::coarse_prepare() {
+  if (coarse_ready) return;
+  if (const char* e = std::getenv("PGN_COARSE_AGGS")) coarse_target = std::max(1, std::atoi(e));
+  const int max_aggs = COARSE_MAX_PAD / 6;
+  const int target = std::min(coarse_target, max_aggs);
+  // bins shaped to the bounding box
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = hi[k] = hX0[k];
+  }
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], hX0[3L * i + k]);
+      hi[k] = std::max(hi[k], hX0[3L * i + k]);
+    }
+  double ext[3], emax = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    ext[k] = hi[k] - lo[k];
+    emax = std::max(emax, ext[k]);
+  }
+  int bins[3] = {1, 1, 1};
+  if (emax > 0.0) {
+    double e[3], vol = 1.0;
+    for (int k = 0; k < 3; ++k) {
+      e[k] = std::max(ext[k], 1e-3 * emax);
+      vol *= e[k];
+    }
+    const double s = std::cbrt((double)target / vol);
+    for (int k = 0; k < 3; ++k) bins[k] = std::max(1, (int)std::lround(s * e[k]));
+    while ((long)bins[0] * bins[1] * bins[2] > max_aggs) {
+      const int k = (bins[0] >= bins[1] && bins[0] >= bins[2]) ? 0 : (bins[1] >= bins[2] ? 1 : 2);
+      bins[k]--;
+    }
+  }
+  std::copy(bins, bins + 3, coarse_bins);
+  std::vector<long> key(n);
+  for (int i = 0; i < n; ++i) {
+    long b[3];
+    for (int k = 0; k < 3; ++k) {
+      long v = 0;
+      if (ext[k] > 0.0) v = (long)std::floor((hX0[3L * i + k] - lo[k]) / ext[k] * bins[k]);
+      b[k] = std::min<long>(std::max<long>(v, 0), bins[k] - 1);
+    }
+    key[i] = (b[2] * bins[1] + b[1]) * bins[0] + b[0];
+  }
+  std::vector<long> keys(key);
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  const int na = (int)keys.size();
+  h_agg_of.assign(n, 0);
+  for (int i = 0; i < n; ++i)
+    h_agg_of[i] = (int)(std::lower_bound(keys.begin(), keys.end(), key[i]) - keys.begin());
+  // centroids, offsets, column masks
+  std::vector<double> cen(3L * na, 0.0);
+  std::vector<int> cnt(na, 0);
+  for (int i = 0; i < n; ++i) {
+    const int a = h_agg_of[i];
+    cnt[a]++;
+    for (int k = 0; k < 3; ++k) cen[3L * a + k] += hX0[3L * i + k];
+  }
+  for (int a = 0; a < na; ++a)
+    for (int k = 0; k < 3; ++k) cen[3L * a + k] /= cnt[a];
+  std::vector<double> rel(3L * n);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) rel[3L * i + k] = hX0[3L * i + k] - cen[3L * h_agg_of[i] + k];
+  // Gram matrix sum_i P_i^T F_i P_i per aggregate, rotation columns scaled by
+  // the aggregate's rms offset (so that the independence test is unit-free)
+  std::vector<double> rms(na, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) rms[h_agg_of[i]] += rel[3L * i + k] * rel[3L * i + k];
+  for (int a = 0; a < na; ++a) rms[a] = std::sqrt(rms[a] / cnt[a]);
+  std::vector<double> G(36L * na, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const int a = h_agg_of[i];
+    const double sc = rms[a] > 0.0 ? 1.0 / rms[a] : 0.0;
+    const double x = rel[3L * i] * sc, y = rel[3L * i + 1] * sc, z = rel[3L * i + 2] * sc;
+    // rows of P_i: component p of (t + w x rel)
+    const double P[3][6] = {{1, 0, 0, 0, z, -y}, {0, 1, 0, -z, 0, x}, {0, 0, 1, y, -x, 0}};
+    for (int p = 0; p < 3; ++p) {
+      if (hfix.size() == 3u * n && hfix[3L * i + p]) continue;
+      for (int u = 0; u < 6; ++u)
+        for (int v = 0; v < 6; ++v) G[36L * a + 6 * u + v] += P[p][u] * P[p][v];
+    }
+  }
+  std::vector<uint8_t> cmask(na);
+  for (int a = 0; a < na; ++a) cmask[a] = independent_columns(&G[36L * a]);
+  // nodes by aggregate, restriction segments
+  std::vector<int> agg_ptr(na + 1, 0);
+  for (int i = 0; i < n; ++i) agg_ptr[h_agg_of[i] + 1]++;
+  for (int a = 0; a < na; ++a) agg_ptr[a + 1] += agg_ptr[a];
+  std::vector<int> nodes(n), cur(agg_ptr.begin(), agg_ptr.end() - 1);
+  for (int i = 0; i < n; ++i) nodes[cur[h_agg_of[i]]++] = i;
+  std::vector<int> seg_agg, seg_lo, agg_seg0(na + 1, 0);
+  for (int a = 0; a < na; ++a) {
+    agg_seg0[a] = (int)seg_agg.size();
+    for (int t = agg_ptr[a]; t < agg_ptr[a + 1]; t += COARSE_SEG) {
+      seg_agg.push_back(a);
+      seg_lo.push_back(t);
+    }
+  }
+  agg_seg0[na] = (int)seg_agg.size();
+  seg_lo.push_back(n);
+  // coarse block pairs (a <= b) and their BSR3 blocks in block order
+  std::vector<int> pcount((size_t)na * na, 0);
+  for (long f = 0; f < pat.nb; ++f) {
+    const int a = h_agg_of[pat.brow[f]], b = h_agg_of[pat.bcol[f]];
+    if (a <= b) pcount[(size_t)a * na + b]++;
+  }
+  std::vector<int> pidx((size_t)na * na, -1), cb_a, cb_b, cb_ptr(1, 0);
+  for (int a = 0; a < na; ++a)
+    for (int b = a; b < na; ++b)
+      if (pcount[(size_t)a * na + b] > 0) {
+        pidx[(size_t)a * na + b] = (int)cb_a.size();
+        cb_a.push_back(a);
+        cb_b.push_back(b);
+        cb_ptr.push_back(cb_ptr.back() + pcount[(size_t)a * na + b]);
+      }
+  std::vector<int> cb_fine(cb_ptr.back());
+  std::vector<int> pos(cb_ptr.begin(), cb_ptr.end() - 1);
+  for (long f = 0; f < pat.nb; ++f) {
+    const int a = h_agg_of[pat.brow[f]], b = h_agg_of[pat.bcol[f]];
+    if (a <= b) cb_fine[pos[pidx[(size_t)a * na + b]]++] = (int)f;
+  }
+  // device copies
+  hipStream_t s = stream();
+  cg_seg_agg.upload(seg_agg, s);
+  cg_seg_lo.upload(seg_lo, s);
+  cg_agg_seg0.upload(agg_seg0, s);
+  cg_agg_nodes.upload(nodes, s);
+  cg_agg_of.upload(h_agg_of, s);
+  cg_cmask.upload(cmask, s);
+  cg_cb_a.upload(cb_a, s);
+  cg_cb_b.upload(cb_b, s);
+  cg_cb_ptr.upload(cb_ptr, s);
+  cg_cb_fine.upload(cb_fine, s);
+  std::vector<float> r32;  // alive until the sync below
+  if (hfp32) {
+    r32.assign(rel.begin(), rel.end());
+    cg_rel.upload(reinterpret_cast<const char*>(r32.data()), r32.size() * sizeof(float), s);
+  } else {
+    cg_rel.upload(reinterpret_cast<const char*>(rel.data()), rel.size() * sizeof(double), s);
+  }
+  const int npad = (6 * na + COARSE_TILE - 1) / COARSE_TILE * COARSE_TILE;
+  cg_A.ensure(2L * npad * npad);
+  cg_D.ensure(2L * COARSE_TILE * COARSE_TILE);
+  cg_ok.ensure(1);
+  cg_spart.ensure(6L * seg_agg.size());
+  cg_uc.ensure(npad);
+  const int gg = coarse_gemv_grid(npad);
+  cg_cpart.ensure(gg);
+  cg_cdot.ensure(1);
+  PGN_HIP(hipMemsetAsync(cg_cdot.p, 0, sizeof(double), s));
+  coarse_ncb = (int)cb_a.size();
+  cga = CoarseArgs{};
+  cga.na = na;
+  cga.npad = npad;
+  cga.nseg = (int)seg_agg.size();
+  cga.gemv_g = gg;
+  cga.seg_agg = cg_seg_agg.p;
+  cga.seg_lo = cg_seg_lo.p;
+  cga.agg_seg0 = cg_agg_seg0.p;
+  cga.agg_nodes = cg_agg_nodes.p;
+  cga.agg_of = cg_agg_of.p;
+  cga.rel = cg_rel.p;
+  cga.cmask = cg_cmask.p;
+  cga.fixmask = fixmask.p;
+  cga.R = nullptr;
+  cga.ok = cg_ok.p;
+  cga.spart = cg_spart.p;
+  cga.uc = cg_uc.p;
+  cga.cpart = cg_cpart.p;
+  cga.cdot = cg_cdot.p;
+  sync();
+  coarse_ready = true;
+}
+
+double* pgn_model::coarse_update(bool assemble_only) {
+  coarse_prepare();
+  CoarseBuild b;
+  b.ncb = coarse_ncb;
+  b.cb_a = cg_cb_a.p;
+  b.cb_b = cg_cb_b.p;
+  b.cb_ptr = cg_cb_ptr.p;
+  b.cb_fine = cg_cb_fine.p;
+  b.brow = brow.p;
+  b.bcol = bcol.p;
+  b.bval = bval.p;
+  const long nn = (long)cga.npad * cga.npad;
+  b.A0 = cg_A.p;
+  b.A1 = cg_A.p + nn;
+  b.D = cg_D.p;
+  b.ok = cg_ok.p;
+  double* R = launch_coarse_operator(stream(), hfp32, cga, b, assemble_only);
+  if (!assemble_only) cga.R = R;
+  return R;
+}

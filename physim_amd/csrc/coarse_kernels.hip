@@ -1,0 +1,439 @@
+// coarse_kernels.hip -- the coarse level of the two-level PCG preconditioner.
+//
+// M^-1 r = B^-1 r + P Ac^-1 P^T r: B is the 3x3 block-Jacobi part (applied by
+// the PCG update launch), P the aggregated rigid-body prolongation described
+// with CoarseArgs (kernels.hpp).  Per linear solve try this file builds
+// Ac = P^T A P from the assembled BSR3 values (FP64 sums, one workgroup per
+// coarse block pair, fixed order) and inverts it in place by block
+// Gauss-Jordan sweeps (the sweep operator on symmetric positive definite
+// matrices: after sweeping every pivot the matrix holds -Ac^-1; no pivoting
+// is needed since every pivot is a Schur complement of an SPD matrix).
+// Per PCG iteration it restricts r (segment partials), multiplies by the
+// inverse and prolongs into z.  Every sum has a fixed order: the results do
+// not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "pgn_internal.hpp"
+#include "reduce.hpp"
+
+namespace pgn {
+
+namespace {
+
+constexpr int TL = COARSE_TILE;  // 32
+constexpr int CB_BS = 256;       // coarse assembly workgroup
+constexpr int GJ_BS = 256;       // sweep workgroup: 4 entries of a 32 x 32 tile each
+constexpr int RS_BS = 256;       // restriction workgroup
+constexpr int GV_BS = 256;       // coarse product workgroup: 16 rows
+constexpr int GV_ROWS = 16;
+constexpr int PR_BS = 256;       // prolongation workgroup
+
+// w x r as the 3x3 matrix W(r) acting on w: rows (0, rz, -ry), (-rz, 0, rx),
+// (ry, -rx, 0)
+__device__ __forceinline__ void rot_block(const double r[3], double W[9]) {
+  W[0] = 0.0;   W[1] = r[2];  W[2] = -r[1];
+  W[3] = -r[2]; W[4] = 0.0;   W[5] = r[0];
+  W[6] = r[1];  W[7] = -r[0]; W[8] = 0.0;
+}
+
+// ---- Ac = P^T A P ----------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void coarse_fill_kernel(int npad, int nact, double* A,
+                                                          int* ok) {
+  const long total = (long)npad * npad;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long i = e / npad, j = e % npad;
+    A[e] = (i == j && i >= nact) ? 1.0 : 0.0;  // padding DoFs: identity
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ok = 1;
+}
+
+// One workgroup per coarse block pair (a, b), a <= b: the 6x6 block
+// sum_{i in a, j in b} P_i^T A_ij P_j over the listed BSR3 blocks, with
+// A_ij's fixed rows / columns masked.  Writes (a, b) and its transpose at
+// (b, a); inactive coarse columns get zero rows / columns and a unit diagonal.
+template <class T>
+__global__ __launch_bounds__(CB_BS) void coarse_assemble_kernel(CoarseArgs c, CoarseBuild b) {
+  __shared__ double red[CB_BS / 64][36];
+  const int cb = blockIdx.x;
+  const int ca = b.cb_a[cb], cbb = b.cb_b[cb];
+  const T* __restrict__ bval = reinterpret_cast<const T*>(b.bval);
+  const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
+  double acc[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) acc[e] = 0.0;
+  const int lo = b.cb_ptr[cb], hi = b.cb_ptr[cb + 1];
+  for (int t = lo + threadIdx.x; t < hi; t += CB_BS) {
+    const int f = b.cb_fine[t];
+    const int i = b.brow[f], j = b.bcol[f];
+    const unsigned fi = c.fixmask[i], fj = c.fixmask[j];
+    double B[9];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        B[3 * p + q] = (((fi >> p) | (fj >> q)) & 1u) ? 0.0 : (double)bval[9L * f + 3 * p + q];
+    const double ri[3] = {(double)rel[3L * i], (double)rel[3L * i + 1], (double)rel[3L * i + 2]};
+    const double rj[3] = {(double)rel[3L * j], (double)rel[3L * j + 1], (double)rel[3L * j + 2]};
+    double Wi[9], Wj[9];
+    rot_block(ri, Wi);
+    rot_block(rj, Wj);
+    double BW[9];  // B W_j
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        BW[3 * p + q] = B[3 * p] * Wj[q] + B[3 * p + 1] * Wj[3 + q] + B[3 * p + 2] * Wj[6 + q];
+    // rows 0-2: [B, B W_j]; rows 3-5: [W_i^T B, W_i^T B W_j]
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        acc[6 * p + q] += B[3 * p + q];
+        acc[6 * p + 3 + q] += BW[3 * p + q];
+        acc[6 * (3 + p) + q] +=
+            Wi[p] * B[q] + Wi[3 + p] * B[3 + q] + Wi[6 + p] * B[6 + q];
+        acc[6 * (3 + p) + 3 + q] +=
+            Wi[p] * BW[q] + Wi[3 + p] * BW[3 + q] + Wi[6 + p] * BW[6 + q];
+      }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int e = 0; e < 36; ++e) {
+    const double v = wave_sum(acc[e]);
+    if (lane == 0) red[w][e] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 36) {
+    const int e = threadIdx.x, p = e / 6, q = e % 6;
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < CB_BS / 64; ++k) v += red[k][e];
+    const bool act_p = (c.cmask[ca] >> p) & 1u, act_q = (c.cmask[cbb] >> q) & 1u;
+    if (!act_p || !act_q) v = (ca == cbb && p == q) ? 1.0 : 0.0;
+    const long rI = 6L * ca + p, cJ = 6L * cbb + q;
+    b.A0[rI * c.npad + cJ] = v;
+    if (ca != cbb) b.A0[cJ * c.npad + rI] = v;
+  }
+}
+
+// ---- block Gauss-Jordan sweeps -----------------------------------------------------
+
+// In-place sweep of the 32x32 matrix M (LDS, row-major) over all its pivots:
+// leaves -M^-1.  Every thread of the workgroup takes 4 entries per pivot.
+// Returns false when a pivot is not positive relative to its starting
+// diagonal value (all threads agree).
+__device__ bool sweep_tile(double* M, double* diag0) {
+  const int t = threadIdx.x;
+  if (t < TL) diag0[t] = M[t * TL + t];
+  __syncthreads();
+  bool good = true;
+  for (int p = 0; p < TL; ++p) {
+    double d = M[p * TL + p];
+    if (!(d > 1e-13 * fabs(diag0[p]) && d > 0.0)) {
+      good = false;
+      d = 1.0;  // keeps the remaining arithmetic finite; the result is flagged
+    }
+    const double inv = 1.0 / d;
+    double mij[4], mip[4], mpj[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = t + GJ_BS * q, i = e / TL, j = e % TL;
+      mij[q] = M[e];
+      mip[q] = M[i * TL + p];
+      mpj[q] = M[p * TL + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = t + GJ_BS * q, i = e / TL, j = e % TL;
+      double v;
+      if (i == p && j == p) v = -inv;
+      else if (i == p) v = mpj[q] * inv;
+      else if (j == p) v = mip[q] * inv;
+      else v = mij[q] - mip[q] * mpj[q] * inv;
+      M[e] = v;
+    }
+    __syncthreads();
+  }
+  return good;
+}
+
+__device__ __forceinline__ void load_tile(const double* __restrict__ A, int npad, int I, int J,
+                                          double* T) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + GJ_BS * q, i = e / TL, j = e % TL;
+    T[e] = A[(long)(I * TL + i) * npad + J * TL + j];
+  }
+}
+
+// C = X Y for 32x32 tiles in LDS (4 entries per thread: row t / 8, columns
+// t % 8 + 8 q); C may not alias X or Y
+__device__ __forceinline__ void tile_mul(const double* X, const double* Y, double out[4]) {
+  const int r = threadIdx.x >> 3, c0 = threadIdx.x & 7;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll 8
+  for (int m = 0; m < TL; ++m) {
+    const double x = X[r * TL + m];
+    a0 += x * Y[m * TL + c0];
+    a1 += x * Y[m * TL + c0 + 8];
+    a2 += x * Y[m * TL + c0 + 16];
+    a3 += x * Y[m * TL + c0 + 24];
+  }
+  out[0] = a0;
+  out[1] = a1;
+  out[2] = a2;
+  out[3] = a3;
+}
+
+// D0 = inverse of the first pivot tile
+__global__ __launch_bounds__(GJ_BS) void coarse_pivot0_kernel(int npad, const double* A,
+                                                              double* D, int* ok) {
+  __shared__ double M[TL * TL];
+  __shared__ double d0[TL];
+  load_tile(A, npad, 0, 0, M);
+  __syncthreads();
+  const bool good = sweep_tile(M, d0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + GJ_BS * q;
+    D[e] = -M[e];
+  }
+  if (!good && threadIdx.x == 0) *ok = 0;
+}
+
+// Sweep of pivot tile k for tile (I, J) = (blockIdx.y, blockIdx.x), src ->
+// dst, with Dk = A_kk^-1:
+//   (k, k): -Dk;  (k, J): Dk A_kJ;  (I, k): A_Ik Dk;
+//   else:   A_IJ - (A_Ik Dk) A_kJ.
+// The workgroup of tile (k+1, k+1) then inverts its new tile: the next pivot.
+__global__ __launch_bounds__(GJ_BS) void coarse_sweep_kernel(int npad, int k,
+                                                             const double* __restrict__ src,
+                                                             double* __restrict__ dst,
+                                                             const double* __restrict__ Dk,
+                                                             double* __restrict__ Dnext, int* ok) {
+  __shared__ double sD[TL * TL];
+  __shared__ double sX[TL * TL];
+  __shared__ double sY[TL * TL];
+  __shared__ double sC[TL * TL];
+  __shared__ double d0[TL];
+  const int I = blockIdx.y, J = blockIdx.x;
+  const int r = threadIdx.x >> 3, c0 = threadIdx.x & 7;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + GJ_BS * q;
+    sD[e] = Dk[e];
+  }
+  double out[4];
+  if (I == k && J == k) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[q] = -sD[r * TL + c0 + 8 * q];
+  } else if (I == k) {
+    load_tile(src, npad, k, J, sY);
+    __syncthreads();
+    tile_mul(sD, sY, out);
+  } else if (J == k) {
+    load_tile(src, npad, I, k, sX);
+    __syncthreads();
+    tile_mul(sX, sD, out);
+  } else {
+    load_tile(src, npad, I, k, sX);
+    load_tile(src, npad, k, J, sY);
+    __syncthreads();
+    tile_mul(sX, sD, out);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sC[r * TL + c0 + 8 * q] = out[q];
+    __syncthreads();
+    tile_mul(sC, sY, out);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = r, j = c0 + 8 * q;
+      out[q] = src[(long)(I * TL + i) * npad + J * TL + j] - out[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = r, j = c0 + 8 * q;
+    dst[(long)(I * TL + i) * npad + J * TL + j] = out[q];
+  }
+  if (I == k + 1 && J == k + 1) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sX[r * TL + c0 + 8 * q] = out[q];
+    __syncthreads();
+    const bool good = sweep_tile(sX, d0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = threadIdx.x + GJ_BS * q;
+      Dnext[e] = -sX[e];
+    }
+    if (!good && threadIdx.x == 0) *ok = 0;
+  }
+}
+
+// R := (R + R^T) / 2 (the sweeps' rounding leaves the inverse symmetric only
+// to rounding; the preconditioner must be exactly symmetric)
+__global__ __launch_bounds__(256) void coarse_symmetrize_kernel(int npad, double* R) {
+  const long total = (long)npad * npad;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long i = e / npad, j = e % npad;
+    if (i < j) {
+      const double v = 0.5 * (R[e] + R[j * npad + i]);
+      R[e] = v;
+      R[j * npad + i] = v;
+    }
+  }
+}
+
+// ---- per PCG iteration ------------------------------------------------------------
+
+// Segment partials of P^T r: translations sum F_i r_i, rotations sum
+// rel_i x F_i r_i, over the segment's nodes in order.
+template <class T>
+__global__ __launch_bounds__(RS_BS) void coarse_restrict_kernel(PcgArgs A) {
+  __shared__ double red[RS_BS / 64][6];
+  if (A.sc->done) return;
+  const CoarseArgs& c = A.cg;
+  const T* __restrict__ r = reinterpret_cast<const T*>(A.r);
+  const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
+  const int s = blockIdx.x;
+  const int lo = c.seg_lo[s], hi = c.seg_lo[s + 1];
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t = lo + threadIdx.x; t < hi; t += RS_BS) {
+    const int i = c.agg_nodes[t];
+    const unsigned fm = c.fixmask[i];
+    double v[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) v[p] = ((fm >> p) & 1u) ? 0.0 : (double)r[3L * i + p];
+    const double x = rel[3L * i], y = rel[3L * i + 1], z = rel[3L * i + 2];
+    acc[0] += v[0];
+    acc[1] += v[1];
+    acc[2] += v[2];
+    acc[3] += y * v[2] - z * v[1];
+    acc[4] += z * v[0] - x * v[2];
+    acc[5] += x * v[1] - y * v[0];
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const double v = wave_sum(acc[e]);
+    if (lane == 0) red[w][e] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int e = threadIdx.x;
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < RS_BS / 64; ++k) v += red[k][e];
+    if (!((c.cmask[c.seg_agg[s]] >> e) & 1u)) v = 0.0;
+    c.spart[6L * s + e] = v;
+  }
+}
+
+// uc = Ac^-1 rc (R holds -Ac^-1) for 16 rows per workgroup; rc summed from
+// the segment partials by every workgroup (same order everywhere); one
+// partial of rc.uc per workgroup.  uc = 0 when the sweeps failed.
+__global__ __launch_bounds__(GV_BS) void coarse_gemv_kernel(PcgArgs A) {
+  __shared__ double rc[COARSE_MAX_PAD];
+  __shared__ double red[GV_BS / 64];
+  if (A.sc->done) return;
+  const CoarseArgs& c = A.cg;
+  for (int j = threadIdx.x; j < c.npad; j += GV_BS) {
+    const int a = j / 6, e = j % 6;
+    double v = 0.0;
+    if (a < c.na)
+      for (int s = c.agg_seg0[a]; s < c.agg_seg0[a + 1]; ++s) v += c.spart[6L * s + e];
+    rc[j] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool good = *c.ok != 0;
+  double dl = 0.0;
+#pragma unroll
+  for (int q = 0; q < GV_ROWS / (GV_BS / 64); ++q) {
+    const int row = blockIdx.x * GV_ROWS + w * (GV_ROWS / (GV_BS / 64)) + q;
+    const double* Rr = c.R + (long)row * c.npad;
+    double v = 0.0;
+    for (int j = lane; j < c.npad; j += 64) v += Rr[j] * rc[j];
+    v = wave_sum(v);
+    if (lane == 0) {
+      const double u = good ? -v : 0.0;
+      c.uc[row] = u;
+      dl += rc[row] * u;
+    }
+  }
+  const double sdl = block_sum<GV_BS>(dl, red);
+  if (threadIdx.x == 0) c.cpart[blockIdx.x] = sdl;
+}
+
+// z_i += F_i (t_a + w_a x rel_i); block 0 also sums the rc.uc partials
+template <class T>
+__global__ __launch_bounds__(PR_BS) void coarse_prolong_kernel(PcgArgs A) {
+  if (A.sc->done) return;
+  const CoarseArgs& c = A.cg;
+  T* __restrict__ z = reinterpret_cast<T*>(A.z);
+  const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double v = 0.0;
+    for (int g = 0; g < c.gemv_g; ++g) v += c.cpart[g];
+    *c.cdot = v;
+  }
+  for (long i = (long)blockIdx.x * PR_BS + threadIdx.x; i < A.n; i += (long)gridDim.x * PR_BS) {
+    const int a = c.agg_of[i];
+    const double* u = c.uc + 6L * a;
+    const double x = rel[3L * i], y = rel[3L * i + 1], zz = rel[3L * i + 2];
+    const double d[3] = {u[0] + u[4] * zz - u[5] * y, u[1] + u[5] * x - u[3] * zz,
+                         u[2] + u[3] * y - u[4] * x};
+    const unsigned fm = c.fixmask[i];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+      if (!((fm >> p) & 1u)) z[3 * i + p] = (T)((double)z[3 * i + p] + d[p]);
+  }
+}
+
+}  // namespace
+
+double* launch_coarse_operator(hipStream_t s, bool fp32, const CoarseArgs& c,
+                               const CoarseBuild& b, bool assemble_only) {
+  const int nt = c.npad / TL;
+  const int fg = (int)std::min<long>(((long)c.npad * c.npad + 255) / 256, 1024);
+  hipLaunchKernelGGL(coarse_fill_kernel, dim3(fg), dim3(256), 0, s, c.npad, 6 * c.na, b.A0,
+                     b.ok);
+  if (b.ncb > 0) {
+    if (fp32)
+      hipLaunchKernelGGL(coarse_assemble_kernel<float>, dim3(b.ncb), dim3(CB_BS), 0, s, c, b);
+    else
+      hipLaunchKernelGGL(coarse_assemble_kernel<double>, dim3(b.ncb), dim3(CB_BS), 0, s, c, b);
+  }
+  if (assemble_only) return b.A0;
+  hipLaunchKernelGGL(coarse_pivot0_kernel, dim3(1), dim3(GJ_BS), 0, s, c.npad, b.A0, b.D, b.ok);
+  double* bufs[2] = {b.A0, b.A1};
+  for (int k = 0; k < nt; ++k)
+    hipLaunchKernelGGL(coarse_sweep_kernel, dim3(nt, nt), dim3(GJ_BS), 0, s, c.npad, k,
+                       bufs[k & 1], bufs[(k + 1) & 1], b.D + (k & 1) * TL * TL,
+                       b.D + ((k + 1) & 1) * TL * TL, b.ok);
+  double* R = bufs[nt & 1];
+  hipLaunchKernelGGL(coarse_symmetrize_kernel, dim3(fg), dim3(256), 0, s, c.npad, R);
+  return R;
+}
+
+void launch_coarse_apply(hipStream_t s, bool fp32, const PcgArgs& a) {
+  const CoarseArgs& c = a.cg;
+  const int pg = (int)std::min<long>((a.n + PR_BS - 1) / PR_BS, 1024);
+  if (fp32) {
+    hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.nseg), dim3(RS_BS), 0, s, a);
+    hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.gemv_g), dim3(GV_BS), 0, s, a);
+    hipLaunchKernelGGL(coarse_prolong_kernel<float>, dim3(pg), dim3(PR_BS), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.nseg), dim3(RS_BS), 0, s, a);
+    hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.gemv_g), dim3(GV_BS), 0, s, a);
+    hipLaunchKernelGGL(coarse_prolong_kernel<double>, dim3(pg), dim3(PR_BS), 0, s, a);
+  }
+}
+
+int coarse_gemv_grid(int npad) { return npad / GV_ROWS; }
+
+}  // namespace pgn

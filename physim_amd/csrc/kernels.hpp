@@ -162,6 +162,39 @@ struct MatFreeArgs {
 void launch_matfree_product(hipStream_t s, bool fp32, const MatFreeArgs& a, const void* p,
                             void* q, double* part, int grid);
 
+// Two-level preconditioner (precond 2): block Jacobi plus a coarse
+// correction, M^-1 r = B^-1 r + P Ac^-1 P^T r.  The coarse space is spanned,
+// per aggregate of nodes (rest positions binned into boxes), by the three
+// translations and the three infinitesimal rotations about the aggregate's
+// centroid, restricted to the free DoFs: node i of aggregate a contributes
+// P_i = F_i [I | W_i] with W_i w = w x rel_i (rel_i = X0_i - centroid_a) and
+// F_i zeroing its fixed components.  Ac = P^T A P (6 x 6 blocks, FP64) is
+// inverted densely on the device by block Gauss-Jordan sweeps; its inverse is
+// applied to the restricted residual each PCG iteration (coarse_kernels.hip).
+constexpr int COARSE_TILE = 32;     // Gauss-Jordan tile
+constexpr int COARSE_MAX_PAD = 2048;  // coarse DoFs, padded to the tile, at most
+constexpr int COARSE_SEG = 1024;    // nodes per restriction segment at most
+struct CoarseArgs {
+  int na = 0;                  // aggregates
+  int npad = 0;                // 6 na padded to a multiple of COARSE_TILE
+  int nseg = 0;                // restriction segments (one aggregate each)
+  int gemv_g = 0;              // workgroups of the coarse product
+  const int* seg_agg = nullptr;   // nseg
+  const int* seg_lo = nullptr;    // nseg + 1: ranges into agg_nodes
+  const int* agg_seg0 = nullptr;  // na + 1: segments of each aggregate
+  const int* agg_nodes = nullptr; // n: nodes ordered by aggregate, ascending
+  const int* agg_of = nullptr;    // n
+  const void* rel = nullptr;      // 3n, Hessian precision
+  const uint8_t* cmask = nullptr; // na: active coarse columns (bits 0-5)
+  const uint8_t* fixmask = nullptr;
+  const double* R = nullptr;      // npad x npad: -Ac^-1
+  const int* ok = nullptr;        // 0: the sweeps met a non-positive pivot
+  double* spart = nullptr;        // nseg x 6 restriction partials
+  double* uc = nullptr;           // npad: Ac^-1 P^T r
+  double* cpart = nullptr;        // gemv_g partials of (P^T r).uc
+  double* cdot = nullptr;         // their sum: r.(P uc) of the last application
+};
+
 struct PcgArgs {
   int n;                 // block rows (nodes)
   const long* rowptr;    // n+1
@@ -210,6 +243,10 @@ struct PcgArgs {
   // split form only: the SpMV launch replaced by the matrix-free product
   int mf = 0;
   MatFreeArgs mfa;
+  // Chronopoulos-Gear form only: the coarse correction follows every z =
+  // B^-1 r (init and update launches); r.z adds cg.cdot
+  int coarse = 0;
+  CoarseArgs cg;
 };
 int pcg_grid1(int n, int lpr);
 // SELL-64 layout: slices per workgroup and grid (<= PCG_MAXP) for ns slices
@@ -377,5 +414,33 @@ void launch_ldlt_factor(hipStream_t s, const LdltArgs& a, const int* list, int n
 // vectors); y: scratch of the problems' DoF count
 void launch_ldlt_solve(hipStream_t s, const LdltArgs& a, const int* list, int nlist,
                        const double* b, double* x, double* y);
+
+// ---- two-level preconditioner: coarse operator and its application ----------
+// Pattern of Ac = P^T A P: coarse block pairs (a, b), a <= b, each with the
+// BSR3 blocks (i in a, j in b) that sum into it, in block order.
+struct CoarseBuild {
+  int ncb;                // coarse block pairs
+  const int* cb_a;        // ncb
+  const int* cb_b;        // ncb
+  const int* cb_ptr;      // ncb + 1
+  const int* cb_fine;     // BSR3 block indexes
+  const int* brow;        // BSR3 block row / column (nodes)
+  const int* bcol;
+  const void* bval;       // BSR3 values, Hessian precision
+  double* A0;             // npad x npad: Ac, then the sweeps' ping-pong pair
+  double* A1;
+  double* D;              // 2 x 32 x 32: inverses of the next pivot tiles
+  int* ok;
+};
+// Ac into A0 (dense FP64: inactive and padding DoFs get identity rows), then
+// -Ac^-1 by block Gauss-Jordan sweeps; returns the buffer that holds it.
+// ok is cleared when a pivot is not positive (Ac not SPD to FP64 precision).
+double* launch_coarse_operator(hipStream_t s, bool fp32, const CoarseArgs& c,
+                               const CoarseBuild& b, bool assemble_only);
+// z += P Ac^-1 P^T r after z = B^-1 r, and cdot = (P^T r).(Ac^-1 P^T r);
+// returns at once when the PCG has converged
+void launch_coarse_apply(hipStream_t s, bool fp32, const PcgArgs& a);
+// workgroups of the coarse product for npad coarse DoFs
+int coarse_gemv_grid(int npad);
 
 }  // namespace pgn

@@ -572,7 +572,13 @@ PcgArgs pgn_model::pcg_args(double tol, int precond) const {
   a.bval = bval.p;
   a.dinv = dinv.p;
   a.binv = binv.p;
-  a.precond = precond;
+  // precond 2: block Jacobi in the update launch plus the coarse correction
+  a.precond = precond >= 1 ? 1 : 0;
+  if (precond == 2) {
+    PGN_CHECK(coarse_ready && cga.R, "two-level preconditioner: coarse operator not built");
+    a.coarse = 1;
+    a.cg = cga;
+  }
   a.x = base;
   a.r = base + v;
   a.z = base + 2 * v;

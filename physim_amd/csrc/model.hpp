@@ -113,6 +113,26 @@ struct pgn_model {
   bool tuned = false;
   double tune_us[3] = {0, 0, 0};  // SpMV time for 4, 8, 16 lanes per row
   pgn::DBuf<pgn::PcgScalars> sc;
+  // Two-level preconditioner (precond 2): aggregates of nodes binned by rest
+  // position (about coarse_target of them), built on the first solve that
+  // asks for it and again after the fixed DoFs change; the coarse operator is
+  // assembled and inverted before every solve try (coarse.cpp)
+  bool coarse_ready = false;
+  int coarse_target = 128;
+  int coarse_bins[3] = {1, 1, 1};
+  pgn::CoarseArgs cga;  // device pointers of the application (R set per build)
+  int coarse_ncb = 0;
+  pgn::DBuf<int> cg_seg_agg, cg_seg_lo, cg_agg_seg0, cg_agg_nodes, cg_agg_of;
+  pgn::DBuf<int> cg_cb_a, cg_cb_b, cg_cb_ptr, cg_cb_fine;
+  pgn::DBuf<char> cg_rel;
+  pgn::DBuf<uint8_t> cg_cmask;
+  pgn::DBuf<double> cg_A, cg_D, cg_spart, cg_uc, cg_cpart, cg_cdot;
+  pgn::DBuf<int> cg_ok;
+  std::vector<int> h_agg_of;  // host copy of the aggregation
+  void coarse_prepare();
+  // assembles Ac from the current BSR3 values and inverts it (no sync);
+  // assemble_only leaves Ac itself in the first buffer (tests)
+  double* coarse_update(bool assemble_only = false);
   pgn::DBuf<double> r64, tmp64a, tmp64b;
   pgn::DBuf<double> nw_xsave, nw_v0, nw_dx, nw_b;  // Newton work vectors
   // LBFGS (QPSolverType::LBFGS): x and g of the last LBFGS step, a ring of

@@ -597,12 +597,14 @@ __global__ __launch_bounds__(SELL_BS) void cg1_sell_kernel(PcgArgs A, int k) {
   // the flag is read beside the loads above and the partial sums (all in
   // bounds whether or not the solve has ended): their latencies overlap
   const int done = sc->done;
+  const double cdot = A.coarse ? *A.cg.cdot : 0.0;  // r.z of the coarse correction
   const double* prev = rr_slot(A, k - 1);
   const int cprev = rr_count(A, k - 1);
   double rr, bb, gam;
   all_sum3<SELL_BS>(prev, cprev, 2, A.partb, k == 0 ? A.g1 : 0, 1, prev + 1, cprev, 2, red, rr,
                     bb, gam);
   if (done) return;
+  gam += cdot;
   if (k == 0) {
     double thr = A.tol * A.tol * bb;
     if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
@@ -742,12 +744,14 @@ __global__ __launch_bounds__(K2_BS) void cg1_b_kernel(PcgArgs A, int k) {
   long i0 = rg.lo + threadIdx.x;
   cg1_load<T, QF, NPT>(A, i0, rg.hi, b);
   const int done = A.sc->done;
+  const double cdot = (!QF && A.coarse) ? *A.cg.cdot : 0.0;
   double gam, del, unused;
   all_sum3<K2_BS>(rr_slot(A, k - 1) + 1, QF ? 0 : rr_count(A, k - 1), 2, A.part1, A.g1, 1,
                   A.part1, 0, 1, red, gam, del, unused);
   if (done) return;
   double alpha, beta;
   if (QF) gam = A.cgs[5];
+  else gam += cdot;
   if (k == 0) {
     beta = 0.0;
     alpha = gam / del;

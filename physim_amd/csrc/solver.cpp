@@ -20,6 +20,7 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
   hipStream_t s = M.stream();
   const PcgArgs a = M.pcg_args(tol, precond);
   launch_pcg_init(s, M.hfp32, a, M.pb.p, x_zero ? 1 : 0);
+  if (a.coarse) launch_coarse_apply(s, M.hfp32, a);
   if (M.profiling && M.prof_ev.size() < 4) {
     while (M.prof_ev.size() < 4) {
       hipEvent_t e;
@@ -44,6 +45,7 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
           launch_pcg_k1a(cs, M.hfp32, ga, j);
           launch_pcg_k1(cs, M.hfp32, ga, j, nullptr, nullptr);
           launch_pcg_k2(cs, M.hfp32, ga, j, nullptr, nullptr);
+          if (ga.coarse) launch_coarse_apply(cs, M.hfp32, ga);
         }
         launch_pcg_status(cs, ga, c, 0);
       });
@@ -63,6 +65,7 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
                       ev ? M.prof_ev[1] : nullptr);
         launch_pcg_k2(s, M.hfp32, a, k + j, ev ? M.prof_ev[2] : nullptr,
                       ev ? M.prof_ev[3] : nullptr);
+        if (a.coarse) launch_coarse_apply(s, M.hfp32, a);
       }
       k += c;
       launch_pcg_status(s, a, k, maxit);
@@ -155,6 +158,15 @@ void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opt
     return;
   }
   PGN_CHECK(M.hess_valid, "Hessian not assembled");
+  PGN_CHECK(o.precond >= 0 && o.precond <= 2,
+            "precond: 0 (Jacobi), 1 (block Jacobi) or 2 (two-level)");
+  const bool coarse = o.precond == 2;
+  if (coarse) {
+    PGN_CHECK(M.pcg_cg1 && !M.use_matfree,
+              "the two-level preconditioner needs the Chronopoulos-Gear PCG over the "
+              "assembled matrix (PGN_PCG and PGN_MATFREE unset)");
+    M.coarse_prepare();
+  }
   M.tune();
   if (M.use_matfree) M.mf_args();  // its element product buffer
   hipStream_t s = M.stream();
@@ -178,6 +190,7 @@ void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opt
     }
     PGN_HIP(hipMemcpyAsync(x, xprev, N * sizeof(double), hipMemcpyDeviceToDevice, s));
     M.sell_pack();  // the SpMV's copy of this try's matrix
+    if (coarse) M.coarse_update();  // and the coarse operator of this try
     int it = 0, refs = 0;
     double rel = 0.0;
     solve_once(M, b, x, o, it, refs, rel);
@@ -343,10 +356,18 @@ void newton_solve(pgn_model& M, const pgn_newton_opts& o, pgn_newton_result& R, 
       pgn_linear_result lr;
       linear_solve(M, bvec, dx, lo, lr);
       R.pcg_iterations += lr.iterations;
-      if (trace)
-        std::fprintf(stderr, "[pgn] newton %d |g| %.3e lin tol %.1e pcg %d refine %d reg %d relres %.2e\n",
+      if (trace) {
+        int cok = -1;  // two-level: the coarse operator of the last try was SPD
+        if (lo.precond == 2) {
+          PGN_HIP(hipMemcpyAsync(&cok, M.cg_ok.p, sizeof(int), hipMemcpyDeviceToHost, s));
+          M.sync();
+        }
+        std::fprintf(stderr,
+                     "[pgn] newton %d |g| %.3e lin tol %.1e pcg %d refine %d reg %d relres "
+                     "%.2e coarse %d\n",
                      itCount, opt, lo.tol, lr.iterations, lr.refinements, lr.reg_used,
-                     lr.rel_residual);
+                     lr.rel_residual, cok);
+      }
     } else {
       inexact = false;
       launch_lincomb64(s, N, -1.0, M.g.p, 0.0, M.g.p, dx);
