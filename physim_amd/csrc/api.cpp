@@ -234,9 +234,9 @@ size_t pgn_model_device_bytes(const pgn_model* M) {
                    M->lb_S, M->lb_Y, M->vel, M->xhat,
                    M->xstep0, M->ldlt_prob, M->ldlt_perm, M->ldlt_iperm, M->ldlt_nprob,
                    M->ldlt_list, M->ldlt_status, M->ldlt_band, M->ldlt_D, M->ldlt_sinT,
-                   M->ldlt_y, M->cg_seg_agg, M->cg_seg_lo, M->cg_agg_seg0, M->cg_agg_nodes,
-                   M->cg_agg_of, M->cg_cb_a, M->cg_cb_b, M->cg_cb_ptr, M->cg_cb_fine,
-                   M->cg_rel, M->cg_cmask, M->cg_A, M->cg_D, M->cg_spart, M->cg_uc,
+                   M->ldlt_y, M->cg_agg_ptr, M->cg_agg_nodes, M->cg_agg_of, M->cg_cb_a,
+                   M->cg_cb_b, M->cg_pair_piece0, M->cg_piece_lo, M->cg_cb_fine, M->cg_rel,
+                   M->cg_cmask, M->cg_A, M->cg_D, M->cg_piece_sum, M->cg_rc, M->cg_uc,
                    M->cg_cpart, M->cg_cdot, M->cg_ok);
 }
 

@@ -138,22 +138,12 @@ void pgn_model::coarse_prepare() {
   }
   std::vector<uint8_t> cmask(na);
   for (int a = 0; a < na; ++a) cmask[a] = independent_columns(&G[36L * a]);
-  // nodes by aggregate, restriction segments
+  // nodes by aggregate
   std::vector<int> agg_ptr(na + 1, 0);
   for (int i = 0; i < n; ++i) agg_ptr[h_agg_of[i] + 1]++;
   for (int a = 0; a < na; ++a) agg_ptr[a + 1] += agg_ptr[a];
   std::vector<int> nodes(n), cur(agg_ptr.begin(), agg_ptr.end() - 1);
   for (int i = 0; i < n; ++i) nodes[cur[h_agg_of[i]]++] = i;
-  std::vector<int> seg_agg, seg_lo, agg_seg0(na + 1, 0);
-  for (int a = 0; a < na; ++a) {
-    agg_seg0[a] = (int)seg_agg.size();
-    for (int t = agg_ptr[a]; t < agg_ptr[a + 1]; t += COARSE_SEG) {
-      seg_agg.push_back(a);
-      seg_lo.push_back(t);
-    }
-  }
-  agg_seg0[na] = (int)seg_agg.size();
-  seg_lo.push_back(n);
   // coarse block pairs (a <= b) and their BSR3 blocks in block order
   std::vector<int> pcount((size_t)na * na, 0);
   for (long f = 0; f < pat.nb; ++f) {
@@ -175,17 +165,24 @@ void pgn_model::coarse_prepare() {
     const int a = h_agg_of[pat.brow[f]], b = h_agg_of[pat.bcol[f]];
     if (a <= b) cb_fine[pos[pidx[(size_t)a * na + b]]++] = (int)f;
   }
+  // pieces of at most COARSE_PIECE blocks, pair by pair
+  std::vector<int> pair_piece0(cb_a.size() + 1, 0), piece_lo;
+  for (size_t c = 0; c < cb_a.size(); ++c) {
+    pair_piece0[c] = (int)piece_lo.size();
+    for (int t = cb_ptr[c]; t < cb_ptr[c + 1]; t += COARSE_PIECE) piece_lo.push_back(t);
+  }
+  pair_piece0[cb_a.size()] = (int)piece_lo.size();
+  piece_lo.push_back(cb_ptr.back());
   // device copies
   hipStream_t s = stream();
-  cg_seg_agg.upload(seg_agg, s);
-  cg_seg_lo.upload(seg_lo, s);
-  cg_agg_seg0.upload(agg_seg0, s);
+  cg_agg_ptr.upload(agg_ptr, s);
   cg_agg_nodes.upload(nodes, s);
   cg_agg_of.upload(h_agg_of, s);
   cg_cmask.upload(cmask, s);
   cg_cb_a.upload(cb_a, s);
   cg_cb_b.upload(cb_b, s);
-  cg_cb_ptr.upload(cb_ptr, s);
+  cg_pair_piece0.upload(pair_piece0, s);
+  cg_piece_lo.upload(piece_lo, s);
   cg_cb_fine.upload(cb_fine, s);
   std::vector<float> r32;  // alive until the sync below
   if (hfp32) {
@@ -198,7 +195,10 @@ void pgn_model::coarse_prepare() {
   cg_A.ensure(2L * npad * npad);
   cg_D.ensure(2L * COARSE_TILE * COARSE_TILE);
   cg_ok.ensure(1);
-  cg_spart.ensure(6L * seg_agg.size());
+  coarse_npiece = (int)piece_lo.size() - 1;
+  cg_piece_sum.ensure(36L * std::max(coarse_npiece, 1));
+  cg_rc.ensure(npad);
+  PGN_HIP(hipMemsetAsync(cg_rc.p, 0, npad * sizeof(double), s));  // padding stays zero
   cg_uc.ensure(npad);
   const int gg = coarse_gemv_grid(npad);
   cg_cpart.ensure(gg);
@@ -208,11 +208,8 @@ void pgn_model::coarse_prepare() {
   cga = CoarseArgs{};
   cga.na = na;
   cga.npad = npad;
-  cga.nseg = (int)seg_agg.size();
   cga.gemv_g = gg;
-  cga.seg_agg = cg_seg_agg.p;
-  cga.seg_lo = cg_seg_lo.p;
-  cga.agg_seg0 = cg_agg_seg0.p;
+  cga.agg_ptr = cg_agg_ptr.p;
   cga.agg_nodes = cg_agg_nodes.p;
   cga.agg_of = cg_agg_of.p;
   cga.rel = cg_rel.p;
@@ -220,7 +217,7 @@ void pgn_model::coarse_prepare() {
   cga.fixmask = fixmask.p;
   cga.R = nullptr;
   cga.ok = cg_ok.p;
-  cga.spart = cg_spart.p;
+  cga.rc = cg_rc.p;
   cga.uc = cg_uc.p;
   cga.cpart = cg_cpart.p;
   cga.cdot = cg_cdot.p;
@@ -232,10 +229,13 @@ double* pgn_model::coarse_update(bool assemble_only) {
   coarse_prepare();
   CoarseBuild b;
   b.ncb = coarse_ncb;
+  b.npiece = coarse_npiece;
   b.cb_a = cg_cb_a.p;
   b.cb_b = cg_cb_b.p;
-  b.cb_ptr = cg_cb_ptr.p;
+  b.pair_piece0 = cg_pair_piece0.p;
+  b.piece_lo = cg_piece_lo.p;
   b.cb_fine = cg_cb_fine.p;
+  b.piece_sum = cg_piece_sum.p;
   b.brow = brow.p;
   b.bcol = bcol.p;
   b.bval = bval.p;

@@ -3,13 +3,14 @@
 // M^-1 r = B^-1 r + P Ac^-1 P^T r: B is the 3x3 block-Jacobi part (applied by
 // the PCG update launch), P the aggregated rigid-body prolongation described
 // with CoarseArgs (kernels.hpp).  Per linear solve try this file builds
-// Ac = P^T A P from the assembled BSR3 values (FP64 sums, one workgroup per
-// coarse block pair, fixed order) and inverts it in place by block
-// Gauss-Jordan sweeps (the sweep operator on symmetric positive definite
-// matrices: after sweeping every pivot the matrix holds -Ac^-1; no pivoting
-// is needed since every pivot is a Schur complement of an SPD matrix).
-// Per PCG iteration it restricts r (segment partials), multiplies by the
-// inverse and prolongs into z.  Every sum has a fixed order: the results do
+// Ac = P^T A P from the assembled BSR3 values (FP64 sums over pieces of at
+// most COARSE_PIECE blocks of a coarse block pair, then the pieces in order)
+// and inverts it in place by block Gauss-Jordan sweeps (the sweep operator on
+// symmetric positive definite matrices: after sweeping every pivot the matrix
+// holds -Ac^-1; no pivoting is needed since every pivot is a Schur
+// complement of an SPD matrix).  Per PCG iteration it restricts r (one
+// workgroup per aggregate), multiplies by the inverse (one row per
+// wavefront) and prolongs into z (one node per thread).  Every sum has a fixed order: the results do
 // not depend on scheduling.
 #include <hip/hip_runtime.h>
 
@@ -24,10 +25,11 @@ namespace {
 constexpr int TL = COARSE_TILE;  // 32
 constexpr int CB_BS = 256;       // coarse assembly workgroup
 constexpr int GJ_BS = 256;       // sweep workgroup: 4 entries of a 32 x 32 tile each
-constexpr int RS_BS = 256;       // restriction workgroup
-constexpr int GV_BS = 256;       // coarse product workgroup: 16 rows
-constexpr int GV_ROWS = 16;
-constexpr int PR_BS = 256;       // prolongation workgroup
+constexpr int RS_BS = 1024;      // restriction workgroup: one aggregate
+constexpr int RS_U = 4;          // nodes in flight per thread
+constexpr int GV_BS = 256;       // coarse product workgroup: one row per wavefront
+constexpr int GV_ROWS = GV_BS / 64;
+constexpr int PR_BS = 256;       // prolongation workgroup: one node per thread
 
 // w x r as the 3x3 matrix W(r) acting on w: rows (0, rz, -ry), (-rz, 0, rx),
 // (ry, -rx, 0)
@@ -49,21 +51,19 @@ __global__ __launch_bounds__(256) void coarse_fill_kernel(int npad, int nact, do
   if (blockIdx.x == 0 && threadIdx.x == 0) *ok = 1;
 }
 
-// One workgroup per coarse block pair (a, b), a <= b: the 6x6 block
-// sum_{i in a, j in b} P_i^T A_ij P_j over the listed BSR3 blocks, with
-// A_ij's fixed rows / columns masked.  Writes (a, b) and its transpose at
-// (b, a); inactive coarse columns get zero rows / columns and a unit diagonal.
+// One workgroup per piece (at most COARSE_PIECE listed BSR3 blocks of one
+// coarse block pair (a, b), a <= b): the partial 6x6 block
+// sum P_i^T A_ij P_j over its blocks, with A_ij's fixed rows / columns masked.
 template <class T>
-__global__ __launch_bounds__(CB_BS) void coarse_assemble_kernel(CoarseArgs c, CoarseBuild b) {
+__global__ __launch_bounds__(CB_BS) void coarse_piece_kernel(CoarseArgs c, CoarseBuild b) {
   __shared__ double red[CB_BS / 64][36];
-  const int cb = blockIdx.x;
-  const int ca = b.cb_a[cb], cbb = b.cb_b[cb];
+  const int pc = blockIdx.x;
   const T* __restrict__ bval = reinterpret_cast<const T*>(b.bval);
   const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
   double acc[36];
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
-  const int lo = b.cb_ptr[cb], hi = b.cb_ptr[cb + 1];
+  const int lo = b.piece_lo[pc], hi = b.piece_lo[pc + 1];
   for (int t = lo + threadIdx.x; t < hi; t += CB_BS) {
     const int f = b.cb_fine[t];
     const int i = b.brow[f], j = b.bcol[f];
@@ -106,16 +106,29 @@ __global__ __launch_bounds__(CB_BS) void coarse_assemble_kernel(CoarseArgs c, Co
   }
   __syncthreads();
   if (threadIdx.x < 36) {
-    const int e = threadIdx.x, p = e / 6, q = e % 6;
+    const int e = threadIdx.x;
     double v = 0.0;
 #pragma unroll
     for (int k = 0; k < CB_BS / 64; ++k) v += red[k][e];
-    const bool act_p = (c.cmask[ca] >> p) & 1u, act_q = (c.cmask[cbb] >> q) & 1u;
-    if (!act_p || !act_q) v = (ca == cbb && p == q) ? 1.0 : 0.0;
-    const long rI = 6L * ca + p, cJ = 6L * cbb + q;
-    b.A0[rI * c.npad + cJ] = v;
-    if (ca != cbb) b.A0[cJ * c.npad + rI] = v;
+    b.piece_sum[36L * pc + e] = v;
   }
+}
+
+// One thread per (pair, entry): the pair's pieces summed in order, written
+// at (a, b) and transposed at (b, a); inactive coarse columns get zero rows
+// and columns and a unit diagonal.
+__global__ __launch_bounds__(256) void coarse_pairs_kernel(CoarseArgs c, CoarseBuild b) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= 36L * b.ncb) return;
+  const int cb = (int)(t / 36), e = (int)(t % 36), p = e / 6, q = e % 6;
+  const int ca = b.cb_a[cb], cbb = b.cb_b[cb];
+  double v = 0.0;
+  for (int k = b.pair_piece0[cb]; k < b.pair_piece0[cb + 1]; ++k) v += b.piece_sum[36L * k + e];
+  const bool act_p = (c.cmask[ca] >> p) & 1u, act_q = (c.cmask[cbb] >> q) & 1u;
+  if (!act_p || !act_q) v = (ca == cbb && p == q) ? 1.0 : 0.0;
+  const long rI = 6L * ca + p, cJ = 6L * cbb + q;
+  b.A0[rI * c.npad + cJ] = v;
+  if (ca != cbb) b.A0[cJ * c.npad + rI] = v;
 }
 
 // ---- block Gauss-Jordan sweeps -----------------------------------------------------
@@ -290,8 +303,9 @@ __global__ __launch_bounds__(256) void coarse_symmetrize_kernel(int npad, double
 
 // ---- per PCG iteration ------------------------------------------------------------
 
-// Segment partials of P^T r: translations sum F_i r_i, rotations sum
-// rel_i x F_i r_i, over the segment's nodes in order.
+// rc = P^T r for one aggregate per workgroup: translations sum F_i r_i,
+// rotations sum rel_i x F_i r_i, over the aggregate's nodes in order (RS_U
+// nodes per thread in flight); dropped columns get zero.
 template <class T>
 __global__ __launch_bounds__(RS_BS) void coarse_restrict_kernel(PcgArgs A) {
   __shared__ double red[RS_BS / 64][6];
@@ -299,22 +313,41 @@ __global__ __launch_bounds__(RS_BS) void coarse_restrict_kernel(PcgArgs A) {
   const CoarseArgs& c = A.cg;
   const T* __restrict__ r = reinterpret_cast<const T*>(A.r);
   const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
-  const int s = blockIdx.x;
-  const int lo = c.seg_lo[s], hi = c.seg_lo[s + 1];
+  const int a = blockIdx.x;
+  const int lo = c.agg_ptr[a], hi = c.agg_ptr[a + 1];
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int t = lo + threadIdx.x; t < hi; t += RS_BS) {
-    const int i = c.agg_nodes[t];
-    const unsigned fm = c.fixmask[i];
-    double v[3];
+  for (int t0 = lo + threadIdx.x; t0 < hi; t0 += RS_BS * RS_U) {
+    int idx[RS_U];
 #pragma unroll
-    for (int p = 0; p < 3; ++p) v[p] = ((fm >> p) & 1u) ? 0.0 : (double)r[3L * i + p];
-    const double x = rel[3L * i], y = rel[3L * i + 1], z = rel[3L * i + 2];
-    acc[0] += v[0];
-    acc[1] += v[1];
-    acc[2] += v[2];
-    acc[3] += y * v[2] - z * v[1];
-    acc[4] += z * v[0] - x * v[2];
-    acc[5] += x * v[1] - y * v[0];
+    for (int u = 0; u < RS_U; ++u) {
+      const int t = t0 + u * RS_BS;
+      idx[u] = t < hi ? c.agg_nodes[t] : -1;
+    }
+    T rv[RS_U][3], rl[RS_U][3];
+    unsigned fm[RS_U];
+#pragma unroll
+    for (int u = 0; u < RS_U; ++u) {
+      const long i = idx[u] < 0 ? 0 : idx[u];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        rv[u][p] = r[3 * i + p];
+        rl[u][p] = rel[3 * i + p];
+      }
+      fm[u] = idx[u] < 0 ? 7u : (unsigned)c.fixmask[i];
+    }
+#pragma unroll
+    for (int u = 0; u < RS_U; ++u) {
+      double v[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) v[p] = ((fm[u] >> p) & 1u) ? 0.0 : (double)rv[u][p];
+      const double x = rl[u][0], y = rl[u][1], z = rl[u][2];
+      acc[0] += v[0];
+      acc[1] += v[1];
+      acc[2] += v[2];
+      acc[3] += y * v[2] - z * v[1];
+      acc[4] += z * v[0] - x * v[2];
+      acc[5] += x * v[1] - y * v[0];
+    }
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -328,70 +361,68 @@ __global__ __launch_bounds__(RS_BS) void coarse_restrict_kernel(PcgArgs A) {
     double v = 0.0;
 #pragma unroll
     for (int k = 0; k < RS_BS / 64; ++k) v += red[k][e];
-    if (!((c.cmask[c.seg_agg[s]] >> e) & 1u)) v = 0.0;
-    c.spart[6L * s + e] = v;
+    if (!((c.cmask[a] >> e) & 1u)) v = 0.0;
+    c.rc[6L * a + e] = v;
   }
 }
 
-// uc = Ac^-1 rc (R holds -Ac^-1) for 16 rows per workgroup; rc summed from
-// the segment partials by every workgroup (same order everywhere); one
-// partial of rc.uc per workgroup.  uc = 0 when the sweeps failed.
+// uc = Ac^-1 rc (R holds -Ac^-1), one row per wavefront; one partial of
+// rc.uc per workgroup.  uc = 0 when the sweeps failed.
 __global__ __launch_bounds__(GV_BS) void coarse_gemv_kernel(PcgArgs A) {
-  __shared__ double rc[COARSE_MAX_PAD];
   __shared__ double red[GV_BS / 64];
   if (A.sc->done) return;
   const CoarseArgs& c = A.cg;
-  for (int j = threadIdx.x; j < c.npad; j += GV_BS) {
-    const int a = j / 6, e = j % 6;
-    double v = 0.0;
-    if (a < c.na)
-      for (int s = c.agg_seg0[a]; s < c.agg_seg0[a + 1]; ++s) v += c.spart[6L * s + e];
-    rc[j] = v;
-  }
-  __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row = blockIdx.x * GV_ROWS + w;
+  const double* __restrict__ Rr = c.R + (long)row * c.npad;
+  const double* __restrict__ rc = c.rc;
   const bool good = *c.ok != 0;
+  double v0 = 0.0, v1 = 0.0;
+  int j = lane;
+  for (; j + 64 < c.npad; j += 128) {
+    v0 += Rr[j] * rc[j];
+    v1 += Rr[j + 64] * rc[j + 64];
+  }
+  if (j < c.npad) v0 += Rr[j] * rc[j];
+  const double v = wave_sum(v0 + v1);
   double dl = 0.0;
-#pragma unroll
-  for (int q = 0; q < GV_ROWS / (GV_BS / 64); ++q) {
-    const int row = blockIdx.x * GV_ROWS + w * (GV_ROWS / (GV_BS / 64)) + q;
-    const double* Rr = c.R + (long)row * c.npad;
-    double v = 0.0;
-    for (int j = lane; j < c.npad; j += 64) v += Rr[j] * rc[j];
-    v = wave_sum(v);
-    if (lane == 0) {
-      const double u = good ? -v : 0.0;
-      c.uc[row] = u;
-      dl += rc[row] * u;
-    }
+  if (lane == 0) {
+    const double u = good ? -v : 0.0;
+    c.uc[row] = u;
+    dl = rc[row] * u;
   }
   const double sdl = block_sum<GV_BS>(dl, red);
   if (threadIdx.x == 0) c.cpart[blockIdx.x] = sdl;
 }
 
-// z_i += F_i (t_a + w_a x rel_i); block 0 also sums the rc.uc partials
+// z_i += F_i (t_a + w_a x rel_i), one node per thread; block 0 also sums
+// the rc.uc partials
 template <class T>
 __global__ __launch_bounds__(PR_BS) void coarse_prolong_kernel(PcgArgs A) {
+  __shared__ double red[PR_BS / 64];
   if (A.sc->done) return;
   const CoarseArgs& c = A.cg;
+  if (blockIdx.x == 0) {
+    double v = 0.0;
+    for (int g = threadIdx.x; g < c.gemv_g; g += PR_BS) v += c.cpart[g];
+    v = block_sum<PR_BS>(v, red);
+    if (threadIdx.x == 0) *c.cdot = v;
+  }
+  const long i = (long)blockIdx.x * PR_BS + threadIdx.x;
+  if (i >= A.n) return;
   T* __restrict__ z = reinterpret_cast<T*>(A.z);
   const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double v = 0.0;
-    for (int g = 0; g < c.gemv_g; ++g) v += c.cpart[g];
-    *c.cdot = v;
-  }
-  for (long i = (long)blockIdx.x * PR_BS + threadIdx.x; i < A.n; i += (long)gridDim.x * PR_BS) {
-    const int a = c.agg_of[i];
-    const double* u = c.uc + 6L * a;
-    const double x = rel[3L * i], y = rel[3L * i + 1], zz = rel[3L * i + 2];
-    const double d[3] = {u[0] + u[4] * zz - u[5] * y, u[1] + u[5] * x - u[3] * zz,
-                         u[2] + u[3] * y - u[4] * x};
-    const unsigned fm = c.fixmask[i];
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-      if (!((fm >> p) & 1u)) z[3 * i + p] = (T)((double)z[3 * i + p] + d[p]);
-  }
+  const int a = c.agg_of[i];
+  const unsigned fm = c.fixmask[i];
+  const T z0 = z[3 * i], z1 = z[3 * i + 1], z2 = z[3 * i + 2];
+  const double x = rel[3 * i], y = rel[3 * i + 1], zz = rel[3 * i + 2];
+  const double* u = c.uc + 6L * a;
+  const double d0 = u[0] + u[4] * zz - u[5] * y;
+  const double d1 = u[1] + u[5] * x - u[3] * zz;
+  const double d2 = u[2] + u[3] * y - u[4] * x;
+  if (!(fm & 1u)) z[3 * i] = (T)((double)z0 + d0);
+  if (!(fm & 2u)) z[3 * i + 1] = (T)((double)z1 + d1);
+  if (!(fm & 4u)) z[3 * i + 2] = (T)((double)z2 + d2);
 }
 
 }  // namespace
@@ -402,11 +433,13 @@ double* launch_coarse_operator(hipStream_t s, bool fp32, const CoarseArgs& c,
   const int fg = (int)std::min<long>(((long)c.npad * c.npad + 255) / 256, 1024);
   hipLaunchKernelGGL(coarse_fill_kernel, dim3(fg), dim3(256), 0, s, c.npad, 6 * c.na, b.A0,
                      b.ok);
-  if (b.ncb > 0) {
+  if (b.npiece > 0) {
     if (fp32)
-      hipLaunchKernelGGL(coarse_assemble_kernel<float>, dim3(b.ncb), dim3(CB_BS), 0, s, c, b);
+      hipLaunchKernelGGL(coarse_piece_kernel<float>, dim3(b.npiece), dim3(CB_BS), 0, s, c, b);
     else
-      hipLaunchKernelGGL(coarse_assemble_kernel<double>, dim3(b.ncb), dim3(CB_BS), 0, s, c, b);
+      hipLaunchKernelGGL(coarse_piece_kernel<double>, dim3(b.npiece), dim3(CB_BS), 0, s, c, b);
+    const int pg = (int)((36L * b.ncb + 255) / 256);
+    hipLaunchKernelGGL(coarse_pairs_kernel, dim3(pg), dim3(256), 0, s, c, b);
   }
   if (assemble_only) return b.A0;
   hipLaunchKernelGGL(coarse_pivot0_kernel, dim3(1), dim3(GJ_BS), 0, s, c.npad, b.A0, b.D, b.ok);
@@ -422,13 +455,13 @@ double* launch_coarse_operator(hipStream_t s, bool fp32, const CoarseArgs& c,
 
 void launch_coarse_apply(hipStream_t s, bool fp32, const PcgArgs& a) {
   const CoarseArgs& c = a.cg;
-  const int pg = (int)std::min<long>((a.n + PR_BS - 1) / PR_BS, 1024);
+  const int pg = (a.n + PR_BS - 1) / PR_BS;
   if (fp32) {
-    hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.nseg), dim3(RS_BS), 0, s, a);
+    hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.na), dim3(RS_BS), 0, s, a);
     hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.gemv_g), dim3(GV_BS), 0, s, a);
     hipLaunchKernelGGL(coarse_prolong_kernel<float>, dim3(pg), dim3(PR_BS), 0, s, a);
   } else {
-    hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.nseg), dim3(RS_BS), 0, s, a);
+    hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.na), dim3(RS_BS), 0, s, a);
     hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.gemv_g), dim3(GV_BS), 0, s, a);
     hipLaunchKernelGGL(coarse_prolong_kernel<double>, dim3(pg), dim3(PR_BS), 0, s, a);
   }

@@ -173,15 +173,12 @@ void launch_matfree_product(hipStream_t s, bool fp32, const MatFreeArgs& a, cons
 // applied to the restricted residual each PCG iteration (coarse_kernels.hip).
 constexpr int COARSE_TILE = 32;     // Gauss-Jordan tile
 constexpr int COARSE_MAX_PAD = 2048;  // coarse DoFs, padded to the tile, at most
-constexpr int COARSE_SEG = 1024;    // nodes per restriction segment at most
+constexpr int COARSE_PIECE = 2048;  // BSR3 blocks per coarse assembly workgroup at most
 struct CoarseArgs {
   int na = 0;                  // aggregates
   int npad = 0;                // 6 na padded to a multiple of COARSE_TILE
-  int nseg = 0;                // restriction segments (one aggregate each)
   int gemv_g = 0;              // workgroups of the coarse product
-  const int* seg_agg = nullptr;   // nseg
-  const int* seg_lo = nullptr;    // nseg + 1: ranges into agg_nodes
-  const int* agg_seg0 = nullptr;  // na + 1: segments of each aggregate
+  const int* agg_ptr = nullptr;   // na + 1: ranges into agg_nodes
   const int* agg_nodes = nullptr; // n: nodes ordered by aggregate, ascending
   const int* agg_of = nullptr;    // n
   const void* rel = nullptr;      // 3n, Hessian precision
@@ -189,9 +186,9 @@ struct CoarseArgs {
   const uint8_t* fixmask = nullptr;
   const double* R = nullptr;      // npad x npad: -Ac^-1
   const int* ok = nullptr;        // 0: the sweeps met a non-positive pivot
-  double* spart = nullptr;        // nseg x 6 restriction partials
+  double* rc = nullptr;           // npad: P^T r (padding entries stay zero)
   double* uc = nullptr;           // npad: Ac^-1 P^T r
-  double* cpart = nullptr;        // gemv_g partials of (P^T r).uc
+  double* cpart = nullptr;        // gemv_g partials of rc.uc
   double* cdot = nullptr;         // their sum: r.(P uc) of the last application
 };
 
@@ -420,10 +417,13 @@ void launch_ldlt_solve(hipStream_t s, const LdltArgs& a, const int* list, int nl
 // BSR3 blocks (i in a, j in b) that sum into it, in block order.
 struct CoarseBuild {
   int ncb;                // coarse block pairs
+  int npiece;             // pieces of at most COARSE_PIECE blocks, pair by pair
   const int* cb_a;        // ncb
   const int* cb_b;        // ncb
-  const int* cb_ptr;      // ncb + 1
+  const int* pair_piece0; // ncb + 1: pieces of each pair
+  const int* piece_lo;    // npiece + 1: ranges into cb_fine
   const int* cb_fine;     // BSR3 block indexes
+  double* piece_sum;      // npiece x 36 partial 6x6 blocks
   const int* brow;        // BSR3 block row / column (nodes)
   const int* bcol;
   const void* bval;       // BSR3 values, Hessian precision

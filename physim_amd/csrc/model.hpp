@@ -122,11 +122,12 @@ struct pgn_model {
   int coarse_bins[3] = {1, 1, 1};
   pgn::CoarseArgs cga;  // device pointers of the application (R set per build)
   int coarse_ncb = 0;
-  pgn::DBuf<int> cg_seg_agg, cg_seg_lo, cg_agg_seg0, cg_agg_nodes, cg_agg_of;
-  pgn::DBuf<int> cg_cb_a, cg_cb_b, cg_cb_ptr, cg_cb_fine;
+  int coarse_npiece = 0;
+  pgn::DBuf<int> cg_agg_ptr, cg_agg_nodes, cg_agg_of;
+  pgn::DBuf<int> cg_cb_a, cg_cb_b, cg_pair_piece0, cg_piece_lo, cg_cb_fine;
   pgn::DBuf<char> cg_rel;
   pgn::DBuf<uint8_t> cg_cmask;
-  pgn::DBuf<double> cg_A, cg_D, cg_spart, cg_uc, cg_cpart, cg_cdot;
+  pgn::DBuf<double> cg_A, cg_D, cg_piece_sum, cg_rc, cg_uc, cg_cpart, cg_cdot;
   pgn::DBuf<int> cg_ok;
   std::vector<int> h_agg_of;  // host copy of the aggregation
   void coarse_prepare();
