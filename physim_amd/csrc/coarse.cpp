@@ -197,8 +197,8 @@ void pgn_model::coarse_prepare() {
   cg_ok.ensure(1);
   coarse_npiece = (int)piece_lo.size() - 1;
   cg_piece_sum.ensure(36L * std::max(coarse_npiece, 1));
-  cg_rc.ensure(npad);
-  PGN_HIP(hipMemsetAsync(cg_rc.p, 0, npad * sizeof(double), s));  // padding stays zero
+  cg_rc.ensure((size_t)COARSE_SPLIT * npad);
+  PGN_HIP(hipMemsetAsync(cg_rc.p, 0, cg_rc.bytes(), s));  // padding stays zero
   cg_uc.ensure(npad);
   const int gg = coarse_gemv_grid(npad);
   cg_cpart.ensure(gg);
@@ -225,8 +225,21 @@ void pgn_model::coarse_prepare() {
   coarse_ready = true;
 }
 
-double* pgn_model::coarse_update(bool assemble_only) {
+double* pgn_model::coarse_update(bool assemble_only, bool async) {
   coarse_prepare();
+  hipStream_t st = stream();
+  if (async) {
+    if (!side) {
+      PGN_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+      PGN_HIP(hipEventCreateWithFlags(&ev_hess, hipEventDisableTiming));
+      PGN_HIP(hipEventCreateWithFlags(&ev_coarse, hipEventDisableTiming));
+    }
+    PGN_HIP(hipEventRecord(ev_hess, stream()));
+    PGN_HIP(hipStreamWaitEvent(side, ev_hess, 0));
+    st = side;
+  } else {
+    coarse_join();
+  }
   CoarseBuild b;
   b.ncb = coarse_ncb;
   b.npiece = coarse_npiece;
@@ -244,7 +257,18 @@ double* pgn_model::coarse_update(bool assemble_only) {
   b.A1 = cg_A.p + nn;
   b.D = cg_D.p;
   b.ok = cg_ok.p;
-  double* R = launch_coarse_operator(stream(), hfp32, cga, b, assemble_only);
+  double* R = launch_coarse_operator(st, hfp32, cga, b, assemble_only);
+  PGN_HIP(hipGetLastError());
   if (!assemble_only) cga.R = R;
+  if (async) {
+    PGN_HIP(hipEventRecord(ev_coarse, side));
+    coarse_pending = true;
+  }
   return R;
+}
+
+void pgn_model::coarse_join() {
+  if (!coarse_pending) return;
+  PGN_HIP(hipStreamWaitEvent(stream(), ev_coarse, 0));
+  coarse_pending = false;
 }

@@ -8,8 +8,8 @@
 // and inverts it in place by block Gauss-Jordan sweeps (the sweep operator on
 // symmetric positive definite matrices: after sweeping every pivot the matrix
 // holds -Ac^-1; no pivoting is needed since every pivot is a Schur
-// complement of an SPD matrix).  Per PCG iteration it restricts r (one
-// workgroup per aggregate), multiplies by the inverse (one row per
+// complement of an SPD matrix).  Per PCG iteration it restricts r (four
+// workgroups per aggregate), multiplies by the inverse (one row per
 // wavefront) and prolongs into z (one node per thread).  Every sum has a fixed order: the results do
 // not depend on scheduling.
 #include <hip/hip_runtime.h>
@@ -25,7 +25,7 @@ namespace {
 constexpr int TL = COARSE_TILE;  // 32
 constexpr int CB_BS = 256;       // coarse assembly workgroup
 constexpr int GJ_BS = 256;       // sweep workgroup: 4 entries of a 32 x 32 tile each
-constexpr int RS_BS = 1024;      // restriction workgroup: one aggregate
+constexpr int RS_BS = 256;       // restriction workgroup: a quarter of an aggregate
 constexpr int RS_U = 4;          // nodes in flight per thread
 constexpr int GV_BS = 256;       // coarse product workgroup: one row per wavefront
 constexpr int GV_ROWS = GV_BS / 64;
@@ -133,40 +133,36 @@ __global__ __launch_bounds__(256) void coarse_pairs_kernel(CoarseArgs c, CoarseB
 
 // ---- block Gauss-Jordan sweeps -----------------------------------------------------
 
-// In-place sweep of the 32x32 matrix M (LDS, row-major) over all its pivots:
-// leaves -M^-1.  Every thread of the workgroup takes 4 entries per pivot.
+// Sweep of the 32x32 matrix in Ma (LDS, row-major) over all its pivots by
+// the whole workgroup, 4 entries per thread: pivot p reads one of the
+// buffers Ma, Mb and writes the other (one barrier per pivot); after the 32
+// pivots Ma holds -M^-1.  d0 (LDS, 32) receives the starting diagonal.
 // Returns false when a pivot is not positive relative to its starting
 // diagonal value (all threads agree).
-__device__ bool sweep_tile(double* M, double* diag0) {
+__device__ bool sweep_tile(double* Ma, double* Mb, double* d0) {
   const int t = threadIdx.x;
-  if (t < TL) diag0[t] = M[t * TL + t];
+  if (t < TL) d0[t] = Ma[t * TL + t];
   __syncthreads();
   bool good = true;
   for (int p = 0; p < TL; ++p) {
-    double d = M[p * TL + p];
-    if (!(d > 1e-13 * fabs(diag0[p]) && d > 0.0)) {
+    const double* src = (p & 1) ? Mb : Ma;
+    double* dst = (p & 1) ? Ma : Mb;
+    double d = src[p * TL + p];
+    if (!(d > 1e-13 * fabs(d0[p]) && d > 0.0)) {
       good = false;
       d = 1.0;  // keeps the remaining arithmetic finite; the result is flagged
     }
     const double inv = 1.0 / d;
-    double mij[4], mip[4], mpj[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = t + GJ_BS * q, i = e / TL, j = e % TL;
-      mij[q] = M[e];
-      mip[q] = M[i * TL + p];
-      mpj[q] = M[p * TL + j];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = t + GJ_BS * q, i = e / TL, j = e % TL;
+      const double mij = src[e], mip = src[i * TL + p], mpj = src[p * TL + j];
       double v;
       if (i == p && j == p) v = -inv;
-      else if (i == p) v = mpj[q] * inv;
-      else if (j == p) v = mip[q] * inv;
-      else v = mij[q] - mip[q] * mpj[q] * inv;
-      M[e] = v;
+      else if (i == p) v = mpj * inv;
+      else if (j == p) v = mip * inv;
+      else v = mij - mip * mpj * inv;
+      dst[e] = v;
     }
     __syncthreads();
   }
@@ -205,10 +201,11 @@ __device__ __forceinline__ void tile_mul(const double* X, const double* Y, doubl
 __global__ __launch_bounds__(GJ_BS) void coarse_pivot0_kernel(int npad, const double* A,
                                                               double* D, int* ok) {
   __shared__ double M[TL * TL];
+  __shared__ double M2[TL * TL];
   __shared__ double d0[TL];
   load_tile(A, npad, 0, 0, M);
   __syncthreads();
-  const bool good = sweep_tile(M, d0);
+  const bool good = sweep_tile(M, M2, d0);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int e = threadIdx.x + GJ_BS * q;
@@ -277,7 +274,7 @@ __global__ __launch_bounds__(GJ_BS) void coarse_sweep_kernel(int npad, int k,
 #pragma unroll
     for (int q = 0; q < 4; ++q) sX[r * TL + c0 + 8 * q] = out[q];
     __syncthreads();
-    const bool good = sweep_tile(sX, d0);
+    const bool good = sweep_tile(sX, sC, d0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = threadIdx.x + GJ_BS * q;
@@ -303,9 +300,10 @@ __global__ __launch_bounds__(256) void coarse_symmetrize_kernel(int npad, double
 
 // ---- per PCG iteration ------------------------------------------------------------
 
-// rc = P^T r for one aggregate per workgroup: translations sum F_i r_i,
-// rotations sum rel_i x F_i r_i, over the aggregate's nodes in order (RS_U
-// nodes per thread in flight); dropped columns get zero.
+// Part s (blockIdx.y) of P^T r for aggregate a (blockIdx.x): translations sum
+// F_i r_i, rotations sum rel_i x F_i r_i, over the s-th quarter of the
+// aggregate's nodes in order (RS_U nodes per thread in flight), into
+// rc[s npad + 6 a + e]; dropped columns get zero.
 template <class T>
 __global__ __launch_bounds__(RS_BS) void coarse_restrict_kernel(PcgArgs A) {
   __shared__ double red[RS_BS / 64][6];
@@ -313,8 +311,10 @@ __global__ __launch_bounds__(RS_BS) void coarse_restrict_kernel(PcgArgs A) {
   const CoarseArgs& c = A.cg;
   const T* __restrict__ r = reinterpret_cast<const T*>(A.r);
   const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
-  const int a = blockIdx.x;
-  const int lo = c.agg_ptr[a], hi = c.agg_ptr[a + 1];
+  const int a = blockIdx.x, part = blockIdx.y;
+  const int a0 = c.agg_ptr[a], len = c.agg_ptr[a + 1] - a0;
+  const int lo = a0 + (int)((long)len * part / COARSE_SPLIT);
+  const int hi = a0 + (int)((long)len * (part + 1) / COARSE_SPLIT);
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int t0 = lo + threadIdx.x; t0 < hi; t0 += RS_BS * RS_U) {
     int idx[RS_U];
@@ -362,8 +362,16 @@ __global__ __launch_bounds__(RS_BS) void coarse_restrict_kernel(PcgArgs A) {
 #pragma unroll
     for (int k = 0; k < RS_BS / 64; ++k) v += red[k][e];
     if (!((c.cmask[a] >> e) & 1u)) v = 0.0;
-    c.rc[6L * a + e] = v;
+    c.rc[(long)part * c.npad + 6L * a + e] = v;
   }
+}
+
+// rc_j: its COARSE_SPLIT parts summed in order
+__device__ __forceinline__ double rc_at(const double* __restrict__ rc, int npad, int j) {
+  double v = rc[j];
+#pragma unroll
+  for (int s = 1; s < COARSE_SPLIT; ++s) v += rc[(long)s * npad + j];
+  return v;
 }
 
 // uc = Ac^-1 rc (R holds -Ac^-1), one row per wavefront; one partial of
@@ -380,16 +388,16 @@ __global__ __launch_bounds__(GV_BS) void coarse_gemv_kernel(PcgArgs A) {
   double v0 = 0.0, v1 = 0.0;
   int j = lane;
   for (; j + 64 < c.npad; j += 128) {
-    v0 += Rr[j] * rc[j];
-    v1 += Rr[j + 64] * rc[j + 64];
+    v0 += Rr[j] * rc_at(rc, c.npad, j);
+    v1 += Rr[j + 64] * rc_at(rc, c.npad, j + 64);
   }
-  if (j < c.npad) v0 += Rr[j] * rc[j];
+  if (j < c.npad) v0 += Rr[j] * rc_at(rc, c.npad, j);
   const double v = wave_sum(v0 + v1);
   double dl = 0.0;
   if (lane == 0) {
     const double u = good ? -v : 0.0;
     c.uc[row] = u;
-    dl = rc[row] * u;
+    dl = rc_at(rc, c.npad, row) * u;
   }
   const double sdl = block_sum<GV_BS>(dl, red);
   if (threadIdx.x == 0) c.cpart[blockIdx.x] = sdl;
@@ -457,11 +465,13 @@ void launch_coarse_apply(hipStream_t s, bool fp32, const PcgArgs& a) {
   const CoarseArgs& c = a.cg;
   const int pg = (a.n + PR_BS - 1) / PR_BS;
   if (fp32) {
-    hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.na), dim3(RS_BS), 0, s, a);
+    hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.na, COARSE_SPLIT), dim3(RS_BS), 0,
+                       s, a);
     hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.gemv_g), dim3(GV_BS), 0, s, a);
     hipLaunchKernelGGL(coarse_prolong_kernel<float>, dim3(pg), dim3(PR_BS), 0, s, a);
   } else {
-    hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.na), dim3(RS_BS), 0, s, a);
+    hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.na, COARSE_SPLIT), dim3(RS_BS),
+                       0, s, a);
     hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.gemv_g), dim3(GV_BS), 0, s, a);
     hipLaunchKernelGGL(coarse_prolong_kernel<double>, dim3(pg), dim3(PR_BS), 0, s, a);
   }

@@ -174,6 +174,7 @@ void launch_matfree_product(hipStream_t s, bool fp32, const MatFreeArgs& a, cons
 constexpr int COARSE_TILE = 32;     // Gauss-Jordan tile
 constexpr int COARSE_MAX_PAD = 2048;  // coarse DoFs, padded to the tile, at most
 constexpr int COARSE_PIECE = 2048;  // BSR3 blocks per coarse assembly workgroup at most
+constexpr int COARSE_SPLIT = 4;     // restriction workgroups per aggregate
 struct CoarseArgs {
   int na = 0;                  // aggregates
   int npad = 0;                // 6 na padded to a multiple of COARSE_TILE
@@ -186,7 +187,7 @@ struct CoarseArgs {
   const uint8_t* fixmask = nullptr;
   const double* R = nullptr;      // npad x npad: -Ac^-1
   const int* ok = nullptr;        // 0: the sweeps met a non-positive pivot
-  double* rc = nullptr;           // npad: P^T r (padding entries stay zero)
+  double* rc = nullptr;           // COARSE_SPLIT x npad parts of P^T r (padding stays zero)
   double* uc = nullptr;           // npad: Ac^-1 P^T r
   double* cpart = nullptr;        // gemv_g partials of rc.uc
   double* cdot = nullptr;         // their sum: r.(P uc) of the last application

@@ -95,6 +95,12 @@ void build_bsr_pattern(int n, int m, const int* T, BsrPattern& P) {
 using namespace pgn;
 
 pgn_model::~pgn_model() {
+  if (side) {
+    (void)hipStreamSynchronize(side);
+    (void)hipStreamDestroy(side);
+  }
+  if (ev_hess) (void)hipEventDestroy(ev_hess);
+  if (ev_coarse) (void)hipEventDestroy(ev_coarse);
   pcg_graphs.clear();
   if (hkctl) (void)hipHostFree(hkctl);
   for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);

@@ -132,8 +132,16 @@ struct pgn_model {
   std::vector<int> h_agg_of;  // host copy of the aggregation
   void coarse_prepare();
   // assembles Ac from the current BSR3 values and inverts it (no sync);
-  // assemble_only leaves Ac itself in the first buffer (tests)
-  double* coarse_update(bool assemble_only = false);
+  // assemble_only leaves Ac itself in the first buffer (tests).  async: on
+  // the side stream, after the work queued so far on the model's stream, so
+  // that the sweeps (a few workgroups at a time) overlap the SELL pack, the
+  // FP64 residual and the PCG init; coarse_join() makes the model's stream
+  // wait for it before the first coarse application
+  double* coarse_update(bool assemble_only = false, bool async = false);
+  void coarse_join();
+  hipStream_t side = nullptr;
+  hipEvent_t ev_hess = nullptr, ev_coarse = nullptr;
+  bool coarse_pending = false;
   pgn::DBuf<double> r64, tmp64a, tmp64b;
   pgn::DBuf<double> nw_xsave, nw_v0, nw_dx, nw_b;  // Newton work vectors
   // LBFGS (QPSolverType::LBFGS): x and g of the last LBFGS step, a ring of

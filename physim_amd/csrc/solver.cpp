@@ -20,7 +20,10 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
   hipStream_t s = M.stream();
   const PcgArgs a = M.pcg_args(tol, precond);
   launch_pcg_init(s, M.hfp32, a, M.pb.p, x_zero ? 1 : 0);
-  if (a.coarse) launch_coarse_apply(s, M.hfp32, a);
+  if (a.coarse) {
+    M.coarse_join();  // the coarse operator built on the side stream
+    launch_coarse_apply(s, M.hfp32, a);
+  }
   if (M.profiling && M.prof_ev.size() < 4) {
     while (M.prof_ev.size() < 4) {
       hipEvent_t e;
@@ -181,6 +184,7 @@ void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opt
   double regT = -1.0;
   const int regs = std::max(o.reg_iters, 1);
   for (int i = 0; i < regs; ++i) {
+    M.coarse_join();  // the last try's coarse build reads the matrix changed below
     if (i != 0) {
       if (regT < 0.0) regT = 1e-6 * M.frob_lower();  // LinearSolver.cpp:57-66
       launch_add_diag(s, M.hfp32, M.n, M.diag.p, M.bval.p, regT * std::pow(10.0, i - 1));
@@ -189,8 +193,10 @@ void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opt
       res.reg_used = i;
     }
     PGN_HIP(hipMemcpyAsync(x, xprev, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-    M.sell_pack();  // the SpMV's copy of this try's matrix
-    if (coarse) M.coarse_update();  // and the coarse operator of this try
+    // this try's coarse operator (side stream), then the SpMV's copy of the
+    // matrix, which overlaps it
+    if (coarse) M.coarse_update(false, true);
+    M.sell_pack();
     int it = 0, refs = 0;
     double rel = 0.0;
     solve_once(M, b, x, o, it, refs, rel);
@@ -201,6 +207,7 @@ void linear_solve(pgn_model& M, const double* b, double* x, const pgn_linear_opt
     res.status = 0;
     break;
   }
+  M.coarse_join();
 }
 
 // ---- Newton -----------------------------------------------------------------
