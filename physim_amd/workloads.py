@@ -39,6 +39,13 @@ import physim_amd as pg
 from physim_amd import scenes
 
 SOLVE_OPTS = dict(pg.CONVERGE_OPTS, precond=1, adaptive_tol=1, lsNumMaxIters=2000)
+PRECOND_NAMES = ["jacobi", "block-jacobi", "two-level"]
+
+
+def _opts(args, **kw):
+    """SOLVE_OPTS with bench.py's --precond (the fused batch PCG takes the
+    two-level preconditioner as block Jacobi; single-model solves use it)."""
+    return dict(SOLVE_OPTS, precond=getattr(args, "precond", 1), **kw)
 
 
 def _stats(vals):
@@ -60,7 +67,9 @@ def batch(args, log):
     n = sum(s.n for s in scs)
     log(f"batch setup {time.perf_counter() - t0:.1f}s: {count} beams, {m} tets, {3 * n} DoF, "
         f"{B.device_bytes() / 2**30:.2f} GiB on device")
-    opts = dict(SOLVE_OPTS, numMaxIters=200)
+    # the reference's own Newton iteration cap (OptimSolver.h:56): buckling
+    # StVK instances that stall under rounding-level energy changes run to it
+    opts = _opts(args, numMaxIters=100)
     warm = min(args.warmup, 1)
     for _ in range(warm):
         B.reset()
@@ -96,7 +105,7 @@ def batch(args, log):
         "dtype": "f32 Hessian/PCG, f64 energy/gradient/residual",
         "config": {"workload": f"batch_{count}_beams: {m} tets, {3 * n} DoF, one fused "
                                "launch per kernel, per-instance Newton",
-                   "precision": "mixed", "precond": "block-jacobi",
+                   "precision": "mixed", "precond": PRECOND_NAMES[min(opts["precond"], 1)],
                    "linear_tol": "eisenstat-walker"},
         "converged": sum(r["result"] == "SUCCESS" for r in res),
         "instances": count,
@@ -223,7 +232,7 @@ def dynamic(args, log):
     M = pg.TetModel(ctx, sc.X0, sc.T, sc.material, sc.lame1, sc.lame2, sc.density, pg.MIXED)
     sc.apply(M)
     log(f"dynamic setup {time.perf_counter() - t0:.1f}s: {sc.m} tets, {3 * sc.n} DoF")
-    opts = dict(SOLVE_OPTS)
+    opts = _opts(args)
     if args.warmup:
         M.dynamic_step(dt, **opts)
         M.reset()
@@ -242,7 +251,7 @@ def dynamic(args, log):
         "scaling": "weak", "dtype": "f32 Hessian/PCG, f64 energy/gradient/residual",
         "config": {"workload": f"dynamic_beam_40x40x160: {sc.m} tets, {3 * sc.n} DoF, "
                                f"backward Euler dt 1/60 s, gravity over 10 steps",
-                   "precision": "mixed", "precond": "block-jacobi",
+                   "precision": "mixed", "precond": PRECOND_NAMES[opts["precond"]],
                    "linear_tol": "eisenstat-walker"},
         "converged": sum(r["result"] == "SUCCESS" for r in res),
         "newton_iterations": _stats([r["iterations"] for r in res]),
@@ -288,7 +297,7 @@ def _serving_mix(count, max_tets, max_cloth, seed=5):
 def _serve_once(reqs, rate, args, log):
     S = pg.Server(hbm_budget_bytes=200 << 30, lanes=args.lanes, max_batch=16,
                   precision=pg.MIXED, group_by_size=args.group_by_size,
-                  cohorts_per_lane=args.cohorts, **dict(SOLVE_OPTS, numMaxIters=300))
+                  cohorts_per_lane=args.cohorts, **_opts(args, numMaxIters=300))
     count = len(reqs)
     rng = np.random.default_rng(6)
     gaps = rng.exponential(1.0 / rate, size=count)
