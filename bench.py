@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--group-by-size", type=int, default=1,
                     help="serving: 1 (default) batches of similar footprint, cheapest "
                          "first; 0 arrival order")
+    ap.add_argument("--rod-fp32", type=int, default=1,
+                    help="rods: 1 (default) bend-twist Hessians in FP32 (PGN_MIXED), 0 FP64")
     ap.add_argument("--rod-density", type=float, default=3.14e-6,
                     help="rods: line density, kg per metre of rod")
     ap.add_argument("--cpu-study", action="store_true",

@@ -589,8 +589,20 @@ int pgn_rods_set_frames(pgn_rods* rods, const double* tan, const double* nor,
  * (KEleDER.cpp:94-98,216-220) */
 int pgn_rods_update_kinematics(pgn_rods* rods);
 /* bend-twist element outputs at every vertex (zero where no element): E,
- * g over (e1, e2, theta1, theta2) (8), H (64, row-major) */
+ * g over (e1, e2, theta1, theta2) (8), H (64, row-major; a PGN_MIXED batch
+ * evaluates its Hessians in FP32, energies and gradients in FP64) */
 int pgn_rods_element_bt(pgn_rods* rods, double* E, double* g8, double* H64);
+/* EnergyElement_DER_BendTwist::ComputeAndStore_{Energy,Gradient,Hessian}
+ * (EnergyElement_DER_BendTwist.cpp:67-249) on caller-given element states,
+ * evaluated by the device kernel: per element the 27 parameters of the
+ * reference's getBT{Energy,Gradient,Hessian}OPT kernels (shear and Young's
+ * moduli, radii and lengths of both edges, rest curvatures K0 K1, rest
+ * twist, frames n1 u1 n2 u2, reference twist, one unused), the edge
+ * vectors e1 e2 (3 each) and the material angles (2).  precision PGN_MIXED
+ * evaluates in FP32 (the mixed batches' Hessian path). */
+int pgn_rod_bt_eval(pgn_ctx* ctx, int count, const double* prm, const double* e1,
+                    const double* e2, const double* theta, int precision, double* E,
+                    double* g8, double* H64);
 /* OptimSolver::SolveFull for every rod (LinearSolver_EigenLDLT semantics on
  * each rod's banded Hessian); res: n_rods results */
 int pgn_rods_newton(pgn_rods* rods, const pgn_newton_opts* opts,

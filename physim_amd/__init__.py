@@ -39,6 +39,7 @@ EXPORTS = [
     "pgn_model_hessian_pattern", "pgn_model_get_hessian",
     "pgn_model_element_egh", "pgn_linear_default_opts", "pgn_linear_solve",
     "pgn_model_hessian_product", "pgn_model_precond_apply", "pgn_model_coarse_space",
+    "pgn_rod_bt_eval",
     "pgn_newton_default_opts", "pgn_newton_solve", "pgn_time_kernel",
     "pgn_kernel_bytes", "pgn_host_bsr_pattern", "pgn_model_set_profiling",
     "pgn_model_get_profile", "pgn_model_create_shell", "pgn_batch_create",
@@ -155,6 +156,8 @@ def lib():
         L.pgn_model_hessian_product.argtypes = [_vp, ctypes.c_int, _dp, _dp]
         L.pgn_model_precond_apply.argtypes = [_vp, ctypes.c_int, _dp, _dp]
         L.pgn_model_coarse_space.argtypes = [_vp, _ip, _ip, _ucp, _dp, _dp]
+        L.pgn_rod_bt_eval.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                      _dp, _dp, _dp]
         L.pgn_linear_default_opts.argtypes = [ctypes.POINTER(LinearOpts)]
         L.pgn_linear_default_opts.restype = None
         L.pgn_linear_solve.argtypes = [_vp, _dp, _dp, ctypes.POINTER(LinearOpts),
@@ -861,6 +864,21 @@ def host_bsr_pattern(n_nodes, T):
     _ok(lib().pgn_host_bsr_pattern(n_nodes, m, _p(T, _ip), ctypes.byref(nb),
                                    _p(rp, _lp), _p(col, _ip)))
     return rp, col
+
+
+def rod_bt_eval(ctx, prm, e1, e2, theta, precision=FP64):
+    """Bend-twist elements on given states through the device kernel
+    (pgn_rod_bt_eval): prm (k, 27), e1 / e2 (k, 3), theta (k, 2) ->
+    E (k), g (k, 8), H (k, 8, 8)."""
+    prm = _f64(prm).reshape(-1, 27)
+    k = prm.shape[0]
+    E = np.zeros(k)
+    g = np.zeros((k, 8))
+    H = np.zeros((k, 8, 8))
+    _ok(lib().pgn_rod_bt_eval(ctx.handle, k, _p(prm), _p(_f64(e1).reshape(-1)),
+                              _p(_f64(e2).reshape(-1)), _p(_f64(theta).reshape(-1)),
+                              precision, _p(E), _p(g), _p(H)))
+    return E, g, H
 
 
 class Rods:

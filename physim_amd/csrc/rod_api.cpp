@@ -219,6 +219,53 @@ int pgn_rods_update_kinematics(pgn_rods* R) {
   });
 }
 
+int pgn_rod_bt_eval(pgn_ctx* ctx, int count, const double* prm, const double* e1,
+                    const double* e2, const double* theta, int precision, double* E, double* g8,
+                    double* H64) {
+  return rguard([&] {
+    PGN_CHECK(ctx && prm && e1 && e2 && theta && E && g8 && H64, "null argument");
+    PGN_CHECK(count >= 0, "negative element count");
+    PGN_CHECK(precision == PGN_FP64 || precision == PGN_MIXED, "unknown precision");
+    PGN_HIP(hipSetDevice(ctx->device));
+    if (count == 0) return;
+    hipStream_t s = ctx->stream;
+    DBuf<double> dp, d1, d2, dt, dE, dg;
+    DBuf<char> dH;
+    dp.upload(prm, 27L * count, s);
+    d1.upload(e1, 3L * count, s);
+    d2.upload(e2, 3L * count, s);
+    dt.upload(theta, 2L * count, s);
+    dE.alloc(count);
+    dg.alloc(8L * count);
+    const bool fp32 = precision == PGN_MIXED;
+    dH.alloc(36L * count * (fp32 ? sizeof(float) : sizeof(double)));
+    rod_launch_bt_raw(s, count, dp.p, d1.p, d2.p, dt.p, fp32, dE.p, dg.p, dH.p);
+    PGN_HIP(hipGetLastError());
+    std::vector<double> h(36L * count);
+    dE.download(E, count, s);
+    dg.download(g8, 8L * count, s);
+    if (fp32) {
+      std::vector<float> hf(h.size());
+      PGN_HIP(hipMemcpyAsync(hf.data(), dH.p, hf.size() * sizeof(float), hipMemcpyDeviceToHost,
+                             s));
+      PGN_HIP(hipStreamSynchronize(s));
+      std::copy(hf.begin(), hf.end(), h.begin());
+    } else {
+      PGN_HIP(hipMemcpyAsync(h.data(), dH.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             s));
+      PGN_HIP(hipStreamSynchronize(s));
+    }
+    for (long k = 0; k < count; ++k) {
+      int q = 0;
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j <= i; ++j, ++q) {
+          H64[64 * k + 8 * i + j] = h[36 * k + q];
+          H64[64 * k + 8 * j + i] = h[36 * k + q];
+        }
+    }
+  });
+}
+
 int pgn_rods_element_bt(pgn_rods* R, double* E, double* g8, double* H64) {
   return rguard([&] {
     PGN_CHECK(R, "null argument");
@@ -226,8 +273,16 @@ int pgn_rods_element_bt(pgn_rods* R, double* E, double* g8, double* H64) {
     std::vector<double> e(R->nv), g(8 * R->nv), h(36 * R->nv);
     R->eb.download(e.data(), e.size(), R->stream());
     R->gb.download(g.data(), g.size(), R->stream());
-    PGN_HIP(hipMemcpyAsync(h.data(), R->Hb.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           R->stream()));
+    if (R->prec == PGN_MIXED) {  // FP32 Hessians
+      std::vector<float> hf(h.size());
+      PGN_HIP(hipMemcpyAsync(hf.data(), R->Hb.p, hf.size() * sizeof(float),
+                             hipMemcpyDeviceToHost, R->stream()));
+      R->sync();
+      std::copy(hf.begin(), hf.end(), h.begin());
+    } else {
+      PGN_HIP(hipMemcpyAsync(h.data(), R->Hb.p, h.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, R->stream()));
+    }
     R->sync();
     for (long v = 0; v < R->nv; ++v) {
       const bool has = R->hflags[v] & ROD_HAS_BT;

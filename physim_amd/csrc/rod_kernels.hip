@@ -139,14 +139,17 @@ __global__ __launch_bounds__(ROD_BS) void rod_stretch_kernel(RodArgs A, double* 
   }
 }
 
-template <class T, bool WH>
+// EG: writes the element energy and gradient (FP64 outputs from arithmetic
+// in T); WH: the packed Hessian in T.  The mixed path runs <double, false,
+// true> for E and g and <float, true, false> for the Hessian alone.
+template <class T, bool WH, bool EG>
 __global__ __launch_bounds__(ROD_BS) void rod_bt_kernel(RodArgs A, double* __restrict__ eb,
                                                          double* __restrict__ gb,
                                                          T* __restrict__ Hb) {
   const long v = blockIdx.x * (long)ROD_BS + threadIdx.x;
   if (v >= A.nv) return;
   if (!(A.flags[v] & ROD_HAS_BT)) {
-    eb[v] = 0.0;
+    if (EG) eb[v] = 0.0;
     return;
   }
   const RodBtConst c = load_bt_const(A, v);
@@ -160,12 +163,70 @@ __global__ __launch_bounds__(ROD_BS) void rod_bt_kernel(RodArgs A, double* __res
   xv[7] = (T)A.x[4 * v + 3];
   T g8[8], h36[WH ? 36 : 1];
   const T E = rod_bt_energy<T, WH>(xv, c, g8, WH ? h36 : nullptr);
-  eb[v] = (double)E;
+  if (EG) {
+    eb[v] = (double)E;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) gb[8 * v + i] = (double)g8[i];
+    for (int i = 0; i < 8; ++i) gb[8 * v + i] = (double)g8[i];
+  }
   if (WH)
 #pragma unroll
     for (int i = 0; i < 36; ++i) Hb[36 * v + i] = h36[i];
+}
+
+// The bend-twist element on caller-given inputs, one element per lane: the
+// parameter vector of the reference's getBT* kernels (G, Y, rw, rh and the
+// lengths of both edges, K0, K1, tau0, n1, u1, n2, u2, reference twist), the
+// edge vectors and the two material angles; E, g8 and the packed Hessian
+// (36, WH) in T.
+template <class T, bool WH>
+__global__ __launch_bounds__(ROD_BS) void rod_bt_raw_kernel(long count,
+                                                             const double* __restrict__ prm,
+                                                             const double* __restrict__ e1,
+                                                             const double* __restrict__ e2,
+                                                             const double* __restrict__ th,
+                                                             double* __restrict__ E,
+                                                             double* __restrict__ g,
+                                                             T* __restrict__ H) {
+  const long k = blockIdx.x * (long)ROD_BS + threadIdx.x;
+  if (k >= count) return;
+  const double* p = prm + 27 * k;
+  const double pi = 3.14159265358979323846;
+  const double G = 0.5 * (p[0] + p[1]), Y = 0.5 * (p[2] + p[3]);
+  const double rw = 0.5 * (p[4] + p[5]), rh = 0.5 * (p[6] + p[7]);
+  const double lbar = 0.5 * (p[8] + p[9]);
+  RodBtConst c;
+  c.kb0 = pi / 8.0 * Y * rw * rh * rh * rh / lbar;
+  c.kb1 = pi / 8.0 * Y * rw * rw * rw * rh / lbar;
+  c.kt = pi / 8.0 * G * rw * rh * (rw * rw + rh * rh) / lbar;
+  c.K0 = p[10];
+  c.K1 = p[11];
+  c.tau0 = p[12];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    c.n1[i] = p[13 + i];
+    c.u1[i] = p[16 + i];
+    c.n2[i] = p[19 + i];
+    c.u2[i] = p[22 + i];
+  }
+  c.rt = p[25];
+  c.crt = cos(c.rt);
+  c.srt = sin(c.rt);
+  T xv[8];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    xv[i] = (T)e1[3 * k + i];
+    xv[3 + i] = (T)e2[3 * k + i];
+  }
+  xv[6] = (T)th[2 * k];
+  xv[7] = (T)th[2 * k + 1];
+  T g8[8], h36[WH ? 36 : 1];
+  const T v = rod_bt_energy<T, WH>(xv, c, g8, WH ? h36 : nullptr);
+  E[k] = (double)v;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) g[8 * k + i] = (double)g8[i];
+  if (WH)
+#pragma unroll
+    for (int i = 0; i < 36; ++i) H[36 * k + i] = h36[i];
 }
 
 // ---- gradient gather + per-vertex energies --------------------------------------
@@ -602,15 +663,21 @@ static inline unsigned blocks(long n, int bs) { return (unsigned)((n + bs - 1) /
 void rod_launch_eval(hipStream_t s, const RodArgs& a, bool hess, bool fp32_hess, RodBufs& b) {
   if (a.nv == 0) return;
   const unsigned g = blocks(a.nv, ROD_BS);
-  (void)fp32_hess;  // rods evaluate their Hessians in FP64
-  if (hess) {
+  if (hess && fp32_hess) {
+    // FP64 energy and gradient (line search, residual), FP32 Hessian
     hipLaunchKernelGGL(rod_stretch_kernel<true>, dim3(g), dim3(ROD_BS), 0, s, a, b.es, b.gs, b.Js);
-    hipLaunchKernelGGL((rod_bt_kernel<double, true>), dim3(g), dim3(ROD_BS), 0, s, a, b.eb, b.gb,
-                       reinterpret_cast<double*>(b.Hb));
+    hipLaunchKernelGGL((rod_bt_kernel<double, false, true>), dim3(g), dim3(ROD_BS), 0, s, a, b.eb,
+                       b.gb, reinterpret_cast<double*>(b.Hb));
+    hipLaunchKernelGGL((rod_bt_kernel<float, true, false>), dim3(g), dim3(ROD_BS), 0, s, a, b.eb,
+                       b.gb, reinterpret_cast<float*>(b.Hb));
+  } else if (hess) {
+    hipLaunchKernelGGL(rod_stretch_kernel<true>, dim3(g), dim3(ROD_BS), 0, s, a, b.es, b.gs, b.Js);
+    hipLaunchKernelGGL((rod_bt_kernel<double, true, true>), dim3(g), dim3(ROD_BS), 0, s, a, b.eb,
+                       b.gb, reinterpret_cast<double*>(b.Hb));
   } else {
     hipLaunchKernelGGL(rod_stretch_kernel<false>, dim3(g), dim3(ROD_BS), 0, s, a, b.es, b.gs, b.Js);
-    hipLaunchKernelGGL((rod_bt_kernel<double, false>), dim3(g), dim3(ROD_BS), 0, s, a, b.eb, b.gb,
-                       reinterpret_cast<double*>(b.Hb));
+    hipLaunchKernelGGL((rod_bt_kernel<double, false, true>), dim3(g), dim3(ROD_BS), 0, s, a, b.eb,
+                       b.gb, reinterpret_cast<double*>(b.Hb));
   }
   hipLaunchKernelGGL(rod_gather_kernel, dim3(g), dim3(ROD_BS), 0, s, a, b.es, b.gs, b.eb, b.gb,
                      b.g, b.vE);
@@ -619,9 +686,12 @@ void rod_launch_eval(hipStream_t s, const RodArgs& a, bool hess, bool fp32_hess,
 
 void rod_launch_band(hipStream_t s, const RodArgs& a, bool fp32_hess, RodBufs& b) {
   const unsigned g = blocks(4 * a.nv, ROD_BS);
-  (void)fp32_hess;
-  hipLaunchKernelGGL(rod_band_kernel<double>, dim3(g), dim3(ROD_BS), 0, s, a, b.Js,
-                     reinterpret_cast<const double*>(b.Hb), b.band);
+  if (fp32_hess)
+    hipLaunchKernelGGL(rod_band_kernel<float>, dim3(g), dim3(ROD_BS), 0, s, a, b.Js,
+                       reinterpret_cast<const float*>(b.Hb), b.band);
+  else
+    hipLaunchKernelGGL(rod_band_kernel<double>, dim3(g), dim3(ROD_BS), 0, s, a, b.Js,
+                       reinterpret_cast<const double*>(b.Hb), b.band);
 }
 
 void rod_launch_ldlt(hipStream_t s, const RodArgs& a, RodBufs& b, const double* rhs, double* x,
@@ -629,6 +699,19 @@ void rod_launch_ldlt(hipStream_t s, const RodArgs& a, RodBufs& b, const double* 
   if (a.nrods == 0) return;
   hipLaunchKernelGGL(rod_ldlt_kernel, dim3(a.nrods), dim3(LDLT_BS), 0, s, a, b.band, b.work,
                      rhs, x, reg_iters, max_error, b.status, active);
+}
+
+void rod_launch_bt_raw(hipStream_t s, long count, const double* prm, const double* e1,
+                       const double* e2, const double* th, bool fp32, double* E, double* g,
+                       void* H) {
+  if (count <= 0) return;
+  const unsigned gr = blocks(count, ROD_BS);
+  if (fp32)
+    hipLaunchKernelGGL((rod_bt_raw_kernel<float, true>), dim3(gr), dim3(ROD_BS), 0, s, count, prm,
+                       e1, e2, th, E, g, reinterpret_cast<float*>(H));
+  else
+    hipLaunchKernelGGL((rod_bt_raw_kernel<double, true>), dim3(gr), dim3(ROD_BS), 0, s, count,
+                       prm, e1, e2, th, E, g, reinterpret_cast<double*>(H));
 }
 
 void rod_launch_kinematics(hipStream_t s, const RodArgs& a, double* frame, double* rt) {

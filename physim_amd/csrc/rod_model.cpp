@@ -222,8 +222,10 @@ RodBufs pgn_rods::bufs() {
 
 void pgn_rods::eval(bool hess) {
   RodBufs b = bufs();
-  rod_launch_eval(stream(), args(), hess, false, b);
-  if (hess) rod_launch_band(stream(), args(), false, b);
+  // PGN_MIXED: the bend-twist Hessians in FP32 (energy and gradient FP64)
+  const bool fp32 = prec == PGN_MIXED;
+  rod_launch_eval(stream(), args(), hess, fp32, b);
+  if (hess) rod_launch_band(stream(), args(), fp32, b);
 }
 
 void pgn_rods::download_rodE(std::vector<double>& out) {

@@ -182,8 +182,9 @@ def rods(args, log):
     t0 = time.perf_counter()
     sc = scenes.rod_batch(count, seed=4, density=args.rod_density)
     ctx = pg.Context(0)
+    prec = pg.MIXED if getattr(args, "rod_fp32", 1) else pg.FP64
     R = pg.Rods(ctx, sc.rods, sc.frame0_normals(), sc.young, sc.shear, sc.radius, sc.radius,
-                sc.density)
+                sc.density, precision=prec)
     R.set_fixed(sc.fixed_mask())
     R.set_gravity([0.0, -9.81, 0.0], sc.stages)
     nv = int(sc.nverts.sum())
@@ -211,7 +212,8 @@ def rods(args, log):
     return {
         "value": count / el, "unit": "Newton solves/s", "ms_per_step": el * 1e3,
         "steps": steps, "warmup": warm, "scaling": "weak",
-        "dtype": "f64 elements and banded LDL^T",
+        "dtype": ("f32 bend-twist Hessians, f64 energy/gradient/banded LDL^T"
+                  if prec == pg.MIXED else "f64 elements and banded LDL^T"),
         "config": {"workload": f"rods_{count}_helices: {nv} vertices, line density "
                                f"{args.rod_density:g} kg/m, 10 load stages",
                    "linear_solver": "banded LDL^T per rod"},

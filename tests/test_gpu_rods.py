@@ -5,6 +5,7 @@ Tolerances: element and assembled quantities are FP64 on both sides:
 relative 1e-10 (normwise); frames and reference twists absolute 1e-12;
 converged positions within 1e-6 x each rod's bounding-box diagonal.
 """
+import os
 import sys
 
 import numpy as np
@@ -28,9 +29,9 @@ def _rel(a, b):
     return np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
 
 
-def _pair(ctx, s, gravity=None):
+def _pair(ctx, s, gravity=None, precision=pg.FP64):
     G = pg.Rods(ctx, s.rods, s.frame0_normals(), s.young, s.shear, s.radius, s.radius,
-                s.density)
+                s.density, precision=precision)
     mask = s.fixed_mask()
     G.set_fixed(mask)
     O = []
@@ -121,6 +122,57 @@ def test_rods_update_kinematics_matches_oracle(ctx):
         assert np.abs(rt[v0:v0 + R.n] - rto).max() <= 1e-12, r
         e0 += ne
         v0 += R.n
+
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "rod_golden.npz")
+
+
+@pytest.mark.parametrize("precision,tol", [(pg.FP64, 1e-10), (pg.MIXED, 1e-5)])
+def test_rod_bt_kernel_matches_reference_golden(ctx, precision, tol):
+    """The device bend-twist kernel (FP64, and FP32 as the mixed batches use
+    it for their Hessians) on the inputs of the golden vectors written from
+    the reference's own getBT{Energy,Gradient,Hessian}OPT kernels: E, g and
+    H normwise within 1e-10 (FP64) / 1e-5 (FP32) relative."""
+    _say(f"rod bend-twist kernel vs golden precision={precision}")
+    g = np.load(GOLD)
+    E, gr, H = pg.rod_bt_eval(ctx, g["b_prm"], g["b_e1"], g["b_e2"], g["b_th"], precision)
+    for i in range(len(E)):
+        assert abs(E[i] - g["b_E"][i]) <= tol * max(abs(g["b_E"][i]), 1e-300), i
+        assert _rel(gr[i], g["b_g"][i]) <= tol, i
+        assert _rel(H[i], g["b_H"][i].reshape(8, 8)) <= tol, i
+        assert np.array_equal(H[i], H[i].T), i
+
+
+def test_rods_mixed_element_hessians_and_newton(ctx):
+    """A PGN_MIXED rod batch: FP32 bend-twist Hessians within 1e-5 of the
+    FP64 oracle's, FP64 energies and gradients within 1e-10, and Newton
+    converged to the oracle's positions within 1e-6 x bbox diagonal."""
+    _say("rods mixed precision vs oracle")
+    s = scenes.rod_batch(6, n_min=10, n_max=60, seed=14, density=3e-6)
+    G, O = _pair(ctx, s, precision=pg.MIXED)
+    G64, _ = _pair(ctx, s)
+    x = _perturb(s, G, 2e-3, 3)
+    G.set_x(x)
+    G64.set_x(x)
+    E, g8, H = G.element_bt()
+    E6, g6, H6 = G64.element_bt()
+    assert _rel(E, E6) <= 1e-12 and _rel(g8, g6) <= 1e-12
+    assert _rel(H, H6) <= 1e-5
+    G.reset()
+    G.set_gravity([0.0, -9.81, 0.0], 10)
+    for R in O:
+        R.set_gravity([0.0, -9.81, 0.0], 10)
+    res = G.newton(**ROD_OPTS)
+    x = G.get_x()
+    off = G.offsets
+    diag = s.bbox_diags()
+    for r, R in enumerate(O):
+        ro = R.newton(**ROD_OPTS)
+        assert ro["result"] == "SUCCESS", (r, ro)
+        assert res[r]["result"] == "SUCCESS", (r, res[r])
+        n = R.n
+        err = np.abs(x[off[r]:off[r] + 3 * n] - R.get_x()[:3 * n]).max()
+        assert err <= 1e-6 * diag[r], (r, err)
 
 
 def test_rods_newton_matches_oracle(ctx):
