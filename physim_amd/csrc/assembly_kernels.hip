@@ -35,9 +35,9 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_gather_kernel(
     NodeArgs A, const T* __restrict__ eg, double* __restrict__ g,
     double* __restrict__ part) {
   __shared__ double red[2 * NODE_BLOCK / 64];
-  const long i = xcd_block() * NODE_BLOCK + threadIdx.x;
+  const long i = A.i0 + xcd_block() * NODE_BLOCK + threadIdx.x;
   double eN = 0.0, gg = 0.0;
-  if (i < A.n) {
+  if (i < A.i0 + A.n) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     const int p1 = A.inc_ptr[i + 1];
     int p = A.inc_ptr[i];

@@ -168,18 +168,64 @@ void pgn_batch::upload_loads() {
   falpha.upload(fa, ctx->stream);
 }
 
-void pgn_batch::eval_eg(std::vector<double>& o4) {
+void pgn_batch::eval_eg(std::vector<double>& o4, const std::vector<uint8_t>* act) {
   hipStream_t s = ctx->stream;
-  M->launch_elements(false, EVAL_E | EVAL_G, nullptr, elemE.p, M->eg64.p, nullptr);
+  const int Kn = K();
   NodeArgs na = M->node_args();
   na.node_inst = node_inst.p;
   na.grav_inst = grav.p;
   na.falpha_inst = falpha.p;
   na.nodeE = nodeE.p;
   na.nodeG2 = nodeG2.p;
-  launch_node_gather(s, false, na, M->eg64.p, M->g.p, M->nodeP.p);
-  launch_batch_reduce(s, args(), elemE.p, nodeE.p, nodeG2.p, out.p);
-  o4.resize(4 * K());
+  // runs of consecutive marked instances (their elements and nodes are
+  // contiguous); many runs, or all instances: one whole-batch launch each
+  std::vector<std::pair<int, int>> runs;
+  if (act) {
+    for (int k = 0; k < Kn; ++k) {
+      if (!(*act)[k]) continue;
+      if (!runs.empty() && runs.back().second == k) runs.back().second = k + 1;
+      else runs.emplace_back(k, k + 1);
+    }
+  }
+  const bool whole = !act || runs.size() > 16 || (runs.size() == 1 && runs[0].first == 0 &&
+                                                  runs[0].second == Kn);
+  if (whole) {
+    M->launch_elements(false, EVAL_E | EVAL_G, nullptr, elemE.p, M->eg64.p, nullptr);
+    launch_node_gather(s, false, na, M->eg64.p, M->g.p, M->nodeP.p);
+    launch_batch_reduce(s, args(), elemE.p, nodeE.p, nodeG2.p, out.p);
+  } else {
+    const int tile = std::max(0, M->tet_tile_choice[0][EVAL_E | EVAL_G]);
+    for (const auto& rk : runs) {
+      const int k0 = rk.first, k1 = rk.second;
+      for (int c = 0; c < 3; ++c) {
+        const long e0 = heoff[c * (Kn + 1) + k0], e1 = heoff[c * (Kn + 1) + k1];
+        if (e1 <= e0) continue;
+        if (c == 0) {
+          TetArgs ta = M->tet_args(false);
+          ta.m = (int)(e1 - e0);
+          ta.tets += e0;
+          ta.kind += e0;
+          ta.prm = reinterpret_cast<const double*>(ta.prm) + e0;
+          launch_tet_eval(s, false, EVAL_E | EVAL_G, ta, nullptr, elemE.p + e0,
+                          M->eg64.p + 12 * e0, nullptr, tile);
+        } else {
+          const int kind = c == 1 ? SHELL_MEMBRANE : SHELL_HINGE;
+          ShellArgs sa = M->shell_args(false, kind);
+          sa.first = (int)e0;
+          sa.count = (int)(e1 - e0);
+          launch_shell_eval(s, false, kind, EVAL_E | EVAL_G, sa, nullptr, elemE.p, M->eg64.p,
+                            nullptr);
+        }
+      }
+      NodeArgs nr = na;
+      nr.i0 = hnoff[k0];
+      nr.n = (int)(hnoff[k1] - hnoff[k0]);
+      launch_node_gather(s, false, nr, M->eg64.p, M->g.p, M->nodeP.p);
+    }
+    eg_mask.upload(*act, s);
+    launch_batch_reduce(s, args(), elemE.p, nodeE.p, nodeG2.p, out.p, eg_mask.p);
+  }
+  o4.resize(4 * Kn);
   out.download(o4.data(), o4.size(), s);
   PGN_HIP(hipStreamSynchronize(s));
 }
@@ -634,7 +680,9 @@ std::vector<int> pgn_batch::newton_step() {
     if (st[k] == ACTIVE) step_load(k);
   upload_loads();
   d2d(xsave.p, M->x.p);
-  eval_eg(E);
+  std::vector<uint8_t> live(Kn);
+  for (int k = 0; k < Kn; ++k) live[k] = st[k] == ACTIVE;
+  eval_eg(E, &live);
   bool need_h = false;
   std::vector<double> tol_lin(Kn, o.lsTolMaxError);
   for (int k = 0; k < Kn; ++k) {
@@ -714,7 +762,8 @@ std::vector<int> pgn_batch::newton_step() {
     if (!anys) break;
     fac.upload(hfac, s);
     launch_batch_step(s, ba, n, fac.p, xsave.p, dx.p, M->x.p);
-    eval_eg(E);
+    std::vector<uint8_t> srch(searching.begin(), searching.end());
+    eval_eg(E, &srch);
     bool restore = false;
     for (int k = 0; k < Kn; ++k) {
       hfac[k] = -1.0;

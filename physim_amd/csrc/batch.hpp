@@ -59,8 +59,12 @@ struct pgn_batch {
   void upload_loads();
   bool step_load(int k);
   bool is_loaded(int k) const;
-  // per-instance {E, |g|^2, energy scale, elastic energy} at the current x
-  void eval_eg(std::vector<double>& out4);
+  // per-instance {E, |g|^2, energy scale, elastic energy} at the current x;
+  // with act, only the instances it marks are evaluated (their element and
+  // node ranges alone, the same bits as a whole-batch evaluation) and the
+  // others' entries keep their last values
+  void eval_eg(std::vector<double>& out4, const std::vector<uint8_t>* act = nullptr);
+  pgn::DBuf<uint8_t> eg_mask;
   void reset();
   void seg_plan(const std::vector<uint8_t>& act);
   pgn::SegPcgArgs seg_args(int precond) const;

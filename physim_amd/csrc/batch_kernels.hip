@@ -17,9 +17,11 @@ constexpr int BR_BLOCK = 256;
 
 __global__ __launch_bounds__(BR_BLOCK) void batch_reduce_kernel(
     BatchArgs A, const double* __restrict__ elemE, const double* __restrict__ nodeE,
-    const double* __restrict__ nodeG2, double* __restrict__ out) {
+    const double* __restrict__ nodeG2, double* __restrict__ out,
+    const uint8_t* __restrict__ mask) {
   __shared__ double red[2 * BR_BLOCK / 64];
   const int k = blockIdx.x;
+  if (mask && !mask[k]) return;
   double el = 0.0, ld = 0.0, gg = 0.0;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -98,10 +100,11 @@ __global__ __launch_bounds__(256) void batch_copy_kernel(long nd, const int* __r
 static inline unsigned grid_of(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 void launch_batch_reduce(hipStream_t s, const BatchArgs& a, const double* elemE,
-                         const double* nodeE, const double* nodeG2, double* out) {
+                         const double* nodeE, const double* nodeG2, double* out,
+                         const uint8_t* mask) {
   if (a.K == 0) return;
   hipLaunchKernelGGL(batch_reduce_kernel, dim3(a.K), dim3(BR_BLOCK), 0, s, a, elemE, nodeE,
-                     nodeG2, out);
+                     nodeG2, out, mask);
 }
 
 void launch_batch_norm2(hipStream_t s, const BatchArgs& a, const double* v, double* out) {

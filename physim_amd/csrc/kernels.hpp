@@ -83,6 +83,9 @@ struct NodeArgs {
   // backward-Euler step toward the predictor xhat (inv_dt2 = 1/dt^2)
   const double* xhat = nullptr;
   double inv_dt2 = 0.0;
+  // nodes [i0, i0 + n) of the model (a batch gathers its active instances'
+  // node ranges alone)
+  long i0 = 0;
 };
 void launch_node_gather(hipStream_t s, bool fp32_eg, const NodeArgs& a,
                         const void* eg, double* g, double* node_partials);
@@ -336,7 +339,8 @@ struct BatchArgs {
 // and per-node load energies / squared gradients (one workgroup per
 // instance, fixed order)
 void launch_batch_reduce(hipStream_t s, const BatchArgs& a, const double* elemE,
-                         const double* nodeE, const double* nodeG2, double* out);
+                         const double* nodeE, const double* nodeG2, double* out,
+                         const uint8_t* mask = nullptr);  // mask: instances to reduce
 // out[k] = |v_k|^2 over the instance's 3 DoFs per node
 void launch_batch_norm2(hipStream_t s, const BatchArgs& a, const double* v, double* out);
 // x = xsave + fac[k] dx on instances with fac[k] >= 0 (others untouched)
