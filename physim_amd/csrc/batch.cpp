@@ -331,7 +331,7 @@ SegPcgArgs pgn_batch::seg_args(int precond) const {
   a.bval = M->bval.p;
   a.dinv = M->dinv.p;
   a.binv = M->binv.p;
-  a.precond = precond >= 1 ? 1 : 0;  // the two-level preconditioner is for single models
+  a.precond = precond >= 1 ? 1 : 0;
   a.lpr = seg_lpr;
   const size_t v = 3L * M->n * M->tsize();
   char* base = M->pv.p;
@@ -345,7 +345,113 @@ SegPcgArgs pgn_batch::seg_args(int precond) const {
   a.part2 = seg_part2.p;
   a.partb = seg_partb.p;
   a.sc = seg_sc.p;
+  if (precond == 2 && sc_ready) {
+    a.coarse = 1;
+    SegCoarseArgs& c = a.cg;
+    c.NA = sc_na;
+    c.agg_ptr = sc_agg_ptr.p;
+    c.agg_nodes = sc_agg_nodes.p;
+    c.agg_of = sc_agg_of.p;
+    c.agg_inst = sc_agg_inst.p;
+    c.inst_aoff = sc_aoff.p;
+    c.inst_moff = sc_moff.p;
+    c.rel = sc_rel.p;
+    c.cmask = sc_cmask.p;
+    c.fixmask = M->fixmask.p;
+    c.R = sc_A.p;
+    c.ok = sc_ok.p;
+    c.rc = sc_rc.p;
+    c.uc = sc_uc.p;
+    c.cdot = sc_cdot.p;
+  }
   return a;
+}
+
+void pgn_batch::seg_coarse_prepare() {
+  if (sc_ready) return;
+  if (const char* e = std::getenv("PGN_SEG_COARSE_NODES")) sc_nodes = std::max(1, std::atoi(e));
+  const int Kn = K();
+  const long n = M->n;
+  const uint8_t* fix = M->hfix.size() == 3u * n ? M->hfix.data() : nullptr;
+  std::vector<int> agg_of(n), agg_inst;
+  std::vector<double> rel(3 * n);
+  std::vector<uint8_t> cmask;
+  std::vector<long> moff(Kn + 1, 0);
+  h_sc_aoff.assign(Kn + 1, 0);
+  for (int k = 0; k < Kn; ++k) {
+    const long o = hnoff[k], nk = hnoff[k + 1] - o;
+    int na = 0;
+    if (nk > 0) {
+      const int target =
+          (int)std::min<long>(std::max<long>(std::lround((double)nk / sc_nodes), 2), SEG_COARSE_MAXAGG);
+      na = bin_aggregates(M->hX0.data() + 3 * o, fix ? fix + 3 * o : nullptr, nk, target,
+                          SEG_COARSE_MAXAGG, agg_of.data() + o, rel.data() + 3 * o, cmask,
+                          nullptr);
+      for (long i = o; i < o + nk; ++i) agg_of[i] += h_sc_aoff[k];
+    }
+    h_sc_aoff[k + 1] = h_sc_aoff[k] + na;
+    agg_inst.insert(agg_inst.end(), na, k);
+    moff[k + 1] = moff[k] + 36L * na * na;
+  }
+  const int NA = h_sc_aoff[Kn];
+  std::vector<int> agg_ptr(NA + 1, 0);
+  for (long i = 0; i < n; ++i) agg_ptr[agg_of[i] + 1]++;
+  for (int a = 0; a < NA; ++a) agg_ptr[a + 1] += agg_ptr[a];
+  std::vector<int> nodes(n), cur(agg_ptr.begin(), agg_ptr.end() - 1);
+  for (long i = 0; i < n; ++i) nodes[cur[agg_of[i]]++] = (int)i;
+  CoarsePairs cp;
+  coarse_pairs(M->pat, agg_of, NA, cp);
+  hipStream_t s = ctx->stream;
+  sc_agg_ptr.upload(agg_ptr, s);
+  sc_agg_nodes.upload(nodes, s);
+  sc_agg_of.upload(agg_of, s);
+  sc_agg_inst.upload(agg_inst, s);
+  sc_aoff.upload(h_sc_aoff, s);
+  sc_moff.upload(moff, s);
+  sc_cmask.upload(cmask, s);
+  sc_cb_a.upload(cp.cb_a, s);
+  sc_cb_b.upload(cp.cb_b, s);
+  sc_pair_piece0.upload(cp.pair_piece0, s);
+  sc_piece_lo.upload(cp.piece_lo, s);
+  sc_cb_fine.upload(cp.cb_fine, s);
+  std::vector<float> r32;
+  if (M->hfp32) {
+    r32.assign(rel.begin(), rel.end());
+    sc_rel.upload(reinterpret_cast<const char*>(r32.data()), r32.size() * sizeof(float), s);
+  } else {
+    sc_rel.upload(reinterpret_cast<const char*>(rel.data()), rel.size() * sizeof(double), s);
+  }
+  sc_A.ensure(std::max<long>(moff[Kn], 1));
+  sc_ok.ensure(Kn);
+  sc_npiece = (int)cp.piece_lo.size() - 1;
+  sc_ncb = (int)cp.cb_a.size();
+  sc_piece_sum.ensure(36L * std::max(sc_npiece, 1));
+  sc_rc.ensure(6L * std::max(NA, 1));
+  sc_uc.ensure(6L * std::max(NA, 1));
+  sc_cdot.ensure(2L * Kn);
+  PGN_HIP(hipMemsetAsync(sc_cdot.p, 0, sc_cdot.bytes(), s));
+  sc_na = NA;
+  PGN_HIP(hipStreamSynchronize(s));  // the host vectors above are released on return
+  sc_ready = true;
+}
+
+void pgn_batch::seg_coarse_update() {
+  seg_coarse_prepare();
+  CoarseBuild b{};
+  b.ncb = sc_ncb;
+  b.npiece = sc_npiece;
+  b.cb_a = sc_cb_a.p;
+  b.cb_b = sc_cb_b.p;
+  b.pair_piece0 = sc_pair_piece0.p;
+  b.piece_lo = sc_piece_lo.p;
+  b.cb_fine = sc_cb_fine.p;
+  b.piece_sum = sc_piece_sum.p;
+  b.brow = M->brow.p;
+  b.bcol = M->bcol.p;
+  b.bval = M->bval.p;
+  b.ok = sc_ok.p;
+  launch_seg_coarse_operator(ctx->stream, M->hfp32, seg_args(2), b, sc_A.p, sc_ok.p);
+  PGN_HIP(hipGetLastError());
 }
 
 void pgn_batch::seg_pcg(const std::vector<double>& tol, int maxit, int precond, bool x_zero,
@@ -360,6 +466,7 @@ void pgn_batch::seg_pcg(const std::vector<double>& tol, int maxit, int precond, 
     return;
   }
   launch_seg_init(s, f32, a, M->pb.p, x_zero ? 1 : 0);
+  if (a.coarse) launch_seg_coarse_apply(s, f32, a, -1);
   // chunks after the first replay a captured graph when they have the full
   // length (launch iterations relative to the chunk, read from kctl)
   SegPcgArgs ga = a;
@@ -373,6 +480,7 @@ void pgn_batch::seg_pcg(const std::vector<double>& tol, int maxit, int precond, 
         for (int j = 0; j < c; ++j) {
           launch_seg_k1(cs, f32, ga, j);
           launch_seg_k2(cs, f32, ga, j);
+          if (ga.coarse) launch_seg_coarse_apply(cs, f32, ga, j);
         }
         launch_seg_status(cs, ga, c, 0);
       });
@@ -386,6 +494,7 @@ void pgn_batch::seg_pcg(const std::vector<double>& tol, int maxit, int precond, 
       for (int j = 0; j < c; ++j) {
         launch_seg_k1(s, f32, a, k + j);
         launch_seg_k2(s, f32, a, k + j);
+        if (a.coarse) launch_seg_coarse_apply(s, f32, a, k + j);
       }
       k += c;
       launch_seg_status(s, a, k, maxit);
@@ -413,6 +522,7 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
   if (xprev.n < (size_t)N) xprev.alloc(N);
   if (kvec.n < (size_t)Kn) kvec.alloc(Kn);
   if (kvec2.n < (size_t)Kn) kvec2.alloc(Kn);
+  PGN_CHECK(o.precond >= 0 && o.precond <= 2, "precond: 0 (none), 1 (block Jacobi) or 2 (two-level)");
   if (o.solver == PGN_LS_EIGENLDLT) {
     iters.assign(Kn, 0);
     ldlt_solve(b, x, active, tol, o);
@@ -460,6 +570,7 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
     mask.upload(F, s);
     launch_batch_copy(s, ba, n, mask.p, xprev.p, x);
     seg_plan(F);
+    if (o.precond == 2) seg_coarse_update();  // the instances of F, this try's matrix
     // ---- one unregularized solve of the instances in F ----
     if (!M->hfp32) {
       std::vector<double> tv(Kn);

@@ -68,6 +68,24 @@ struct pgn_batch {
   void reset();
   void seg_plan(const std::vector<uint8_t>& act);
   pgn::SegPcgArgs seg_args(int precond) const;
+  // Two-level preconditioner of the segmented PCG (precond 2): aggregates of
+  // each instance's rest positions (about one per sc_nodes nodes, 2 to
+  // SEG_COARSE_MAXAGG per instance; PGN_SEG_COARSE_NODES in the environment
+  // sets sc_nodes), built on the first solve that asks for it; each
+  // instance's coarse operator is assembled and inverted for the instances of
+  // the current workgroup table before every solve try (seg_coarse_update)
+  bool sc_ready = false;
+  int sc_nodes = 1000;
+  int sc_na = 0, sc_ncb = 0, sc_npiece = 0;
+  std::vector<int> h_sc_aoff;  // K + 1
+  pgn::DBuf<int> sc_agg_ptr, sc_agg_nodes, sc_agg_of, sc_agg_inst, sc_aoff;
+  pgn::DBuf<long> sc_moff;
+  pgn::DBuf<int> sc_cb_a, sc_cb_b, sc_pair_piece0, sc_piece_lo, sc_cb_fine, sc_ok;
+  pgn::DBuf<char> sc_rel;
+  pgn::DBuf<uint8_t> sc_cmask;
+  pgn::DBuf<double> sc_A, sc_piece_sum, sc_rc, sc_uc, sc_cdot;
+  void seg_coarse_prepare();
+  void seg_coarse_update();
   // PCG of every instance at once on the PCG rhs (model precision, M->pb) and
   // solution (M->pv x); tol[k] = +inf leaves instance k untouched.  Returns
   // per-instance iteration counts.

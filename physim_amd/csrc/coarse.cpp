@@ -48,24 +48,16 @@ uint8_t independent_columns(const double G[36]) {
 
 }  // namespace
 
-}  // namespace pgn
-
-using namespace pgn;
-
-void pgn_model::coarse_prepare() {
-  if (coarse_ready) return;
-  if (const char* e = std::getenv("PGN_COARSE_AGGS")) coarse_target = std::max(1, std::atoi(e));
-  const int max_aggs = COARSE_MAX_PAD / 6;
-  const int target = std::min(coarse_target, max_aggs);
+int bin_aggregates(const double* X, const uint8_t* fix3, long n, int target, int max_aggs,
+                   int* agg_of, double* rel, std::vector<uint8_t>& cmask, int bins_out[3]) {
+  target = std::max(1, std::min(target, max_aggs));
   // bins shaped to the bounding box
   double lo[3], hi[3];
-  for (int k = 0; k < 3; ++k) {
-    lo[k] = hi[k] = hX0[k];
-  }
-  for (int i = 0; i < n; ++i)
+  for (int k = 0; k < 3; ++k) lo[k] = hi[k] = X[k];
+  for (long i = 0; i < n; ++i)
     for (int k = 0; k < 3; ++k) {
-      lo[k] = std::min(lo[k], hX0[3L * i + k]);
-      hi[k] = std::max(hi[k], hX0[3L * i + k]);
+      lo[k] = std::min(lo[k], X[3 * i + k]);
+      hi[k] = std::max(hi[k], X[3 * i + k]);
     }
   double ext[3], emax = 0.0;
   for (int k = 0; k < 3; ++k) {
@@ -86,13 +78,13 @@ void pgn_model::coarse_prepare() {
       bins[k]--;
     }
   }
-  std::copy(bins, bins + 3, coarse_bins);
+  if (bins_out) std::copy(bins, bins + 3, bins_out);
   std::vector<long> key(n);
-  for (int i = 0; i < n; ++i) {
+  for (long i = 0; i < n; ++i) {
     long b[3];
     for (int k = 0; k < 3; ++k) {
       long v = 0;
-      if (ext[k] > 0.0) v = (long)std::floor((hX0[3L * i + k] - lo[k]) / ext[k] * bins[k]);
+      if (ext[k] > 0.0) v = (long)std::floor((X[3 * i + k] - lo[k]) / ext[k] * bins[k]);
       b[k] = std::min<long>(std::max<long>(v, 0), bins[k] - 1);
     }
     key[i] = (b[2] * bins[1] + b[1]) * bins[0] + b[0];
@@ -101,78 +93,116 @@ void pgn_model::coarse_prepare() {
   std::sort(keys.begin(), keys.end());
   keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
   const int na = (int)keys.size();
-  h_agg_of.assign(n, 0);
-  for (int i = 0; i < n; ++i)
-    h_agg_of[i] = (int)(std::lower_bound(keys.begin(), keys.end(), key[i]) - keys.begin());
+  for (long i = 0; i < n; ++i)
+    agg_of[i] = (int)(std::lower_bound(keys.begin(), keys.end(), key[i]) - keys.begin());
   // centroids, offsets, column masks
   std::vector<double> cen(3L * na, 0.0);
   std::vector<int> cnt(na, 0);
-  for (int i = 0; i < n; ++i) {
-    const int a = h_agg_of[i];
+  for (long i = 0; i < n; ++i) {
+    const int a = agg_of[i];
     cnt[a]++;
-    for (int k = 0; k < 3; ++k) cen[3L * a + k] += hX0[3L * i + k];
+    for (int k = 0; k < 3; ++k) cen[3L * a + k] += X[3 * i + k];
   }
   for (int a = 0; a < na; ++a)
     for (int k = 0; k < 3; ++k) cen[3L * a + k] /= cnt[a];
-  std::vector<double> rel(3L * n);
-  for (int i = 0; i < n; ++i)
-    for (int k = 0; k < 3; ++k) rel[3L * i + k] = hX0[3L * i + k] - cen[3L * h_agg_of[i] + k];
+  for (long i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) rel[3 * i + k] = X[3 * i + k] - cen[3L * agg_of[i] + k];
   // Gram matrix sum_i P_i^T F_i P_i per aggregate, rotation columns scaled by
   // the aggregate's rms offset (so that the independence test is unit-free)
   std::vector<double> rms(na, 0.0);
-  for (int i = 0; i < n; ++i)
-    for (int k = 0; k < 3; ++k) rms[h_agg_of[i]] += rel[3L * i + k] * rel[3L * i + k];
+  for (long i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) rms[agg_of[i]] += rel[3 * i + k] * rel[3 * i + k];
   for (int a = 0; a < na; ++a) rms[a] = std::sqrt(rms[a] / cnt[a]);
   std::vector<double> G(36L * na, 0.0);
-  for (int i = 0; i < n; ++i) {
-    const int a = h_agg_of[i];
+  for (long i = 0; i < n; ++i) {
+    const int a = agg_of[i];
     const double sc = rms[a] > 0.0 ? 1.0 / rms[a] : 0.0;
-    const double x = rel[3L * i] * sc, y = rel[3L * i + 1] * sc, z = rel[3L * i + 2] * sc;
+    const double x = rel[3 * i] * sc, y = rel[3 * i + 1] * sc, z = rel[3 * i + 2] * sc;
     // rows of P_i: component p of (t + w x rel)
     const double P[3][6] = {{1, 0, 0, 0, z, -y}, {0, 1, 0, -z, 0, x}, {0, 0, 1, y, -x, 0}};
     for (int p = 0; p < 3; ++p) {
-      if (hfix.size() == 3u * n && hfix[3L * i + p]) continue;
+      if (fix3 && fix3[3 * i + p]) continue;
       for (int u = 0; u < 6; ++u)
         for (int v = 0; v < 6; ++v) G[36L * a + 6 * u + v] += P[p][u] * P[p][v];
     }
   }
-  std::vector<uint8_t> cmask(na);
-  for (int a = 0; a < na; ++a) cmask[a] = independent_columns(&G[36L * a]);
+  for (int a = 0; a < na; ++a) cmask.push_back(independent_columns(&G[36L * a]));
+  return na;
+}
+
+void coarse_pairs(const BsrPattern& P, const std::vector<int>& agg_of, int na, CoarsePairs& out) {
+  // per aggregate a, the aggregates b >= a its blocks reach (few: a box and
+  // its neighbours), with their block counts
+  std::vector<std::vector<std::pair<int, int>>> reach(na);
+  auto slot = [&](int a, int b) -> std::pair<int, int>& {
+    for (auto& e : reach[a])
+      if (e.first == b) return e;
+    reach[a].emplace_back(b, 0);
+    return reach[a].back();
+  };
+  for (long f = 0; f < P.nb; ++f) {
+    const int a = agg_of[P.brow[f]], b = agg_of[P.bcol[f]];
+    if (a <= b) slot(a, b).second++;
+  }
+  out.cb_a.clear();
+  out.cb_b.clear();
+  std::vector<int> ptr(1, 0);
+  std::vector<std::vector<int>> index(na);
+  for (int a = 0; a < na; ++a) {
+    std::sort(reach[a].begin(), reach[a].end());
+    for (const auto& e : reach[a]) {
+      index[a].push_back((int)out.cb_a.size());
+      out.cb_a.push_back(a);
+      out.cb_b.push_back(e.first);
+      ptr.push_back(ptr.back() + e.second);
+    }
+  }
+  out.cb_fine.assign(ptr.back(), 0);
+  std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+  for (long f = 0; f < P.nb; ++f) {
+    const int a = agg_of[P.brow[f]], b = agg_of[P.bcol[f]];
+    if (a > b) continue;
+    const auto& r = reach[a];
+    const int j = (int)(std::lower_bound(r.begin(), r.end(), std::make_pair(b, 0)) - r.begin());
+    out.cb_fine[pos[index[a][j]]++] = (int)f;
+  }
+  // pieces of at most COARSE_PIECE blocks, pair by pair
+  out.pair_piece0.assign(out.cb_a.size() + 1, 0);
+  out.piece_lo.clear();
+  for (size_t c = 0; c < out.cb_a.size(); ++c) {
+    out.pair_piece0[c] = (int)out.piece_lo.size();
+    for (int t = ptr[c]; t < ptr[c + 1]; t += COARSE_PIECE) out.piece_lo.push_back(t);
+  }
+  out.pair_piece0[out.cb_a.size()] = (int)out.piece_lo.size();
+  out.piece_lo.push_back(ptr.back());
+}
+
+}  // namespace pgn
+
+using namespace pgn;
+
+void pgn_model::coarse_prepare() {
+  if (coarse_ready) return;
+  if (const char* e = std::getenv("PGN_COARSE_AGGS")) coarse_target = std::max(1, std::atoi(e));
+  h_agg_of.assign(n, 0);
+  std::vector<double> rel(3L * n);
+  std::vector<uint8_t> cmask;
+  const int na = bin_aggregates(hX0.data(), hfix.size() == 3u * n ? hfix.data() : nullptr, n,
+                                coarse_target, COARSE_MAX_PAD / 6, h_agg_of.data(), rel.data(),
+                                cmask, coarse_bins);
   // nodes by aggregate
   std::vector<int> agg_ptr(na + 1, 0);
   for (int i = 0; i < n; ++i) agg_ptr[h_agg_of[i] + 1]++;
   for (int a = 0; a < na; ++a) agg_ptr[a + 1] += agg_ptr[a];
   std::vector<int> nodes(n), cur(agg_ptr.begin(), agg_ptr.end() - 1);
   for (int i = 0; i < n; ++i) nodes[cur[h_agg_of[i]]++] = i;
-  // coarse block pairs (a <= b) and their BSR3 blocks in block order
-  std::vector<int> pcount((size_t)na * na, 0);
-  for (long f = 0; f < pat.nb; ++f) {
-    const int a = h_agg_of[pat.brow[f]], b = h_agg_of[pat.bcol[f]];
-    if (a <= b) pcount[(size_t)a * na + b]++;
-  }
-  std::vector<int> pidx((size_t)na * na, -1), cb_a, cb_b, cb_ptr(1, 0);
-  for (int a = 0; a < na; ++a)
-    for (int b = a; b < na; ++b)
-      if (pcount[(size_t)a * na + b] > 0) {
-        pidx[(size_t)a * na + b] = (int)cb_a.size();
-        cb_a.push_back(a);
-        cb_b.push_back(b);
-        cb_ptr.push_back(cb_ptr.back() + pcount[(size_t)a * na + b]);
-      }
-  std::vector<int> cb_fine(cb_ptr.back());
-  std::vector<int> pos(cb_ptr.begin(), cb_ptr.end() - 1);
-  for (long f = 0; f < pat.nb; ++f) {
-    const int a = h_agg_of[pat.brow[f]], b = h_agg_of[pat.bcol[f]];
-    if (a <= b) cb_fine[pos[pidx[(size_t)a * na + b]]++] = (int)f;
-  }
-  // pieces of at most COARSE_PIECE blocks, pair by pair
-  std::vector<int> pair_piece0(cb_a.size() + 1, 0), piece_lo;
-  for (size_t c = 0; c < cb_a.size(); ++c) {
-    pair_piece0[c] = (int)piece_lo.size();
-    for (int t = cb_ptr[c]; t < cb_ptr[c + 1]; t += COARSE_PIECE) piece_lo.push_back(t);
-  }
-  pair_piece0[cb_a.size()] = (int)piece_lo.size();
-  piece_lo.push_back(cb_ptr.back());
+  CoarsePairs cp;
+  coarse_pairs(pat, h_agg_of, na, cp);
+  const std::vector<int>& cb_a = cp.cb_a;
+  const std::vector<int>& cb_b = cp.cb_b;
+  const std::vector<int>& pair_piece0 = cp.pair_piece0;
+  const std::vector<int>& piece_lo = cp.piece_lo;
+  const std::vector<int>& cb_fine = cp.cb_fine;
   // device copies
   hipStream_t s = stream();
   cg_agg_ptr.upload(agg_ptr, s);

@@ -64,6 +64,7 @@ __global__ __launch_bounds__(CB_BS) void coarse_piece_kernel(CoarseArgs c, Coars
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
   const int lo = b.piece_lo[pc], hi = b.piece_lo[pc + 1];
+  if (b.live && b.live[b.agg_inst[c.agg_of[b.brow[b.cb_fine[lo]]]]] == 0) return;
   for (int t = lo + threadIdx.x; t < hi; t += CB_BS) {
     const int f = b.cb_fine[t];
     const int i = b.brow[f], j = b.bcol[f];
@@ -433,6 +434,207 @@ __global__ __launch_bounds__(PR_BS) void coarse_prolong_kernel(PcgArgs A) {
   if (!(fm & 4u)) z[3 * i + 2] = (T)((double)z2 + d2);
 }
 
+// ---- segmented (one coarse operator per instance of a batch) ------------------
+
+constexpr int SG_BS = 512;  // sweep / fill workgroup: one instance each
+constexpr int SG_MAXDOF = 6 * SEG_COARSE_MAXAGG;
+
+__device__ __forceinline__ int seg_k_abs(const SegPcgArgs& A, int k) {
+  return A.kctl ? A.kctl[0] + k : k;
+}
+
+// one workgroup per instance: its dense operator zeroed, ok = 1
+__global__ __launch_bounds__(SG_BS) void seg_coarse_fill_kernel(SegPcgArgs A, double* M, int* ok) {
+  const int sg = blockIdx.x;
+  if (A.seg_G[sg] == 0) return;
+  const SegCoarseArgs& c = A.cg;
+  const int nc = 6 * (c.inst_aoff[sg + 1] - c.inst_aoff[sg]);
+  double* Mk = M + c.inst_moff[sg];
+  for (int e = threadIdx.x; e < nc * nc; e += SG_BS) Mk[e] = 0.0;
+  if (threadIdx.x == 0) ok[sg] = 1;
+}
+
+// as coarse_pairs_kernel, into the instance's own operator
+__global__ __launch_bounds__(256) void seg_coarse_pairs_kernel(SegPcgArgs A, CoarseBuild b,
+                                                               double* M) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= 36L * b.ncb) return;
+  const SegCoarseArgs& c = A.cg;
+  const int cb = (int)(t / 36), e = (int)(t % 36), p = e / 6, q = e % 6;
+  const int ca = b.cb_a[cb], cbb = b.cb_b[cb];
+  const int sg = c.agg_inst[ca];
+  if (A.seg_G[sg] == 0) return;
+  double v = 0.0;
+  for (int k = b.pair_piece0[cb]; k < b.pair_piece0[cb + 1]; ++k) v += b.piece_sum[36L * k + e];
+  const bool act_p = (c.cmask[ca] >> p) & 1u, act_q = (c.cmask[cbb] >> q) & 1u;
+  if (!act_p || !act_q) v = (ca == cbb && p == q) ? 1.0 : 0.0;
+  const int a0 = c.inst_aoff[sg], nc = 6 * (c.inst_aoff[sg + 1] - a0);
+  const long rI = 6L * (ca - a0) + p, cJ = 6L * (cbb - a0) + q;
+  double* Mk = M + c.inst_moff[sg];
+  Mk[rI * nc + cJ] = v;
+  if (ca != cbb) Mk[cJ * nc + rI] = v;
+}
+
+// One workgroup per instance: Gauss-Jordan sweep of its nc x nc operator in
+// place (pivot row and column staged in LDS, two barriers per pivot; the
+// entries stay in the workgroup's own cache lines), then symmetrized.  After
+// the nc pivots the storage holds -Ac^-1; a pivot that is not positive
+// relative to its starting diagonal clears ok.
+__global__ __launch_bounds__(SG_BS) void seg_coarse_sweep_kernel(SegPcgArgs A, double* M,
+                                                                 int* ok) {
+  __shared__ double rowp[SG_MAXDOF], colp[SG_MAXDOF], dst0[SG_MAXDOF];
+  const int sg = blockIdx.x;
+  if (A.seg_G[sg] == 0) return;
+  const SegCoarseArgs& c = A.cg;
+  const int nc = 6 * (c.inst_aoff[sg + 1] - c.inst_aoff[sg]);
+  double* Mk = M + c.inst_moff[sg];
+  const int t = threadIdx.x;
+  for (int e = t; e < nc; e += SG_BS) dst0[e] = Mk[e * nc + e];
+  bool good = true;
+  for (int p = 0; p < nc; ++p) {
+    __syncthreads();
+    for (int e = t; e < nc; e += SG_BS) {
+      rowp[e] = Mk[p * nc + e];
+      colp[e] = Mk[e * nc + p];
+    }
+    __syncthreads();
+    double d = rowp[p];
+    if (!(d > 1e-13 * fabs(dst0[p]) && d > 0.0)) {
+      good = false;
+      d = 1.0;  // keeps the remaining arithmetic finite; the result is flagged
+    }
+    const double inv = 1.0 / d;
+    for (int e = t; e < nc * nc; e += SG_BS) {
+      const int i = e / nc, j = e - i * nc;
+      double v;
+      if (i == p && j == p) v = -inv;
+      else if (i == p) v = rowp[j] * inv;
+      else if (j == p) v = colp[i] * inv;
+      else v = Mk[e] - colp[i] * rowp[j] * inv;
+      Mk[e] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < nc * nc; e += SG_BS) {
+    const int i = e / nc, j = e - i * nc;
+    if (i < j) {
+      const double v = 0.5 * (Mk[e] + Mk[j * nc + i]);
+      Mk[e] = v;
+      Mk[j * nc + i] = v;
+    }
+  }
+  if (!good && t == 0) ok[sg] = 0;
+}
+
+constexpr int SA_BS = 1024;  // segmented coarse product: one instance per workgroup
+
+// P^T r for aggregate a (blockIdx.x), its nodes in order, RS_U per thread in
+// flight, into rc[6 a .. 6 a + 5]
+template <class T>
+__global__ __launch_bounds__(RS_BS) void seg_coarse_restrict_kernel(SegPcgArgs A) {
+  __shared__ double red[RS_BS / 64][6];
+  const SegCoarseArgs& c = A.cg;
+  const int a = blockIdx.x, sg = c.agg_inst[a];
+  if (A.seg_G[sg] == 0 || A.sc[sg].done) return;
+  const T* __restrict__ r = reinterpret_cast<const T*>(A.r);
+  const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
+  const int lo = c.agg_ptr[a], hi = c.agg_ptr[a + 1];
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t0 = lo + threadIdx.x; t0 < hi; t0 += RS_BS * RS_U) {
+    int idx[RS_U];
+#pragma unroll
+    for (int u = 0; u < RS_U; ++u) {
+      const int t = t0 + u * RS_BS;
+      idx[u] = t < hi ? c.agg_nodes[t] : -1;
+    }
+    T rv[RS_U][3], rl[RS_U][3];
+    unsigned fm[RS_U];
+#pragma unroll
+    for (int u = 0; u < RS_U; ++u) {
+      const long i = idx[u] < 0 ? 0 : idx[u];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        rv[u][p] = r[3 * i + p];
+        rl[u][p] = rel[3 * i + p];
+      }
+      fm[u] = idx[u] < 0 ? 7u : (unsigned)c.fixmask[i];
+    }
+#pragma unroll
+    for (int u = 0; u < RS_U; ++u) {
+      double v[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) v[p] = ((fm[u] >> p) & 1u) ? 0.0 : (double)rv[u][p];
+      const double x = rl[u][0], y = rl[u][1], z = rl[u][2];
+      acc[0] += v[0];
+      acc[1] += v[1];
+      acc[2] += v[2];
+      acc[3] += y * v[2] - z * v[1];
+      acc[4] += z * v[0] - x * v[2];
+      acc[5] += x * v[1] - y * v[0];
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const double v = wave_sum(acc[e]);
+    if (lane == 0) red[w][e] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int e = threadIdx.x;
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < RS_BS / 64; ++q) v += red[q][e];
+    if (!((c.cmask[a] >> e) & 1u)) v = 0.0;
+    c.rc[6L * a + e] = v;
+  }
+}
+
+// uc = Ac^-1 rc for instance blockIdx.x: R is symmetric, so thread (q, i)
+// sums column i over the q-th of SA_BS / nc column ranges (loads coalesced
+// across i), the ranges are added in order, and rc.uc goes into
+// cdot[(k & 1) K + instance].  uc = 0 when the instance's sweep failed.
+__global__ __launch_bounds__(SA_BS) void seg_coarse_gemv_kernel(SegPcgArgs A, int k) {
+  static_assert(SG_MAXDOF <= SA_BS, "one coarse row per thread at least");
+  __shared__ double src[SG_MAXDOF];
+  __shared__ double part[SA_BS];
+  __shared__ double red1[SA_BS / 64];
+  const int sg = blockIdx.x;
+  if (A.seg_G[sg] == 0 || A.sc[sg].done) return;
+  k = seg_k_abs(A, k);
+  const SegCoarseArgs& c = A.cg;
+  const int t = threadIdx.x;
+  const int a0 = c.inst_aoff[sg], nc = 6 * (c.inst_aoff[sg + 1] - a0);
+  if (nc == 0) return;
+  if (t < nc) src[t] = c.rc[6L * a0 + t];
+  __syncthreads();
+  const double* __restrict__ Mk = c.R + c.inst_moff[sg];
+  const int P = SA_BS / nc;
+  if (t < P * nc) {
+    const int i = t % nc, q = t / nc;
+    const int j0 = q * nc / P, j1 = (q + 1) * nc / P;
+    double v0 = 0.0, v1 = 0.0;
+    int j = j0;
+    for (; j + 2 <= j1; j += 2) {
+      v0 += Mk[(long)j * nc + i] * src[j];
+      v1 += Mk[(long)(j + 1) * nc + i] * src[j + 1];
+    }
+    if (j < j1) v0 += Mk[(long)j * nc + i] * src[j];
+    part[t] = v0 + v1;
+  }
+  __syncthreads();
+  double dl = 0.0;
+  if (t < nc) {
+    double v = 0.0;
+    for (int q = 0; q < P; ++q) v += part[q * nc + t];
+    const double u = c.ok[sg] != 0 ? -v : 0.0;
+    c.uc[6L * a0 + t] = u;
+    dl = src[t] * u;
+  }
+  const double s = block_sum<SA_BS>(dl, red1);
+  if (t == 0) c.cdot[(long)(k & 1) * A.K + sg] = s;
+}
+
 }  // namespace
 
 double* launch_coarse_operator(hipStream_t s, bool fp32, const CoarseArgs& c,
@@ -478,5 +680,35 @@ void launch_coarse_apply(hipStream_t s, bool fp32, const PcgArgs& a) {
 }
 
 int coarse_gemv_grid(int npad) { return npad / GV_ROWS; }
+
+void launch_seg_coarse_operator(hipStream_t s, bool fp32, const SegPcgArgs& a,
+                                const CoarseBuild& b, double* A, int* ok) {
+  hipLaunchKernelGGL(seg_coarse_fill_kernel, dim3(a.K), dim3(SG_BS), 0, s, a, A, ok);
+  if (b.npiece > 0) {
+    CoarseArgs c;  // what the piece kernel reads
+    c.rel = a.cg.rel;
+    c.fixmask = a.cg.fixmask;
+    c.agg_of = a.cg.agg_of;
+    CoarseBuild bs = b;
+    bs.agg_inst = a.cg.agg_inst;
+    bs.live = a.seg_G;
+    if (fp32)
+      hipLaunchKernelGGL(coarse_piece_kernel<float>, dim3(b.npiece), dim3(CB_BS), 0, s, c, bs);
+    else
+      hipLaunchKernelGGL(coarse_piece_kernel<double>, dim3(b.npiece), dim3(CB_BS), 0, s, c, bs);
+    const int pg = (int)((36L * b.ncb + 255) / 256);
+    hipLaunchKernelGGL(seg_coarse_pairs_kernel, dim3(pg), dim3(256), 0, s, a, bs, A);
+  }
+  hipLaunchKernelGGL(seg_coarse_sweep_kernel, dim3(a.K), dim3(SG_BS), 0, s, a, A, ok);
+}
+
+void launch_seg_coarse_apply(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {
+  if (fp32)
+    hipLaunchKernelGGL(seg_coarse_restrict_kernel<float>, dim3(a.cg.NA), dim3(RS_BS), 0, s, a);
+  else
+    hipLaunchKernelGGL(seg_coarse_restrict_kernel<double>, dim3(a.cg.NA), dim3(RS_BS), 0, s, a);
+  hipLaunchKernelGGL(seg_coarse_gemv_kernel, dim3(a.K), dim3(SA_BS), 0, s, a, k);
+}
+
 
 }  // namespace pgn

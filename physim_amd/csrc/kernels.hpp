@@ -278,6 +278,28 @@ void launch_copy_f32(hipStream_t s, long n, const float* a, float* b);
 // [wg0[k], wg0[k] + G[k]) of every launch (the WG table is wg_seg / wg_loc);
 // each segment has its own tolerance, scalars (alpha, beta from its own
 // partial sums) and convergence state, and stops on its own.
+// Coarse level of the segmented PCG (precond 2 in a batch): each instance
+// has its own aggregates (a contiguous range of the global ids, never
+// spanning two instances) and its own dense coarse operator, so the coarse
+// correction stays per instance like every other part of the iteration.
+constexpr int SEG_COARSE_MAXAGG = 24;  // aggregates per instance at most
+struct SegCoarseArgs {
+  int NA;                   // aggregates over all instances
+  const int* agg_ptr;       // NA + 1: ranges into agg_nodes
+  const int* agg_nodes;     // n: nodes ordered by aggregate, ascending
+  const int* agg_of;        // n: global aggregate of each node
+  const int* agg_inst;      // NA: instance of each aggregate
+  const int* inst_aoff;     // K + 1: aggregate range of each instance
+  const long* inst_moff;    // K: offset of each instance's dense nc x nc operator
+  const void* rel;          // 3n, Hessian precision: node offset from its centroid
+  const uint8_t* cmask;     // NA: active coarse columns (bits 0-5)
+  const uint8_t* fixmask;   // n
+  const double* R;          // per instance -Ac^-1 (nc = 6 x its aggregates)
+  const int* ok;            // K: 0 when the instance's sweep met a non-positive pivot
+  double* rc;               // 6 NA: P^T r
+  double* uc;               // 6 NA: Ac^-1 P^T r
+  double* cdot;             // 2 x K: r.(P uc) by iteration parity
+};
 struct SegPcgArgs {
   int K;                 // segments
   int W;                 // workgroups of every launch (sum of G)
@@ -307,6 +329,8 @@ struct SegPcgArgs {
   double* partb;         // W
   PcgScalars* sc;        // K
   const int* kctl = nullptr;  // as PcgArgs::kctl
+  int coarse = 0;        // 1: two-level (precond 1 + cg)
+  SegCoarseArgs cg;
 };
 constexpr int SEG_MAXG = 256;  // workgroups per segment at most
 void launch_seg_init(hipStream_t s, bool fp32, const SegPcgArgs& a, const void* b, int x_zero);
@@ -436,6 +460,10 @@ struct CoarseBuild {
   double* A1;
   double* D;              // 2 x 32 x 32: inverses of the next pivot tiles
   int* ok;
+  // segmented builds: pieces of instances without workgroups (live[instance]
+  // == 0) are skipped
+  const int* agg_inst = nullptr;
+  const int* live = nullptr;
 };
 // Ac into A0 (dense FP64: inactive and padding DoFs get identity rows), then
 // -Ac^-1 by block Gauss-Jordan sweeps; returns the buffer that holds it.
@@ -445,6 +473,16 @@ double* launch_coarse_operator(hipStream_t s, bool fp32, const CoarseArgs& c,
 // z += P Ac^-1 P^T r after z = B^-1 r, and cdot = (P^T r).(Ac^-1 P^T r);
 // returns at once when the PCG has converged
 void launch_coarse_apply(hipStream_t s, bool fp32, const PcgArgs& a);
+// Segmented versions over the instances with workgroups in a (seg_G > 0):
+// Ac per instance from the pieces of b (b.A0..D unused) into c.R's storage
+// (A, the sum of nc^2), inverted in place by one workgroup per instance.
+// The application after iteration k (k = -1 after the init) is two
+// launches: P^T r per aggregate, then uc = Ac^-1 rc and cdot[(k & 1) K +
+// instance] per instance; the segmented SpMV of iteration k + 1 adds P uc to
+// z where it reads z.
+void launch_seg_coarse_operator(hipStream_t s, bool fp32, const SegPcgArgs& a,
+                                const CoarseBuild& b, double* A, int* ok);
+void launch_seg_coarse_apply(hipStream_t s, bool fp32, const SegPcgArgs& a, int k);
 // workgroups of the coarse product for npad coarse DoFs
 int coarse_gemv_grid(int npad);
 

@@ -35,6 +35,23 @@ struct BsrPattern {
 // Builds the pattern of a tet mesh (throws on bad node ids).
 void build_bsr_pattern(int n, int m, const int* T, BsrPattern& P);
 
+// Aggregation of the two-level preconditioner (coarse.cpp): the n rest
+// positions X binned into about target (at most max_aggs) boxes shaped to
+// their bounding box; fills agg_of (n), rel (3n, offset from the aggregate's
+// centroid) and appends one column mask per aggregate to cmask (bit c: column
+// c of [translations | rotations] kept; fix3, 3n or null, marks fixed DoFs).
+// Returns the number of aggregates; bins_out (or null) receives the grid.
+int bin_aggregates(const double* X, const uint8_t* fix3, long n, int target, int max_aggs,
+                   int* agg_of, double* rel, std::vector<uint8_t>& cmask, int bins_out[3]);
+// Coarse block pairs (a <= b) reached by the lower BSR3 blocks of P, the
+// blocks of each pair (cb_fine, in block order) and the pair's pieces of at
+// most COARSE_PIECE blocks (pair_piece0: first piece of each pair; piece_lo:
+// first entry of cb_fine of each piece, closed by the total).
+struct CoarsePairs {
+  std::vector<int> cb_a, cb_b, pair_piece0, piece_lo, cb_fine;
+};
+void coarse_pairs(const BsrPattern& P, const std::vector<int>& agg_of, int na, CoarsePairs& out);
+
 }  // namespace pgn
 
 struct pgn_model {

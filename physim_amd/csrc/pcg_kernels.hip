@@ -1265,6 +1265,10 @@ int seg_rows_per_wg(int lpr) { return K1_BS / lpr; }
 __device__ __forceinline__ const double* seg_rr_slot(const SegPcgArgs& A, int k) {
   return A.part2 + (long)(k & 1) * 2 * A.W;
 }
+// r.(P uc) of iteration k's coarse correction (two-level only)
+__device__ __forceinline__ double seg_cdot(const SegPcgArgs& A, int k, int sg) {
+  return A.coarse ? A.cg.cdot[(long)(k & 1) * A.K + sg] : 0.0;
+}
 
 template <class T, int L>
 __global__ __launch_bounds__(K1_BS) void seg_init_kernel(SegPcgArgs A, const T* __restrict__ b,
@@ -1330,20 +1334,55 @@ __global__ __launch_bounds__(K1_BS) void seg_init_kernel(SegPcgArgs A, const T* 
   }
 }
 
-template <class T, int L, bool C16>
+// z_i + F_i (t_a + w_a x rel_i): the two-level z of node i from the stored
+// block-Jacobi part and the last coarse solution uc (the prolongation is
+// applied where z is read instead of being stored)
+// ucl: the instance's uc staged in LDS, a0 its first aggregate
+template <class T>
+__device__ __forceinline__ void seg_coarse_z(const SegCoarseArgs& c, const double* ucl, int a0,
+                                             long i, T& z0, T& z1, T& z2) {
+  const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
+  const int a = c.agg_of[i] - a0;
+  const unsigned fm = c.fixmask[i];
+  const double x = rel[3 * i], y = rel[3 * i + 1], zz = rel[3 * i + 2];
+  const double* u = ucl + 6 * a;
+  const double d0 = u[0] + u[4] * zz - u[5] * y;
+  const double d1 = u[1] + u[5] * x - u[3] * zz;
+  const double d2 = u[2] + u[3] * y - u[4] * x;
+  if (!(fm & 1u)) z0 = (T)((double)z0 + d0);
+  if (!(fm & 2u)) z1 = (T)((double)z1 + d1);
+  if (!(fm & 4u)) z2 = (T)((double)z2 + d2);
+}
+
+template <class T, int L, bool C16, bool CG>
 __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
   k = chunk_k(A, k);
-  __shared__ double red[2 * K1_BS / 64 + 2];
+  __shared__ double red[3 * K1_BS / 64 + 3];
   const int w = blockIdx.x, sg = A.wg_seg[w], j = A.wg_loc[w], G = A.seg_G[sg];
   PcgScalars* sc = A.sc + sg;
   if (sc->done) return;
   const int g0 = A.seg_wg0[sg];
   const double* prev = seg_rr_slot(A, k - 1) + 2L * g0;
-  const double rr = all_sum<K1_BS>(prev, G, 2, red);
-  const double rz = all_sum<K1_BS>(prev + 1, G, 2, red);
+  // the instance's coarse solution in LDS (two-level), loaded while the
+  // partial sums are read
+  __shared__ double ucl[CG ? 6 * SEG_COARSE_MAXAGG : 1];
+  int ca0 = 0;
+  if (CG) {
+    ca0 = A.cg.inst_aoff[sg];
+    const int nc = 6 * (A.cg.inst_aoff[sg + 1] - ca0);
+    for (int e = threadIdx.x; e < nc; e += K1_BS) ucl[e] = A.cg.uc[6L * ca0 + e];
+  }
+  // r.r, r.z of iteration k - 1 and b.b (k = 0) or r.z of k - 2 in one pass
+  double rr, rz, third;
+  if (k == 0)
+    all_sum3<K1_BS>(prev, G, 2, prev + 1, G, 2, A.partb + g0, G, 1, red, rr, rz, third);
+  else
+    all_sum3<K1_BS>(prev, G, 2, prev + 1, G, 2, seg_rr_slot(A, k - 2) + 2L * g0 + 1, G, 2, red,
+                    rr, rz, third);
+  rz += seg_cdot(A, k - 1, sg);
   double thr, beta;
   if (k == 0) {
-    const double bb = all_sum<K1_BS>(A.partb + g0, G, 1, red);
+    const double bb = third;
     thr = A.tol[sg] * A.tol[sg] * bb;
     if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
     if (j == 0 && threadIdx.x == 0) {
@@ -1369,7 +1408,7 @@ __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
       }
       return;
     }
-    const double rzm = all_sum<K1_BS>(seg_rr_slot(A, k - 2) + 2L * g0 + 1, G, 2, red);
+    const double rzm = third + seg_cdot(A, k - 2, sg);
     beta = rz / rzm;
   }
   const T tb = (T)beta;
@@ -1388,9 +1427,11 @@ __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
     for (long s = A.rowptr[row] + lane; s < e1; s += L) {
       const long c = C16 ? row + A.bcol16[s] : A.bcol[s];
       const T* B = bval + s * 9;
-      const T pj0 = z[3 * c] + tb * pold[3 * c];
-      const T pj1 = z[3 * c + 1] + tb * pold[3 * c + 1];
-      const T pj2 = z[3 * c + 2] + tb * pold[3 * c + 2];
+      T zj0 = z[3 * c], zj1 = z[3 * c + 1], zj2 = z[3 * c + 2];
+      if (CG) seg_coarse_z<T>(A.cg, ucl, ca0, c, zj0, zj1, zj2);
+      const T pj0 = zj0 + tb * pold[3 * c];
+      const T pj1 = zj1 + tb * pold[3 * c + 1];
+      const T pj2 = zj2 + tb * pold[3 * c + 2];
       y0 += B[0] * pj0 + B[1] * pj1 + B[2] * pj2;
       y1 += B[3] * pj0 + B[4] * pj1 + B[5] * pj2;
       y2 += B[6] * pj0 + B[7] * pj1 + B[8] * pj2;
@@ -1400,9 +1441,11 @@ __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
     y2 = group_sum<L>(y2);
     if (lane == 0) {
       const long i = row;
-      const T pi0 = z[3 * i] + tb * pold[3 * i];
-      const T pi1 = z[3 * i + 1] + tb * pold[3 * i + 1];
-      const T pi2 = z[3 * i + 2] + tb * pold[3 * i + 2];
+      T zi0 = z[3 * i], zi1 = z[3 * i + 1], zi2 = z[3 * i + 2];
+      if (CG) seg_coarse_z<T>(A.cg, ucl, ca0, i, zi0, zi1, zi2);
+      const T pi0 = zi0 + tb * pold[3 * i];
+      const T pi1 = zi1 + tb * pold[3 * i + 1];
+      const T pi2 = zi2 + tb * pold[3 * i + 2];
       pnew[3 * i] = pi0;
       pnew[3 * i + 1] = pi1;
       pnew[3 * i + 2] = pi2;
@@ -1419,7 +1462,7 @@ __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
 template <class T>
 __global__ __launch_bounds__(SEG_K2_BS) void seg_k2_kernel(SegPcgArgs A, int k) {
   k = chunk_k(A, k);
-  __shared__ double red[2 * SEG_K2_BS / 64 + 2];
+  __shared__ double red[3 * SEG_K2_BS / 64 + 3];
   const int w = blockIdx.x, sg = A.wg_seg[w], j = A.wg_loc[w], G = A.seg_G[sg];
   if (A.sc[sg].done) return;
   const int g0 = A.seg_wg0[sg];
@@ -1442,8 +1485,10 @@ __global__ __launch_bounds__(SEG_K2_BS) void seg_k2_kernel(SegPcgArgs A, int k) 
     X = ld3<T>(x, i);
     ld_precond<T>(A.precond, dinv, binv, i, M);
   }
-  const double pq = all_sum<SEG_K2_BS>(A.part1 + g0, G, 1, red);
-  const double rz = all_sum<SEG_K2_BS>(seg_rr_slot(A, k - 1) + 2L * g0 + 1, G, 2, red);
+  double pq, rz, none;
+  all_sum3<SEG_K2_BS>(A.part1 + g0, G, 1, seg_rr_slot(A, k - 1) + 2L * g0 + 1, G, 2, nullptr, 0,
+                      1, red, pq, rz, none);
+  rz += seg_cdot(A, k - 1, sg);
   const T ta = (T)(rz / pq);
   double rr = 0.0, rzn = 0.0;
   while (i < r1) {
@@ -1533,19 +1578,30 @@ void launch_seg_init(hipStream_t s, bool fp32, const SegPcgArgs& a, const void* 
   }
 }
 
-template <int L>
-static void seg_k1_l(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {
+template <int L, bool CG>
+static void seg_k1_lc(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {
   if (a.bcol16) {
     if (fp32)
-      hipLaunchKernelGGL((seg_k1_kernel<float, L, true>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+      hipLaunchKernelGGL((seg_k1_kernel<float, L, true, CG>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
     else
-      hipLaunchKernelGGL((seg_k1_kernel<double, L, true>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+      hipLaunchKernelGGL((seg_k1_kernel<double, L, true, CG>), dim3(a.W), dim3(K1_BS), 0, s, a,
+                         k);
   } else {
     if (fp32)
-      hipLaunchKernelGGL((seg_k1_kernel<float, L, false>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+      hipLaunchKernelGGL((seg_k1_kernel<float, L, false, CG>), dim3(a.W), dim3(K1_BS), 0, s, a,
+                         k);
     else
-      hipLaunchKernelGGL((seg_k1_kernel<double, L, false>), dim3(a.W), dim3(K1_BS), 0, s, a, k);
+      hipLaunchKernelGGL((seg_k1_kernel<double, L, false, CG>), dim3(a.W), dim3(K1_BS), 0, s, a,
+                         k);
   }
+}
+
+template <int L>
+static void seg_k1_l(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {
+  if (a.coarse)
+    seg_k1_lc<L, true>(s, fp32, a, k);
+  else
+    seg_k1_lc<L, false>(s, fp32, a, k);
 }
 
 void launch_seg_k1(hipStream_t s, bool fp32, const SegPcgArgs& a, int k) {

@@ -43,8 +43,8 @@ PRECOND_NAMES = ["jacobi", "block-jacobi", "two-level"]
 
 
 def _opts(args, **kw):
-    """SOLVE_OPTS with bench.py's --precond (the fused batch PCG takes the
-    two-level preconditioner as block Jacobi; single-model solves use it)."""
+    """SOLVE_OPTS with bench.py's --precond (0 Jacobi, 1 block Jacobi, 2
+    two-level; fused batches build one coarse level per instance)."""
     return dict(SOLVE_OPTS, precond=getattr(args, "precond", 1), **kw)
 
 
@@ -105,7 +105,7 @@ def batch(args, log):
         "dtype": "f32 Hessian/PCG, f64 energy/gradient/residual",
         "config": {"workload": f"batch_{count}_beams: {m} tets, {3 * n} DoF, one fused "
                                "launch per kernel, per-instance Newton",
-                   "precision": "mixed", "precond": PRECOND_NAMES[min(opts["precond"], 1)],
+                   "precision": "mixed", "precond": PRECOND_NAMES[opts["precond"]],
                    "linear_tol": "eisenstat-walker"},
         "converged": sum(r["result"] == "SUCCESS" for r in res),
         "instances": count,
