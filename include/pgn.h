@@ -287,7 +287,7 @@ typedef struct {
   int lsNumRegTrials;
   /* MI355X path controls (no reference counterpart) */
   int precond;        /* 0 = Jacobi, 1 = 3x3 block Jacobi, 2 = two-level
-                         (single models; batches take 2 as 1)              */
+                         (fused batches: one coarse level per instance)    */
   double inner_tol;   /* MIXED: FP32 inner tolerance per refinement sweep */
   int adaptive_tol;   /* 0: every linear solve to lsTolMaxError (reference).
                          1: Eisenstat-Walker forcing (choice 2, gamma
