@@ -1290,7 +1290,11 @@ size_t pgn_batch_device_bytes(const pgn_batch* B) {
                    B->seg_row1, B->seg_tol, B->seg_part1, B->seg_part2, B->seg_partb,
                    B->kvec, B->kvec2, B->xprev, B->seg_sc, B->bcol16, B->ldlt_prob,
                    B->ldlt_perm, B->ldlt_iperm, B->ldlt_list, B->ldlt_status, B->ldlt_band,
-                   B->ldlt_D, B->ldlt_sinT, B->ldlt_y);
+                   B->ldlt_D, B->ldlt_sinT, B->ldlt_y, B->eg_mask) +
+         sum_bytes(B->sc_agg_ptr, B->sc_agg_nodes, B->sc_agg_of, B->sc_agg_inst, B->sc_aoff,
+                   B->sc_moff, B->sc_cb_a, B->sc_cb_b, B->sc_pair_piece0, B->sc_piece_lo,
+                   B->sc_cb_fine, B->sc_ok, B->sc_rel, B->sc_cmask, B->sc_A, B->sc_piece_sum,
+                   B->sc_rc, B->sc_uc, B->sc_cdot);
 }
 
 int pgn_batch_reset(pgn_batch* B) {
