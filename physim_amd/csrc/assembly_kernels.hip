@@ -177,7 +177,52 @@ __global__ __launch_bounds__(256) void bsr_gather_kernel(BsrArgs A,
 #pragma unroll
   for (int q = 0; q < 9; ++q) acc[q] = T(0);
   const long c1 = A.cptr[s + 1];
-  for (long p = A.cptr[s]; p < c1; ++p) {
+  long p = A.cptr[s];
+  // GU contributions at a time: their codes, then all their value loads are
+  // issued before any is added (the adds stay in element order)
+  constexpr int GU = 4;
+  for (; p + GU <= c1; p += GU) {
+    uint32_t cc[GU];
+#pragma unroll
+    for (int u = 0; u < GU; ++u) cc[u] = A.code[p + u];
+    T v[GU][9];
+#pragma unroll
+    for (int u = 0; u < GU; ++u) {
+      const long e = cc[u] >> 4;
+      const int la = (cc[u] >> 2) & 3, lb = cc[u] & 3;
+      const T* __restrict__ h = eH + e * TET_H_PACK;
+      if (la == lb) {
+        const T* d = h + 54 + 6 * la;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[u][q] = d[q];
+      } else {
+        const int hi = la > lb ? la : lb, lo = la > lb ? lb : la;
+        const T* o = h + 9 * (hi * (hi - 1) / 2 + lo);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v[u][q] = o[q];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < GU; ++u) {
+      const int la = (cc[u] >> 2) & 3, lb = cc[u] & 3;
+      if (la == lb) {
+        const T d0 = v[u][0], d1 = v[u][1], d2 = v[u][2], d3 = v[u][3], d4 = v[u][4],
+                d5 = v[u][5];
+        acc[0] += d0; acc[1] += d1; acc[2] += d3;
+        acc[3] += d1; acc[4] += d2; acc[5] += d4;
+        acc[6] += d3; acc[7] += d4; acc[8] += d5;
+      } else if (la > lb) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] += v[u][q];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) acc[3 * i + k] += v[u][3 * k + i];
+      }
+    }
+  }
+  for (; p < c1; ++p) {
     const uint32_t c = A.code[p];
     const long e = c >> 4;
     const int la = (c >> 2) & 3, lb = c & 3;

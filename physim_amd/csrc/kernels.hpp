@@ -342,6 +342,9 @@ int seg_rows_per_wg(int lpr);
 
 // r64 = b64 - A x64 with A in precision T and FP64 accumulation; writes
 // |r|^2 into out[0].  partials: PCG_MAXP doubles + one unsigned ticket.
+// the same over the SELL-64 copy described by a (a.sell set; grid a.g1 <= PCG_MAXP)
+void launch_residual64_sell(hipStream_t s, bool fp32_mat, const PcgArgs& a, const double* x,
+                            const double* b, double* r, double* partials, double* out);
 void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
                        const int* bcol, const void* bval, const double* x,
                        const double* b, double* r, double* partials,

@@ -1086,6 +1086,73 @@ __global__ __launch_bounds__(PCG_BS) void residual64_kernel(
   }
 }
 
+// The same residual over the SELL-64 copy of the matrix (one row per lane,
+// the row's blocks in column order; grid and slice walk of cg1_sell_kernel).
+template <class TM, bool C16>
+__global__ __launch_bounds__(SELL_BS) void residual64_sell_kernel(
+    PcgArgs A, const double* __restrict__ x, const double* __restrict__ b,
+    double* __restrict__ r, double* part, double* out, unsigned* ticket) {
+  __shared__ double red[SELL_BS / 64];
+  __shared__ bool last;
+  constexpr int WAVES = SELL_BS / 64;
+  const TM* __restrict__ sval = reinterpret_cast<const TM*>(A.sell_val);
+  const int* __restrict__ scol = A.sell_col;
+  const short* __restrict__ s16 = A.sell_c16;
+  const int G = gridDim.x, bk = blockIdx.x;
+  const int lb = (G % 8 == 0) ? (bk % 8) * (G / 8) + bk / 8 : bk;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double rr = 0.0;
+  for (int t = wave; t < A.sell_spw; t += WAVES) {
+    const long slice = (long)lb * A.sell_spw + t;
+    if (slice >= A.sell_ns) break;
+    const long row = slice * 64 + lane;
+    if (row >= A.n) continue;
+    const int len = A.sell_len[row];
+    const long base = A.sell_so[slice];
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    for (int j = 0; j < len; ++j) {
+      const long c = C16 ? row + s16[base + 64L * j + lane] : scol[base + 64L * j + lane];
+      const TM* V = sval + (base + 64L * j) * 9 + lane;
+      const double a0 = x[3 * c], a1 = x[3 * c + 1], a2 = x[3 * c + 2];
+      y0 += (double)V[0] * a0 + (double)V[64] * a1 + (double)V[128] * a2;
+      y1 += (double)V[192] * a0 + (double)V[256] * a1 + (double)V[320] * a2;
+      y2 += (double)V[384] * a0 + (double)V[448] * a1 + (double)V[512] * a2;
+    }
+    const double r0 = b[3 * row] - y0, r1 = b[3 * row + 1] - y1, r2 = b[3 * row + 2] - y2;
+    r[3 * row] = r0;
+    r[3 * row + 1] = r1;
+    r[3 * row + 2] = r2;
+    rr += r0 * r0 + r1 * r1 + r2 * r2;
+  }
+  const double sum = block_sum<SELL_BS>(rr, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = sum;
+  if (last_block(ticket, &last)) {
+    const double t = block_sum_array<SELL_BS>(part, gridDim.x, 1, red);
+    if (threadIdx.x == 0) out[0] = t;
+  }
+}
+
+void launch_residual64_sell(hipStream_t s, bool fp32_mat, const PcgArgs& a, const double* x,
+                            const double* b, double* r, double* partials, double* out) {
+  unsigned* ticket = reinterpret_cast<unsigned*>(partials + PCG_MAXP);
+  const dim3 g(a.g1), bs(SELL_BS);
+  if (fp32_mat) {
+    if (a.sell_c16)
+      hipLaunchKernelGGL((residual64_sell_kernel<float, true>), g, bs, 0, s, a, x, b, r,
+                         partials, out, ticket);
+    else
+      hipLaunchKernelGGL((residual64_sell_kernel<float, false>), g, bs, 0, s, a, x, b, r,
+                         partials, out, ticket);
+  } else {
+    if (a.sell_c16)
+      hipLaunchKernelGGL((residual64_sell_kernel<double, true>), g, bs, 0, s, a, x, b, r,
+                         partials, out, ticket);
+    else
+      hipLaunchKernelGGL((residual64_sell_kernel<double, false>), g, bs, 0, s, a, x, b, r,
+                         partials, out, ticket);
+  }
+}
+
 // partials: PCG_MAXP doubles followed by an unsigned ticket
 void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
                        const int* bcol, const void* bval, const double* x,

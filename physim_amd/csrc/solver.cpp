@@ -128,9 +128,15 @@ static void solve_once(pgn_model& M, const double* b, double* x, const pgn_linea
   }
   const double thr = o.tol * o.tol * bb;
   double rr = 0.0;
+  // the FP64 residual reads the SELL-64 copy when the PCG does (packed for
+  // this try), else the BSR3 matrix
+  const PcgArgs ra = M.pcg_args(0.0, 0);
   for (;;) {
-    launch_residual64(s, true, M.n, M.rowptr.p, M.bcol.p, M.bval.p, x, b, M.r64.p,
-                      M.red64.p, M.scal.p + 4);
+    if (ra.sell)
+      launch_residual64_sell(s, true, ra, x, b, M.r64.p, M.red64.p, M.scal.p + 4);
+    else
+      launch_residual64(s, true, M.n, M.rowptr.p, M.bcol.p, M.bval.p, x, b, M.r64.p,
+                        M.red64.p, M.scal.p + 4);
     PGN_HIP(hipMemcpyAsync(M.hpin + 4, M.scal.p + 4, sizeof(double), hipMemcpyDeviceToHost, s));
     M.sync();
     rr = M.hpin[4];

@@ -61,9 +61,14 @@ def _batch_subset():
     return out
 
 
-def test_beam_batch_real_sizes(ctx):
+@pytest.mark.parametrize("precond", [0, 2])
+def test_beam_batch_real_sizes(ctx, precond):
+    """precond 0: the reference's Jacobi; 2: one two-level coarse level per
+    instance (the batch workload's setting)."""
     sc = _batch_subset()
-    _say("fused beam batch: " + ", ".join(f"{s.name} ({s.m} tets)" for s in sc))
+    opts = dict(OPTS, precond=precond)
+    _say(f"fused beam batch (precond {precond}): "
+         + ", ".join(f"{s.name} ({s.m} tets)" for s in sc))
     small = [k for k, s in enumerate(sc) if s.m <= 6000]
     assert len(small) >= 2
     pool = ThreadPoolExecutor(max_workers=4)
@@ -73,7 +78,7 @@ def test_beam_batch_real_sizes(ctx):
         B.add_scene(s)
     B.build()
     t = time.perf_counter()
-    res = B.newton(**OPTS)
+    res = B.newton(**opts)
     _say(f"  fused solve {time.perf_counter() - t:.2f}s: "
          + ", ".join(f"{r['result']}/{r['iterations']}" for r in res))
     xs = [B.get_x(k).copy() for k in range(len(sc))]
@@ -83,7 +88,7 @@ def test_beam_batch_real_sizes(ctx):
         A = pg.Batch(ctx, pg.MIXED)
         A.add_scene(s)
         A.build()
-        ra = A.newton(**OPTS)[0]
+        ra = A.newton(**opts)[0]
         err = np.abs(A.get_x(0) - xs[k]).max() / s.bbox_diag()
         A.close()
         _say(f"  {s.name}: alone {ra['result']}/{ra['iterations']}, |mix - alone| "
@@ -98,7 +103,8 @@ def test_beam_batch_real_sizes(ctx):
     pool.shutdown()
 
 
-def test_serving_mix_matches_oracle():
+@pytest.mark.parametrize("precond", [0, 2])
+def test_serving_mix_matches_oracle(precond):
     reqs = workloads._serving_mix(8, 50000, 128, seed=5)
     # both ends of the ranges: 2k-tet and ~44k-tet beams, 32- and 128-node cloths
     reqs += [scenes.cantilever(dims=(5, 5, 20), young_sigma=0.3, jitter=0.05, seed=1,
@@ -110,7 +116,7 @@ def test_serving_mix_matches_oracle():
     _say("serving mix: " + ", ".join(s.name for s in reqs))
     pool = ThreadPoolExecutor(max_workers=4)
     futs = [pool.submit(_oracle_x, s) for s in reqs]
-    S = pg.Server(hbm_budget_bytes=4 << 30, lanes=2, max_batch=4, **OPTS)
+    S = pg.Server(hbm_budget_bytes=4 << 30, lanes=2, max_batch=4, **dict(OPTS, precond=precond))
     tickets = [None] * len(reqs)
 
     def producer():
