@@ -171,8 +171,8 @@ template <class T>
 __global__ __launch_bounds__(256) void bsr_gather_kernel(BsrArgs A,
                                                          const T* __restrict__ eH,
                                                          T* __restrict__ bval) {
-  const long s = xcd_block() * 256L + threadIdx.x;
-  if (s >= A.nb) return;
+  const long s = A.s0 + xcd_block() * 256L + threadIdx.x;
+  if (s >= A.s0 + A.nb) return;
   T acc[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) acc[q] = T(0);
@@ -289,9 +289,9 @@ template <class T>
 __global__ __launch_bounds__(256) void precond_kernel(int n, const long* __restrict__ diag,
                                                      const T* __restrict__ bval,
                                                      T* __restrict__ dinv,
-                                                     T* __restrict__ binv) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+                                                     T* __restrict__ binv, long i0) {
+  const long i = i0 + blockIdx.x * 256L + threadIdx.x;
+  if (i >= i0 + n) return;
   const T* B = bval + diag[i] * 9;
   T a[9];
 #pragma unroll
@@ -320,17 +320,17 @@ __global__ __launch_bounds__(256) void precond_kernel(int n, const long* __restr
 }
 
 void launch_precond(hipStream_t s, bool fp32, int n, const long* diag,
-                    const void* bval, void* dinv, void* binv) {
+                    const void* bval, void* dinv, void* binv, long i0) {
   const int grid = (n + 255) / 256;
-  if (grid == 0) return;
+  if (grid <= 0) return;
   if (fp32)
     hipLaunchKernelGGL(precond_kernel<float>, dim3(grid), dim3(256), 0, s, n, diag,
                        reinterpret_cast<const float*>(bval), reinterpret_cast<float*>(dinv),
-                       reinterpret_cast<float*>(binv));
+                       reinterpret_cast<float*>(binv), i0);
   else
     hipLaunchKernelGGL(precond_kernel<double>, dim3(grid), dim3(256), 0, s, n, diag,
                        reinterpret_cast<const double*>(bval), reinterpret_cast<double*>(dinv),
-                       reinterpret_cast<double*>(binv));
+                       reinterpret_cast<double*>(binv), i0);
 }
 
 template <class T>

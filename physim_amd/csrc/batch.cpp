@@ -168,6 +168,128 @@ void pgn_batch::upload_loads() {
   falpha.upload(fa, ctx->stream);
 }
 
+std::vector<std::pair<int, int>> pgn_batch::active_runs(const std::vector<uint8_t>& act) const {
+  std::vector<std::pair<int, int>> runs;
+  for (int k = 0; k < K(); ++k) {
+    if (!act[k]) continue;
+    if (!runs.empty() && runs.back().second == k) runs.back().second = k + 1;
+    else runs.emplace_back(k, k + 1);
+  }
+  return runs;
+}
+
+void pgn_batch::eval_hessian(const std::vector<uint8_t>& act) {
+  const int Kn = K();
+  const auto runs = active_runs(act);
+  if (runs.size() > 16 || (runs.size() == 1 && runs[0].first == 0 && runs[0].second == Kn)) {
+    M->eval_hessian();
+    return;
+  }
+  hipStream_t s = ctx->stream;
+  const bool f32 = M->hfp32;
+  const size_t ts = M->tsize();
+  const int tile = std::max(0, M->tet_tile_choice[f32 ? 1 : 0][EVAL_H]);
+  const BsrArgs ba0 = M->bsr_args();
+  for (const auto& rk : runs) {
+    const int k0 = rk.first, k1 = rk.second;
+    for (int c = 0; c < 3; ++c) {
+      const long e0 = heoff[c * (Kn + 1) + k0], e1 = heoff[c * (Kn + 1) + k1];
+      if (e1 <= e0) continue;
+      if (c == 0) {
+        TetArgs ta = M->tet_args(f32);
+        ta.m = (int)(e1 - e0);
+        ta.tets += e0;
+        ta.kind += e0;
+        ta.prm = reinterpret_cast<const char*>(ta.prm) + e0 * ts;
+        launch_tet_eval(s, f32, EVAL_H, ta, nullptr, nullptr, nullptr,
+                        M->eH.p + e0 * TET_H_PACK * ts, tile);
+      } else {
+        const int kind = c == 1 ? SHELL_MEMBRANE : SHELL_HINGE;
+        ShellArgs sa = M->shell_args(f32, kind);
+        sa.first = (int)e0;
+        sa.count = (int)(e1 - e0);
+        launch_shell_eval(s, f32, kind, EVAL_H, sa, nullptr, nullptr, nullptr, M->eH.p);
+      }
+    }
+    const long n0 = hnoff[k0], n1 = hnoff[k1];
+    BsrArgs ba = ba0;
+    ba.s0 = M->pat.rowptr[n0];
+    ba.nb = M->pat.rowptr[n1] - ba.s0;
+    launch_bsr_gather(s, f32, ba, M->eH.p, M->bval.p);
+    launch_precond(s, f32, (int)(n1 - n0), M->diag.p, M->bval.p, M->dinv.p, M->binv.p, n0);
+  }
+  M->mf_shift = 0.0;
+  M->hess_valid = true;
+}
+
+void pgn_batch::seg_red_prepare() {
+  if (sr_ready) return;
+  const int Kn = K();
+  hipStream_t s = ctx->stream;
+  auto table = [&](const std::vector<long>& r, long chunk, DBuf<int>& inst, DBuf<int>& wg0,
+                   DBuf<long>& lo, DBuf<long>& hi, DBuf<double>& part, SegRedTable& t) {
+    std::vector<int> vi, vw(Kn + 1, 0);
+    std::vector<long> vl, vh;
+    for (int k = 0; k < Kn; ++k) {
+      vw[k] = (int)vi.size();
+      for (long a = r[k]; a < r[k + 1]; a += chunk) {
+        vi.push_back(k);
+        vl.push_back(a);
+        vh.push_back(std::min(a + chunk, r[k + 1]));
+      }
+    }
+    vw[Kn] = (int)vi.size();
+    inst.upload(vi, s);
+    wg0.upload(vw, s);
+    lo.upload(vl, s);
+    hi.upload(vh, s);
+    part.ensure(std::max<size_t>(vi.size(), 1));
+    PGN_HIP(hipStreamSynchronize(s));  // the host vectors are released on return
+    t.W = (int)vi.size();
+    t.inst = inst.p;
+    t.lo = lo.p;
+    t.hi = hi.p;
+    t.wg0 = wg0.p;
+    t.part = part.p;
+  };
+  std::vector<long> dofs(Kn + 1), blks(Kn + 1);
+  for (int k = 0; k <= Kn; ++k) {
+    dofs[k] = 3 * hnoff[k];
+    blks[k] = M->pat.rowptr[hnoff[k]];
+  }
+  table(dofs, 4096, sr_inst_d, sr_wg0_d, sr_lo_d, sr_hi_d, sr_part_d, tab_dof);
+  table(blks, 2048, sr_inst_b, sr_wg0_b, sr_lo_b, sr_hi_b, sr_part_b, tab_blk);
+  sr_ready = true;
+}
+
+void pgn_batch::residual64(const std::vector<uint8_t>& act, const double* x, const double* b) {
+  hipStream_t s = ctx->stream;
+  const auto runs = active_runs(act);
+  if (runs.size() > 16) {
+    launch_residual64(s, M->hfp32, M->n, M->rowptr.p, M->bcol.p, M->bval.p, x, b, M->r64.p,
+                      M->red64.p, M->scal.p + 4);
+    return;
+  }
+  for (const auto& rk : runs)
+    launch_residual64(s, M->hfp32, (int)(hnoff[rk.second] - hnoff[rk.first]), M->rowptr.p,
+                      M->bcol.p, M->bval.p, x, b, M->r64.p, M->red64.p, M->scal.p + 4,
+                      hnoff[rk.first]);
+}
+
+void pgn_batch::inst_dot(const std::vector<uint8_t>& act, const double* x, const double* y,
+                         double* out) {
+  seg_red_prepare();
+  rmask.upload(act, ctx->stream);
+  launch_seg_dot(ctx->stream, K(), tab_dof, rmask.p, x, y, out);
+}
+
+void pgn_batch::inst_frob(const std::vector<uint8_t>& act, double* out) {
+  seg_red_prepare();
+  rmask.upload(act, ctx->stream);
+  launch_seg_frob(ctx->stream, M->hfp32, K(), tab_blk, rmask.p, M->brow.p, M->bcol.p,
+                  M->bval.p, out);
+}
+
 void pgn_batch::eval_eg(std::vector<double>& o4, const std::vector<uint8_t>* act) {
   hipStream_t s = ctx->stream;
   const int Kn = K();
@@ -544,6 +666,7 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
   iters.assign(Kn, 0);
   std::vector<uint8_t> F = active;
   std::vector<double> regT(Kn, -1.0), h(Kn), shift(Kn);
+  bool reg_known = false;
   auto download_k = [&](const double* d, std::vector<double>& out) {
     PGN_HIP(hipMemcpyAsync(out.data(), d, Kn * sizeof(double), hipMemcpyDeviceToHost, s));
     PGN_HIP(hipStreamSynchronize(s));
@@ -556,11 +679,12 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
     if (it != 0) {
       // LinearSolver_EigenCG.cpp:73 + LinearSolver.cpp:57-66 per instance:
       // A += 1e-6 |A_lower|_F 10^(it-1) I, cumulative over the tries
-      if (regT[0] < 0.0) {
-        launch_batch_frob(s, M->hfp32, ba, M->rowptr.p, M->brow.p, M->bcol.p, M->bval.p,
-                          kvec.p);
+      if (!reg_known) {  // of the unshifted matrix, for the instances still trying
+        inst_frob(F, kvec.p);
         download_k(kvec.p, h);
-        for (int k = 0; k < Kn; ++k) regT[k] = 1e-6 * std::sqrt(h[k]);
+        for (int k = 0; k < Kn; ++k)
+          if (F[k]) regT[k] = 1e-6 * std::sqrt(h[k]);
+        reg_known = true;
       }
       for (int k = 0; k < Kn; ++k) shift[k] = F[k] ? regT[k] * std::pow(10.0, it - 1) : 0.0;
       kvec2.upload(shift.data(), Kn, s);
@@ -584,7 +708,7 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
         if (F[k]) iters[k] += itk[k];
     } else {
       // FP32 PCG corrections inside FP64 iterative refinement, per instance
-      launch_batch_dot(s, ba, b, b, kvec.p);
+      inst_dot(F, b, b, kvec.p);
       std::vector<double> bb(Kn), rr(Kn);
       download_k(kvec.p, bb);
       std::vector<uint8_t> act(Kn, 0), zero(Kn, 0);
@@ -604,9 +728,8 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
       }
       std::vector<int> nref(Kn, 0);
       for (;;) {
-        launch_residual64(s, true, (int)n, M->rowptr.p, M->bcol.p, M->bval.p, x, b, M->r64.p,
-                          M->red64.p, M->scal.p + 4);
-        launch_batch_norm2(s, ba, M->r64.p, kvec.p);
+        residual64(act, x, b);
+        inst_dot(act, M->r64.p, M->r64.p, kvec.p);
         download_k(kvec.p, rr);
         bool any_act = false;
         int min_it = o.max_iters;
@@ -641,7 +764,7 @@ void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8
       }
     }
     // indefinite (b.x < 0): try again with a larger shift
-    launch_batch_dot(s, ba, b, x, kvec.p);
+    inst_dot(F, b, x, kvec.p);
     download_k(kvec.p, h);
     for (int k = 0; k < Kn; ++k)
       if (F[k] && !(h[k] < 0.0)) F[k] = 0;
@@ -831,17 +954,20 @@ std::vector<int> pgn_batch::newton_step() {
     }
   }
   mode.upload(hmode, s);
-  if (need_h) M->eval_hessian();
+  std::vector<uint8_t> hact(Kn), stepping(Kn);
+  for (int k = 0; k < Kn; ++k) {
+    hact[k] = hmode[k] == 1;
+    stepping[k] = hmode[k] != 0;
+  }
+  if (need_h) eval_hessian(hact);
   launch_batch_rhs(s, ba, n, mode.p, M->g.p, xsave.p, v0.p, bvec.p, dx.p);
   if (need_h) {
-    std::vector<uint8_t> act(Kn);
-    for (int k = 0; k < Kn; ++k) act[k] = hmode[k] == 1;
     std::vector<int> its_lin;
-    linear_solve(bvec.p, dx.p, act, tol_lin, lo, its_lin);
+    linear_solve(bvec.p, dx.p, hact, tol_lin, lo, its_lin);
     for (int k = 0; k < Kn; ++k)
       if (hmode[k] == 1) pcg[k] += its_lin[k];
   }
-  launch_batch_norm2(s, ba, dx.p, n2.p);
+  inst_dot(stepping, dx.p, dx.p, n2.p);
   n2.download(hn2.data(), Kn, s);
   PGN_HIP(hipStreamSynchronize(s));
   for (int k = 0; k < Kn; ++k) {

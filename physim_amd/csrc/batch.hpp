@@ -65,6 +65,25 @@ struct pgn_batch {
   // others' entries keep their last values
   void eval_eg(std::vector<double>& out4, const std::vector<uint8_t>* act = nullptr);
   pgn::DBuf<uint8_t> eg_mask;
+  // runs [k0, k1) of consecutive instances marked in act
+  std::vector<std::pair<int, int>> active_runs(const std::vector<uint8_t>& act) const;
+  // element Hessians, BSR3 gather and block-Jacobi inverses of the instances
+  // marked in act only (their element, block and node ranges); many runs or
+  // all instances: the whole-batch evaluation
+  void eval_hessian(const std::vector<uint8_t>& act);
+  // FP64 residual r64 = b - A x over the rows of the instances in act
+  void residual64(const std::vector<uint8_t>& act, const double* x, const double* b);
+  // per-instance reductions over the instances in act (others keep out[k]):
+  // out[k] = x_k . y_k, or |A_k|_F^2 over the stored lower entry set
+  void inst_dot(const std::vector<uint8_t>& act, const double* x, const double* y, double* out);
+  void inst_frob(const std::vector<uint8_t>& act, double* out);
+  bool sr_ready = false;
+  pgn::SegRedTable tab_dof, tab_blk;
+  pgn::DBuf<int> sr_inst_d, sr_wg0_d, sr_inst_b, sr_wg0_b;
+  pgn::DBuf<long> sr_lo_d, sr_hi_d, sr_lo_b, sr_hi_b;
+  pgn::DBuf<double> sr_part_d, sr_part_b;
+  pgn::DBuf<uint8_t> rmask;
+  void seg_red_prepare();
   void reset();
   void seg_plan(const std::vector<uint8_t>& act);
   pgn::SegPcgArgs seg_args(int precond) const;

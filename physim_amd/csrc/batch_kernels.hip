@@ -249,3 +249,76 @@ void launch_batch_axpy_f32(hipStream_t s, const BatchArgs& a, long n_nodes, cons
 }
 
 }  // namespace pgn
+
+namespace pgn {
+
+constexpr int SR_BS = 256;
+
+__global__ __launch_bounds__(SR_BS) void seg_dot_kernel(SegRedTable t,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const double* __restrict__ x,
+                                                        const double* __restrict__ y) {
+  __shared__ double red[SR_BS / 64];
+  const int w = blockIdx.x, k = t.inst[w];
+  if (mask && !mask[k]) return;
+  double v = 0.0;
+  const long hi = t.hi[w];
+  for (long d = t.lo[w] + threadIdx.x; d < hi; d += SR_BS) v += x[d] * y[d];
+  v = block_sum<SR_BS>(v, red);
+  if (threadIdx.x == 0) t.part[w] = v;
+}
+
+template <class T>
+__global__ __launch_bounds__(SR_BS) void seg_frob_kernel(SegRedTable t,
+                                                         const uint8_t* __restrict__ mask,
+                                                         const int* __restrict__ brow,
+                                                         const int* __restrict__ bcol,
+                                                         const T* __restrict__ bval) {
+  __shared__ double red[SR_BS / 64];
+  const int w = blockIdx.x, k = t.inst[w];
+  if (mask && !mask[k]) return;
+  double v = 0.0;
+  const long hi = t.hi[w];
+  for (long b = t.lo[w] + threadIdx.x; b < hi; b += SR_BS)
+    if (brow[b] >= bcol[b]) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        const double e = bval[b * 9 + q];
+        v += e * e;
+      }
+    }
+  v = block_sum<SR_BS>(v, red);
+  if (threadIdx.x == 0) t.part[w] = v;
+}
+
+// one thread per instance: its partials in order
+__global__ __launch_bounds__(64) void seg_red_fin_kernel(int K, SegRedTable t,
+                                                         const uint8_t* __restrict__ mask,
+                                                         double* __restrict__ out) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= K || (mask && !mask[k])) return;
+  double v = 0.0;
+  for (int w = t.wg0[k]; w < t.wg0[k + 1]; ++w) v += t.part[w];
+  out[k] = v;
+}
+
+void launch_seg_dot(hipStream_t s, int K, const SegRedTable& t, const uint8_t* mask,
+                    const double* x, const double* y, double* out) {
+  if (K == 0 || t.W == 0) return;
+  hipLaunchKernelGGL(seg_dot_kernel, dim3(t.W), dim3(SR_BS), 0, s, t, mask, x, y);
+  hipLaunchKernelGGL(seg_red_fin_kernel, dim3((K + 63) / 64), dim3(64), 0, s, K, t, mask, out);
+}
+
+void launch_seg_frob(hipStream_t s, bool fp32, int K, const SegRedTable& t, const uint8_t* mask,
+                     const int* brow, const int* bcol, const void* bval, double* out) {
+  if (K == 0 || t.W == 0) return;
+  if (fp32)
+    hipLaunchKernelGGL(seg_frob_kernel<float>, dim3(t.W), dim3(SR_BS), 0, s, t, mask, brow, bcol,
+                       reinterpret_cast<const float*>(bval));
+  else
+    hipLaunchKernelGGL(seg_frob_kernel<double>, dim3(t.W), dim3(SR_BS), 0, s, t, mask, brow,
+                       bcol, reinterpret_cast<const double*>(bval));
+  hipLaunchKernelGGL(seg_red_fin_kernel, dim3((K + 63) / 64), dim3(64), 0, s, K, t, mask, out);
+}
+
+}  // namespace pgn

@@ -109,13 +109,15 @@ struct BsrArgs {
   // implicit dynamics: m_i / dt^2 added to the diagonal of block (i, i)
   const double* mass = nullptr;
   double inv_dt2 = 0.0;
+  long s0 = 0;  // blocks [s0, s0 + nb) are gathered
 };
 void launch_bsr_gather(hipStream_t s, bool fp32, const BsrArgs& a,
                        const void* eH, void* bval);
 // diag: n block indexes of the diagonal blocks.  Writes dinv (3n, Jacobi) and
 // binv (6n, block-Jacobi inverse as upper triangles) in precision T.
+// (nodes [i0, i0 + n))
 void launch_precond(hipStream_t s, bool fp32, int n, const long* diag,
-                    const void* bval, void* dinv, void* binv);
+                    const void* bval, void* dinv, void* binv, long i0 = 0);
 // bval[diag blocks] += shift on the scalar diagonal
 void launch_add_diag(hipStream_t s, bool fp32, int n, const long* diag,
                      void* bval, double shift);
@@ -345,10 +347,11 @@ int seg_rows_per_wg(int lpr);
 // the same over the SELL-64 copy described by a (a.sell set; grid a.g1 <= PCG_MAXP)
 void launch_residual64_sell(hipStream_t s, bool fp32_mat, const PcgArgs& a, const double* x,
                             const double* b, double* r, double* partials, double* out);
+// (rows [row0, row0 + n))
 void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
                        const int* bcol, const void* bval, const double* x,
                        const double* b, double* r, double* partials,
-                       double* out);
+                       double* out, long row0 = 0);
 // y = A x in precision T with `lpr` lanes per block row (tuning, timing)
 void launch_spmv(hipStream_t s, bool fp32, int n, const long* rowptr,
                  const int* bcol, const void* bval, const void* x, void* y, int lpr);
@@ -393,6 +396,26 @@ void launch_batch_axpy_f32(hipStream_t s, const BatchArgs& a, long n_nodes, cons
 // dst = src on instances with mask[k] != 0
 void launch_batch_copy(hipStream_t s, const BatchArgs& a, long n_nodes, const uint8_t* mask,
                        const double* src, double* dst);
+
+// Per-instance reductions over a static workgroup table (each instance's
+// DoF or block range cut into chunks): one partial per workgroup, then each
+// instance's partials summed in order by a second launch.  Instances with
+// mask[k] == 0 (mask may be null) are skipped and keep out[k].
+struct SegRedTable {
+  int W = 0;                  // workgroups
+  const int* inst = nullptr;  // W: instance of each workgroup
+  const long* lo = nullptr;   // W: chunk start (DoF, or BSR3 block)
+  const long* hi = nullptr;   // W: chunk end
+  const int* wg0 = nullptr;   // K + 1: first workgroup of each instance
+  double* part = nullptr;     // W partials
+};
+// out[k] = x_k . y_k (FP64) over the DoF table
+void launch_seg_dot(hipStream_t s, int K, const SegRedTable& t, const uint8_t* mask,
+                    const double* x, const double* y, double* out);
+// out[k] = |A_k|_F^2 over the stored lower entry set (blocks with row >= column)
+// over the block table
+void launch_seg_frob(hipStream_t s, bool fp32, int K, const SegRedTable& t, const uint8_t* mask,
+                     const int* brow, const int* bcol, const void* bval, double* out);
 
 // small vector helpers
 void launch_cast_f64_to_f32(hipStream_t s, long n, const double* a, float* b);

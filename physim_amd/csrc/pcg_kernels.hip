@@ -1049,12 +1049,13 @@ __global__ __launch_bounds__(PCG_BS) void residual64_kernel(
     int n, const long* __restrict__ rowptr, const int* __restrict__ bcol,
     const TM* __restrict__ bval, const double* __restrict__ x,
     const double* __restrict__ b, double* __restrict__ r, double* part,
-    double* out, unsigned* ticket) {
+    double* out, unsigned* ticket, long row0) {
   __shared__ double red[PCG_BS / 64];
   __shared__ bool last;
   const int lane = threadIdx.x & (LPR - 1);
   double rr = 0.0;
-  for (int row = (blockIdx.x * PCG_BS + threadIdx.x) / LPR; row < n;
+  const long row1 = row0 + n;
+  for (long row = row0 + (blockIdx.x * PCG_BS + threadIdx.x) / LPR; row < row1;
        row += gridDim.x * ROWS_PER_BLOCK) {
     double y0 = 0, y1 = 0, y2 = 0;
     const long e1 = rowptr[row + 1];
@@ -1157,17 +1158,18 @@ void launch_residual64_sell(hipStream_t s, bool fp32_mat, const PcgArgs& a, cons
 void launch_residual64(hipStream_t s, bool fp32_mat, int n, const long* rowptr,
                        const int* bcol, const void* bval, const double* x,
                        const double* b, double* r, double* partials,
-                       double* out) {
+                       double* out, long row0) {
+  if (n <= 0) return;
   const int g = grid_rows(n);
   unsigned* ticket = reinterpret_cast<unsigned*>(partials + PCG_MAXP);
   if (fp32_mat)
     hipLaunchKernelGGL(residual64_kernel<float>, dim3(g), dim3(PCG_BS), 0, s, n, rowptr,
                        bcol, reinterpret_cast<const float*>(bval), x, b, r, partials,
-                       out, ticket);
+                       out, ticket, row0);
   else
     hipLaunchKernelGGL(residual64_kernel<double>, dim3(g), dim3(PCG_BS), 0, s, n, rowptr,
                        bcol, reinterpret_cast<const double*>(bval), x, b, r, partials,
-                       out, ticket);
+                       out, ticket, row0);
 }
 
 // ---- plain SpMV (tuning, timing) --------------------------------------------
