@@ -443,6 +443,11 @@ __global__ __launch_bounds__(ROD_BS) void rod_band_kernel(RodArgs A, const doubl
 // slide them one lane per step.  Norms, the indefiniteness test and the
 // accuracy residual are sums over the rod's DoFs by the whole group.
 constexpr int LDLT_BS = 64;
+// Rows of the band (and of L, b, x in the solves) are staged into LDS RCH
+// at a time, chunk c in slot c & 1, so that the sequential steps read LDS
+// instead of waiting on a global load each.
+constexpr int RCH = 64;
+static_assert(RCH > ROD_BW, "a step reads rows j .. j + ROD_BW: two chunks at most");
 
 __device__ __forceinline__ double group_allsum(double v) {
   v = wave_sum(v);
@@ -458,12 +463,28 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
   constexpr int W = ROD_BW + 1;
   __shared__ double Lw[W][W];  // Lw[i % W][k] = L(i, i - k), k = 1..ROD_BW
   __shared__ double Dw[W];     // Dw[m % W] = D(m)
+  __shared__ double Bs[2][RCH][W];  // staged rows (band or L) of chunks c, c + 1
+  __shared__ double Vs[2][RCH];     // staged b or x of the same chunks
   const int r = blockIdx.x;
   if (r >= A.nrods) return;
   if (active && !active[r]) return;
   const int t = threadIdx.x;
   const long d0 = 4L * A.voff[r], d1 = 4L * A.voff[r + 1];
   const long N = d1 - d0;
+  // chunk c of a row-major N x W array (rows past N read as zero) into slot
+  // c & 1, and of a vector v when given
+  auto stage = [&](long c, const double* __restrict__ src, const double* __restrict__ v) {
+    if (c < 0) return;
+    const long r0 = c * RCH;
+    double* dst = &Bs[c & 1][0][0];
+    for (int q = t; q < RCH * W; q += LDLT_BS) {
+      const long row = r0 + q / W;
+      dst[q] = row < N ? src[W * (d0 + row) + q % W] : 0.0;
+    }
+    if (v)
+      for (int q = t; q < RCH; q += LDLT_BS) Vs[c & 1][q] = r0 + q < N ? v[d0 + r0 + q] : 0.0;
+  };
+  auto brow = [&](long i) -> const double* { return Bs[(i / RCH) & 1][i % RCH]; };
   // |A| over the entry set AMatrixSd stores: lower triangle + the upper
   // entries of every vertex's 3x3 position block
   double nA = 0.0, nbb = 0.0;
@@ -487,9 +508,14 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
     for (int q = 1; q <= it; ++q) cum += regT * pow(10.0, q - 1);
     for (int q = t; q < W * W; q += LDLT_BS) (&Lw[0][0])[q] = 0.0;
     if (t < W) Dw[t] = 0.0;
+    stage(0, band, nullptr);
     __syncthreads();
     bool bad = false;
     for (long j = 0; j < N; ++j) {
+      if (j % RCH == 0) {  // rows up to j + ROD_BW: chunk j / RCH + 1 as well
+        stage(j / RCH + 1, band, nullptr);
+        __syncthreads();
+      }
       const long gj = d0 + j;
       const int sj = (int)(j % W);
       double term = 0.0;
@@ -497,7 +523,7 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
         const double l = Lw[sj][t];
         term = l * l * Dw[(j - t) % W];
       }
-      const double dj = band[W * gj] + cum - group_allsum(term);
+      const double dj = brow(j)[0] + cum - group_allsum(term);
       if (!(fabs(dj) >= sinT)) {  // uniform: dj is the same in every lane
         bad = true;
         break;
@@ -510,7 +536,7 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
       if (row) {
         const int o = t + 1;  // i - j
         const int si = (int)(i % W);
-        double sacc = band[W * (d0 + i) + o];
+        double sacc = brow(i)[o];
         for (int k = o + 1; k < W && (long)k <= i; ++k) {
           const long m = i - k;
           sacc -= Lw[si][k] * Lw[sj][k - o] * Dw[m % W];
@@ -530,12 +556,18 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
       __syncthreads();
     }
     if (bad) continue;
-    // L y = b: lane k in 1..ROD_BW holds y(j - k)
+    // L y = b: lane k in 1..ROD_BW holds y(j - k); rows of L and b staged
+    __syncthreads();  // the factor's global rows are complete
     double win = 0.0;
     for (long j = 0; j < N; ++j) {
+      if (j % RCH == 0) {
+        __syncthreads();
+        stage(j / RCH, work, b);
+        __syncthreads();
+      }
       const long gj = d0 + j;
-      const double term = (t >= 1 && t < W && t <= j) ? work[W * gj + t] * win : 0.0;
-      const double y = b[gj] - group_allsum(term);
+      const double term = (t >= 1 && t < W && t <= j) ? brow(j)[t] * win : 0.0;
+      const double y = Vs[(j / RCH) & 1][j % RCH] - group_allsum(term);
       if (t == 0) x[gj] = y;
       win = __shfl_up(win, 1, 64);
       if (t == 1) win = y;
@@ -544,13 +576,24 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
     // D z = y
     for (long d = d0 + t; d < d1; d += LDLT_BS) x[d] /= work[W * d];
     __syncthreads();
-    // L^T x = z: lane k in 1..ROD_BW holds x(j + k)
+    // L^T x = z: lane k in 1..ROD_BW holds x(j + k); step j reads rows j ..
+    // j + ROD_BW of L (chunks c, c + 1) and z(j)
     win = 0.0;
+    if (N > 0) {
+      const long cl = (N - 1) / RCH;
+      stage(cl + 1, work, nullptr);
+      stage(cl, work, x);
+      __syncthreads();
+    }
     for (long j = N - 1; j >= 0; --j) {
+      if (j % RCH == RCH - 1 && j != N - 1) {
+        __syncthreads();
+        stage(j / RCH, work, x);
+        __syncthreads();
+      }
       const long gj = d0 + j;
-      const double term =
-          (t >= 1 && t < W && j + t < N) ? work[W * (gj + t) + t] * win : 0.0;
-      const double xj = x[gj] - group_allsum(term);
+      const double term = (t >= 1 && t < W && j + t < N) ? brow(j + t)[t] * win : 0.0;
+      const double xj = Vs[(j / RCH) & 1][j % RCH] - group_allsum(term);
       if (t == 0) x[gj] = xj;
       win = __shfl_up(win, 1, 64);
       if (t == 1) win = xj;
