@@ -454,6 +454,13 @@ __device__ __forceinline__ double group_allsum(double v) {
   return __shfl(v, 0, 64);
 }
 
+// sum of lanes 0..15 (the band lanes), in every lane
+__device__ __forceinline__ double band_allsum(double v) {
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+  return __shfl(v, 0, 64);
+}
+
 __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const double* __restrict__ band,
                                                           double* __restrict__ work,
                                                           const double* __restrict__ b,
@@ -461,8 +468,11 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
                                                           double max_error, int* __restrict__ status,
                                                           const unsigned char* __restrict__ active) {
   constexpr int W = ROD_BW + 1;
-  __shared__ double Lw[W][W];  // Lw[i % W][k] = L(i, i - k), k = 1..ROD_BW
-  __shared__ double Dw[W];     // Dw[m % W] = D(m)
+  // rings of the last RING rows (RING >= W, a power of two: slot = row & (RING - 1))
+  constexpr int RING = 16;
+  static_assert(RING >= W && (RING & (RING - 1)) == 0, "ring of rows");
+  __shared__ double Lw[RING][W];  // Lw[i & 15][k] = L(i, i - k), k = 1..ROD_BW
+  __shared__ double Dw[RING];     // Dw[m & 15] = D(m)
   __shared__ double Bs[2][RCH][W];  // staged rows (band or L) of chunks c, c + 1
   __shared__ double Vs[2][RCH];     // staged b or x of the same chunks
   const int r = blockIdx.x;
@@ -506,68 +516,67 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
   for (int it = 0; it < reg_iters && st != 0; ++it) {
     double cum = 0.0;  // LinearSolver_EigenLDLT.cpp:68: A += regT 10^(i-1) I, cumulative
     for (int q = 1; q <= it; ++q) cum += regT * pow(10.0, q - 1);
-    for (int q = t; q < W * W; q += LDLT_BS) (&Lw[0][0])[q] = 0.0;
-    if (t < W) Dw[t] = 0.0;
+    for (int q = t; q < RING * W; q += LDLT_BS) (&Lw[0][0])[q] = 0.0;
+    if (t < RING) Dw[t] = 0.0;
     stage(0, band, nullptr);
     __syncthreads();
     bool bad = false;
-    for (long j = 0; j < N; ++j) {
-      if (j % RCH == 0) {  // rows up to j + ROD_BW: chunk j / RCH + 1 as well
+    const int Ni = (int)N;  // at most 4 x 500 DoFs
+    for (int j = 0; j < Ni; ++j) {
+      if ((j & (RCH - 1)) == 0) {  // rows up to j + ROD_BW: chunk j / RCH + 1 as well
         stage(j / RCH + 1, band, nullptr);
         __syncthreads();
       }
       const long gj = d0 + j;
-      const int sj = (int)(j % W);
+      const int sj = j & (RING - 1);
       double term = 0.0;
       if (t >= 1 && t < W && t <= j) {
         const double l = Lw[sj][t];
-        term = l * l * Dw[(j - t) % W];
+        term = l * l * Dw[(j - t) & (RING - 1)];
       }
-      const double dj = brow(j)[0] + cum - group_allsum(term);
+      const double dj = brow(j)[0] + cum - band_allsum(term);  // terms in lanes 1..ROD_BW
       if (!(fabs(dj) >= sinT)) {  // uniform: dj is the same in every lane
         bad = true;
         break;
       }
       // L(i, j), i = j + 1 + t: (A(i, j) - sum_m L(i, m) L(j, m) D(m)) / D_j over
       // the columns m < j within the band of both rows
-      const long i = j + 1 + t;
-      const bool row = t < ROD_BW && i < N;
+      const int i = j + 1 + t;
+      const bool row = t < ROD_BW && i < Ni;
       double lij = 0.0;
       if (row) {
         const int o = t + 1;  // i - j
-        const int si = (int)(i % W);
+        const int si = i & (RING - 1);
         double sacc = brow(i)[o];
-        for (int k = o + 1; k < W && (long)k <= i; ++k) {
-          const long m = i - k;
-          sacc -= Lw[si][k] * Lw[sj][k - o] * Dw[m % W];
-        }
+        for (int k = o + 1; k < W && k <= i; ++k)
+          sacc -= Lw[si][k] * Lw[sj][k - o] * Dw[(i - k) & (RING - 1)];
         lij = sacc / dj;
       }
       __syncthreads();  // the window of step j has been read
       if (t == 0) {
-        Dw[j % W] = dj;
+        Dw[sj] = dj;
         work[W * gj] = dj;
       }
       if (row) {
-        Lw[i % W][t + 1] = lij;
+        Lw[i & (RING - 1)][t + 1] = lij;
         work[W * (d0 + i) + t + 1] = lij;
       }
-      if (t >= 1 && t < W) Lw[sj][t] = 0.0;  // slot of row j becomes row j + W
+      if (t >= 1 && t < W) Lw[sj][t] = 0.0;  // slot of row j becomes row j + RING
       __syncthreads();
     }
     if (bad) continue;
     // L y = b: lane k in 1..ROD_BW holds y(j - k); rows of L and b staged
     __syncthreads();  // the factor's global rows are complete
     double win = 0.0;
-    for (long j = 0; j < N; ++j) {
-      if (j % RCH == 0) {
+    for (int j = 0; j < Ni; ++j) {
+      if ((j & (RCH - 1)) == 0) {
         __syncthreads();
         stage(j / RCH, work, b);
         __syncthreads();
       }
       const long gj = d0 + j;
       const double term = (t >= 1 && t < W && t <= j) ? brow(j)[t] * win : 0.0;
-      const double y = Vs[(j / RCH) & 1][j % RCH] - group_allsum(term);
+      const double y = Vs[(j / RCH) & 1][j & (RCH - 1)] - band_allsum(term);
       if (t == 0) x[gj] = y;
       win = __shfl_up(win, 1, 64);
       if (t == 1) win = y;
@@ -585,15 +594,15 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
       stage(cl, work, x);
       __syncthreads();
     }
-    for (long j = N - 1; j >= 0; --j) {
-      if (j % RCH == RCH - 1 && j != N - 1) {
+    for (int j = Ni - 1; j >= 0; --j) {
+      if ((j & (RCH - 1)) == RCH - 1 && j != Ni - 1) {
         __syncthreads();
         stage(j / RCH, work, x);
         __syncthreads();
       }
       const long gj = d0 + j;
-      const double term = (t >= 1 && t < W && j + t < N) ? brow(j + t)[t] * win : 0.0;
-      const double xj = Vs[(j / RCH) & 1][j % RCH] - group_allsum(term);
+      const double term = (t >= 1 && t < W && j + t < Ni) ? brow(j + t)[t] * win : 0.0;
+      const double xj = Vs[(j / RCH) & 1][j & (RCH - 1)] - band_allsum(term);
       if (t == 0) x[gj] = xj;
       win = __shfl_up(win, 1, 64);
       if (t == 1) win = xj;
