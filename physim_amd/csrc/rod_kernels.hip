@@ -114,6 +114,7 @@ __global__ __launch_bounds__(ROD_BS) void rod_stretch_kernel(RodArgs A, double* 
                                                               double* __restrict__ Js) {
   const long v = blockIdx.x * (long)ROD_BS + threadIdx.x;
   if (v >= A.nv) return;
+  if (A.live && !A.live[A.rod_of[v]]) return;
   if (!(A.flags[v] & ROD_HAS_EDGE)) {
     es[v] = 0.0;
     return;
@@ -148,6 +149,7 @@ __global__ __launch_bounds__(ROD_BS) void rod_bt_kernel(RodArgs A, double* __res
                                                          T* __restrict__ Hb) {
   const long v = blockIdx.x * (long)ROD_BS + threadIdx.x;
   if (v >= A.nv) return;
+  if (A.live && !A.live[A.rod_of[v]]) return;
   if (!(A.flags[v] & ROD_HAS_BT)) {
     if (EG) eb[v] = 0.0;
     return;
@@ -242,6 +244,7 @@ __global__ __launch_bounds__(ROD_BS) void rod_gather_kernel(RodArgs A, const dou
                                                              double* __restrict__ vE) {
   const long v = blockIdx.x * (long)ROD_BS + threadIdx.x;
   if (v >= A.nv) return;
+  if (A.live && !A.live[A.rod_of[v]]) return;
   const unsigned f = A.flags[v];
   double r[4] = {0.0, 0.0, 0.0, 0.0};
   if (f & ROD_HAS_PREV_EDGE)
@@ -288,6 +291,7 @@ __global__ __launch_bounds__(256) void rod_reduce_kernel(RodArgs A, const double
                                                           double* __restrict__ out) {
   __shared__ double red[3 * 256 / 64 + 4];
   const int r = blockIdx.x;
+  if (A.live && !A.live[r]) return;
   const long v0 = A.voff[r], v1 = A.voff[r + 1];
   double el = 0.0, ld = 0.0, gg = 0.0;
   for (long v = v0 + threadIdx.x; v < v1; v += 256) {
@@ -364,6 +368,7 @@ __global__ __launch_bounds__(ROD_BS) void rod_band_kernel(RodArgs A, const doubl
   const long d = blockIdx.x * (long)ROD_BS + threadIdx.x;
   if (d >= 4 * A.nv) return;
   const long v = d >> 2;
+  if (A.live && !A.live[A.rod_of[v]]) return;
   const int comp = (int)(d & 3);
   const long dof0 = 4L * A.voff[A.rod_of[v]];  // first DoF of this rod
   double row[ROD_BW + 1];

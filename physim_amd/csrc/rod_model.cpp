@@ -220,12 +220,17 @@ RodBufs pgn_rods::bufs() {
   return b;
 }
 
-void pgn_rods::eval(bool hess) {
+void pgn_rods::eval(bool hess, const std::vector<unsigned char>* live) {
   RodBufs b = bufs();
   // PGN_MIXED: the bend-twist Hessians in FP32 (energy and gradient FP64)
   const bool fp32 = prec == PGN_MIXED;
-  rod_launch_eval(stream(), args(), hess, fp32, b);
-  if (hess) rod_launch_band(stream(), args(), fp32, b);
+  RodArgs a = args();
+  if (live) {
+    livem.upload(reinterpret_cast<const uint8_t*>(live->data()), live->size(), stream());
+    a.live = livem.p;
+  }
+  rod_launch_eval(stream(), a, hess, fp32, b);
+  if (hess) rod_launch_band(stream(), a, fp32, b);
 }
 
 void pgn_rods::download_rodE(std::vector<double>& out) {
@@ -308,7 +313,9 @@ void pgn_rods::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
     // one load stage counter serves all of them
     if (has_g && g_cur < g_stages) ++g_cur;
     PGN_HIP(hipMemcpyAsync(xsave.p, x.p, nd * sizeof(double), hipMemcpyDeviceToDevice, s));
-    eval(false);
+    std::vector<unsigned char> live(nrods);
+    for (int r = 0; r < nrods; ++r) live[r] = st[r] == ACTIVE;
+    eval(false, &live);
     download_rodE(E);
     bool need_h = false, need_steep = false;
     for (int r = 0; r < nrods; ++r) {
@@ -327,12 +334,12 @@ void pgn_rods::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
       else need_h = true;
     }
     if (need_h) {
-      eval(true);
-      hipLaunchKernelGGL(rod_neg_kernel, dim3(gd), dim3(256), 0, s, nd, g.p, rhs.p);
       for (int r = 0; r < nrods; ++r) {
         hact[r] = stepping[r] && !steep[r];
         if (hact[r]) hevals[r]++;
       }
+      eval(true, &hact);
+      hipLaunchKernelGGL(rod_neg_kernel, dim3(gd), dim3(256), 0, s, nd, g.p, rhs.p);
       PGN_HIP(hipMemcpyAsync(sel.p, hact.data(), nrods, hipMemcpyHostToDevice, s));
       RodBufs b = bufs();
       rod_launch_ldlt(s, args(), b, rhs.p, dx.p, std::max(o.lsNumRegTrials, 1), o.lsTolMaxError,
@@ -377,7 +384,8 @@ void pgn_rods::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
       if (!anys) break;
       upload_fac();
       rod_launch_step(s, args(), fac.p, xsave.p, dx.p, x.p);
-      eval(false);
+      for (int r = 0; r < nrods; ++r) live[r] = stepping[r] && searching[r];
+      eval(false, &live);
       download_rodE(E);
       bool restore = false;
       for (int r = 0; r < nrods; ++r) {

@@ -34,7 +34,10 @@ struct pgn_rods {
   void sync();
   pgn::RodArgs args() const;
   pgn::RodBufs bufs();
-  void eval(bool hess);
+  // live (nrods) or null: only the marked rods are evaluated, the others'
+  // per-rod outputs keep their values
+  void eval(bool hess, const std::vector<unsigned char>* live = nullptr);
+  pgn::DBuf<uint8_t> livem;
   void download_rodE(std::vector<double>& out);
   long ref_dofs() const;
   void to_ref(const double* in4, double* out) const;

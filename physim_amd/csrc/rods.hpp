@@ -35,6 +35,7 @@ struct RodArgs {
   const double* rt;     // nv (bend-twist slot): reference twist
   const double* mass;   // nv
   double grav[3];
+  const uint8_t* live = nullptr;  // nrods or null: rods evaluated (others keep their outputs)
 };
 
 struct RodBufs {
