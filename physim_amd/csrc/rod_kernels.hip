@@ -466,6 +466,17 @@ __device__ __forceinline__ double band_allsum(double v) {
   return __shfl(v, 0, 64);
 }
 
+// two such sums at once (same order as band_allsum for each)
+__device__ __forceinline__ void band_allsum2(double& a, double& b) {
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 16);
+    b += __shfl_xor(b, o, 16);
+  }
+  a = __shfl(a, 0, 64);
+  b = __shfl(b, 0, 64);
+}
+
 __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const double* __restrict__ band,
                                                           double* __restrict__ work,
                                                           const double* __restrict__ b,
@@ -523,27 +534,33 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
     for (int q = 1; q <= it; ++q) cum += regT * pow(10.0, q - 1);
     for (int q = t; q < RING * W; q += LDLT_BS) (&Lw[0][0])[q] = 0.0;
     if (t < RING) Dw[t] = 0.0;
-    stage(0, band, nullptr);
+    stage(0, band, b);
     __syncthreads();
     bool bad = false;
     const int Ni = (int)N;  // at most 4 x 500 DoFs
+    // the forward substitution L y = b runs inside the factorization: row j
+    // of L is complete at step j; lane k in 1..ROD_BW holds y(j - k)
+    double win = 0.0;
     for (int j = 0; j < Ni; ++j) {
       if ((j & (RCH - 1)) == 0) {  // rows up to j + ROD_BW: chunk j / RCH + 1 as well
-        stage(j / RCH + 1, band, nullptr);
+        stage(j / RCH + 1, band, b);
         __syncthreads();
       }
       const long gj = d0 + j;
       const int sj = j & (RING - 1);
-      double term = 0.0;
+      double term = 0.0, ty = 0.0;
       if (t >= 1 && t < W && t <= j) {
         const double l = Lw[sj][t];
         term = l * l * Dw[(j - t) & (RING - 1)];
+        ty = l * win;
       }
-      const double dj = brow(j)[0] + cum - band_allsum(term);  // terms in lanes 1..ROD_BW
+      band_allsum2(term, ty);  // terms in lanes 1..ROD_BW
+      const double dj = brow(j)[0] + cum - term;
       if (!(fabs(dj) >= sinT)) {  // uniform: dj is the same in every lane
         bad = true;
         break;
       }
+      const double yj = Vs[(j / RCH) & 1][j & (RCH - 1)] - ty;
       // L(i, j), i = j + 1 + t: (A(i, j) - sum_m L(i, m) L(j, m) D(m)) / D_j over
       // the columns m < j within the band of both rows
       const int i = j + 1 + t;
@@ -561,32 +578,19 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
       if (t == 0) {
         Dw[sj] = dj;
         work[W * gj] = dj;
+        x[gj] = yj;
       }
       if (row) {
         Lw[i & (RING - 1)][t + 1] = lij;
         work[W * (d0 + i) + t + 1] = lij;
       }
       if (t >= 1 && t < W) Lw[sj][t] = 0.0;  // slot of row j becomes row j + RING
+      win = __shfl_up(win, 1, 64);
+      if (t == 1) win = yj;
       __syncthreads();
     }
     if (bad) continue;
-    // L y = b: lane k in 1..ROD_BW holds y(j - k); rows of L and b staged
-    __syncthreads();  // the factor's global rows are complete
-    double win = 0.0;
-    for (int j = 0; j < Ni; ++j) {
-      if ((j & (RCH - 1)) == 0) {
-        __syncthreads();
-        stage(j / RCH, work, b);
-        __syncthreads();
-      }
-      const long gj = d0 + j;
-      const double term = (t >= 1 && t < W && t <= j) ? brow(j)[t] * win : 0.0;
-      const double y = Vs[(j / RCH) & 1][j & (RCH - 1)] - band_allsum(term);
-      if (t == 0) x[gj] = y;
-      win = __shfl_up(win, 1, 64);
-      if (t == 1) win = y;
-    }
-    __syncthreads();
+    __syncthreads();  // the factor's and y's global rows are complete
     // D z = y
     for (long d = d0 + t; d < d1; d += LDLT_BS) x[d] /= work[W * d];
     __syncthreads();
