@@ -459,22 +459,41 @@ __device__ __forceinline__ double group_allsum(double v) {
   return __shfl(v, 0, 64);
 }
 
-// sum of lanes 0..15 (the band lanes), in every lane
+// Sum of lanes 0..15 (the band lanes), in every lane: an inclusive scan
+// within the 16-lane DPP row (row_shr 1, 2, 4, 8; lanes shifted in from
+// outside the row read 0) leaves the row total in lane 15, which is then
+// read into a scalar register.  DPP moves cost a few cycles where the
+// generic shuffle goes through LDS permutes.
+template <int N>
+__device__ __forceinline__ double row_shr(double v) {
+  return __builtin_amdgcn_update_dpp(0.0, v, 0x110 + N, 0xF, 0xF, true);
+}
+__device__ __forceinline__ double lane15(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), 15);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 15);
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
 __device__ __forceinline__ double band_allsum(double v) {
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
-  return __shfl(v, 0, 64);
+  v += row_shr<1>(v);
+  v += row_shr<2>(v);
+  v += row_shr<4>(v);
+  v += row_shr<8>(v);
+  return lane15(v);
 }
 
-// two such sums at once (same order as band_allsum for each)
+// two such sums at once
 __device__ __forceinline__ void band_allsum2(double& a, double& b) {
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) {
-    a += __shfl_xor(a, o, 16);
-    b += __shfl_xor(b, o, 16);
-  }
-  a = __shfl(a, 0, 64);
-  b = __shfl(b, 0, 64);
+  a += row_shr<1>(a);
+  b += row_shr<1>(b);
+  a += row_shr<2>(a);
+  b += row_shr<2>(b);
+  a += row_shr<4>(a);
+  b += row_shr<4>(b);
+  a += row_shr<8>(a);
+  b += row_shr<8>(b);
+  a = lane15(a);
+  b = lane15(b);
 }
 
 __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const double* __restrict__ band,
@@ -585,7 +604,7 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
         work[W * (d0 + i) + t + 1] = lij;
       }
       if (t >= 1 && t < W) Lw[sj][t] = 0.0;  // slot of row j becomes row j + RING
-      win = __shfl_up(win, 1, 64);
+      win = row_shr<1>(win);  // lane k takes lane k - 1 (lanes 1..ROD_BW matter)
       if (t == 1) win = yj;
       __syncthreads();
     }
@@ -613,7 +632,7 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
       const double term = (t >= 1 && t < W && j + t < Ni) ? brow(j + t)[t] * win : 0.0;
       const double xj = Vs[(j / RCH) & 1][j & (RCH - 1)] - band_allsum(term);
       if (t == 0) x[gj] = xj;
-      win = __shfl_up(win, 1, 64);
+      win = row_shr<1>(win);  // lane k takes lane k - 1 (lanes 1..ROD_BW matter)
       if (t == 1) win = xj;
     }
     __syncthreads();
