@@ -65,9 +65,17 @@ static inline int grid_rows(int n) {
 
 template <int L, class T>
 __device__ __forceinline__ T group_sum(T v) {
+  if constexpr (L == 16) {
+    // a 16-lane group is a DPP row: an inclusive scan, the total in its last lane
+    v += dpp_shr<1>(v);
+    v += dpp_shr<2>(v);
+    v += dpp_shr<4>(v);
+    v += dpp_shr<8>(v);
+  } else {
 #pragma unroll
-  for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  }
+  return v;  // valid in lane L - 1 of the group (with the shuffles: in every lane)
 }
 
 template <class T>
@@ -269,7 +277,7 @@ __global__ __launch_bounds__(K1_BS) void pcg_init_kernel(PcgArgs A, const T* __r
       y1 = group_sum<L>(y1);
       y2 = group_sum<L>(y2);
     }
-    if (lane == 0) {
+    if (lane == L - 1) {
       const long i = row;
       T ri[3] = {b[3 * i] - y0, b[3 * i + 1] - y1, b[3 * i + 2] - y2};
       T zi[3];
@@ -379,7 +387,7 @@ __global__ __launch_bounds__(K1_BS) void pcg_k1_kernel(PcgArgs A, int k) {
     y0 = group_sum<L>(y0);
     y1 = group_sum<L>(y1);
     y2 = group_sum<L>(y2);
-    if (lane == 0) {
+    if (lane == L - 1) {
       const long i = row;
       const T pi0 = z[3 * i] + tb * pold[3 * i];
       const T pi1 = z[3 * i + 1] + tb * pold[3 * i + 1];
@@ -473,7 +481,7 @@ __global__ __launch_bounds__(K1_BS) void pcg_k1b_kernel(PcgArgs A, int k) {
     y0 = group_sum<L>(y0);
     y1 = group_sum<L>(y1);
     y2 = group_sum<L>(y2);
-    if (lane == 0) {
+    if (lane == L - 1) {
       const long i = row;
       q[3 * i] = y0;
       q[3 * i + 1] = y1;
@@ -551,7 +559,7 @@ __global__ __launch_bounds__(K1_BS) void cg1_a_kernel(PcgArgs A, int k) {
     y0 = group_sum<L>(y0);
     y1 = group_sum<L>(y1);
     y2 = group_sum<L>(y2);
-    if (lane == 0) {
+    if (lane == L - 1) {
       const long i = row;
       w[3 * i] = y0;
       w[3 * i + 1] = y1;
@@ -1195,7 +1203,7 @@ __global__ __launch_bounds__(K1_BS) void spmv_kernel(int n, const long* __restri
     y0 = group_sum<L>(y0);
     y1 = group_sum<L>(y1);
     y2 = group_sum<L>(y2);
-    if (lane == 0) {
+    if (lane == L - 1) {
       y[3L * row] = y0;
       y[3L * row + 1] = y1;
       y[3L * row + 2] = y2;
@@ -1370,7 +1378,7 @@ __global__ __launch_bounds__(K1_BS) void seg_init_kernel(SegPcgArgs A, const T* 
       y1 = group_sum<L>(y1);
       y2 = group_sum<L>(y2);
     }
-    if (lane == 0) {
+    if (lane == L - 1) {
       const long i = row;
       T ri[3] = {b[3 * i] - y0, b[3 * i + 1] - y1, b[3 * i + 2] - y2};
       T zi[3];
@@ -1508,7 +1516,7 @@ __global__ __launch_bounds__(K1_BS) void seg_k1_kernel(SegPcgArgs A, int k) {
     y0 = group_sum<L>(y0);
     y1 = group_sum<L>(y1);
     y2 = group_sum<L>(y2);
-    if (lane == 0) {
+    if (lane == L - 1) {
       const long i = row;
       T zi0 = z[3 * i], zi1 = z[3 * i + 1], zi2 = z[3 * i + 2];
       if (CG) seg_coarse_z<T>(A.cg, ucl, ca0, i, zi0, zi1, zi2);

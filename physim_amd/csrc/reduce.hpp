@@ -5,10 +5,31 @@
 
 namespace pgn {
 
+// DPP row shift right by N within each 16-lane row (lanes shifted in from
+// outside the row read 0): v[lane - N] or 0
+template <int N, class T>
+__device__ __forceinline__ T dpp_shr(T v) {
+  return __builtin_amdgcn_update_dpp(T(0), v, 0x110 + N, 0xF, 0xF, true);
+}
+
+// v of one lane as a wavefront-uniform value
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Sum over the wavefront (all 64 lanes active), in every lane: inclusive
+// scans within the four 16-lane rows by DPP shifts leave the row totals in
+// lanes 15, 31, 47, 63, added in that fixed order.  DPP moves take a few
+// cycles where shuffles go through LDS permutes.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;  // valid in lane 0
+  v += dpp_shr<1>(v);
+  v += dpp_shr<2>(v);
+  v += dpp_shr<4>(v);
+  v += dpp_shr<8>(v);
+  return (read_lane(v, 15) + read_lane(v, 31)) + (read_lane(v, 47) + read_lane(v, 63));
 }
 
 // Sum over the workgroup in a fixed order; the result is valid in thread 0.

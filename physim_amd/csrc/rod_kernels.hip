@@ -455,8 +455,7 @@ constexpr int RCH = 64;
 static_assert(RCH > ROD_BW, "a step reads rows j .. j + ROD_BW: two chunks at most");
 
 __device__ __forceinline__ double group_allsum(double v) {
-  v = wave_sum(v);
-  return __shfl(v, 0, 64);
+  return wave_sum(v);  // in every lane
 }
 
 // Sum of lanes 0..15 (the band lanes), in every lane: an inclusive scan
