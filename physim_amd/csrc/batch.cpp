@@ -485,6 +485,7 @@ SegPcgArgs pgn_batch::seg_args(int precond) const {
     c.rc = sc_rc.p;
     c.uc = sc_uc.p;
     c.cdot = sc_cdot.p;
+    c.use = sc_use_on ? sc_use.p : nullptr;
   }
   return a;
 }
@@ -633,10 +634,28 @@ void pgn_batch::seg_pcg(const std::vector<double>& tol, int maxit, int precond, 
 }
 
 void pgn_batch::linear_solve(const double* b, double* x, const std::vector<uint8_t>& active,
-                             const std::vector<double>& tol, const pgn_linear_opts& o,
-                             std::vector<int>& iters) {
+                             const std::vector<double>& tol, const pgn_linear_opts& oin,
+                             std::vector<int>& iters, const std::vector<uint8_t>* coarse) {
   hipStream_t s = ctx->stream;
   const int Kn = K();
+  // instances that gave the coarse level up run block Jacobi; a solve where
+  // none of the active ones keeps it runs precond 1 outright
+  pgn_linear_opts o = oin;
+  sc_use_on = false;
+  if (o.precond == 2 && coarse) {
+    bool any = false, all = true;
+    for (int k = 0; k < Kn; ++k)
+      if (active[k]) {
+        any |= (*coarse)[k] != 0;
+        all &= (*coarse)[k] != 0;
+      }
+    if (!any) {
+      o.precond = 1;
+    } else if (!all) {
+      sc_use.upload(*coarse, s);
+      sc_use_on = true;
+    }
+  }
   const long n = M->n, N = 3 * n;
   const BatchArgs ba = args();
   const double inf = std::numeric_limits<double>::infinity();
@@ -788,6 +807,13 @@ struct pgn_batch::NewtonRun {
   // search (the retry rule of the single-model solver); inexact: this step's
   // linear solve was looser than lsTolMaxError
   std::vector<uint8_t> ew_off, inexact;
+  // two-level: instances still keeping the coarse level (an instance whose
+  // linear solve took more than coarse_giveup PCG iterations drops it: the
+  // aggregated rigid-body modes are then not what limits its convergence,
+  // typically a buckling instance, and the cheaper block-Jacobi iteration
+  // wins); PGN_SEG_COARSE_GIVEUP in the environment sets the count, 0 never
+  std::vector<uint8_t> coarse_on;
+  int coarse_giveup = 400;
   std::vector<double> E;
 };
 
@@ -826,6 +852,8 @@ void pgn_batch::newton_begin(const pgn_newton_opts& o) {
   r.eta.assign(Kn, o.ew_max);
   r.gprev.assign(Kn, -1.0);
   r.ew_off.assign(Kn, 0);
+  r.coarse_on.assign(Kn, 1);
+  if (const char* e = std::getenv("PGN_SEG_COARSE_GIVEUP")) r.coarse_giveup = std::atoi(e);
   r.inexact.assign(Kn, 0);
   // OptimSolver::Init (OptimSolver.cpp:36-55)
   PGN_HIP(hipMemcpyAsync(v0.p, M->x.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -902,6 +930,7 @@ std::vector<int> pgn_batch::newton_step() {
   auto& hn2 = r.hn2;
   auto& ew_off = r.ew_off;
   auto& inexact = r.inexact;
+  auto& coarse_on = r.coarse_on;
   auto& E = r.E;
   auto d2d = [&](double* dst, const double* src) {
     PGN_HIP(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -963,9 +992,12 @@ std::vector<int> pgn_batch::newton_step() {
   launch_batch_rhs(s, ba, n, mode.p, M->g.p, xsave.p, v0.p, bvec.p, dx.p);
   if (need_h) {
     std::vector<int> its_lin;
-    linear_solve(bvec.p, dx.p, hact, tol_lin, lo, its_lin);
+    linear_solve(bvec.p, dx.p, hact, tol_lin, lo, its_lin, &coarse_on);
     for (int k = 0; k < Kn; ++k)
-      if (hmode[k] == 1) pcg[k] += its_lin[k];
+      if (hmode[k] == 1) {
+        pcg[k] += its_lin[k];
+        if (r.coarse_giveup > 0 && its_lin[k] > r.coarse_giveup) coarse_on[k] = 0;
+      }
   }
   inst_dot(stepping, dx.p, dx.p, n2.p);
   n2.download(hn2.data(), Kn, s);

@@ -111,10 +111,14 @@ struct pgn_batch {
   void seg_pcg(const std::vector<double>& tol, int maxit, int precond, bool x_zero,
                std::vector<int>& iters);
   // LinearSolver_EigenCG::SolveInternal per instance on the batch Hessian:
-  // active[k] selects the instances solved; tol per instance
+  // active[k] selects the instances solved; tol per instance.  With precond
+  // 2, coarse (or null: all) marks the instances that keep the coarse level;
+  // the others run block Jacobi (a solve where none keeps it runs precond 1).
   void linear_solve(const double* b, double* x, const std::vector<uint8_t>& active,
                     const std::vector<double>& tol, const pgn_linear_opts& o,
-                    std::vector<int>& iters);
+                    std::vector<int>& iters, const std::vector<uint8_t>* coarse = nullptr);
+  pgn::DBuf<uint8_t> sc_use;  // device copy of the coarse mask of the current solve
+  bool sc_use_on = false;     // the mask applies (some instances run without the level)
   void newton(const pgn_newton_opts& o, pgn_newton_result* R);
   // The same run one Newton iteration at a time: newton_begin starts it
   // (OptimSolver::Init), newton_step advances every active instance by one

@@ -448,6 +448,10 @@ __global__ __launch_bounds__(SG_BS) void seg_coarse_fill_kernel(SegPcgArgs A, do
   const int sg = blockIdx.x;
   if (A.seg_G[sg] == 0) return;
   const SegCoarseArgs& c = A.cg;
+  if (c.use && !c.use[sg]) {  // block Jacobi for this instance: uc = 0
+    if (threadIdx.x == 0) ok[sg] = 0;
+    return;
+  }
   const int nc = 6 * (c.inst_aoff[sg + 1] - c.inst_aoff[sg]);
   double* Mk = M + c.inst_moff[sg];
   for (int e = threadIdx.x; e < nc * nc; e += SG_BS) Mk[e] = 0.0;
@@ -463,7 +467,7 @@ __global__ __launch_bounds__(256) void seg_coarse_pairs_kernel(SegPcgArgs A, Coa
   const int cb = (int)(t / 36), e = (int)(t % 36), p = e / 6, q = e % 6;
   const int ca = b.cb_a[cb], cbb = b.cb_b[cb];
   const int sg = c.agg_inst[ca];
-  if (A.seg_G[sg] == 0) return;
+  if (A.seg_G[sg] == 0 || (c.use && !c.use[sg])) return;
   double v = 0.0;
   for (int k = b.pair_piece0[cb]; k < b.pair_piece0[cb + 1]; ++k) v += b.piece_sum[36L * k + e];
   const bool act_p = (c.cmask[ca] >> p) & 1u, act_q = (c.cmask[cbb] >> q) & 1u;
@@ -486,6 +490,7 @@ __global__ __launch_bounds__(SG_BS) void seg_coarse_sweep_kernel(SegPcgArgs A, d
   const int sg = blockIdx.x;
   if (A.seg_G[sg] == 0) return;
   const SegCoarseArgs& c = A.cg;
+  if (c.use && !c.use[sg]) return;
   const int nc = 6 * (c.inst_aoff[sg + 1] - c.inst_aoff[sg]);
   double* Mk = M + c.inst_moff[sg];
   const int t = threadIdx.x;
@@ -535,7 +540,7 @@ __global__ __launch_bounds__(RS_BS) void seg_coarse_restrict_kernel(SegPcgArgs A
   __shared__ double red[RS_BS / 64][6];
   const SegCoarseArgs& c = A.cg;
   const int a = blockIdx.x, sg = c.agg_inst[a];
-  if (A.seg_G[sg] == 0 || A.sc[sg].done) return;
+  if (A.seg_G[sg] == 0 || A.sc[sg].done || (c.use && !c.use[sg])) return;
   const T* __restrict__ r = reinterpret_cast<const T*>(A.r);
   const T* __restrict__ rel = reinterpret_cast<const T*>(c.rel);
   const int lo = c.agg_ptr[a], hi = c.agg_ptr[a + 1];
@@ -624,12 +629,13 @@ __global__ __launch_bounds__(SA_BS) void seg_coarse_gemv_kernel(SegPcgArgs A, in
   }
   __syncthreads();
   double dl = 0.0;
+  const bool good = c.ok[sg] != 0;  // 0 also for instances without the coarse level
   if (t < nc) {
     double v = 0.0;
     for (int q = 0; q < P; ++q) v += part[q * nc + t];
-    const double u = c.ok[sg] != 0 ? -v : 0.0;
+    const double u = good ? -v : 0.0;
     c.uc[6L * a0 + t] = u;
-    dl = src[t] * u;
+    dl = good ? src[t] * u : 0.0;  // rc may be stale where the level is off
   }
   const double s = block_sum<SA_BS>(dl, red1);
   if (t == 0) c.cdot[(long)(k & 1) * A.K + sg] = s;

@@ -301,6 +301,7 @@ struct SegCoarseArgs {
   double* rc;               // 6 NA: P^T r
   double* uc;               // 6 NA: Ac^-1 P^T r
   double* cdot;             // 2 x K: r.(P uc) by iteration parity
+  const uint8_t* use;       // K or null: instances with the coarse level (others: uc = 0)
 };
 struct SegPcgArgs {
   int K;                 // segments
