@@ -318,6 +318,14 @@ typedef struct {
      back to one pair when its ring wraps; the history starts afresh with
      every solve)                                                         */
   int qpSolverType;
+  /* MI355X path control (no reference counterpart): fused batches with
+     precond = 2 only.  An instance whose linear solve takes more than
+     coarse_giveup PCG iterations continues its solve with block Jacobi (the
+     aggregated rigid-body coarse level then does not hold what limits its
+     convergence, as on buckling instances, and the cheaper iteration wins);
+     a solve where no active instance keeps the level runs block Jacobi
+     outright.  0 (default): every instance keeps it                      */
+  int coarse_giveup;
 } pgn_newton_opts;
 
 /* QPSolverType (BasicTypes.h:48-55) */

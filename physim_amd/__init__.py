@@ -90,7 +90,7 @@ class NewtonOpts(ctypes.Structure):
                 ("inner_tol", ctypes.c_double), ("adaptive_tol", ctypes.c_int),
                 ("ls_noise_guard", ctypes.c_double), ("ew_gamma", ctypes.c_double),
                 ("ew_max", ctypes.c_double), ("lsSolverType", ctypes.c_int),
-                ("qpSolverType", ctypes.c_int)]
+                ("qpSolverType", ctypes.c_int), ("coarse_giveup", ctypes.c_int)]
 
 
 class ServerOpts(ctypes.Structure):

@@ -630,6 +630,7 @@ void pgn_newton_default_opts(pgn_newton_opts* o) {
   o->ew_max = 0.2;
   o->lsSolverType = PGN_LS_EIGENCG;
   o->qpSolverType = PGN_QP_NEWTON;
+  o->coarse_giveup = 0;
 }
 
 int pgn_newton_solve(pgn_model* M, const pgn_newton_opts* opts, pgn_newton_result* res) {

@@ -807,13 +807,10 @@ struct pgn_batch::NewtonRun {
   // search (the retry rule of the single-model solver); inexact: this step's
   // linear solve was looser than lsTolMaxError
   std::vector<uint8_t> ew_off, inexact;
-  // two-level: instances still keeping the coarse level (an instance whose
-  // linear solve took more than coarse_giveup PCG iterations drops it: the
-  // aggregated rigid-body modes are then not what limits its convergence,
-  // typically a buckling instance, and the cheaper block-Jacobi iteration
-  // wins); PGN_SEG_COARSE_GIVEUP in the environment sets the count, 0 never
+  // two-level: instances still keeping the coarse level (pgn_newton_opts::
+  // coarse_giveup; PGN_SEG_COARSE_GIVEUP in the environment overrides it)
   std::vector<uint8_t> coarse_on;
-  int coarse_giveup = 400;
+  int coarse_giveup = 0;
   std::vector<double> E;
 };
 
@@ -853,6 +850,7 @@ void pgn_batch::newton_begin(const pgn_newton_opts& o) {
   r.gprev.assign(Kn, -1.0);
   r.ew_off.assign(Kn, 0);
   r.coarse_on.assign(Kn, 1);
+  r.coarse_giveup = o.coarse_giveup;
   if (const char* e = std::getenv("PGN_SEG_COARSE_GIVEUP")) r.coarse_giveup = std::atoi(e);
   r.inexact.assign(Kn, 0);
   // OptimSolver::Init (OptimSolver.cpp:36-55)
