@@ -166,3 +166,27 @@ def test_batch_twolevel_matches_block_jacobi(ctx, precision, nodes, monkeypatch)
         for pc in (1, 2):
             err = np.abs(out[pc][0][k] - xo).max()
             assert err <= 1e-6 * s.bbox_diag(), (k, pc, err)
+
+
+def test_batch_coarse_giveup_matches_oracle(ctx):
+    """Two-level batch with coarse_giveup low enough that some instances
+    drop their coarse level mid-run (solves then mix instances with and
+    without it, and later run block Jacobi outright): every instance still
+    converges to the oracle's equilibrium within 1e-6 x its bounding-box
+    diagonal."""
+    _say("batch coarse give-up")
+    sc = _small_mix(6)
+    B = pg.Batch(ctx, pg.MIXED)
+    for s in sc:
+        B.add_scene(s)
+    B.build()
+    res = B.newton(**OPTS, precond=2, coarse_giveup=15)
+    _say("  " + ", ".join(f"{r['result']}/{r['iterations']}/{r['pcg_iterations']}" for r in res))
+    for k, s in enumerate(sc):
+        assert res[k]["result"] == "SUCCESS", (k, res[k])
+        O = _oracle(s)
+        ro = O.newton(**OPTS)
+        assert ro["result"] == "SUCCESS"
+        err = np.abs(B.get_x(k) - O.get_x().reshape(-1, 3)).max()
+        assert err <= 1e-6 * s.bbox_diag(), (k, err)
+    B.close()
