@@ -500,6 +500,7 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
                                                           const double* __restrict__ b,
                                                           double* __restrict__ x, int reg_iters,
                                                           double max_error, int* __restrict__ status,
+                                                          const int* __restrict__ order,
                                                           const unsigned char* __restrict__ active) {
   constexpr int W = ROD_BW + 1;
   // rings of the last RING rows (RING >= W, a power of two: slot = row & (RING - 1))
@@ -509,8 +510,8 @@ __global__ __launch_bounds__(LDLT_BS) void rod_ldlt_kernel(RodArgs A, const doub
   __shared__ double Dw[RING];     // Dw[m & 15] = D(m)
   __shared__ double Bs[2][RCH][W];  // staged rows (band or L) of chunks c, c + 1
   __shared__ double Vs[2][RCH];     // staged b or x of the same chunks
-  const int r = blockIdx.x;
-  if (r >= A.nrods) return;
+  if ((int)blockIdx.x >= A.nrods) return;
+  const int r = order ? order[blockIdx.x] : (int)blockIdx.x;
   if (active && !active[r]) return;
   const int t = threadIdx.x;
   const long d0 = 4L * A.voff[r], d1 = 4L * A.voff[r + 1];
@@ -774,10 +775,11 @@ void rod_launch_band(hipStream_t s, const RodArgs& a, bool fp32_hess, RodBufs& b
 }
 
 void rod_launch_ldlt(hipStream_t s, const RodArgs& a, RodBufs& b, const double* rhs, double* x,
-                     int reg_iters, double max_error, const unsigned char* active) {
+                     int reg_iters, double max_error, const int* order,
+                     const unsigned char* active) {
   if (a.nrods == 0) return;
   hipLaunchKernelGGL(rod_ldlt_kernel, dim3(a.nrods), dim3(LDLT_BS), 0, s, a, b.band, b.work,
-                     rhs, x, reg_iters, max_error, b.status, active);
+                     rhs, x, reg_iters, max_error, b.status, order, active);
 }
 
 void rod_launch_bt_raw(hipStream_t s, long count, const double* prm, const double* e1,

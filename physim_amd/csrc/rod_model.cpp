@@ -150,6 +150,16 @@ void pgn_rods::setup(int n_rods, const int* nverts, const double* X0h, const dou
   hipStream_t s = stream();
   voff.upload(hvoff, s);
   rod_of.upload(hrod_of, s);
+  // the LDL^T is a chain of steps per rod: the longest chains start first so
+  // that no long rod waits behind short ones for a slot (stable: ties in
+  // rod order)
+  std::vector<int> order(nrods);
+  for (int r = 0; r < nrods; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return hvoff[a + 1] - hvoff[a] > hvoff[b + 1] - hvoff[b];
+  });
+  ldlt_order.upload(order, s);
+  PGN_HIP(hipStreamSynchronize(s));  // order is released at the end of this function
   flags.upload(hflags, s);
   fixmask.upload(hfixmask, s);
   x.upload(hx, s);
@@ -343,7 +353,7 @@ void pgn_rods::newton(const pgn_newton_opts& o, pgn_newton_result* R) {
       PGN_HIP(hipMemcpyAsync(sel.p, hact.data(), nrods, hipMemcpyHostToDevice, s));
       RodBufs b = bufs();
       rod_launch_ldlt(s, args(), b, rhs.p, dx.p, std::max(o.lsNumRegTrials, 1), o.lsTolMaxError,
-                      sel.p);
+                      ldlt_order.p, sel.p);
     }
     if (need_steep) {
       for (int r = 0; r < nrods; ++r) hsel[r] = stepping[r] && steep[r];

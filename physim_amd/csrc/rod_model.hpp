@@ -18,6 +18,7 @@ struct pgn_rods {
   std::vector<double> hframe0, hx0;
   pgn::DBuf<long> voff;
   pgn::DBuf<int> rod_of;
+  pgn::DBuf<int> ldlt_order;  // rods by decreasing length: the LDL^T workgroup order
   pgn::DBuf<uint8_t> flags, fixmask, sel;
   pgn::DBuf<double> x, L0, kst, btc, frame, rt, mass;
   pgn::DBuf<double> es, gs, Js, eb, gb, g, vE, rodE, band, work, xsave, dx, rhs, fac, rn2;

@@ -51,8 +51,10 @@ struct RodBufs {
 
 void rod_launch_eval(hipStream_t s, const RodArgs& a, bool hess, bool fp32_hess, RodBufs& b);
 void rod_launch_band(hipStream_t s, const RodArgs& a, bool fp32_hess, RodBufs& b);
+// order (nrods or null): the rod of each workgroup (longest first)
 void rod_launch_ldlt(hipStream_t s, const RodArgs& a, RodBufs& b, const double* rhs, double* x,
-                     int reg_iters, double max_error, const unsigned char* active);
+                     int reg_iters, double max_error, const int* order,
+                     const unsigned char* active);
 // bend-twist elements on given inputs (27 parameters, e1, e2, two angles per
 // element); H: 36 packed values per element in FP32 (fp32) or FP64
 void rod_launch_bt_raw(hipStream_t s, long count, const double* prm, const double* e1,
