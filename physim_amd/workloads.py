@@ -108,6 +108,7 @@ def batch(args, log):
         "config": {"workload": f"batch_{count}_beams: {m} tets, {3 * n} DoF, one fused "
                                "launch per kernel, per-instance Newton",
                    "precision": "mixed", "precond": PRECOND_NAMES[opts["precond"]],
+                   "coarse_giveup": opts["coarse_giveup"],
                    "linear_tol": "eisenstat-walker"},
         "converged": sum(r["result"] == "SUCCESS" for r in res),
         "instances": count,
