@@ -259,6 +259,42 @@ void pgn_batch::seg_red_prepare() {
   }
   table(dofs, 4096, sr_inst_d, sr_wg0_d, sr_lo_d, sr_hi_d, sr_part_d, tab_dof);
   table(blks, 2048, sr_inst_b, sr_wg0_b, sr_lo_b, sr_hi_b, sr_part_b, tab_blk);
+  // energy table: each instance's element chunks (tets, membranes, hinges),
+  // then its node chunks
+  {
+    constexpr long CH = 4096;
+    std::vector<int> vi, vw(Kn + 1, 0);
+    std::vector<long> vl, vh;
+    std::vector<uint8_t> vk;
+    auto add = [&](int k, long a0, long a1, uint8_t kind) {
+      for (long a = a0; a < a1; a += CH) {
+        vi.push_back(k);
+        vl.push_back(a);
+        vh.push_back(std::min(a + CH, a1));
+        vk.push_back(kind);
+      }
+    };
+    for (int k = 0; k < Kn; ++k) {
+      vw[k] = (int)vi.size();
+      for (int c = 0; c < 3; ++c) add(k, heoff[c * (Kn + 1) + k], heoff[c * (Kn + 1) + k + 1], 0);
+      add(k, hnoff[k], hnoff[k + 1], 1);
+    }
+    vw[Kn] = (int)vi.size();
+    sr_inst_e.upload(vi, s);
+    sr_wg0_e.upload(vw, s);
+    sr_lo_e.upload(vl, s);
+    sr_hi_e.upload(vh, s);
+    sr_kind_e.upload(vk, s);
+    sr_part_e.ensure(3 * std::max<size_t>(vi.size(), 1));
+    PGN_HIP(hipStreamSynchronize(s));
+    tab_eg.W = (int)vi.size();
+    tab_eg.inst = sr_inst_e.p;
+    tab_eg.lo = sr_lo_e.p;
+    tab_eg.hi = sr_hi_e.p;
+    tab_eg.wg0 = sr_wg0_e.p;
+    tab_eg.part = sr_part_e.p;
+    tab_eg.kind = sr_kind_e.p;
+  }
   sr_ready = true;
 }
 
@@ -314,7 +350,8 @@ void pgn_batch::eval_eg(std::vector<double>& o4, const std::vector<uint8_t>* act
   if (whole) {
     M->launch_elements(false, EVAL_E | EVAL_G, nullptr, elemE.p, M->eg64.p, nullptr);
     launch_node_gather(s, false, na, M->eg64.p, M->g.p, M->nodeP.p);
-    launch_batch_reduce(s, args(), elemE.p, nodeE.p, nodeG2.p, out.p);
+    seg_red_prepare();
+    launch_seg_energy(s, Kn, tab_eg, nullptr, elemE.p, nodeE.p, nodeG2.p, out.p);
   } else {
     const int tile = std::max(0, M->tet_tile_choice[0][EVAL_E | EVAL_G]);
     for (const auto& rk : runs) {
@@ -345,7 +382,8 @@ void pgn_batch::eval_eg(std::vector<double>& o4, const std::vector<uint8_t>* act
       launch_node_gather(s, false, nr, M->eg64.p, M->g.p, M->nodeP.p);
     }
     eg_mask.upload(*act, s);
-    launch_batch_reduce(s, args(), elemE.p, nodeE.p, nodeG2.p, out.p, eg_mask.p);
+    seg_red_prepare();
+    launch_seg_energy(s, Kn, tab_eg, eg_mask.p, elemE.p, nodeE.p, nodeG2.p, out.p);
   }
   o4.resize(4 * Kn);
   out.download(o4.data(), o4.size(), s);
@@ -1450,7 +1488,10 @@ size_t pgn_batch_device_bytes(const pgn_batch* B) {
          sum_bytes(B->sc_agg_ptr, B->sc_agg_nodes, B->sc_agg_of, B->sc_agg_inst, B->sc_aoff,
                    B->sc_moff, B->sc_cb_a, B->sc_cb_b, B->sc_pair_piece0, B->sc_piece_lo,
                    B->sc_cb_fine, B->sc_ok, B->sc_rel, B->sc_cmask, B->sc_A, B->sc_piece_sum,
-                   B->sc_rc, B->sc_uc, B->sc_cdot);
+                   B->sc_rc, B->sc_uc, B->sc_cdot, B->sc_use) +
+         sum_bytes(B->sr_inst_d, B->sr_wg0_d, B->sr_inst_b, B->sr_wg0_b, B->sr_lo_d, B->sr_hi_d,
+                   B->sr_lo_b, B->sr_hi_b, B->sr_part_d, B->sr_part_b, B->rmask, B->sr_inst_e,
+                   B->sr_wg0_e, B->sr_lo_e, B->sr_hi_e, B->sr_kind_e, B->sr_part_e);
 }
 
 int pgn_batch_reset(pgn_batch* B) {

@@ -78,7 +78,11 @@ struct pgn_batch {
   void inst_dot(const std::vector<uint8_t>& act, const double* x, const double* y, double* out);
   void inst_frob(const std::vector<uint8_t>& act, double* out);
   bool sr_ready = false;
-  pgn::SegRedTable tab_dof, tab_blk;
+  pgn::SegRedTable tab_dof, tab_blk, tab_eg;
+  pgn::DBuf<int> sr_inst_e, sr_wg0_e;
+  pgn::DBuf<long> sr_lo_e, sr_hi_e;
+  pgn::DBuf<uint8_t> sr_kind_e;
+  pgn::DBuf<double> sr_part_e;
   pgn::DBuf<int> sr_inst_d, sr_wg0_d, sr_inst_b, sr_wg0_b;
   pgn::DBuf<long> sr_lo_d, sr_hi_d, sr_lo_b, sr_hi_b;
   pgn::DBuf<double> sr_part_d, sr_part_b;

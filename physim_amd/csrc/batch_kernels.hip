@@ -15,34 +15,6 @@ namespace pgn {
 
 constexpr int BR_BLOCK = 256;
 
-__global__ __launch_bounds__(BR_BLOCK) void batch_reduce_kernel(
-    BatchArgs A, const double* __restrict__ elemE, const double* __restrict__ nodeE,
-    const double* __restrict__ nodeG2, double* __restrict__ out,
-    const uint8_t* __restrict__ mask) {
-  __shared__ double red[2 * BR_BLOCK / 64];
-  const int k = blockIdx.x;
-  if (mask && !mask[k]) return;
-  double el = 0.0, ld = 0.0, gg = 0.0;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const long e0 = A.eoff[c * (A.K + 1) + k], e1 = A.eoff[c * (A.K + 1) + k + 1];
-    for (long e = e0 + threadIdx.x; e < e1; e += BR_BLOCK) el += elemE[e];
-  }
-  const long n0 = A.noff[k], n1 = A.noff[k + 1];
-  for (long i = n0 + threadIdx.x; i < n1; i += BR_BLOCK) {
-    ld += nodeE[i];
-    gg += nodeG2[i];
-  }
-  block_sum2<BR_BLOCK>(el, ld, red);
-  const double sg = block_sum<BR_BLOCK>(gg, red);
-  if (threadIdx.x == 0) {
-    out[4 * k] = el + ld;
-    out[4 * k + 1] = sg;
-    out[4 * k + 2] = fabs(el) + fabs(ld);
-    out[4 * k + 3] = el;
-  }
-}
-
 __global__ __launch_bounds__(BR_BLOCK) void batch_norm2_kernel(BatchArgs A,
                                                                const double* __restrict__ v,
                                                                double* __restrict__ out) {
@@ -98,14 +70,6 @@ __global__ __launch_bounds__(256) void batch_copy_kernel(long nd, const int* __r
 }
 
 static inline unsigned grid_of(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
-
-void launch_batch_reduce(hipStream_t s, const BatchArgs& a, const double* elemE,
-                         const double* nodeE, const double* nodeG2, double* out,
-                         const uint8_t* mask) {
-  if (a.K == 0) return;
-  hipLaunchKernelGGL(batch_reduce_kernel, dim3(a.K), dim3(BR_BLOCK), 0, s, a, elemE, nodeE,
-                     nodeG2, out, mask);
-}
 
 void launch_batch_norm2(hipStream_t s, const BatchArgs& a, const double* v, double* out) {
   if (a.K == 0) return;
@@ -289,6 +253,60 @@ __global__ __launch_bounds__(SR_BS) void seg_frob_kernel(SegRedTable t,
     }
   v = block_sum<SR_BS>(v, red);
   if (threadIdx.x == 0) t.part[w] = v;
+}
+
+__global__ __launch_bounds__(SR_BS) void seg_energy_kernel(SegRedTable t,
+                                                           const uint8_t* __restrict__ mask,
+                                                           const double* __restrict__ elemE,
+                                                           const double* __restrict__ nodeE,
+                                                           const double* __restrict__ nodeG2) {
+  __shared__ double red[2 * SR_BS / 64];
+  const int w = blockIdx.x, k = t.inst[w];
+  if (mask && !mask[k]) return;
+  double el = 0.0, ld = 0.0, gg = 0.0;
+  const long hi = t.hi[w];
+  if (t.kind[w] == 0) {
+    for (long e = t.lo[w] + threadIdx.x; e < hi; e += SR_BS) el += elemE[e];
+  } else {
+    for (long i = t.lo[w] + threadIdx.x; i < hi; i += SR_BS) {
+      ld += nodeE[i];
+      gg += nodeG2[i];
+    }
+  }
+  block_sum2<SR_BS>(el, ld, red);
+  gg = block_sum<SR_BS>(gg, red);
+  if (threadIdx.x == 0) {
+    t.part[3L * w] = el;
+    t.part[3L * w + 1] = ld;
+    t.part[3L * w + 2] = gg;
+  }
+}
+
+__global__ __launch_bounds__(64) void seg_energy_fin_kernel(int K, SegRedTable t,
+                                                            const uint8_t* __restrict__ mask,
+                                                            double* __restrict__ out) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= K || (mask && !mask[k])) return;
+  double el = 0.0, ld = 0.0, gg = 0.0;
+  for (int w = t.wg0[k]; w < t.wg0[k + 1]; ++w) {
+    el += t.part[3L * w];
+    ld += t.part[3L * w + 1];
+    gg += t.part[3L * w + 2];
+  }
+  out[4 * k] = el + ld;
+  out[4 * k + 1] = gg;
+  out[4 * k + 2] = fabs(el) + fabs(ld);
+  out[4 * k + 3] = el;
+}
+
+void launch_seg_energy(hipStream_t s, int K, const SegRedTable& t, const uint8_t* mask,
+                       const double* elemE, const double* nodeE, const double* nodeG2,
+                       double* out) {
+  if (K == 0 || t.W == 0) return;
+  hipLaunchKernelGGL(seg_energy_kernel, dim3(t.W), dim3(SR_BS), 0, s, t, mask, elemE, nodeE,
+                     nodeG2);
+  hipLaunchKernelGGL(seg_energy_fin_kernel, dim3((K + 63) / 64), dim3(64), 0, s, K, t, mask,
+                     out);
 }
 
 // one thread per instance: its partials in order

@@ -366,12 +366,6 @@ struct BatchArgs {
   const long* eoff;
   const int* node_inst;
 };
-// out[4k..4k+2] = {E_k, |g_k|^2, |E_el| + |E_load|} from per-element energies
-// and per-node load energies / squared gradients (one workgroup per
-// instance, fixed order)
-void launch_batch_reduce(hipStream_t s, const BatchArgs& a, const double* elemE,
-                         const double* nodeE, const double* nodeG2, double* out,
-                         const uint8_t* mask = nullptr);  // mask: instances to reduce
 // out[k] = |v_k|^2 over the instance's 3 DoFs per node
 void launch_batch_norm2(hipStream_t s, const BatchArgs& a, const double* v, double* out);
 // x = xsave + fac[k] dx on instances with fac[k] >= 0 (others untouched)
@@ -408,8 +402,14 @@ struct SegRedTable {
   const long* lo = nullptr;   // W: chunk start (DoF, or BSR3 block)
   const long* hi = nullptr;   // W: chunk end
   const int* wg0 = nullptr;   // K + 1: first workgroup of each instance
-  double* part = nullptr;     // W partials
+  double* part = nullptr;     // W partials (3 W for the energy table)
+  const uint8_t* kind = nullptr;  // energy table: 0 element chunk, 1 node chunk
 };
+// out[4k..4k+3] = {E_k, |g_k|^2, |E_el| + |E_load|, E_el} over the energy
+// table (element chunks of elemE, node chunks of nodeE / nodeG2)
+void launch_seg_energy(hipStream_t s, int K, const SegRedTable& t, const uint8_t* mask,
+                       const double* elemE, const double* nodeE, const double* nodeG2,
+                       double* out);
 // out[k] = x_k . y_k (FP64) over the DoF table
 void launch_seg_dot(hipStream_t s, int K, const SegRedTable& t, const uint8_t* mask,
                     const double* x, const double* y, double* out);
