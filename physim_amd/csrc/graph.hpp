@@ -99,13 +99,21 @@ struct ChunkGraphs {
   }
 };
 
+// Single models with more nodes than this queue their PCG chunks as direct
+// launches by default: the GPU, not the host, then bounds a chunk, and the
+// replayed graph was the slower of the two (per solve, replay against direct:
+// 25.1 / 25.7 ms with 4,961 nodes, 48.5 / 48.8 ms with 35,721, 78.1 / 76.8
+// ms with 116,281, 115.5 / 114.3 ms with 270,641).
+constexpr long GRAPH_AUTO_MAX_NODES = 65536;
+
 // PGN_GRAPH in the environment: 0 launches every PCG chunk directly, 1
-// (default) replays graphs for single models, 2 for fused batches as well
-// (whose segmented launch arguments change as instances converge, so that
-// captures seldom repeat: 51 against 44 ms on 64 small beams)
+// replays graphs for every single model, 2 for fused batches as well (whose
+// segmented launch arguments change as instances converge, so that captures
+// seldom repeat: 51 against 44 ms on 64 small beams); unset (-1), single
+// models of at most GRAPH_AUTO_MAX_NODES nodes replay
 inline int graphs_level() {
   const char* e = std::getenv("PGN_GRAPH");
-  return e ? std::atoi(e) : 1;
+  return e ? std::atoi(e) : -1;
 }
 
 }  // namespace pgn

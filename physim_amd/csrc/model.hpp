@@ -185,7 +185,7 @@ struct pgn_model {
   bool hess_valid = false;
   int pcg_chunk_max = 64;
   // PCG chunks after the first replay captured graphs (PGN_GRAPH=0: direct
-  // launches); kctl = {first iteration of the chunk, iteration cap}
+  // launches; unset: models of at most GRAPH_AUTO_MAX_NODES nodes); kctl = {first iteration of the chunk, iteration cap}
   int use_graphs = pgn::graphs_level();
   pgn::DBuf<int> kctl;
   int* hkctl = nullptr;  // pinned host copy

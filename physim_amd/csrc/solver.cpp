@@ -36,7 +36,8 @@ static int pcg_run(pgn_model& M, double tol, int maxit, int precond, bool x_zero
   PcgArgs ga = a;
   ga.tol = 0.0;  // read at iteration 0 only
   ga.kctl = M.kctl.p;
-  const bool graphs = M.use_graphs >= 1 && !M.profiling;
+  const bool graphs = !M.profiling && (M.use_graphs >= 1 ||
+                                        (M.use_graphs < 0 && M.n <= pgn::GRAPH_AUTO_MAX_NODES));
   int k = 0, chunk = 8;
   for (;;) {
     const int c = std::min(chunk, maxit - k);
